@@ -1,0 +1,221 @@
+/*
+ * lentils_hip.h -- C ABI of liblentils_hip.so, the MI355X (gfx950) forward
+ * model + source inversion path of Lentils.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in symbols.  Same names, argument meaning and by-value struct
+ *     layouts as the reference C backends that lentils/backend/load_backend.py
+ *     binds through ctypes; host pointers, caller-owned zero-filled outputs,
+ *     void return.  They copy in, run the HIP kernels, copy out.
+ *
+ *  2. lt_* entry points.  Device-resident handles (context, lens matrices,
+ *     NUFFT plan, solver) so that the hot loop never crosses PCIe; every
+ *     lt_* function returns an lt_status (0 = success).
+ *
+ * No torch or HIP types appear in any signature.
+ */
+#ifndef LENTILS_HIP_H
+#define LENTILS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* By-value structs.  Field order and types are the ABI of                   */
+/* lentils/backend/common.h:13-61 and lentils/backend/deflect.h:8-26.        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int nx, ny, nchannels, nstokes;
+    double xmin, xmax, ymin, ymax;
+    double dx, dy;
+    double *channels;
+    char *mask; /* nx*ny bytes, x-major, nonzero = pixel used */
+} image_space;
+
+typedef struct {
+    int nu, nv, half_nv, nchannels, nstokes;
+    double du, dv, gcx, gcy;
+    double *channels;
+} fourier_space;
+
+typedef struct {
+    int nrows, nchannels, nstokes;
+    double *uv;       /* nrows x 3 (u, v, w); w is ignored */
+    double *channels; /* per-channel scale applied to u and v */
+} visibility_space;
+
+typedef struct {
+    unsigned int type; /* 0 = PEMD (2F1 series), 1 = external shear */
+    unsigned int _pad;
+    double z_l, z_s, d_l, d_s, d_ls, sigma_c, beta;
+    double fpars[24];
+    int ipars[8];
+    unsigned char flags[32];
+} generic_mass_model; /* 320 bytes */
+
+typedef struct {
+    int num_lenses;
+    int num_zplanes;
+    generic_mass_model *lenses;
+} global_lens_model;
+
+/* ------------------------------------------------------------------------ */
+/* 1. Drop-in symbols                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces deflect_points, lentils/backend/deflect.c:60-66 (bound at
+ * load_backend.py:59-61).  p_out = p_in - sum of lens deflections, FP64.
+ * want_deriv != 0 is not supported: p_out is left untouched and an error is
+ * recorded (lt_last_error_message). */
+void deflect_points(global_lens_model glm, double *p_in, int npoints, double *p_out, int want_deriv,
+                    double *deriv);
+
+/* Replaces manifold_lens_matrix_csr, lentils/backend/raster.c:272-395
+ * (bound at load_backend.py:122-126).  Writes row_inds[nx*ny+1], cols[nnz],
+ * vals[nnz]; the caller's cols/vals must hold nnz entries (query with
+ * lt_lens_matrix_nnz), as in the reference. */
+void manifold_lens_matrix_csr(image_space imspace, image_space srcspace, global_lens_model lensmodel,
+                              int *row_inds, int *cols, double *vals);
+
+/* Replaces grid_cpu, lentils/backend/nufft.c:42-122 (bound at
+ * load_backend.py:95-97).  direction 0: vis += degrid(grid);
+ * direction 1: grid += grid(vis).  vis and grid are complex128 arrays.
+ * Requires every kernel tap inside the Nyquist band (the check
+ * GriddingOperator performs, fourier_operators.py:77-80); otherwise the
+ * outputs are left untouched and an error is recorded. */
+void grid_cpu(visibility_space visspace, double *vis, fourier_space gspace, double *grid, int wsup,
+              double kb_beta, int direction);
+
+/* Replaces dft_matrix_csr, lentils/backend/nufft.c:128-161
+ * (load_backend.py:99-101). */
+void dft_matrix_csr(visibility_space visspace, image_space imspace, int *row_inds, int *cols, double *vals);
+
+/* Replaces curv_matrix_findiff_csr, lentils/backend/nufft.c:197-229
+ * (load_backend.py:111-113). */
+void curv_matrix_findiff_csr(image_space imspace, int *row_inds, int *cols, double *vals);
+
+/* Replaces grad_matrix_findiff_csr, lentils/backend/nufft.c:163-194
+ * (load_backend.py:107-109). */
+void grad_matrix_findiff_csr(image_space imspace, int *row_inds, int *cols, double *vals);
+
+/* ------------------------------------------------------------------------ */
+/* 2. lt_* entry points                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef int lt_status;
+#define LT_OK 0
+#define LT_ERR_ARG 1        /* invalid argument / shape */
+#define LT_ERR_HIP 2        /* HIP runtime error */
+#define LT_ERR_NYQUIST 3    /* a kernel tap falls outside the Nyquist band */
+#define LT_ERR_UNSUPPORTED 4
+#define LT_ERR_CAPACITY 5   /* caller buffer too small */
+#define LT_ERR_FFT 6        /* rocFFT error */
+#define LT_ERR_NOT_CONVERGED 7
+
+#define LT_F64 0
+#define LT_F32 1
+
+/* raster status bits, per lens matrix (the reference only printf's these) */
+#define LT_RASTER_STACK_OVERFLOW 1 /* raster.c:262-266 */
+#define LT_RASTER_FRAG_OVERFLOW 2  /* raster.c:208-212 */
+#define LT_RASTER_POP_LIMIT 4      /* a degenerate split that would never end */
+#define LT_RASTER_BAD_LENS 8       /* unsupported lens type */
+
+/* No reference counterpart (the reference is CPU only): library set-up. */
+lt_status lt_init(int device);
+const char *lt_last_error_message(void);
+const char *lt_version(void);
+
+/* Host-pointer deflection with a status code; same arithmetic as
+ * deflect_points. */
+lt_status lt_deflect_points(const global_lens_model *glm, const double *p_in, long npoints, double *p_out);
+
+/* Capacity query for manifold_lens_matrix_csr (replaces the 50-per-row guess
+ * of lens_operators.py:30-34).  raster_status receives LT_RASTER_* bits. */
+lt_status lt_lens_matrix_nnz(image_space imspace, image_space srcspace, global_lens_model lensmodel, long *nnz,
+                             int *raster_status);
+
+/* Checked variant of manifold_lens_matrix_csr. */
+lt_status lt_lens_matrix_csr(image_space imspace, image_space srcspace, global_lens_model lensmodel, int *row_inds,
+                             int *cols, double *vals, long capacity, long *nnz, int *raster_status);
+
+/* Deflected stencil lattice ((2nx+1) x (2ny+1) x 2, x-major) of an image
+ * space: the 9-point stencils of raster.c:319-351 share their points through
+ * it.  p_out is a host pointer. */
+lt_status lt_deflect_lattice(image_space imspace, global_lens_model lensmodel, double *p_out);
+
+/* Lens matrix rasterized from a caller-supplied deflected lattice (host),
+ * e.g. the oracle's deflections; used to separate raster parity from
+ * deflection rounding. */
+lt_status lt_lens_matrix_from_lattice(image_space imspace, image_space srcspace, const double *lattice,
+                                      int *row_inds, int *cols, double *vals, long capacity, long *nnz,
+                                      int *raster_status);
+
+/* --- device-resident handles ---------------------------------------------- */
+
+typedef struct lt_lensbatch_s *lt_lensbatch;
+typedef struct lt_nufft_s *lt_nufft;
+typedef struct lt_solver_s *lt_solver;
+
+/* A batch of lens matrices L_s (image -> source s), one per lens-parameter
+ * sample, built on the GPU in one pass (deflect lattice, rasterize, scan,
+ * fill) and kept resident together with L_s^T.  All samples share imspace;
+ * srcspaces[s] may differ in size.  lens_counts[s] components of `lenses`
+ * (concatenated) belong to sample s. */
+lt_status lt_lensbatch_create(image_space imspace, const image_space *srcspaces, int nsamples,
+                              const generic_mass_model *lenses, const int *lens_counts, lt_lensbatch *out);
+lt_status lt_lensbatch_info(lt_lensbatch lb, int sample, long *nnz, int *raster_status);
+lt_status lt_lensbatch_download(lt_lensbatch lb, int sample, int *row_inds, int *cols, double *vals);
+lt_status lt_lensbatch_destroy(lt_lensbatch lb);
+
+/* NUFFT plan: gridder * rfft2 * zero-pad * apodization, as NUFFTOperator
+ * (fourier_operators.py:96-126).  kb_beta <= 0 selects the reference default
+ * pi*sqrt((2W/a)^2 (a-1/2)^2 - 0.8).  precision LT_F64 or LT_F32 for the
+ * gridding arithmetic and grid storage.  The uv set is binned into uv tiles
+ * once here. */
+lt_status lt_nufft_create(visibility_space visspace, image_space imspace, int pad_factor, int wsup,
+                          double kb_beta, int precision, lt_nufft *out);
+/* host-pointer applies: img is nx*ny float64, vis is nchannels*nrows complex128 */
+lt_status lt_nufft_forward_host(lt_nufft h, const double *img, double *vis);
+lt_status lt_nufft_adjoint_host(lt_nufft h, const double *vis, double *img);
+/* grid / degrid only (the GriddingOperator), host pointers, complex128
+ * grid of nu x half_nv; accumulates into the output like grid_cpu */
+lt_status lt_nufft_degrid_host(lt_nufft h, const double *grid, double *vis);
+lt_status lt_nufft_grid_host(lt_nufft h, const double *vis, double *grid);
+lt_status lt_nufft_info(lt_nufft h, int *nu, int *nv, int *half_nv, double *kb_beta, long *nvis);
+/* Times `reps` device-resident degrid (direction 0) or grid (1) launches
+ * with HIP events on the plan's stream; returns the mean per-launch time. */
+lt_status lt_nufft_time(lt_nufft h, int direction, int reps, double *ms_per_launch);
+lt_status lt_nufft_destroy(lt_nufft h);
+
+/* Source-inversion solver for every sample of a lens batch:
+ *   (L^T D^T C D L + lambda_curv Cc^T Cc + lambda_tikh I) x = L^T D^T C d
+ * with D the NUFFT, C = diag(weights) (sigma^-2 on unmasked visibilities,
+ * data_base.py:36-44) and Cc the curvature matrix of each source grid.
+ * op_mode LT_OP_NUFFT applies D^T C D explicitly through degrid/grid;
+ * LT_OP_BCB uses the dirty-beam convolution (data_base.py:115-129). */
+#define LT_OP_NUFFT 0
+#define LT_OP_BCB 1
+#define LT_PC_NONE 0
+#define LT_PC_JACOBI 1
+lt_status lt_solver_create(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *data, double lambda_curv,
+                           double lambda_tikh, int op_mode, lt_solver *out);
+/* CG to ||b - A x|| / ||b|| <= rtol (FP64 true residual).  x: host array
+ * of all samples' sources concatenated (may be NULL); iters/resid: per
+ * sample (may be NULL). */
+lt_status lt_solver_solve(lt_solver s, double rtol, int maxiter, int precond, double *x, int *iters, double *resid);
+lt_status lt_solver_rhs(lt_solver s, double *rhs);
+lt_status lt_solver_apply(lt_solver s, const double *x, double *y);
+lt_status lt_solver_destroy(lt_solver s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LENTILS_HIP_H */

@@ -1,0 +1,111 @@
+// FP64 deflection kernels: arbitrary points, and the stencil lattice of an
+// image grid for a batch of lens-parameter samples.
+//
+// The per-lens constants (including the pow() of deflect.c:284) are computed
+// once on the host; each thread evaluates one point of one sample with the
+// arithmetic of deflect_math.h.  Work is pure FP64 ALU (~420 flop/point), so
+// the kernels are sized for occupancy: 256 threads, one point per thread,
+// lens records read through the scalar cache (uniform across the wave).
+#include <cmath>
+
+#include "deflect_math.h"
+#include "lt_internal.h"
+
+namespace lt {
+
+lt_status pack_lenses(const generic_mass_model *in, int n, std::vector<DevLens> &out) {
+    out.clear();
+    for (int k = 0; k < n; ++k) {
+        const generic_mass_model &m = in[k];
+        const double *f = m.fpars;
+        DevLens L{};
+        L.type = (int)m.type;
+        L.x = f[0];
+        L.y = f[1];
+        if (m.type == 0) {
+            // lentils/backend/deflect.c:281-290 (per-lens part)
+            const double q = f[2], t = 2 * f[7];
+            L.q = q;
+            L.qq = q * q;
+            L.qt = (1 - q * q) / (q * q);
+            L.t = t;
+            L.tm2 = t - 2;
+            L.b = std::pow(0.5 * f[6] * std::pow(q, t - 0.5) * (3.0 - t) / (2.0 - t), 1.0 / t);
+            L.bb = L.b * L.b;
+            L.cr = f[5];
+            L.ci = -f[4];
+        } else if (m.type == 1) {
+            // lentils/backend/deflect.c:82-89; fpars layout lens_models.py:129-148
+            L.ss = f[2];
+            L.c2 = -f[5] * f[5] + f[4] * f[4];
+            L.s2 = -2 * f[4] * f[5];
+            L.gks = f[6];
+            L.s_gka = f[8];
+            L.c_gka = f[9];
+            L.gss = f[10];
+            L.s_gsa = f[12];
+            L.c_gsa = f[13];
+        } else {
+            set_error("lens type %u is not supported (0 = PEMD, 1 = external shear)", m.type);
+            return LT_ERR_UNSUPPORTED;
+        }
+        out.push_back(L);
+    }
+    return LT_OK;
+}
+
+__global__ void __launch_bounds__(256) k_deflect_points(const DevLens *__restrict__ lenses, int nl,
+                                                         const double *__restrict__ pin, double *__restrict__ pout,
+                                                         long n) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double ox, oy;
+    deflect_point(lenses, nl, pin[2 * p], pin[2 * p + 1], ox, oy);
+    pout[2 * p] = ox;
+    pout[2 * p + 1] = oy;
+}
+
+// lattice point (a, b) of sample s; lattice is x-major, (2nx+1)*(2ny+1)
+// points per sample, 2 doubles each
+__global__ void __launch_bounds__(256) k_deflect_lattice(const DevLens *__restrict__ lenses,
+                                                          const int *__restrict__ lens_off, int nsamples, int lx,
+                                                          int ly, double xmin, double dx, double ymin, double dy,
+                                                          double *__restrict__ lat) {
+    const long per = (long)lx * ly;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= per * nsamples) return;
+    const int s = (int)(g / per);
+    const long p = g - (long)s * per;
+    const int a = (int)(p / ly), b = (int)(p - (long)a * ly);
+    const double px = lattice_coord(xmin, dx, a);
+    const double py = lattice_coord(ymin, dy, b);
+    const int l0 = lens_off[s], l1 = lens_off[s + 1];
+    double ox, oy;
+    deflect_point(lenses + l0, l1 - l0, px, py, ox, oy);
+    lat[2 * g] = ox;
+    lat[2 * g + 1] = oy;
+}
+
+lt_status deflect_points_device(const std::vector<DevLens> &lens, const double *pin_dev, double *pout_dev, long n) {
+    DevBuf<DevLens> dl;
+    LT_TRY(dl.upload(lens.data(), lens.size()));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_deflect_points, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), dl.p,
+                           (int)lens.size(), pin_dev, pout_dev, n);
+        LT_CHECK_LAUNCH();
+    }
+    return sync();
+}
+
+lt_status deflect_lattice_device(const DevLens *lens_dev, const int *lens_off_dev, int nsamples, const image_space &im,
+                                 double *lat_dev) {
+    const int lx = 2 * im.nx + 1, ly = 2 * im.ny + 1;
+    const long n = (long)lx * ly * nsamples;
+    if (n == 0) return LT_OK;
+    hipLaunchKernelGGL(k_deflect_lattice, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), lens_dev, lens_off_dev,
+                       nsamples, lx, ly, im.xmin, im.dx, im.ymin, im.dy, lat_dev);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+}  // namespace lt

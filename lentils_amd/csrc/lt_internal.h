@@ -1,0 +1,134 @@
+// Internal helpers shared by the HIP translation units of liblentils_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/lentils_hip.h"
+
+namespace lt {
+
+// ---------------------------------------------------------------------------
+// error reporting: every lt_* entry point returns a code and leaves a
+// message retrievable through lt_last_error_message()
+// ---------------------------------------------------------------------------
+void set_error(const char *fmt, ...);
+const char *last_error();
+
+#define LT_HIP(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess) {                                                                 \
+            ::lt::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(_e)); \
+            return LT_ERR_HIP;                                                                  \
+        }                                                                                       \
+    } while (0)
+
+#define LT_TRY(expr)                  \
+    do {                              \
+        lt_status _s = (expr);        \
+        if (_s != LT_OK) return _s;   \
+    } while (0)
+
+#define LT_CHECK_LAUNCH() LT_HIP(hipGetLastError())
+
+// library-wide device and stream (default priority; one process per GPU)
+lt_status ensure_init();
+hipStream_t stream();
+int device_cu_count();
+
+// owning device buffer
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    lt_status alloc(size_t count) {
+        if (count == n && p) return LT_OK;
+        release();
+        if (count == 0) return LT_OK;
+        LT_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+        return LT_OK;
+    }
+    lt_status zero() {
+        if (n) LT_HIP(hipMemsetAsync(p, 0, n * sizeof(T), stream()));
+        return LT_OK;
+    }
+    lt_status upload(const T *h, size_t count) {
+        LT_TRY(alloc(count));
+        if (count) LT_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+        return LT_OK;
+    }
+    lt_status download(T *h, size_t count, size_t offset = 0) const {
+        if (count) LT_HIP(hipMemcpyAsync(h, p + offset, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+        return LT_OK;
+    }
+};
+
+inline lt_status sync() {
+    LT_HIP(hipStreamSynchronize(stream()));
+    return LT_OK;
+}
+
+inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// ---------------------------------------------------------------------------
+// device-wide exclusive scan of int64 counts (scan.hip)
+// ---------------------------------------------------------------------------
+// out[i] = sum_{j<i} in[j], out[n] = total.  `in` and `out` may alias.
+lt_status exclusive_scan_i64(const int64_t *in, int64_t *out, long n);
+// inclusive running max of int32 values (used for "last non-empty row")
+lt_status inclusive_max_i32(const int32_t *in, int32_t *out, long n);
+// ascending sort of keys[seg[s]..seg[s+1]) for each segment (seg_host: the
+// same offsets on the host)
+lt_status segmented_sort_u64(uint64_t *keys, const int64_t *seg, const std::vector<int64_t> &seg_host);
+
+// ---------------------------------------------------------------------------
+// per-lens constants, precomputed on the host with the host libm so that the
+// per-lens pow() of deflect.c:284 is bit-identical to the reference
+// ---------------------------------------------------------------------------
+struct DevLens {
+    int type;  // 0 PEMD, 1 shear
+    int pad;
+    double x, y;
+    // PEMD
+    double q, qq, qt, t, tm2, b, bb, cr, ci;
+    // shear and Keeton gradient terms
+    double ss, c2, s2, gks, c_gka, s_gka, gss, c_gsa, s_gsa;
+};
+
+lt_status pack_lenses(const generic_mass_model *in, int n, std::vector<DevLens> &out);
+
+// CSR on device (block-diagonal batch of matrices in one array)
+struct DevCSR {
+    long nrows = 0, ncols = 0, nnz = 0;
+    DevBuf<int32_t> rp;  // nrows + 1 (global offsets)
+    DevBuf<int32_t> ci;  // global column indices
+    DevBuf<double> va;
+};
+
+// y = A x (FP64); optional accumulate y += alpha A x
+lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes_per_row);
+int csr_pick_lanes(const DevCSR &A);
+// B = A^T with column order inside each row ascending (deterministic)
+lt_status csr_transpose(const DevCSR &A, DevCSR &B);
+
+}  // namespace lt

@@ -1,0 +1,35 @@
+// Device-resident batch of lens matrices (one per lens-parameter sample).
+#pragma once
+
+#include "lt_internal.h"
+#include "raster_core.h"
+
+namespace lt {
+
+struct LensBatch {
+    image_space im{};
+    int nsamples = 0;
+    long npix = 0;
+    int nlong_rows = 0;
+    std::vector<SrcGrid> src_host;
+    std::vector<int64_t> srcoff_host;  // cumulative source sizes (ns + 1)
+    std::vector<int64_t> nnzoff_host;  // cumulative nonzeros (ns + 1)
+    std::vector<int> status_host;      // LT_RASTER_* bits per sample
+    DevBuf<uint8_t> mask;
+    DevBuf<SrcGrid> src;
+    DevBuf<int64_t> srcoff;
+    DevBuf<double> lattice;  // deflected stencil lattices, ns x (2nx+1)(2ny+1) x 2
+    DevCSR L;                // block diagonal: rows ns*npix, cols sum(nsrc)
+    DevCSR LT;               // transpose, built on demand
+
+    // lattice_host != nullptr: rasterize given lattices instead of deflecting
+    lt_status build(const image_space &im, const image_space *srcs, int ns, const generic_mass_model *lenses,
+                    const int *lens_counts, const double *lattice_host);
+    lt_status download(int s, int *row_inds, int *cols, double *vals) const;
+    lt_status ensure_transpose() {
+        if (LT.nrows == 0 && L.nrows > 0) return csr_transpose(L, LT);
+        return LT_OK;
+    }
+};
+
+}  // namespace lt

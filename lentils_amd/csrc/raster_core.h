@@ -1,0 +1,233 @@
+// Exact triangle rasterization of one image pixel into source-plane
+// fragments, written once for device and host.
+//
+// Follows lentils/backend/raster.c:120-269 (rasterize_triangle) and
+// :319-363 (9-point stencil, 8 fan triangles): a triangle is split
+// depth-first along source grid lines until each piece covers one source
+// pixel; each such piece emits {area, source index}.  The emission order is
+// part of the result (the reduction drops the first fragment of each column
+// group, raster.c:375-391), so the split tables and the LIFO order are kept.
+// Compile with -ffp-contract=off -fwrapv.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace lt {
+
+#define LT_HD __host__ __device__ inline
+
+constexpr int kRasterStack = 64;        // raster.c:11
+constexpr int kRasterFragCap = 128000;  // raster.c:12 (reported, not enforced)
+constexpr int kRasterPopLimit = 1 << 20;
+
+struct SrcGrid {
+    int nx, ny;
+    double xmin, ymin, dx, dy;
+};
+
+// (int) conversion with x86-64 cvttsd2si semantics (NaN / out of range ->
+// INT_MIN), which is what the reference's int = floor(...) executes.
+LT_HD int f2i_x86(double v) {
+    if (!(v >= -2147483648.0 && v < 2147483648.0)) return INT32_MIN;
+    return (int)v;
+}
+
+struct RTri {
+    double v[3][2];
+    double b[3][2];
+    int lo[2], hi[2];
+};
+
+// cut tables, raster.c:135-142 (indexed by the mask of vertices on the +
+// side of the cut line)
+struct CutTables {
+    int ncut[8];
+    int edge[8][2][2];
+    int nchild[8];
+    int vert[8][3][3];
+    int side[8][3];
+};
+
+#define LT_CUT_TABLES_INIT                                                                     \
+    {                                                                                          \
+        {0, 2, 2, 2, 2, 2, 2, 0},                                                              \
+        {{{0, 0}, {0, 0}}, {{0, 1}, {0, 2}}, {{0, 1}, {1, 2}}, {{0, 2}, {1, 2}},              \
+         {{0, 2}, {1, 2}}, {{0, 1}, {1, 2}}, {{0, 1}, {0, 2}}, {{0, 0}, {0, 0}}},             \
+        {1, 3, 3, 3, 3, 3, 3, 1},                                                              \
+        {{{0, 1, 2}, {0, 0, 0}, {0, 0, 0}}, {{1, 2, 4}, {1, 4, 3}, {0, 3, 4}},                \
+         {{0, 3, 4}, {0, 4, 2}, {1, 4, 3}}, {{2, 3, 4}, {0, 4, 3}, {0, 1, 4}},                \
+         {{0, 1, 4}, {0, 4, 3}, {2, 3, 4}}, {{1, 4, 3}, {0, 3, 4}, {0, 4, 2}},                \
+         {{0, 3, 4}, {1, 2, 4}, {1, 4, 3}}, {{0, 1, 2}, {0, 0, 0}, {0, 0, 0}}},               \
+        {{0, 0, 0}, {0, 0, 1}, {0, 0, 1}, {0, 1, 1}, {0, 0, 1}, {0, 1, 1}, {0, 1, 1}, {1, 0, 0}} \
+    }
+
+static __constant__ CutTables kCutTablesDev = LT_CUT_TABLES_INIT;
+static const CutTables kCutTablesHost = LT_CUT_TABLES_INIT;
+
+__host__ __device__ inline const CutTables &cut_tables() {
+#ifdef __HIP_DEVICE_COMPILE__
+    return kCutTablesDev;
+#else
+    return kCutTablesHost;
+#endif
+}
+
+// Rasterize one triangle (vertices tv[0..5]); sink(area, idx) is called per
+// fragment in emission order.  Returns LT_RASTER_* status bits.
+template <class Sink>
+__host__ __device__ int raster_triangle(const double *tv, const SrcGrid &g, Sink &sink) {
+    const CutTables &T = cut_tables();
+    RTri stk[kRasterStack];
+    RTri tri;
+    const double cell[2] = {g.dx, g.dy};
+    const double org[2] = {g.xmin, g.ymin};
+    tri.lo[0] = g.nx;
+    tri.lo[1] = g.ny;
+    tri.hi[0] = 0;
+    tri.hi[1] = 0;
+    tri.b[0][0] = 0.0; tri.b[0][1] = 0.0;
+    tri.b[1][0] = 0.0; tri.b[1][1] = 1.0;
+    tri.b[2][0] = 1.0; tri.b[2][1] = 0.0;
+    for (int v = 0; v < 3; ++v) {
+        tri.v[v][0] = tv[2 * v];
+        tri.v[v][1] = tv[2 * v + 1];
+        int i = f2i_x86(floor((tri.v[v][0] - g.xmin) / g.dx));
+        int j = f2i_x86(floor((tri.v[v][1] - g.ymin) / g.dy));
+        if (i < tri.lo[0]) tri.lo[0] = i;
+        if (j < tri.lo[1]) tri.lo[1] = j;
+        if (i + 1 > tri.hi[0]) tri.hi[0] = i + 1;
+        if (j + 1 > tri.hi[1]) tri.hi[1] = j + 1;
+    }
+    int ns = 0, pops = 0;
+    stk[ns++] = tri;
+    while (ns > 0) {
+        if (++pops > kRasterPopLimit) return 4;  // LT_RASTER_POP_LIMIT
+        tri = stk[--ns];
+        int ax = 0, ext = 0;
+        for (int a = 0; a < 2; ++a) {
+            int s = tri.hi[a] - tri.lo[a];
+            if (s > ext) {
+                ext = s;
+                ax = a;
+            }
+        }
+        const int cut = tri.lo[ax] + ext / 2;
+        const double cutc = cut * cell[ax] + org[ax];
+        if (ext == 1) {
+            const int i = tri.lo[0], j = tri.lo[1];
+            if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) continue;
+            const double x1 = tri.b[0][0], y1 = tri.b[0][1];
+            const double x2 = tri.b[1][0], y2 = tri.b[1][1];
+            const double x3 = tri.b[2][0], y3 = tri.b[2][1];
+            sink(-0.25 * (x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2)), g.ny * i + j);
+            continue;
+        }
+        double pv[5][2], pb[5][2], dist[3];
+        int mask = 0;
+        for (int v = 0; v < 3; ++v) {
+            pv[v][0] = tri.v[v][0];
+            pv[v][1] = tri.v[v][1];
+            pb[v][0] = tri.b[v][0];
+            pb[v][1] = tri.b[v][1];
+            dist[v] = pv[v][ax] - cutc;
+            mask |= ((dist[v] > 0.) << v);
+        }
+        const int sidx[3] = {tri.lo[ax], cut, tri.hi[ax]};
+        for (int k = 0; k < T.ncut[mask]; ++k) {
+            const int e0 = T.edge[mask][k][0], e1 = T.edge[mask][k][1];
+            const double w0 = dist[e1], w1 = -dist[e0];
+            const double wn = 1.0 / (w0 + w1);
+            const double a0 = w0 * wn, a1 = w1 * wn;
+            pv[3 + k][0] = a0 * pv[e0][0] + a1 * pv[e1][0];
+            pv[3 + k][1] = a0 * pv[e0][1] + a1 * pv[e1][1];
+            pb[3 + k][0] = a0 * pb[e0][0] + a1 * pb[e1][0];
+            pb[3 + k][1] = a0 * pb[e0][1] + a1 * pb[e1][1];
+        }
+        for (int c = 0; c < T.nchild[mask]; ++c) {
+            const int s = T.side[mask][c];
+            tri.lo[ax] = sidx[s];
+            tri.hi[ax] = sidx[s + 1];
+            if (tri.hi[0] <= 0 || tri.lo[0] >= g.nx) continue;
+            if (tri.hi[1] <= 0 || tri.lo[1] >= g.ny) continue;
+            for (int v = 0; v < 3; ++v) {
+                const int src = T.vert[mask][c][v];
+                tri.v[v][0] = pv[src][0];
+                tri.v[v][1] = pv[src][1];
+                tri.b[v][0] = pb[src][0];
+                tri.b[v][1] = pb[src][1];
+            }
+            stk[ns++] = tri;
+            if (ns >= kRasterStack) return 1;  // LT_RASTER_STACK_OVERFLOW
+        }
+    }
+    return 0;
+}
+
+// lattice offsets of the 9 stencil points of a pixel: corners, edge
+// midpoints, centre, in the order raster.c:319-347 lists them
+LT_HD void stencil_offsets(int k, int &ox, int &oy) {
+    const int dxo[9] = {0, 1, 2, 2, 2, 1, 0, 0, 1};
+    const int dyo[9] = {0, 0, 0, 1, 2, 2, 2, 1, 1};
+    ox = dxo[k];
+    oy = dyo[k];
+}
+
+// All fragments of image pixel (i, j) from the deflected lattice (x-major,
+// ly = 2 ny + 1 points per column): 8 fan triangles
+// (p[t+1], p[t], p[8]).  raster.c:351-363
+template <class Sink>
+__host__ __device__ int raster_pixel(const double *lat, int ly, int i, int j, const SrcGrid &g, Sink &sink) {
+    double sp[9][2];
+    for (int k = 0; k < 9; ++k) {
+        int ox, oy;
+        stencil_offsets(k, ox, oy);
+        const long a = 2L * i + ox, b = 2L * j + oy;
+        sp[k][0] = lat[2 * (a * ly + b)];
+        sp[k][1] = lat[2 * (a * ly + b) + 1];
+    }
+    int st = 0;
+    for (int t = 0; t < 8; ++t) {
+        const int t1 = (t + 1) & 7;
+        const double tv[6] = {sp[t1][0], sp[t1][1], sp[t][0], sp[t][1], sp[8][0], sp[8][1]};
+        st |= raster_triangle(tv, g, sink);
+    }
+    return st;
+}
+
+// Stable insertion sort of a short fragment list by source index followed
+// by the reference's reduction (raster.c:372-391).  Calls emit(col, val)
+// per CSR entry; returns the number of entries.  `stale` is the column a
+// row without fragments reports.
+template <class Emit>
+__host__ __device__ int reduce_fragments(double *area, int *idx, int n, int stale, Emit &emit) {
+    for (int a = 1; a < n; ++a) {
+        const double ar = area[a];
+        const int id = idx[a];
+        int b = a - 1;
+        while (b >= 0 && idx[b] > id) {
+            area[b + 1] = area[b];
+            idx[b + 1] = idx[b];
+            --b;
+        }
+        area[b + 1] = ar;
+        idx[b + 1] = id;
+    }
+    double val = 0.0;
+    int col = n > 0 ? idx[0] : stale;
+    int k = 0;
+    for (int f = 0; f < n; ++f) {
+        if (idx[f] != col) {
+            emit(k++, col, val);
+            val = 0.0;
+            col = idx[f];
+        } else {
+            val += area[f];
+        }
+    }
+    emit(k++, col, val);
+    return k;
+}
+
+}  // namespace lt
