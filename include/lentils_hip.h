@@ -157,6 +157,20 @@ lt_status lt_lens_matrix_from_lattice(image_space imspace, image_space srcspace,
                                       int *row_inds, int *cols, double *vals, long capacity, long *nnz,
                                       int *raster_status);
 
+/* Stand-alone pieces of the operator surface, host pointers.
+ * rfft2 / irfft2 of `batch` real nx x ny arrays (x-major) with the
+ * FFTOperator conventions, fourier_operators.py:12-24: forward unnormalised,
+ * inverse unnormalised (norm='forward'); the inverse treats columns 0 and
+ * ny/2 as Hermitian like numpy/scipy irfft2.  Complex data are interleaved
+ * complex128 of shape (nx, ny/2+1). */
+lt_status lt_fft_r2c_host(int nx, int ny, int batch, const double *in, double *out);
+lt_status lt_fft_c2r_host(int nx, int ny, int batch, const double *in, double *out);
+
+/* y = A x for a CSR matrix given by host arrays (Operator.apply on an
+ * explicit matrix, operator_base.py:96-98). */
+lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
+                           const double *x, double *y);
+
 /* --- device-resident handles ---------------------------------------------- */
 
 typedef struct lt_lensbatch_s *lt_lensbatch;

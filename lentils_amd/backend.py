@@ -106,6 +106,9 @@ def lib():
         'lt_solver_rhs': ([vp, vp], i),
         'lt_solver_apply': ([vp, vp, vp], i),
         'lt_solver_destroy': ([vp], i),
+        'lt_fft_r2c_host': ([i, i, i, vp, vp], i),
+        'lt_fft_c2r_host': ([i, i, i, vp, vp], i),
+        'lt_csr_spmv_host': ([l, l, vp, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -123,7 +126,8 @@ def exported_symbols():
             'lt_lensbatch_create', 'lt_lensbatch_info', 'lt_lensbatch_download', 'lt_lensbatch_destroy',
             'lt_nufft_create', 'lt_nufft_forward_host', 'lt_nufft_adjoint_host', 'lt_nufft_degrid_host',
             'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_solver_create',
-            'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_apply', 'lt_solver_destroy']
+            'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
+            'lt_fft_c2r_host', 'lt_csr_spmv_host']
 
 
 def check(status):

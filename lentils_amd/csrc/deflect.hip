@@ -16,37 +16,9 @@ namespace lt {
 lt_status pack_lenses(const generic_mass_model *in, int n, std::vector<DevLens> &out) {
     out.clear();
     for (int k = 0; k < n; ++k) {
-        const generic_mass_model &m = in[k];
-        const double *f = m.fpars;
-        DevLens L{};
-        L.type = (int)m.type;
-        L.x = f[0];
-        L.y = f[1];
-        if (m.type == 0) {
-            // lentils/backend/deflect.c:281-290 (per-lens part)
-            const double q = f[2], t = 2 * f[7];
-            L.q = q;
-            L.qq = q * q;
-            L.qt = (1 - q * q) / (q * q);
-            L.t = t;
-            L.tm2 = t - 2;
-            L.b = std::pow(0.5 * f[6] * std::pow(q, t - 0.5) * (3.0 - t) / (2.0 - t), 1.0 / t);
-            L.bb = L.b * L.b;
-            L.cr = f[5];
-            L.ci = -f[4];
-        } else if (m.type == 1) {
-            // lentils/backend/deflect.c:82-89; fpars layout lens_models.py:129-148
-            L.ss = f[2];
-            L.c2 = -f[5] * f[5] + f[4] * f[4];
-            L.s2 = -2 * f[4] * f[5];
-            L.gks = f[6];
-            L.s_gka = f[8];
-            L.c_gka = f[9];
-            L.gss = f[10];
-            L.s_gsa = f[12];
-            L.c_gsa = f[13];
-        } else {
-            set_error("lens type %u is not supported (0 = PEMD, 1 = external shear)", m.type);
+        DevLens L;
+        if (!pack_lens_one(in[k], L)) {
+            set_error("lens type %u is not supported (0 = PEMD, 1 = external shear)", in[k].type);
             return LT_ERR_UNSUPPORTED;
         }
         out.push_back(L);

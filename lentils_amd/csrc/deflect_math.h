@@ -18,6 +18,44 @@ namespace lt {
 
 #define LT_HD __host__ __device__ inline
 
+// per-lens constants of one mass-model record; returns false for an
+// unsupported type.  PEMD: deflect.c:281-290 (per-lens part, host pow so the
+// bits match the reference); shear: deflect.c:82-89 with the fpars layout of
+// lens_models.py:129-148.
+inline bool pack_lens_one(const generic_mass_model &m, DevLens &L) {
+    const double *f = m.fpars;
+    L = DevLens{};
+    L.type = (int)m.type;
+    L.x = f[0];
+    L.y = f[1];
+    if (m.type == 0) {
+        const double q = f[2], t = 2 * f[7];
+        L.q = q;
+        L.qq = q * q;
+        L.qt = (1 - q * q) / (q * q);
+        L.t = t;
+        L.tm2 = t - 2;
+        L.b = pow(0.5 * f[6] * pow(q, t - 0.5) * (3.0 - t) / (2.0 - t), 1.0 / t);
+        L.bb = L.b * L.b;
+        L.cr = f[5];
+        L.ci = -f[4];
+        return true;
+    }
+    if (m.type == 1) {
+        L.ss = f[2];
+        L.c2 = -f[5] * f[5] + f[4] * f[4];
+        L.s2 = -2 * f[4] * f[5];
+        L.gks = f[6];
+        L.s_gka = f[8];
+        L.c_gka = f[9];
+        L.gss = f[10];
+        L.s_gsa = f[12];
+        L.c_gsa = f[13];
+        return true;
+    }
+    return false;
+}
+
 LT_HD void cmul(double a, double b, double c, double d, double &re, double &im) {
     re = a * c - b * d;
     im = a * d + b * c;
@@ -91,12 +129,10 @@ LT_HD void csqrt_d(double x, double y, double &re, double &im) {
     im = copysign(s, y);
 }
 
-// x^y for the radial factor (b/rs)^(t-2); y = -1 (isothermal, gamma = 2) is
-// the exact reciprocal, which is what a correctly rounded pow returns.
-LT_HD double pow_radial(double x, double y) {
-    if (y == -1.0) return 1.0 / x;
-    return pow(x, y);
-}
+// x^y for the radial factor (b/rs)^(t-2).  pow is kept even for y = -1:
+// the reference calls pow, and glibc's pow(x, -1) differs from 1/x in the
+// last bit for ~0.1% of arguments.
+LT_HD double pow_radial(double x, double y) { return pow(x, y); }
 
 // 2F1(1, 2-t; 2-t/2; u) by 20 terms of its series.  deflect.c:179-239
 LT_HD void hyp2f1_series(const DevLens &L, double zr, double zi, double &fr, double &fi) {

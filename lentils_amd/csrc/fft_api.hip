@@ -1,0 +1,116 @@
+// Host-pointer entry points for the stand-alone operators of the drop-in
+// surface: 2-D real FFTs with the FFTOperator conventions
+// (fourier_operators.py:12-24: rfft2 unnormalised, irfft2 with norm='forward',
+// i.e. also unnormalised) and CSR SpMV (Operator.apply on an explicit
+// matrix, operator_base.py:96-98).
+#include <rocfft/rocfft.h>
+
+#include "lt_internal.h"
+
+namespace lt {
+namespace {
+
+__global__ void k_herm_col(int nu, int half_nv, int col, double *__restrict__ X) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nu / 2) return;
+    const int m = (nu - k) % nu;
+    double ar = X[2 * ((long)k * half_nv + col)], ai = X[2 * ((long)k * half_nv + col) + 1];
+    double br = X[2 * ((long)m * half_nv + col)], bi = X[2 * ((long)m * half_nv + col) + 1];
+    X[2 * ((long)k * half_nv + col)] = 0.5 * (ar + br);
+    X[2 * ((long)k * half_nv + col) + 1] = 0.5 * (ai - bi);
+    if (m != k) {
+        X[2 * ((long)m * half_nv + col)] = 0.5 * (br + ar);
+        X[2 * ((long)m * half_nv + col) + 1] = 0.5 * (bi - ai);
+    }
+}
+
+lt_status run_fft(int nx, int ny, int batch, bool forward, const double *in, double *out) {
+    LT_TRY(ensure_init());
+    if (nx <= 0 || ny <= 0 || batch <= 0) {
+        set_error("fft: bad shape");
+        return LT_ERR_ARG;
+    }
+    const int hn = ny / 2 + 1;
+    const long nr = (long)nx * ny * batch, nc = 2L * nx * hn * batch;
+    rocfft_setup();
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    const size_t len[2] = {(size_t)ny, (size_t)nx};
+    if (rocfft_plan_create(&plan, rocfft_placement_notinplace,
+                           forward ? rocfft_transform_type_real_forward : rocfft_transform_type_real_inverse,
+                           rocfft_precision_double, 2, len, (size_t)batch, nullptr) != rocfft_status_success) {
+        set_error("rocFFT plan creation failed");
+        return LT_ERR_FFT;
+    }
+    size_t ws = 0;
+    rocfft_plan_get_work_buffer_size(plan, &ws);
+    DevBuf<uint8_t> work;
+    DevBuf<double> din, dout;
+    lt_status st = work.alloc(ws ? ws : 1);
+    if (st == LT_OK) st = din.upload(in, forward ? nr : nc);
+    if (st == LT_OK) st = dout.alloc(forward ? nc : nr);
+    if (st == LT_OK && rocfft_execution_info_create(&info) != rocfft_status_success) st = LT_ERR_FFT;
+    if (st == LT_OK && ws) rocfft_execution_info_set_work_buffer(info, work.p, ws);
+    if (st == LT_OK) rocfft_execution_info_set_stream(info, stream());
+    if (st == LT_OK && !forward) {
+        // c2r: symmetrise columns 0 and ny/2 as numpy's irfft2 does implicitly
+        for (int b = 0; b < batch && st == LT_OK; ++b) {
+            double *X = din.p + 2L * nx * hn * b;
+            hipLaunchKernelGGL(k_herm_col, dim3(blocks_for(nx / 2 + 1, 256)), dim3(256), 0, stream(), nx, hn, 0, X);
+            if (ny % 2 == 0)
+                hipLaunchKernelGGL(k_herm_col, dim3(blocks_for(nx / 2 + 1, 256)), dim3(256), 0, stream(), nx, hn,
+                                   ny / 2, X);
+            if (hipGetLastError() != hipSuccess) st = LT_ERR_HIP;
+        }
+    }
+    if (st == LT_OK) {
+        void *i[1] = {din.p};
+        void *o[1] = {dout.p};
+        if (rocfft_execute(plan, i, o, info) != rocfft_status_success) st = LT_ERR_FFT;
+    }
+    if (st == LT_OK) st = dout.download(out, forward ? nc : nr);
+    if (st == LT_OK) st = sync();
+    if (info) rocfft_execution_info_destroy(info);
+    rocfft_plan_destroy(plan);
+    if (st == LT_ERR_FFT) set_error("rocFFT execution failed");
+    return st;
+}
+
+}  // namespace
+}  // namespace lt
+
+using namespace lt;
+
+extern "C" {
+
+lt_status lt_fft_r2c_host(int nx, int ny, int batch, const double *in, double *out) {
+    return run_fft(nx, ny, batch, true, in, out);
+}
+
+lt_status lt_fft_c2r_host(int nx, int ny, int batch, const double *in, double *out) {
+    return run_fft(nx, ny, batch, false, in, out);
+}
+
+lt_status lt_csr_spmv_host(long nrows, long ncols, const int *rp, const int *ci, const double *va, const double *x,
+                           double *y) {
+    LT_TRY(ensure_init());
+    if (nrows < 0 || ncols < 0 || !rp || !y || (ncols > 0 && !x)) {
+        set_error("lt_csr_spmv_host: bad arguments");
+        return LT_ERR_ARG;
+    }
+    DevCSR A;
+    A.nrows = nrows;
+    A.ncols = ncols;
+    A.nnz = rp[nrows];
+    LT_TRY(A.rp.upload(rp, nrows + 1));
+    LT_TRY(A.ci.upload(ci, A.nnz));
+    LT_TRY(A.va.upload(va, A.nnz));
+    DevBuf<double> dx, dy;
+    LT_TRY(dx.upload(x, ncols));
+    LT_TRY(dy.alloc(nrows > 0 ? nrows : 1));
+    LT_TRY(csr_spmv(A, dx.p, dy.p, 1.0, 0.0, 0));
+    LT_TRY(dy.download(y, nrows));
+    return sync();
+}
+
+}  // extern "C"

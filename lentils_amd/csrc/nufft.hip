@@ -1,0 +1,1174 @@
+// Kaiser-Bessel NUFFT on gfx950: degridding, atomic-free (in HBM) gridding,
+// apodization / zero padding, rocFFT r2c / c2r.
+//
+// Semantics follow lentils/backend/nufft.c:42-122 (grid_cpu) and
+// lentils/operators/fourier_operators.py:12-126:
+//   * u = U ch, v = -V ch; NGP by round(); taps ngp +- W; kernel
+//     1/(4W^2) I0(beta sqrt(1-(k/W)^2)) I0(...) with k = u/du - i_u;
+//     checkerboard sign (-1)^(i_u+i_v) on pre-wrap indices; phase
+//     2 pi (u gcx + v gcy) with the truncated 2 pi of common.h.
+//   * taps with wrapped i_v >= half_nv fold to ((nu - i_u) % nu, nv - i_v)
+//     with the imaginary weight negated; the adjoint doubles final
+//     column 0.
+// Layout: visibilities are binned once into uv tiles of TU x TV cells (raw,
+// centred tap coordinates) and kept in that order on the device.  A
+// workgroup takes a chunk of one tile: degridding stages the tile's
+// (TU+2W) x (TV+2W) window (fold / wrap / sign applied while staging) in LDS
+// and reads taps from it; gridding accumulates the window in LDS with LDS
+// atomics and writes it once as a partial.  A reduction gathers the partials
+// covering each final grid cell in a fixed order -- no global atomics, and
+// the result does not depend on scheduling.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "nufft_plan.h"
+
+namespace lt {
+
+namespace {
+
+constexpr double kTwoPiRef = 6.28318530718;  // lentils/backend/common.h:9
+constexpr int kNThreads = 256;
+
+__host__ __device__ inline uint32_t pack_ngp(int a, int b) {
+    return (uint32_t)(a + 32768) | ((uint32_t)(b + 32768) << 16);
+}
+__device__ inline void unpack_ngp(uint32_t p, int &a, int &b) {
+    a = (int)(p & 0xffff) - 32768;
+    b = (int)(p >> 16) - 32768;
+}
+
+struct Geo {
+    int nu, nv, half_nv, nrows, W, TU, TV, ntu, ntv, cu, cv, WU, WV, bu, bv;
+    double du, dv, gcx, gcy;
+};
+
+// pass 1: tile of every channel-expanded visibility, Nyquist-band check
+__global__ void k_vis_tile(Geo G, long nvis, const double *__restrict__ uvw, const double *__restrict__ chan,
+                           uint32_t *__restrict__ tile, int32_t *__restrict__ idx, int *__restrict__ bad) {
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nvis) return;
+    const int ch = (int)(e / G.nrows);
+    const long r = e - (long)ch * G.nrows;
+    const double u = uvw[3 * r] * chan[ch];
+    const double v = -uvw[3 * r + 1] * chan[ch];
+    const double tu = u / G.du, tv = v / G.dv;
+    idx[e] = (int32_t)e;
+    if (!(fabs(tu) < 1e9 && fabs(tv) < 1e9)) {
+        atomicOr(bad, 1);
+        tile[e] = 0;
+        return;
+    }
+    const int a = (int)round(tu), b = (int)round(tv);
+    if (abs(a) + G.W > G.bu || abs(b) + G.W > G.bv) {
+        atomicOr(bad, 1);
+        tile[e] = 0;
+        return;
+    }
+    tile[e] = (uint32_t)(((a + G.cu) / G.TU) * G.ntv + (b + G.cv) / G.TV);
+}
+
+// pass 2: per sorted visibility, NGP and exact fractional offsets
+template <class T>
+__global__ void k_vis_fill(Geo G, long nvis, const double *__restrict__ uvw, const double *__restrict__ chan,
+                           const int32_t *__restrict__ perm, int32_t *__restrict__ ngp, T *__restrict__ frac,
+                           T *__restrict__ ph) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nvis) return;
+    const long e = perm[p];
+    const int ch = (int)(e / G.nrows);
+    const long r = e - (long)ch * G.nrows;
+    const double u = uvw[3 * r] * chan[ch];
+    const double v = -uvw[3 * r + 1] * chan[ch];
+    const double tu = u / G.du, tv = v / G.dv;
+    const int a = (int)round(tu), b = (int)round(tv);
+    ngp[p] = (int32_t)pack_ngp(a, b);
+    frac[2 * p] = (T)(tu - a);  // exact in double (Sterbenz)
+    frac[2 * p + 1] = (T)(tv - b);
+    if (ph) {
+        const double arg = kTwoPiRef * (u * G.gcx + v * G.gcy);
+        ph[2 * p] = (T)cos(arg);
+        ph[2 * p + 1] = (T)sin(arg);
+    }
+}
+
+__global__ void k_count_tiles(long n, const uint32_t *__restrict__ tile, int32_t *__restrict__ cnt) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[tile[i]], 1);
+}
+
+template <class T>
+__device__ inline T kb_value(const KbPoly &P, T arg, T W) {
+    if (!(arg >= -W && arg <= W)) return (T)0;
+    T q = arg / W;
+    T s = (T)1 - q * q;
+    if (s < (T)0) s = (T)0;
+    T acc = (T)P.c[P.deg];
+    for (int m = P.deg - 1; m >= 0; --m) acc = acc * s + (T)P.c[m];
+    return acc;
+}
+
+// value of raw tap (a, b) of the half-plane grid, with the reference's
+// wrap/fold and the checkerboard sign applied; zero outside the band
+template <class T>
+__device__ inline Cplx<T> fetch_folded(const Geo &G, const Cplx<T> *__restrict__ grid, int a, int b) {
+    Cplx<T> z{(T)0, (T)0};
+    if (abs(a) > G.bu || abs(b) > G.bv) return z;
+    const T sg = ((a + b) & 1) ? (T)-1 : (T)1;
+    if (b >= 0) {
+        const int iu = a < 0 ? a + G.nu : a;
+        z = grid[(long)iu * G.half_nv + b];
+        z.re *= sg;
+        z.im *= sg;
+    } else {
+        const int iu = -a < 0 ? -a + G.nu : -a;
+        z = grid[(long)iu * G.half_nv - b];
+        z.re *= sg;
+        z.im *= -sg;
+    }
+    return z;
+}
+
+template <class T, int W>
+__global__ void __launch_bounds__(kNThreads) k_degrid(Geo G, KbPoly P, const Cplx<T> *__restrict__ grid,
+                                                      const int32_t *__restrict__ ck_tile,
+                                                      const int32_t *__restrict__ ck_beg,
+                                                      const int32_t *__restrict__ ck_end,
+                                                      const int32_t *__restrict__ ngp, const T *__restrict__ frac,
+                                                      const T *__restrict__ ph, const double *__restrict__ wts,
+                                                      Cplx<T> *__restrict__ vis, int accumulate) {
+    extern __shared__ unsigned char smem[];
+    Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem);
+    constexpr int NT = 2 * W + 1;
+    const int c = blockIdx.x;
+    const int tile = ck_tile[c];
+    const int tu = tile / G.ntv, tv = tile - tu * G.ntv;
+    const int a0 = tu * G.TU - W - G.cu, b0 = tv * G.TV - W - G.cv;  // raw coords of window (0, 0)
+    const int nwin = G.WU * G.WV;
+    for (int i = threadIdx.x; i < nwin; i += kNThreads) {
+        const int wa = i / G.WV, wb = i - wa * G.WV;
+        win[i] = fetch_folded(G, grid, a0 + wa, b0 + wb);
+    }
+    __syncthreads();
+    const int beg = ck_beg[c], end = ck_end[c];
+    const T Wt = (T)W;
+    const T c0 = (T)(1.0 / (4.0 * W * W));
+    for (int p = beg + threadIdx.x; p < end; p += kNThreads) {
+        int na, nb;
+        unpack_ngp((uint32_t)ngp[p], na, nb);
+        const T fu = frac[2 * (long)p], fv = frac[2 * (long)p + 1];
+        T ku[NT], kv[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            ku[k] = kb_value<T>(P, fu + (T)(W - k), Wt);
+            kv[k] = kb_value<T>(P, fv + (T)(W - k), Wt);
+        }
+        const int r0 = na - a0 - W, s0 = nb - b0 - W;  // window row/col of tap (0, 0)
+        T ar = 0, ai = 0;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const Cplx<T> *row = win + (r0 + k) * G.WV + s0;
+            T rr = 0, ri = 0;
+#pragma unroll
+            for (int l = 0; l < NT; ++l) {
+                const Cplx<T> g = row[l];
+                rr += kv[l] * g.re;
+                ri += kv[l] * g.im;
+            }
+            ar += ku[k] * rr;
+            ai += ku[k] * ri;
+        }
+        T cs = (T)1, sn = (T)0;
+        if (ph) {
+            cs = ph[2 * (long)p];
+            sn = ph[2 * (long)p + 1];
+        }
+        T outr = c0 * (cs * ar + sn * ai);
+        T outi = c0 * (-sn * ar + cs * ai);
+        if (wts) {
+            const T w = (T)wts[p];
+            outr *= w;
+            outi *= w;
+        }
+        if (accumulate) {
+            outr += vis[p].re;
+            outi += vis[p].im;
+        }
+        vis[p] = Cplx<T>{outr, outi};
+    }
+}
+
+__device__ inline int gcd_i(int a, int b) {
+    while (b) {
+        int t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+__device__ inline int coprime_stride(int n) {
+    if (n <= 2) return 1;
+    int s = (int)(((long)n * 40503L) >> 16);
+    if (s < 1) s = 1;
+    while (gcd_i(s, n) != 1) ++s;
+    return s;
+}
+
+template <class T>
+__device__ inline void lds_add(T *addr, T v) {
+    atomicAdd(addr, v);
+}
+
+template <class T, int W>
+__global__ void __launch_bounds__(kNThreads) k_grid(Geo G, KbPoly P, const Cplx<T> *__restrict__ vis,
+                                                    const double *__restrict__ wts,
+                                                    const int32_t *__restrict__ ck_tile,
+                                                    const int32_t *__restrict__ ck_beg,
+                                                    const int32_t *__restrict__ ck_end,
+                                                    const int32_t *__restrict__ ngp, const T *__restrict__ frac,
+                                                    const T *__restrict__ ph, Cplx<T> *__restrict__ partial) {
+    extern __shared__ unsigned char smem[];
+    T *win = reinterpret_cast<T *>(smem);  // interleaved re, im
+    constexpr int NT = 2 * W + 1;
+    const int c = blockIdx.x;
+    const int tile = ck_tile[c];
+    const int tu = tile / G.ntv, tv = tile - tu * G.ntv;
+    const int a0 = tu * G.TU - W - G.cu, b0 = tv * G.TV - W - G.cv;
+    const int nwin = G.WU * G.WV;
+    for (int i = threadIdx.x; i < 2 * nwin; i += kNThreads) win[i] = (T)0;
+    __syncthreads();
+    const int beg = ck_beg[c], end = ck_end[c];
+    const int n = end - beg;
+    const T Wt = (T)W;
+    // a stride coprime to n (near the golden ratio of n) spreads consecutive
+    // threads over the chunk so that a wavefront's LDS atomics rarely meet on
+    // one address; q -> q*stride mod n is a permutation of the chunk
+    const int stride = coprime_stride(n);
+    for (int q = threadIdx.x; q < n; q += kNThreads) {
+        const int p = beg + (int)(((long)q * stride) % n);
+        int na, nb;
+        unpack_ngp((uint32_t)ngp[p], na, nb);
+        const T fu = frac[2 * (long)p], fv = frac[2 * (long)p + 1];
+        Cplx<T> V = vis[p];
+        if (wts) {
+            const T w = (T)wts[p];
+            V.re *= w;
+            V.im *= w;
+        }
+        T cs = (T)1, sn = (T)0;
+        if (ph) {
+            cs = ph[2 * (long)p];
+            sn = ph[2 * (long)p + 1];
+        }
+        const T rr = (T)0.5 * (cs * V.re - sn * V.im);
+        const T ri = (T)0.5 * (sn * V.re + cs * V.im);
+        T kv[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) kv[k] = kb_value<T>(P, fv + (T)(W - k), Wt);
+        const int r0 = na - a0 - W, s0 = nb - b0 - W;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const T ku = kb_value<T>(P, fu + (T)(W - k), Wt);
+            const T tr = ku * rr, ti = ku * ri;
+            T *row = win + 2 * ((r0 + k) * G.WV + s0);
+#pragma unroll
+            for (int l = 0; l < NT; ++l) {
+                lds_add(row + 2 * l, tr * kv[l]);
+                lds_add(row + 2 * l + 1, ti * kv[l]);
+            }
+        }
+    }
+    __syncthreads();
+    // sign (-1)^(a+b) and 1/(4W^2) applied once per window cell
+    const T c0 = (T)(1.0 / (4.0 * W * W));
+    Cplx<T> *out = partial + (long)c * nwin;
+    for (int i = threadIdx.x; i < nwin; i += kNThreads) {
+        const int wa = i / G.WV, wb = i - wa * G.WV;
+        const T sg = (((a0 + wa) + (b0 + wb)) & 1) ? -c0 : c0;
+        out[i] = Cplx<T>{win[2 * i] * sg, win[2 * i + 1] * sg};
+    }
+}
+
+// sum of the partial windows of every chunk whose window covers raw cell
+// (a, b), in tile / chunk order
+template <class T>
+__device__ inline Cplx<T> gather_raw(const Geo &G, int a, int b, const int32_t *__restrict__ tile_ck0,
+                                     const int32_t *__restrict__ tile_ck1, const Cplx<T> *__restrict__ partial) {
+    Cplx<T> s{(T)0, (T)0};
+    const int ac = a + G.cu, bc = b + G.cv;
+    const int nwin = G.WU * G.WV;
+    const int W = G.W;
+    int tu_lo = (ac + W - G.WU + 1);
+    tu_lo = tu_lo <= 0 ? 0 : (tu_lo + G.TU - 1) / G.TU;
+    const int tu_hi = min((ac + W) / G.TU, G.ntu - 1);
+    int tv_lo = (bc + W - G.WV + 1);
+    tv_lo = tv_lo <= 0 ? 0 : (tv_lo + G.TV - 1) / G.TV;
+    const int tv_hi = min((bc + W) / G.TV, G.ntv - 1);
+    for (int tu = tu_lo; tu <= tu_hi; ++tu) {
+        const int wa = ac - (tu * G.TU - W);
+        for (int tv = tv_lo; tv <= tv_hi; ++tv) {
+            const int wb = bc - (tv * G.TV - W);
+            const int t = tu * G.ntv + tv;
+            const int k0 = tile_ck0[t], k1 = tile_ck1[t];
+            for (int k = k0; k < k1; ++k) {
+                const Cplx<T> v = partial[(long)k * nwin + wa * G.WV + wb];
+                s.re += v.re;
+                s.im += v.im;
+            }
+        }
+    }
+    return s;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kNThreads) k_grid_reduce(Geo G, const int32_t *__restrict__ tile_ck0,
+                                                           const int32_t *__restrict__ tile_ck1,
+                                                           const Cplx<T> *__restrict__ partial,
+                                                           Cplx<T> *__restrict__ grid) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)G.nu * G.half_nv) return;
+    const int iu = (int)(g / G.half_nv), iv = (int)(g - (long)iu * G.half_nv);
+    const int a = iu <= G.bu ? iu : iu - G.nu;
+    Cplx<T> tot{(T)0, (T)0};
+    if (abs(a) <= G.bu && iv <= G.bv) {
+        tot = gather_raw(G, a, iv, tile_ck0, tile_ck1, partial);
+        if (iv >= 1 && -a >= -G.bu && -a <= G.bu) {
+            const Cplx<T> f = gather_raw(G, -a, -iv, tile_ck0, tile_ck1, partial);
+            tot.re += f.re;
+            tot.im -= f.im;
+        }
+        if (iv == 0) {
+            tot.re *= (T)2;
+            tot.im *= (T)2;
+        }
+    }
+    grid[g] = tot;
+}
+
+// zero-padding * apodization into the real FFT input (whole buffer written)
+template <class T>
+__global__ void k_apod_pad(int nx, int ny, int padx, int pady, int nxp, int nyp, const double *__restrict__ img,
+                           const double *__restrict__ apod, T *__restrict__ out) {
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nxp * nyp) return;
+    const int i = (int)(g / nyp) - padx, j = (int)(g % nyp) - pady;
+    T v = (T)0;
+    if (i >= 0 && i < nx && j >= 0 && j < ny) v = (T)(img[(long)i * ny + j] * apod[(long)i * ny + j]);
+    out[g] = v;
+}
+
+// crop * apodization of the c2r output: img = alpha * crop(x) * apod (+ img)
+template <class T>
+__global__ void k_crop_apod(int nx, int ny, int padx, int pady, int nyp, const T *__restrict__ in,
+                            const double *__restrict__ apod, double alpha, int accumulate, double *__restrict__ img) {
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nx * ny) return;
+    const int i = (int)(g / ny), j = (int)(g % ny);
+    double v = alpha * ((double)in[(long)(i + padx) * nyp + (j + pady)] * apod[g]);
+    img[g] = accumulate ? img[g] + v : v;
+}
+
+// c2r treats columns 0 and nv/2 as Hermitian: make them so explicitly,
+// X[k] <- (X[k] + conj(X[-k])) / 2, which is what taking the real part of
+// the inverse transform along the first axis does (numpy/pocketfft)
+template <class T>
+__global__ void k_hermitian_cols(int nu, int half_nv, int col, Cplx<T> *__restrict__ X) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nu / 2) return;
+    const int m = (nu - k) % nu;
+    Cplx<T> a = X[(long)k * half_nv + col], b = X[(long)m * half_nv + col];
+    Cplx<T> na{(T)0.5 * (a.re + b.re), (T)0.5 * (a.im - b.im)};
+    Cplx<T> nb{(T)0.5 * (b.re + a.re), (T)0.5 * (b.im - a.im)};
+    X[(long)k * half_nv + col] = na;
+    if (m != k) X[(long)m * half_nv + col] = nb;
+}
+
+template <class T>
+__global__ void k_to_sorted(long n, const int32_t *__restrict__ perm, const double *__restrict__ src,
+                            Cplx<T> *__restrict__ dst) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long e = perm[p];
+    dst[p] = Cplx<T>{(T)src[2 * e], (T)src[2 * e + 1]};
+}
+
+template <class T>
+__global__ void k_from_sorted(long n, const int32_t *__restrict__ perm, const Cplx<T> *__restrict__ src,
+                              double *__restrict__ dst, int accumulate) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long e = perm[p];
+    double r = (double)src[p].re, i = (double)src[p].im;
+    if (accumulate) {
+        r += dst[2 * e];
+        i += dst[2 * e + 1];
+    }
+    dst[2 * e] = r;
+    dst[2 * e + 1] = i;
+}
+
+template <class T>
+__global__ void k_c128_to_T(long n, const double *__restrict__ a, T *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (T)a[i];
+}
+
+template <class T>
+__global__ void k_T_add_c128(long n, const T *__restrict__ a, double *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] += (double)a[i];
+}
+
+
+// Fused D^T C D in the spectral domain for one chunk of one sample: degrid
+// the staged input window, weight, and grid back into an LDS accumulator.
+// The phase factor cancels (e^{-i phi} then e^{+i phi}) and the kernel
+// weights are computed once per visibility for both directions.
+template <class T, int W>
+__global__ void __launch_bounds__(kNThreads) k_normal(Geo G, KbPoly P, const Cplx<T> *__restrict__ spec_in,
+                                                      long sstride, const int32_t *__restrict__ ck_tile,
+                                                      const int32_t *__restrict__ ck_beg,
+                                                      const int32_t *__restrict__ ck_end,
+                                                      const int32_t *__restrict__ ngp, const T *__restrict__ frac,
+                                                      const double *__restrict__ wts, Cplx<T> *__restrict__ partial,
+                                                      long pstride, const int *__restrict__ active) {
+    extern __shared__ unsigned char smem[];
+    constexpr int NT = 2 * W + 1;
+    const int nwin = G.WU * G.WV;
+    Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem);
+    T *acc = reinterpret_cast<T *>(smem + (size_t)nwin * sizeof(Cplx<T>));
+    const int smp = active ? active[blockIdx.y] : (int)blockIdx.y;
+    const Cplx<T> *grid = spec_in + (long)smp * sstride;
+    const int c = blockIdx.x;
+    const int tile = ck_tile[c];
+    const int tu = tile / G.ntv, tv = tile - tu * G.ntv;
+    const int a0 = tu * G.TU - W - G.cu, b0 = tv * G.TV - W - G.cv;
+    for (int i = threadIdx.x; i < nwin; i += kNThreads) {
+        const int wa = i / G.WV, wb = i - wa * G.WV;
+        win[i] = fetch_folded(G, grid, a0 + wa, b0 + wb);
+        acc[2 * i] = (T)0;
+        acc[2 * i + 1] = (T)0;
+    }
+    __syncthreads();
+    const int beg = ck_beg[c], end = ck_end[c];
+    const int n = end - beg;
+    const T Wt = (T)W;
+    const T c0h = (T)(0.5 / (4.0 * W * W));
+    const int stride = coprime_stride(n);
+    for (int q = threadIdx.x; q < n; q += kNThreads) {
+        const int p = beg + (int)(((long)q * stride) % n);
+        int na, nb;
+        unpack_ngp((uint32_t)ngp[p], na, nb);
+        const T fu = frac[2 * (long)p], fv = frac[2 * (long)p + 1];
+        T ku[NT], kv[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            ku[k] = kb_value<T>(P, fu + (T)(W - k), Wt);
+            kv[k] = kb_value<T>(P, fv + (T)(W - k), Wt);
+        }
+        const int r0 = na - a0 - W, s0 = nb - b0 - W;
+        T ar = 0, ai = 0;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const Cplx<T> *row = win + (r0 + k) * G.WV + s0;
+            T rr = 0, ri = 0;
+#pragma unroll
+            for (int l = 0; l < NT; ++l) {
+                const Cplx<T> g = row[l];
+                rr += kv[l] * g.re;
+                ri += kv[l] * g.im;
+            }
+            ar += ku[k] * rr;
+            ai += ku[k] * ri;
+        }
+        const T wv = (T)wts[p] * c0h;
+        ar *= wv;
+        ai *= wv;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const T tr = ku[k] * ar, ti = ku[k] * ai;
+            T *row = acc + 2 * ((r0 + k) * G.WV + s0);
+#pragma unroll
+            for (int l = 0; l < NT; ++l) {
+                lds_add(row + 2 * l, tr * kv[l]);
+                lds_add(row + 2 * l + 1, ti * kv[l]);
+            }
+        }
+    }
+    __syncthreads();
+    const T c0 = (T)(1.0 / (4.0 * W * W));
+    Cplx<T> *out = partial + (long)smp * pstride + (long)c * nwin;
+    for (int i = threadIdx.x; i < nwin; i += kNThreads) {
+        const int wa = i / G.WV, wb = i - wa * G.WV;
+        const T sg = (((a0 + wa) + (b0 + wb)) & 1) ? -c0 : c0;
+        out[i] = Cplx<T>{acc[2 * i] * sg, acc[2 * i + 1] * sg};
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kNThreads) k_grid_reduce_b(Geo G, const int32_t *__restrict__ tile_ck0,
+                                                             const int32_t *__restrict__ tile_ck1,
+                                                             const Cplx<T> *__restrict__ partial, long pstride,
+                                                             Cplx<T> *__restrict__ grid, long gstride,
+                                                             const int *__restrict__ active) {
+    const int smp = active ? active[blockIdx.y] : (int)blockIdx.y;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)G.nu * G.half_nv) return;
+    const Cplx<T> *pp = partial + (long)smp * pstride;
+    const int iu = (int)(g / G.half_nv), iv = (int)(g - (long)iu * G.half_nv);
+    const int a = iu <= G.bu ? iu : iu - G.nu;
+    Cplx<T> tot{(T)0, (T)0};
+    if (abs(a) <= G.bu && iv <= G.bv) {
+        tot = gather_raw(G, a, iv, tile_ck0, tile_ck1, pp);
+        if (iv >= 1) {
+            const Cplx<T> f = gather_raw(G, -a, -iv, tile_ck0, tile_ck1, pp);
+            tot.re += f.re;
+            tot.im -= f.im;
+        }
+        if (iv == 0) {
+            tot.re *= (T)2;
+            tot.im *= (T)2;
+        }
+    }
+    grid[(long)smp * gstride + g] = tot;
+}
+
+// batched zero padding (x apod when apod != nullptr)
+template <class T>
+__global__ void k_pad_b(int nx, int ny, int padx, int pady, int nxp, int nyp, const double *__restrict__ img,
+                        const double *__restrict__ apod, T *__restrict__ out) {
+    const int smp = blockIdx.y;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nxp * nyp) return;
+    const int i = (int)(g / nyp) - padx, j = (int)(g % nyp) - pady;
+    T v = (T)0;
+    if (i >= 0 && i < nx && j >= 0 && j < ny) {
+        const long k = (long)i * ny + j;
+        const double x = img[(long)smp * nx * ny + k];
+        v = (T)(apod ? x * apod[k] : x);
+    }
+    out[(long)smp * nxp * nyp + g] = v;
+}
+
+template <class T>
+__global__ void k_crop_b(int nx, int ny, int padx, int pady, int nxp, int nyp, const T *__restrict__ in,
+                         const double *__restrict__ apod, double alpha, int accumulate, double *__restrict__ img,
+                         const int *__restrict__ active) {
+    const int smp = active ? active[blockIdx.y] : (int)blockIdx.y;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nx * ny) return;
+    const int i = (int)(g / ny), j = (int)(g % ny);
+    double x = (double)in[(long)smp * nxp * nyp + (long)(i + padx) * nyp + (j + pady)];
+    double v = alpha * (apod ? x * apod[g] : x);
+    double *o = img + (long)smp * nx * ny + g;
+    *o = accumulate ? *o + v : v;
+}
+
+template <class T>
+__global__ void k_hermitian_cols_b(int nu, int half_nv, int col, Cplx<T> *__restrict__ X, long stride,
+                                   const int *__restrict__ active) {
+    const int smp = active ? active[blockIdx.y] : (int)blockIdx.y;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nu / 2) return;
+    Cplx<T> *Y = X + (long)smp * stride;
+    const int m = (nu - k) % nu;
+    Cplx<T> a = Y[(long)k * half_nv + col], b = Y[(long)m * half_nv + col];
+    Y[(long)k * half_nv + col] = Cplx<T>{(T)0.5 * (a.re + b.re), (T)0.5 * (a.im - b.im)};
+    if (m != k) Y[(long)m * half_nv + col] = Cplx<T>{(T)0.5 * (b.re + a.re), (T)0.5 * (b.im - a.im)};
+}
+
+template <class T>
+__global__ void k_spec_mul_b(long n, const double *__restrict__ mul, Cplx<T> *__restrict__ X, long stride,
+                             const int *__restrict__ active) {
+    const int smp = active ? active[blockIdx.y] : (int)blockIdx.y;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    Cplx<T> *Y = X + (long)smp * stride;
+    const double mr = mul[2 * g], mi = mul[2 * g + 1];
+    const double xr = (double)Y[g].re, xi = (double)Y[g].im;
+    Y[g] = Cplx<T>{(T)(xr * mr - xi * mi), (T)(xr * mi + xi * mr)};
+}
+
+Geo make_geo(const NufftPlan &P) {
+    Geo G;
+    G.nu = P.nu;
+    G.nv = P.nv;
+    G.half_nv = P.half_nv;
+    G.nrows = P.nrows;
+    G.W = P.W;
+    G.TU = P.TU;
+    G.TV = P.TV;
+    G.ntu = P.ntu;
+    G.ntv = P.ntv;
+    G.cu = P.cu;
+    G.cv = P.cv;
+    G.WU = P.WU;
+    G.WV = P.WV;
+    G.bu = (P.nu - 1) / 2;
+    G.bv = (P.nv - 1) / 2;
+    G.du = P.du;
+    G.dv = P.dv;
+    G.gcx = P.gcx;
+    G.gcy = P.gcy;
+    return G;
+}
+
+lt_status fft_check(rocfft_status s, const char *what) {
+    if (s != rocfft_status_success) {
+        set_error("rocFFT %s failed (status %d)", what, (int)s);
+        return LT_ERR_FFT;
+    }
+    return LT_OK;
+}
+
+lt_status kb_poly(double beta, int prec, KbPoly &P) {
+    // c_m = (beta^2/4)^m / (m!)^2; keep terms until the tail is below the
+    // target relative to I0(beta) (the value at s = 1)
+    const double q = 0.25 * beta * beta;
+    double c = 1.0, sum = 1.0;
+    P.c[0] = 1.0;
+    int m = 1;
+    for (; m < kMaxPoly; ++m) {
+        c *= q / ((double)m * (double)m);
+        P.c[m] = c;
+        sum += c;
+    }
+    const double eps = (prec == LT_F64) ? 1e-18 : 1e-9;
+    double tail = 0.0;
+    int deg = kMaxPoly - 1;
+    for (int k = kMaxPoly - 1; k >= 1; --k) {
+        tail += P.c[k];
+        if (tail > eps * sum) {
+            deg = k;
+            break;
+        }
+    }
+    if (deg >= kMaxPoly - 1) {
+        set_error("kb_beta %.3g too large for the kernel polynomial", beta);
+        return LT_ERR_ARG;
+    }
+    P.deg = deg;
+    return LT_OK;
+}
+
+template <class T>
+lt_status launch_degrid(NufftPlan &P, const void *grid, void *vis, const double *wts, bool acc) {
+    if (P.nchunks == 0) return LT_OK;
+    Geo G = make_geo(P);
+    const size_t sh = (size_t)P.WU * P.WV * sizeof(Cplx<T>);
+    const T *ph = P.phase ? (P.prec == LT_F64 ? (const T *)P.ph64.p : (const T *)P.ph32.p) : nullptr;
+    const T *fr = P.prec == LT_F64 ? (const T *)P.frac64.p : (const T *)P.frac32.p;
+#define LT_DEGRID_CASE(WW)                                                                                        \
+    case WW:                                                                                                      \
+        hipLaunchKernelGGL((k_degrid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, P.poly, \
+                           (const Cplx<T> *)grid, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph, wts,       \
+                           (Cplx<T> *)vis, acc ? 1 : 0);                                                          \
+        break;
+    switch (P.W) {
+        LT_DEGRID_CASE(1)
+        LT_DEGRID_CASE(2)
+        LT_DEGRID_CASE(3)
+        LT_DEGRID_CASE(4)
+        LT_DEGRID_CASE(5)
+        LT_DEGRID_CASE(6)
+        LT_DEGRID_CASE(7)
+        LT_DEGRID_CASE(8)
+        default:
+            set_error("kernel_support %d not supported (1..8)", P.W);
+            return LT_ERR_UNSUPPORTED;
+    }
+#undef LT_DEGRID_CASE
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+template <class T>
+lt_status launch_grid(NufftPlan &P, const void *vis, const double *wts, void *grid_out) {
+    Geo G = make_geo(P);
+    const T *ph = P.phase ? (P.prec == LT_F64 ? (const T *)P.ph64.p : (const T *)P.ph32.p) : nullptr;
+    const T *fr = P.prec == LT_F64 ? (const T *)P.frac64.p : (const T *)P.frac32.p;
+    if (P.nchunks > 0) {
+        const size_t sh = (size_t)P.WU * P.WV * sizeof(Cplx<T>);
+#define LT_GRID_CASE(WW)                                                                                        \
+    case WW:                                                                                                    \
+        hipLaunchKernelGGL((k_grid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, P.poly, \
+                           (const Cplx<T> *)vis, wts, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph,      \
+                           (Cplx<T> *)P.partial.p);                                                             \
+        break;
+        switch (P.W) {
+            LT_GRID_CASE(1)
+            LT_GRID_CASE(2)
+            LT_GRID_CASE(3)
+            LT_GRID_CASE(4)
+            LT_GRID_CASE(5)
+            LT_GRID_CASE(6)
+            LT_GRID_CASE(7)
+            LT_GRID_CASE(8)
+            default:
+                set_error("kernel_support %d not supported (1..8)", P.W);
+                return LT_ERR_UNSUPPORTED;
+        }
+#undef LT_GRID_CASE
+        LT_CHECK_LAUNCH();
+    }
+    const long ng = (long)P.nu * P.half_nv;
+    hipLaunchKernelGGL((k_grid_reduce<T>), dim3(blocks_for(ng, kNThreads)), dim3(kNThreads), 0, stream(), G,
+                       P.tile_ck0.p, P.tile_ck1.p, (const Cplx<T> *)P.partial.p, (Cplx<T> *)grid_out);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+}  // namespace
+
+NufftPlan::~NufftPlan() {
+    if (fwdB) rocfft_plan_destroy(fwdB);
+    if (invB) rocfft_plan_destroy(invB);
+    if (fwdB_info) rocfft_execution_info_destroy(fwdB_info);
+    if (invB_info) rocfft_execution_info_destroy(invB_info);
+    if (fwd) rocfft_plan_destroy(fwd);
+    if (inv) rocfft_plan_destroy(inv);
+    if (fwd_info) rocfft_execution_info_destroy(fwd_info);
+    if (inv_info) rocfft_execution_info_destroy(inv_info);
+}
+
+lt_status NufftPlan::create(const visibility_space &vs, const fourier_space &fs, int wsup, double kb_beta,
+                            int precision) {
+    LT_TRY(ensure_init());
+    if (wsup < 1 || wsup > 8) {
+        set_error("kernel_support must be in 1..8 (got %d)", wsup);
+        return LT_ERR_UNSUPPORTED;
+    }
+    if (precision != LT_F64 && precision != LT_F32) {
+        set_error("precision must be LT_F64 or LT_F32");
+        return LT_ERR_ARG;
+    }
+    if (fs.nu <= 2 * wsup + 2 || fs.nv <= 2 * wsup + 2 || fs.half_nv != fs.nv / 2 + 1 || fs.nu > 65000 ||
+        fs.nv > 65000) {
+        set_error("bad fourier space %d x %d (half %d)", fs.nu, fs.nv, fs.half_nv);
+        return LT_ERR_ARG;
+    }
+    prec = precision;
+    W = wsup;
+    beta = kb_beta;
+    nu = fs.nu;
+    nv = fs.nv;
+    half_nv = fs.half_nv;
+    du = fs.du;
+    dv = fs.dv;
+    gcx = fs.gcx;
+    gcy = fs.gcy;
+    phase = (gcx != 0.0 || gcy != 0.0);
+    nrows = vs.nrows;
+    nch = vs.nchannels;
+    nvis = (long)nrows * nch;
+    uvw_host.assign(vs.uv, vs.uv + 3L * nrows);
+    chan_host.assign(vs.channels, vs.channels + nch);
+    if (nvis >= INT32_MAX) {
+        set_error("too many visibilities (%ld)", nvis);
+        return LT_ERR_CAPACITY;
+    }
+    LT_TRY(kb_poly(beta, prec, poly));
+    cu = nu / 2;
+    cv = nv / 2;
+    TU = std::min(TU, nu);
+    TV = std::min(TV, nv);
+    ntu = (nu + TU - 1) / TU;
+    ntv = (nv + TV - 1) / TV;
+    WU = TU + 2 * W;
+    WV = TV + 2 * W;
+    const long ntiles = (long)ntu * ntv;
+    Geo G = make_geo(*this);
+    DevBuf<double> duvw, dch;
+    DevBuf<uint32_t> tile, tile_sorted;
+    DevBuf<int32_t> idx;
+    DevBuf<int> bad;
+    DevBuf<int32_t> cnt;
+    LT_TRY(duvw.upload(vs.uv, 3L * nrows));
+    LT_TRY(dch.upload(vs.channels, nch));
+    LT_TRY(tile.alloc(std::max<long>(nvis, 1)));
+    LT_TRY(tile_sorted.alloc(std::max<long>(nvis, 1)));
+    LT_TRY(idx.alloc(std::max<long>(nvis, 1)));
+    LT_TRY(perm.alloc(std::max<long>(nvis, 1)));
+    LT_TRY(bad.alloc(1));
+    LT_TRY(bad.zero());
+    LT_TRY(cnt.alloc(ntiles));
+    LT_TRY(cnt.zero());
+    if (nvis > 0) {
+        hipLaunchKernelGGL(k_vis_tile, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), G, nvis, duvw.p, dch.p,
+                           tile.p, idx.p, bad.p);
+        LT_CHECK_LAUNCH();
+        int hbad = 0;
+        LT_TRY(bad.download(&hbad, 1));
+        LT_TRY(sync());
+        if (hbad) {
+            set_error("visibility taps fall outside the Nyquist band of the %d x %d grid", nu, nv);
+            return LT_ERR_NYQUIST;
+        }
+        // stable radix sort by tile: inside a tile, visibilities keep the
+        // caller's order
+        unsigned bits = 1;
+        while ((1L << bits) < ntiles) ++bits;
+        size_t tmpb = 0;
+        LT_HIP(rocprim::radix_sort_pairs(nullptr, tmpb, tile.p, tile_sorted.p, idx.p, perm.p, (size_t)nvis, 0u, bits,
+                                         stream()));
+        DevBuf<uint8_t> tmp;
+        LT_TRY(tmp.alloc(std::max<size_t>(tmpb, 1)));
+        LT_HIP(rocprim::radix_sort_pairs(tmp.p, tmpb, tile.p, tile_sorted.p, idx.p, perm.p, (size_t)nvis, 0u, bits,
+                                         stream()));
+        hipLaunchKernelGGL(k_count_tiles, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, tile.p, cnt.p);
+        LT_CHECK_LAUNCH();
+    }
+    LT_TRY(ngp.alloc(std::max<long>(nvis, 1)));
+    if (prec == LT_F64) {
+        LT_TRY(frac64.alloc(2 * std::max<long>(nvis, 1)));
+        if (phase) LT_TRY(ph64.alloc(2 * std::max<long>(nvis, 1)));
+    } else {
+        LT_TRY(frac32.alloc(2 * std::max<long>(nvis, 1)));
+        if (phase) LT_TRY(ph32.alloc(2 * std::max<long>(nvis, 1)));
+    }
+    if (nvis > 0) {
+        if (prec == LT_F64)
+            hipLaunchKernelGGL(k_vis_fill<double>, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), G, nvis,
+                               duvw.p, dch.p, perm.p, ngp.p, frac64.p, phase ? ph64.p : nullptr);
+        else
+            hipLaunchKernelGGL(k_vis_fill<float>, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), G, nvis,
+                               duvw.p, dch.p, perm.p, ngp.p, frac32.p, phase ? ph32.p : nullptr);
+        LT_CHECK_LAUNCH();
+    }
+    // chunks: each tile's range split into near-equal pieces of <= chunk
+    std::vector<int32_t> hc(ntiles);
+    LT_TRY(cnt.download(hc.data(), ntiles));
+    LT_TRY(sync());
+    std::vector<int32_t> ct, cb, ce, t0(ntiles), t1(ntiles);
+    long pos = 0;
+    for (long t = 0; t < ntiles; ++t) {
+        t0[t] = (int32_t)ct.size();
+        const long n = hc[t];
+        if (n > 0) {
+            const long k = (n + chunk - 1) / chunk;
+            for (long j = 0; j < k; ++j) {
+                ct.push_back((int32_t)t);
+                cb.push_back((int32_t)(pos + n * j / k));
+                ce.push_back((int32_t)(pos + n * (j + 1) / k));
+            }
+        }
+        pos += n;
+        t1[t] = (int32_t)ct.size();
+    }
+    nchunks = (long)ct.size();
+    if (nchunks) {
+        LT_TRY(ck_tile.upload(ct.data(), nchunks));
+        LT_TRY(ck_beg.upload(cb.data(), nchunks));
+        LT_TRY(ck_end.upload(ce.data(), nchunks));
+    }
+    LT_TRY(tile_ck0.upload(t0.data(), ntiles));
+    LT_TRY(tile_ck1.upload(t1.data(), ntiles));
+    LT_TRY(partial.alloc(std::max<long>(nchunks, 1) * WU * WV * 2 * tsize()));
+    LT_TRY(spec.alloc((long)nu * half_nv * 2 * tsize()));
+    LT_TRY(visbuf.alloc(std::max<long>(nvis, 1) * 2 * tsize()));
+    return sync();
+}
+
+lt_status NufftPlan::attach_image(const image_space &im, int pad) {
+    if (pad < 2) {
+        set_error("pad_factor must be >= 2 (the reference's zero padding breaks for pad = 1)");
+        return LT_ERR_ARG;
+    }
+    has_image = true;
+    nx = im.nx;
+    ny = im.ny;
+    padx = ((pad - 1) * nx + 1) / 2;  // fourier_operators.py:34-35
+    pady = ((pad - 1) * ny + 1) / 2;
+    nxp = nx + 2 * padx;
+    nyp = ny + 2 * pady;
+    if (nxp != nu || nyp != nv) {
+        set_error("padded image %d x %d does not match the grid %d x %d", nxp, nyp, nu, nv);
+        return LT_ERR_ARG;
+    }
+    // ApodizationCorrectionOperator, fourier_operators.py:54-68
+    std::vector<double> ap((long)nx * ny), sx(nx), sy(ny);
+    for (int i = 0; i < nx; ++i) {
+        const double c = (i - 0.5 * nx) / nxp;
+        const double a = std::sqrt(beta * beta - (2 * M_PI * W * c) * (2 * M_PI * W * c));
+        sx[i] = std::sinh(a) / a;
+    }
+    for (int j = 0; j < ny; ++j) {
+        const double c = (j - 0.5 * ny) / nyp;
+        const double a = std::sqrt(beta * beta - (2 * M_PI * W * c) * (2 * M_PI * W * c));
+        sy[j] = std::sinh(a) / a;
+    }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) ap[(long)i * ny + j] = 1.0 / (sx[i] * sy[j]);
+    LT_TRY(apod.upload(ap.data(), ap.size()));
+    LT_TRY(realbuf.alloc((long)nxp * nyp * tsize()));
+    LT_TRY(imgbuf.alloc((long)nx * ny));
+    // rocFFT plans: row-major (nxp, nyp) real <-> (nxp, nyp/2+1) complex
+    LT_TRY(fft_check(rocfft_setup(), "setup"));
+    const size_t len[2] = {(size_t)nyp, (size_t)nxp};
+    const rocfft_precision rp = prec == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
+    LT_TRY(fft_check(rocfft_plan_create(&fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, rp, 2,
+                                        len, 1, nullptr),
+                     "plan (r2c)"));
+    LT_TRY(fft_check(rocfft_plan_create(&inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, rp, 2,
+                                        len, 1, nullptr),
+                     "plan (c2r)"));
+    size_t wf = 0, wi = 0;
+    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(fwd, &wf), "work size"));
+    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(inv, &wi), "work size"));
+    LT_TRY(fft_check(rocfft_execution_info_create(&fwd_info), "info"));
+    LT_TRY(fft_check(rocfft_execution_info_create(&inv_info), "info"));
+    if (wf) {
+        LT_TRY(fwd_work.alloc(wf));
+        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(fwd_info, fwd_work.p, wf), "work buffer"));
+    }
+    if (wi) {
+        LT_TRY(inv_work.alloc(wi));
+        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(inv_info, inv_work.p, wi), "work buffer"));
+    }
+    LT_TRY(fft_check(rocfft_execution_info_set_stream(fwd_info, stream()), "stream"));
+    LT_TRY(fft_check(rocfft_execution_info_set_stream(inv_info, stream()), "stream"));
+    return sync();
+}
+
+lt_status NufftPlan::degrid(const void *grid_in, void *vis, const double *w, bool acc) {
+    return prec == LT_F64 ? launch_degrid<double>(*this, grid_in, vis, w, acc)
+                          : launch_degrid<float>(*this, grid_in, vis, w, acc);
+}
+
+lt_status NufftPlan::grid(const void *vis, const double *w, void *grid_out) {
+    return prec == LT_F64 ? launch_grid<double>(*this, vis, w, grid_out) : launch_grid<float>(*this, vis, w, grid_out);
+}
+
+lt_status NufftPlan::forward(const double *img, void *vis, const double *w) {
+    if (!has_image) {
+        set_error("NUFFT plan has no image space");
+        return LT_ERR_ARG;
+    }
+    const long n = (long)nxp * nyp;
+    if (prec == LT_F64)
+        hipLaunchKernelGGL(k_apod_pad<double>, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), nx, ny, padx, pady,
+                           nxp, nyp, img, apod.p, (double *)realbuf.p);
+    else
+        hipLaunchKernelGGL(k_apod_pad<float>, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), nx, ny, padx, pady,
+                           nxp, nyp, img, apod.p, (float *)realbuf.p);
+    LT_CHECK_LAUNCH();
+    void *in[1] = {realbuf.p};
+    void *out[1] = {spec.p};
+    LT_TRY(fft_check(rocfft_execute(fwd, in, out, fwd_info), "execute r2c"));
+    return degrid(spec.p, vis, w, false);
+}
+
+lt_status NufftPlan::adjoint(const void *vis, const double *w, double *img, double alpha, bool acc) {
+    if (!has_image) {
+        set_error("NUFFT plan has no image space");
+        return LT_ERR_ARG;
+    }
+    LT_TRY(grid(vis, w, spec.p));
+    const int cols[2] = {0, nv / 2};
+    const int ncol = (nv % 2 == 0) ? 2 : 1;
+    for (int k = 0; k < ncol; ++k) {
+        if (prec == LT_F64)
+            hipLaunchKernelGGL(k_hermitian_cols<double>, dim3(blocks_for(nu / 2 + 1, 256)), dim3(256), 0, stream(), nu,
+                               half_nv, cols[k], (Cplx<double> *)spec.p);
+        else
+            hipLaunchKernelGGL(k_hermitian_cols<float>, dim3(blocks_for(nu / 2 + 1, 256)), dim3(256), 0, stream(), nu,
+                               half_nv, cols[k], (Cplx<float> *)spec.p);
+        LT_CHECK_LAUNCH();
+    }
+    void *in[1] = {spec.p};
+    void *out[1] = {realbuf.p};
+    LT_TRY(fft_check(rocfft_execute(inv, in, out, inv_info), "execute c2r"));
+    const long n = (long)nx * ny;
+    if (prec == LT_F64)
+        hipLaunchKernelGGL(k_crop_apod<double>, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), nx, ny, padx, pady,
+                           nyp, (const double *)realbuf.p, apod.p, alpha, acc ? 1 : 0, img);
+    else
+        hipLaunchKernelGGL(k_crop_apod<float>, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), nx, ny, padx, pady,
+                           nyp, (const float *)realbuf.p, apod.p, alpha, acc ? 1 : 0, img);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+
+lt_status NufftPlan::ensure_batch(int nb) {
+    if (nb <= batch) return LT_OK;
+    if (!has_image) {
+        set_error("NUFFT plan has no image space");
+        return LT_ERR_ARG;
+    }
+    if (fwdB) rocfft_plan_destroy(fwdB);
+    if (invB) rocfft_plan_destroy(invB);
+    if (fwdB_info) rocfft_execution_info_destroy(fwdB_info);
+    if (invB_info) rocfft_execution_info_destroy(invB_info);
+    fwdB = invB = nullptr;
+    fwdB_info = invB_info = nullptr;
+    const size_t len[2] = {(size_t)nyp, (size_t)nxp};
+    const rocfft_precision rp = prec == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
+    LT_TRY(fft_check(rocfft_plan_create(&fwdB, rocfft_placement_notinplace, rocfft_transform_type_real_forward, rp, 2,
+                                        len, (size_t)nb, nullptr),
+                     "batched plan (r2c)"));
+    LT_TRY(fft_check(rocfft_plan_create(&invB, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, rp, 2,
+                                        len, (size_t)nb, nullptr),
+                     "batched plan (c2r)"));
+    size_t wf = 0, wi = 0;
+    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(fwdB, &wf), "work size"));
+    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(invB, &wi), "work size"));
+    LT_TRY(fft_check(rocfft_execution_info_create(&fwdB_info), "info"));
+    LT_TRY(fft_check(rocfft_execution_info_create(&invB_info), "info"));
+    if (wf) {
+        LT_TRY(fwdB_work.alloc(wf));
+        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(fwdB_info, fwdB_work.p, wf), "work buffer"));
+    }
+    if (wi) {
+        LT_TRY(invB_work.alloc(wi));
+        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(invB_info, invB_work.p, wi), "work buffer"));
+    }
+    LT_TRY(fft_check(rocfft_execution_info_set_stream(fwdB_info, stream()), "stream"));
+    LT_TRY(fft_check(rocfft_execution_info_set_stream(invB_info, stream()), "stream"));
+    LT_TRY(realB.alloc((long)nb * nxp * nyp * tsize()));
+    LT_TRY(specB.alloc((long)nb * nu * half_nv * 2 * tsize()));
+    LT_TRY(specB2.alloc((long)nb * nu * half_nv * 2 * tsize()));
+    LT_TRY(partialB.alloc((long)nb * std::max<long>(nchunks, 1) * WU * WV * 2 * tsize()));
+    batch = nb;
+    return LT_OK;
+}
+
+template <class T>
+static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, const double *w, const int *active,
+                             int nactive, double alpha, bool acc, const double *spec_mul) {
+    const int nb = P.batch;
+    const long nreal = (long)P.nxp * P.nyp, nspec = (long)P.nu * P.half_nv, npix = (long)P.nx * P.ny;
+    const bool bcb = spec_mul != nullptr;
+    hipLaunchKernelGGL((k_pad_b<T>), dim3(blocks_for(nreal, 256), nb), dim3(256), 0, stream(), P.nx, P.ny, P.padx,
+                       P.pady, P.nxp, P.nyp, imgs, bcb ? nullptr : P.apod.p, (T *)P.realB.p);
+    LT_CHECK_LAUNCH();
+    void *in[1] = {P.realB.p};
+    void *o1[1] = {P.specB.p};
+    LT_TRY(fft_check(rocfft_execute(P.fwdB, in, o1, P.fwdB_info), "execute batched r2c"));
+    Cplx<T> *spec_out = (Cplx<T> *)P.specB2.p;
+    if (nactive > 0) {
+        Geo G = make_geo(P);
+        if (bcb) {
+            spec_out = (Cplx<T> *)P.specB.p;
+            hipLaunchKernelGGL((k_spec_mul_b<T>), dim3(blocks_for(nspec, 256), nactive), dim3(256), 0, stream(), nspec,
+                               spec_mul, spec_out, nspec, active);
+            LT_CHECK_LAUNCH();
+        } else {
+            const long pstride = std::max<long>(P.nchunks, 1) * P.WU * P.WV;
+            if (P.nchunks > 0) {
+                const size_t sh = (size_t)P.WU * P.WV * sizeof(Cplx<T>) * 2;
+                const T *fr = P.prec == LT_F64 ? (const T *)P.frac64.p : (const T *)P.frac32.p;
+#define LT_NORMAL_CASE(WW)                                                                                         \
+    case WW:                                                                                                       \
+        hipLaunchKernelGGL((k_normal<T, WW>), dim3((unsigned)P.nchunks, nactive), dim3(kNThreads), sh, stream(), G, \
+                           P.poly, (const Cplx<T> *)P.specB.p, nspec, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, \
+                           fr, w, (Cplx<T> *)P.partialB.p, pstride, active);                                       \
+        break;
+                switch (P.W) {
+                    LT_NORMAL_CASE(1)
+                    LT_NORMAL_CASE(2)
+                    LT_NORMAL_CASE(3)
+                    LT_NORMAL_CASE(4)
+                    LT_NORMAL_CASE(5)
+                    LT_NORMAL_CASE(6)
+                    LT_NORMAL_CASE(7)
+                    LT_NORMAL_CASE(8)
+                    default:
+                        set_error("kernel_support %d not supported (1..8)", P.W);
+                        return LT_ERR_UNSUPPORTED;
+                }
+#undef LT_NORMAL_CASE
+                LT_CHECK_LAUNCH();
+            }
+            hipLaunchKernelGGL((k_grid_reduce_b<T>), dim3(blocks_for(nspec, kNThreads), nactive), dim3(kNThreads), 0,
+                               stream(), G, P.tile_ck0.p, P.tile_ck1.p, (const Cplx<T> *)P.partialB.p, pstride,
+                               spec_out, nspec, active);
+            LT_CHECK_LAUNCH();
+        }
+        const int cols[2] = {0, P.nv / 2};
+        const int ncol = (P.nv % 2 == 0) ? 2 : 1;
+        for (int k = 0; k < ncol; ++k) {
+            hipLaunchKernelGGL((k_hermitian_cols_b<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0,
+                               stream(), P.nu, P.half_nv, cols[k], spec_out, nspec, active);
+            LT_CHECK_LAUNCH();
+        }
+    }
+    void *i2[1] = {(void *)spec_out};
+    void *o2[1] = {P.realB.p};
+    LT_TRY(fft_check(rocfft_execute(P.invB, i2, o2, P.invB_info), "execute batched c2r"));
+    if (nactive > 0) {
+        hipLaunchKernelGGL((k_crop_b<T>), dim3(blocks_for(npix, 256), nactive), dim3(256), 0, stream(), P.nx, P.ny,
+                           P.padx, P.pady, P.nxp, P.nyp, (const T *)P.realB.p, bcb ? nullptr : P.apod.p, alpha,
+                           acc ? 1 : 0, out, active);
+        LT_CHECK_LAUNCH();
+    }
+    (void)nb;
+    return LT_OK;
+}
+
+lt_status NufftPlan::normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
+                                    double alpha, bool acc) {
+    return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr)
+                          : normal_impl<float>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr);
+}
+
+lt_status NufftPlan::convolve_batched(const double *imgs, double *out, const double *mul, const int *active,
+                                      int nactive, double alpha, bool acc) {
+    return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, nullptr, active, nactive, alpha, acc, mul)
+                          : normal_impl<float>(*this, imgs, out, nullptr, active, nactive, alpha, acc, mul);
+}
+
+lt_status NufftPlan::to_sorted(const double *src, void *dst) {
+    if (nvis == 0) return LT_OK;
+    if (prec == LT_F64)
+        hipLaunchKernelGGL(k_to_sorted<double>, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, perm.p, src,
+                           (Cplx<double> *)dst);
+    else
+        hipLaunchKernelGGL(k_to_sorted<float>, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, perm.p, src,
+                           (Cplx<float> *)dst);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+lt_status NufftPlan::from_sorted(const void *src, double *dst, bool acc) {
+    if (nvis == 0) return LT_OK;
+    if (prec == LT_F64)
+        hipLaunchKernelGGL(k_from_sorted<double>, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, perm.p,
+                           (const Cplx<double> *)src, dst, acc ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_from_sorted<float>, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, perm.p,
+                           (const Cplx<float> *)src, dst, acc ? 1 : 0);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+// grid (complex128 host layout) <-> plan precision
+lt_status grid_c128_to_plan(NufftPlan &P, const double *src, void *dst) {
+    const long n = 2L * P.nu * P.half_nv;
+    if (P.prec == LT_F64) {
+        LT_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+    } else {
+        hipLaunchKernelGGL(k_c128_to_T<float>, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, src, (float *)dst);
+        LT_CHECK_LAUNCH();
+    }
+    return LT_OK;
+}
+
+lt_status grid_plan_add_c128(NufftPlan &P, const void *src, double *dst) {
+    const long n = 2L * P.nu * P.half_nv;
+    if (P.prec == LT_F64)
+        hipLaunchKernelGGL(k_T_add_c128<double>, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n,
+                           (const double *)src, dst);
+    else
+        hipLaunchKernelGGL(k_T_add_c128<float>, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n,
+                           (const float *)src, dst);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+}  // namespace lt

@@ -1,0 +1,187 @@
+// C ABI of the NUFFT plan and the grid_cpu drop-in symbol.
+#include <cmath>
+#include <memory>
+
+#include "nufft_plan.h"
+
+namespace lt {
+lt_status grid_c128_to_plan(NufftPlan &P, const double *src, void *dst);
+lt_status grid_plan_add_c128(NufftPlan &P, const void *src, double *dst);
+
+// fourier_space of the zero-padded image, space_base.py:62-84 and
+// fourier_operators.py:27-41
+fourier_space padded_fourier_space(const image_space &im, int pad) {
+    const double arcsec = 4.8481368111e-6;
+    const int padx = ((pad - 1) * im.nx + 1) / 2, pady = ((pad - 1) * im.ny + 1) / 2;
+    const double xmin = im.xmin - padx * im.dx, xmax = im.xmax + padx * im.dx;
+    const double ymin = im.ymin - pady * im.dy, ymax = im.ymax + pady * im.dy;
+    fourier_space fs{};
+    fs.nu = im.nx + 2 * padx;
+    fs.nv = im.ny + 2 * pady;
+    fs.half_nv = fs.nv / 2 + 1;
+    fs.nchannels = 1;
+    fs.nstokes = 1;
+    fs.du = 1.0 / (arcsec * (xmax - xmin));
+    fs.dv = 1.0 / (arcsec * (ymax - ymin));
+    fs.gcx = arcsec * 0.5 * (xmax + xmin);
+    fs.gcy = arcsec * 0.5 * (ymax + ymin);
+    fs.channels = nullptr;
+    return fs;
+}
+
+double default_kb_beta(int wsup, int pad) {  // fourier_operators.py:108
+    const double a = 2.0 * wsup / pad;
+    return M_PI * std::sqrt(a * a * (pad - 0.5) * (pad - 0.5) - 0.8);
+}
+
+}  // namespace lt
+
+using namespace lt;
+
+struct lt_nufft_s {
+    NufftPlan plan;
+};
+
+namespace lt {
+NufftPlan *nufft_of(lt_nufft h) { return h ? &h->plan : nullptr; }
+}  // namespace lt
+
+extern "C" {
+
+lt_status lt_nufft_create(visibility_space vs, image_space im, int pad, int wsup, double beta, int precision,
+                          lt_nufft *out) {
+    if (!out || im.nx <= 0 || im.ny <= 0 || vs.nrows < 0 || vs.nchannels <= 0 || !vs.channels ||
+        (vs.nrows > 0 && !vs.uv)) {
+        set_error("lt_nufft_create: bad arguments");
+        return LT_ERR_ARG;
+    }
+    if (beta <= 0) beta = default_kb_beta(wsup, pad);
+    auto h = std::make_unique<lt_nufft_s>();
+    fourier_space fs = padded_fourier_space(im, pad);
+    LT_TRY(h->plan.create(vs, fs, wsup, beta, precision));
+    LT_TRY(h->plan.attach_image(im, pad));
+    *out = h.release();
+    return LT_OK;
+}
+
+lt_status lt_nufft_info(lt_nufft h, int *nu, int *nv, int *half_nv, double *beta, long *nvis) {
+    if (!h) return LT_ERR_ARG;
+    if (nu) *nu = h->plan.nu;
+    if (nv) *nv = h->plan.nv;
+    if (half_nv) *half_nv = h->plan.half_nv;
+    if (beta) *beta = h->plan.beta;
+    if (nvis) *nvis = h->plan.nvis;
+    return LT_OK;
+}
+
+lt_status lt_nufft_forward_host(lt_nufft h, const double *img, double *vis) {
+    if (!h) return LT_ERR_ARG;
+    NufftPlan &P = h->plan;
+    DevBuf<double> out;
+    LT_TRY(P.imgbuf.upload(img, (long)P.nx * P.ny));
+    LT_TRY(out.alloc(2 * std::max<long>(P.nvis, 1)));
+    LT_TRY(P.forward(P.imgbuf.p, P.visbuf.p, nullptr));
+    LT_TRY(P.from_sorted(P.visbuf.p, out.p, false));
+    LT_TRY(out.download(vis, 2 * P.nvis));
+    return sync();
+}
+
+lt_status lt_nufft_adjoint_host(lt_nufft h, const double *vis, double *img) {
+    if (!h) return LT_ERR_ARG;
+    NufftPlan &P = h->plan;
+    DevBuf<double> in;
+    LT_TRY(in.upload(vis, 2 * P.nvis));
+    LT_TRY(P.to_sorted(in.p, P.visbuf.p));
+    LT_TRY(P.adjoint(P.visbuf.p, nullptr, P.imgbuf.p, 1.0, false));
+    LT_TRY(P.imgbuf.download(img, (long)P.nx * P.ny));
+    return sync();
+}
+
+lt_status lt_nufft_degrid_host(lt_nufft h, const double *grid, double *vis) {
+    if (!h) return LT_ERR_ARG;
+    NufftPlan &P = h->plan;
+    const long ng = 2L * P.nu * P.half_nv;
+    DevBuf<double> g, v;
+    LT_TRY(g.upload(grid, ng));
+    LT_TRY(v.upload(vis, 2 * P.nvis));
+    LT_TRY(grid_c128_to_plan(P, g.p, P.spec.p));
+    LT_TRY(P.degrid(P.spec.p, P.visbuf.p, nullptr, false));
+    LT_TRY(P.from_sorted(P.visbuf.p, v.p, true));
+    LT_TRY(v.download(vis, 2 * P.nvis));
+    return sync();
+}
+
+lt_status lt_nufft_grid_host(lt_nufft h, const double *vis, double *grid) {
+    if (!h) return LT_ERR_ARG;
+    NufftPlan &P = h->plan;
+    const long ng = 2L * P.nu * P.half_nv;
+    DevBuf<double> g, v;
+    LT_TRY(g.upload(grid, ng));
+    LT_TRY(v.upload(vis, 2 * P.nvis));
+    LT_TRY(P.to_sorted(v.p, P.visbuf.p));
+    LT_TRY(P.grid(P.visbuf.p, nullptr, P.spec.p));
+    LT_TRY(grid_plan_add_c128(P, P.spec.p, g.p));
+    LT_TRY(g.download(grid, ng));
+    return sync();
+}
+
+lt_status lt_nufft_time(lt_nufft h, int direction, int reps, double *ms) {
+    if (!h || reps <= 0 || !ms) return LT_ERR_ARG;
+    NufftPlan &P = h->plan;
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    // warm-up launch, then reps launches bracketed by events on the plan's stream
+    if (direction == 0)
+        LT_TRY(P.degrid(P.spec.p, P.visbuf.p, nullptr, false));
+    else
+        LT_TRY(P.grid(P.visbuf.p, nullptr, P.spec.p));
+    LT_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r) {
+        if (direction == 0)
+            LT_TRY(P.degrid(P.spec.p, P.visbuf.p, nullptr, false));
+        else
+            LT_TRY(P.grid(P.visbuf.p, nullptr, P.spec.p));
+    }
+    LT_HIP(hipEventRecord(e1, stream()));
+    LT_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    LT_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return LT_OK;
+}
+
+lt_status lt_nufft_destroy(lt_nufft h) {
+    delete h;
+    return LT_OK;
+}
+
+void grid_cpu(visibility_space vs, double *vis, fourier_space gs, double *grid, int wsup, double kb_beta,
+              int direction) {
+    NufftPlan P;
+    lt_status s = P.create(vs, gs, wsup, kb_beta, LT_F64);
+    if (s == LT_OK) {
+        const long ng = 2L * P.nu * P.half_nv;
+        DevBuf<double> g, v;
+        s = g.upload(grid, ng);
+        if (s == LT_OK) s = v.upload(vis, 2 * P.nvis);
+        if (s == LT_OK) {
+            if (direction == 0) {
+                s = P.degrid(g.p, P.visbuf.p, nullptr, false);
+                if (s == LT_OK) s = P.from_sorted(P.visbuf.p, v.p, true);
+                if (s == LT_OK) s = v.download(vis, 2 * P.nvis);
+            } else {
+                s = P.to_sorted(v.p, P.visbuf.p);
+                if (s == LT_OK) s = P.grid(P.visbuf.p, nullptr, P.spec.p);
+                if (s == LT_OK) s = grid_plan_add_c128(P, P.spec.p, g.p);
+                if (s == LT_OK) s = g.download(grid, ng);
+            }
+        }
+        if (s == LT_OK) s = sync();
+    }
+    if (s != LT_OK) fprintf(stderr, "lentils_hip: grid_cpu: %s\n", last_error());
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Kaiser-Bessel NUFFT plan: uv set binned into uv tiles, grid geometry,
+// apodization, rocFFT plans and scratch, all device resident.
+#pragma once
+
+#include <rocfft/rocfft.h>
+
+#include "lt_internal.h"
+
+namespace lt {
+
+template <class T>
+struct Cplx {
+    T re, im;
+};
+
+// Taylor coefficients of I0(beta*sqrt(s)) in s (all positive), truncated
+// where the tail is below the working precision relative to I0(beta).
+constexpr int kMaxPoly = 64;
+struct KbPoly {
+    double c[kMaxPoly];
+    int deg;
+};
+
+struct NufftPlan {
+    int prec = LT_F64;
+    int W = 4;
+    double beta = 0.0;
+    // image side (only when built from an image space)
+    bool has_image = false;
+    int nx = 0, ny = 0, padx = 0, pady = 0, nxp = 0, nyp = 0;
+    // Fourier grid (fourier_space of the padded image)
+    int nu = 0, nv = 0, half_nv = 0;
+    double du = 0, dv = 0, gcx = 0, gcy = 0;
+    // visibilities (channel-expanded: index = nrows * ch + row)
+    int nrows = 0, nch = 0;
+    long nvis = 0;
+    std::vector<double> uvw_host, chan_host;  // kept for derived plans (dirty beam)
+    bool phase = false;
+    // uv tiles of TU x TV grid cells in centred raw coordinates
+    int TU = 32, TV = 32, ntu = 0, ntv = 0, cu = 0, cv = 0, WU = 0, WV = 0;
+    int chunk = 8192;
+    long nchunks = 0;
+    KbPoly poly{};
+    DevBuf<int32_t> perm;  // sorted position -> channel-expanded index
+    DevBuf<int32_t> ngp;   // packed (ngp_u + 32768) | (ngp_v + 32768) << 16
+    DevBuf<double> frac64, ph64;
+    DevBuf<float> frac32, ph32;
+    DevBuf<int32_t> ck_tile, ck_beg, ck_end, tile_ck0, tile_ck1;
+    DevBuf<uint8_t> partial;  // nchunks x WU x WV complex T
+    DevBuf<double> apod;      // nx*ny
+    DevBuf<uint8_t> realbuf;  // nxp*nyp T
+    DevBuf<uint8_t> spec;     // nu*half_nv complex T
+    DevBuf<uint8_t> visbuf;   // nvis complex T (scratch for host-pointer applies)
+    DevBuf<double> imgbuf;    // nx*ny
+    rocfft_plan fwd = nullptr, inv = nullptr;
+    rocfft_execution_info fwd_info = nullptr, inv_info = nullptr;
+    DevBuf<uint8_t> fwd_work, inv_work;
+
+    // batched images (normal operator D^T C D inside the solver)
+    int batch = 0;
+    rocfft_plan fwdB = nullptr, invB = nullptr;
+    rocfft_execution_info fwdB_info = nullptr, invB_info = nullptr;
+    DevBuf<uint8_t> fwdB_work, invB_work, realB, specB, specB2, partialB;
+
+    ~NufftPlan();
+    size_t tsize() const { return prec == LT_F64 ? sizeof(double) : sizeof(float); }
+
+    lt_status create(const visibility_space &vs, const fourier_space &fs, int wsup, double kb_beta, int precision);
+    lt_status attach_image(const image_space &im, int pad_factor);
+
+    // device pointers; vis are complex T in sorted order
+    lt_status degrid(const void *grid, void *vis, const double *weights, bool accumulate);
+    lt_status grid(const void *vis, const double *weights, void *grid_out);
+    lt_status forward(const double *img, void *vis, const double *weights);
+    lt_status adjoint(const void *vis, const double *weights, double *img, double alpha, bool accumulate);
+    // out_s = alpha * D^T diag(w) D img_s (+ out_s) for the samples in
+    // `active` (device list of sample ids); images are nx*ny float64 each
+    lt_status ensure_batch(int nb);
+    lt_status normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
+                             double alpha, bool accumulate);
+    // out_s = crop(irfft2(spec_mul * rfft2(zpad(img_s)))) (BCB), spec_mul is
+    // nu x half_nv complex128
+    lt_status convolve_batched(const double *imgs, double *out, const double *spec_mul, const int *active,
+                               int nactive, double alpha, bool accumulate);
+    // permute between caller order (complex128) and sorted complex T
+    lt_status to_sorted(const double *vis_c128, void *sorted);
+    lt_status from_sorted(const void *sorted, double *vis_c128, bool accumulate);
+};
+
+}  // namespace lt

@@ -389,7 +389,7 @@ static void or_stencil(or_image_space im, int i, int j, double pts[9][2]) {
 typedef void (*or_stencil_fn)(void *ctx, int i, int j, double out[9][2]);
 
 static int or_lens_matrix_core(or_image_space im, or_image_space src, or_stencil_fn fn, void *ctx,
-                               int *row_inds, int *cols, double *vals, long *nnz_out) {
+                               int *row_inds, int *cols, double *vals, long *nnz_out, int drop) {
     or_fragbuf fb = {0};
     or_frag *tmp = NULL;
     int tmpcap = 0;
@@ -427,6 +427,7 @@ static int or_lens_matrix_core(or_image_space im, or_image_space src, or_stencil
                     k++;
                     val = 0.0;
                     col = fb.buf[f].idx;
+                    if (!drop) val += fb.buf[f].area;
                 } else {
                     val += fb.buf[f].area;
                 }
@@ -465,7 +466,7 @@ static void or_stencil_deflect(void *vctx, int i, int j, double out[9][2]) {
 int oracle_lens_matrix_csr(or_image_space im, or_image_space src, or_lens_model glm, int *row_inds, int *cols,
                            double *vals, long *nnz_out) {
     or_defl_ctx c = {im, glm, 0};
-    int st = or_lens_matrix_core(im, src, or_stencil_deflect, &c, row_inds, cols, vals, nnz_out);
+    int st = or_lens_matrix_core(im, src, or_stencil_deflect, &c, row_inds, cols, vals, nnz_out, 1);
     return st | (c.bad ? OR_RASTER_BAD_LENS : 0);
 }
 
@@ -490,7 +491,17 @@ static void or_stencil_lattice(void *vctx, int i, int j, double out[9][2]) {
 int oracle_lens_matrix_from_lattice(or_image_space im, or_image_space src, const double *lattice, int *row_inds,
                                     int *cols, double *vals, long *nnz_out) {
     or_lat_ctx c = {lattice, 2 * im.ny + 1};
-    return or_lens_matrix_core(im, src, or_stencil_lattice, &c, row_inds, cols, vals, nnz_out);
+    return or_lens_matrix_core(im, src, or_stencil_lattice, &c, row_inds, cols, vals, nnz_out, 1);
+}
+
+/* Same reduction without the reference's dropped first fragment of each
+ * later column group: every entry is then the exact clipped area of the
+ * fan triangles over that source pixel (x 0.25 per triangle).  Used only to
+ * pin the rasterization geometry against brute-force polygon clipping. */
+int oracle_lens_matrix_from_lattice_nodrop(or_image_space im, or_image_space src, const double *lattice,
+                                           int *row_inds, int *cols, double *vals, long *nnz_out) {
+    or_lat_ctx c = {lattice, 2 * im.ny + 1};
+    return or_lens_matrix_core(im, src, or_stencil_lattice, &c, row_inds, cols, vals, nnz_out, 0);
 }
 
 /* image-plane lattice of stencil points, (2nx+1) x (2ny+1), x-major, with

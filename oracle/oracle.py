@@ -103,6 +103,8 @@ def lib():
         L.oracle_lens_matrix_from_lattice.argtypes = [CImageSpace, CImageSpace, C.c_void_p, C.c_void_p,
                                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_long)]
         L.oracle_lens_matrix_from_lattice.restype = C.c_int
+        L.oracle_lens_matrix_from_lattice_nodrop.argtypes = L.oracle_lens_matrix_from_lattice.argtypes
+        L.oracle_lens_matrix_from_lattice_nodrop.restype = C.c_int
         L.oracle_stencil_lattice.argtypes = [CImageSpace, C.c_void_p]
         L.oracle_bessel_i0.argtypes = [C.c_double]
         L.oracle_bessel_i0.restype = C.c_double
@@ -212,6 +214,18 @@ def lens_matrix_from_lattice(im, src, lattice):
     va = np.zeros(max(nnz.value, 1), dtype=np.float64)
     st = lib().oracle_lens_matrix_from_lattice(im.cstruct(), src.cstruct(), _ptr(lattice), _ptr(rp), _ptr(ci),
                                                _ptr(va), C.byref(nnz))
+    return rp, ci[:nnz.value], va[:nnz.value], st
+
+
+def lens_matrix_from_lattice_nodrop(im, src, lattice):
+    lattice = np.ascontiguousarray(lattice, dtype=np.float64)
+    nnz = C.c_long(0)
+    f = lib().oracle_lens_matrix_from_lattice_nodrop
+    f(im.cstruct(), src.cstruct(), _ptr(lattice), None, None, None, C.byref(nnz))
+    rp = np.zeros(im.nx * im.ny + 1, dtype=np.int32)
+    ci = np.zeros(max(nnz.value, 1), dtype=np.int32)
+    va = np.zeros(max(nnz.value, 1), dtype=np.float64)
+    st = f(im.cstruct(), src.cstruct(), _ptr(lattice), _ptr(rp), _ptr(ci), _ptr(va), C.byref(nnz))
     return rp, ci[:nnz.value], va[:nnz.value], st
 
 

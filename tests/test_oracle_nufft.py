@@ -1,0 +1,80 @@
+"""Pins the oracle's NUFFT chain (CPU, no GPU).
+
+nufft.c's grid_cpu cannot be built here (it needs GSL), so the oracle's
+gridding is pinned by what the mathematics fixes:
+* against the brute-force DFT (dft_matrix_csr restated), the reference's own
+  test nufft_tests.py:43-67 at its sizes and its 1e-6 bar;
+* adjointness <N x, y> = <x, N^T y> in the real inner product;
+* I0 against scipy.special.i0; the kernel's support and normalisation.
+GSL's exact I0 bits remain unpinned (tolerances do not depend on them).
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.special as sps
+
+from helpers import reference_test_uv, rng
+
+
+def _dft(O, uv, ch, im):
+    rp, ci, va = O.dft_matrix(uv, ch, im)
+    return sp.csr_matrix((va, ci, rp), shape=(2 * uv.shape[0], im.nx * im.ny))
+
+
+def test_nufft_matches_dft_reference_test(oracle):
+    O = oracle
+    uv = reference_test_uv()
+    ch = [0.5]
+    im = O.Grid2D(236, 257, [-0.5, 0.7, -0.7, 0.5])
+    geo = O.NUFFTGeometry(im, pad=2, wsup=4)
+    D = _dft(O, uv, ch, im)
+    v = rng(9).normal(size=(uv.shape[0], 2)).view(np.complex128).reshape(-1)
+    dftvec = (D.T @ v.view(np.float64)).reshape(236, 257)
+    nuvec = O.nufft_adjoint(v.reshape(1, -1), uv, ch, geo)
+    assert O.max_relative_error(nuvec, dftvec) < 1e-6
+    nuvis = O.nufft_forward(dftvec, uv, ch, geo).reshape(-1)
+    dftvis = (D @ dftvec.reshape(-1)).view(np.complex128)
+    assert O.max_relative_error(nuvis, dftvis) < 1e-6
+
+
+def test_nufft_adjointness(oracle):
+    O = oracle
+    uv = reference_test_uv(seed=8)
+    im = O.Grid2D(64, 48, [-0.3, 0.34, -0.2, 0.28])
+    for wsup, beta in [(4, None), (3, 14.04)]:
+        geo = O.NUFFTGeometry(im, pad=2, wsup=wsup, beta=beta)
+        g = rng(10)
+        x = g.normal(size=(64, 48))
+        y = g.normal(size=(uv.shape[0], 2)).view(np.complex128).reshape(-1)
+        Nx = O.nufft_forward(x, uv, [1.0], geo).reshape(-1)
+        NTy = O.nufft_adjoint(y.reshape(1, -1), uv, [1.0], geo)
+        lhs = np.sum(Nx.real * y.real + Nx.imag * y.imag)
+        rhs = np.sum(x * NTy)
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs))
+
+
+def test_bessel_i0_and_kernel(oracle):
+    O = oracle
+    xs = np.linspace(0, 40, 2001)
+    ours = np.array([O.bessel_i0(x) for x in xs])
+    assert np.max(np.abs(ours / sps.i0(xs) - 1)) < 5e-15
+    beta, w = 14.04, 3.0
+    assert O.lib().oracle_kb_g(0.0, 0.0, beta, w) == 1.0 / (4 * w * w) * O.bessel_i0(beta) ** 2
+    assert O.lib().oracle_kb_g(3.0001, 0.0, beta, w) == 0.0
+    assert O.lib().oracle_kb_g(3.0, 0.0, beta, w) == 1.0 / (4 * w * w) * O.bessel_i0(beta)
+
+
+def test_curvature_matrix_is_laplacian(oracle):
+    O = oracle
+    im = O.Grid2D(5, 4, [-1, 1, -0.8, 0.8])
+    rp, ci, va = O.curv_matrix(im)
+    M = sp.csr_matrix((va, ci, rp), shape=(20, 20)).toarray()
+    h = 1.0 / (im.dx * im.dy)
+    assert np.allclose(M, M.T)
+    assert np.allclose(np.diag(M), -4 * h)
+    x = rng(3).normal(size=(5, 4))
+    lap = -4 * x
+    lap[1:] += x[:-1]
+    lap[:-1] += x[1:]
+    lap[:, 1:] += x[:, :-1]
+    lap[:, :-1] += x[:, 1:]
+    assert np.allclose(M @ x.reshape(-1), h * lap.reshape(-1))
