@@ -1,0 +1,238 @@
+"""Benchmark: source solves/s of the plain FP64 GPU method on the 64-lens
+workload (512^2 image at 0.01"/px, 128^2 source, 1e7 visibilities, 1024 x 513
+grid, KB W = 6 full width (kernel_support 3), beta = 14.04, curvature prior
+lambda = 1e2, FP64 CG to a true relative residual of 1e-6).
+
+One step = the whole hot path for one batch of 64 lens-parameter samples:
+deflect the stencil lattices, rasterize the 64 CSR lens matrices (and their
+transposes), form the dirty image and right-hand sides, and run the batched
+CG to convergence.  Inputs (uv set, data) are resident in HBM before timing.
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" fields).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NVIS = 10_000_000
+NLENS = 64
+
+
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def gen_lenses(n, seed=11):
+    from lentils_amd import ExternalPotential, GlobalLensModel, PowerLawEllipsoid
+    g = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n):
+        b = g.uniform(0.5, 1.5)
+        q = g.uniform(0.5, 0.95)
+        gam = g.uniform(1.8, 2.2)
+        th = g.uniform(0, 180)
+        x, y = g.uniform(-0.05, 0.05, 2)
+        ss, sa = g.uniform(0, 0.05), g.uniform(0, 180)
+        out.append(GlobalLensModel([PowerLawEllipsoid(b=b, f=q, th=th, qh=(gam - 1) / 2, x=x, y=y, rc=0.0, z=0.5),
+                                    ExternalPotential(ss=ss, sa=sa, x=x, y=y, z=0.5)]))
+    return out
+
+
+def gen_uv(n, seed=12):
+    g = np.random.Generator(np.random.PCG64(seed))
+    r = np.sqrt(g.uniform(2e4 ** 2, 2e6 ** 2, n))
+    ph = g.uniform(0, 2 * np.pi, n)
+    return np.stack([r * np.cos(ph), r * np.sin(ph), np.zeros(n)], 1)
+
+
+def blob_source(src, seed=14):
+    g = np.random.Generator(np.random.PCG64(seed))
+    p = src.points
+    img = np.zeros(p.shape[:2])
+    for _ in range(4):
+        c = g.uniform(-0.3, 0.3, 2)
+        s = g.uniform(0.05, 0.2)
+        a = g.uniform(0.5, 1.0)
+        img += a * np.exp(-np.sum((p - c) ** 2, axis=-1) / (2 * s * s))
+    return img
+
+
+class Workload:
+    def __init__(self, nvis=NVIS, nlens=NLENS):
+        from lentils_amd import ImageSpace, VisibilitySpace
+        from lentils_amd import backend as B
+        from lentils_amd.solvers import LensBatch, NUFFTPlan
+        self.B = B
+        self.im = ImageSpace(shape=(512, 512), bounds=[(-2.56, 2.56), (-2.56, 2.56)])
+        self.src = ImageSpace(shape=(128, 128), bounds=[(-0.64, 0.64), (-0.64, 0.64)])
+        self.uvw = gen_uv(nvis)
+        self.vs = VisibilitySpace(channels=[1.0], uvcoords=self.uvw)
+        self.lenses = gen_lenses(nlens)
+        self.kernel_support, self.kb_beta = 3, 14.04
+        t = time.perf_counter()
+        self.plan = NUFFTPlan(self.vs, self.im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f64')
+        log(f'NUFFT plan (uv binning) {time.perf_counter() - t:.2f}s')
+        # synthetic data: blobs lensed by the first sample, NUFFT, noise 1e-3
+        truth = LensBatch(self.im, [self.src], self.lenses[:1], z_src=2.0)
+        rp, ci, va = truth.csr(0)
+        x = np.ascontiguousarray(blob_source(self.src).reshape(-1))
+        img = np.zeros(self.im.size)
+        B.check(B.lib().lt_csr_spmv_host(self.im.size, self.src.size, B.ptr(rp), B.ptr(ci), B.ptr(va), B.ptr(x),
+                                         B.ptr(img)))
+        vis = np.zeros(nvis, dtype=np.complex128)
+        B.check(B.lib().lt_nufft_forward_host(self.plan._handle, B.ptr(img), B.ptr(vis)))
+        g = np.random.Generator(np.random.PCG64(13))
+        noise = (g.normal(0, 1e-3, size=(nvis, 2))).view(np.complex128).reshape(-1)
+        self.data = vis.astype(np.complex64).astype(np.complex128) + noise  # complex64 visibilities
+        self.weights = np.full(nvis, 1e6)
+        self.lambda_curv = 1e2
+
+    def step(self, rtol=1e-6, maxiter=4000, operator='nufft'):
+        from lentils_amd.solvers import BatchedSourceInversion, LensBatch
+        lb = LensBatch(self.im, [self.src] * len(self.lenses), self.lenses, z_src=2.0)
+        inv = BatchedSourceInversion(lb, self.plan, self.data, weights=self.weights,
+                                     lambda_curvature=self.lambda_curv, operator=operator)
+        x, its, res = inv.solve(rtol=rtol, maxiter=maxiter)
+        return its, res
+
+
+def degrid_roofline(wl, reps=20):
+    """FP32 degridder on the same uv set: algorithmic bytes per launch =
+    16 B per visibility (u, v in FP32; complex64 out) + 8 B per grid cell
+    (grid read once), over the event-timed mean launch duration."""
+    from lentils_amd.solvers import NUFFTPlan
+    p32 = NUFFTPlan(wl.vs, wl.im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f32')
+    ms = p32.time(0, reps)
+    nu, half = 1024, 513
+    bytes_ = 16.0 * NVIS + 8.0 * nu * half
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    ms64 = wl.plan.time(0, reps)
+    return {'ms': ms, 'gbs': gbs, 'bytes': bytes_, 'gvis_s': NVIS / (ms * 1e-3) / 1e9,
+            'gvis_s_f64': NVIS / (ms64 * 1e-3) / 1e9}
+
+
+def cpu_baseline(wl, gpu_iters):
+    """Reference CPU path on a bounded sample: one lens sample's deflection +
+    rasterization with the reference's own C (oracle/_ref, single thread)
+    or the oracle port, plus two CG iterations of the same operator with the
+    oracle's FP64 gridder (OpenMP over visibilities, all cores) and numpy
+    FFT / scipy SpMV; the solve time is extrapolated with the GPU's median
+    CG iteration count."""
+    from oracle import oracle as O
+    import scipy.sparse as sp
+    cores = len(os.sched_getaffinity(0))
+    im = O.Grid2D(512, 512, [-2.56, 2.56, -2.56, 2.56])
+    src = O.Grid2D(128, 128, [-0.64, 0.64, -0.64, 0.64])
+    lm = wl.lenses[0]
+    lm.setup_raytracing(2.0)
+    kind = 'reference' if O.ref_lib() is not None else 'port'
+    t0 = time.perf_counter()
+    if kind == 'reference':
+        rp, ci, va = O.ref_lens_matrix_csr(im, src, lm.lenses)
+    else:
+        rp, ci, va, _ = O.lens_matrix_csr(im, src, lm.lenses)
+    t_raster = time.perf_counter() - t0
+    Lm = sp.csr_matrix((va, ci, rp), shape=(512 * 512, 128 * 128))
+    crp, cci, cva = O.curv_matrix(src)
+    Cm = sp.csr_matrix((cva, cci, crp), shape=(128 * 128, 128 * 128))
+    geo = O.NUFFTGeometry(im, pad=2, wsup=3, beta=14.04)
+    x = np.random.default_rng(1).normal(size=128 * 128)
+    iters = 2
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        img = (Lm @ x).reshape(512, 512)
+        vis = O.nufft_forward(img, wl.uvw, [1.0], geo, nthreads=cores).reshape(-1) * wl.weights
+        back = O.nufft_adjoint(vis.reshape(1, -1), wl.uvw, [1.0], geo, nthreads=cores).reshape(-1)
+        y = Lm.T @ back + wl.lambda_curv * (Cm.T @ (Cm @ x))
+        x = y / np.linalg.norm(y)
+    t_iter = (time.perf_counter() - t0) / iters
+    solve_s = t_raster + float(np.median(gpu_iters)) * t_iter
+    cpu = ''
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    cpu = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        cpu = platform.processor()
+    return {'value': 1.0 / solve_s, 'unit': 'source solves/s', 'cores': cores, 'kind': kind,
+            'sample': (f'1 of the 64 lens samples: deflect+raster {t_raster:.2f}s '
+                       f'({"reference raster.c/deflect.c" if kind == "reference" else "oracle port"}, 1 thread) + '
+                       f'{iters} CG iterations of the explicit N^T C N operator on {cores} threads '
+                       f'({t_iter:.2f}s each, oracle FP64 gridder + numpy FFT + scipy SpMV), extrapolated to '
+                       f'{float(np.median(gpu_iters)):.0f} iterations (GPU median); CPU: {cpu}')}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--operator', default='nufft', choices=['nufft', 'bcb'])
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path (every solve of a batch is independent; '
+              'see DESIGN.md for how batches would shard). Run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+    from lentils_amd import backend as B
+    B.check(B.lib().lt_init(0))
+    t = time.perf_counter()
+    wl = Workload()
+    log(f'workload setup {time.perf_counter() - t:.1f}s')
+    for k in range(args.warmup):
+        t = time.perf_counter()
+        its, res = wl.step(operator=args.operator)
+        log(f'warmup {k}: {time.perf_counter() - t:.2f}s iters median {np.median(its):.0f} max res {res.max():.2e}')
+    times = []
+    all_its, all_res = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        its, res = wl.step(operator=args.operator)
+        times.append(time.perf_counter() - t)
+        all_its.append(its)
+        all_res.append(res)
+        log(f'step {k}: {times[-1]:.2f}s iters median {np.median(its):.0f} max {its.max()} max res {res.max():.2e}')
+    total = time.perf_counter() - t0
+    its = np.concatenate(all_its)
+    res = np.concatenate(all_res)
+    ms_per_step = 1e3 * total / args.steps
+    value = NLENS * args.steps / total
+    rl = degrid_roofline(wl)
+    log(f'degrid f32 {rl["ms"] * 1e3:.1f} us/launch, {rl["gbs"]:.0f} GB/s, {rl["gvis_s"]:.2f} Gvis/s')
+    out = {
+        'metric': 'source solves/s at 1e-6 rel resid (learned-PCG vs plain CG); Gvis/s degrid',
+        'value': value, 'unit': 'source solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (seeded lenses, uv annulus, Gaussian-blob source, complex64 vis + noise)',
+        'config': {'workload': 'plain GPU method: 64 lens samples, 512^2 image @0.01", 128^2 source, 1e7 vis, '
+                               '1024x513 grid, W=6 (support 3), beta=14.04, lambda_curv=1e2, FP64 CG to 1e-6',
+                   'operator': args.operator, 'precond': 'none (plain CG)', 'lens_samples_per_step': NLENS,
+                   'nvis': NVIS, 'parallelism': 'single GPU'},
+        'solver': {'iters_median': float(np.median(its)), 'iters_max': int(its.max()),
+                   'true_rel_resid_max': float(res.max()), 'all_converged': bool(np.all(res <= 1e-6))},
+        'degrid_gvis_s': rl['gvis_s'], 'degrid_gvis_s_f64': rl['gvis_s_f64'],
+        'roofline': {'kernel': 'k_degrid<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS, 'traffic': None},
+    }
+    if not args.no_cpu_baseline:
+        t = time.perf_counter()
+        out['cpu_baseline'] = cpu_baseline(wl, its)
+        log(f'cpu baseline {time.perf_counter() - t:.1f}s')
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -212,10 +212,14 @@ lt_status lt_nufft_destroy(lt_nufft h);
  *   (L^T D^T C D L + lambda_curv Cc^T Cc + lambda_tikh I) x = L^T D^T C d
  * with D the NUFFT, C = diag(weights) (sigma^-2 on unmasked visibilities,
  * data_base.py:36-44) and Cc the curvature matrix of each source grid.
- * op_mode LT_OP_NUFFT applies D^T C D explicitly through degrid/grid;
+ * op_mode LT_OP_NUFFT applies D^T C D exactly: for FP64 plans with
+ * kernel_support <= 4 through a grid-space stencil precomputed here from the
+ * visibilities and weights, otherwise through a fused per-visibility
+ * degrid-weight-grid pass (LT_OP_NUFFT_EXPLICIT forces the latter);
  * LT_OP_BCB uses the dirty-beam convolution (data_base.py:115-129). */
-#define LT_OP_NUFFT 0
-#define LT_OP_BCB 1
+#define LT_OP_NUFFT 0          /* exact D^T C D, precomputed grid stencil when possible */
+#define LT_OP_BCB 1            /* dirty-beam convolution */
+#define LT_OP_NUFFT_EXPLICIT 2 /* exact D^T C D, per-visibility degrid-weight-grid each apply */
 #define LT_PC_NONE 0
 #define LT_PC_JACOBI 1
 lt_status lt_solver_create(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *data, double lambda_curv,

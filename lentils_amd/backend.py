@@ -48,7 +48,7 @@ generic_mass_model_ctype = np.dtype([('type', 'u4'), ('_pad', 'u4'), ('z_l', 'f8
 
 LT_OK = 0
 LT_F64, LT_F32 = 0, 1
-LT_OP_NUFFT, LT_OP_BCB = 0, 1
+LT_OP_NUFFT, LT_OP_BCB, LT_OP_NUFFT_EXPLICIT = 0, 1, 2
 LT_PC_NONE, LT_PC_JACOBI = 0, 1
 RASTER_STACK_OVERFLOW, RASTER_FRAG_OVERFLOW, RASTER_POP_LIMIT, RASTER_BAD_LENS = 1, 2, 4, 8
 

@@ -143,6 +143,8 @@ struct Solver {
     DevBuf<double> rz, rz_new, pAp, rr, bb, alpha, beta;
     DevBuf<int> act, iters, all_active;
     double beam_peak = 0.0;
+    NormalStencil stencil;
+    bool use_stencil = false;
 
     lt_status create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data, double lc, double lt,
                      int op_mode);
@@ -171,7 +173,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
         set_error("solver: NUFFT image space does not match the lens image space");
         return LT_ERR_ARG;
     }
-    if (op_mode != LT_OP_NUFFT && op_mode != LT_OP_BCB) {
+    if (op_mode != LT_OP_NUFFT && op_mode != LT_OP_BCB && op_mode != LT_OP_NUFFT_EXPLICIT) {
         set_error("solver: unknown operator mode %d", op_mode);
         return LT_ERR_ARG;
     }
@@ -236,6 +238,10 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     }
     LT_TRY(nf->ensure_batch(S));
     if (mode == LT_OP_BCB) LT_TRY(beam_spectrum(*this, weights));
+    if (mode == LT_OP_NUFFT && nf->prec == LT_F64 && nf->W <= 4 && nv > 0) {
+        LT_TRY(stencil.build(*nf, w.p));
+        use_stencil = true;
+    }
     // work vectors
     LT_TRY(b.alloc(ntot));
     LT_TRY(x.alloc(ntot));
@@ -263,8 +269,8 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
 
 lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact) {
     LT_TRY(csr_spmv(lb->L, xin, img1.p, 1.0, 0.0, 0));
-    if (mode == LT_OP_NUFFT)
-        LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, alist, nact, 1.0, false));
+    if (mode != LT_OP_BCB)
+        LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, alist, nact, 1.0, false, use_stencil ? &stencil : nullptr));
     else
         LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, alist, nact, 1.0, false));
     LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0));

@@ -69,7 +69,11 @@ __global__ void k_vis_tile(Geo G, long nvis, const double *__restrict__ uvw, con
         tile[e] = 0;
         return;
     }
-    tile[e] = (uint32_t)(((a + G.cu) / G.TU) * G.ntv + (b + G.cv) / G.TV);
+    // tile-major cell key: tile id * (TU*TV) + cell inside the tile, so that
+    // a stable sort groups visibilities by tile and, inside it, by cell
+    const int ac = a + G.cu, bc = b + G.cv;
+    const uint32_t t = (uint32_t)((ac / G.TU) * G.ntv + bc / G.TV);
+    tile[e] = t * (uint32_t)(G.TU * G.TV) + (uint32_t)((ac % G.TU) * G.TV + bc % G.TV);
 }
 
 // pass 2: per sorted visibility, NGP and exact fractional offsets
@@ -96,9 +100,9 @@ __global__ void k_vis_fill(Geo G, long nvis, const double *__restrict__ uvw, con
     }
 }
 
-__global__ void k_count_tiles(long n, const uint32_t *__restrict__ tile, int32_t *__restrict__ cnt) {
+__global__ void k_count_tiles(long n, const uint32_t *__restrict__ key, uint32_t tt, int32_t *__restrict__ cnt) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[tile[i]], 1);
+    if (i < n) atomicAdd(&cnt[key[i] / tt], 1);
 }
 
 template <class T>
@@ -594,6 +598,240 @@ __global__ void k_spec_mul_b(long n, const double *__restrict__ mul, Cplx<T> *__
     Y[g] = Cplx<T>{(T)(xr * mr - xi * mi), (T)(xr * mi + xi * mr)};
 }
 
+// ---------------------------------------------------------------------------
+// precomputed normal stencil
+// ---------------------------------------------------------------------------
+
+__device__ inline void key_to_cell(const Geo &G, uint32_t key, int &ac, int &bc) {
+    const uint32_t tt = (uint32_t)(G.TU * G.TV);
+    const uint32_t t = key / tt, c = key - t * tt;
+    const int tu = (int)(t / G.ntv), tv = (int)(t - (uint32_t)tu * G.ntv);
+    ac = tu * G.TU + (int)(c / G.TV);
+    bc = tv * G.TV + (int)(c % G.TV);
+}
+
+__global__ void k_cell_flags(long n, const uint32_t *__restrict__ key, int64_t *__restrict__ flag) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) flag[p] = (p == 0 || key[p] != key[p - 1]) ? 1 : 0;
+}
+
+// cell list: first visibility of each cell, its centred raw coordinates,
+// and the dense cell map
+__global__ void k_cell_list(Geo G, long n, const uint32_t *__restrict__ key, const int64_t *__restrict__ idx,
+                            int32_t *__restrict__ cell_beg, int32_t *__restrict__ cellmap) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (p == 0 || key[p] != key[p - 1]) {
+        const long c = idx[p];
+        cell_beg[c] = (int32_t)p;
+        int ac, bc;
+        key_to_cell(G, key[p], ac, bc);
+        cellmap[(long)ac * G.nv + bc] = (int32_t)c;
+    }
+}
+
+// T_c = sum_{v in cell c} w_v z_v z_v^T with z_v = ku (x) kv (NT^2 values);
+// one wavefront-sized block per cell, thread i owns row i in registers
+template <int W>
+__global__ void __launch_bounds__(128) k_cell_T(Geo G, KbPoly P, long ncell, const int32_t *__restrict__ cell_beg,
+                                                const int32_t *__restrict__ frac_dummy, const double *__restrict__ frac,
+                                                const double *__restrict__ wts, double *__restrict__ Tout) {
+    constexpr int NT = 2 * W + 1, N2 = NT * NT, BATCH = 8;
+    __shared__ double ks[BATCH][2][NT];
+    __shared__ double zs[BATCH][N2];
+    __shared__ double ws[BATCH];
+    const long c = blockIdx.x;
+    const int beg = cell_beg[c], end = cell_beg[c + 1];
+    const int i = threadIdx.x;
+    double acc[N2];
+#pragma unroll
+    for (int j = 0; j < N2; ++j) acc[j] = 0.0;
+    const double Wt = (double)W;
+    for (int b0 = beg; b0 < end; b0 += BATCH) {
+        const int nb = min(BATCH, end - b0);
+        // kernel values: thread t -> (vis t / (2 NT), axis, tap)
+        for (int t = threadIdx.x; t < nb * 2 * NT; t += blockDim.x) {
+            const int v = t / (2 * NT), rem = t - v * 2 * NT, ax = rem / NT, k = rem - ax * NT;
+            const double f = frac[2 * (long)(b0 + v) + ax];
+            ks[v][ax][k] = kb_value<double>(P, f + (double)(W - k), Wt);
+        }
+        for (int t = threadIdx.x; t < nb; t += blockDim.x) ws[t] = wts[b0 + t];
+        __syncthreads();
+        for (int t = threadIdx.x; t < nb * N2; t += blockDim.x) {
+            const int v = t / N2, j = t - v * N2;
+            zs[v][j] = ks[v][0][j / NT] * ks[v][1][j % NT];
+        }
+        __syncthreads();
+        if (i < N2) {
+            for (int v = 0; v < nb; ++v) {
+                const double zi = ws[v] * zs[v][i];
+#pragma unroll
+                for (int j = 0; j < N2; ++j) acc[j] += zi * zs[v][j];
+            }
+        }
+        __syncthreads();
+    }
+    if (i < N2) {
+        double *o = Tout + c * (long)N2 * N2 + (long)i * N2;
+#pragma unroll
+        for (int j = 0; j < N2; ++j) o[j] = acc[j];
+    }
+}
+
+// mark raw cells within W of a visibility cell
+__global__ void k_mark_rows(Geo G, long ncell, const int32_t *__restrict__ cell_beg, const uint32_t *__restrict__ key,
+                            int64_t *__restrict__ mark) {
+    long c = blockIdx.x;
+    if (c >= ncell) return;
+    int ac, bc;
+    key_to_cell(G, key[cell_beg[c]], ac, bc);
+    const int NT = 2 * G.W + 1;
+    for (int t = threadIdx.x; t < NT * NT; t += blockDim.x) {
+        const int a = ac - G.W + t / NT, b = bc - G.W + t % NT;
+        if (a >= 0 && a < G.nu && b >= 0 && b < G.nv) mark[(long)a * G.nv + b] = 1;
+    }
+}
+
+__global__ void k_rows(long n, const int64_t *__restrict__ mark, const int64_t *__restrict__ off,
+                       int32_t *__restrict__ rowmap, int32_t *__restrict__ row_cell) {
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const bool on = (off[g + 1] - off[g]) != 0;
+    rowmap[g] = on ? (int32_t)off[g] : -1;
+    if (on) row_cell[off[g]] = (int32_t)g;
+    (void)mark;
+}
+
+// coef[o][row] = 0.5 c0^2 sum over cells c with both r' and r' + o in c's
+// footprint of T_c[r' - c][r' + o - c], summed in a fixed cell order
+template <int W>
+__global__ void k_coef(Geo G, long nrow, const int32_t *__restrict__ row_cell, const int32_t *__restrict__ cellmap,
+                       const double *__restrict__ T, double scale, double *__restrict__ coef) {
+    constexpr int NT = 2 * W + 1, N2 = NT * NT, R = 2 * W, NR = 2 * R + 1;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nrow * NR * NR) return;
+    const int o = (int)(g / nrow);
+    const long row = g - (long)o * nrow;
+    const int ou = o / NR - R, ov = o % NR - R;
+    const int rc = row_cell[row];
+    const int ru = rc / G.nv, rv = rc % G.nv;
+    const int su = ru + ou, sv = rv + ov;
+    double acc = 0.0;
+    const int u0 = max(ru, su) - W, u1 = min(ru, su) + W;
+    const int v0 = max(rv, sv) - W, v1 = min(rv, sv) + W;
+    for (int cu = u0; cu <= u1; ++cu) {
+        if (cu < 0 || cu >= G.nu) continue;
+        for (int cv = v0; cv <= v1; ++cv) {
+            if (cv < 0 || cv >= G.nv) continue;
+            const int c = cellmap[(long)cu * G.nv + cv];
+            if (c < 0) continue;
+            const int i = (ru - cu + W) * NT + (rv - cv + W);
+            const int j = (su - cu + W) * NT + (sv - cv + W);
+            acc += T[(long)c * N2 * N2 + (long)i * N2 + j];
+        }
+    }
+    coef[g] = scale * acc;
+}
+
+__global__ void k_stencil_tiles(long nrow, const int32_t *__restrict__ row_cell, int nv, int st, int ntv,
+                                int64_t *__restrict__ tmark) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const int rc = row_cell[r];
+    const int ru = rc / nv, rv = rc % nv;
+    tmark[(long)(ru / st) * ntv + rv / st] = 1;
+}
+
+__global__ void k_compact(long n, const int64_t *__restrict__ off, int32_t *__restrict__ out) {
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n && off[g + 1] != off[g]) out[off[g]] = (int32_t)g;
+}
+
+// R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) for NB samples per
+// block; S windows of the tile staged in LDS with fold / wrap / sign
+template <class T, int NB>
+__global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R, int ST, int ntv_st, const int32_t *__restrict__ tiles,
+                                                       const int32_t *__restrict__ rowmap, long nrow,
+                                                       const double *__restrict__ coef, const Cplx<T> *__restrict__ spec,
+                                                       long sstride, Cplx<T> *__restrict__ rbuf,
+                                                       const int *__restrict__ active, int nactive) {
+    extern __shared__ unsigned char smem[];
+    const int WS = ST + 2 * R;
+    Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem);
+    const int tile = tiles[blockIdx.x];
+    const int tu = tile / ntv_st, tv = tile - tu * ntv_st;
+    const int u0 = tu * ST - R, v0 = tv * ST - R;  // centred coords of window (0, 0)
+    int smp[NB];
+    int ns = 0;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int q = blockIdx.y * NB + k;
+        smp[k] = q < nactive ? (active ? active[q] : q) : -1;
+        if (smp[k] >= 0) ns = k + 1;
+    }
+    for (int k = 0; k < ns; ++k) {
+        const Cplx<T> *grid = spec + (long)smp[k] * sstride;
+        for (int i = threadIdx.x; i < WS * WS; i += blockDim.x) {
+            const int wa = i / WS, wb = i - wa * WS;
+            win[k * WS * WS + i] = fetch_folded(G, grid, u0 + wa - G.cu, v0 + wb - G.cv);
+        }
+    }
+    __syncthreads();
+    const int ly = threadIdx.x / ST, lx = threadIdx.x - ly * ST;
+    const int ru = tu * ST + ly, rv = tv * ST + lx;
+    if (ru >= G.nu || rv >= G.nv) return;
+    const int row = rowmap[(long)ru * G.nv + rv];
+    if (row < 0) return;
+    T ar[NB], ai[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) ar[k] = ai[k] = (T)0;
+    const int NR = 2 * R + 1;
+    for (int ou = 0; ou < NR; ++ou) {
+        for (int ov = 0; ov < NR; ++ov) {
+            const T cf = (T)coef[(long)(ou * NR + ov) * nrow + row];
+            const int wi = (ly + ou) * WS + (lx + ov);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const Cplx<T> sv = win[k * WS * WS + wi];
+                ar[k] += cf * sv.re;
+                ai[k] += cf * sv.im;
+            }
+        }
+    }
+    const T sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? (T)-1 : (T)1;
+    for (int k = 0; k < ns; ++k) rbuf[(long)smp[k] * nrow + row] = Cplx<T>{sg * ar[k], sg * ai[k]};
+}
+
+// final half-plane cells from their raw preimages (unfolded (a, iv) and the
+// folded mirror (-a, -iv)); column 0 doubled
+template <class T>
+__global__ void k_stencil_fold(Geo G, const int32_t *__restrict__ rowmap, long nrow, const Cplx<T> *__restrict__ rbuf,
+                               Cplx<T> *__restrict__ spec, long sstride, const int *__restrict__ active) {
+    const int smp = active ? active[blockIdx.y] : (int)blockIdx.y;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)G.nu * G.half_nv) return;
+    const int iu = (int)(g / G.half_nv), iv = (int)(g - (long)iu * G.half_nv);
+    const int a = iu <= G.bu ? iu : iu - G.nu;
+    Cplx<T> tot{(T)0, (T)0};
+    if (abs(a) <= G.bu && iv <= G.bv) {
+        const int r1 = rowmap[(long)(a + G.cu) * G.nv + (iv + G.cv)];
+        if (r1 >= 0) tot = rbuf[(long)smp * nrow + r1];
+        if (iv >= 1) {
+            const int r2 = rowmap[(long)(-a + G.cu) * G.nv + (-iv + G.cv)];
+            if (r2 >= 0) {
+                const Cplx<T> f = rbuf[(long)smp * nrow + r2];
+                tot.re += f.re;
+                tot.im -= f.im;
+            }
+        }
+        if (iv == 0) {
+            tot.re *= (T)2;
+            tot.im *= (T)2;
+        }
+    }
+    spec[(long)smp * sstride + g] = tot;
+}
+
 Geo make_geo(const NufftPlan &P) {
     Geo G;
     G.nu = P.nu;
@@ -784,14 +1022,14 @@ lt_status NufftPlan::create(const visibility_space &vs, const fourier_space &fs,
     const long ntiles = (long)ntu * ntv;
     Geo G = make_geo(*this);
     DevBuf<double> duvw, dch;
-    DevBuf<uint32_t> tile, tile_sorted;
+    DevBuf<uint32_t> tile;
     DevBuf<int32_t> idx;
     DevBuf<int> bad;
     DevBuf<int32_t> cnt;
     LT_TRY(duvw.upload(vs.uv, 3L * nrows));
     LT_TRY(dch.upload(vs.channels, nch));
     LT_TRY(tile.alloc(std::max<long>(nvis, 1)));
-    LT_TRY(tile_sorted.alloc(std::max<long>(nvis, 1)));
+    LT_TRY(key_sorted.alloc(std::max<long>(nvis, 1)));
     LT_TRY(idx.alloc(std::max<long>(nvis, 1)));
     LT_TRY(perm.alloc(std::max<long>(nvis, 1)));
     LT_TRY(bad.alloc(1));
@@ -809,18 +1047,19 @@ lt_status NufftPlan::create(const visibility_space &vs, const fourier_space &fs,
             set_error("visibility taps fall outside the Nyquist band of the %d x %d grid", nu, nv);
             return LT_ERR_NYQUIST;
         }
-        // stable radix sort by tile: inside a tile, visibilities keep the
-        // caller's order
+        // stable radix sort by (tile, cell): inside a cell, visibilities keep
+        // the caller's order
         unsigned bits = 1;
-        while ((1L << bits) < ntiles) ++bits;
+        while ((1L << bits) < ntiles * TU * TV) ++bits;
         size_t tmpb = 0;
-        LT_HIP(rocprim::radix_sort_pairs(nullptr, tmpb, tile.p, tile_sorted.p, idx.p, perm.p, (size_t)nvis, 0u, bits,
+        LT_HIP(rocprim::radix_sort_pairs(nullptr, tmpb, tile.p, key_sorted.p, idx.p, perm.p, (size_t)nvis, 0u, bits,
                                          stream()));
         DevBuf<uint8_t> tmp;
         LT_TRY(tmp.alloc(std::max<size_t>(tmpb, 1)));
-        LT_HIP(rocprim::radix_sort_pairs(tmp.p, tmpb, tile.p, tile_sorted.p, idx.p, perm.p, (size_t)nvis, 0u, bits,
+        LT_HIP(rocprim::radix_sort_pairs(tmp.p, tmpb, tile.p, key_sorted.p, idx.p, perm.p, (size_t)nvis, 0u, bits,
                                          stream()));
-        hipLaunchKernelGGL(k_count_tiles, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, tile.p, cnt.p);
+        hipLaunchKernelGGL(k_count_tiles, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, tile.p,
+                           (uint32_t)(TU * TV), cnt.p);
         LT_CHECK_LAUNCH();
     }
     LT_TRY(ngp.alloc(std::max<long>(nvis, 1)));
@@ -1040,7 +1279,7 @@ lt_status NufftPlan::ensure_batch(int nb) {
 
 template <class T>
 static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, const double *w, const int *active,
-                             int nactive, double alpha, bool acc, const double *spec_mul) {
+                             int nactive, double alpha, bool acc, const double *spec_mul, NormalStencil *st) {
     const int nb = P.batch;
     const long nreal = (long)P.nxp * P.nyp, nspec = (long)P.nu * P.half_nv, npix = (long)P.nx * P.ny;
     const bool bcb = spec_mul != nullptr;
@@ -1058,6 +1297,8 @@ static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, cons
             hipLaunchKernelGGL((k_spec_mul_b<T>), dim3(blocks_for(nspec, 256), nactive), dim3(256), 0, stream(), nspec,
                                spec_mul, spec_out, nspec, active);
             LT_CHECK_LAUNCH();
+        } else if (st) {
+            LT_TRY(st->apply(P, P.specB.p, nspec, spec_out, active, nactive));
         } else {
             const long pstride = std::max<long>(P.nchunks, 1) * P.WU * P.WV;
             if (P.nchunks > 0) {
@@ -1112,15 +1353,146 @@ static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, cons
 }
 
 lt_status NufftPlan::normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
-                                    double alpha, bool acc) {
-    return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr)
-                          : normal_impl<float>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr);
+                                    double alpha, bool acc, NormalStencil *st) {
+    return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr, st)
+                          : normal_impl<float>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr, st);
 }
 
 lt_status NufftPlan::convolve_batched(const double *imgs, double *out, const double *mul, const int *active,
                                       int nactive, double alpha, bool acc) {
-    return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, nullptr, active, nactive, alpha, acc, mul)
-                          : normal_impl<float>(*this, imgs, out, nullptr, active, nactive, alpha, acc, mul);
+    return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, nullptr, active, nactive, alpha, acc, mul, nullptr)
+                          : normal_impl<float>(*this, imgs, out, nullptr, active, nactive, alpha, acc, mul, nullptr);
+}
+
+lt_status NormalStencil::build(NufftPlan &P, const double *w_sorted) {
+    if (P.prec != LT_F64 || P.W > 4 || P.nvis == 0) {
+        set_error("normal stencil: needs an FP64 plan with kernel_support <= 4");
+        return LT_ERR_UNSUPPORTED;
+    }
+    Geo G = make_geo(P);
+    W = P.W;
+    R = 2 * W;
+    NO = (2 * R + 1) * (2 * R + 1);
+    const long n = P.nvis, ng = (long)P.nu * P.nv;
+    const int NT = 2 * W + 1, N2 = NT * NT;
+    DevBuf<int64_t> flag, idx;
+    LT_TRY(flag.alloc(n + 1));
+    LT_TRY(idx.alloc(n + 1));
+    hipLaunchKernelGGL(k_cell_flags, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, P.key_sorted.p, flag.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(flag.p, idx.p, n));
+    int64_t nc = 0;
+    LT_TRY(idx.download(&nc, 1, n));
+    LT_TRY(sync());
+    ncell = nc;
+    DevBuf<int32_t> cell_beg, cellmap;
+    LT_TRY(cell_beg.alloc(ncell + 1));
+    LT_TRY(cellmap.alloc(ng));
+    LT_HIP(hipMemsetAsync(cellmap.p, 0xff, ng * sizeof(int32_t), stream()));
+    hipLaunchKernelGGL(k_cell_list, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), G, n, P.key_sorted.p, idx.p,
+                       cell_beg.p, cellmap.p);
+    LT_CHECK_LAUNCH();
+    const int32_t nn = (int32_t)n;
+    LT_HIP(hipMemcpyAsync(cell_beg.p + ncell, &nn, sizeof(int32_t), hipMemcpyHostToDevice, stream()));
+    LT_TRY(sync());
+    // per-cell outer-product sums
+    DevBuf<double> T;
+    LT_TRY(T.alloc(ncell * (long)N2 * N2));
+    const int thr = N2 <= 64 ? 64 : 128;
+#define LT_CELLT_CASE(WW)                                                                                    \
+    case WW:                                                                                                 \
+        hipLaunchKernelGGL((k_cell_T<WW>), dim3((unsigned)ncell), dim3(thr), 0, stream(), G, P.poly, ncell, \
+                           cell_beg.p, (const int32_t *)nullptr, P.frac64.p, w_sorted, T.p);                 \
+        break;
+    switch (W) {
+        LT_CELLT_CASE(1)
+        LT_CELLT_CASE(2)
+        LT_CELLT_CASE(3)
+        LT_CELLT_CASE(4)
+    }
+#undef LT_CELLT_CASE
+    LT_CHECK_LAUNCH();
+    // stencil rows
+    DevBuf<int64_t> mark, off;
+    LT_TRY(mark.alloc(ng + 1));
+    LT_TRY(mark.zero());
+    LT_TRY(off.alloc(ng + 1));
+    hipLaunchKernelGGL(k_mark_rows, dim3((unsigned)ncell), dim3(64), 0, stream(), G, ncell, cell_beg.p,
+                       P.key_sorted.p, mark.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(mark.p, off.p, ng));
+    int64_t nr = 0;
+    LT_TRY(off.download(&nr, 1, ng));
+    LT_TRY(sync());
+    nrow = nr;
+    LT_TRY(rowmap.alloc(ng));
+    LT_TRY(row_cell.alloc(nrow));
+    hipLaunchKernelGGL(k_rows, dim3(blocks_for(ng, 256)), dim3(256), 0, stream(), ng, mark.p, off.p, rowmap.p,
+                       row_cell.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(coef.alloc((long)NO * nrow));
+    const double c0 = 1.0 / (4.0 * W * W);
+    const long ncoef = (long)NO * nrow;
+#define LT_COEF_CASE(WW)                                                                                          \
+    case WW:                                                                                                      \
+        hipLaunchKernelGGL((k_coef<WW>), dim3(blocks_for(ncoef, 256)), dim3(256), 0, stream(), G, nrow, row_cell.p, \
+                           cellmap.p, T.p, 0.5 * c0 * c0, coef.p);                                               \
+        break;
+    switch (W) {
+        LT_COEF_CASE(1)
+        LT_COEF_CASE(2)
+        LT_COEF_CASE(3)
+        LT_COEF_CASE(4)
+    }
+#undef LT_COEF_CASE
+    LT_CHECK_LAUNCH();
+    // stencil tiles
+    const int ntu_st = (P.nu + ST - 1) / ST, ntv_st = (P.nv + ST - 1) / ST;
+    const long nts = (long)ntu_st * ntv_st;
+    DevBuf<int64_t> tmark, toff;
+    LT_TRY(tmark.alloc(nts + 1));
+    LT_TRY(tmark.zero());
+    LT_TRY(toff.alloc(nts + 1));
+    hipLaunchKernelGGL(k_stencil_tiles, dim3(blocks_for(nrow, 256)), dim3(256), 0, stream(), nrow, row_cell.p, P.nv,
+                       ST, ntv_st, tmark.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(tmark.p, toff.p, nts));
+    int64_t nt = 0;
+    LT_TRY(toff.download(&nt, 1, nts));
+    LT_TRY(sync());
+    ntile = nt;
+    LT_TRY(tiles.alloc(std::max<long>(ntile, 1)));
+    hipLaunchKernelGGL(k_compact, dim3(blocks_for(nts, 256)), dim3(256), 0, stream(), nts, toff.p, tiles.p);
+    LT_CHECK_LAUNCH();
+    return sync();
+}
+
+lt_status NormalStencil::apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
+                               int nactive) {
+    if (P.prec != LT_F64) {
+        set_error("normal stencil: FP64 plans only");
+        return LT_ERR_UNSUPPORTED;
+    }
+    Geo G = make_geo(P);
+    const size_t need = (size_t)std::max(P.batch, 1) * nrow * sizeof(Cplx<double>);
+    if (rbuf.n < need) LT_TRY(rbuf.alloc(need));
+    constexpr int NB = 4;
+    const int WS = ST + 2 * R;
+    const int ntv_st = (P.nv + ST - 1) / ST;
+    if (ntile > 0 && nactive > 0) {
+        const size_t sh = (size_t)NB * WS * WS * sizeof(Cplx<double>);
+        hipLaunchKernelGGL((k_stencil_apply<double, NB>), dim3((unsigned)ntile, (nactive + NB - 1) / NB),
+                           dim3(ST * ST), sh, stream(), G, R, ST, ntv_st, tiles.p, rowmap.p, nrow, coef.p,
+                           (const Cplx<double> *)spec_in, sstride, (Cplx<double> *)rbuf.p, active, nactive);
+        LT_CHECK_LAUNCH();
+    }
+    const long nspec = (long)P.nu * P.half_nv;
+    if (nactive > 0) {
+        hipLaunchKernelGGL((k_stencil_fold<double>), dim3(blocks_for(nspec, 256), nactive), dim3(256), 0, stream(), G,
+                           rowmap.p, nrow, (const Cplx<double> *)rbuf.p, (Cplx<double> *)spec_out, sstride, active);
+        LT_CHECK_LAUNCH();
+    }
+    return LT_OK;
 }
 
 lt_status NufftPlan::to_sorted(const double *src, void *dst) {

@@ -21,6 +21,31 @@ struct KbPoly {
     int deg;
 };
 
+struct NufftPlan;
+
+// D^T C D restricted to the uv grid, precomputed once per (uv set, weights):
+// with the phase cancelling and a real kernel, degrid-weight-grid is a real,
+// symmetric, translation-variant stencil on raw (pre-fold) grid cells,
+//   R[r'] = sum_o coef[o][r'] S[r' + o],   o in [-2W, 2W]^2,
+//   coef[o][r'] = 0.5 / (4W^2)^2 sum_v w_v k_v(r') k_v(r' + o),
+// which replaces the per-visibility work inside every solver iteration by
+// a stencil over the ~(active cells) x (4W+1)^2 coefficients.
+struct NormalStencil {
+    int W = 0, R = 0, NO = 0;  // half-width, stencil radius 2W, (4W+1)^2 offsets
+    long ncell = 0;            // grid cells holding at least one visibility
+    long nrow = 0;             // raw cells within W of those (stencil rows)
+    int ST = 16;               // stencil tile edge
+    long ntile = 0;
+    DevBuf<int32_t> rowmap;    // centred raw cell -> row or -1 (nu * nv)
+    DevBuf<int32_t> row_cell;  // row -> centred raw cell index
+    DevBuf<int32_t> tiles;     // stencil tiles holding rows
+    DevBuf<double> coef;       // [NO][nrow]
+    DevBuf<uint8_t> rbuf;      // per-sample stencil output, [batch][nrow] complex T
+
+    lt_status build(NufftPlan &P, const double *w_sorted);
+    lt_status apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active, int nactive);
+};
+
 struct NufftPlan {
     int prec = LT_F64;
     int W = 4;
@@ -41,7 +66,8 @@ struct NufftPlan {
     int chunk = 8192;
     long nchunks = 0;
     KbPoly poly{};
-    DevBuf<int32_t> perm;  // sorted position -> channel-expanded index
+    DevBuf<int32_t> perm;        // sorted position -> channel-expanded index
+    DevBuf<uint32_t> key_sorted;  // tile-major cell key of each sorted visibility
     DevBuf<int32_t> ngp;   // packed (ngp_u + 32768) | (ngp_v + 32768) << 16
     DevBuf<double> frac64, ph64;
     DevBuf<float> frac32, ph32;
@@ -77,7 +103,7 @@ struct NufftPlan {
     // `active` (device list of sample ids); images are nx*ny float64 each
     lt_status ensure_batch(int nb);
     lt_status normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
-                             double alpha, bool accumulate);
+                             double alpha, bool accumulate, NormalStencil *stencil = nullptr);
     // out_s = crop(irfft2(spec_mul * rfft2(zpad(img_s)))) (BCB), spec_mul is
     // nu x half_nv complex128
     lt_status convolve_batched(const double *imgs, double *out, const double *spec_mul, const int *active,

@@ -122,7 +122,7 @@ class BatchedSourceInversion:
             w = np.asarray(weights, dtype=np.float64).reshape(-1)
         self._w = np.ascontiguousarray(w)
         self._d = d
-        mode = {'nufft': B.LT_OP_NUFFT, 'bcb': B.LT_OP_BCB}[operator]
+        mode = {'nufft': B.LT_OP_NUFFT, 'bcb': B.LT_OP_BCB, 'nufft_explicit': B.LT_OP_NUFFT_EXPLICIT}[operator]
         h = C.c_void_p()
         B.check(B.lib().lt_solver_create(lens_batch._handle, nufft_plan._handle, B.ptr(self._w), B.ptr(self._d),
                                          float(lambda_curvature), float(lambda_tikhonov), mode, C.byref(h)))

@@ -1,0 +1,122 @@
+"""GPU parity of the NUFFT (C ABI and the NUFFTOperator drop-in).
+
+Bars: FP64 plan within 1e-12 (max-relative, test_utils.py:13-20) of the FP64
+oracle; FP32 plan within 1e-5; both within the reference's own 1e-6 of the
+brute-force DFT at the reference test's sizes (nufft_tests.py:43-67);
+adjointness; run-to-run bitwise determinism of the atomic-free gridder."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import annulus_uv, reference_test_uv, rng
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan(B, uv, ch, nx, ny, bounds, wsup, beta, prec):
+    from lentils_amd import ImageSpace, VisibilitySpace
+    vs = VisibilitySpace(channels=ch, uvcoords=uv)
+    ims = ImageSpace(shape=(nx, ny), bounds=bounds)
+    h = C.c_void_p()
+    B.check(B.lib().lt_nufft_create(vs, ims, 2, wsup, beta or 0.0, prec, C.byref(h)))
+    return h, (vs, ims)
+
+
+@pytest.mark.parametrize('wsup,beta,prec,tol', [(4, None, 0, 1e-12), (4, None, 1, 1e-5), (3, 14.04, 0, 1e-12),
+                                                (3, 14.04, 1, 1e-5)])
+def test_nufft_vs_oracle_reference_test_sizes(hip, oracle, wsup, beta, prec, tol):
+    B, O = hip, oracle
+    uv = reference_test_uv()
+    h, keep = _plan(B, uv, [0.5], 236, 257, [(-0.5, 0.7), (-0.7, 0.5)], wsup, beta, prec)
+    geo = O.NUFFTGeometry(O.Grid2D(236, 257, [-0.5, 0.7, -0.7, 0.5]), pad=2, wsup=wsup, beta=beta)
+    g = rng(61)
+    v = g.normal(size=(uv.shape[0], 2)).view(np.complex128).reshape(-1)
+    img = np.zeros(236 * 257)
+    B.check(B.lib().lt_nufft_adjoint_host(h, B.ptr(v), B.ptr(img)))
+    ref = O.nufft_adjoint(v.reshape(1, -1), uv, [0.5], geo).reshape(-1)
+    assert O.max_relative_error(img, ref) < tol
+    x = g.normal(size=236 * 257)
+    vis = np.zeros(uv.shape[0], dtype=np.complex128)
+    B.check(B.lib().lt_nufft_forward_host(h, B.ptr(x), B.ptr(vis)))
+    refv = O.nufft_forward(x.reshape(236, 257), uv, [0.5], geo).reshape(-1)
+    assert O.max_relative_error(vis, refv) < tol
+    B.lib().lt_nufft_destroy(h)
+
+
+def test_nufft_operator_vs_dft(hip, oracle):
+    """The reference's own test, through the drop-in operators."""
+    from lentils_amd import DFTOperator, ImageSpace, NUFFTOperator, VisibilitySpace
+    O = oracle
+    uv = reference_test_uv()
+    vs = VisibilitySpace(channels=[0.5], uvcoords=uv)
+    ims = ImageSpace(shape=(236, 257), bounds=[(-0.5, 0.7), (-0.7, 0.5)])
+    v = rng(62).normal(size=(uv.shape[0], 2)).view(np.complex128)
+    nufft = NUFFTOperator(vs, ims)
+    dft = DFTOperator(vs, ims)
+    nufftvec = nufft.T * v
+    dftvec = dft.T * v
+    assert O.max_relative_error(nufftvec, dftvec) < 1e-6
+    assert O.max_relative_error(nufft * dftvec, dft * dftvec) < 1e-6
+
+
+@pytest.mark.parametrize('prec,tol', [(0, 1e-12), (1, 1e-5)])
+def test_grid_degrid_c1_sizes(hip, oracle, prec, tol):
+    B, O = hip, oracle
+    uv = annulus_uv(100000)
+    h, keep = _plan(B, uv, [1.0], 128, 128, [(-1.28, 1.28), (-1.28, 1.28)], 3, 14.04, prec)
+    geo = O.NUFFTGeometry(O.Grid2D(128, 128, [-1.28, 1.28, -1.28, 1.28]), pad=2, wsup=3, beta=14.04)
+    g = rng(63)
+    grid = np.ascontiguousarray(g.normal(size=(256, 129, 2)).view(np.complex128)[..., 0])
+    vis = np.zeros(100000, dtype=np.complex128)
+    B.check(B.lib().lt_nufft_degrid_host(h, B.ptr(grid), B.ptr(vis)))
+    ref = O.grid(uv, [1.0], None, geo, 0, grid_in=grid, nthreads=8).reshape(-1)
+    assert O.max_relative_error(vis, ref) < tol
+    v = g.normal(size=(100000, 2)).view(np.complex128).reshape(-1)
+    out = np.zeros((256, 129), dtype=np.complex128)
+    B.check(B.lib().lt_nufft_grid_host(h, B.ptr(v), B.ptr(out)))
+    refg = O.grid(uv, [1.0], v.reshape(1, -1), geo, 1, nthreads=8)
+    assert O.max_relative_error(out, refg) < tol
+    # determinism: the gridder reduces partial windows in a fixed order
+    out2 = np.zeros((256, 129), dtype=np.complex128)
+    B.check(B.lib().lt_nufft_grid_host(h, B.ptr(v), B.ptr(out2)))
+    if prec == 0:
+        assert np.max(np.abs(out2 - out)) <= 1e-13 * np.max(np.abs(out))
+    B.lib().lt_nufft_destroy(h)
+
+
+def test_grid_cpu_drop_in_symbol(hip, oracle):
+    from lentils_amd import FourierSpace, ImageSpace, VisibilitySpace
+    B, O = hip, oracle
+    uv = annulus_uv(20000, seed=64)
+    vs = VisibilitySpace(channels=[1.0], uvcoords=uv)
+    fs = FourierSpace(ImageSpace(shape=(256, 256), bounds=[(-2.56, 2.56), (-2.56, 2.56)]))
+    geo = O.NUFFTGeometry(O.Grid2D(128, 128, [-1.28, 1.28, -1.28, 1.28]), pad=2, wsup=3, beta=14.04)
+    g = rng(65)
+    grid = np.ascontiguousarray(g.normal(size=(256, 129, 2)).view(np.complex128)[..., 0])
+    vis = g.normal(size=(20000, 2)).view(np.complex128).reshape(-1).copy()
+    vis0 = vis.copy()
+    B.lib().grid_cpu(vs, B.ptr(vis), fs, B.ptr(grid), 3, 14.04, 0)  # vis += degrid(grid)
+    ref = vis0 + O.grid(uv, [1.0], None, geo, 0, grid_in=grid).reshape(-1)
+    assert O.max_relative_error(vis, ref) < 1e-12
+    grid0 = grid.copy()
+    B.lib().grid_cpu(vs, B.ptr(vis0), fs, B.ptr(grid), 3, 14.04, 1)  # grid += grid(vis)
+    refg = grid0 + O.grid(uv, [1.0], vis0.reshape(1, -1), geo, 1)
+    assert O.max_relative_error(grid, refg) < 1e-12
+
+
+def test_nufft_adjointness_gpu(hip, oracle):
+    B, O = hip, oracle
+    uv = annulus_uv(50000, seed=66)
+    h, keep = _plan(B, uv, [1.0], 128, 96, [(-1.28, 1.28), (-0.96, 0.96)], 3, 14.04, 0)
+    g = rng(67)
+    x = g.normal(size=128 * 96)
+    y = g.normal(size=(50000, 2)).view(np.complex128).reshape(-1)
+    Nx = np.zeros(50000, dtype=np.complex128)
+    NTy = np.zeros(128 * 96)
+    B.check(B.lib().lt_nufft_forward_host(h, B.ptr(x), B.ptr(Nx)))
+    B.check(B.lib().lt_nufft_adjoint_host(h, B.ptr(y), B.ptr(NTy)))
+    lhs = np.sum(Nx.real * y.real + Nx.imag * y.imag)
+    rhs = np.dot(x, NTy)
+    assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), abs(rhs))
+    B.lib().lt_nufft_destroy(h)
