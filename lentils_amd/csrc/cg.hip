@@ -123,6 +123,59 @@ __global__ void k_cg_beta(int ns, int *__restrict__ act, double *__restrict__ rz
     if (!(rr[s] > rtol2 * bb[s])) act[s] = 0;  // also stops on NaN
 }
 
+// Jacobi diagonal of L^T B L with the data operator B approximated by its
+// (translation-invariant) response beta to a unit impulse at the image
+// centre: diag_j = sum_{a,b in column j of L} L_a L_b beta(p_a - p_b).
+// One wavefront per source pixel (row of L^T).
+__global__ void __launch_bounds__(256) k_jacobi_diag(long nrow, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci, const double *__restrict__ va,
+                                                      const int32_t *__restrict__ seg, long npix, int nx, int ny,
+                                                      const double *__restrict__ beta, double *__restrict__ diag) {
+    const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    if (row >= nrow) return;
+    const long base = (long)seg[row] * npix;
+    const int b0 = rp[row], b1 = rp[row + 1];
+    const int ic = nx / 2, jc = ny / 2;
+    double acc = 0.0;
+    for (int a = b0 + lane; a < b1; a += 64) {
+        const long pa = ci[a] - base;
+        const int ia = (int)(pa / ny), ja = (int)(pa % ny);
+        const double va_ = va[a];
+        for (int b = b0; b < b1; ++b) {
+            const long pb = ci[b] - base;
+            const int di = (int)(pb / ny) - ia + ic, dj = (int)(pb % ny) - ja + jc;
+            if (di >= 0 && di < nx && dj >= 0 && dj < ny) acc += va_ * va[b] * beta[(long)di * ny + dj];
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) diag[row] = acc;
+}
+
+// sum of squares of each CSR row (diag of C^T C for a symmetric C)
+__global__ void k_row_sumsq(long nrow, const int32_t *__restrict__ rp, const double *__restrict__ va,
+                            double *__restrict__ out) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    double s = 0.0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) s += va[k] * va[k];
+    out[r] = s;
+}
+
+__global__ void k_jacobi_inv(long n, const double *__restrict__ dd, const double *__restrict__ dc, double lc, double lt,
+                             double *__restrict__ dinv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = dd[i] + lc * dc[i] + lt;
+    dinv[i] = (d > 0.0 && isfinite(d)) ? 1.0 / d : 1.0;
+}
+
+__global__ void k_delta(long n, long at, double *__restrict__ v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (i == at) ? 1.0 : 0.0;
+}
+
 }  // namespace
 
 struct Solver {
@@ -150,6 +203,7 @@ struct Solver {
                      int op_mode);
     lt_status apply(const double *xin, double *yout, const int *active_list, int nactive);
     lt_status solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out);
+    lt_status jacobi_setup();
 };
 
 fourier_space padded_fourier_space(const image_space &im, int pad);
@@ -213,6 +267,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
             ro += parts[s].nrows;
             no += parts[s].nnz;
         }
+        LT_TRY(csr_prepare(C));
     }
     // weights and data into plan order; dirty image D^T C d
     const long nv = nf->nvis;
@@ -285,28 +340,37 @@ lt_status Solver::apply(const double *xin, double *yout, const int *alist, int n
     return LT_OK;
 }
 
+lt_status Solver::jacobi_setup() {
+    // impulse response of the data operator at the image centre (sample 0)
+    const long c = (long)(lb->im.nx / 2) * lb->im.ny + lb->im.ny / 2;
+    hipLaunchKernelGGL(k_delta, dim3(blocks_for(npix * S, 256)), dim3(256), 0, stream(), npix * S, c, img1.p);
+    LT_CHECK_LAUNCH();
+    if (mode != LT_OP_BCB)
+        LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, all_active.p, 1, 1.0, false, use_stencil ? &stencil : nullptr));
+    else
+        LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, all_active.p, 1, 1.0, false));
+    DevBuf<double> dd, dc;
+    LT_TRY(dd.alloc(ntot));
+    LT_TRY(dc.alloc(ntot));
+    LT_TRY(dinv.alloc(ntot));
+    const DevCSR &T = lb->LT;
+    hipLaunchKernelGGL(k_jacobi_diag, dim3(blocks_for(T.nrows * 64, 256)), dim3(256), 0, stream(), T.nrows, T.rp.p,
+                       T.ci.p, T.va.p, seg.p, npix, lb->im.nx, lb->im.ny, img2.p, dd.p);
+    LT_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_row_sumsq, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, C.rp.p, C.va.p, dc.p);
+    LT_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_jacobi_inv, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, dd.p, dc.p, lam_c,
+                       lam_t, dinv.p);
+    LT_CHECK_LAUNCH();
+    return sync();
+}
+
 lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out) {
     const unsigned gb = blocks_for(ntot, 256), gs = blocks_for(S, 64);
     // Jacobi: diag(A) ~ beam peak * diag(L^T L) + lambda_c diag(Cc^T Cc) + lambda_t
     const double *dinv_p = nullptr;
     if (precond == LT_PC_JACOBI) {
-        if (!dinv.p) {
-            LT_TRY(dinv.alloc(ntot));
-            // diag(A) estimated column by column is too costly; use the exact
-            // diagonal of the regularizer plus the data term on a probe basis
-            // of one source pixel per sample is not separable either, so the
-            // data term is approximated by A applied to the all-ones vector
-            // scaled per pixel (row sums) -- a standard Jacobi surrogate.
-            std::vector<double> ones(ntot, 1.0);
-            DevBuf<double> o;
-            LT_TRY(o.upload(ones.data(), ntot));
-            LT_TRY(apply(o.p, dinv.p, all_active.p, S));
-            std::vector<double> h(ntot);
-            LT_TRY(dinv.download(h.data(), ntot));
-            LT_TRY(sync());
-            for (auto &v : h) v = (v > 0.0 && std::isfinite(v)) ? 1.0 / v : 1.0;
-            LT_TRY(dinv.upload(h.data(), ntot));
-        }
+        if (!dinv.p) LT_TRY(jacobi_setup());
         dinv_p = dinv.p;
     }
     std::vector<int> ones(S, 1), zeros(S, 0);

@@ -105,6 +105,7 @@ lt_status lt_csr_spmv_host(long nrows, long ncols, const int *rp, const int *ci,
     LT_TRY(A.rp.upload(rp, nrows + 1));
     LT_TRY(A.ci.upload(ci, A.nnz));
     LT_TRY(A.va.upload(va, A.nnz));
+    LT_TRY(csr_prepare(A));
     DevBuf<double> dx, dy;
     LT_TRY(dx.upload(x, ncols));
     LT_TRY(dy.alloc(nrows > 0 ? nrows : 1));

@@ -123,9 +123,18 @@ struct DevCSR {
     DevBuf<int32_t> rp;  // nrows + 1 (global offsets)
     DevBuf<int32_t> ci;  // global column indices
     DevBuf<double> va;
+    // long-row plan (csr_prepare): rows above kLongRow nonzeros are summed
+    // in fixed-size chunks by whole workgroups, then combined in order
+    bool prepared = false;
+    int lanes = 1;
+    long nlong = 0, nchunk = 0;
+    DevBuf<int32_t> long_rows, long_chunk0, chunk_beg, long_rows_of_chunk;
+    DevBuf<double> chunk_sum;
 };
 
-// y = A x (FP64); optional accumulate y += alpha A x
+// analyse row lengths once (lane group size, long-row chunk plan)
+lt_status csr_prepare(DevCSR &A);
+// y = alpha A x + beta y (FP64); lanes_per_row <= 0 uses the prepared choice
 lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes_per_row);
 int csr_pick_lanes(const DevCSR &A);
 // B = A^T with column order inside each row ascending (deterministic)

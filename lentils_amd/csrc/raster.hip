@@ -199,6 +199,25 @@ __global__ void k_long_copy(const int64_t *__restrict__ rows, const int64_t *__r
     }
 }
 
+__global__ void k_long_flags(long n, const int32_t *__restrict__ nfrag, int64_t *__restrict__ flag) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) flag[r] = nfrag[r] > kShort ? 1 : 0;
+}
+
+__global__ void k_long_compact(long n, const int64_t *__restrict__ pos, const int32_t *__restrict__ nfrag,
+                               int64_t *__restrict__ rows, int32_t *__restrict__ cnt) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n && pos[r + 1] != pos[r]) {
+        rows[pos[r]] = r;
+        cnt[pos[r]] = nfrag[r];
+    }
+}
+
+__global__ void k_sample_bounds(int ns, long npix, const int64_t *__restrict__ rowoff, int64_t *__restrict__ out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= ns) out[s] = rowoff[(long)s * npix];
+}
+
 __global__ void k_rowptr(const int64_t *__restrict__ rowoff, long n, int32_t *__restrict__ rp) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r <= n) rp[r] = (int32_t)rowoff[r];
@@ -273,15 +292,32 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
                        src.p, lattice.p, mask.p, nnz_row.p, nfrag.p, minidx.p, lastne.p, dstatus.p);
     LT_CHECK_LAUNCH();
     // long rows
-    std::vector<int32_t> nf(R);
-    LT_TRY(nfrag.download(nf.data(), R));
+    // long rows: compacted on the device; only their (few) ids and fragment
+    // counts come back to size the staging area
+    DevBuf<int64_t> lflag, lpos;
+    LT_TRY(lflag.alloc(R + 1));
+    LT_TRY(lpos.alloc(R + 1));
+    hipLaunchKernelGGL(k_long_flags, dim3(blocks_for(R, 256)), dim3(256), 0, stream(), R, nfrag.p, lflag.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(lflag.p, lpos.p, R));
+    int64_t nl64 = 0;
+    LT_TRY(lpos.download(&nl64, 1, R));
     LT_TRY(sync());
-    std::vector<int64_t> lrows, loff(1, 0);
-    for (long r = 0; r < R; ++r)
-        if (nf[r] > kShort) {
-            lrows.push_back(r);
-            loff.push_back(loff.back() + nf[r]);
-        }
+    std::vector<int64_t> lrows(nl64), loff(nl64 + 1, 0);
+    if (nl64 > 0) {
+        DevBuf<int64_t> lr;
+        DevBuf<int32_t> ln;
+        LT_TRY(lr.alloc(nl64));
+        LT_TRY(ln.alloc(nl64));
+        hipLaunchKernelGGL(k_long_compact, dim3(blocks_for(R, 256)), dim3(256), 0, stream(), R, lpos.p, nfrag.p, lr.p,
+                           ln.p);
+        LT_CHECK_LAUNCH();
+        std::vector<int32_t> hn(nl64);
+        LT_TRY(lr.download(lrows.data(), nl64));
+        LT_TRY(ln.download(hn.data(), nl64));
+        LT_TRY(sync());
+        for (long k = 0; k < nl64; ++k) loff[k + 1] = loff[k] + hn[k];
+    }
     const int nlong = (int)lrows.size();
     DevBuf<int64_t> drows, dfoff;
     DevBuf<double> farea, sval;
@@ -330,14 +366,19 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
         LT_CHECK_LAUNCH();
     }
     LT_TRY(dstatus.download(status_host.data(), ns));
-    // per-sample nnz offsets
+    // per-sample nnz offsets (row pointer at every sample boundary)
     nnzoff_host.assign(ns + 1, 0);
-    std::vector<int64_t> ro(R + 1);
-    LT_TRY(rowoff.download(ro.data(), R + 1));
-    LT_TRY(sync());
-    for (int s = 0; s <= ns; ++s) nnzoff_host[s] = ro[(long)s * npix];
+    {
+        DevBuf<int64_t> bnd;
+        LT_TRY(bnd.alloc(ns + 1));
+        hipLaunchKernelGGL(k_sample_bounds, dim3(blocks_for(ns + 1, 64)), dim3(64), 0, stream(), ns, npix, rowoff.p,
+                           bnd.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(bnd.download(nnzoff_host.data(), ns + 1));
+        LT_TRY(sync());
+    }
     nlong_rows = nlong;
-    return LT_OK;
+    return csr_prepare(L);
 }
 
 lt_status LensBatch::download(int s, int *row_inds, int *cols, double *vals) const {

@@ -27,7 +27,10 @@ struct LensBatch {
                     const int *lens_counts, const double *lattice_host);
     lt_status download(int s, int *row_inds, int *cols, double *vals) const;
     lt_status ensure_transpose() {
-        if (LT.nrows == 0 && L.nrows > 0) return csr_transpose(L, LT);
+        if (LT.nrows == 0 && L.nrows > 0) {
+            LT_TRY(csr_transpose(L, LT));
+            LT_TRY(csr_prepare(LT));
+        }
         return LT_OK;
     }
 };

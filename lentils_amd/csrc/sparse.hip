@@ -12,21 +12,82 @@ namespace lt {
 
 namespace {
 
+constexpr int kLongRow = 2048;  // rows longer than this go to the chunked path
+constexpr int kChunk = 2048;
+
 template <int LANES>
 __global__ void __launch_bounds__(256) k_spmv(long nrows, const int32_t *__restrict__ rp,
                                                const int32_t *__restrict__ ci, const double *__restrict__ va,
                                                const double *__restrict__ x, double *__restrict__ y, double alpha,
-                                               double beta) {
+                                               double beta, int skip_long) {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long row = gid / LANES;
     const int lane = (int)(gid % LANES);
     if (row >= nrows) return;  // whole lane groups leave together
     double s = 0.0;
-    const int e = rp[row + 1];
-    for (int k = rp[row] + lane; k < e; k += LANES) s += va[k] * x[ci[k]];
+    const int b = rp[row], e = rp[row + 1];
+    if (skip_long && e - b > kLongRow) return;  // summed by k_spmv_chunks / k_spmv_long
+    for (int k = b + lane; k < e; k += LANES) s += va[k] * x[ci[k]];
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
     if (lane == 0) y[row] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[row];
+}
+
+// partial sums of fixed-size chunks of the long rows, one workgroup each
+__global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t *__restrict__ chunk_beg,
+                                                      const int32_t *__restrict__ chunk_end_row,
+                                                      const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci, const double *__restrict__ va,
+                                                      const double *__restrict__ x, double *__restrict__ out) {
+    __shared__ double sh[4];
+    const long c = blockIdx.x;
+    const int b = chunk_beg[c];
+    const int e = min(b + kChunk, rp[chunk_end_row[c] + 1]);
+    double s = 0.0;
+    for (int k = b + threadIdx.x; k < e; k += 256) s += va[k] * x[ci[k]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// long rows: sum their chunks in order
+__global__ void k_spmv_long(long nlong, const int32_t *__restrict__ rows, const int32_t *__restrict__ chunk0,
+                            const double *__restrict__ part, double *__restrict__ y, double alpha, double beta) {
+    long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlong) return;
+    double s = 0.0;
+    for (int c = chunk0[l]; c < chunk0[l + 1]; ++c) s += part[c];
+    const int row = rows[l];
+    y[row] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[row];
+}
+
+__global__ void k_long_rows_flag(long n, const int32_t *__restrict__ rp, int64_t *__restrict__ flag,
+                                 int64_t *__restrict__ nch) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int len = rp[r + 1] - rp[r];
+    const bool lg = len > kLongRow;
+    flag[r] = lg ? 1 : 0;
+    nch[r] = lg ? (len + kChunk - 1) / kChunk : 0;
+}
+
+__global__ void k_long_rows_fill(long n, const int32_t *__restrict__ rp, const int64_t *__restrict__ pos,
+                                 const int64_t *__restrict__ choff, int32_t *__restrict__ rows,
+                                 int32_t *__restrict__ chunk0, int32_t *__restrict__ chunk_beg,
+                                 int32_t *__restrict__ chunk_row) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || pos[r + 1] == pos[r]) return;
+    const long l = pos[r];
+    rows[l] = (int32_t)r;
+    chunk0[l] = (int32_t)choff[r];
+    const int b = rp[r], e = rp[r + 1];
+    long c = choff[r];
+    for (int k = b; k < e; k += kChunk, ++c) {
+        chunk_beg[c] = k;
+        chunk_row[c] = (int32_t)r;
+    }
 }
 
 __global__ void k_col_count(long nnz, const int32_t *__restrict__ ci, int64_t *__restrict__ cnt) {
@@ -72,14 +133,57 @@ int csr_pick_lanes(const DevCSR &A) {
     return L;
 }
 
+lt_status csr_prepare(DevCSR &A) {
+    int32_t last = 0;  // host source of an async copy: lives until the final sync
+    A.lanes = csr_pick_lanes(A);
+    A.nlong = A.nchunk = 0;
+    if (A.nrows > 0) {
+        const long n = A.nrows;
+        DevBuf<int64_t> flag, pos, nch, choff;
+        LT_TRY(flag.alloc(n + 1));
+        LT_TRY(pos.alloc(n + 1));
+        LT_TRY(nch.alloc(n + 1));
+        LT_TRY(choff.alloc(n + 1));
+        hipLaunchKernelGGL(k_long_rows_flag, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, flag.p,
+                           nch.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(exclusive_scan_i64(flag.p, pos.p, n));
+        LT_TRY(exclusive_scan_i64(nch.p, choff.p, n));
+        int64_t nl = 0, nc = 0;
+        LT_TRY(pos.download(&nl, 1, n));
+        LT_TRY(choff.download(&nc, 1, n));
+        LT_TRY(sync());
+        A.nlong = nl;
+        A.nchunk = nc;
+        if (nl > 0) {
+            DevBuf<int32_t> chunk_row;
+            LT_TRY(A.long_rows.alloc(nl));
+            LT_TRY(A.long_chunk0.alloc(nl + 1));
+            LT_TRY(A.chunk_beg.alloc(nc));
+            LT_TRY(chunk_row.alloc(nc));
+            LT_TRY(A.chunk_sum.alloc(nc));
+            hipLaunchKernelGGL(k_long_rows_fill, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, pos.p,
+                               choff.p, A.long_rows.p, A.long_chunk0.p, A.chunk_beg.p, chunk_row.p);
+            LT_CHECK_LAUNCH();
+            last = (int32_t)nc;
+            LT_HIP(hipMemcpyAsync(A.long_chunk0.p + nl, &last, sizeof(int32_t), hipMemcpyHostToDevice, stream()));
+            // chunk -> row mapping is needed by the chunk kernel for the row end
+            A.long_rows_of_chunk = std::move(chunk_row);
+        }
+    }
+    A.prepared = true;
+    return sync();
+}
+
 lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes) {
     if (A.nrows == 0) return LT_OK;
-    if (lanes <= 0) lanes = csr_pick_lanes(A);
+    if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
+    const int skip = (A.prepared && A.nlong > 0) ? 1 : 0;
     const long threads = A.nrows * lanes;
     const dim3 g(blocks_for(threads, 256)), b(256);
-#define LT_SPMV_CASE(L)                                                                                        \
-    case L:                                                                                                    \
-        hipLaunchKernelGGL(k_spmv<L>, g, b, 0, stream(), A.nrows, A.rp.p, A.ci.p, A.va.p, x, y, alpha, beta_y); \
+#define LT_SPMV_CASE(L)                                                                                              \
+    case L:                                                                                                          \
+        hipLaunchKernelGGL(k_spmv<L>, g, b, 0, stream(), A.nrows, A.rp.p, A.ci.p, A.va.p, x, y, alpha, beta_y, skip); \
         break;
     switch (lanes) {
         LT_SPMV_CASE(1)
@@ -95,6 +199,14 @@ lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, do
     }
 #undef LT_SPMV_CASE
     LT_CHECK_LAUNCH();
+    if (skip) {
+        hipLaunchKernelGGL(k_spmv_chunks, dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk, A.chunk_beg.p,
+                           A.long_rows_of_chunk.p, A.rp.p, A.ci.p, A.va.p, x, A.chunk_sum.p);
+        LT_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_spmv_long, dim3(blocks_for(A.nlong, 256)), dim3(256), 0, stream(), A.nlong,
+                           A.long_rows.p, A.long_chunk0.p, A.chunk_sum.p, y, alpha, beta_y);
+        LT_CHECK_LAUNCH();
+    }
     return LT_OK;
 }
 
