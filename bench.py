@@ -68,11 +68,18 @@ def blob_source(src, seed=14):
 
 
 class Workload:
-    def __init__(self, nvis=NVIS, nlens=NLENS):
+    """Inputs of one step, resident before timing.  Default: every lens
+    sample has its own observation (its own Gaussian-blob source lensed by
+    that sample's lens, complex64 visibilities + noise on the shared uv
+    set), entering the solver through its dirty image.  shared_data: one
+    observation (lensed by the first sample) for all 64 samples."""
+
+    def __init__(self, nvis=NVIS, nlens=NLENS, shared_data=False):
         from lentils_amd import ImageSpace, VisibilitySpace
         from lentils_amd import backend as B
         from lentils_amd.solvers import LensBatch, NUFFTPlan
         self.B = B
+        self.shared_data = shared_data
         self.im = ImageSpace(shape=(512, 512), bounds=[(-2.56, 2.56), (-2.56, 2.56)])
         self.src = ImageSpace(shape=(128, 128), bounds=[(-0.64, 0.64), (-0.64, 0.64)])
         self.uvw = gen_uv(nvis)
@@ -82,27 +89,39 @@ class Workload:
         t = time.perf_counter()
         self.plan = NUFFTPlan(self.vs, self.im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f64')
         log(f'NUFFT plan (uv binning) {time.perf_counter() - t:.2f}s')
-        # synthetic data: blobs lensed by the first sample, NUFFT, noise 1e-3
-        truth = LensBatch(self.im, [self.src], self.lenses[:1], z_src=2.0)
-        rp, ci, va = truth.csr(0)
-        x = np.ascontiguousarray(blob_source(self.src).reshape(-1))
-        img = np.zeros(self.im.size)
-        B.check(B.lib().lt_csr_spmv_host(self.im.size, self.src.size, B.ptr(rp), B.ptr(ci), B.ptr(va), B.ptr(x),
-                                         B.ptr(img)))
-        vis = np.zeros(nvis, dtype=np.complex128)
-        B.check(B.lib().lt_nufft_forward_host(self.plan._handle, B.ptr(img), B.ptr(vis)))
-        g = np.random.Generator(np.random.PCG64(13))
-        noise = (g.normal(0, 1e-3, size=(nvis, 2))).view(np.complex128).reshape(-1)
-        self.data = vis.astype(np.complex64).astype(np.complex128) + noise  # complex64 visibilities
         self.weights = np.full(nvis, 1e6)
         self.lambda_curv = 1e2
+        g = np.random.Generator(np.random.PCG64(13))
+        nobs = 1 if shared_data else nlens
+        truth = LensBatch(self.im, [self.src] * nobs, self.lenses[:nobs], z_src=2.0)
+        self.data = None
+        self.dirty = []
+        for s in range(nobs):
+            rp, ci, va = truth.csr(s)
+            x = np.ascontiguousarray(blob_source(self.src, seed=14 + s).reshape(-1))
+            img = np.zeros(self.im.size)
+            B.check(B.lib().lt_csr_spmv_host(self.im.size, self.src.size, B.ptr(rp), B.ptr(ci), B.ptr(va),
+                                             B.ptr(x), B.ptr(img)))
+            vis = self.plan.forward(img).astype(np.complex64)  # complex64 visibilities
+            vis.real += g.standard_normal(nvis, dtype=np.float32) * np.float32(1e-3)
+            vis.imag += g.standard_normal(nvis, dtype=np.float32) * np.float32(1e-3)
+            d = vis.astype(np.complex128)
+            if shared_data:
+                self.data = d
+            else:
+                self.dirty.append(self.plan.adjoint(self.weights * d))
+        del truth
 
-    def step(self, rtol=1e-6, maxiter=20000, operator='nufft'):
+    def step(self, rtol=1e-6, maxiter=20000, operator='nufft', precond='none'):
         from lentils_amd.solvers import BatchedSourceInversion, LensBatch
         lb = LensBatch(self.im, [self.src] * len(self.lenses), self.lenses, z_src=2.0)
-        inv = BatchedSourceInversion(lb, self.plan, self.data, weights=self.weights,
-                                     lambda_curvature=self.lambda_curv, operator=operator)
-        x, its, res = inv.solve(rtol=rtol, maxiter=maxiter)
+        if self.shared_data:
+            inv = BatchedSourceInversion(lb, self.plan, self.data, weights=self.weights,
+                                         lambda_curvature=self.lambda_curv, operator=operator)
+        else:
+            inv = BatchedSourceInversion(lb, self.plan, weights=self.weights, dirty_images=self.dirty,
+                                         lambda_curvature=self.lambda_curv, operator=operator)
+        x, its, res = inv.solve(rtol=rtol, maxiter=maxiter, precond=precond)
         return its, res
 
 
@@ -199,6 +218,9 @@ def main():
     ap.add_argument('--operator', default='nufft', choices=['nufft', 'bcb'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--maxiter', type=int, default=20000)
+    ap.add_argument('--precond', default='none', choices=['none', 'jacobi'])
+    ap.add_argument('--shared-data', action='store_true',
+                    help='one observation for all samples (most samples then fit the data with the wrong lens)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (every solve of a batch is independent; '
@@ -207,18 +229,18 @@ def main():
     from lentils_amd import backend as B
     B.check(B.lib().lt_init(0))
     t = time.perf_counter()
-    wl = Workload()
+    wl = Workload(shared_data=args.shared_data)
     log(f'workload setup {time.perf_counter() - t:.1f}s')
     for k in range(args.warmup):
         t = time.perf_counter()
-        its, res = wl.step(operator=args.operator, maxiter=args.maxiter)
+        its, res = wl.step(operator=args.operator, maxiter=args.maxiter, precond=args.precond)
         log(f'warmup {k}: {time.perf_counter() - t:.2f}s iters median {np.median(its):.0f} max res {res.max():.2e}')
     times = []
     all_its, all_res = [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
         t = time.perf_counter()
-        its, res = wl.step(operator=args.operator, maxiter=args.maxiter)
+        its, res = wl.step(operator=args.operator, maxiter=args.maxiter, precond=args.precond)
         times.append(time.perf_counter() - t)
         all_its.append(its)
         all_res.append(res)
@@ -234,10 +256,15 @@ def main():
         'metric': 'source solves/s at 1e-6 rel resid (learned-PCG vs plain CG); Gvis/s degrid',
         'value': value, 'unit': 'source solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f64', 'data': 'synthetic (seeded lenses, uv annulus, Gaussian-blob source, complex64 vis + noise)',
+        'dtype': 'f64',
+        'data': ('synthetic: seeded lenses and uv annulus; ' +
+                 ('one observation (blob source lensed by sample 0) shared by all samples'
+                  if args.shared_data else 'one observation per lens sample (its own blob source lensed by its '
+                  'own lens), complex64 vis + noise 1e-3')),
         'config': {'workload': 'plain GPU method: 64 lens samples, 512^2 image @0.01", 128^2 source, 1e7 vis, '
                                '1024x513 grid, W=6 (support 3), beta=14.04, lambda_curv=1e2, FP64 CG to 1e-6',
-                   'operator': args.operator, 'precond': 'none (plain CG)', 'lens_samples_per_step': NLENS,
+                   'operator': args.operator,
+                   'precond': 'none (plain CG)' if args.precond == 'none' else args.precond, 'lens_samples_per_step': NLENS,
                    'nvis': NVIS, 'parallelism': 'single GPU'},
         'solver': {'iters_median': float(np.median(its)), 'iters_max': int(its.max()),
                    'true_rel_resid_max': float(res.max()), 'all_converged': bool(np.all(res <= 1e-6))},

@@ -208,6 +208,14 @@ lt_status lt_nufft_info(lt_nufft h, int *nu, int *nv, int *half_nv, double *kb_b
 lt_status lt_nufft_time(lt_nufft h, int direction, int reps, double *ms_per_launch);
 lt_status lt_nufft_destroy(lt_nufft h);
 
+/* No reference counterpart: the per-tap kernel polynomials the gridder
+ * evaluates instead of kb_g (nufft.c:11-21).  Tap k of 2W+1 at fractional
+ * offset f in [-1/2, 1/2] (distance f + W - k) has value
+ * sum_m coeffs[k*(D+1) + m] f^m, D = 16 (LT_F64) or 9 (LT_F32), times
+ * 1/(4W^2) per axis pair; edge taps are zero where |f + W - k| > W.
+ * max_err: fit error relative to I0(kb_beta).  Host only. */
+lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, double *max_err);
+
 /* Source-inversion solver for every sample of a lens batch:
  *   (L^T D^T C D L + lambda_curv Cc^T Cc + lambda_tikh I) x = L^T D^T C d
  * with D the NUFFT, C = diag(weights) (sigma^-2 on unmasked visibilities,
@@ -224,6 +232,12 @@ lt_status lt_nufft_destroy(lt_nufft h);
 #define LT_PC_JACOBI 1
 lt_status lt_solver_create(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *data, double lambda_curv,
                            double lambda_tikh, int op_mode, lt_solver *out);
+/* As lt_solver_create, with the data given through their dirty images
+ * D^T C d (nx*ny float64 each, e.g. lt_nufft_adjoint_host of weights * data):
+ * ndirty = 1 shares one data set between all samples, ndirty = nsamples gives
+ * every sample its own (one observation per lens model, same uv coverage). */
+lt_status lt_solver_create_dirty(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *dirty,
+                                 int ndirty, double lambda_curv, double lambda_tikh, int op_mode, lt_solver *out);
 /* CG to ||b - A x|| / ||b|| <= rtol (FP64 true residual).  x: host array
  * of all samples' sources concatenated (may be NULL); iters/resid: per
  * sample (may be NULL). */

@@ -47,6 +47,7 @@ generic_mass_model_ctype = np.dtype([('type', 'u4'), ('_pad', 'u4'), ('z_l', 'f8
                                      ('fpars', 'f8', 24), ('ipars', 'i4', 8), ('flags', 'b', 32)])
 
 LT_OK = 0
+LT_ERR_ARG, LT_ERR_HIP, LT_ERR_NYQUIST, LT_ERR_UNSUPPORTED, LT_ERR_CAPACITY, LT_ERR_FFT, LT_ERR_NOT_CONVERGED = 1, 2, 3, 4, 5, 6, 7
 LT_F64, LT_F32 = 0, 1
 LT_OP_NUFFT, LT_OP_BCB, LT_OP_NUFFT_EXPLICIT = 0, 1, 2
 LT_PC_NONE, LT_PC_JACOBI = 0, 1
@@ -100,8 +101,10 @@ def lib():
         'lt_nufft_grid_host': ([vp, vp, vp], i),
         'lt_nufft_info': ([vp, ip, ip, ip, dp, lp], i),
         'lt_nufft_time': ([vp, i, i, dp], i),
+        'lt_kb_taps': ([d, i, i, vp, dp], i),
         'lt_nufft_destroy': ([vp], i),
         'lt_solver_create': ([vp, vp, vp, vp, d, d, i, C.POINTER(vp)], i),
+        'lt_solver_create_dirty': ([vp, vp, vp, vp, i, d, d, i, C.POINTER(vp)], i),
         'lt_solver_solve': ([vp, d, i, i, vp, vp, vp], i),
         'lt_solver_rhs': ([vp, vp], i),
         'lt_solver_apply': ([vp, vp, vp], i),
@@ -125,7 +128,7 @@ def exported_symbols():
             'lt_lens_matrix_nnz', 'lt_lens_matrix_csr', 'lt_deflect_lattice', 'lt_lens_matrix_from_lattice',
             'lt_lensbatch_create', 'lt_lensbatch_info', 'lt_lensbatch_download', 'lt_lensbatch_destroy',
             'lt_nufft_create', 'lt_nufft_forward_host', 'lt_nufft_adjoint_host', 'lt_nufft_degrid_host',
-            'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_solver_create',
+            'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_kb_taps', 'lt_solver_create', 'lt_solver_create_dirty',
             'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
             'lt_fft_c2r_host', 'lt_csr_spmv_host']
 

@@ -190,7 +190,8 @@ struct Solver {
     DevBuf<int32_t> seg;
     DevCSR C;                 // block-diagonal curvature of every source grid
     DevBuf<double> w;         // visibility weights in plan order
-    DevBuf<double> dirty;     // D^T C d (nx*ny)
+    DevBuf<double> dirty;     // D^T C d_s (ndirty x nx*ny)
+    int ndirty = 1;           // 1: one data set shared by every sample; S: one per sample
     DevBuf<double> beamspec;  // BCB multiplier, nu x half_nv complex
     DevBuf<double> b, x, r, z, p, Ap, img1, img2, tmp, dinv;
     DevBuf<double> rz, rz_new, pAp, rr, bb, alpha, beta;
@@ -199,8 +200,8 @@ struct Solver {
     NormalStencil stencil;
     bool use_stencil = false;
 
-    lt_status create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data, double lc, double lt,
-                     int op_mode);
+    lt_status create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data,
+                     const double *dirty_in, int ndirty_in, double lc, double lt, int op_mode);
     lt_status apply(const double *xin, double *yout, const int *active_list, int nactive);
     lt_status solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out);
     lt_status jacobi_setup();
@@ -214,8 +215,8 @@ lt_status beam_spectrum(Solver &S, const double *weights_caller_order);
 
 }  // namespace
 
-lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data, double lc,
-                         double lt_, int op_mode) {
+lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data,
+                         const double *dirty_in, int ndirty_in, double lc, double lt_, int op_mode) {
     lb = lb_;
     nf = nf_;
     S = lb->nsamples;
@@ -227,6 +228,11 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
         set_error("solver: NUFFT image space does not match the lens image space");
         return LT_ERR_ARG;
     }
+    if (dirty_in && ndirty_in != 1 && ndirty_in != S) {
+        set_error("solver: %d dirty images for %d samples (1 or one per sample)", ndirty_in, S);
+        return LT_ERR_ARG;
+    }
+    ndirty = dirty_in ? ndirty_in : 1;
     if (op_mode != LT_OP_NUFFT && op_mode != LT_OP_BCB && op_mode != LT_OP_NUFFT_EXPLICIT) {
         set_error("solver: unknown operator mode %d", op_mode);
         return LT_ERR_ARG;
@@ -282,8 +288,10 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
             for (long q = 0; q < nv; ++q) hs[q] = weights[hp[q]];
             LT_TRY(w.upload(hs.data(), nv));
         }
-        LT_TRY(dirty.alloc(npix));
-        if (nv) {
+        LT_TRY(dirty.alloc(npix * ndirty));
+        if (dirty_in) {
+            LT_TRY(dirty.upload(dirty_in, npix * ndirty));
+        } else if (nv) {
             LT_TRY(dc.upload(data, 2 * nv));
             LT_TRY(nf->to_sorted(dc.p, nf->visbuf.p));
             LT_TRY(nf->adjoint(nf->visbuf.p, w.p, dirty.p, 1.0, false));
@@ -314,10 +322,10 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     std::vector<int> ids(S);
     for (int s = 0; s < S; ++s) ids[s] = s;
     LT_TRY(all_active.upload(ids.data(), S));
-    // rhs b_s = L_s^T dirty: replicate the dirty image per sample
+    // rhs b_s = L_s^T dirty_s (a shared dirty image is replicated)
     for (int s = 0; s < S; ++s)
-        LT_HIP(hipMemcpyAsync(img2.p + (long)s * npix, dirty.p, npix * sizeof(double), hipMemcpyDeviceToDevice,
-                              stream()));
+        LT_HIP(hipMemcpyAsync(img2.p + (long)s * npix, dirty.p + (ndirty == 1 ? 0L : (long)s * npix),
+                              npix * sizeof(double), hipMemcpyDeviceToDevice, stream()));
     LT_TRY(csr_spmv(lb->LT, img2.p, b.p, 1.0, 0.0, 0));
     return sync();
 }
@@ -560,7 +568,21 @@ lt_status lt_solver_create(lt_lensbatch lbh, lt_nufft nfh, const double *weights
         return LT_ERR_ARG;
     }
     auto h = std::make_unique<lt_solver_s>();
-    LT_TRY(h->s.create(lb, nf, weights, data, lambda_curv, lambda_tikh, op_mode));
+    LT_TRY(h->s.create(lb, nf, weights, data, nullptr, 1, lambda_curv, lambda_tikh, op_mode));
+    *out = h.release();
+    return LT_OK;
+}
+
+lt_status lt_solver_create_dirty(lt_lensbatch lbh, lt_nufft nfh, const double *weights, const double *dirty,
+                                 int ndirty, double lambda_curv, double lambda_tikh, int op_mode, lt_solver *out) {
+    LensBatch *lb = lensbatch_of(lbh);
+    NufftPlan *nf = nufft_of(nfh);
+    if (!lb || !nf || !out || !dirty || (nf->nvis > 0 && !weights)) {
+        set_error("lt_solver_create_dirty: bad arguments");
+        return LT_ERR_ARG;
+    }
+    auto h = std::make_unique<lt_solver_s>();
+    LT_TRY(h->s.create(lb, nf, weights, nullptr, dirty, ndirty, lambda_curv, lambda_tikh, op_mode));
     *out = h.release();
     return LT_OK;
 }

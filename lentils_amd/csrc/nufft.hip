@@ -116,6 +116,23 @@ __device__ inline T kb_value(const KbPoly &P, T arg, T W) {
     return acc;
 }
 
+// the 2W+1 kernel values of one axis at fractional offset f (|f| <= 1/2)
+// from the per-tap polynomials kc[k][m]
+template <class T, int W>
+__device__ inline void kb_taps(const T *__restrict__ kc, T f, T (&out)[2 * W + 1]) {
+    constexpr int NT = 2 * W + 1;
+    constexpr int D = TapDeg<T>::D;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        T acc = kc[k * (D + 1) + D];
+#pragma unroll
+        for (int m = D - 1; m >= 0; --m) acc = acc * f + kc[k * (D + 1) + m];
+        out[k] = acc;
+    }
+    if (f > (T)0) out[0] = (T)0;
+    if (f < (T)0) out[NT - 1] = (T)0;
+}
+
 // value of raw tap (a, b) of the half-plane grid, with the reference's
 // wrap/fold and the checkerboard sign applied; zero outside the band
 template <class T>
@@ -138,7 +155,7 @@ __device__ inline Cplx<T> fetch_folded(const Geo &G, const Cplx<T> *__restrict__
 }
 
 template <class T, int W>
-__global__ void __launch_bounds__(kNThreads) k_degrid(Geo G, KbPoly P, const Cplx<T> *__restrict__ grid,
+__global__ void __launch_bounds__(kNThreads) k_degrid(Geo G, const T *__restrict__ kc, const Cplx<T> *__restrict__ grid,
                                                       const int32_t *__restrict__ ck_tile,
                                                       const int32_t *__restrict__ ck_beg,
                                                       const int32_t *__restrict__ ck_end,
@@ -159,18 +176,14 @@ __global__ void __launch_bounds__(kNThreads) k_degrid(Geo G, KbPoly P, const Cpl
     }
     __syncthreads();
     const int beg = ck_beg[c], end = ck_end[c];
-    const T Wt = (T)W;
     const T c0 = (T)(1.0 / (4.0 * W * W));
     for (int p = beg + threadIdx.x; p < end; p += kNThreads) {
         int na, nb;
         unpack_ngp((uint32_t)ngp[p], na, nb);
         const T fu = frac[2 * (long)p], fv = frac[2 * (long)p + 1];
         T ku[NT], kv[NT];
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            ku[k] = kb_value<T>(P, fu + (T)(W - k), Wt);
-            kv[k] = kb_value<T>(P, fv + (T)(W - k), Wt);
-        }
+        kb_taps<T, W>(kc, fu, ku);
+        kb_taps<T, W>(kc, fv, kv);
         const int r0 = na - a0 - W, s0 = nb - b0 - W;  // window row/col of tap (0, 0)
         T ar = 0, ai = 0;
 #pragma unroll
@@ -229,7 +242,7 @@ __device__ inline void lds_add(T *addr, T v) {
 }
 
 template <class T, int W>
-__global__ void __launch_bounds__(kNThreads) k_grid(Geo G, KbPoly P, const Cplx<T> *__restrict__ vis,
+__global__ void __launch_bounds__(kNThreads) k_grid(Geo G, const T *__restrict__ kc, const Cplx<T> *__restrict__ vis,
                                                     const double *__restrict__ wts,
                                                     const int32_t *__restrict__ ck_tile,
                                                     const int32_t *__restrict__ ck_beg,
@@ -248,7 +261,6 @@ __global__ void __launch_bounds__(kNThreads) k_grid(Geo G, KbPoly P, const Cplx<
     __syncthreads();
     const int beg = ck_beg[c], end = ck_end[c];
     const int n = end - beg;
-    const T Wt = (T)W;
     // a stride coprime to n (near the golden ratio of n) spreads consecutive
     // threads over the chunk so that a wavefront's LDS atomics rarely meet on
     // one address; q -> q*stride mod n is a permutation of the chunk
@@ -271,14 +283,13 @@ __global__ void __launch_bounds__(kNThreads) k_grid(Geo G, KbPoly P, const Cplx<
         }
         const T rr = (T)0.5 * (cs * V.re - sn * V.im);
         const T ri = (T)0.5 * (sn * V.re + cs * V.im);
-        T kv[NT];
-#pragma unroll
-        for (int k = 0; k < NT; ++k) kv[k] = kb_value<T>(P, fv + (T)(W - k), Wt);
+        T ku[NT], kv[NT];
+        kb_taps<T, W>(kc, fu, ku);
+        kb_taps<T, W>(kc, fv, kv);
         const int r0 = na - a0 - W, s0 = nb - b0 - W;
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const T ku = kb_value<T>(P, fu + (T)(W - k), Wt);
-            const T tr = ku * rr, ti = ku * ri;
+            const T tr = ku[k] * rr, ti = ku[k] * ri;
             T *row = win + 2 * ((r0 + k) * G.WV + s0);
 #pragma unroll
             for (int l = 0; l < NT; ++l) {
@@ -434,7 +445,7 @@ __global__ void k_T_add_c128(long n, const T *__restrict__ a, double *__restrict
 // The phase factor cancels (e^{-i phi} then e^{+i phi}) and the kernel
 // weights are computed once per visibility for both directions.
 template <class T, int W>
-__global__ void __launch_bounds__(kNThreads) k_normal(Geo G, KbPoly P, const Cplx<T> *__restrict__ spec_in,
+__global__ void __launch_bounds__(kNThreads) k_normal(Geo G, const T *__restrict__ kc, const Cplx<T> *__restrict__ spec_in,
                                                       long sstride, const int32_t *__restrict__ ck_tile,
                                                       const int32_t *__restrict__ ck_beg,
                                                       const int32_t *__restrict__ ck_end,
@@ -461,7 +472,6 @@ __global__ void __launch_bounds__(kNThreads) k_normal(Geo G, KbPoly P, const Cpl
     __syncthreads();
     const int beg = ck_beg[c], end = ck_end[c];
     const int n = end - beg;
-    const T Wt = (T)W;
     const T c0h = (T)(0.5 / (4.0 * W * W));
     const int stride = coprime_stride(n);
     for (int q = threadIdx.x; q < n; q += kNThreads) {
@@ -470,11 +480,8 @@ __global__ void __launch_bounds__(kNThreads) k_normal(Geo G, KbPoly P, const Cpl
         unpack_ngp((uint32_t)ngp[p], na, nb);
         const T fu = frac[2 * (long)p], fv = frac[2 * (long)p + 1];
         T ku[NT], kv[NT];
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            ku[k] = kb_value<T>(P, fu + (T)(W - k), Wt);
-            kv[k] = kb_value<T>(P, fv + (T)(W - k), Wt);
-        }
+        kb_taps<T, W>(kc, fu, ku);
+        kb_taps<T, W>(kc, fv, kv);
         const int r0 = na - a0 - W, s0 = nb - b0 - W;
         T ar = 0, ai = 0;
 #pragma unroll
@@ -894,16 +901,97 @@ lt_status kb_poly(double beta, int prec, KbPoly &P) {
     return LT_OK;
 }
 
+}  // namespace
+
+// g(x) = I0(beta sqrt(1 - (x/W)^2)) continued analytically to |x| > W
+// (where it is J0(beta sqrt((x/W)^2 - 1))): the entire power series in s
+double kb_entire(double beta, int W, double x) {
+    const double s = 1.0 - (x / W) * (x / W);
+    const double q = 0.25 * beta * beta * s;
+    double term = 1.0, acc = 1.0;
+    for (int m = 1; m < 400; ++m) {
+        term *= q / ((double)m * (double)m);
+        acc += term;
+        if (m > std::fabs(q) && std::fabs(term) < 1e-19 * std::fabs(acc)) break;
+    }
+    return acc;
+}
+
+// Chebyshev interpolation of every tap on f in [-1/2, 1/2] at D+1 nodes,
+// converted to monomials in f; returns the max error over a dense sample
+// (edge taps only where they are nonzero) relative to the peak I0(beta)
+double kb_taps_fit(double beta, int W, int D, std::vector<double> &mono) {
+    const int NT = 2 * W + 1, n = D + 1;
+    mono.assign((size_t)NT * n, 0.0);
+    std::vector<double> y(n), cheb(n), tk(n), tkm1(n), tkp1(n);
+    for (int k = 0; k < NT; ++k) {
+        for (int j = 0; j < n; ++j) {
+            const double t = std::cos(M_PI * (j + 0.5) / n);
+            y[j] = kb_entire(beta, W, 0.5 * t + W - k);
+        }
+        for (int i = 0; i < n; ++i) {
+            double a = 0.0;
+            for (int j = 0; j < n; ++j) a += y[j] * std::cos(M_PI * i * (j + 0.5) / n);
+            cheb[i] = (i == 0 ? 1.0 : 2.0) * a / n;
+        }
+        // sum_i cheb_i T_i(t) as monomials in t = 2 f, with
+        // T_0 = 1, T_1 = t, T_{i+1} = 2 t T_i - T_{i-1}
+        std::vector<double> m(n, 0.0);
+        std::fill(tkm1.begin(), tkm1.end(), 0.0);
+        std::fill(tk.begin(), tk.end(), 0.0);
+        tkm1[0] = 1.0;
+        for (int e = 0; e < n; ++e) m[e] += cheb[0] * tkm1[e];
+        if (n > 1) {
+            tk[1] = 1.0;
+            for (int e = 0; e < n; ++e) m[e] += cheb[1] * tk[e];
+        }
+        for (int i = 2; i < n; ++i) {
+            for (int e = 0; e < n; ++e) tkp1[e] = (e > 0 ? 2.0 * tk[e - 1] : 0.0) - tkm1[e];
+            for (int e = 0; e < n; ++e) m[e] += cheb[i] * tkp1[e];
+            tkm1.swap(tk);
+            tk.swap(tkp1);
+        }
+        double sc = 1.0;
+        for (int e = 0; e < n; ++e, sc *= 2.0) mono[(size_t)k * n + e] = m[e] * sc;
+    }
+    const double peak = kb_entire(beta, W, 0.0);
+    double err = 0.0;
+    for (int k = 0; k < NT; ++k) {
+        for (int i = 0; i <= 512; ++i) {
+            const double f = -0.5 + i / 512.0;
+            if ((k == 0 && f > 0) || (k == NT - 1 && f < 0)) continue;
+            double acc = mono[(size_t)k * n + D];
+            for (int e = D - 1; e >= 0; --e) acc = acc * f + mono[(size_t)k * n + e];
+            err = std::max(err, std::fabs(acc - kb_entire(beta, W, f + W - k)) / peak);
+        }
+    }
+    return err;
+}
+
+namespace {
+
+template <class T>
+const T *taps_of(const NufftPlan &P);
+template <>
+const float *taps_of<float>(const NufftPlan &P) {
+    return P.taps32.p;
+}
+template <>
+const double *taps_of<double>(const NufftPlan &P) {
+    return P.taps64.p;
+}
+
 template <class T>
 lt_status launch_degrid(NufftPlan &P, const void *grid, void *vis, const double *wts, bool acc) {
     if (P.nchunks == 0) return LT_OK;
     Geo G = make_geo(P);
+    const T *kc = taps_of<T>(P);
     const size_t sh = (size_t)P.WU * P.WV * sizeof(Cplx<T>);
     const T *ph = P.phase ? (P.prec == LT_F64 ? (const T *)P.ph64.p : (const T *)P.ph32.p) : nullptr;
     const T *fr = P.prec == LT_F64 ? (const T *)P.frac64.p : (const T *)P.frac32.p;
 #define LT_DEGRID_CASE(WW)                                                                                        \
     case WW:                                                                                                      \
-        hipLaunchKernelGGL((k_degrid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, P.poly, \
+        hipLaunchKernelGGL((k_degrid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, kc, \
                            (const Cplx<T> *)grid, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph, wts,       \
                            (Cplx<T> *)vis, acc ? 1 : 0);                                                          \
         break;
@@ -928,13 +1016,14 @@ lt_status launch_degrid(NufftPlan &P, const void *grid, void *vis, const double 
 template <class T>
 lt_status launch_grid(NufftPlan &P, const void *vis, const double *wts, void *grid_out) {
     Geo G = make_geo(P);
+    const T *kc = taps_of<T>(P);
     const T *ph = P.phase ? (P.prec == LT_F64 ? (const T *)P.ph64.p : (const T *)P.ph32.p) : nullptr;
     const T *fr = P.prec == LT_F64 ? (const T *)P.frac64.p : (const T *)P.frac32.p;
     if (P.nchunks > 0) {
         const size_t sh = (size_t)P.WU * P.WV * sizeof(Cplx<T>);
 #define LT_GRID_CASE(WW)                                                                                        \
     case WW:                                                                                                    \
-        hipLaunchKernelGGL((k_grid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, P.poly, \
+        hipLaunchKernelGGL((k_grid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, kc, \
                            (const Cplx<T> *)vis, wts, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph,      \
                            (Cplx<T> *)P.partial.p);                                                             \
         break;
@@ -1011,6 +1100,23 @@ lt_status NufftPlan::create(const visibility_space &vs, const fourier_space &fs,
         return LT_ERR_CAPACITY;
     }
     LT_TRY(kb_poly(beta, prec, poly));
+    {
+        std::vector<double> mono;
+        const int D = (prec == LT_F64) ? kTapDeg64 : kTapDeg32;
+        taps_err = kb_taps_fit(beta, W, D, mono);
+        const double tol = (prec == LT_F64) ? 1e-13 : 1e-6;
+        if (!(taps_err <= tol)) {
+            set_error("kb_beta %.4g with kernel_support %d: kernel polynomial error %.2g above %.1g", beta, W,
+                      taps_err, tol);
+            return LT_ERR_ARG;
+        }
+        if (prec == LT_F64) {
+            LT_TRY(taps64.upload(mono.data(), (long)mono.size()));
+        } else {
+            std::vector<float> m32(mono.begin(), mono.end());
+            LT_TRY(taps32.upload(m32.data(), (long)m32.size()));
+        }
+    }
     cu = nu / 2;
     cv = nv / 2;
     TU = std::min(TU, nu);
@@ -1307,7 +1413,7 @@ static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, cons
 #define LT_NORMAL_CASE(WW)                                                                                         \
     case WW:                                                                                                       \
         hipLaunchKernelGGL((k_normal<T, WW>), dim3((unsigned)P.nchunks, nactive), dim3(kNThreads), sh, stream(), G, \
-                           P.poly, (const Cplx<T> *)P.specB.p, nspec, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, \
+                           taps_of<T>(P), (const Cplx<T> *)P.specB.p, nspec, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, \
                            fr, w, (Cplx<T> *)P.partialB.p, pstride, active);                                       \
         break;
                 switch (P.W) {

@@ -21,7 +21,31 @@ struct KbPoly {
     int deg;
 };
 
+// Per-tap polynomials of the kernel in the fractional offset f = u/du -
+// round(u/du) in [-1/2, 1/2]: tap k (0..2W) sits at distance x = f + W - k
+// and its value I0(beta sqrt(1 - (x/W)^2)) is an entire function of x, so a
+// Chebyshev fit of modest degree on f in [-1/2, 1/2] is accurate to the
+// working precision.  The edge taps are zero where |x| > W (f > 0 for k = 0,
+// f < 0 for k = 2W).  Coefficients are monomials in f, [k][m], m = 0..D.
+constexpr int kTapDeg32 = 9;
+constexpr int kTapDeg64 = 16;
+template <class T>
+struct TapDeg;
+template <>
+struct TapDeg<float> {
+    static constexpr int D = kTapDeg32;
+};
+template <>
+struct TapDeg<double> {
+    static constexpr int D = kTapDeg64;
+};
+
 struct NufftPlan;
+
+// per-tap monomial coefficients [2W+1][D+1] of the kernel (see TapDeg);
+// returns the max fit error relative to the kernel peak
+double kb_taps_fit(double beta, int W, int D, std::vector<double> &mono);
+double kb_entire(double beta, int W, double x);
 
 // D^T C D restricted to the uv grid, precomputed once per (uv set, weights):
 // with the phase cancelling and a real kernel, degrid-weight-grid is a real,
@@ -66,6 +90,9 @@ struct NufftPlan {
     int chunk = 8192;
     long nchunks = 0;
     KbPoly poly{};
+    DevBuf<float> taps32;   // (2W+1) x (kTapDeg32+1)
+    DevBuf<double> taps64;  // (2W+1) x (kTapDeg64+1)
+    double taps_err = 0.0;  // max fit error relative to the kernel peak
     DevBuf<int32_t> perm;        // sorted position -> channel-expanded index
     DevBuf<uint32_t> key_sorted;  // tile-major cell key of each sorted visibility
     DevBuf<int32_t> ngp;   // packed (ngp_u + 32768) | (ngp_v + 32768) << 16

@@ -91,6 +91,20 @@ class NUFFTPlan:
                                         prec, C.byref(h)))
         self._handle = h
 
+    def forward(self, image):
+        """Visibilities (complex128, channels x rows) of an image: NUFFTOperator.apply."""
+        img = np.ascontiguousarray(image, dtype=np.float64).reshape(-1)
+        vis = np.zeros(self.vis_space.size, dtype=np.complex128)
+        B.check(B.lib().lt_nufft_forward_host(self._handle, B.ptr(img), B.ptr(vis)))
+        return vis
+
+    def adjoint(self, vis):
+        """Image of N^T applied to visibilities (NUFFTOperator.T.apply)."""
+        v = np.ascontiguousarray(vis, dtype=np.complex128).reshape(-1)
+        img = np.zeros(self.image_space.size)
+        B.check(B.lib().lt_nufft_adjoint_host(self._handle, B.ptr(v), B.ptr(img)))
+        return img.reshape(self.image_space.shape)
+
     def time(self, direction, reps=20):
         """Mean device time (ms) of one degrid (0) or grid (1) launch."""
         ms = C.c_double(0)
@@ -108,11 +122,16 @@ class BatchedSourceInversion:
     """(L_s^T D^T C D L_s + lambda_c Cc^T Cc + lambda_t I) x_s = L_s^T D^T C d
     for every sample s of a LensBatch."""
 
-    def __init__(self, lens_batch, nufft_plan, data, sigma=None, weights=None, mask=None, lambda_curvature=0.0,
-                 lambda_tikhonov=0.0, operator='nufft'):
+    def __init__(self, lens_batch, nufft_plan, data=None, sigma=None, weights=None, mask=None, lambda_curvature=0.0,
+                 lambda_tikhonov=0.0, operator='nufft', dirty_images=None):
+        """data: one visibility vector shared by every sample; or
+        dirty_images: D^T C d_s per sample (or one), e.g. from
+        NUFFTPlan.adjoint(weights * d_s), when each sample has its own data."""
         self.lens_batch = lens_batch
         self.plan = nufft_plan
-        d = np.ascontiguousarray(data, dtype=np.complex128).reshape(-1)
+        if (data is None) == (dirty_images is None):
+            raise ValueError('give either data or dirty_images')
+        d = None if data is None else np.ascontiguousarray(data, dtype=np.complex128).reshape(-1)
         if weights is None:
             s = np.asarray(sigma, dtype=np.float64).reshape(-1)
             m = np.ones(s.shape, dtype=bool) if mask is None else np.asarray(mask, dtype=bool).reshape(-1)
@@ -124,8 +143,19 @@ class BatchedSourceInversion:
         self._d = d
         mode = {'nufft': B.LT_OP_NUFFT, 'bcb': B.LT_OP_BCB, 'nufft_explicit': B.LT_OP_NUFFT_EXPLICIT}[operator]
         h = C.c_void_p()
-        B.check(B.lib().lt_solver_create(lens_batch._handle, nufft_plan._handle, B.ptr(self._w), B.ptr(self._d),
-                                         float(lambda_curvature), float(lambda_tikhonov), mode, C.byref(h)))
+        if d is not None:
+            B.check(B.lib().lt_solver_create(lens_batch._handle, nufft_plan._handle, B.ptr(self._w), B.ptr(self._d),
+                                             float(lambda_curvature), float(lambda_tikhonov), mode, C.byref(h)))
+        else:
+            dl = dirty_images if isinstance(dirty_images, (list, tuple)) else [dirty_images]
+            npix = lens_batch.image_space.size
+            if len(dl) not in (1, lens_batch.nsamples) or any(np.size(a) != npix for a in dl):
+                raise ValueError('dirty_images: one image-space array, or one per sample')
+            self._dirty = np.ascontiguousarray(np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1)
+                                                               for a in dl]))
+            B.check(B.lib().lt_solver_create_dirty(lens_batch._handle, nufft_plan._handle, B.ptr(self._w),
+                                                   B.ptr(self._dirty), len(dl), float(lambda_curvature),
+                                                   float(lambda_tikhonov), mode, C.byref(h)))
         self._handle = h
         self._sizes = [s.size for s in lens_batch.source_spaces]
         self._off = np.concatenate([[0], np.cumsum(self._sizes)])

@@ -93,3 +93,35 @@ def test_bcb_matches_explicit(hip, oracle):
     err = O.max_relative_error(y_bcb, y_exp)
     print(f'BCB vs N^T C N: max-relative {err:.3e}')
     assert err < 1e-2  # BCB omits the apodization correction and truncates the beam
+
+
+def test_per_sample_dirty_images(hip, oracle):
+    """Each sample with its own observation: rhs_s = L_s^T D^T C d_s, given
+    through per-sample dirty images; a single dirty image equals passing the
+    data themselves."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    im, src, uv, vs, models, data = _problem(nlens=3)
+    lb = LensBatch(im, [src] * 3, models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    og = O.Grid2D(im.nx, im.ny, im.bounds)
+    geo = O.NUFFTGeometry(og, pad=2, wsup=3, beta=14.04)
+    datas = [data * (k + 1) + (rng(80 + k).normal(size=(uv.shape[0], 2)) * 1e-3).view(np.complex128).reshape(-1)
+             for k in range(3)]
+    dirty = [plan.adjoint(w * d) for d in datas]
+    for k in range(3):
+        ref = O.nufft_adjoint((w * datas[k]).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
+        assert O.max_relative_error(dirty[k].reshape(-1), ref) < 1e-12
+    inv = BatchedSourceInversion(lb, plan, weights=w, dirty_images=dirty, lambda_curvature=1e2)
+    rhs = inv.rhs()
+    for k in range(3):
+        rp, ci, va = lb.csr(k)
+        L = sp.csr_matrix((va, ci, rp), shape=(im.size, src.size))
+        assert O.max_relative_error(rhs[k].reshape(-1), L.T @ dirty[k].reshape(-1)) < 1e-12
+    one = BatchedSourceInversion(lb, plan, weights=w, dirty_images=[dirty[0]], lambda_curvature=1e2).rhs()
+    direct = BatchedSourceInversion(lb, plan, datas[0], weights=w, lambda_curvature=1e2).rhs()
+    for k in range(3):
+        assert O.max_relative_error(one[k].reshape(-1), direct[k].reshape(-1)) < 1e-12
+    with pytest.raises(ValueError):
+        BatchedSourceInversion(lb, plan, weights=w, dirty_images=dirty[:2])
