@@ -6,8 +6,12 @@
 //      triangles depth-first (raster_core.h), keeps up to kShort fragments
 //      in registers/scratch, sorts and reduces them to the CSR entry count;
 //      pixels with more fragments are marked long
-//   3. long rows: one thread re-emits the fragments to HBM, a segmented sort
-//      orders them by (source index, emission position), one thread reduces
+//   3. long rows (the pixels around the lens centre): each fan triangle's
+//      split tree is expanded breadth first by a workgroup into a frontier
+//      kept in the reference's depth-first order, every frontier subtree is
+//      walked by its own thread (count, scan, emit to HBM), a segmented radix
+//      sort orders the fragments by (source index, emission position) and
+//      the reduction runs in parallel over groups of equal source index
 //   4. exclusive scan of entry counts -> row pointers; running max of the
 //      last row with fragments -> the stale column of empty rows
 //   5. fill pass: short rows are rasterized again and written in place, long
@@ -15,6 +19,10 @@
 // The result is a block-diagonal CSR (global row / column indices) whose
 // per-sample blocks are bit-identical to manifold_lens_matrix_csr
 // (lentils/backend/raster.c:272-395) given the same deflections.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include <algorithm>
 #include <numeric>
 
@@ -43,6 +51,14 @@ struct LocalFrags {
         }
         ++n;
     }
+    // a row with more than kShort fragments is long: stop counting it here
+    __device__ bool stop() const { return n > kShort; }
+};
+
+struct CountSink {
+    long n = 0;
+    __device__ void operator()(double, int) { ++n; }
+    __device__ bool stop() const { return false; }
 };
 
 struct CountEmit {
@@ -100,23 +116,215 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_count(BatchGeom G, co
     minidx[r] = fr.n > 0 ? fr.idx[0] : -1;
 }
 
+// fragment n of a long row goes to base + n; its sort key is
+// (source index, emission position)
 struct GlobalFrags {
     double *area;
     uint64_t *key;
     long base;
-    int n = 0;
+    long n = 0;
     __device__ void operator()(double a, int i) {
         area[base + n] = a;
         key[base + n] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)n;
         ++n;
     }
+    __device__ bool stop() const { return false; }
 };
 
-__global__ void k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src, const double *__restrict__ lat,
-                            const int64_t *__restrict__ rows, const int64_t *__restrict__ foff, int nlong,
-                            double *__restrict__ area, uint64_t *__restrict__ key) {
+// Long rows with at most kMedium fragments are walked by one thread each
+// (sequentially, like short rows); longer ("huge") rows take the frontier
+// decomposition below.  This pass counts every long row up to kMedium + 1.
+constexpr long kMedium = 2048;
+
+struct CappedCount {
+    long n = 0;
+    __device__ void operator()(double, int) { ++n; }
+    __device__ bool stop() const { return n > kMedium; }
+};
+
+__global__ void k_long_count_capped(BatchGeom G, const SrcGrid *__restrict__ src, const double *__restrict__ lat,
+                                    const int64_t *__restrict__ rows, int nlong, int64_t *__restrict__ cnt,
+                                    int *__restrict__ status) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlong) return;
+    const long r = rows[l];
+    const int s = (int)(r / G.npix);
+    const long p = r - (long)s * G.npix;
+    const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
+    CappedCount cc;
+    const int st = raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cc);
+    if (!cc.stop() && st) atomicOr(&status[s], st);
+    cnt[l] = cc.n;
+}
+
+// Huge rows, step 1: one workgroup per (huge row, fan triangle) expands the
+// triangle's split tree breadth first in depth-first order (ritem_count /
+// ritem_write) while the frontier fits kLongFrontier items.  front holds two
+// kLongFrontier buffers per workgroup; the final frontier ends in the first.
+// A row whose expansion would overflow the reference's stack is flagged for
+// the sequential path.
+constexpr int kExpandThreads = 256;
+
+__global__ void __launch_bounds__(kExpandThreads) k_long_expand(BatchGeom G, const SrcGrid *__restrict__ src,
+                                                                 const double *__restrict__ lat,
+                                                                 const int64_t *__restrict__ rows, int nhuge,
+                                                                 RItem *__restrict__ front,
+                                                                 int32_t *__restrict__ nfront,
+                                                                 int *__restrict__ fallback) {
+    __shared__ int cnt[kLongFrontier + 1];
+    __shared__ int sh_n, sh_sel, sh_ovf, sh_sub, sh_stop;
+    const int b = blockIdx.x;
+    const int l = b >> 3, t = b & 7;
+    if (l >= nhuge) return;
+    const long r = rows[l];
+    const int s = (int)(r / G.npix);
+    const long p = r - (long)s * G.npix;
+    const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
+    const SrcGrid g = src[s];
+    RItem *bufA = front + (long)b * 2 * kLongFrontier;
+    RItem *bufB = bufA + kLongFrontier;
+    if (threadIdx.x == 0) {
+        double tv[6];
+        fan_vertices(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, t, tv);
+        RItem it;
+        raster_root(tv, g, it.tri);
+        it.depth = 0;
+        it.kind = 0;
+        it.area = 0.0;
+        it.idx = 0;
+        bufA[0] = it;
+        sh_n = 1;
+        sh_sel = 0;
+    }
+    __syncthreads();
+    for (int level = 0; level < kLongLevels; ++level) {
+        const int n = sh_n;
+        const RItem *cur = sh_sel ? bufB : bufA;
+        RItem *nxt = sh_sel ? bufA : bufB;
+        if (threadIdx.x == 0) {
+            sh_ovf = 0;
+            sh_sub = 0;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < n; q += kExpandThreads) {
+            int ovf = 0;
+            cnt[q] = ritem_count(cur[q], g, &ovf);
+            if (ovf) atomicOr(&sh_ovf, 1);
+            if (cur[q].kind == 0) atomicOr(&sh_sub, 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int q = 0; q < n; ++q) {
+                const int c = cnt[q];
+                cnt[q] = acc;
+                acc += c;
+            }
+            cnt[n] = acc;
+            sh_stop = (sh_ovf || !sh_sub || acc > kLongFrontier) ? 1 : 0;
+        }
+        __syncthreads();
+        if (sh_stop) break;
+        for (int q = threadIdx.x; q < n; q += kExpandThreads) ritem_write(cur[q], g, nxt + cnt[q]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            sh_n = cnt[n];
+            sh_sel ^= 1;
+        }
+        __syncthreads();
+    }
+    const int n = sh_n;
+    if (sh_sel)
+        for (int q = threadIdx.x; q < n; q += kExpandThreads) bufA[q] = bufB[q];
+    if (threadIdx.x == 0) {
+        nfront[b] = n;
+        if (sh_ovf) atomicOr(&fallback[l], 1);
+    }
+}
+
+// Huge rows, step 2 (count): fragments of every frontier item, each subtree
+// walked depth first from the stack depth the reference has at its root.
+// A subtree that overflows that stack or hits the pop limit flags its row
+// for the sequential path.
+__global__ void k_long_sub_count(const int64_t *__restrict__ rows, long npix, const SrcGrid *__restrict__ src,
+                                 const RItem *__restrict__ front, const int32_t *__restrict__ nfront, int nb,
+                                 int *__restrict__ fallback, int64_t *__restrict__ cnt) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = (int)(gid / kLongFrontier), q = (int)(gid - (long)b * kLongFrontier);
+    if (b >= nb) return;
+    if (q >= nfront[b]) {
+        cnt[gid] = 0;
+        return;
+    }
+    const int l = b >> 3;
+    const RItem &it = front[(long)b * 2 * kLongFrontier + q];
+    if (it.kind == 1) {
+        cnt[gid] = 1;
+        return;
+    }
+    const SrcGrid g = src[(int)(rows[l] / npix)];
+    CountSink cs;
+    if (raster_dfs(it.tri, it.depth, g, cs)) atomicOr(&fallback[l], 1);
+    cnt[gid] = cs.n;
+}
+
+// fragment count of every huge row: from the frontier counts, or by the
+// sequential walk for flagged rows
+__global__ void k_long_rowtot(BatchGeom G, const SrcGrid *__restrict__ src, const double *__restrict__ lat,
+                              const int64_t *__restrict__ rows, int nhuge, const int *__restrict__ fallback,
+                              const int64_t *__restrict__ pos, int64_t *__restrict__ tot, int *__restrict__ status) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nhuge) return;
+    const long r = rows[l];
+    const int s = (int)(r / G.npix);
+    long n;
+    if (!fallback[l]) {
+        n = pos[(long)(l + 1) * 8 * kLongFrontier] - pos[(long)l * 8 * kLongFrontier];
+    } else {
+        const long p = r - (long)s * G.npix;
+        const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
+        CountSink cs;
+        const int st = raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cs);
+        if (st) atomicOr(&status[s], st);
+        n = cs.n;
+    }
+    // the reference's fragment buffer holds kRasterFragCap entries and is
+    // overrun beyond that (raster.c:204-212); every fragment is kept here
+    if (n >= kRasterFragCap) atomicOr(&status[s], 2);  // LT_RASTER_FRAG_OVERFLOW
+    tot[l] = n;
+}
+
+// Huge rows, step 3 (emit): every frontier item writes its fragments at its
+// row's staging offset (foff, indexed by long-row number hmap[h]) + its
+// emission position
+__global__ void k_long_sub_emit(const int64_t *__restrict__ rows, const int32_t *__restrict__ hmap, long npix,
+                                const SrcGrid *__restrict__ src, const RItem *__restrict__ front,
+                                const int32_t *__restrict__ nfront, int nb, const int *__restrict__ fallback,
+                                const int64_t *__restrict__ pos, const int64_t *__restrict__ foff,
+                                double *__restrict__ area, uint64_t *__restrict__ key) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = (int)(gid / kLongFrontier), q = (int)(gid - (long)b * kLongFrontier);
+    if (b >= nb) return;
+    const int l = b >> 3;
+    if (q >= nfront[b] || fallback[l]) return;
+    const long n0 = pos[gid] - pos[(long)l * 8 * kLongFrontier];
+    const RItem &it = front[(long)b * 2 * kLongFrontier + q];
+    GlobalFrags gf{area, key, foff[hmap[l]], n0};
+    if (it.kind == 1) {
+        gf(it.area, it.idx);
+        return;
+    }
+    const SrcGrid g = src[(int)(rows[l] / npix)];
+    raster_dfs(it.tri, it.depth, g, gf);
+}
+
+// sequential emission of the long rows flagged for it (medium rows and huge
+// rows the frontier path gave back)
+__global__ void k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src, const double *__restrict__ lat,
+                            const int64_t *__restrict__ rows, const int64_t *__restrict__ foff, int nlong,
+                            const int *__restrict__ seq, double *__restrict__ area, uint64_t *__restrict__ key) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlong || !seq[l]) return;
     const long r = rows[l];
     const int s = (int)(r / G.npix);
     const long p = r - (long)s * G.npix;
@@ -125,36 +333,55 @@ __global__ void k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src, const 
     raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], gf);
 }
 
-// sequential reduction of one sorted long row (raster.c:375-391); entries
-// staged at the row's fragment offset
-__global__ void k_long_reduce(const int64_t *__restrict__ rows, const int64_t *__restrict__ foff, int nlong,
-                              const double *__restrict__ area, const uint64_t *__restrict__ key,
-                              int32_t *__restrict__ scol, double *__restrict__ sval, int64_t *__restrict__ nnz_row,
-                              int32_t *__restrict__ minidx) {
+// Reduction of the sorted long rows (raster.c:375-391) in parallel over
+// groups of equal source index: a group starts where the index changes; its
+// value is the in-order sum of its fragments, without the first one unless
+// the group opens the row (the reference's dropped fragment).
+__device__ inline int row_of(const int64_t *__restrict__ foff, int nlong, long f) {
+    int lo = 0, hi = nlong;  // foff[lo] <= f < foff[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (foff[mid] <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_group_flags(const int64_t *__restrict__ foff, int nlong, const uint64_t *__restrict__ key,
+                              int64_t *__restrict__ flag) {
+    const long F = foff[nlong];
+    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const int l = row_of(foff, nlong, f);
+    flag[f] = (f == foff[l] || (key[f] >> 32) != (key[f - 1] >> 32)) ? 1 : 0;
+}
+
+__global__ void k_group_sum(const int64_t *__restrict__ foff, int nlong, const uint64_t *__restrict__ key,
+                            const double *__restrict__ area, const int64_t *__restrict__ gpos,
+                            int32_t *__restrict__ scol, double *__restrict__ sval) {
+    const long F = foff[nlong];
+    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F || gpos[f + 1] == gpos[f]) return;  // not a group start
+    const int l = row_of(foff, nlong, f);
+    const long b = foff[l], e = foff[l + 1];
+    const uint32_t idx = (uint32_t)(key[f] >> 32);
+    double val = 0.0;
+    for (long k = (f == b) ? f : f + 1; k < e && (uint32_t)(key[k] >> 32) == idx; ++k)
+        val += area[b + (long)(uint32_t)key[k]];
+    scol[gpos[f]] = (int32_t)idx;
+    sval[gpos[f]] = val;
+}
+
+__global__ void k_group_rows(const int64_t *__restrict__ rows, const int64_t *__restrict__ foff, int nlong,
+                             const uint64_t *__restrict__ key, const int64_t *__restrict__ gpos,
+                             int64_t *__restrict__ nnz_row, int32_t *__restrict__ minidx,
+                             int64_t *__restrict__ ebase) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlong) return;
-    const long b = foff[l], e = foff[l + 1];
     const long r = rows[l];
-    int col = (int)(key[b] >> 32);
-    minidx[r] = col;
-    double val = 0.0;
-    long k = 0;
-    for (long f = b; f < e; ++f) {
-        const int idx = (int)(key[f] >> 32);
-        if (idx != col) {
-            scol[b + k] = col;
-            sval[b + k] = val;
-            ++k;
-            val = 0.0;
-            col = idx;
-        } else {
-            val += area[b + (long)(uint32_t)key[f]];
-        }
-    }
-    scol[b + k] = col;
-    sval[b + k] = val;
-    ++k;
-    nnz_row[r] = k;
+    nnz_row[r] = gpos[foff[l + 1]] - gpos[foff[l]];
+    minidx[r] = (int32_t)(key[foff[l]] >> 32);
+    ebase[l] = gpos[foff[l]];
 }
 
 __global__ void __launch_bounds__(kRasterThreads) k_raster_fill(BatchGeom G, const SrcGrid *__restrict__ src,
@@ -183,7 +410,7 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_fill(BatchGeom G, con
     reduce_fragments(fr.area, fr.idx, fr.n, stale, se);
 }
 
-__global__ void k_long_copy(const int64_t *__restrict__ rows, const int64_t *__restrict__ foff, int nlong,
+__global__ void k_long_copy(const int64_t *__restrict__ rows, const int64_t *__restrict__ ebase, int nlong,
                             const int64_t *__restrict__ rowoff, const int64_t *__restrict__ nnz_row,
                             const int32_t *__restrict__ scol, const double *__restrict__ sval,
                             const int64_t *__restrict__ srcoff, long npix, int32_t *__restrict__ cols,
@@ -192,7 +419,7 @@ __global__ void k_long_copy(const int64_t *__restrict__ rows, const int64_t *__r
     if (l >= nlong) return;
     const long r = rows[l];
     const int s = (int)(r / npix);
-    const long b = foff[l], n = nnz_row[r], o = rowoff[r];
+    const long b = ebase[l], n = nnz_row[r], o = rowoff[r];
     for (long k = threadIdx.x; k < n; k += blockDim.x) {
         cols[o + k] = (int32_t)srcoff[s] + scol[b + k];
         vals[o + k] = sval[b + k];
@@ -291,9 +518,9 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
     hipLaunchKernelGGL(k_raster_count, dim3(blocks_for(R, kRasterThreads)), dim3(kRasterThreads), 0, stream(), G,
                        src.p, lattice.p, mask.p, nnz_row.p, nfrag.p, minidx.p, lastne.p, dstatus.p);
     LT_CHECK_LAUNCH();
-    // long rows
-    // long rows: compacted on the device; only their (few) ids and fragment
-    // counts come back to size the staging area
+    // long rows: compacted on the device; their fragments are produced by the
+    // frontier decomposition (k_long_expand / k_long_sub_*), sorted by
+    // (source index, emission position) and reduced group-parallel
     DevBuf<int64_t> lflag, lpos;
     LT_TRY(lflag.alloc(R + 1));
     LT_TRY(lpos.alloc(R + 1));
@@ -303,41 +530,129 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
     int64_t nl64 = 0;
     LT_TRY(lpos.download(&nl64, 1, R));
     LT_TRY(sync());
-    std::vector<int64_t> lrows(nl64), loff(nl64 + 1, 0);
-    if (nl64 > 0) {
-        DevBuf<int64_t> lr;
-        DevBuf<int32_t> ln;
-        LT_TRY(lr.alloc(nl64));
-        LT_TRY(ln.alloc(nl64));
-        hipLaunchKernelGGL(k_long_compact, dim3(blocks_for(R, 256)), dim3(256), 0, stream(), R, lpos.p, nfrag.p, lr.p,
-                           ln.p);
-        LT_CHECK_LAUNCH();
-        std::vector<int32_t> hn(nl64);
-        LT_TRY(lr.download(lrows.data(), nl64));
-        LT_TRY(ln.download(hn.data(), nl64));
-        LT_TRY(sync());
-        for (long k = 0; k < nl64; ++k) loff[k + 1] = loff[k] + hn[k];
-    }
-    const int nlong = (int)lrows.size();
-    DevBuf<int64_t> drows, dfoff;
-    DevBuf<double> farea, sval;
-    DevBuf<uint64_t> fkey;
+    const int nlong = (int)nl64;
+    DevBuf<int64_t> drows, dfoff, ebase;
+    DevBuf<double> sval;
     DevBuf<int32_t> scol;
     if (nlong) {
+        LT_TRY(drows.alloc(nlong));
+        {
+            DevBuf<int32_t> ln;
+            LT_TRY(ln.alloc(nlong));
+            hipLaunchKernelGGL(k_long_compact, dim3(blocks_for(R, 256)), dim3(256), 0, stream(), R, lpos.p, nfrag.p,
+                               drows.p, ln.p);
+            LT_CHECK_LAUNCH();
+        }
+        // exact counts of medium rows; rows beyond kMedium are huge
+        DevBuf<int64_t> lcnt;
+        LT_TRY(lcnt.alloc(nlong));
+        hipLaunchKernelGGL(k_long_count_capped, dim3(blocks_for(nlong, 64)), dim3(64), 0, stream(), G, src.p,
+                           lattice.p, drows.p, nlong, lcnt.p, dstatus.p);
+        LT_CHECK_LAUNCH();
+        std::vector<int64_t> hcnt(nlong), hrows_all(nlong);
+        LT_TRY(lcnt.download(hcnt.data(), nlong));
+        LT_TRY(drows.download(hrows_all.data(), nlong));
+        LT_TRY(sync());
+        std::vector<int32_t> hmap;
+        std::vector<int64_t> hrow;
+        for (int l = 0; l < nlong; ++l)
+            if (hcnt[l] > kMedium) {
+                hmap.push_back(l);
+                hrow.push_back(hrows_all[l]);
+            }
+        const int nhuge = (int)hmap.size();
+        std::vector<int> seq(nlong, 1);
+        std::vector<int64_t> loff(nlong + 1, 0);
+        DevBuf<int32_t> dhmap, nfront;
+        DevBuf<int64_t> dhrow, cnt, pos, tot;
+        DevBuf<RItem> front;
+        DevBuf<int> hfb;
+        const int nb = nhuge * 8;
+        const long nitems = (long)nb * kLongFrontier;
+        if (nhuge) {
+            LT_TRY(dhmap.upload(hmap.data(), nhuge));
+            LT_TRY(dhrow.upload(hrow.data(), nhuge));
+            LT_TRY(front.alloc(2 * nitems));
+            LT_TRY(nfront.alloc(nb));
+            LT_TRY(hfb.alloc(nhuge));
+            LT_TRY(hfb.zero());
+            LT_TRY(cnt.alloc(nitems));
+            LT_TRY(pos.alloc(nitems + 1));
+            LT_TRY(tot.alloc(nhuge));
+            hipLaunchKernelGGL(k_long_expand, dim3(nb), dim3(kExpandThreads), 0, stream(), G, src.p, lattice.p,
+                               dhrow.p, nhuge, front.p, nfront.p, hfb.p);
+            LT_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_long_sub_count, dim3(blocks_for(nitems, 256)), dim3(256), 0, stream(), dhrow.p,
+                               npix, src.p, front.p, nfront.p, nb, hfb.p, cnt.p);
+            LT_CHECK_LAUNCH();
+            LT_TRY(exclusive_scan_i64(cnt.p, pos.p, nitems));
+            hipLaunchKernelGGL(k_long_rowtot, dim3(blocks_for(nhuge, 64)), dim3(64), 0, stream(), G, src.p,
+                               lattice.p, dhrow.p, nhuge, hfb.p, pos.p, tot.p, dstatus.p);
+            LT_CHECK_LAUNCH();
+            std::vector<int64_t> htot(nhuge);
+            std::vector<int> hf(nhuge);
+            LT_TRY(tot.download(htot.data(), nhuge));
+            LT_TRY(hfb.download(hf.data(), nhuge));
+            LT_TRY(sync());
+            for (int h = 0; h < nhuge; ++h) {
+                hcnt[hmap[h]] = htot[h];
+                seq[hmap[h]] = hf[h] ? 1 : 0;
+            }
+        }
+        for (int l = 0; l < nlong; ++l) loff[l + 1] = loff[l] + hcnt[l];
         const long F = loff.back();
-        LT_TRY(drows.upload(lrows.data(), nlong));
+        if (F >= (long)UINT32_MAX) {
+            set_error("lens batch: %ld long-row fragments exceed the sort range", F);
+            return LT_ERR_CAPACITY;
+        }
+        DevBuf<int> dseq;
+        LT_TRY(dseq.upload(seq.data(), nlong));
         LT_TRY(dfoff.upload(loff.data(), nlong + 1));
+        DevBuf<double> farea;
+        DevBuf<uint64_t> fkey, fkey2;
         LT_TRY(farea.alloc(F));
         LT_TRY(fkey.alloc(F));
-        LT_TRY(scol.alloc(F));
-        LT_TRY(sval.alloc(F));
+        LT_TRY(fkey2.alloc(F));
+        if (nhuge) {
+            hipLaunchKernelGGL(k_long_sub_emit, dim3(blocks_for(nitems, 256)), dim3(256), 0, stream(), dhrow.p,
+                               dhmap.p, npix, src.p, front.p, nfront.p, nb, hfb.p, pos.p, dfoff.p, farea.p, fkey.p);
+            LT_CHECK_LAUNCH();
+        }
         hipLaunchKernelGGL(k_long_emit, dim3(blocks_for(nlong, 64)), dim3(64), 0, stream(), G, src.p, lattice.p,
-                           drows.p, dfoff.p, nlong, farea.p, fkey.p);
+                           drows.p, dfoff.p, nlong, dseq.p, farea.p, fkey.p);
         LT_CHECK_LAUNCH();
-        LT_TRY(segmented_sort_u64(fkey.p, dfoff.p, loff));
-        hipLaunchKernelGGL(k_long_reduce, dim3(blocks_for(nlong, 64)), dim3(64), 0, stream(), drows.p, dfoff.p,
-                           nlong, farea.p, fkey.p, scol.p, sval.p, nnz_row.p, minidx.p);
+        {
+            size_t tb = 0;
+            if (rocprim::segmented_radix_sort_keys(nullptr, tb, fkey.p, fkey2.p, (unsigned)F, (unsigned)nlong,
+                                                   dfoff.p, dfoff.p + 1, 0, 64, stream()) != hipSuccess) {
+                set_error("segmented sort sizing failed");
+                return LT_ERR_HIP;
+            }
+            DevBuf<uint8_t> tmp;
+            LT_TRY(tmp.alloc(std::max<size_t>(tb, 1)));
+            LT_HIP(rocprim::segmented_radix_sort_keys(tmp.p, tb, fkey.p, fkey2.p, (unsigned)F, (unsigned)nlong,
+                                                      dfoff.p, dfoff.p + 1, 0, 64, stream()));
+        }
+        DevBuf<int64_t> gflag, gpos;
+        LT_TRY(gflag.alloc(F));
+        LT_TRY(gpos.alloc(F + 1));
+        hipLaunchKernelGGL(k_group_flags, dim3(blocks_for(F, 256)), dim3(256), 0, stream(), dfoff.p, nlong, fkey2.p,
+                           gflag.p);
         LT_CHECK_LAUNCH();
+        LT_TRY(exclusive_scan_i64(gflag.p, gpos.p, F));
+        int64_t ngroups = 0;
+        LT_TRY(gpos.download(&ngroups, 1, F));
+        LT_TRY(sync());
+        LT_TRY(scol.alloc(std::max<int64_t>(ngroups, 1)));
+        LT_TRY(sval.alloc(std::max<int64_t>(ngroups, 1)));
+        LT_TRY(ebase.alloc(nlong));
+        hipLaunchKernelGGL(k_group_sum, dim3(blocks_for(F, 256)), dim3(256), 0, stream(), dfoff.p, nlong, fkey2.p,
+                           farea.p, gpos.p, scol.p, sval.p);
+        LT_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_group_rows, dim3(blocks_for(nlong, 64)), dim3(64), 0, stream(), drows.p, dfoff.p, nlong,
+                           fkey2.p, gpos.p, nnz_row.p, minidx.p, ebase.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(sync());
     }
     LT_TRY(exclusive_scan_i64(nnz_row.p, rowoff.p, R));
     LT_TRY(inclusive_max_i32(lastne.p, lastne.p, R));
@@ -361,7 +676,7 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
                        src.p, srcoff.p, lattice.p, mask.p, rowoff.p, nfrag.p, minidx.p, lastne.p, L.ci.p, L.va.p);
     LT_CHECK_LAUNCH();
     if (nlong) {
-        hipLaunchKernelGGL(k_long_copy, dim3(nlong), dim3(256), 0, stream(), drows.p, dfoff.p, nlong, rowoff.p,
+        hipLaunchKernelGGL(k_long_copy, dim3(nlong), dim3(256), 0, stream(), drows.p, ebase.p, nlong, rowoff.p,
                            nnz_row.p, scol.p, sval.p, srcoff.p, npix, L.ci.p, L.va.p);
         LT_CHECK_LAUNCH();
     }

@@ -74,15 +74,10 @@ __host__ __device__ inline const CutTables &cut_tables() {
 #endif
 }
 
-// Rasterize one triangle (vertices tv[0..5]); sink(area, idx) is called per
-// fragment in emission order.  Returns LT_RASTER_* status bits.
-template <class Sink>
-__host__ __device__ int raster_triangle(const double *tv, const SrcGrid &g, Sink &sink) {
-    const CutTables &T = cut_tables();
-    RTri stk[kRasterStack];
-    RTri tri;
-    const double cell[2] = {g.dx, g.dy};
-    const double org[2] = {g.xmin, g.ymin};
+// Root node of one fan triangle (raster.c:144-170): its three source-plane
+// vertices, barycentric corners (0,0), (0,1), (1,0) and the index range of
+// the source cells it touches.
+LT_HD void raster_root(const double *tv, const SrcGrid &g, RTri &tri) {
     tri.lo[0] = g.nx;
     tri.lo[1] = g.ny;
     tri.hi[0] = 0;
@@ -100,69 +95,189 @@ __host__ __device__ int raster_triangle(const double *tv, const SrcGrid &g, Sink
         if (i + 1 > tri.hi[0]) tri.hi[0] = i + 1;
         if (j + 1 > tri.hi[1]) tri.hi[1] = j + 1;
     }
-    int ns = 0, pops = 0;
-    stk[ns++] = tri;
-    while (ns > 0) {
-        if (++pops > kRasterPopLimit) return 4;  // LT_RASTER_POP_LIMIT
-        tri = stk[--ns];
-        int ax = 0, ext = 0;
-        for (int a = 0; a < 2; ++a) {
-            int s = tri.hi[a] - tri.lo[a];
-            if (s > ext) {
-                ext = s;
-                ax = a;
-            }
+}
+
+// What the reference does with a node it pops (raster.c:180-268).  A node of
+// extent 1 is a leaf: returns -1 and sets emit (inside the grid) with the
+// fragment's area and source index.  Otherwise the node is cut at the middle
+// grid line of its larger extent (ties -> x) and the pieces that overlap the
+// grid are written to kids[0..m-1] in the order the reference pushes them;
+// returns m (the reference processes them last to first).
+LT_HD int raster_expand(const RTri &tri, const SrcGrid &g, RTri *kids, double &area, int &idx, bool &emit) {
+    const CutTables &T = cut_tables();
+    int ax = 0, ext = 0;
+    for (int a = 0; a < 2; ++a) {
+        int s = tri.hi[a] - tri.lo[a];
+        if (s > ext) {
+            ext = s;
+            ax = a;
         }
-        const int cut = tri.lo[ax] + ext / 2;
-        const double cutc = cut * cell[ax] + org[ax];
-        if (ext == 1) {
-            const int i = tri.lo[0], j = tri.lo[1];
-            if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) continue;
+    }
+    const int cut = tri.lo[ax] + ext / 2;
+    const double cell = ax ? g.dy : g.dx;
+    const double org = ax ? g.ymin : g.xmin;
+    const double cutc = cut * cell + org;
+    if (ext == 1) {
+        const int i = tri.lo[0], j = tri.lo[1];
+        emit = !(i < 0 || i >= g.nx || j < 0 || j >= g.ny);
+        if (emit) {
             const double x1 = tri.b[0][0], y1 = tri.b[0][1];
             const double x2 = tri.b[1][0], y2 = tri.b[1][1];
             const double x3 = tri.b[2][0], y3 = tri.b[2][1];
-            sink(-0.25 * (x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2)), g.ny * i + j);
-            continue;
+            area = -0.25 * (x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2));
+            idx = g.ny * i + j;
         }
-        double pv[5][2], pb[5][2], dist[3];
-        int mask = 0;
+        return -1;
+    }
+    double pv[5][2], pb[5][2], dist[3];
+    int mask = 0;
+    for (int v = 0; v < 3; ++v) {
+        pv[v][0] = tri.v[v][0];
+        pv[v][1] = tri.v[v][1];
+        pb[v][0] = tri.b[v][0];
+        pb[v][1] = tri.b[v][1];
+        dist[v] = pv[v][ax] - cutc;
+        mask |= ((dist[v] > 0.) << v);
+    }
+    const int sidx[3] = {tri.lo[ax], cut, tri.hi[ax]};
+    for (int k = 0; k < T.ncut[mask]; ++k) {
+        const int e0 = T.edge[mask][k][0], e1 = T.edge[mask][k][1];
+        const double w0 = dist[e1], w1 = -dist[e0];
+        const double wn = 1.0 / (w0 + w1);
+        const double a0 = w0 * wn, a1 = w1 * wn;
+        pv[3 + k][0] = a0 * pv[e0][0] + a1 * pv[e1][0];
+        pv[3 + k][1] = a0 * pv[e0][1] + a1 * pv[e1][1];
+        pb[3 + k][0] = a0 * pb[e0][0] + a1 * pb[e1][0];
+        pb[3 + k][1] = a0 * pb[e0][1] + a1 * pb[e1][1];
+    }
+    int m = 0;
+    for (int c = 0; c < T.nchild[mask]; ++c) {
+        const int s = T.side[mask][c];
+        int lo[2] = {tri.lo[0], tri.lo[1]}, hi[2] = {tri.hi[0], tri.hi[1]};
+        lo[ax] = sidx[s];
+        hi[ax] = sidx[s + 1];
+        if (hi[0] <= 0 || lo[0] >= g.nx) continue;
+        if (hi[1] <= 0 || lo[1] >= g.ny) continue;
+        RTri &k = kids[m++];
+        k.lo[0] = lo[0];
+        k.lo[1] = lo[1];
+        k.hi[0] = hi[0];
+        k.hi[1] = hi[1];
         for (int v = 0; v < 3; ++v) {
-            pv[v][0] = tri.v[v][0];
-            pv[v][1] = tri.v[v][1];
-            pb[v][0] = tri.b[v][0];
-            pb[v][1] = tri.b[v][1];
-            dist[v] = pv[v][ax] - cutc;
-            mask |= ((dist[v] > 0.) << v);
-        }
-        const int sidx[3] = {tri.lo[ax], cut, tri.hi[ax]};
-        for (int k = 0; k < T.ncut[mask]; ++k) {
-            const int e0 = T.edge[mask][k][0], e1 = T.edge[mask][k][1];
-            const double w0 = dist[e1], w1 = -dist[e0];
-            const double wn = 1.0 / (w0 + w1);
-            const double a0 = w0 * wn, a1 = w1 * wn;
-            pv[3 + k][0] = a0 * pv[e0][0] + a1 * pv[e1][0];
-            pv[3 + k][1] = a0 * pv[e0][1] + a1 * pv[e1][1];
-            pb[3 + k][0] = a0 * pb[e0][0] + a1 * pb[e1][0];
-            pb[3 + k][1] = a0 * pb[e0][1] + a1 * pb[e1][1];
-        }
-        for (int c = 0; c < T.nchild[mask]; ++c) {
-            const int s = T.side[mask][c];
-            tri.lo[ax] = sidx[s];
-            tri.hi[ax] = sidx[s + 1];
-            if (tri.hi[0] <= 0 || tri.lo[0] >= g.nx) continue;
-            if (tri.hi[1] <= 0 || tri.lo[1] >= g.ny) continue;
-            for (int v = 0; v < 3; ++v) {
-                const int src = T.vert[mask][c][v];
-                tri.v[v][0] = pv[src][0];
-                tri.v[v][1] = pv[src][1];
-                tri.b[v][0] = pb[src][0];
-                tri.b[v][1] = pb[src][1];
-            }
-            stk[ns++] = tri;
-            if (ns >= kRasterStack) return 1;  // LT_RASTER_STACK_OVERFLOW
+            const int src = T.vert[mask][c][v];
+            k.v[v][0] = pv[src][0];
+            k.v[v][1] = pv[src][1];
+            k.b[v][0] = pb[src][0];
+            k.b[v][1] = pb[src][1];
         }
     }
+    return m;
+}
+
+// Depth-first rasterization from node `root`, which sits on the reference's
+// 64-entry stack above `base` other entries (0 for a whole triangle).
+// sink(area, idx) is called per fragment in emission order; a sink whose
+// stop() turns true ends the walk early (count passes that only need to know
+// that a row is long).  Returns LT_RASTER_* status bits: the reference gives
+// up on the triangle when a push fills its stack (raster.c:262-266).
+template <class Sink>
+__host__ __device__ int raster_dfs(const RTri &root, int base, const SrcGrid &g, Sink &sink) {
+    RTri stk[kRasterStack + 3];
+    int ns = 0, pops = 0;
+    stk[ns++] = root;
+    while (ns > 0) {
+        if (++pops > kRasterPopLimit) return 4;  // LT_RASTER_POP_LIMIT
+        const RTri tri = stk[--ns];
+        double area = 0.0;
+        int idx = 0;
+        bool emit = false;
+        const int m = raster_expand(tri, g, &stk[ns], area, idx, emit);
+        if (m < 0) {
+            if (emit) {
+                sink(area, idx);
+                if (sink.stop()) return 0;
+            }
+            continue;
+        }
+        // the reference pushes one piece at a time and stops as soon as its
+        // stack holds kRasterStack entries
+        if (m > 0 && ns + base + m >= kRasterStack) return 1;  // LT_RASTER_STACK_OVERFLOW
+        ns += m;
+    }
     return 0;
+}
+
+// Rasterize one triangle (vertices tv[0..5]).
+template <class Sink>
+__host__ __device__ int raster_triangle(const double *tv, const SrcGrid &g, Sink &sink) {
+    RTri tri;
+    raster_root(tv, g, tri);
+    return raster_dfs(tri, 0, g, sink);
+}
+
+// ---------------------------------------------------------------------------
+// Rows with very many fragments (the pixels around the lens centre fan over
+// much of the source plane) are rasterized in parallel without changing the
+// emission order: the split tree of each fan triangle is expanded breadth
+// first, keeping every level in the reference's depth-first order (children
+// last-pushed first, fragments in place), until the frontier would exceed
+// kLongFrontier items; each remaining subtree is then walked depth first by
+// its own thread, starting with the stack depth the reference would have at
+// that node.  Concatenating the frontier's outputs in order reproduces the
+// reference's fragment sequence exactly.
+constexpr int kLongFrontier = 512;
+constexpr int kLongLevels = 40;
+
+struct RItem {
+    RTri tri;
+    int depth;  // entries below this node on the reference's stack
+    int kind;   // 0: subtree to expand, 1: fragment
+    double area;
+    int idx;
+};
+
+// number of frontier items `it` turns into at the next level; sets *ovf
+// when expanding it would overflow the reference's stack
+LT_HD int ritem_count(const RItem &it, const SrcGrid &g, int *ovf) {
+    if (it.kind == 1) return 1;
+    RTri kids[3];
+    double area = 0.0;
+    int idx = 0;
+    bool emit = false;
+    const int m = raster_expand(it.tri, g, kids, area, idx, emit);
+    if (m < 0) return emit ? 1 : 0;
+    if (m > 0 && it.depth + m >= kRasterStack) *ovf = 1;
+    return m;
+}
+
+// writes the items `it` turns into at out[0..], in depth-first order
+LT_HD void ritem_write(const RItem &it, const SrcGrid &g, RItem *out) {
+    if (it.kind == 1) {
+        out[0] = it;
+        return;
+    }
+    RTri kids[3];
+    double area = 0.0;
+    int idx = 0;
+    bool emit = false;
+    const int m = raster_expand(it.tri, g, kids, area, idx, emit);
+    if (m < 0) {
+        if (emit) {
+            out[0].kind = 1;
+            out[0].depth = it.depth;
+            out[0].area = area;
+            out[0].idx = idx;
+        }
+        return;
+    }
+    for (int k = 0; k < m; ++k) {  // pushed k-th, processed (m-1-k)-th
+        RItem &o = out[m - 1 - k];
+        o.tri = kids[k];
+        o.depth = it.depth + k;
+        o.kind = 0;
+        o.area = 0.0;
+        o.idx = 0;
+    }
 }
 
 // lattice offsets of the 9 stencil points of a pixel: corners, edge
@@ -188,12 +303,25 @@ __host__ __device__ int raster_pixel(const double *lat, int ly, int i, int j, co
         sp[k][1] = lat[2 * (a * ly + b) + 1];
     }
     int st = 0;
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < 8 && !sink.stop(); ++t) {
         const int t1 = (t + 1) & 7;
         const double tv[6] = {sp[t1][0], sp[t1][1], sp[t][0], sp[t][1], sp[8][0], sp[8][1]};
         st |= raster_triangle(tv, g, sink);
     }
     return st;
+}
+
+// vertices of fan triangle t of pixel (i, j), as raster_pixel forms them
+LT_HD void fan_vertices(const double *lat, int ly, int i, int j, int t, double tv[6]) {
+    const int t1 = (t + 1) & 7;
+    const int pts[3] = {t1, t, 8};
+    for (int v = 0; v < 3; ++v) {
+        int ox, oy;
+        stencil_offsets(pts[v], ox, oy);
+        const long a = 2L * i + ox, b = 2L * j + oy;
+        tv[2 * v] = lat[2 * (a * ly + b)];
+        tv[2 * v + 1] = lat[2 * (a * ly + b) + 1];
+    }
 }
 
 // Stable insertion sort of a short fragment list by source index followed

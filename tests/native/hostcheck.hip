@@ -16,6 +16,7 @@ struct VecSink {
         area.push_back(a);
         idx.push_back(i);
     }
+    bool stop() const { return false; }
 };
 struct StoreEmit {
     int *cols;
@@ -64,6 +65,61 @@ long hc_lens_matrix_from_lattice(int nx, int ny, const char *mask, int snx, int 
     rp[(long)nx * ny] = (int)k;
     if (status) *status = st;
     return k;
+}
+
+// Fragments of image pixel (i, j) in emission order.  mode 0: the
+// sequential depth-first walk (raster_pixel); mode 1: the long-row
+// decomposition the GPU uses (breadth-first frontier of at most `front`
+// items per fan triangle, then one depth-first walk per frontier item).
+// Returns the fragment count (at most cap are stored), or -1 when mode 1
+// meets a stack overflow / pop limit (the GPU then falls back to mode 0).
+long hc_pixel_frags(const double *lat, int ny, int i, int j, int snx, int sny, double sxmin, double symin, double sdx,
+                    double sdy, int mode, int front, double *area, int *idx, long cap, int *levels_out) {
+    SrcGrid g{snx, sny, sxmin, symin, sdx, sdy};
+    const int ly = 2 * ny + 1;
+    VecSink s;
+    int levels_max = 0;
+    if (mode == 0) {
+        raster_pixel(lat, ly, i, j, g, s);
+    } else {
+        for (int t = 0; t < 8; ++t) {
+            double tv[6];
+            fan_vertices(lat, ly, i, j, t, tv);
+            std::vector<RItem> cur(1), nxt;
+            raster_root(tv, g, cur[0].tri);
+            cur[0].depth = 0;
+            cur[0].kind = 0;
+            int level = 0;
+            for (; level < kLongLevels; ++level) {
+                std::vector<int> cnt(cur.size() + 1, 0);
+                int ovf = 0, any_sub = 0;
+                for (size_t q = 0; q < cur.size(); ++q) {
+                    cnt[q + 1] = cnt[q] + ritem_count(cur[q], g, &ovf);
+                    any_sub |= (cur[q].kind == 0);
+                }
+                if (ovf) return -1;
+                if (!any_sub || cnt.back() > front) break;
+                nxt.assign(cnt.back(), RItem{});
+                for (size_t q = 0; q < cur.size(); ++q) ritem_write(cur[q], g, nxt.data() + cnt[q]);
+                cur.swap(nxt);
+            }
+            if (level > levels_max) levels_max = level;
+            for (const RItem &it : cur) {
+                if (it.kind == 1) {
+                    s(it.area, it.idx);
+                } else if (raster_dfs(it.tri, it.depth, g, s)) {
+                    return -1;
+                }
+            }
+        }
+    }
+    const long n = (long)s.area.size();
+    for (long k = 0; k < n && k < cap; ++k) {
+        area[k] = s.area[k];
+        idx[k] = s.idx[k];
+    }
+    if (levels_out) *levels_out = levels_max;
+    return n;
 }
 
 }  // extern "C"

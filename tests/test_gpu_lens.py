@@ -74,16 +74,29 @@ def test_drop_in_deflect_points(hip, oracle):
     assert np.max(np.abs(out - O.deflect_points(c1_lens(O), pts))) < 1e-12
 
 
-def test_raster_bit_exact_from_oracle_lattice_c1(hip, oracle):
+@pytest.mark.parametrize('case', ['c1', 'wide'])
+def test_raster_bit_exact_from_oracle_lattice(hip, oracle, case):
+    """Bit-exact CSR from the oracle's deflections, including the rows
+    around the lens centre: c1 (two rows above 2000 fragments) and the
+    0.01"/px, 128^2-source geometry of the plain GPU workload around a random
+    lens (a centre row touching every source pixel, >2048 fragments: the
+    breadth-first frontier path)."""
     B, O = hip, oracle
-    im = O.Grid2D(128, 128, [-1.28, 1.28, -1.28, 1.28])
-    src = O.Grid2D(64, 64, [-0.64, 0.64, -0.64, 0.64])
-    lat = O.deflect_points(c1_lens(O), O.stencil_lattice(im).reshape(-1, 2)).reshape(257, 257, 2)
+    if case == 'c1':
+        im = O.Grid2D(128, 128, [-1.28, 1.28, -1.28, 1.28])
+        src = O.Grid2D(64, 64, [-0.64, 0.64, -0.64, 0.64])
+        lenses = c1_lens(O)
+    else:
+        im = O.Grid2D(96, 96, [-0.48, 0.48, -0.48, 0.48])
+        src = O.Grid2D(128, 128, [-0.64, 0.64, -0.64, 0.64])
+        lenses = random_lenses(O, 1, seed=5)[0]
+    n = im.nx
+    lat = O.deflect_points(lenses, O.stencil_lattice(im).reshape(-1, 2)).reshape(2 * n + 1, 2 * n + 1, 2)
     rr, cr, vr, _ = O.lens_matrix_from_lattice(im, src, lat)
     rp, ci, va, st = _csr(B, B.lib().lt_lens_matrix_from_lattice, im, src, B.ptr(np.ascontiguousarray(lat)))
     assert np.array_equal(rp, rr) and np.array_equal(ci, cr)
     assert np.array_equal(va.view(np.int64), vr.view(np.int64))
-    assert np.diff(rp).max() > 1000  # long-row path (lens-centre pixel) covered
+    assert np.diff(rp).max() > 3000  # long-row path (lens-centre pixel) covered
 
 
 def test_raster_bit_exact_masked_empty_rows(hip, oracle):

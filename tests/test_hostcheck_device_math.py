@@ -39,6 +39,10 @@ def hc():
                                               C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
     L.hc_lens_matrix_from_lattice.restype = C.c_long
+    L.hc_pixel_frags.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_long,
+                                 C.POINTER(C.c_int)]
+    L.hc_pixel_frags.restype = C.c_long
     return L
 
 
@@ -74,3 +78,51 @@ def test_device_raster_code_bitwise_on_host(hc, oracle):
                                    _p(ci), _p(va), C.byref(st))
     assert np.array_equal(rp, rr) and np.array_equal(ci, cr)
     assert np.array_equal(va.view(np.int64), vr.view(np.int64))
+
+
+def _frags(hc, lat, ny, i, j, src, mode, front, cap=4_000_000):
+    area = np.zeros(cap)
+    idx = np.zeros(cap, dtype=np.int32)
+    lev = C.c_int(0)
+    n = hc.hc_pixel_frags(_p(lat), ny, i, j, src.nx, src.ny, src.xmin, src.ymin, src.dx, src.dy, mode, front,
+                          _p(area), _p(idx), cap, C.byref(lev))
+    assert n <= cap
+    return (None if n < 0 else (area[:n].copy(), idx[:n].copy())), lev.value
+
+
+@pytest.mark.parametrize('case', ['c1', 'wide'])
+def test_long_row_decomposition_reproduces_emission_order(hc, oracle, case):
+    """The breadth-first frontier + per-subtree depth-first decomposition the
+    GPU uses for long rows emits exactly the reference's fragment sequence
+    (same areas bit for bit, same order), for frontier caps from 1 to 512."""
+    O = oracle
+    if case == 'c1':
+        im = O.Grid2D(128, 128, [-1.28, 1.28, -1.28, 1.28])
+        src = O.Grid2D(64, 64, [-0.64, 0.64, -0.64, 0.64])
+        lenses = c1_lens(O)
+        ci, cj = 65, 63
+    else:  # bench geometry around the lens centre: 0.01" pixels, 128^2 source
+        im = O.Grid2D(96, 96, [-0.48, 0.48, -0.48, 0.48])
+        src = O.Grid2D(128, 128, [-0.64, 0.64, -0.64, 0.64])
+        lenses = random_lenses(O, 1, seed=5)[0]
+        fp = np.array(lenses, dtype=O.MASS_MODEL_DTYPE)[0]['fpars']
+        ci, cj = int((fp[0] - im.xmin) / im.dx), int((fp[1] - im.ymin) / im.dy)
+    n = im.nx
+    lat = O.deflect_points(lenses, O.stencil_lattice(im).reshape(-1, 2)).reshape(2 * n + 1, 2 * n + 1, 2)
+    lat = np.ascontiguousarray(lat)
+    g = rng(33)
+    pix = [(ci + a, cj + b) for a in range(-2, 3) for b in range(-2, 3)]
+    pix += [tuple(v) for v in g.integers(0, n, size=(10, 2))]
+    longest = 0
+    compared = 0
+    for (i, j) in pix:
+        ref, _ = _frags(hc, lat, n, i, j, src, 0, 0)
+        longest = max(longest, ref[0].size)
+        for front in (1, 4, 16, 512):
+            got, lev = _frags(hc, lat, n, i, j, src, 1, front)
+            if got is None:  # the GPU would take the sequential path for this row
+                continue
+            assert np.array_equal(got[1], ref[1]), (i, j, front)
+            assert np.array_equal(got[0].view(np.int64), ref[0].view(np.int64)), (i, j, front)
+            compared += ref[0].size > 1000
+    assert longest > 1000 and compared >= 4
