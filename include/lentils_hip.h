@@ -230,6 +230,7 @@ lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, do
 #define LT_OP_NUFFT_EXPLICIT 2 /* exact D^T C D, per-visibility degrid-weight-grid each apply */
 #define LT_PC_NONE 0
 #define LT_PC_JACOBI 1
+#define LT_PC_LEARNED 2 /* learned network (lt_solver_set_pcnet) in flexible CG, Jacobi as its guard */
 lt_status lt_solver_create(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *data, double lambda_curv,
                            double lambda_tikh, int op_mode, lt_solver *out);
 /* As lt_solver_create, with the data given through their dirty images
@@ -243,8 +244,59 @@ lt_status lt_solver_create_dirty(lt_lensbatch lb, lt_nufft nf, const double *wei
  * sample (may be NULL). */
 lt_status lt_solver_solve(lt_solver s, double rtol, int maxiter, int precond, double *x, int *iters, double *resid);
 lt_status lt_solver_rhs(lt_solver s, double *rhs);
+/* replace the right-hand sides (all samples concatenated, host) */
+lt_status lt_solver_set_rhs(lt_solver s, const double *rhs);
 lt_status lt_solver_apply(lt_solver s, const double *x, double *y);
 lt_status lt_solver_destroy(lt_solver s);
+
+/* --- learned preconditioner ------------------------------------------------
+ * No reference counterpart (the reference preconditions its CG with a sparse
+ * LU of R + w L^T L, test_all.py:63-72).  A small convolutional network on
+ * the H x W source grid (H, W multiples of 8): 2 -> 16 -> 32 -> 64 -> 128
+ * channels down by 2x2 average pooling, back up by nearest upsampling with
+ * skip connections, 3x3 kernels, ReLU; BF16 operands on MFMA, FP32
+ * accumulate, FP32 master weights.  Input per sample: the residual over its
+ * RMS and diag(A) over its mean; output y, preconditioned direction
+ * z = y * rms(r) / mean(diag(A)).  lt_pcnet_param_count() floats of
+ * parameters: per layer, weights [co][ci][3][3] then biases [co]. */
+typedef struct lt_pcnet_s *lt_pcnet;
+long lt_pcnet_param_count(void);
+/* params NULL: He-initialised from seed.  batch: largest batch a forward /
+ * training step / solve may use. */
+lt_status lt_pcnet_create(int H, int W, int batch, const float *params, unsigned seed, lt_pcnet *out);
+lt_status lt_pcnet_get_params(lt_pcnet net, float *params);
+lt_status lt_pcnet_set_params(lt_pcnet net, const float *params);
+/* x0: nb x 2 x H x W, y: nb x H x W (host, FP32) */
+lt_status lt_pcnet_forward_host(lt_pcnet net, const float *x0, int nb, float *y);
+/* parameter gradient of sum(dy * y(x0)) (and y), for gradient checks */
+lt_status lt_pcnet_grad_host(lt_pcnet net, const float *x0, const float *dy, int nb, float *y, float *grad);
+/* Adam on mean_s |y_s - t_s|^2 / |t_s|^2 over n pairs (inputs n x 2 x H x W,
+ * targets n x H x W), shuffled minibatches, cosine learning-rate decay from
+ * lr to lr_final; loss_hist[epoch] (may be NULL). */
+lt_status lt_pcnet_train(lt_pcnet net, const float *inputs, const float *targets, long n, int epochs, int batch,
+                         float lr, float lr_final, unsigned seed, double *loss_hist);
+lt_status lt_pcnet_eval(lt_pcnet net, const float *inputs, const float *targets, long n, double *loss);
+lt_status lt_pcnet_destroy(lt_pcnet net);
+/* use `net` for LT_PC_LEARNED solves (every sample's source grid must be the
+ * network's H x W and the batch within its capacity) */
+lt_status lt_solver_set_pcnet(lt_solver s, lt_pcnet net);
+/* Train the network (epochs > 0) or evaluate it (epochs = 0) with this
+ * solver's operators A_s: residuals holds nres vectors of all samples'
+ * sources concatenated (host, FP64), e.g. right-hand sides and residuals of
+ * truncated solves; each step takes one vector per sample and descends the
+ * energy loss mean_s (z^T A z - 2 z^T r) / (r^T r / mean diag), which is the
+ * A-norm error ||z - A^{-1} r||_A^2 up to a constant.  loss_hist[epoch]: mean
+ * loss (minimum -r^T A^{-1} r / c; the scaled identity z = r / mean diag
+ * gives r^T A r / (mean diag r^T r) - 2). */
+lt_status lt_solver_train_pcnet(lt_solver s, const double *residuals, int nres, int epochs, float lr, float lr_final,
+                                unsigned seed, double *loss_hist);
+/* the diagonal of A by 8 x 8 probing (all samples concatenated) */
+lt_status lt_solver_diag(lt_solver s, double *diag);
+/* per sample, for the last LT_PC_LEARNED solve: counts = iterations in which
+ * the network's direction was rejected (r.z <= 0) and the Jacobi one taken;
+ * plain = 1 if the sample left the network for plain CG (its residual norm
+ * fell less than 10x in 100 iterations) (may be NULL) */
+lt_status lt_solver_pc_fallbacks(lt_solver s, int *counts, int *plain);
 
 #ifdef __cplusplus
 }

@@ -50,7 +50,7 @@ LT_OK = 0
 LT_ERR_ARG, LT_ERR_HIP, LT_ERR_NYQUIST, LT_ERR_UNSUPPORTED, LT_ERR_CAPACITY, LT_ERR_FFT, LT_ERR_NOT_CONVERGED = 1, 2, 3, 4, 5, 6, 7
 LT_F64, LT_F32 = 0, 1
 LT_OP_NUFFT, LT_OP_BCB, LT_OP_NUFFT_EXPLICIT = 0, 1, 2
-LT_PC_NONE, LT_PC_JACOBI = 0, 1
+LT_PC_NONE, LT_PC_JACOBI, LT_PC_LEARNED = 0, 1, 2
 RASTER_STACK_OVERFLOW, RASTER_FRAG_OVERFLOW, RASTER_POP_LIMIT, RASTER_BAD_LENS = 1, 2, 4, 8
 
 
@@ -105,8 +105,22 @@ def lib():
         'lt_nufft_destroy': ([vp], i),
         'lt_solver_create': ([vp, vp, vp, vp, d, d, i, C.POINTER(vp)], i),
         'lt_solver_create_dirty': ([vp, vp, vp, vp, i, d, d, i, C.POINTER(vp)], i),
+        'lt_pcnet_param_count': ([], l),
+        'lt_pcnet_create': ([i, i, i, vp, C.c_uint, C.POINTER(vp)], i),
+        'lt_pcnet_get_params': ([vp, vp], i),
+        'lt_pcnet_set_params': ([vp, vp], i),
+        'lt_pcnet_forward_host': ([vp, vp, i, vp], i),
+        'lt_pcnet_grad_host': ([vp, vp, vp, i, vp, vp], i),
+        'lt_pcnet_train': ([vp, vp, vp, l, i, i, C.c_float, C.c_float, C.c_uint, vp], i),
+        'lt_pcnet_eval': ([vp, vp, vp, l, dp], i),
+        'lt_pcnet_destroy': ([vp], i),
+        'lt_solver_set_pcnet': ([vp, vp], i),
+        'lt_solver_diag': ([vp, vp], i),
+        'lt_solver_train_pcnet': ([vp, vp, i, i, C.c_float, C.c_float, C.c_uint, vp], i),
+        'lt_solver_pc_fallbacks': ([vp, vp, vp], i),
         'lt_solver_solve': ([vp, d, i, i, vp, vp, vp], i),
         'lt_solver_rhs': ([vp, vp], i),
+        'lt_solver_set_rhs': ([vp, vp], i),
         'lt_solver_apply': ([vp, vp, vp], i),
         'lt_solver_destroy': ([vp], i),
         'lt_fft_r2c_host': ([i, i, i, vp, vp], i),
@@ -128,8 +142,11 @@ def exported_symbols():
             'lt_lens_matrix_nnz', 'lt_lens_matrix_csr', 'lt_deflect_lattice', 'lt_lens_matrix_from_lattice',
             'lt_lensbatch_create', 'lt_lensbatch_info', 'lt_lensbatch_download', 'lt_lensbatch_destroy',
             'lt_nufft_create', 'lt_nufft_forward_host', 'lt_nufft_adjoint_host', 'lt_nufft_degrid_host',
-            'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_kb_taps', 'lt_solver_create', 'lt_solver_create_dirty',
-            'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
+            'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_kb_taps',
+            'lt_solver_create', 'lt_solver_create_dirty', 'lt_pcnet_param_count', 'lt_pcnet_create',
+            'lt_pcnet_get_params', 'lt_pcnet_set_params', 'lt_pcnet_forward_host', 'lt_pcnet_grad_host',
+            'lt_pcnet_train', 'lt_pcnet_eval', 'lt_pcnet_destroy', 'lt_solver_set_pcnet', 'lt_solver_diag', 'lt_solver_train_pcnet',
+            'lt_solver_pc_fallbacks', 'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_set_rhs', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
             'lt_fft_c2r_host', 'lt_csr_spmv_host']
 
 

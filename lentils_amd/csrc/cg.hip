@@ -13,8 +13,10 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <random>
 
 #include "nufft_plan.h"
+#include "pcnet.h"
 #include "raster_batch.h"
 
 namespace lt {
@@ -123,44 +125,27 @@ __global__ void k_cg_beta(int ns, int *__restrict__ act, double *__restrict__ rz
     if (!(rr[s] > rtol2 * bb[s])) act[s] = 0;  // also stops on NaN
 }
 
-// Jacobi diagonal of L^T B L with the data operator B approximated by its
-// (translation-invariant) response beta to a unit impulse at the image
-// centre: diag_j = sum_{a,b in column j of L} L_a L_b beta(p_a - p_b).
-// One wavefront per source pixel (row of L^T).
-__global__ void __launch_bounds__(256) k_jacobi_diag(long nrow, const int32_t *__restrict__ rp,
-                                                      const int32_t *__restrict__ ci, const double *__restrict__ va,
-                                                      const int32_t *__restrict__ seg, long npix, int nx, int ny,
-                                                      const double *__restrict__ beta, double *__restrict__ diag) {
-    const long row = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = threadIdx.x & 63;
-    if (row >= nrow) return;
-    const long base = (long)seg[row] * npix;
-    const int b0 = rp[row], b1 = rp[row + 1];
-    const int ic = nx / 2, jc = ny / 2;
-    double acc = 0.0;
-    for (int a = b0 + lane; a < b1; a += 64) {
-        const long pa = ci[a] - base;
-        const int ia = (int)(pa / ny), ja = (int)(pa % ny);
-        const double va_ = va[a];
-        for (int b = b0; b < b1; ++b) {
-            const long pb = ci[b] - base;
-            const int di = (int)(pb / ny) - ia + ic, dj = (int)(pb % ny) - ja + jc;
-            if (di >= 0 && di < nx && dj >= 0 && dj < ny) acc += va_ * va[b] * beta[(long)di * ny + dj];
-        }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) diag[row] = acc;
+// probing vectors for the diagonal: 1 on source pixels (i, j) with
+// i mod m == cx and j mod m == cy of every sample, else 0
+__global__ void k_probe_set(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                            const int *__restrict__ ny, int m, int cx, int cy, double *__restrict__ v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const long q = i - off[s];
+    const int a = (int)(q / ny[s]), b = (int)(q - (long)a * ny[s]);
+    v[i] = (a % m == cx && b % m == cy) ? 1.0 : 0.0;
 }
 
-// sum of squares of each CSR row (diag of C^T C for a symmetric C)
-__global__ void k_row_sumsq(long nrow, const int32_t *__restrict__ rp, const double *__restrict__ va,
-                            double *__restrict__ out) {
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrow) return;
-    double s = 0.0;
-    for (int k = rp[r]; k < rp[r + 1]; ++k) s += va[k] * va[k];
-    out[r] = s;
+__global__ void k_probe_take(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                             const int *__restrict__ ny, int m, int cx, int cy, const double *__restrict__ w,
+                             double *__restrict__ d) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const long q = i - off[s];
+    const int a = (int)(q / ny[s]), b = (int)(q - (long)a * ny[s]);
+    if (a % m == cx && b % m == cy) d[i] = w[i];
 }
 
 __global__ void k_jacobi_inv(long n, const double *__restrict__ dd, const double *__restrict__ dc, double lc, double lt,
@@ -171,9 +156,120 @@ __global__ void k_jacobi_inv(long n, const double *__restrict__ dd, const double
     dinv[i] = (d > 0.0 && isfinite(d)) ? 1.0 / d : 1.0;
 }
 
-__global__ void k_delta(long n, long at, double *__restrict__ v) {
+// flexible (Polak-Ribiere) beta for a preconditioner that changes between
+// iterations: beta = z_new . (r_new - r_old) / (z_old . r_old)
+//                  = -alpha (z_new . A p) / (z_old . r_old)
+__global__ void k_cg_beta_flex(int ns, int *__restrict__ act, double *__restrict__ rz,
+                               const double *__restrict__ rz_new, const double *__restrict__ zap,
+                               const double *__restrict__ alpha, const double *__restrict__ rr,
+                               const double *__restrict__ bb, double rtol2, double *__restrict__ beta,
+                               int *__restrict__ iters) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    if (!act[s]) return;
+    iters[s] += 1;
+    beta[s] = rz[s] != 0.0 ? -alpha[s] * zap[s] / rz[s] : 0.0;
+    rz[s] = rz_new[s];
+    if (!(rr[s] > rtol2 * bb[s])) act[s] = 0;
+}
+
+__global__ void k_recip(long n, const double *__restrict__ a, double *__restrict__ b) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) v[i] = (i == at) ? 1.0 : 0.0;
+    if (i < n) b[i] = 1.0 / a[i];
+}
+
+__global__ void k_fill(long n, double v, double *__restrict__ a) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// network input of the learned preconditioner: (r / rms(r), diag / mean diag)
+// per sample, FP32 [s][2][HW]
+__global__ void k_pc_in(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                        const double *__restrict__ r, const double *__restrict__ rr, const float *__restrict__ dhat,
+                        int HW, float *__restrict__ x0) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const long pix = i - off[s];
+    const double N = (double)(off[s + 1] - off[s]);
+    const double rms = sqrt(rr[s] / N);
+    x0[((long)s * 2) * HW + pix] = rms > 0.0 ? (float)(r[i] / rms) : 0.f;
+    x0[((long)s * 2 + 1) * HW + pix] = dhat[i];
+}
+
+// z = y rms(r) / mean diag
+__global__ void k_pc_out(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                         const float *__restrict__ y, const double *__restrict__ rr, const double *__restrict__ ainv,
+                         int HW, double *__restrict__ z) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const long pix = i - off[s];
+    const double N = (double)(off[s + 1] - off[s]);
+    z[i] = (double)y[(long)s * HW + pix] * sqrt(rr[s] / N) * ainv[s];
+}
+
+// samples whose network direction is not a descent direction (r.z <= 0 or
+// not finite) take the Jacobi direction this iteration
+__global__ void k_pc_guard(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
+                           const double *__restrict__ dinv, const double *__restrict__ r, double *__restrict__ z,
+                           int *__restrict__ nfall) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    if (!(rz[s] > 0.0) || !isfinite(rz[s])) {
+        z[i] = dinv[i] * r[i];
+        if (i == 0 || seg[i - 1] != s) atomicAdd(&nfall[s], 1);
+    }
+}
+
+// Gradient of the energy loss of the learned preconditioner,
+//   L = (1/S) sum_s (z_s^T A z_s - 2 z_s^T r_s) / c_s,  c_s = r_s^T r_s / mean diag_s
+// (the A-norm error ||z - A^{-1} r||_A^2 up to a constant), with respect to
+// the network output y (z = y rms(r) / mean diag).
+__global__ void k_energy_grad(long n, int S, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                              const double *__restrict__ Az, const double *__restrict__ r,
+                              const double *__restrict__ rr, const double *__restrict__ ainv, int HW,
+                              float *__restrict__ gy) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const long pix = i - off[s];
+    const double N = (double)(off[s + 1] - off[s]);
+    const double c = rr[s] * ainv[s];
+    const double scale = c > 0.0 ? 2.0 * sqrt(rr[s] / N) * ainv[s] / (c * S) : 0.0;
+    gy[(long)s * HW + pix] = (float)((Az[i] - r[i]) * scale);
+}
+
+// samples that left the network (plain CG fallback) take z = r
+__global__ void k_pc_plain(long n, const int32_t *__restrict__ seg, const int *__restrict__ usenet,
+                           const double *__restrict__ r, double *__restrict__ z) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !usenet[seg[i]]) z[i] = r[i];
+}
+
+// restart of the samples switched to plain CG: p = r, r.z = r.r
+__global__ void k_restart(long n, const int32_t *__restrict__ seg, const int *__restrict__ flag,
+                          const double *__restrict__ r, double *__restrict__ p) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[seg[i]]) p[i] = r[i];
+}
+
+__global__ void k_restart_rz(int ns, const int *__restrict__ flag, const double *__restrict__ rr,
+                             double *__restrict__ rz) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns && flag[s]) rz[s] = rr[s];
+}
+
+// normalised diagonal for the network input and 1 / mean diag per sample
+__global__ void k_pc_dhat(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                          const double *__restrict__ diag, const double *__restrict__ dsum, float *__restrict__ dhat) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const double mean = dsum[s] / (double)(off[s + 1] - off[s]);
+    dhat[i] = mean > 0.0 ? (float)(diag[i] / mean) : 0.f;
 }
 
 }  // namespace
@@ -199,12 +295,24 @@ struct Solver {
     double beam_peak = 0.0;
     NormalStencil stencil;
     bool use_stencil = false;
+    PcNet *net = nullptr;          // learned preconditioner (LT_PC_LEARNED)
+    DevBuf<float> dhat;            // diag / mean diag per sample, network input
+    DevBuf<double> ainv, zap;      // 1 / mean diag; z . A p
+    DevBuf<int> nfall;             // Jacobi fallbacks taken per sample
+    DevBuf<int> usenet;            // 1: network direction, 0: plain CG fallback
+    std::vector<int> nfall_host, usenet_host;
 
     lt_status create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data,
                      const double *dirty_in, int ndirty_in, double lc, double lt, int op_mode);
     lt_status apply(const double *xin, double *yout, const int *active_list, int nactive);
     lt_status solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out);
     lt_status jacobi_setup();
+    lt_status learned_setup();
+    // train (epochs > 0) or evaluate the network on residual vectors R
+    // (nres x ntot, host) with the energy loss of this solver's operators
+    lt_status train_pcnet(const double *R, int nres, int epochs, float lr, float lr_final, unsigned seed,
+                          double *loss_hist);
+    lt_status precondition(int precond, const double *dinv_p);  // z from r
 };
 
 fourier_space padded_fourier_space(const image_space &im, int pad);
@@ -348,45 +456,198 @@ lt_status Solver::apply(const double *xin, double *yout, const int *alist, int n
     return LT_OK;
 }
 
+// diag(A) by probing: for each of the m x m residue classes c of source
+// pixels (i mod m, j mod m), v_c = indicator of the class and
+// (A v_c)_j = A_jj + sum of A_jk over the other pixels k of j's class, so the
+// estimate is exact where pixels m apart are not coupled (the dirty beam
+// and the lens mapping keep couplings local; multiple images add small
+// distant terms).  m^2 applications of A for every sample at once.
 lt_status Solver::jacobi_setup() {
-    // impulse response of the data operator at the image centre (sample 0)
-    const long c = (long)(lb->im.nx / 2) * lb->im.ny + lb->im.ny / 2;
-    hipLaunchKernelGGL(k_delta, dim3(blocks_for(npix * S, 256)), dim3(256), 0, stream(), npix * S, c, img1.p);
-    LT_CHECK_LAUNCH();
-    if (mode != LT_OP_BCB)
-        LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, all_active.p, 1, 1.0, false, use_stencil ? &stencil : nullptr));
-    else
-        LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, all_active.p, 1, 1.0, false));
-    DevBuf<double> dd, dc;
+    constexpr int m = 8;
+    std::vector<int> hny(S);
+    for (int s = 0; s < S; ++s) hny[s] = lb->src_host[s].ny;
+    DevBuf<int> dny;
+    DevBuf<double> v, wv, dd;
+    LT_TRY(dny.upload(hny.data(), S));
+    LT_TRY(v.alloc(ntot));
+    LT_TRY(wv.alloc(ntot));
     LT_TRY(dd.alloc(ntot));
-    LT_TRY(dc.alloc(ntot));
     LT_TRY(dinv.alloc(ntot));
-    const DevCSR &T = lb->LT;
-    hipLaunchKernelGGL(k_jacobi_diag, dim3(blocks_for(T.nrows * 64, 256)), dim3(256), 0, stream(), T.nrows, T.rp.p,
-                       T.ci.p, T.va.p, seg.p, npix, lb->im.nx, lb->im.ny, img2.p, dd.p);
+    const unsigned gb = blocks_for(ntot, 256);
+    for (int cx = 0; cx < m; ++cx)
+        for (int cy = 0; cy < m; ++cy) {
+            hipLaunchKernelGGL(k_probe_set, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, dny.p, m, cx, cy,
+                               v.p);
+            LT_CHECK_LAUNCH();
+            LT_TRY(apply(v.p, wv.p, all_active.p, S));
+            hipLaunchKernelGGL(k_probe_take, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, dny.p, m, cx, cy,
+                               wv.p, dd.p);
+            LT_CHECK_LAUNCH();
+        }
+    hipLaunchKernelGGL(k_jacobi_inv, dim3(gb), dim3(256), 0, stream(), ntot, dd.p, dd.p, 0.0, 0.0, dinv.p);
     LT_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_row_sumsq, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, C.rp.p, C.va.p, dc.p);
+    return sync();
+}
+
+lt_status Solver::precondition(int precond, const double *dinv_p) {
+    const unsigned gb = blocks_for(ntot, 256);
+    if (precond != LT_PC_LEARNED) {
+        hipLaunchKernelGGL(k_precond, dim3(gb), dim3(256), 0, stream(), ntot, dinv_p, r.p, z.p);
+        LT_CHECK_LAUNCH();
+        return LT_OK;
+    }
+    // rr = r . r must be current (network input scale)
+    const int HW = net->H * net->W;
+    hipLaunchKernelGGL(k_pc_in, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, r.p, rr.p, dhat.p, HW,
+                       net->act[PcNet::X0].p);
     LT_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_jacobi_inv, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, dd.p, dc.p, lam_c,
-                       lam_t, dinv.p);
+    LT_TRY(net->forward(S));
+    hipLaunchKernelGGL(k_pc_out, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, net->act[PcNet::Y].p, rr.p,
+                       ainv.p, HW, z.p);
+    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz_new.p);
+    hipLaunchKernelGGL(k_pc_guard, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rz_new.p, dinv_p, r.p, z.p,
+                       nfall.p);
+    hipLaunchKernelGGL(k_pc_plain, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, usenet.p, r.p, z.p);
     LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+lt_status Solver::learned_setup() {
+    if (!net) {
+        set_error("solver: LT_PC_LEARNED needs a network (lt_solver_set_pcnet)");
+        return LT_ERR_ARG;
+    }
+    if (S > net->nbcap) {
+        set_error("solver: %d samples exceed the network's batch capacity %d", S, net->nbcap);
+        return LT_ERR_ARG;
+    }
+    for (int s = 0; s < S; ++s)
+        if (lb->src_host[s].nx != net->H || lb->src_host[s].ny != net->W) {
+            set_error("solver: sample %d source grid %dx%d differs from the network's %dx%d", s, lb->src_host[s].nx,
+                      lb->src_host[s].ny, net->H, net->W);
+            return LT_ERR_ARG;
+        }
+    if (!dinv.p) LT_TRY(jacobi_setup());
+    const unsigned gb = blocks_for(ntot, 256);
+    DevBuf<double> dsum, onesv;
+    LT_TRY(dsum.alloc(S));
+    LT_TRY(onesv.alloc(ntot));
+    LT_TRY(dhat.alloc(ntot));
+    LT_TRY(ainv.alloc(S));
+    LT_TRY(zap.alloc(S));
+    LT_TRY(nfall.alloc(S));
+    LT_TRY(usenet.alloc(S));
+    hipLaunchKernelGGL(k_recip, dim3(gb), dim3(256), 0, stream(), ntot, dinv.p, tmp.p);  // diag
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(256), 0, stream(), ntot, 1.0, onesv.p);
+    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, tmp.p, onesv.p, dsum.p);
+    hipLaunchKernelGGL(k_pc_dhat, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, tmp.p, dsum.p, dhat.p);
+    LT_CHECK_LAUNCH();
+    std::vector<double> hs(S), hinv(S);
+    LT_TRY(dsum.download(hs.data(), S));
+    LT_TRY(sync());
+    for (int s = 0; s < S; ++s) {
+        const double mean = hs[s] / (double)(soff_host[s + 1] - soff_host[s]);
+        hinv[s] = mean > 0 ? 1.0 / mean : 1.0;
+    }
+    LT_TRY(ainv.upload(hinv.data(), S));
+    return sync();
+}
+
+lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, float lr_final, unsigned seed,
+                              double *loss_hist) {
+    LT_TRY(learned_setup());
+    const unsigned gb = blocks_for(ntot, 256);
+    const int HW = net->H * net->W;
+    DevBuf<double> pool, zAz, zr;
+    LT_TRY(pool.upload(R, (long)nres * ntot));
+    LT_TRY(zAz.alloc(S));
+    LT_TRY(zr.alloc(S));
+    std::vector<int> order(nres);
+    for (int k = 0; k < nres; ++k) order[k] = k;
+    std::mt19937 gen(seed);
+    const int nep = epochs > 0 ? epochs : 1;
+    const long total = (long)nep * nres;
+    long it = 0;
+    std::vector<double> hzz(S), hzr(S), hrr(S), hai(S);
+    LT_TRY(ainv.download(hai.data(), S));
+    for (int ep = 0; ep < nep; ++ep) {
+        if (epochs > 0) std::shuffle(order.begin(), order.end(), gen);
+        double acc = 0.0;
+        for (int q = 0; q < nres; ++q, ++it) {
+            hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, pool.p + (long)order[q] * ntot, r.p);
+            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+            hipLaunchKernelGGL(k_pc_in, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, r.p, rr.p, dhat.p, HW,
+                               net->act[PcNet::X0].p);
+            LT_CHECK_LAUNCH();
+            LT_TRY(net->forward(S));
+            hipLaunchKernelGGL(k_pc_out, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p,
+                               net->act[PcNet::Y].p, rr.p, ainv.p, HW, z.p);
+            LT_CHECK_LAUNCH();
+            LT_TRY(apply(z.p, Ap.p, all_active.p, S));
+            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, z.p, Ap.p, zAz.p);
+            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, z.p, r.p, zr.p);
+            LT_CHECK_LAUNCH();
+            if (epochs > 0) {
+                hipLaunchKernelGGL(k_energy_grad, dim3(gb), dim3(256), 0, stream(), ntot, S, seg.p, soff.p, Ap.p,
+                                   r.p, rr.p, ainv.p, HW, net->gact[PcNet::Y].p);
+                LT_CHECK_LAUNCH();
+                LT_TRY(net->backward(S));
+                const float fr = total > 1 ? (float)it / (float)(total - 1) : 1.f;
+                const float lrt = lr_final + 0.5f * (lr - lr_final) * (1.f + std::cos((float)M_PI * fr));
+                LT_TRY(net->adam_step(lrt, 1.0f));
+            }
+            LT_TRY(zAz.download(hzz.data(), S));
+            LT_TRY(zr.download(hzr.data(), S));
+            LT_TRY(rr.download(hrr.data(), S));
+            LT_TRY(sync());
+            double l = 0.0;
+            for (int s = 0; s < S; ++s) {
+                const double c = hrr[s] * hai[s];
+                l += c > 0 ? (hzz[s] - 2.0 * hzr[s]) / c : 0.0;
+            }
+            acc += l / S;
+        }
+        if (loss_hist) loss_hist[ep] = acc / nres;
+    }
     return sync();
 }
 
 lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out) {
     const unsigned gb = blocks_for(ntot, 256), gs = blocks_for(S, 64);
+    if (precond != LT_PC_NONE && precond != LT_PC_JACOBI && precond != LT_PC_LEARNED) {
+        set_error("solver: unknown preconditioner %d", precond);
+        return LT_ERR_ARG;
+    }
     // Jacobi: diag(A) ~ beam peak * diag(L^T L) + lambda_c diag(Cc^T Cc) + lambda_t
     const double *dinv_p = nullptr;
-    if (precond == LT_PC_JACOBI) {
+    if (precond == LT_PC_JACOBI || precond == LT_PC_LEARNED) {
         if (!dinv.p) LT_TRY(jacobi_setup());
         dinv_p = dinv.p;
     }
+    const bool flexible = (precond == LT_PC_LEARNED);
+    if (flexible) {
+        LT_TRY(learned_setup());
+        LT_TRY(nfall.zero());
+        usenet_host.assign(S, 1);
+        LT_TRY(usenet.upload(usenet_host.data(), S));
+    }
+    // plain-CG fallback of the learned preconditioner: a sample whose
+    // residual norm has not fallen 10x over the last kWindow iterations
+    // (about plain CG's own rate on these problems) restarts as plain CG
+    constexpr int kWindow = 100;
+    std::vector<std::vector<double>> rr_hist(S);
+    std::vector<double> hrr_chk(S);
+    std::vector<int> hflag(S);
+    DevBuf<int> dflag;
+    if (flexible) LT_TRY(dflag.alloc(S));
     std::vector<int> ones(S, 1), zeros(S, 0);
     LT_TRY(act.upload(ones.data(), S));
     LT_TRY(iters.upload(zeros.data(), S));
     LT_TRY(x.zero());
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, b.p, r.p);
-    hipLaunchKernelGGL(k_precond, dim3(gb), dim3(256), 0, stream(), ntot, dinv_p, r.p, z.p);
+    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(precondition(precond, dinv_p));
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, z.p, p.p);
     LT_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, b.p, b.p, bb.p);
@@ -405,21 +666,50 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, p.p, Ap.p, pAp.p);
         hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
-        hipLaunchKernelGGL(k_precond, dim3(gb), dim3(256), 0, stream(), ntot, dinv_p, r.p, z.p);
-        hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz_new.p);
         hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
-        hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, rr.p, bb.p, rtol2,
-                           beta.p, iters.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(precondition(precond, dinv_p));
+        hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz_new.p);
+        if (flexible) {
+            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, z.p, Ap.p, zap.p);
+            hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
+                               alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
+        } else {
+            hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, rr.p, bb.p,
+                               rtol2, beta.p, iters.p);
+        }
         hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, z.p, p.p);
         LT_CHECK_LAUNCH();
         ++outer;
         if (outer % check_every == 0 || it == maxiter - 1) {
             LT_TRY(act.download(hact.data(), S));
+            if (flexible) LT_TRY(rr.download(hrr_chk.data(), S));
             LT_TRY(sync());
             nact = 0;
             for (int s = 0; s < S; ++s)
                 if (hact[s]) alist[nact++] = s;
             if (nact > 0) LT_TRY(dlist.upload(alist.data(), nact));
+            if (flexible) {
+                const size_t back = kWindow / check_every;
+                int nsw = 0;
+                for (int s = 0; s < S; ++s) {
+                    hflag[s] = 0;
+                    rr_hist[s].push_back(hrr_chk[s]);
+                    const size_t h = rr_hist[s].size();
+                    if (hact[s] && usenet_host[s] && h > back && !(hrr_chk[s] < 1e-2 * rr_hist[s][h - 1 - back])) {
+                        usenet_host[s] = 0;
+                        hflag[s] = 1;
+                        ++nsw;
+                    }
+                }
+                if (nsw) {
+                    LT_TRY(usenet.upload(usenet_host.data(), S));
+                    LT_TRY(dflag.upload(hflag.data(), S));
+                    hipLaunchKernelGGL(k_restart, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, dflag.p, r.p, p.p);
+                    hipLaunchKernelGGL(k_restart_rz, dim3(gs), dim3(64), 0, stream(), S, dflag.p, rr.p, rz.p);
+                    LT_CHECK_LAUNCH();
+                }
+            }
         }
     }
     // FP64 true residual ||b - A x|| / ||b||
@@ -434,6 +724,8 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(bb.download(hbb.data(), S));
     LT_TRY(iters.download(hit.data(), S));
     if (xout) LT_TRY(x.download(xout, ntot));
+    nfall_host.assign(S, 0);
+    if (flexible) LT_TRY(nfall.download(nfall_host.data(), S));
     LT_TRY(sync());
     for (int s = 0; s < S; ++s) {
         if (it_out) it_out[s] = hit[s];
@@ -601,6 +893,12 @@ lt_status lt_solver_rhs(lt_solver h, double *rhs) {
     return sync();
 }
 
+lt_status lt_solver_set_rhs(lt_solver h, const double *rhs) {
+    if (!h || !rhs) return LT_ERR_ARG;
+    LT_TRY(h->s.b.upload(rhs, h->s.ntot));
+    return sync();
+}
+
 lt_status lt_solver_apply(lt_solver h, const double *x, double *y) {
     if (!h || !x || !y) return LT_ERR_ARG;
     Solver &S = h->s;
@@ -610,6 +908,42 @@ lt_status lt_solver_apply(lt_solver h, const double *x, double *y) {
     LT_TRY(S.apply(dx.p, dy.p, S.all_active.p, S.S));
     LT_TRY(dy.download(y, S.ntot));
     return sync();
+}
+
+lt_status lt_solver_set_pcnet(lt_solver h, lt_pcnet net) {
+    if (!h) return LT_ERR_ARG;
+    h->s.net = pcnet_of(net);
+    return LT_OK;
+}
+
+lt_status lt_solver_train_pcnet(lt_solver h, const double *residuals, int nres, int epochs, float lr, float lr_final,
+                                unsigned seed, double *loss_hist) {
+    if (!h || !residuals || nres <= 0 || epochs < 0) {
+        set_error("lt_solver_train_pcnet: bad arguments");
+        return LT_ERR_ARG;
+    }
+    return h->s.train_pcnet(residuals, nres, epochs, lr, lr_final, seed, loss_hist);
+}
+
+lt_status lt_solver_diag(lt_solver h, double *diag) {
+    if (!h || !diag) return LT_ERR_ARG;
+    Solver &S = h->s;
+    if (!S.dinv.p) LT_TRY(S.jacobi_setup());
+    std::vector<double> d(S.ntot);
+    LT_TRY(S.dinv.download(d.data(), S.ntot));
+    LT_TRY(sync());
+    for (long i = 0; i < S.ntot; ++i) diag[i] = 1.0 / d[i];
+    return LT_OK;
+}
+
+lt_status lt_solver_pc_fallbacks(lt_solver h, int *counts, int *plain) {
+    if (!h || !counts) return LT_ERR_ARG;
+    Solver &S = h->s;
+    for (int s = 0; s < S.S; ++s) {
+        counts[s] = s < (int)S.nfall_host.size() ? S.nfall_host[s] : 0;
+        if (plain) plain[s] = s < (int)S.usenet_host.size() ? !S.usenet_host[s] : 0;
+    }
+    return LT_OK;
 }
 
 lt_status lt_solver_destroy(lt_solver h) {

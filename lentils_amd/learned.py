@@ -1,0 +1,126 @@
+"""Learned preconditioner for the batched source inversion.
+
+A small convolutional network (BF16 operands on MFMA, FP32 accumulate; the
+kernels are in csrc/pcnet.hip) maps a CG residual r of one lens sample, with
+that sample's diagonal of A as a second input channel, to a search direction
+z ~ A^{-1} r.  It is trained on the GPU on (right-hand side, solution) pairs
+that the solver's own operator generates: b = A x for random Gaussian-blob
+sources x (the source family of the workload), over many random lens models,
+and is used inside flexible CG (BatchedSourceInversion.solve(precond=
+'learned')), with the Jacobi direction as the guard whenever the network's
+direction is not a descent direction.  The reference has no counterpart (it
+preconditions with a sparse LU, lentils/tests/test_all.py:63-72).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import backend as B
+
+
+class PCNet:
+    """The network on an H x W source grid, for batches of up to `batch`."""
+
+    def __init__(self, H, W, batch=64, params=None, seed=0):
+        self.H, self.W, self.batch = int(H), int(W), int(batch)
+        h = C.c_void_p()
+        if params is not None:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            if p.size != self.param_count():
+                raise ValueError(f'expected {self.param_count()} parameters, got {p.size}')
+            B.check(B.lib().lt_pcnet_create(self.H, self.W, self.batch, B.ptr(p), 0, C.byref(h)))
+        else:
+            B.check(B.lib().lt_pcnet_create(self.H, self.W, self.batch, None, int(seed), C.byref(h)))
+        self._handle = h
+
+    @staticmethod
+    def param_count():
+        return int(B.lib().lt_pcnet_param_count())
+
+    @property
+    def params(self):
+        p = np.zeros(self.param_count(), dtype=np.float32)
+        B.check(B.lib().lt_pcnet_get_params(self._handle, B.ptr(p)))
+        return p
+
+    @params.setter
+    def params(self, value):
+        p = np.ascontiguousarray(value, dtype=np.float32)
+        B.check(B.lib().lt_pcnet_set_params(self._handle, B.ptr(p)))
+
+    def save(self, path):
+        np.save(path, self.params)
+
+    @classmethod
+    def load(cls, path, H, W, batch=64):
+        return cls(H, W, batch, params=np.load(path, allow_pickle=False))
+
+    def forward(self, x0):
+        x = np.ascontiguousarray(x0, dtype=np.float32)
+        n = x.shape[0]
+        y = np.zeros((n, self.H, self.W), dtype=np.float32)
+        for k in range(0, n, self.batch):
+            m = min(self.batch, n - k)
+            B.check(B.lib().lt_pcnet_forward_host(self._handle, B.ptr(x[k:k + m]), m, B.ptr(y[k:k + m])))
+        return y
+
+    def train(self, inputs, targets, epochs=10, batch=None, lr=2e-3, lr_final=1e-4, seed=0):
+        """Adam on the mean relative squared error; returns the loss per epoch."""
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        hist = np.zeros(epochs)
+        B.check(B.lib().lt_pcnet_train(self._handle, B.ptr(x), B.ptr(t), x.shape[0], int(epochs),
+                                       int(batch or self.batch), float(lr), float(lr_final), int(seed), B.ptr(hist)))
+        return hist
+
+    def evaluate(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        out = C.c_double(0)
+        B.check(B.lib().lt_pcnet_eval(self._handle, B.ptr(x), B.ptr(t), x.shape[0], C.byref(out)))
+        return out.value
+
+    def __del__(self):
+        h = getattr(self, '_handle', None)
+        if h is not None and B._lib is not None:
+            B._lib.lt_pcnet_destroy(h)
+            self._handle = None
+
+
+def blob_sources(src_space, n, rng, nblobs=4, spread=0.3, width=(0.05, 0.2), amp=(0.5, 1.0)):
+    """n random sources, each a sum of Gaussian blobs evaluated at the source
+    grid's pixel positions (the workload's source family)."""
+    p = src_space.points
+    out = np.zeros((n,) + p.shape[:2])
+    for k in range(n):
+        for _ in range(nblobs):
+            c = rng.uniform(-spread, spread, 2)
+            s = rng.uniform(*width)
+            a = rng.uniform(*amp)
+            out[k] += a * np.exp(-np.sum((p - c) ** 2, axis=-1) / (2 * s * s))
+    return out
+
+
+def network_inputs(b, diag):
+    """(b / rms(b), diag / mean(diag)) as the solver forms them; b, diag:
+    arrays of one sample's source grid.  Returns (x0 [2][H][W], rms, mean)."""
+    rms = np.sqrt(np.mean(b * b))
+    mean = np.mean(diag)
+    x0 = np.stack([b / rms if rms > 0 else 0 * b, diag / mean])
+    return x0, rms, mean
+
+
+def training_pairs(inversion, sources):
+    """Pairs for every sample of `inversion` (a BatchedSourceInversion) and
+    every draw in `sources` (list of per-sample source lists): b = A x.
+    Returns (inputs [n][2][H][W], targets [n][H][W]) in float32, targets
+    scaled as the network's output: x * mean(diag) / rms(b)."""
+    diag = inversion.diag()
+    ins, tgs = [], []
+    for xs in sources:
+        bs = inversion.apply(xs)
+        for s, (x, b) in enumerate(zip(xs, bs)):
+            x0, rms, mean = network_inputs(np.asarray(b).reshape(x.shape), diag[s].reshape(x.shape))
+            ins.append(x0.astype(np.float32))
+            tgs.append((x * mean / rms).astype(np.float32))
+    return np.stack(ins), np.stack(tgs)

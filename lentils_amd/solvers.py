@@ -169,15 +169,69 @@ class BatchedSourceInversion:
         B.check(B.lib().lt_solver_rhs(self._handle, B.ptr(out)))
         return self._split(out)
 
+    def set_rhs(self, bs):
+        """Replace the right-hand sides (per-sample arrays)."""
+        b = np.ascontiguousarray(np.concatenate([np.asarray(v).reshape(-1) for v in bs]), dtype=np.float64)
+        B.check(B.lib().lt_solver_set_rhs(self._handle, B.ptr(b)))
+
     def apply(self, xs):
         x = np.ascontiguousarray(np.concatenate([np.asarray(v).reshape(-1) for v in xs]), dtype=np.float64)
         y = np.zeros_like(x)
         B.check(B.lib().lt_solver_apply(self._handle, B.ptr(x), B.ptr(y)))
         return self._split(y)
 
+    def diag(self):
+        """Jacobi estimate of diag(A) per sample."""
+        d = np.zeros(self._off[-1])
+        B.check(B.lib().lt_solver_diag(self._handle, B.ptr(d)))
+        return self._split(d)
+
+    def set_preconditioner(self, net):
+        """Network (learned.PCNet) for solve(precond='learned')."""
+        self._net = net
+        B.check(B.lib().lt_solver_set_pcnet(self._handle, net._handle if net is not None else None))
+
+    def train_preconditioner(self, net, residuals, epochs=1, lr=1e-3, lr_final=1e-4, seed=0):
+        """Train `net` with this inversion's operators on residual vectors
+        (each a list of per-sample arrays, or a flat array of all samples);
+        epochs=0 only evaluates.  Returns the mean energy loss per epoch."""
+        self.set_preconditioner(net)
+        R = np.ascontiguousarray(np.stack([np.concatenate([np.asarray(v).reshape(-1) for v in r])
+                                           if isinstance(r, (list, tuple)) else np.asarray(r).reshape(-1)
+                                           for r in residuals]), dtype=np.float64)
+        hist = np.zeros(max(epochs, 1))
+        B.check(B.lib().lt_solver_train_pcnet(self._handle, B.ptr(R), R.shape[0], int(epochs), float(lr),
+                                              float(lr_final), int(seed), B.ptr(hist)))
+        return hist
+
+    def residuals(self, iterations):
+        """True residuals b - A x_k of plain CG stopped after each k in
+        `iterations` (k = 0 gives b), as flat arrays of all samples."""
+        b = np.concatenate([v.reshape(-1) for v in self.rhs()])
+        out = []
+        for k in iterations:
+            if k == 0:
+                out.append(b.copy())
+                continue
+            xs, _, _ = self.solve(rtol=1e-12, maxiter=int(k))
+            ax = np.concatenate([v.reshape(-1) for v in self.apply(xs)])
+            out.append(b - ax)
+        return out
+
+    def fallbacks(self):
+        """Per sample, for the last learned solve: (iterations that took the
+        Jacobi direction because the network's was not a descent direction,
+        whether the sample fell back to plain CG)."""
+        c = np.zeros(len(self._sizes), dtype=np.int32)
+        pl = np.zeros(len(self._sizes), dtype=np.int32)
+        B.check(B.lib().lt_solver_pc_fallbacks(self._handle, B.ptr(c), B.ptr(pl)))
+        return c, pl.astype(bool)
+
     def solve(self, rtol=1e-6, maxiter=5000, precond='none'):
-        """Returns (sources, iterations, true relative residuals)."""
-        pc = {'none': B.LT_PC_NONE, 'jacobi': B.LT_PC_JACOBI}[precond]
+        """Returns (sources, iterations, true relative residuals).
+        precond: 'none' (plain CG), 'jacobi', or 'learned' (flexible CG with
+        the network given to set_preconditioner)."""
+        pc = {'none': B.LT_PC_NONE, 'jacobi': B.LT_PC_JACOBI, 'learned': B.LT_PC_LEARNED}[precond]
         n = len(self._sizes)
         x = np.zeros(self._off[-1])
         its = np.zeros(n, dtype=np.int32)

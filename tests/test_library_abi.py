@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared_functions():
     text = open(os.path.join(ROOT, 'include', 'lentils_hip.h')).read()
     text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
-    names = re.findall(r'^\s*(?:void|lt_status|const char \*)\s*\*?\s*(\w+)\s*\(', text, flags=re.M)
+    names = re.findall(r'^\s*(?:void|lt_status|long|const char \*)\s*\*?\s*(\w+)\s*\(', text, flags=re.M)
     return sorted(set(names))
 
 
