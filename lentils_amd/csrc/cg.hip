@@ -304,7 +304,10 @@ struct Solver {
 
     lt_status create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data,
                      const double *dirty_in, int ndirty_in, double lc, double lt, int op_mode);
-    lt_status apply(const double *xin, double *yout, const int *active_list, int nactive);
+    // y = A x for the samples in active_list; with actflags (per-sample 0/1)
+    // the sparse products skip the rows of inactive samples as well
+    lt_status apply(const double *xin, double *yout, const int *active_list, int nactive,
+                    const int *actflags = nullptr);
     lt_status solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out);
     lt_status jacobi_setup();
     lt_status learned_setup();
@@ -438,16 +441,16 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     return sync();
 }
 
-lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact) {
-    LT_TRY(csr_spmv(lb->L, xin, img1.p, 1.0, 0.0, 0));
+lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact, const int *af) {
+    LT_TRY(csr_spmv(lb->L, xin, img1.p, 1.0, 0.0, 0, af, npix, nullptr));
     if (mode != LT_OP_BCB)
         LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, alist, nact, 1.0, false, use_stencil ? &stencil : nullptr));
     else
         LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, alist, nact, 1.0, false));
-    LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0));
+    LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0, af, 0, seg.p));
     if (lam_c != 0.0) {
-        LT_TRY(csr_spmv(C, xin, tmp.p, 1.0, 0.0, 0));
-        LT_TRY(csr_spmv(C, tmp.p, yout, lam_c, 1.0, 0));
+        LT_TRY(csr_spmv(C, xin, tmp.p, 1.0, 0.0, 0, af, 0, seg.p));
+        LT_TRY(csr_spmv(C, tmp.p, yout, lam_c, 1.0, 0, af, 0, seg.p));
     }
     if (lam_t != 0.0) {
         hipLaunchKernelGGL(k_axpby, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, lam_t, xin, 1.0, yout);
@@ -662,7 +665,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     const int check_every = 5;
     int outer = 0;
     for (int it = 0; it < maxiter && nact > 0; ++it) {
-        LT_TRY(apply(p.p, Ap.p, dlist.p, nact));
+        LT_TRY(apply(p.p, Ap.p, dlist.p, nact, act.p));
         hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, p.p, Ap.p, pAp.p);
         hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);

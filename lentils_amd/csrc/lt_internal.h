@@ -135,7 +135,10 @@ struct DevCSR {
 // analyse row lengths once (lane group size, long-row chunk plan)
 lt_status csr_prepare(DevCSR &A);
 // y = alpha A x + beta y (FP64); lanes_per_row <= 0 uses the prepared choice
-lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes_per_row);
+// act (device, per sample): skip the rows of samples with act == 0; a row's
+// sample is row / rows_per_sample, or rowseg[row] when rowseg is given
+lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes_per_row,
+                   const int *act = nullptr, long rows_per_sample = 0, const int32_t *rowseg = nullptr);
 int csr_pick_lanes(const DevCSR &A);
 // B = A^T with column order inside each row ascending (deterministic)
 lt_status csr_transpose(const DevCSR &A, DevCSR &B);

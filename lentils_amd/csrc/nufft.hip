@@ -550,10 +550,12 @@ __global__ void __launch_bounds__(kNThreads) k_grid_reduce_b(Geo G, const int32_
 }
 
 // batched zero padding (x apod when apod != nullptr)
+// slot y of the padded batch <- sample active[y] (or y) of the images
 template <class T>
 __global__ void k_pad_b(int nx, int ny, int padx, int pady, int nxp, int nyp, const double *__restrict__ img,
-                        const double *__restrict__ apod, T *__restrict__ out) {
-    const int smp = blockIdx.y;
+                        const double *__restrict__ apod, T *__restrict__ out, const int *__restrict__ active) {
+    const int slot = blockIdx.y;
+    const int smp = active ? active[slot] : slot;
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)nxp * nyp) return;
     const int i = (int)(g / nyp) - padx, j = (int)(g % nyp) - pady;
@@ -563,18 +565,19 @@ __global__ void k_pad_b(int nx, int ny, int padx, int pady, int nxp, int nyp, co
         const double x = img[(long)smp * nx * ny + k];
         v = (T)(apod ? x * apod[k] : x);
     }
-    out[(long)smp * nxp * nyp + g] = v;
+    out[(long)slot * nxp * nyp + g] = v;
 }
 
 template <class T>
 __global__ void k_crop_b(int nx, int ny, int padx, int pady, int nxp, int nyp, const T *__restrict__ in,
                          const double *__restrict__ apod, double alpha, int accumulate, double *__restrict__ img,
                          const int *__restrict__ active) {
-    const int smp = active ? active[blockIdx.y] : (int)blockIdx.y;
+    const int slot = blockIdx.y;
+    const int smp = active ? active[slot] : slot;
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)nx * ny) return;
     const int i = (int)(g / ny), j = (int)(g % ny);
-    double x = (double)in[(long)smp * nxp * nyp + (long)(i + padx) * nyp + (j + pady)];
+    double x = (double)in[(long)slot * nxp * nyp + (long)(i + padx) * nyp + (j + pady)];
     double v = alpha * (apod ? x * apod[g] : x);
     double *o = img + (long)smp * nx * ny + g;
     *o = accumulate ? *o + v : v;
@@ -1053,10 +1056,6 @@ lt_status launch_grid(NufftPlan &P, const void *vis, const double *wts, void *gr
 }  // namespace
 
 NufftPlan::~NufftPlan() {
-    if (fwdB) rocfft_plan_destroy(fwdB);
-    if (invB) rocfft_plan_destroy(invB);
-    if (fwdB_info) rocfft_execution_info_destroy(fwdB_info);
-    if (invB_info) rocfft_execution_info_destroy(invB_info);
     if (fwd) rocfft_plan_destroy(fwd);
     if (inv) rocfft_plan_destroy(inv);
     if (fwd_info) rocfft_execution_info_destroy(fwd_info);
@@ -1340,41 +1339,55 @@ lt_status NufftPlan::adjoint(const void *vis, const double *w, double *img, doub
 }
 
 
+NufftPlan::BatchFFT::~BatchFFT() {
+    if (fwd) rocfft_plan_destroy(fwd);
+    if (inv) rocfft_plan_destroy(inv);
+    if (fi) rocfft_execution_info_destroy(fi);
+    if (ii) rocfft_execution_info_destroy(ii);
+}
+
+lt_status NufftPlan::batch_fft(int n, BatchFFT *&out) {
+    for (auto &b : bfft)
+        if (b->n == n) {
+            out = b.get();
+            return LT_OK;
+        }
+    auto b = std::make_unique<BatchFFT>();
+    b->n = n;
+    const size_t len[2] = {(size_t)nyp, (size_t)nxp};
+    const rocfft_precision rp = prec == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
+    LT_TRY(fft_check(rocfft_plan_create(&b->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, rp,
+                                        2, len, (size_t)n, nullptr),
+                     "batched plan (r2c)"));
+    LT_TRY(fft_check(rocfft_plan_create(&b->inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, rp,
+                                        2, len, (size_t)n, nullptr),
+                     "batched plan (c2r)"));
+    size_t wf = 0, wi = 0;
+    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(b->fwd, &wf), "work size"));
+    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(b->inv, &wi), "work size"));
+    LT_TRY(fft_check(rocfft_execution_info_create(&b->fi), "info"));
+    LT_TRY(fft_check(rocfft_execution_info_create(&b->ii), "info"));
+    if (wf) {
+        LT_TRY(b->fw.alloc(wf));
+        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(b->fi, b->fw.p, wf), "work buffer"));
+    }
+    if (wi) {
+        LT_TRY(b->iw.alloc(wi));
+        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(b->ii, b->iw.p, wi), "work buffer"));
+    }
+    LT_TRY(fft_check(rocfft_execution_info_set_stream(b->fi, stream()), "stream"));
+    LT_TRY(fft_check(rocfft_execution_info_set_stream(b->ii, stream()), "stream"));
+    out = b.get();
+    bfft.push_back(std::move(b));
+    return LT_OK;
+}
+
 lt_status NufftPlan::ensure_batch(int nb) {
     if (nb <= batch) return LT_OK;
     if (!has_image) {
         set_error("NUFFT plan has no image space");
         return LT_ERR_ARG;
     }
-    if (fwdB) rocfft_plan_destroy(fwdB);
-    if (invB) rocfft_plan_destroy(invB);
-    if (fwdB_info) rocfft_execution_info_destroy(fwdB_info);
-    if (invB_info) rocfft_execution_info_destroy(invB_info);
-    fwdB = invB = nullptr;
-    fwdB_info = invB_info = nullptr;
-    const size_t len[2] = {(size_t)nyp, (size_t)nxp};
-    const rocfft_precision rp = prec == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
-    LT_TRY(fft_check(rocfft_plan_create(&fwdB, rocfft_placement_notinplace, rocfft_transform_type_real_forward, rp, 2,
-                                        len, (size_t)nb, nullptr),
-                     "batched plan (r2c)"));
-    LT_TRY(fft_check(rocfft_plan_create(&invB, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, rp, 2,
-                                        len, (size_t)nb, nullptr),
-                     "batched plan (c2r)"));
-    size_t wf = 0, wi = 0;
-    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(fwdB, &wf), "work size"));
-    LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(invB, &wi), "work size"));
-    LT_TRY(fft_check(rocfft_execution_info_create(&fwdB_info), "info"));
-    LT_TRY(fft_check(rocfft_execution_info_create(&invB_info), "info"));
-    if (wf) {
-        LT_TRY(fwdB_work.alloc(wf));
-        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(fwdB_info, fwdB_work.p, wf), "work buffer"));
-    }
-    if (wi) {
-        LT_TRY(invB_work.alloc(wi));
-        LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(invB_info, invB_work.p, wi), "work buffer"));
-    }
-    LT_TRY(fft_check(rocfft_execution_info_set_stream(fwdB_info, stream()), "stream"));
-    LT_TRY(fft_check(rocfft_execution_info_set_stream(invB_info, stream()), "stream"));
     LT_TRY(realB.alloc((long)nb * nxp * nyp * tsize()));
     LT_TRY(specB.alloc((long)nb * nu * half_nv * 2 * tsize()));
     LT_TRY(specB2.alloc((long)nb * nu * half_nv * 2 * tsize()));
@@ -1386,25 +1399,36 @@ lt_status NufftPlan::ensure_batch(int nb) {
 template <class T>
 static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, const double *w, const int *active,
                              int nactive, double alpha, bool acc, const double *spec_mul, NormalStencil *st) {
-    const int nb = P.batch;
+    // active samples are packed into slots 0..nactive-1 (image side reads and
+    // writes through `active`, the spectral side works on slots)
+    if (nactive <= 0) return LT_OK;
+    if (nactive > P.batch) {
+        set_error("normal operator: %d samples exceed the batch %d", nactive, P.batch);
+        return LT_ERR_ARG;
+    }
+    const int nrun = std::min(P.batch, (nactive + NufftPlan::kBatchStep - 1) / NufftPlan::kBatchStep *
+                                           NufftPlan::kBatchStep);
+    NufftPlan::BatchFFT *bf = nullptr;
+    LT_TRY(P.batch_fft(nrun, bf));
     const long nreal = (long)P.nxp * P.nyp, nspec = (long)P.nu * P.half_nv, npix = (long)P.nx * P.ny;
     const bool bcb = spec_mul != nullptr;
-    hipLaunchKernelGGL((k_pad_b<T>), dim3(blocks_for(nreal, 256), nb), dim3(256), 0, stream(), P.nx, P.ny, P.padx,
-                       P.pady, P.nxp, P.nyp, imgs, bcb ? nullptr : P.apod.p, (T *)P.realB.p);
+    hipLaunchKernelGGL((k_pad_b<T>), dim3(blocks_for(nreal, 256), nactive), dim3(256), 0, stream(), P.nx, P.ny,
+                       P.padx, P.pady, P.nxp, P.nyp, imgs, bcb ? nullptr : P.apod.p, (T *)P.realB.p, active);
     LT_CHECK_LAUNCH();
     void *in[1] = {P.realB.p};
     void *o1[1] = {P.specB.p};
-    LT_TRY(fft_check(rocfft_execute(P.fwdB, in, o1, P.fwdB_info), "execute batched r2c"));
+    LT_TRY(fft_check(rocfft_execute(bf->fwd, in, o1, bf->fi), "execute batched r2c"));
     Cplx<T> *spec_out = (Cplx<T> *)P.specB2.p;
-    if (nactive > 0) {
+    const int *slots = nullptr;  // spectral kernels index slots directly
+    {
         Geo G = make_geo(P);
         if (bcb) {
             spec_out = (Cplx<T> *)P.specB.p;
             hipLaunchKernelGGL((k_spec_mul_b<T>), dim3(blocks_for(nspec, 256), nactive), dim3(256), 0, stream(), nspec,
-                               spec_mul, spec_out, nspec, active);
+                               spec_mul, spec_out, nspec, slots);
             LT_CHECK_LAUNCH();
         } else if (st) {
-            LT_TRY(st->apply(P, P.specB.p, nspec, spec_out, active, nactive));
+            LT_TRY(st->apply(P, P.specB.p, nspec, spec_out, slots, nactive));
         } else {
             const long pstride = std::max<long>(P.nchunks, 1) * P.WU * P.WV;
             if (P.nchunks > 0) {
@@ -1414,7 +1438,7 @@ static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, cons
     case WW:                                                                                                       \
         hipLaunchKernelGGL((k_normal<T, WW>), dim3((unsigned)P.nchunks, nactive), dim3(kNThreads), sh, stream(), G, \
                            taps_of<T>(P), (const Cplx<T> *)P.specB.p, nspec, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, \
-                           fr, w, (Cplx<T> *)P.partialB.p, pstride, active);                                       \
+                           fr, w, (Cplx<T> *)P.partialB.p, pstride, slots);                                        \
         break;
                 switch (P.W) {
                     LT_NORMAL_CASE(1)
@@ -1434,27 +1458,24 @@ static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, cons
             }
             hipLaunchKernelGGL((k_grid_reduce_b<T>), dim3(blocks_for(nspec, kNThreads), nactive), dim3(kNThreads), 0,
                                stream(), G, P.tile_ck0.p, P.tile_ck1.p, (const Cplx<T> *)P.partialB.p, pstride,
-                               spec_out, nspec, active);
+                               spec_out, nspec, slots);
             LT_CHECK_LAUNCH();
         }
         const int cols[2] = {0, P.nv / 2};
         const int ncol = (P.nv % 2 == 0) ? 2 : 1;
         for (int k = 0; k < ncol; ++k) {
             hipLaunchKernelGGL((k_hermitian_cols_b<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0,
-                               stream(), P.nu, P.half_nv, cols[k], spec_out, nspec, active);
+                               stream(), P.nu, P.half_nv, cols[k], spec_out, nspec, slots);
             LT_CHECK_LAUNCH();
         }
     }
     void *i2[1] = {(void *)spec_out};
     void *o2[1] = {P.realB.p};
-    LT_TRY(fft_check(rocfft_execute(P.invB, i2, o2, P.invB_info), "execute batched c2r"));
-    if (nactive > 0) {
-        hipLaunchKernelGGL((k_crop_b<T>), dim3(blocks_for(npix, 256), nactive), dim3(256), 0, stream(), P.nx, P.ny,
-                           P.padx, P.pady, P.nxp, P.nyp, (const T *)P.realB.p, bcb ? nullptr : P.apod.p, alpha,
-                           acc ? 1 : 0, out, active);
-        LT_CHECK_LAUNCH();
-    }
-    (void)nb;
+    LT_TRY(fft_check(rocfft_execute(bf->inv, i2, o2, bf->ii), "execute batched c2r"));
+    hipLaunchKernelGGL((k_crop_b<T>), dim3(blocks_for(npix, 256), nactive), dim3(256), 0, stream(), P.nx, P.ny,
+                       P.padx, P.pady, P.nxp, P.nyp, (const T *)P.realB.p, bcb ? nullptr : P.apod.p, alpha,
+                       acc ? 1 : 0, out, active);
+    LT_CHECK_LAUNCH();
     return LT_OK;
 }
 

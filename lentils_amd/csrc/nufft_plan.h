@@ -4,6 +4,9 @@
 
 #include <rocfft/rocfft.h>
 
+#include <memory>
+#include <vector>
+
 #include "lt_internal.h"
 
 namespace lt {
@@ -109,11 +112,21 @@ struct NufftPlan {
     rocfft_execution_info fwd_info = nullptr, inv_info = nullptr;
     DevBuf<uint8_t> fwd_work, inv_work;
 
-    // batched images (normal operator D^T C D inside the solver)
+    // batched images (normal operator D^T C D inside the solver): buffers
+    // for `batch` images; rocFFT plans per transform count, so that a call
+    // transforms only its active samples (rounded up to kBatchStep)
+    static constexpr int kBatchStep = 8;
+    struct BatchFFT {
+        int n = 0;
+        rocfft_plan fwd = nullptr, inv = nullptr;
+        rocfft_execution_info fi = nullptr, ii = nullptr;
+        DevBuf<uint8_t> fw, iw;
+        ~BatchFFT();
+    };
     int batch = 0;
-    rocfft_plan fwdB = nullptr, invB = nullptr;
-    rocfft_execution_info fwdB_info = nullptr, invB_info = nullptr;
-    DevBuf<uint8_t> fwdB_work, invB_work, realB, specB, specB2, partialB;
+    std::vector<std::unique_ptr<BatchFFT>> bfft;
+    DevBuf<uint8_t> realB, specB, specB2, partialB;
+    lt_status batch_fft(int n, BatchFFT *&out);
 
     ~NufftPlan();
     size_t tsize() const { return prec == LT_F64 ? sizeof(double) : sizeof(float); }

@@ -15,15 +15,29 @@ namespace {
 constexpr int kLongRow = 2048;  // rows longer than this go to the chunked path
 constexpr int kChunk = 2048;
 
+// Rows of a block-diagonal batch belong to samples: sample = row / rps, or
+// rowseg[row] when given.  With act set, rows of inactive samples are
+// skipped (their outputs are left as they are).
+struct RowFilter {
+    const int *act;
+    long rps;
+    const int32_t *rowseg;
+    __device__ bool skip(long row) const {
+        if (!act) return false;
+        const long s = rowseg ? (long)rowseg[row] : row / rps;
+        return act[s] == 0;
+    }
+};
+
 template <int LANES>
 __global__ void __launch_bounds__(256) k_spmv(long nrows, const int32_t *__restrict__ rp,
                                                const int32_t *__restrict__ ci, const double *__restrict__ va,
                                                const double *__restrict__ x, double *__restrict__ y, double alpha,
-                                               double beta, int skip_long) {
+                                               double beta, int skip_long, RowFilter F) {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long row = gid / LANES;
     const int lane = (int)(gid % LANES);
-    if (row >= nrows) return;  // whole lane groups leave together
+    if (row >= nrows || F.skip(row)) return;  // whole lane groups leave together
     double s = 0.0;
     const int b = rp[row], e = rp[row + 1];
     if (skip_long && e - b > kLongRow) return;  // summed by k_spmv_chunks / k_spmv_long
@@ -38,9 +52,11 @@ __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t 
                                                       const int32_t *__restrict__ chunk_end_row,
                                                       const int32_t *__restrict__ rp,
                                                       const int32_t *__restrict__ ci, const double *__restrict__ va,
-                                                      const double *__restrict__ x, double *__restrict__ out) {
+                                                      const double *__restrict__ x, double *__restrict__ out,
+                                                      RowFilter F) {
     __shared__ double sh[4];
     const long c = blockIdx.x;
+    if (F.skip(chunk_end_row[c])) return;
     const int b = chunk_beg[c];
     const int e = min(b + kChunk, rp[chunk_end_row[c] + 1]);
     double s = 0.0;
@@ -54,9 +70,10 @@ __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t 
 
 // long rows: sum their chunks in order
 __global__ void k_spmv_long(long nlong, const int32_t *__restrict__ rows, const int32_t *__restrict__ chunk0,
-                            const double *__restrict__ part, double *__restrict__ y, double alpha, double beta) {
+                            const double *__restrict__ part, double *__restrict__ y, double alpha, double beta,
+                            RowFilter F) {
     long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlong) return;
+    if (l >= nlong || F.skip(rows[l])) return;
     double s = 0.0;
     for (int c = chunk0[l]; c < chunk0[l + 1]; ++c) s += part[c];
     const int row = rows[l];
@@ -175,15 +192,18 @@ lt_status csr_prepare(DevCSR &A) {
     return sync();
 }
 
-lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes) {
+lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes,
+                   const int *act, long rows_per_sample, const int32_t *rowseg) {
     if (A.nrows == 0) return LT_OK;
+    const RowFilter F{act, rows_per_sample > 0 ? rows_per_sample : 1, rowseg};
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
     const int skip = (A.prepared && A.nlong > 0) ? 1 : 0;
     const long threads = A.nrows * lanes;
     const dim3 g(blocks_for(threads, 256)), b(256);
 #define LT_SPMV_CASE(L)                                                                                              \
     case L:                                                                                                          \
-        hipLaunchKernelGGL(k_spmv<L>, g, b, 0, stream(), A.nrows, A.rp.p, A.ci.p, A.va.p, x, y, alpha, beta_y, skip); \
+        hipLaunchKernelGGL(k_spmv<L>, g, b, 0, stream(), A.nrows, A.rp.p, A.ci.p, A.va.p, x, y, alpha, beta_y, skip, \
+                           F);                                                                                       \
         break;
     switch (lanes) {
         LT_SPMV_CASE(1)
@@ -201,10 +221,10 @@ lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, do
     LT_CHECK_LAUNCH();
     if (skip) {
         hipLaunchKernelGGL(k_spmv_chunks, dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk, A.chunk_beg.p,
-                           A.long_rows_of_chunk.p, A.rp.p, A.ci.p, A.va.p, x, A.chunk_sum.p);
+                           A.long_rows_of_chunk.p, A.rp.p, A.ci.p, A.va.p, x, A.chunk_sum.p, F);
         LT_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_spmv_long, dim3(blocks_for(A.nlong, 256)), dim3(256), 0, stream(), A.nlong,
-                           A.long_rows.p, A.long_chunk0.p, A.chunk_sum.p, y, alpha, beta_y);
+                           A.long_rows.p, A.long_chunk0.p, A.chunk_sum.p, y, alpha, beta_y, F);
         LT_CHECK_LAUNCH();
     }
     return LT_OK;
