@@ -112,7 +112,7 @@ class Workload:
                 self.dirty.append(self.plan.adjoint(self.weights * d))
         del truth
 
-    def step(self, rtol=1e-6, maxiter=20000, operator='nufft', precond='none'):
+    def step(self, rtol=1e-6, maxiter=20000, operator='nufft', precond='none', precision='f64'):
         from lentils_amd.solvers import BatchedSourceInversion, LensBatch
         lb = LensBatch(self.im, [self.src] * len(self.lenses), self.lenses, z_src=2.0)
         if self.shared_data:
@@ -121,7 +121,7 @@ class Workload:
         else:
             inv = BatchedSourceInversion(lb, self.plan, weights=self.weights, dirty_images=self.dirty,
                                          lambda_curvature=self.lambda_curv, operator=operator)
-        x, its, res = inv.solve(rtol=rtol, maxiter=maxiter, precond=precond)
+        x, its, res = inv.solve(rtol=rtol, maxiter=maxiter, precond=precond, precision=precision)
         return its, res
 
 
@@ -138,6 +138,44 @@ def degrid_roofline(wl, reps=20):
     ms64 = wl.plan.time(0, reps)
     return {'ms': ms, 'gbs': gbs, 'bytes': bytes_, 'gvis_s': NVIS / (ms * 1e-3) / 1e9,
             'gvis_s_f64': NVIS / (ms64 * 1e-3) / 1e9}
+
+
+def stream_copy_gbs(nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth: a plain device-to-device streaming copy."""
+    from lentils_amd import backend as B
+    ms = C.c_double(0)
+    B.check(B.lib().lt_stream_copy(nbytes, reps, C.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
+
+
+def profile_degrid(reps=10):
+    """Workload for the counter runs: the FP32 degridder of the plain GPU
+    workload's uv set (1e7 visibilities, 1024 x 513 grid, support 3) launched
+    `reps` times, then a 256 MiB streaming copy `reps` times (the calibration
+    kernel for FETCH_SIZE / WRITE_SIZE)."""
+    from lentils_amd import ImageSpace, VisibilitySpace
+    from lentils_amd import backend as B
+    from lentils_amd.solvers import NUFFTPlan
+    B.check(B.lib().lt_init(0))
+    im = ImageSpace(shape=(512, 512), bounds=[(-2.56, 2.56), (-2.56, 2.56)])
+    vs = VisibilitySpace(channels=[1.0], uvcoords=gen_uv(NVIS))
+    p32 = NUFFTPlan(vs, im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f32')
+    ms = p32.time(0, reps)
+    nbytes = 256 << 20
+    gbs, cms = stream_copy_gbs(nbytes, reps)
+    print(json.dumps({'degrid_ms': ms, 'degrid_launches': reps + 1, 'copy_bytes': nbytes, 'copy_ms': cms,
+                      'copy_gbs': gbs, 'copy_launches': reps + 1}), flush=True)
+
+
+def load_traffic():
+    """HBM bytes per degrid launch from the committed counter summary
+    (profiles/degrid_f32_c2_pmc.json), or None."""
+    path = os.path.join(ROOT, 'profiles', 'degrid_f32_c2_pmc.json')
+    try:
+        with open(path) as f:
+            return json.load(f).get('traffic_bytes_per_launch')
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(wl, gpu_iters):
@@ -219,9 +257,14 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--maxiter', type=int, default=20000)
     ap.add_argument('--precond', default='none', choices=['none', 'jacobi'])
+    ap.add_argument('--profile-degrid', action='store_true',
+                    help='only run the degridder and the calibration copy (for rocprofv3 --pmc)')
     ap.add_argument('--shared-data', action='store_true',
                     help='one observation for all samples (most samples then fit the data with the wrong lens)')
     args = ap.parse_args()
+    if args.profile_degrid:
+        profile_degrid()
+        return
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (every solve of a batch is independent; '
               'see DESIGN.md for how batches would shard). Run with --gpus 1.', file=sys.stderr)
@@ -250,8 +293,24 @@ def main():
     res = np.concatenate(all_res)
     ms_per_step = 1e3 * total / args.steps
     value = NLENS * args.steps / total
+    # precision form: the same step with the FP32 operator inside the FP64 CG
+    # (FP64 true residual refreshed every 20 iterations, final FP64 check)
+    pf = None
+    if args.operator == 'nufft':
+        for k in range(args.warmup):  # FP32 rocFFT plans and kernels are first built here
+            wl.step(operator=args.operator, maxiter=args.maxiter, precond=args.precond, precision='f32')
+        t = time.perf_counter()
+        its32, res32 = wl.step(operator=args.operator, maxiter=args.maxiter, precond=args.precond, precision='f32')
+        t32 = time.perf_counter() - t
+        pf = {'value': NLENS / t32, 'unit': 'source solves/s', 'ms_per_step': 1e3 * t32,
+              'iters_median': float(np.median(its32)), 'iters_max': int(its32.max()),
+              'true_rel_resid_max': float(res32.max()), 'all_converged': bool(np.all(res32 <= 1e-6)),
+              'dtype': 'f32 operator (SpMV, rocFFT, stencil), f64 CG vectors and inner products'}
+        log(f'precision form: {t32:.2f}s iters median {np.median(its32):.0f} max res {res32.max():.2e}')
     rl = degrid_roofline(wl)
     log(f'degrid f32 {rl["ms"] * 1e3:.1f} us/launch, {rl["gbs"]:.0f} GB/s, {rl["gvis_s"]:.2f} Gvis/s')
+    copy_gbs, _ = stream_copy_gbs()
+    log(f'streaming copy {copy_gbs:.0f} GB/s')
     out = {
         'metric': 'source solves/s at 1e-6 rel resid (learned-PCG vs plain CG); Gvis/s degrid',
         'value': value, 'unit': 'source solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
@@ -268,9 +327,12 @@ def main():
                    'nvis': NVIS, 'parallelism': 'single GPU'},
         'solver': {'iters_median': float(np.median(its)), 'iters_max': int(its.max()),
                    'true_rel_resid_max': float(res.max()), 'all_converged': bool(np.all(res <= 1e-6))},
+        'precision_form': pf,
         'degrid_gvis_s': rl['gvis_s'], 'degrid_gvis_s_f64': rl['gvis_s_f64'],
         'roofline': {'kernel': 'k_degrid<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS, 'traffic': None},
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS,
+                     'traffic': load_traffic(), 'algorithmic_bytes': rl['bytes'],
+                     'stream_copy_gbs': copy_gbs},
     }
     if not args.no_cpu_baseline:
         t = time.perf_counter()

@@ -171,6 +171,11 @@ lt_status lt_fft_c2r_host(int nx, int ny, int batch, const double *in, double *o
 lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                            const double *x, double *y);
 
+/* No reference counterpart: a plain streaming copy of nbytes (multiple of
+ * 32) between two device buffers, reps times; mean time per launch.  The
+ * achievable-bandwidth ceiling and the calibration of HBM traffic counters. */
+lt_status lt_stream_copy(long nbytes, int reps, double *ms_per_launch);
+
 /* --- device-resident handles ---------------------------------------------- */
 
 typedef struct lt_lensbatch_s *lt_lensbatch;
@@ -211,7 +216,7 @@ lt_status lt_nufft_destroy(lt_nufft h);
 /* No reference counterpart: the per-tap kernel polynomials the gridder
  * evaluates instead of kb_g (nufft.c:11-21).  Tap k of 2W+1 at fractional
  * offset f in [-1/2, 1/2] (distance f + W - k) has value
- * sum_m coeffs[k*(D+1) + m] f^m, D = 16 (LT_F64) or 9 (LT_F32), times
+ * sum_m coeffs[k*(D+1) + m] f^m, D = 16 (LT_F64), 9 (LT_F32, W <= 2) or 7 (LT_F32), times
  * 1/(4W^2) per axis pair; edge taps are zero where |f + W - k| > W.
  * max_err: fit error relative to I0(kb_beta).  Host only. */
 lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, double *max_err);
@@ -244,8 +249,17 @@ lt_status lt_solver_create_dirty(lt_lensbatch lb, lt_nufft nf, const double *wei
  * sample (may be NULL). */
 lt_status lt_solver_solve(lt_solver s, double rtol, int maxiter, int precond, double *x, int *iters, double *resid);
 lt_status lt_solver_rhs(lt_solver s, double *rhs);
+/* Precision form of lt_solver_solve: LT_F32 applies A with FP32 values,
+ * spectra, rocFFT and stencil coefficients (the SpMV index arrays are
+ * shared) inside the FP64 CG (FP64 vectors and inner products), replacing
+ * the recurrence residual by the FP64 true residual b - A x every
+ * refine_every iterations; a sample whose final FP64 true residual misses
+ * rtol restarts from its x.  Needs the stencil operator (LT_OP_NUFFT with
+ * an FP64 plan and kernel_support <= 4).  LT_F64 (default): all FP64. */
+lt_status lt_solver_set_precision(lt_solver s, int precision, int refine_every);
 /* replace the right-hand sides (all samples concatenated, host) */
 lt_status lt_solver_set_rhs(lt_solver s, const double *rhs);
+/* y = A x for all samples, in the precision set by lt_solver_set_precision */
 lt_status lt_solver_apply(lt_solver s, const double *x, double *y);
 lt_status lt_solver_destroy(lt_solver s);
 

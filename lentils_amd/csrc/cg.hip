@@ -242,6 +242,51 @@ __global__ void k_energy_grad(long n, int S, const int32_t *__restrict__ seg, co
     gy[(long)s * HW + pix] = (float)((Az[i] - r[i]) * scale);
 }
 
+// x32 = x / rms_s per sample (the FP32 operator works on unit-scale input)
+__global__ void k_to_f32(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                         const double *__restrict__ xx, const double *__restrict__ a, float *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const double rms = sqrt(xx[s] / (double)(off[s + 1] - off[s]));
+    b[i] = rms > 0.0 ? (float)(a[i] / rms) : 0.f;
+}
+
+// y = rms_s (cscale * yd + lc * yc) + lt * x: the FP32 data and curvature
+// terms back into the FP64 CG
+__global__ void k_from_f32(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
+                           const double *__restrict__ xx, const float *__restrict__ yd, double cscale,
+                           const float *__restrict__ yc, double lc, double lt, const double *__restrict__ x,
+                           double *__restrict__ y) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const double rms = sqrt(xx[s] / (double)(off[s + 1] - off[s]));
+    double v = cscale * (double)yd[i];
+    if (yc) v += lc * (double)yc[i];
+    y[i] = rms * v + lt * x[i];
+}
+
+// r = b - A x on active samples (reliable update of the recurrence residual)
+__global__ void k_resid(long n, const int32_t *__restrict__ seg, const int *__restrict__ act,
+                        const double *__restrict__ b, const double *__restrict__ Ax, double *__restrict__ r) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && act[seg[i]]) r[i] = b[i] - Ax[i];
+}
+
+__global__ void k_take_flagged(int ns, const int *__restrict__ flag, const double *__restrict__ a,
+                               double *__restrict__ b) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns && flag[s]) b[s] = a[s];
+}
+
+// restart samples flagged in act: p = z
+__global__ void k_restart_p(long n, const int32_t *__restrict__ seg, const int *__restrict__ act,
+                            const double *__restrict__ z, double *__restrict__ p) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && act[seg[i]]) p[i] = z[i];
+}
+
 // samples that left the network (plain CG fallback) take z = r
 __global__ void k_pc_plain(long n, const int32_t *__restrict__ seg, const int *__restrict__ usenet,
                            const double *__restrict__ r, double *__restrict__ z) {
@@ -295,6 +340,12 @@ struct Solver {
     double beam_peak = 0.0;
     NormalStencil stencil;
     bool use_stencil = false;
+    // precision form: FP32 operator (SpMV, FFT, stencil) inside FP64 CG,
+    // true FP64 residual refreshed every refine_every iterations
+    int prec_mode = LT_F64;
+    int refine_every = 20;
+    DevBuf<float> x32, i1_32, i2_32, y32, t32, c32;
+    DevBuf<double> Ax, xx;
     PcNet *net = nullptr;          // learned preconditioner (LT_PC_LEARNED)
     DevBuf<float> dhat;            // diag / mean diag per sample, network input
     DevBuf<double> ainv, zap;      // 1 / mean diag; z . A p
@@ -309,6 +360,8 @@ struct Solver {
     lt_status apply(const double *xin, double *yout, const int *active_list, int nactive,
                     const int *actflags = nullptr);
     lt_status solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out);
+    lt_status apply32(const double *xin, double *yout, const int *active_list, int nactive, const int *actflags);
+    lt_status prepare32();
     lt_status jacobi_setup();
     lt_status learned_setup();
     // train (epochs > 0) or evaluate the network on residual vectors R
@@ -615,8 +668,49 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
     return sync();
 }
 
+lt_status Solver::prepare32() {
+    if (!use_stencil) {
+        set_error("solver: the FP32 precision form needs the precomputed stencil operator (LT_OP_NUFFT, FP64 plan, "
+                  "kernel_support <= 4)");
+        return LT_ERR_UNSUPPORTED;
+    }
+    if (x32.p) return LT_OK;
+    LT_TRY(csr_make_f32(lb->L));
+    LT_TRY(csr_make_f32(lb->LT));
+    LT_TRY(csr_make_f32(C));
+    LT_TRY(x32.alloc(ntot));
+    LT_TRY(y32.alloc(ntot));
+    LT_TRY(t32.alloc(ntot));
+    LT_TRY(c32.alloc(ntot));
+    LT_TRY(xx.alloc(S));
+    LT_TRY(i1_32.alloc(npix * S));
+    LT_TRY(i2_32.alloc(npix * S));
+    LT_TRY(Ax.alloc(ntot));
+    return LT_OK;
+}
+
+lt_status Solver::apply32(const double *xin, double *yout, const int *alist, int nact, const int *af) {
+    const unsigned gb = blocks_for(ntot, 256);
+    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, xin, xin, xx.p);
+    hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, xin, x32.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(csr_spmv32(lb->L, x32.p, i1_32.p, 1.f, 0.f, af, npix, nullptr));
+    LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, stencil));
+    LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p));
+    if (lam_c != 0.0) {
+        LT_TRY(csr_spmv32(C, x32.p, t32.p, 1.f, 0.f, af, 0, seg.p));
+        LT_TRY(csr_spmv32(C, t32.p, c32.p, 1.f, 0.f, af, 0, seg.p));
+    }
+    hipLaunchKernelGGL(k_from_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, y32.p,
+                       stencil.coef32_scale, lam_c != 0.0 ? c32.p : nullptr, lam_c, lam_t, xin, yout);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
 lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int *it_out, double *res_out) {
     const unsigned gb = blocks_for(ntot, 256), gs = blocks_for(S, 64);
+    const bool mixed = (prec_mode == LT_F32);
+    if (mixed) LT_TRY(prepare32());
     if (precond != LT_PC_NONE && precond != LT_PC_JACOBI && precond != LT_PC_LEARNED) {
         set_error("solver: unknown preconditioner %d", precond);
         return LT_ERR_ARG;
@@ -640,9 +734,11 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     constexpr int kWindow = 100;
     std::vector<std::vector<double>> rr_hist(S);
     std::vector<double> hrr_chk(S);
-    std::vector<int> hflag(S);
-    DevBuf<int> dflag;
-    if (flexible) LT_TRY(dflag.alloc(S));
+    std::vector<int> hflag(S), reflist(S);
+    std::vector<double> rr_ref(S);
+    DevBuf<int> dflag, dref;
+    if (flexible || mixed) LT_TRY(dflag.alloc(S));
+    if (mixed) LT_TRY(dref.alloc(S));
     std::vector<int> ones(S, 1), zeros(S, 0);
     LT_TRY(act.upload(ones.data(), S));
     LT_TRY(iters.upload(zeros.data(), S));
@@ -656,6 +752,10 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, b.p, b.p, bb.p);
     hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz.p);
     LT_CHECK_LAUNCH();
+    if (mixed) {
+        LT_TRY(bb.download(rr_ref.data(), S));
+        LT_TRY(sync());
+    }
     const double rtol2 = rtol * rtol;
     std::vector<int> hact(S, 1), alist(S);
     int nact = S;
@@ -664,8 +764,14 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(dlist.upload(alist.data(), S));
     const int check_every = 5;
     int outer = 0;
-    for (int it = 0; it < maxiter && nact > 0; ++it) {
-        LT_TRY(apply(p.p, Ap.p, dlist.p, nact, act.p));
+    int it = 0;
+    std::vector<double> hrr(S), hbb(S);
+    for (int round = 0;; ++round) {
+    for (; it < maxiter && nact > 0; ++it) {
+        if (mixed)
+            LT_TRY(apply32(p.p, Ap.p, dlist.p, nact, act.p));
+        else
+            LT_TRY(apply(p.p, Ap.p, dlist.p, nact, act.p));
         hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, p.p, Ap.p, pAp.p);
         hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
@@ -684,6 +790,34 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, z.p, p.p);
         LT_CHECK_LAUNCH();
         ++outer;
+        if (mixed && refine_every > 0 && outer % refine_every == 0) {
+            // reliable update: a sample whose recurrence residual fell 10x
+            // since its last update gets the FP64 true residual b - A x in
+            // place of the FP32 operator's recurrence one (direction kept)
+            LT_TRY(rr.download(hrr_chk.data(), S));
+            LT_TRY(act.download(hact.data(), S));
+            LT_TRY(sync());
+            int nref = 0;
+            for (int s = 0; s < S; ++s) {
+                hflag[s] = (hact[s] && hrr_chk[s] < 1e-2 * rr_ref[s]) ? 1 : 0;
+                if (hflag[s]) {
+                    reflist[nref++] = s;
+                    rr_ref[s] = hrr_chk[s];
+                }
+            }
+            if (nref) {
+                LT_TRY(dflag.upload(hflag.data(), S));
+                LT_TRY(dref.upload(reflist.data(), nref));
+                LT_TRY(apply(x.p, Ax.p, dref.p, nref, dflag.p));
+                hipLaunchKernelGGL(k_resid, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, dflag.p, b.p, Ax.p, r.p);
+                hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+                LT_CHECK_LAUNCH();
+                LT_TRY(precondition(precond, dinv_p));
+                hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz_new.p);
+                hipLaunchKernelGGL(k_take_flagged, dim3(gs), dim3(64), 0, stream(), S, dflag.p, rz_new.p, rz.p);
+                LT_CHECK_LAUNCH();
+            }
+        }
         if (outer % check_every == 0 || it == maxiter - 1) {
             LT_TRY(act.download(hact.data(), S));
             if (flexible) LT_TRY(rr.download(hrr_chk.data(), S));
@@ -721,10 +855,25 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     hipLaunchKernelGGL(k_axpby, dim3(gb), dim3(256), 0, stream(), ntot, -1.0, Ap.p, 1.0, r.p);
     hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
     LT_CHECK_LAUNCH();
-    std::vector<double> hrr(S), hbb(S);
-    std::vector<int> hit(S);
     LT_TRY(rr.download(hrr.data(), S));
     LT_TRY(bb.download(hbb.data(), S));
+    LT_TRY(sync());
+    // samples whose true residual is above the target (the recurrence drifted,
+    // or the FP32 operator's error showed) restart from their current x
+    nact = 0;
+    for (int s = 0; s < S; ++s) {
+        hact[s] = (hrr[s] > rtol2 * hbb[s] && std::isfinite(hrr[s])) ? 1 : 0;
+        if (hact[s]) alist[nact++] = s;
+    }
+    if (nact == 0 || it >= maxiter || round >= 3) break;
+    LT_TRY(act.upload(hact.data(), S));
+    LT_TRY(dlist.upload(alist.data(), nact));
+    LT_TRY(precondition(precond, dinv_p));
+    hipLaunchKernelGGL(k_restart_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, z.p, p.p);
+    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz.p);
+    LT_CHECK_LAUNCH();
+    }
+    std::vector<int> hit(S);
     LT_TRY(iters.download(hit.data(), S));
     if (xout) LT_TRY(x.download(xout, ntot));
     nfall_host.assign(S, 0);
@@ -896,6 +1045,17 @@ lt_status lt_solver_rhs(lt_solver h, double *rhs) {
     return sync();
 }
 
+lt_status lt_solver_set_precision(lt_solver h, int precision, int refine_every) {
+    if (!h || (precision != LT_F64 && precision != LT_F32) || refine_every < 0) {
+        set_error("lt_solver_set_precision: bad arguments");
+        return LT_ERR_ARG;
+    }
+    h->s.prec_mode = precision;
+    h->s.refine_every = refine_every;
+    if (precision == LT_F32) return h->s.prepare32();
+    return LT_OK;
+}
+
 lt_status lt_solver_set_rhs(lt_solver h, const double *rhs) {
     if (!h || !rhs) return LT_ERR_ARG;
     LT_TRY(h->s.b.upload(rhs, h->s.ntot));
@@ -908,7 +1068,10 @@ lt_status lt_solver_apply(lt_solver h, const double *x, double *y) {
     DevBuf<double> dx, dy;
     LT_TRY(dx.upload(x, S.ntot));
     LT_TRY(dy.alloc(S.ntot));
-    LT_TRY(S.apply(dx.p, dy.p, S.all_active.p, S.S));
+    if (S.prec_mode == LT_F32)
+        LT_TRY(S.apply32(dx.p, dy.p, S.all_active.p, S.S, nullptr));
+    else
+        LT_TRY(S.apply(dx.p, dy.p, S.all_active.p, S.S));
     LT_TRY(dy.download(y, S.ntot));
     return sync();
 }

@@ -5,6 +5,8 @@
 // matrix, operator_base.py:96-98).
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
+
 #include "lt_internal.h"
 
 namespace lt {
@@ -22,6 +24,11 @@ __global__ void k_herm_col(int nu, int half_nv, int col, double *__restrict__ X)
         X[2 * ((long)m * half_nv + col)] = 0.5 * (br + ar);
         X[2 * ((long)m * half_nv + col) + 1] = 0.5 * (bi - ai);
     }
+}
+
+// plain streaming copy (16 B per thread per iteration, grid-stride)
+__global__ void k_stream_copy(long n4, const double4 *__restrict__ a, double4 *__restrict__ b) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
 lt_status run_fft(int nx, int ny, int batch, bool forward, const double *in, double *out) {
@@ -89,6 +96,36 @@ lt_status lt_fft_r2c_host(int nx, int ny, int batch, const double *in, double *o
 
 lt_status lt_fft_c2r_host(int nx, int ny, int batch, const double *in, double *out) {
     return run_fft(nx, ny, batch, false, in, out);
+}
+
+lt_status lt_stream_copy(long nbytes, int reps, double *ms_per_launch) {
+    LT_TRY(ensure_init());
+    if (nbytes <= 0 || nbytes % 32 || reps <= 0 || !ms_per_launch) {
+        set_error("lt_stream_copy: bad arguments");
+        return LT_ERR_ARG;
+    }
+    const long n4 = nbytes / 32;
+    DevBuf<double4> a, b;
+    LT_TRY(a.alloc(n4));
+    LT_TRY(b.alloc(n4));
+    LT_TRY(a.zero());
+    const unsigned grid = (unsigned)std::min<long>(blocks_for(n4, 256), 8192);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream(), n4, a.p, b.p);
+    LT_CHECK_LAUNCH();
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    LT_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream(), n4, a.p, b.p);
+    LT_HIP(hipEventRecord(e1, stream()));
+    LT_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    LT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    LT_CHECK_LAUNCH();
+    *ms_per_launch = ms / reps;
+    return LT_OK;
 }
 
 lt_status lt_csr_spmv_host(long nrows, long ncols, const int *rp, const int *ci, const double *va, const double *x,

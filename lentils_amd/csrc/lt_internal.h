@@ -130,6 +130,7 @@ struct DevCSR {
     long nlong = 0, nchunk = 0;
     DevBuf<int32_t> long_rows, long_chunk0, chunk_beg, long_rows_of_chunk;
     DevBuf<double> chunk_sum;
+    DevBuf<float> va32, chunk_sum32;  // FP32 values (csr_make_f32)
 };
 
 // analyse row lengths once (lane group size, long-row chunk plan)
@@ -140,6 +141,10 @@ lt_status csr_prepare(DevCSR &A);
 lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes_per_row,
                    const int *act = nullptr, long rows_per_sample = 0, const int32_t *rowseg = nullptr);
 int csr_pick_lanes(const DevCSR &A);
+// FP32 copy of the values (after csr_prepare) and y = alpha A x + beta y in FP32
+lt_status csr_make_f32(DevCSR &A);
+lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, float beta_y, const int *act = nullptr,
+                     long rows_per_sample = 0, const int32_t *rowseg = nullptr);
 // B = A^T with column order inside each row ascending (deterministic)
 lt_status csr_transpose(const DevCSR &A, DevCSR &B);
 

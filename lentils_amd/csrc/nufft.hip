@@ -121,7 +121,7 @@ __device__ inline T kb_value(const KbPoly &P, T arg, T W) {
 template <class T, int W>
 __device__ inline void kb_taps(const T *__restrict__ kc, T f, T (&out)[2 * W + 1]) {
     constexpr int NT = 2 * W + 1;
-    constexpr int D = TapDeg<T>::D;
+    constexpr int D = TapDeg<T, W>::D;
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
         T acc = kc[k * (D + 1) + D];
@@ -551,8 +551,8 @@ __global__ void __launch_bounds__(kNThreads) k_grid_reduce_b(Geo G, const int32_
 
 // batched zero padding (x apod when apod != nullptr)
 // slot y of the padded batch <- sample active[y] (or y) of the images
-template <class T>
-__global__ void k_pad_b(int nx, int ny, int padx, int pady, int nxp, int nyp, const double *__restrict__ img,
+template <class T, class TI = double>
+__global__ void k_pad_b(int nx, int ny, int padx, int pady, int nxp, int nyp, const TI *__restrict__ img,
                         const double *__restrict__ apod, T *__restrict__ out, const int *__restrict__ active) {
     const int slot = blockIdx.y;
     const int smp = active ? active[slot] : slot;
@@ -562,24 +562,24 @@ __global__ void k_pad_b(int nx, int ny, int padx, int pady, int nxp, int nyp, co
     T v = (T)0;
     if (i >= 0 && i < nx && j >= 0 && j < ny) {
         const long k = (long)i * ny + j;
-        const double x = img[(long)smp * nx * ny + k];
-        v = (T)(apod ? x * apod[k] : x);
+        const TI x = img[(long)smp * nx * ny + k];
+        v = (T)(apod ? x * (TI)apod[k] : x);
     }
     out[(long)slot * nxp * nyp + g] = v;
 }
 
-template <class T>
+template <class T, class TO = double>
 __global__ void k_crop_b(int nx, int ny, int padx, int pady, int nxp, int nyp, const T *__restrict__ in,
-                         const double *__restrict__ apod, double alpha, int accumulate, double *__restrict__ img,
+                         const double *__restrict__ apod, TO alpha, int accumulate, TO *__restrict__ img,
                          const int *__restrict__ active) {
     const int slot = blockIdx.y;
     const int smp = active ? active[slot] : slot;
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)nx * ny) return;
     const int i = (int)(g / ny), j = (int)(g % ny);
-    double x = (double)in[(long)slot * nxp * nyp + (long)(i + padx) * nyp + (j + pady)];
-    double v = alpha * (apod ? x * apod[g] : x);
-    double *o = img + (long)smp * nx * ny + g;
+    const TO x = (TO)in[(long)slot * nxp * nyp + (long)(i + padx) * nyp + (j + pady)];
+    const TO v = alpha * (apod ? x * (TO)apod[g] : x);
+    TO *o = img + (long)smp * nx * ny + g;
     *o = accumulate ? *o + v : v;
 }
 
@@ -759,10 +759,10 @@ __global__ void k_compact(long n, const int64_t *__restrict__ off, int32_t *__re
 
 // R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) for NB samples per
 // block; S windows of the tile staged in LDS with fold / wrap / sign
-template <class T, int NB>
+template <class T, int NB, class TC>
 __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R, int ST, int ntv_st, const int32_t *__restrict__ tiles,
                                                        const int32_t *__restrict__ rowmap, long nrow,
-                                                       const double *__restrict__ coef, const Cplx<T> *__restrict__ spec,
+                                                       const TC *__restrict__ coef, const Cplx<T> *__restrict__ spec,
                                                        long sstride, Cplx<T> *__restrict__ rbuf,
                                                        const int *__restrict__ active, int nactive) {
     extern __shared__ unsigned char smem[];
@@ -1101,7 +1101,7 @@ lt_status NufftPlan::create(const visibility_space &vs, const fourier_space &fs,
     LT_TRY(kb_poly(beta, prec, poly));
     {
         std::vector<double> mono;
-        const int D = (prec == LT_F64) ? kTapDeg64 : kTapDeg32;
+        const int D = tap_degree(prec == LT_F64, W);
         taps_err = kb_taps_fit(beta, W, D, mono);
         const double tol = (prec == LT_F64) ? 1e-13 : 1e-6;
         if (!(taps_err <= tol)) {
@@ -1346,16 +1346,18 @@ NufftPlan::BatchFFT::~BatchFFT() {
     if (ii) rocfft_execution_info_destroy(ii);
 }
 
-lt_status NufftPlan::batch_fft(int n, BatchFFT *&out) {
+lt_status NufftPlan::batch_fft(int n, BatchFFT *&out, int precision) {
+    if (precision < 0) precision = prec;
     for (auto &b : bfft)
-        if (b->n == n) {
+        if (b->n == n && b->prec == precision) {
             out = b.get();
             return LT_OK;
         }
     auto b = std::make_unique<BatchFFT>();
     b->n = n;
+    b->prec = precision;
     const size_t len[2] = {(size_t)nyp, (size_t)nxp};
-    const rocfft_precision rp = prec == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
+    const rocfft_precision rp = precision == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
     LT_TRY(fft_check(rocfft_plan_create(&b->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, rp,
                                         2, len, (size_t)n, nullptr),
                      "batched plan (r2c)"));
@@ -1479,6 +1481,40 @@ static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, cons
     return LT_OK;
 }
 
+lt_status NufftPlan::normal_batched32(const float *imgs, float *out, const int *active, int nactive, float alpha,
+                                      bool acc, NormalStencil &st) {
+    if (nactive <= 0) return LT_OK;
+    if (nactive > batch) {
+        set_error("normal operator: %d samples exceed the batch %d", nactive, batch);
+        return LT_ERR_ARG;
+    }
+    const int nrun = std::min(batch, (nactive + kBatchStep - 1) / kBatchStep * kBatchStep);
+    BatchFFT *bf = nullptr;
+    LT_TRY(batch_fft(nrun, bf, LT_F32));
+    const long nreal = (long)nxp * nyp, nspec = (long)nu * half_nv, npix = (long)nx * ny;
+    hipLaunchKernelGGL((k_pad_b<float, float>), dim3(blocks_for(nreal, 256), nactive), dim3(256), 0, stream(), nx, ny,
+                       padx, pady, nxp, nyp, imgs, apod.p, (float *)realB.p, active);
+    LT_CHECK_LAUNCH();
+    void *in[1] = {realB.p};
+    void *o1[1] = {specB.p};
+    LT_TRY(fft_check(rocfft_execute(bf->fwd, in, o1, bf->fi), "execute batched r2c (FP32)"));
+    LT_TRY(st.apply32(*this, specB.p, nspec, specB2.p, nullptr, nactive));
+    const int cols[2] = {0, nv / 2};
+    const int ncol = (nv % 2 == 0) ? 2 : 1;
+    for (int k = 0; k < ncol; ++k) {
+        hipLaunchKernelGGL((k_hermitian_cols_b<float>), dim3(blocks_for(nu / 2 + 1, 256), nactive), dim3(256), 0,
+                           stream(), nu, half_nv, cols[k], (Cplx<float> *)specB2.p, nspec, nullptr);
+        LT_CHECK_LAUNCH();
+    }
+    void *i2[1] = {specB2.p};
+    void *o2[1] = {realB.p};
+    LT_TRY(fft_check(rocfft_execute(bf->inv, i2, o2, bf->ii), "execute batched c2r (FP32)"));
+    hipLaunchKernelGGL((k_crop_b<float, float>), dim3(blocks_for(npix, 256), nactive), dim3(256), 0, stream(), nx, ny,
+                       padx, pady, nxp, nyp, (const float *)realB.p, apod.p, alpha, acc ? 1 : 0, out, active);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
 lt_status NufftPlan::normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
                                     double alpha, bool acc, NormalStencil *st) {
     return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr, st)
@@ -1594,32 +1630,85 @@ lt_status NormalStencil::build(NufftPlan &P, const double *w_sorted) {
     return sync();
 }
 
+template <class T, class TC>
+static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef, const void *spec_in, long sstride,
+                                 void *spec_out, const int *active, int nactive) {
+    Geo G = make_geo(P);
+    const size_t need = (size_t)std::max(P.batch, 1) * St.nrow * sizeof(Cplx<double>);
+    if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
+    constexpr int NB = 4;
+    const int WS = St.ST + 2 * St.R;
+    const int ntv_st = (P.nv + St.ST - 1) / St.ST;
+    if (St.ntile > 0 && nactive > 0) {
+        const size_t sh = (size_t)NB * WS * WS * sizeof(Cplx<T>);
+        hipLaunchKernelGGL((k_stencil_apply<T, NB, TC>), dim3((unsigned)St.ntile, (nactive + NB - 1) / NB),
+                           dim3(St.ST * St.ST), sh, stream(), G, St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p, St.nrow,
+                           coef, (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive);
+        LT_CHECK_LAUNCH();
+    }
+    const long nspec = (long)P.nu * P.half_nv;
+    if (nactive > 0) {
+        hipLaunchKernelGGL((k_stencil_fold<T>), dim3(blocks_for(nspec, 256), nactive), dim3(256), 0, stream(), G,
+                           St.rowmap.p, St.nrow, (const Cplx<T> *)St.rbuf.p, (Cplx<T> *)spec_out, sstride, active);
+        LT_CHECK_LAUNCH();
+    }
+    return LT_OK;
+}
+
+__global__ void k_d2f(long n, const double *__restrict__ a, float *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
 lt_status NormalStencil::apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
                                int nactive) {
     if (P.prec != LT_F64) {
         set_error("normal stencil: FP64 plans only");
         return LT_ERR_UNSUPPORTED;
     }
-    Geo G = make_geo(P);
-    const size_t need = (size_t)std::max(P.batch, 1) * nrow * sizeof(Cplx<double>);
-    if (rbuf.n < need) LT_TRY(rbuf.alloc(need));
-    constexpr int NB = 4;
-    const int WS = ST + 2 * R;
-    const int ntv_st = (P.nv + ST - 1) / ST;
-    if (ntile > 0 && nactive > 0) {
-        const size_t sh = (size_t)NB * WS * WS * sizeof(Cplx<double>);
-        hipLaunchKernelGGL((k_stencil_apply<double, NB>), dim3((unsigned)ntile, (nactive + NB - 1) / NB),
-                           dim3(ST * ST), sh, stream(), G, R, ST, ntv_st, tiles.p, rowmap.p, nrow, coef.p,
-                           (const Cplx<double> *)spec_in, sstride, (Cplx<double> *)rbuf.p, active, nactive);
+    return stencil_apply_t<double, double>(*this, P, coef.p, spec_in, sstride, spec_out, active, nactive);
+}
+
+__global__ void k_absmax_part(long n, const double *__restrict__ a, double *__restrict__ out) {
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) m = fmax(m, fabs(a[i]));
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ void k_d2f_scaled(long n, const double *__restrict__ a, double inv, float *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)(a[i] * inv);
+}
+
+lt_status NormalStencil::apply32(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
+                                 int nactive) {
+    if (!coef32.p) {
+        // FP32 coefficients normalised by their largest magnitude (the FP64
+        // ones carry the visibility weights, ~1e15 and more); the caller
+        // multiplies the result by coef32_scale
+        const long n = (long)NO * nrow;
+        DevBuf<double> part;
+        LT_TRY(part.alloc(256));
+        hipLaunchKernelGGL(k_absmax_part, dim3(256), dim3(256), 0, stream(), n, coef.p, part.p);
+        std::vector<double> h(256);
+        LT_TRY(part.download(h.data(), 256));
+        LT_TRY(sync());
+        double m = 0.0;
+        for (double v : h) m = std::max(m, v);
+        coef32_scale = m > 0.0 ? m : 1.0;
+        LT_TRY(coef32.alloc(std::max<long>(n, 1)));
+        hipLaunchKernelGGL(k_d2f_scaled, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, coef.p,
+                           1.0 / coef32_scale, coef32.p);
         LT_CHECK_LAUNCH();
     }
-    const long nspec = (long)P.nu * P.half_nv;
-    if (nactive > 0) {
-        hipLaunchKernelGGL((k_stencil_fold<double>), dim3(blocks_for(nspec, 256), nactive), dim3(256), 0, stream(), G,
-                           rowmap.p, nrow, (const Cplx<double> *)rbuf.p, (Cplx<double> *)spec_out, sstride, active);
-        LT_CHECK_LAUNCH();
-    }
-    return LT_OK;
+    return stencil_apply_t<float, float>(*this, P, coef32.p, spec_in, sstride, spec_out, active, nactive);
 }
 
 lt_status NufftPlan::to_sorted(const double *src, void *dst) {

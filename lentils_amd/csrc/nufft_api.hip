@@ -131,7 +131,7 @@ lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, do
         return LT_ERR_ARG;
     }
     std::vector<double> mono;
-    const double err = kb_taps_fit(kb_beta, wsup, precision == LT_F64 ? kTapDeg64 : kTapDeg32, mono);
+    const double err = kb_taps_fit(kb_beta, wsup, tap_degree(precision == LT_F64, wsup), mono);
     if (coeffs) std::copy(mono.begin(), mono.end(), coeffs);
     if (max_err) *max_err = err;
     return LT_OK;

@@ -30,17 +30,13 @@ struct KbPoly {
 // Chebyshev fit of modest degree on f in [-1/2, 1/2] is accurate to the
 // working precision.  The edge taps are zero where |x| > W (f > 0 for k = 0,
 // f < 0 for k = 2W).  Coefficients are monomials in f, [k][m], m = 0..D.
-constexpr int kTapDeg32 = 9;
+// degree per precision and support: FP32 keeps the fit error below ~3e-7
+// of the kernel peak (narrow kernels vary faster per unit offset)
 constexpr int kTapDeg64 = 16;
-template <class T>
-struct TapDeg;
-template <>
-struct TapDeg<float> {
-    static constexpr int D = kTapDeg32;
-};
-template <>
-struct TapDeg<double> {
-    static constexpr int D = kTapDeg64;
+__host__ __device__ constexpr int tap_degree(bool fp64, int W) { return fp64 ? kTapDeg64 : (W <= 2 ? 9 : 7); }
+template <class T, int W>
+struct TapDeg {
+    static constexpr int D = tap_degree(sizeof(T) == 8, W);
 };
 
 struct NufftPlan;
@@ -67,10 +63,14 @@ struct NormalStencil {
     DevBuf<int32_t> row_cell;  // row -> centred raw cell index
     DevBuf<int32_t> tiles;     // stencil tiles holding rows
     DevBuf<double> coef;       // [NO][nrow]
+    DevBuf<float> coef32;      // FP32 copy / coef32_scale for the precision form (made on first use)
+    double coef32_scale = 1.0;
     DevBuf<uint8_t> rbuf;      // per-sample stencil output, [batch][nrow] complex T
 
     lt_status build(NufftPlan &P, const double *w_sorted);
     lt_status apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active, int nactive);
+    lt_status apply32(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
+                      int nactive);  // FP32 spectra
 };
 
 struct NufftPlan {
@@ -93,7 +93,7 @@ struct NufftPlan {
     int chunk = 8192;
     long nchunks = 0;
     KbPoly poly{};
-    DevBuf<float> taps32;   // (2W+1) x (kTapDeg32+1)
+    DevBuf<float> taps32;   // (2W+1) x (tap_degree(false, W)+1)
     DevBuf<double> taps64;  // (2W+1) x (kTapDeg64+1)
     double taps_err = 0.0;  // max fit error relative to the kernel peak
     DevBuf<int32_t> perm;        // sorted position -> channel-expanded index
@@ -117,7 +117,7 @@ struct NufftPlan {
     // transforms only its active samples (rounded up to kBatchStep)
     static constexpr int kBatchStep = 8;
     struct BatchFFT {
-        int n = 0;
+        int n = 0, prec = LT_F64;
         rocfft_plan fwd = nullptr, inv = nullptr;
         rocfft_execution_info fi = nullptr, ii = nullptr;
         DevBuf<uint8_t> fw, iw;
@@ -126,7 +126,7 @@ struct NufftPlan {
     int batch = 0;
     std::vector<std::unique_ptr<BatchFFT>> bfft;
     DevBuf<uint8_t> realB, specB, specB2, partialB;
-    lt_status batch_fft(int n, BatchFFT *&out);
+    lt_status batch_fft(int n, BatchFFT *&out, int precision = -1);
 
     ~NufftPlan();
     size_t tsize() const { return prec == LT_F64 ? sizeof(double) : sizeof(float); }
@@ -148,6 +148,11 @@ struct NufftPlan {
     // nu x half_nv complex128
     lt_status convolve_batched(const double *imgs, double *out, const double *spec_mul, const int *active,
                                int nactive, double alpha, bool accumulate);
+    // FP32 form of the stencil normal operator (precision form of the
+    // solver): FP32 images, spectra, rocFFT and stencil coefficients; the
+    // result is D^T C D img / stencil.coef32_scale
+    lt_status normal_batched32(const float *imgs, float *out, const int *active, int nactive, float alpha,
+                               bool accumulate, NormalStencil &stencil);
     // permute between caller order (complex128) and sorted complex T
     lt_status to_sorted(const double *vis_c128, void *sorted);
     lt_status from_sorted(const void *sorted, double *vis_c128, bool accumulate);

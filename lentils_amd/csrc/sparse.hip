@@ -29,37 +29,38 @@ struct RowFilter {
     }
 };
 
-template <int LANES>
+// T: value and vector type (double, or float for the precision form)
+template <int LANES, class T>
 __global__ void __launch_bounds__(256) k_spmv(long nrows, const int32_t *__restrict__ rp,
-                                               const int32_t *__restrict__ ci, const double *__restrict__ va,
-                                               const double *__restrict__ x, double *__restrict__ y, double alpha,
-                                               double beta, int skip_long, RowFilter F) {
+                                               const int32_t *__restrict__ ci, const T *__restrict__ va,
+                                               const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+                                               int skip_long, RowFilter F) {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long row = gid / LANES;
     const int lane = (int)(gid % LANES);
     if (row >= nrows || F.skip(row)) return;  // whole lane groups leave together
-    double s = 0.0;
+    T s = (T)0;
     const int b = rp[row], e = rp[row + 1];
     if (skip_long && e - b > kLongRow) return;  // summed by k_spmv_chunks / k_spmv_long
     for (int k = b + lane; k < e; k += LANES) s += va[k] * x[ci[k]];
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
-    if (lane == 0) y[row] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[row];
+    if (lane == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
 }
 
 // partial sums of fixed-size chunks of the long rows, one workgroup each
+template <class T>
 __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t *__restrict__ chunk_beg,
                                                       const int32_t *__restrict__ chunk_end_row,
                                                       const int32_t *__restrict__ rp,
-                                                      const int32_t *__restrict__ ci, const double *__restrict__ va,
-                                                      const double *__restrict__ x, double *__restrict__ out,
-                                                      RowFilter F) {
-    __shared__ double sh[4];
+                                                      const int32_t *__restrict__ ci, const T *__restrict__ va,
+                                                      const T *__restrict__ x, T *__restrict__ out, RowFilter F) {
+    __shared__ T sh[4];
     const long c = blockIdx.x;
     if (F.skip(chunk_end_row[c])) return;
     const int b = chunk_beg[c];
     const int e = min(b + kChunk, rp[chunk_end_row[c] + 1]);
-    double s = 0.0;
+    T s = (T)0;
     for (int k = b + threadIdx.x; k < e; k += 256) s += va[k] * x[ci[k]];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -69,15 +70,20 @@ __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t 
 }
 
 // long rows: sum their chunks in order
+template <class T>
 __global__ void k_spmv_long(long nlong, const int32_t *__restrict__ rows, const int32_t *__restrict__ chunk0,
-                            const double *__restrict__ part, double *__restrict__ y, double alpha, double beta,
-                            RowFilter F) {
+                            const T *__restrict__ part, T *__restrict__ y, T alpha, T beta, RowFilter F) {
     long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlong || F.skip(rows[l])) return;
-    double s = 0.0;
+    T s = (T)0;
     for (int c = chunk0[l]; c < chunk0[l + 1]; ++c) s += part[c];
     const int row = rows[l];
-    y[row] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[row];
+    y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
+}
+
+__global__ void k_d2f_sp(long n, const double *__restrict__ a, float *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 __global__ void k_long_rows_flag(long n, const int32_t *__restrict__ rp, int64_t *__restrict__ flag,
@@ -192,18 +198,19 @@ lt_status csr_prepare(DevCSR &A) {
     return sync();
 }
 
-lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes,
-                   const int *act, long rows_per_sample, const int32_t *rowseg) {
+template <class T>
+static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, T *y, T alpha, T beta_y, int lanes,
+                        const int *act, long rows_per_sample, const int32_t *rowseg) {
     if (A.nrows == 0) return LT_OK;
     const RowFilter F{act, rows_per_sample > 0 ? rows_per_sample : 1, rowseg};
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
     const int skip = (A.prepared && A.nlong > 0) ? 1 : 0;
     const long threads = A.nrows * lanes;
     const dim3 g(blocks_for(threads, 256)), b(256);
-#define LT_SPMV_CASE(L)                                                                                              \
-    case L:                                                                                                          \
-        hipLaunchKernelGGL(k_spmv<L>, g, b, 0, stream(), A.nrows, A.rp.p, A.ci.p, A.va.p, x, y, alpha, beta_y, skip, \
-                           F);                                                                                       \
+#define LT_SPMV_CASE(L)                                                                                            \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv<L, T>), g, b, 0, stream(), A.nrows, A.rp.p, A.ci.p, va, x, y, alpha, beta_y,    \
+                           skip, F);                                                                               \
         break;
     switch (lanes) {
         LT_SPMV_CASE(1)
@@ -220,14 +227,37 @@ lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, do
 #undef LT_SPMV_CASE
     LT_CHECK_LAUNCH();
     if (skip) {
-        hipLaunchKernelGGL(k_spmv_chunks, dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk, A.chunk_beg.p,
-                           A.long_rows_of_chunk.p, A.rp.p, A.ci.p, A.va.p, x, A.chunk_sum.p, F);
+        hipLaunchKernelGGL((k_spmv_chunks<T>), dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk,
+                           A.chunk_beg.p, A.long_rows_of_chunk.p, A.rp.p, A.ci.p, va, x, chunk_sum, F);
         LT_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_spmv_long, dim3(blocks_for(A.nlong, 256)), dim3(256), 0, stream(), A.nlong,
-                           A.long_rows.p, A.long_chunk0.p, A.chunk_sum.p, y, alpha, beta_y, F);
+        hipLaunchKernelGGL((k_spmv_long<T>), dim3(blocks_for(A.nlong, 256)), dim3(256), 0, stream(), A.nlong,
+                           A.long_rows.p, A.long_chunk0.p, chunk_sum, y, alpha, beta_y, F);
         LT_CHECK_LAUNCH();
     }
     return LT_OK;
+}
+
+lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes,
+                   const int *act, long rows_per_sample, const int32_t *rowseg) {
+    return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, alpha, beta_y, lanes, act, rows_per_sample, rowseg);
+}
+
+lt_status csr_make_f32(DevCSR &A) {
+    if (A.va32.p || A.nnz == 0) return LT_OK;
+    LT_TRY(A.va32.alloc(A.nnz));
+    hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(A.nnz, 256)), dim3(256), 0, stream(), A.nnz, A.va.p, A.va32.p);
+    LT_CHECK_LAUNCH();
+    if (A.nchunk > 0) LT_TRY(A.chunk_sum32.alloc(A.nchunk));
+    return LT_OK;
+}
+
+lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, float beta_y, const int *act,
+                     long rows_per_sample, const int32_t *rowseg) {
+    if (A.nnz > 0 && !A.va32.p) {
+        set_error("spmv: FP32 values not prepared");
+        return LT_ERR_ARG;
+    }
+    return spmv_t<float>(A, A.va32.p, A.chunk_sum32.p, x, y, alpha, beta_y, 0, act, rows_per_sample, rowseg);
 }
 
 // Transpose: counting sort of the nonzeros by column, then a segmented sort

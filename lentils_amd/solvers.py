@@ -174,7 +174,9 @@ class BatchedSourceInversion:
         b = np.ascontiguousarray(np.concatenate([np.asarray(v).reshape(-1) for v in bs]), dtype=np.float64)
         B.check(B.lib().lt_solver_set_rhs(self._handle, B.ptr(b)))
 
-    def apply(self, xs):
+    def apply(self, xs, precision='f64'):
+        """A x per sample (FP64, or the FP32 operator of the precision form)."""
+        B.check(B.lib().lt_solver_set_precision(self._handle, {'f64': B.LT_F64, 'f32': B.LT_F32}[precision], 20))
         x = np.ascontiguousarray(np.concatenate([np.asarray(v).reshape(-1) for v in xs]), dtype=np.float64)
         y = np.zeros_like(x)
         B.check(B.lib().lt_solver_apply(self._handle, B.ptr(x), B.ptr(y)))
@@ -227,11 +229,16 @@ class BatchedSourceInversion:
         B.check(B.lib().lt_solver_pc_fallbacks(self._handle, B.ptr(c), B.ptr(pl)))
         return c, pl.astype(bool)
 
-    def solve(self, rtol=1e-6, maxiter=5000, precond='none'):
+    def solve(self, rtol=1e-6, maxiter=5000, precond='none', precision='f64', refine_every=20):
         """Returns (sources, iterations, true relative residuals).
         precond: 'none' (plain CG), 'jacobi', or 'learned' (flexible CG with
-        the network given to set_preconditioner)."""
+        the network given to set_preconditioner).  precision 'f32': the
+        operator runs in FP32 inside the FP64 CG, with the FP64 true residual
+        refreshed every `refine_every` iterations (the residuals returned are
+        always FP64 true residuals)."""
         pc = {'none': B.LT_PC_NONE, 'jacobi': B.LT_PC_JACOBI, 'learned': B.LT_PC_LEARNED}[precond]
+        B.check(B.lib().lt_solver_set_precision(self._handle, {'f64': B.LT_F64, 'f32': B.LT_F32}[precision],
+                                                int(refine_every)))
         n = len(self._sizes)
         x = np.zeros(self._off[-1])
         its = np.zeros(n, dtype=np.int32)

@@ -125,3 +125,41 @@ def test_per_sample_dirty_images(hip, oracle):
         assert O.max_relative_error(one[k].reshape(-1), direct[k].reshape(-1)) < 1e-12
     with pytest.raises(ValueError):
         BatchedSourceInversion(lb, plan, weights=w, dirty_images=dirty[:2])
+
+
+def test_precision_form(hip, oracle):
+    """FP32 operator inside FP64 CG with true-residual refresh: the FP32
+    operator agrees with the FP64 one to FP32 rounding, every sample still
+    reaches an FP64 true residual of 1e-6 (checked with the oracle's
+    operator), and the solutions agree with the FP64 solve."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    im, src, uv, vs, models, data = _problem(nlens=3)
+    lb = LensBatch(im, [src] * 3, models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    xs = [rng(90 + k).normal(size=src.size) for k in range(3)]
+    y64 = inv.apply(xs)
+    y32 = inv.apply(xs, precision='f32')
+    for k in range(3):
+        err = O.max_relative_error(y32[k].reshape(-1), y64[k].reshape(-1))
+        print(f'sample {k}: FP32 operator max-relative error {err:.2e}')
+        assert err < 1e-5
+    # the FP32 operator is scale-free: inputs of 1e15 (a first CG direction
+    # with large weights) must not overflow
+    big = [v * 1e15 for v in xs]
+    yb32, yb64 = inv.apply(big, precision='f32'), inv.apply(big)
+    for k in range(3):
+        assert O.max_relative_error(yb32[k].reshape(-1), yb64[k].reshape(-1)) < 1e-5
+    x64, it64, r64 = inv.solve(rtol=1e-6, maxiter=20000)
+    x32, it32, r32 = inv.solve(rtol=1e-6, maxiter=20000, precision='f32', refine_every=20)
+    print(f'iterations FP64 {it64}, FP32 form {it32}; true residuals {r32}')
+    assert np.all(r32 <= 1e-6)
+    for k, m in enumerate(models):
+        x = x32[k].reshape(-1)
+        Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x, w, 1e2)
+        b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 2e-6
+        d = np.linalg.norm(x - x64[k].reshape(-1)) / np.linalg.norm(x64[k])
+        print(f'sample {k}: |x32 - x64| / |x64| = {d:.2e}')

@@ -12,11 +12,14 @@ import pytest
 
 from lentils_amd import backend as B
 
-DEG = {B.LT_F32: 9, B.LT_F64: 16}
+
+
+def _deg(prec, W):
+    return 16 if prec == B.LT_F64 else (9 if W <= 2 else 7)
 
 
 def _taps(beta, W, prec):
-    D = DEG[prec]
+    D = _deg(prec, W)
     c = np.zeros((2 * W + 1) * (D + 1))
     err = C.c_double(0.0)
     B.check(B.lib().lt_kb_taps(float(beta), int(W), int(prec), B.ptr(c), C.byref(err)))
@@ -37,7 +40,7 @@ def test_taps_match_oracle_kernel(oracle, W, beta, prec):
     if beta is None:
         beta = oracle.kb_beta_default(W, 2)
     c, err = _taps(beta, W, prec)
-    tol = 2e-7 if prec == B.LT_F32 else 1e-13
+    tol = 5e-7 if prec == B.LT_F32 else 1e-13
     dtype = np.float32 if prec == B.LT_F32 else np.float64
     peak = oracle.bessel_i0(beta)
     f = np.linspace(-0.5, 0.5, 401)
@@ -60,7 +63,7 @@ def test_edge_taps_at_zero_offset(oracle):
     """f = 0: both edge taps sit exactly at |x| = W, where the reference's
     kernel is I0(0) = 1 (|k| <= W is inclusive, nufft.c:12)."""
     peak = oracle.bessel_i0(14.04)
-    for prec, tol in ((B.LT_F32, 2e-7), (B.LT_F64, 1e-13)):
+    for prec, tol in ((B.LT_F32, 5e-7), (B.LT_F64, 1e-13)):
         c, _ = _taps(14.04, 3, prec)
         assert abs(c[0][0] - 1.0) / peak < tol and abs(c[6][0] - 1.0) / peak < tol
         assert abs(c[3][0] - peak) / peak < tol
