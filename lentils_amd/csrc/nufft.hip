@@ -133,6 +133,27 @@ __device__ inline void kb_taps(const T *__restrict__ kc, T f, T (&out)[2 * W + 1
     if (f < (T)0) out[NT - 1] = (T)0;
 }
 
+// same as fetch_folded, from a band of the half-plane spectrum stored
+// [v][u] for v < nbv (columns beyond the band are zero)
+template <class T>
+__device__ inline Cplx<T> fetch_band(const Geo &G, const Cplx<T> *__restrict__ band, int a, int b, int nbv) {
+    Cplx<T> z{(T)0, (T)0};
+    if (abs(a) > G.bu || abs(b) > G.bv || abs(b) >= nbv) return z;
+    const T sg = ((a + b) & 1) ? (T)-1 : (T)1;
+    if (b >= 0) {
+        const int iu = a < 0 ? a + G.nu : a;
+        z = band[(long)b * G.nu + iu];
+        z.re *= sg;
+        z.im *= sg;
+    } else {
+        const int iu = -a < 0 ? -a + G.nu : -a;
+        z = band[(long)(-b) * G.nu + iu];
+        z.re *= sg;
+        z.im *= -sg;
+    }
+    return z;
+}
+
 // value of raw tap (a, b) of the half-plane grid, with the reference's
 // wrap/fold and the checkerboard sign applied; zero outside the band
 template <class T>
@@ -759,12 +780,12 @@ __global__ void k_compact(long n, const int64_t *__restrict__ off, int32_t *__re
 
 // R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) for NB samples per
 // block; S windows of the tile staged in LDS with fold / wrap / sign
-template <class T, int NB, class TC>
+template <class T, int NB, class TC, bool BAND = false>
 __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R, int ST, int ntv_st, const int32_t *__restrict__ tiles,
                                                        const int32_t *__restrict__ rowmap, long nrow,
                                                        const TC *__restrict__ coef, const Cplx<T> *__restrict__ spec,
                                                        long sstride, Cplx<T> *__restrict__ rbuf,
-                                                       const int *__restrict__ active, int nactive) {
+                                                       const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
     const int WS = ST + 2 * R;
     Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem);
@@ -783,7 +804,8 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R, int ST, int
         const Cplx<T> *grid = spec + (long)smp[k] * sstride;
         for (int i = threadIdx.x; i < WS * WS; i += blockDim.x) {
             const int wa = i / WS, wb = i - wa * WS;
-            win[k * WS * WS + i] = fetch_folded(G, grid, u0 + wa - G.cu, v0 + wb - G.cv);
+            win[k * WS * WS + i] = BAND ? fetch_band(G, grid, u0 + wa - G.cu, v0 + wb - G.cv, nbv)
+                                        : fetch_folded(G, grid, u0 + wa - G.cu, v0 + wb - G.cv);
         }
     }
     __syncthreads();
@@ -840,6 +862,136 @@ __global__ void k_stencil_fold(Geo G, const int32_t *__restrict__ rowmap, long n
         }
     }
     spec[(long)smp * sstride + g] = tot;
+}
+
+// The same fold into a band stored [v][u] (v < nbv): unfolded (a, v) plus
+// the conjugate of (-a, -v), column 0 doubled
+template <class T>
+__global__ void k_stencil_fold_band(Geo G, const int32_t *__restrict__ rowmap, long nrow,
+                                    const Cplx<T> *__restrict__ rbuf, Cplx<T> *__restrict__ band, int nbv) {
+    const int slot = blockIdx.y;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nbv * G.nu) return;
+    const int iv = (int)(g / G.nu), iu = (int)(g - (long)iv * G.nu);
+    const int a = iu <= G.bu ? iu : iu - G.nu;
+    Cplx<T> tot{(T)0, (T)0};
+    if (abs(a) <= G.bu && iv <= G.bv) {
+        const int r1 = rowmap[(long)(a + G.cu) * G.nv + (iv + G.cv)];
+        if (r1 >= 0) tot = rbuf[(long)slot * nrow + r1];
+        if (iv >= 1) {
+            const int r2 = rowmap[(long)(-a + G.cu) * G.nv + (-iv + G.cv)];
+            if (r2 >= 0) {
+                const Cplx<T> f = rbuf[(long)slot * nrow + r2];
+                tot.re += f.re;
+                tot.im -= f.im;
+            }
+        }
+        if (iv == 0) {
+            tot.re *= (T)2;
+            tot.im *= (T)2;
+        }
+    }
+    band[(long)slot * nbv * G.nu + g] = tot;
+}
+
+// Hermitian symmetrisation of band row v = 0 (what c2r assumes of column 0)
+template <class T>
+__global__ void k_herm_band0(int nu, long sstride, Cplx<T> *__restrict__ band) {
+    Cplx<T> *Y = band + (long)blockIdx.y * sstride;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nu / 2) return;
+    const int m = (nu - k) % nu;
+    const Cplx<T> a = Y[k], b = Y[m];
+    Y[k] = Cplx<T>{(T)0.5 * (a.re + b.re), (T)0.5 * (a.im - b.im)};
+    if (m != k) Y[m] = Cplx<T>{(T)0.5 * (b.re + a.re), (T)0.5 * (b.im - a.im)};
+}
+
+// slot rows of the zero-padded, apodised image: R[slot][i][k], k < nyp
+template <class T, class TI>
+__global__ void k_band_pad(int nx, int ny, int pady, int nyp, const TI *__restrict__ img,
+                           const double *__restrict__ apod, const int *__restrict__ active, T *__restrict__ R) {
+    const int slot = blockIdx.y;
+    const int smp = active ? active[slot] : slot;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nx * nyp) return;
+    const int i = (int)(g / nyp), k = (int)(g - (long)i * nyp);
+    const int j = k - pady;
+    T v = (T)0;
+    if (j >= 0 && j < ny) {
+        const long q = (long)i * ny + j;
+        v = (T)(img[(long)smp * nx * ny + q] * (TI)apod[q]);
+    }
+    R[(long)slot * nx * nyp + g] = v;
+}
+
+// B[slot][v][u] = C[slot][u - padx][v] (0 outside the image rows), v < nbv:
+// a 32 x 32 tile transpose through LDS
+template <class T>
+__global__ void __launch_bounds__(256) k_band_gather(int nx, int padx, int nu, int half_nv, int nbv,
+                                                     const Cplx<T> *__restrict__ C, Cplx<T> *__restrict__ B) {
+    __shared__ Cplx<T> tile[32][33];
+    const int slot = blockIdx.z;
+    const int u0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+    const Cplx<T> *Cs = C + (long)slot * nx * half_nv;
+    Cplx<T> *Bs = B + (long)slot * nbv * nu;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {  // read rows u (image rows), columns v
+        const int u = u0 + r, v = v0 + tx;
+        const int i = u - padx;
+        Cplx<T> z{(T)0, (T)0};
+        if (i >= 0 && i < nx && v < nbv) z = Cs[(long)i * half_nv + v];
+        tile[r][tx] = z;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {  // write rows v, columns u
+        const int v = v0 + r, u = u0 + tx;
+        if (v < nbv && u < nu) Bs[(long)v * nu + u] = tile[tx][r];
+    }
+}
+
+// C[slot][i][v] = F[slot][v][i + padx] for v < nbv, 0 for nbv <= v < half_nv
+template <class T>
+__global__ void __launch_bounds__(256) k_band_scatter(int nx, int padx, int nu, int half_nv, int nbv,
+                                                      const Cplx<T> *__restrict__ F, Cplx<T> *__restrict__ C) {
+    __shared__ Cplx<T> tile[32][33];
+    const int slot = blockIdx.z;
+    const int i0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+    const Cplx<T> *Fs = F + (long)slot * nbv * nu;
+    Cplx<T> *Cs = C + (long)slot * nx * half_nv;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {  // read rows v, columns i
+        const int v = v0 + r, i = i0 + tx;
+        Cplx<T> z{(T)0, (T)0};
+        if (v < nbv && i < nx) z = Fs[(long)v * nu + i + padx];
+        tile[r][tx] = z;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {  // write rows i, columns v
+        const int i = i0 + r, v = v0 + tx;
+        if (i < nx && v < half_nv) Cs[(long)i * half_nv + v] = tile[tx][r];
+    }
+}
+
+// out[smp] (+)= alpha * apod * R[slot][i][j + pady]
+template <class T, class TO>
+__global__ void k_band_crop(int nx, int ny, int pady, int nyp, const T *__restrict__ R, const double *__restrict__ apod,
+                            TO alpha, int accumulate, const int *__restrict__ active, TO *__restrict__ out) {
+    const int slot = blockIdx.y;
+    const int smp = active ? active[slot] : slot;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nx * ny) return;
+    const int i = (int)(g / ny), j = (int)(g - (long)i * ny);
+    const TO x = (TO)R[(long)slot * nx * nyp + (long)i * nyp + j + pady];
+    const TO v = alpha * (x * (TO)apod[g]);
+    TO *o = out + (long)smp * nx * ny + g;
+    *o = accumulate ? *o + v : v;
+}
+
+__global__ void k_max_abs_b(long nrow, const int32_t *__restrict__ row_cell, int nv, int cv, int *__restrict__ out) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const int b = (int)(row_cell[r] % nv) - cv;
+    atomicMax(out, b < 0 ? -b : b);
 }
 
 Geo make_geo(const NufftPlan &P) {
@@ -1481,8 +1633,129 @@ static lt_status normal_impl(NufftPlan &P, const double *imgs, double *out, cons
     return LT_OK;
 }
 
+NufftPlan::BandFFT::~BandFFT() {
+    for (int k = 0; k < 4; ++k) {
+        if (plan[k]) rocfft_plan_destroy(plan[k]);
+        if (info[k]) rocfft_execution_info_destroy(info[k]);
+    }
+}
+
+lt_status NufftPlan::band_fft(int n, int precision, int nbv, BandFFT *&out) {
+    for (auto &b : bandfft)
+        if (b->n == n && b->prec == precision && b->nbv == nbv) {
+            out = b.get();
+            return LT_OK;
+        }
+    auto b = std::make_unique<BandFFT>();
+    b->n = n;
+    b->prec = precision;
+    b->nbv = nbv;
+    const rocfft_precision rp = precision == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
+    const size_t ly = (size_t)nyp, lu = (size_t)nu;
+    // rows along y (r2c / c2r, out of place), columns along u (c2c, in place)
+    LT_TRY(fft_check(rocfft_plan_create(&b->plan[0], rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                        rp, 1, &ly, (size_t)n * nx, nullptr),
+                     "band plan (rows r2c)"));
+    LT_TRY(fft_check(rocfft_plan_create(&b->plan[1], rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                                        rp, 1, &lu, (size_t)n * nbv, nullptr),
+                     "band plan (columns forward)"));
+    LT_TRY(fft_check(rocfft_plan_create(&b->plan[2], rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                                        rp, 1, &lu, (size_t)n * nbv, nullptr),
+                     "band plan (columns inverse)"));
+    LT_TRY(fft_check(rocfft_plan_create(&b->plan[3], rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                                        rp, 1, &ly, (size_t)n * nx, nullptr),
+                     "band plan (rows c2r)"));
+    for (int k = 0; k < 4; ++k) {
+        size_t ws = 0;
+        LT_TRY(fft_check(rocfft_plan_get_work_buffer_size(b->plan[k], &ws), "work size"));
+        LT_TRY(fft_check(rocfft_execution_info_create(&b->info[k]), "info"));
+        if (ws) {
+            LT_TRY(b->work[k].alloc(ws));
+            LT_TRY(fft_check(rocfft_execution_info_set_work_buffer(b->info[k], b->work[k].p, ws), "work buffer"));
+        }
+        LT_TRY(fft_check(rocfft_execution_info_set_stream(b->info[k], stream()), "stream"));
+    }
+    out = b.get();
+    bandfft.push_back(std::move(b));
+    return LT_OK;
+}
+
+// D^T C D through the stencil with a band-pruned separable transform: rows of
+// the padded image are transformed along y (only the image's rows exist),
+// only the stencil's band of nbv spectral columns is carried to the
+// transform along u, and back the same way.  Equals the full 2-D path.
+template <class T, class TI>
+static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *imgs, TI *out, const int *active,
+                                  int nactive, TI alpha, bool acc) {
+    if (nactive <= 0) return LT_OK;
+    if (nactive > P.batch) {
+        set_error("normal operator: %d samples exceed the batch %d", nactive, P.batch);
+        return LT_ERR_ARG;
+    }
+    const int prec = sizeof(T) == 8 ? LT_F64 : LT_F32;
+    const int nbv = St.nbv;
+    const int nrun = std::min(P.batch, (nactive + NufftPlan::kBatchStep - 1) / NufftPlan::kBatchStep *
+                                           NufftPlan::kBatchStep);
+    NufftPlan::BandFFT *bf = nullptr;
+    LT_TRY(P.band_fft(nrun, prec, nbv, bf));
+    const long rsz = (long)P.nx * P.nyp, csz = (long)P.nx * P.half_nv, bsz = (long)nbv * P.nu;
+    const size_t need_r = (size_t)P.batch * rsz * sizeof(T), need_c = (size_t)P.batch * csz * 2 * sizeof(T),
+                 need_b = (size_t)P.batch * bsz * 2 * sizeof(T);
+    if (P.bandR.n < need_r) LT_TRY(P.bandR.alloc(need_r));
+    if (P.bandC.n < need_c) LT_TRY(P.bandC.alloc(need_c));
+    if (P.bandB.n < need_b) LT_TRY(P.bandB.alloc(need_b));
+    if (P.bandF.n < need_b) LT_TRY(P.bandF.alloc(need_b));
+    T *R = (T *)P.bandR.p;
+    Cplx<T> *Cb = (Cplx<T> *)P.bandC.p, *Bb = (Cplx<T> *)P.bandB.p, *Fb = (Cplx<T> *)P.bandF.p;
+    hipLaunchKernelGGL((k_band_pad<T, TI>), dim3(blocks_for(rsz, 256), nactive), dim3(256), 0, stream(), P.nx, P.ny,
+                       P.pady, P.nyp, imgs, P.apod.p, active, R);
+    LT_CHECK_LAUNCH();
+    {
+        void *in[1] = {R};
+        void *o[1] = {Cb};
+        LT_TRY(fft_check(rocfft_execute(bf->plan[0], in, o, bf->info[0]), "execute band rows r2c"));
+    }
+    hipLaunchKernelGGL((k_band_gather<T>), dim3((P.nu + 31) / 32, (nbv + 31) / 32, nactive), dim3(256), 0, stream(),
+                       P.nx, P.padx, P.nu, P.half_nv, nbv, Cb, Bb);
+    LT_CHECK_LAUNCH();
+    {
+        void *io[1] = {Bb};
+        LT_TRY(fft_check(rocfft_execute(bf->plan[1], io, nullptr, bf->info[1]), "execute band columns forward"));
+    }
+    if (prec == LT_F64)
+        LT_TRY((stencil_apply_t<double, double>(St, P, St.coef.p, Bb, bsz, nullptr, nullptr, nactive, true)));
+    else
+        LT_TRY((stencil_apply_t<float, float>(St, P, St.coef32.p, Bb, bsz, nullptr, nullptr, nactive, true)));
+    Geo G = make_geo(P);
+    hipLaunchKernelGGL((k_stencil_fold_band<T>), dim3(blocks_for(bsz, 256), nactive), dim3(256), 0, stream(), G,
+                       St.rowmap.p, St.nrow, (const Cplx<T> *)St.rbuf.p, Fb, nbv);
+    hipLaunchKernelGGL((k_herm_band0<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0, stream(), P.nu,
+                       bsz, Fb);
+    LT_CHECK_LAUNCH();
+    {
+        void *io[1] = {Fb};
+        LT_TRY(fft_check(rocfft_execute(bf->plan[2], io, nullptr, bf->info[2]), "execute band columns inverse"));
+    }
+    hipLaunchKernelGGL((k_band_scatter<T>), dim3((P.nx + 31) / 32, (P.half_nv + 31) / 32, nactive), dim3(256), 0,
+                       stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Fb, Cb);
+    LT_CHECK_LAUNCH();
+    {
+        void *in[1] = {Cb};
+        void *o[1] = {R};
+        LT_TRY(fft_check(rocfft_execute(bf->plan[3], in, o, bf->info[3]), "execute band rows c2r"));
+    }
+    hipLaunchKernelGGL((k_band_crop<T, TI>), dim3(blocks_for((long)P.nx * P.ny, 256), nactive), dim3(256), 0, stream(),
+                       P.nx, P.ny, P.pady, P.nyp, R, P.apod.p, alpha, acc ? 1 : 0, active, out);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
 lt_status NufftPlan::normal_batched32(const float *imgs, float *out, const int *active, int nactive, float alpha,
                                       bool acc, NormalStencil &st) {
+    if (st.use_band) {
+        if (!st.coef32.p) LT_TRY(st.apply32(*this, nullptr, 0, nullptr, nullptr, 0));  // makes coef32
+        return normal_band_impl<float, float>(*this, st, imgs, out, active, nactive, alpha, acc);
+    }
     if (nactive <= 0) return LT_OK;
     if (nactive > batch) {
         set_error("normal operator: %d samples exceed the batch %d", nactive, batch);
@@ -1517,6 +1790,8 @@ lt_status NufftPlan::normal_batched32(const float *imgs, float *out, const int *
 
 lt_status NufftPlan::normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
                                     double alpha, bool acc, NormalStencil *st) {
+    if (st && st->use_band && prec == LT_F64)
+        return normal_band_impl<double, double>(*this, *st, imgs, out, active, nactive, alpha, acc);
     return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr, st)
                           : normal_impl<float>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr, st);
 }
@@ -1627,12 +1902,27 @@ lt_status NormalStencil::build(NufftPlan &P, const double *w_sorted) {
     LT_TRY(tiles.alloc(std::max<long>(ntile, 1)));
     hipLaunchKernelGGL(k_compact, dim3(blocks_for(nts, 256)), dim3(256), 0, stream(), nts, toff.p, tiles.p);
     LT_CHECK_LAUNCH();
+    // spectral band the stencil reads: |v| up to the rows' largest + R
+    {
+        DevBuf<int> mb;
+        LT_TRY(mb.alloc(1));
+        LT_TRY(mb.zero());
+        hipLaunchKernelGGL(k_max_abs_b, dim3(blocks_for(nrow, 256)), dim3(256), 0, stream(), nrow, row_cell.p, P.nv,
+                           P.cv, mb.p);
+        LT_CHECK_LAUNCH();
+        int hb = 0;
+        LT_TRY(mb.download(&hb, 1));
+        LT_TRY(sync());
+        nbv = std::min(P.half_nv, hb + R + 1);
+    }
     return sync();
 }
 
+// band: spec_in is a [v][u] band of nbv rows (the stencil's rows only); the
+// fold is then left to the caller (spec_out unused)
 template <class T, class TC>
 static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef, const void *spec_in, long sstride,
-                                 void *spec_out, const int *active, int nactive) {
+                                 void *spec_out, const int *active, int nactive, bool band = false) {
     Geo G = make_geo(P);
     const size_t need = (size_t)std::max(P.batch, 1) * St.nrow * sizeof(Cplx<double>);
     if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
@@ -1641,13 +1931,20 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
     const int ntv_st = (P.nv + St.ST - 1) / St.ST;
     if (St.ntile > 0 && nactive > 0) {
         const size_t sh = (size_t)NB * WS * WS * sizeof(Cplx<T>);
-        hipLaunchKernelGGL((k_stencil_apply<T, NB, TC>), dim3((unsigned)St.ntile, (nactive + NB - 1) / NB),
-                           dim3(St.ST * St.ST), sh, stream(), G, St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p, St.nrow,
-                           coef, (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive);
+        if (band)
+            hipLaunchKernelGGL((k_stencil_apply<T, NB, TC, true>), dim3((unsigned)St.ntile, (nactive + NB - 1) / NB),
+                               dim3(St.ST * St.ST), sh, stream(), G, St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p,
+                               St.nrow, coef, (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active,
+                               nactive, St.nbv);
+        else
+            hipLaunchKernelGGL((k_stencil_apply<T, NB, TC, false>), dim3((unsigned)St.ntile, (nactive + NB - 1) / NB),
+                               dim3(St.ST * St.ST), sh, stream(), G, St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p,
+                               St.nrow, coef, (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active,
+                               nactive, St.nbv);
         LT_CHECK_LAUNCH();
     }
     const long nspec = (long)P.nu * P.half_nv;
-    if (nactive > 0) {
+    if (nactive > 0 && !band) {
         hipLaunchKernelGGL((k_stencil_fold<T>), dim3(blocks_for(nspec, 256), nactive), dim3(256), 0, stream(), G,
                            St.rowmap.p, St.nrow, (const Cplx<T> *)St.rbuf.p, (Cplx<T> *)spec_out, sstride, active);
         LT_CHECK_LAUNCH();

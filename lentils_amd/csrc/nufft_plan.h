@@ -65,6 +65,8 @@ struct NormalStencil {
     DevBuf<double> coef;       // [NO][nrow]
     DevBuf<float> coef32;      // FP32 copy / coef32_scale for the precision form (made on first use)
     double coef32_scale = 1.0;
+    int nbv = 0;               // spectral columns 0..nbv-1 hold every value the stencil reads
+    bool use_band = true;      // band-pruned separable transform instead of the full 2-D one
     DevBuf<uint8_t> rbuf;      // per-sample stencil output, [batch][nrow] complex T
 
     lt_status build(NufftPlan &P, const double *w_sorted);
@@ -125,6 +127,18 @@ struct NufftPlan {
     };
     int batch = 0;
     std::vector<std::unique_ptr<BatchFFT>> bfft;
+    // band-pruned transforms of the stencil path: rows along y, nbv columns
+    // along u (see normal_band_impl)
+    struct BandFFT {
+        int n = 0, prec = LT_F64, nbv = 0;
+        rocfft_plan plan[4] = {nullptr, nullptr, nullptr, nullptr};
+        rocfft_execution_info info[4] = {nullptr, nullptr, nullptr, nullptr};
+        DevBuf<uint8_t> work[4];
+        ~BandFFT();
+    };
+    std::vector<std::unique_ptr<BandFFT>> bandfft;
+    DevBuf<uint8_t> bandR, bandC, bandB, bandF;
+    lt_status band_fft(int n, int precision, int nbv, BandFFT *&out);
     DevBuf<uint8_t> realB, specB, specB2, partialB;
     lt_status batch_fft(int n, BatchFFT *&out, int precision = -1);
 
