@@ -486,6 +486,14 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     std::vector<int> ids(S);
     for (int s = 0; s < S; ++s) ids[s] = s;
     LT_TRY(all_active.upload(ids.data(), S));
+    // autotuned launch shapes: SpMV lane groups of L, L^T, Cc; stencil batching
+    LT_TRY(csr_autotune(lb->L, false));
+    LT_TRY(csr_autotune(lb->LT, false));
+    LT_TRY(csr_autotune(C, false));
+    if (use_stencil) {
+        LT_TRY(img1.zero());
+        LT_TRY(stencil_autotune(*nf, stencil, img1.p, img2.p, all_active.p, S));
+    }
     // rhs b_s = L_s^T dirty_s (a shared dirty image is replicated)
     for (int s = 0; s < S; ++s)
         LT_HIP(hipMemcpyAsync(img2.p + (long)s * npix, dirty.p + (ndirty == 1 ? 0L : (long)s * npix),
@@ -675,9 +683,9 @@ lt_status Solver::prepare32() {
         return LT_ERR_UNSUPPORTED;
     }
     if (x32.p) return LT_OK;
-    LT_TRY(csr_make_f32(lb->L));
-    LT_TRY(csr_make_f32(lb->LT));
-    LT_TRY(csr_make_f32(C));
+    LT_TRY(csr_autotune(lb->L, true));
+    LT_TRY(csr_autotune(lb->LT, true));
+    LT_TRY(csr_autotune(C, true));
     LT_TRY(x32.alloc(ntot));
     LT_TRY(y32.alloc(ntot));
     LT_TRY(t32.alloc(ntot));

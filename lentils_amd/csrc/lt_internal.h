@@ -131,6 +131,7 @@ struct DevCSR {
     DevBuf<int32_t> long_rows, long_chunk0, chunk_beg, long_rows_of_chunk;
     DevBuf<double> chunk_sum;
     DevBuf<float> va32, chunk_sum32;  // FP32 values (csr_make_f32)
+    int lanes32 = 0;                  // lane-group width of the FP32 product (0: as lanes)
 };
 
 // analyse row lengths once (lane group size, long-row chunk plan)
@@ -143,6 +144,8 @@ lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, do
 int csr_pick_lanes(const DevCSR &A);
 // FP32 copy of the values (after csr_prepare) and y = alpha A x + beta y in FP32
 lt_status csr_make_f32(DevCSR &A);
+// pick the fastest lane-group width for this matrix by timing (FP64 or FP32)
+lt_status csr_autotune(DevCSR &A, bool fp32);
 lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, float beta_y, const int *act = nullptr,
                      long rows_per_sample = 0, const int32_t *rowseg = nullptr);
 // B = A^T with column order inside each row ascending (deterministic)

@@ -1750,6 +1750,36 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     return LT_OK;
 }
 
+// Autotuner of the stencil's samples-per-workgroup (2, 4, 8): times the
+// whole band normal operator at the full batch for each and keeps the
+// fastest.  imgs / out: nb images of scratch.
+lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, double *out, const int *all, int nb) {
+    if (!St.use_band || nb <= 0) return LT_OK;
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_nb = 4;
+    for (int cand : {2, 4, 8}) {
+        St.nb_sel = cand;
+        LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));  // warm-up
+        LT_HIP(hipEventRecord(e0, stream()));
+        for (int r = 0; r < 2; ++r) LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));
+        LT_HIP(hipEventRecord(e1, stream()));
+        LT_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_nb = cand;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    St.nb_sel = best_nb;
+    return LT_OK;
+}
+
 lt_status NufftPlan::normal_batched32(const float *imgs, float *out, const int *active, int nactive, float alpha,
                                       bool acc, NormalStencil &st) {
     if (st.use_band) {
@@ -1926,21 +1956,30 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
     Geo G = make_geo(P);
     const size_t need = (size_t)std::max(P.batch, 1) * St.nrow * sizeof(Cplx<double>);
     if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
-    constexpr int NB = 4;
     const int WS = St.ST + 2 * St.R;
     const int ntv_st = (P.nv + St.ST - 1) / St.ST;
     if (St.ntile > 0 && nactive > 0) {
-        const size_t sh = (size_t)NB * WS * WS * sizeof(Cplx<T>);
-        if (band)
-            hipLaunchKernelGGL((k_stencil_apply<T, NB, TC, true>), dim3((unsigned)St.ntile, (nactive + NB - 1) / NB),
-                               dim3(St.ST * St.ST), sh, stream(), G, St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p,
-                               St.nrow, coef, (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active,
-                               nactive, St.nbv);
+        // NB samples share one staged coefficient read (autotuned: 2, 4 or 8)
+#define LT_STENCIL_NB(NBV)                                                                                        \
+    {                                                                                                             \
+        const size_t sh = (size_t)NBV * WS * WS * sizeof(Cplx<T>);                                                \
+        const dim3 grid((unsigned)St.ntile, (nactive + NBV - 1) / NBV);                                           \
+        if (band)                                                                                                 \
+            hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, true>), grid, dim3(St.ST * St.ST), sh, stream(), G,   \
+                               St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p, St.nrow, coef,                       \
+                               (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, false>), grid, dim3(St.ST * St.ST), sh, stream(), G,  \
+                               St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p, St.nrow, coef,                       \
+                               (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
+    }
+        if (St.nb_sel == 2)
+            LT_STENCIL_NB(2)
+        else if (St.nb_sel == 8)
+            LT_STENCIL_NB(8)
         else
-            hipLaunchKernelGGL((k_stencil_apply<T, NB, TC, false>), dim3((unsigned)St.ntile, (nactive + NB - 1) / NB),
-                               dim3(St.ST * St.ST), sh, stream(), G, St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p,
-                               St.nrow, coef, (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active,
-                               nactive, St.nbv);
+            LT_STENCIL_NB(4)
+#undef LT_STENCIL_NB
         LT_CHECK_LAUNCH();
     }
     const long nspec = (long)P.nu * P.half_nv;

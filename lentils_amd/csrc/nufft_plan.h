@@ -67,6 +67,7 @@ struct NormalStencil {
     double coef32_scale = 1.0;
     int nbv = 0;               // spectral columns 0..nbv-1 hold every value the stencil reads
     bool use_band = true;      // band-pruned separable transform instead of the full 2-D one
+    int nb_sel = 4;            // samples per stencil workgroup (stencil_autotune)
     DevBuf<uint8_t> rbuf;      // per-sample stencil output, [batch][nrow] complex T
 
     lt_status build(NufftPlan &P, const double *w_sorted);
@@ -171,5 +172,8 @@ struct NufftPlan {
     lt_status to_sorted(const double *vis_c128, void *sorted);
     lt_status from_sorted(const void *sorted, double *vis_c128, bool accumulate);
 };
+
+// pick the stencil's samples per workgroup by timing the normal operator
+lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, double *out, const int *all, int nb);
 
 }  // namespace lt

@@ -257,7 +257,51 @@ lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, flo
         set_error("spmv: FP32 values not prepared");
         return LT_ERR_ARG;
     }
-    return spmv_t<float>(A, A.va32.p, A.chunk_sum32.p, x, y, alpha, beta_y, 0, act, rows_per_sample, rowseg);
+    return spmv_t<float>(A, A.va32.p, A.chunk_sum32.p, x, y, alpha, beta_y, A.lanes32, act, rows_per_sample,
+                         rowseg);
+}
+
+// Autotuner: times the product with every lane-group width (1..64 lanes per
+// row) on this matrix and keeps the fastest (A.lanes, or A.lanes32 for the
+// FP32 values).  A few milliseconds per matrix.
+template <class T>
+static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, int &best_out) {
+    DevBuf<T> x, y;
+    LT_TRY(x.alloc(std::max<long>(A.ncols, 1)));
+    LT_TRY(y.alloc(std::max<long>(A.nrows, 1)));
+    LT_TRY(x.zero());
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_l = A.lanes;
+    for (int L = 1; L <= 64; L <<= 1) {
+        LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr));  // warm-up
+        LT_HIP(hipEventRecord(e0, stream()));
+        for (int r = 0; r < 3; ++r) LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr));
+        LT_HIP(hipEventRecord(e1, stream()));
+        LT_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_l = L;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    best_out = best_l;
+    return LT_OK;
+}
+
+lt_status csr_autotune(DevCSR &A, bool fp32) {
+    if (A.nrows == 0 || A.nnz == 0) return LT_OK;
+    if (!A.prepared) LT_TRY(csr_prepare(A));
+    if (fp32) {
+        LT_TRY(csr_make_f32(A));
+        return tune_lanes<float>(A, A.va32.p, A.chunk_sum32.p, A.lanes32);
+    }
+    return tune_lanes<double>(A, A.va.p, A.chunk_sum.p, A.lanes);
 }
 
 // Transpose: counting sort of the nonzeros by column, then a segmented sort
