@@ -74,7 +74,7 @@ class Workload:
     set), entering the solver through its dirty image.  shared_data: one
     observation (lensed by the first sample) for all 64 samples."""
 
-    def __init__(self, nvis=NVIS, nlens=NLENS, shared_data=False):
+    def __init__(self, nvis=NVIS, nlens=NLENS, shared_data=False, src_sizes=None):
         from lentils_amd import ImageSpace, VisibilitySpace
         from lentils_amd import backend as B
         from lentils_amd.solvers import LensBatch, NUFFTPlan
@@ -82,6 +82,10 @@ class Workload:
         self.shared_data = shared_data
         self.im = ImageSpace(shape=(512, 512), bounds=[(-2.56, 2.56), (-2.56, 2.56)])
         self.src = ImageSpace(shape=(128, 128), bounds=[(-0.64, 0.64), (-0.64, 0.64)])
+        # per-sample source grids (all 128^2 unless uneven sizes are given)
+        self.srcs = [self.src if n == 128 else ImageSpace(shape=(n, n), bounds=[(-0.64, 0.64), (-0.64, 0.64)])
+                     for n in (src_sizes if src_sizes is not None else [128] * nlens)]
+        self.nvis = nvis
         self.uvw = gen_uv(nvis)
         self.vs = VisibilitySpace(channels=[1.0], uvcoords=self.uvw)
         self.lenses = gen_lenses(nlens)
@@ -93,14 +97,14 @@ class Workload:
         self.lambda_curv = 1e2
         g = np.random.Generator(np.random.PCG64(13))
         nobs = 1 if shared_data else nlens
-        truth = LensBatch(self.im, [self.src] * nobs, self.lenses[:nobs], z_src=2.0)
+        truth = LensBatch(self.im, self.srcs[:nobs], self.lenses[:nobs], z_src=2.0)
         self.data = None
         self.dirty = []
         for s in range(nobs):
             rp, ci, va = truth.csr(s)
-            x = np.ascontiguousarray(blob_source(self.src, seed=14 + s).reshape(-1))
+            x = np.ascontiguousarray(blob_source(self.srcs[s], seed=14 + s).reshape(-1))
             img = np.zeros(self.im.size)
-            B.check(B.lib().lt_csr_spmv_host(self.im.size, self.src.size, B.ptr(rp), B.ptr(ci), B.ptr(va),
+            B.check(B.lib().lt_csr_spmv_host(self.im.size, self.srcs[s].size, B.ptr(rp), B.ptr(ci), B.ptr(va),
                                              B.ptr(x), B.ptr(img)))
             vis = self.plan.forward(img).astype(np.complex64)  # complex64 visibilities
             vis.real += g.standard_normal(nvis, dtype=np.float32) * np.float32(1e-3)
@@ -114,7 +118,7 @@ class Workload:
 
     def step(self, rtol=1e-6, maxiter=20000, operator='nufft', precond='none', precision='f64'):
         from lentils_amd.solvers import BatchedSourceInversion, LensBatch
-        lb = LensBatch(self.im, [self.src] * len(self.lenses), self.lenses, z_src=2.0)
+        lb = LensBatch(self.im, self.srcs, self.lenses, z_src=2.0)
         if self.shared_data:
             inv = BatchedSourceInversion(lb, self.plan, self.data, weights=self.weights,
                                          lambda_curvature=self.lambda_curv, operator=operator)
@@ -176,6 +180,51 @@ def load_traffic():
             return json.load(f).get('traffic_bytes_per_launch')
     except (OSError, ValueError):
         return None
+
+
+def batching_form(args):
+    """Batching workload: 256 lens-parameter samples in one batch, source grids
+    of uneven sizes drawn from {32^2, 48^2, 64^2, 96^2, 128^2}, a shared 1e6
+    visibility uv set (complex64 data, one observation per sample), 512^2
+    image at 0.01", every solve to a 1e-6 FP64 true residual.  Prints one
+    JSON line."""
+    from lentils_amd import backend as B
+    B.check(B.lib().lt_init(0))
+    nlens = 256
+    sizes = list(np.random.Generator(np.random.PCG64(15)).choice([32, 48, 64, 96, 128], size=nlens))
+    t = time.perf_counter()
+    wl = Workload(nvis=1_000_000, nlens=nlens, src_sizes=sizes)
+    log(f'batching workload setup {time.perf_counter() - t:.1f}s')
+    for k in range(args.warmup):
+        t = time.perf_counter()
+        its, res = wl.step(maxiter=args.maxiter)
+        log(f'warmup {k}: {time.perf_counter() - t:.2f}s iters median {np.median(its):.0f}')
+    t0 = time.perf_counter()
+    all_its, all_res = [], []
+    for k in range(args.steps):
+        its, res = wl.step(maxiter=args.maxiter)
+        all_its.append(its)
+        all_res.append(res)
+    total = time.perf_counter() - t0
+    its, res = np.concatenate(all_its), np.concatenate(all_res)
+    t = time.perf_counter()
+    its32, res32 = wl.step(maxiter=args.maxiter, precision='f32')
+    t32 = time.perf_counter() - t
+    out = {'metric': 'source solves/s at 1e-6 rel resid (batching form)', 'value': nlens * args.steps / total,
+           'unit': 'source solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+           'ms_per_step': 1e3 * total / args.steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f64', 'data': 'synthetic: one observation per lens sample, complex64 vis + noise 1e-3',
+           'config': {'workload': 'batching form: 256 lens samples, uneven source grids {32,48,64,96,128}^2, '
+                                  '512^2 image @0.01", 1e6 shared vis, 1024x513 grid, W=6, FP64 CG to 1e-6',
+                      'source_sizes': {str(n): int(sizes.count(n)) for n in (32, 48, 64, 96, 128)},
+                      'parallelism': 'single GPU'},
+           'solver': {'iters_median': float(np.median(its)), 'iters_max': int(its.max()),
+                      'true_rel_resid_max': float(res.max()), 'all_converged': bool(np.all(res <= 1e-6))},
+           'precision_form': {'value': nlens / t32, 'ms_per_step': 1e3 * t32,
+                              'iters_median': float(np.median(its32)),
+                              'true_rel_resid_max': float(res32.max()),
+                              'all_converged': bool(np.all(res32 <= 1e-6))}}
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(wl, gpu_iters):
@@ -257,6 +306,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--maxiter', type=int, default=20000)
     ap.add_argument('--precond', default='none', choices=['none', 'jacobi'])
+    ap.add_argument('--batching', action='store_true',
+                    help='the batching workload (256 samples, uneven source grids, 1e6 visibilities) instead')
     ap.add_argument('--profile-degrid', action='store_true',
                     help='only run the degridder and the calibration copy (for rocprofv3 --pmc)')
     ap.add_argument('--shared-data', action='store_true',
@@ -264,6 +315,9 @@ def main():
     args = ap.parse_args()
     if args.profile_degrid:
         profile_degrid()
+        return
+    if args.batching:
+        batching_form(args)
         return
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (every solve of a batch is independent; '

@@ -31,12 +31,12 @@ struct RowFilter {
 
 // T: value and vector type (double, or float for the precision form)
 template <int LANES, class T>
-__global__ void __launch_bounds__(256) k_spmv(long nrows, const int32_t *__restrict__ rp,
+__global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32_t *__restrict__ rp,
                                                const int32_t *__restrict__ ci, const T *__restrict__ va,
                                                const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
                                                int skip_long, RowFilter F) {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long row = gid / LANES;
+    const long row = row0 + gid / LANES;
     const int lane = (int)(gid % LANES);
     if (row >= nrows || F.skip(row)) return;  // whole lane groups leave together
     T s = (T)0;
@@ -205,27 +205,31 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
     const RowFilter F{act, rows_per_sample > 0 ? rows_per_sample : 1, rowseg};
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
     const int skip = (A.prepared && A.nlong > 0) ? 1 : 0;
-    const long threads = A.nrows * lanes;
-    const dim3 g(blocks_for(threads, 256)), b(256);
+    // a dispatch covers at most 2^30 work-items (the grid size is 32-bit)
+    const long rows_per_launch = (1L << 30) / lanes;
+    for (long r0 = 0; r0 < A.nrows; r0 += rows_per_launch) {
+        const long nr = std::min(rows_per_launch, A.nrows - r0);
+        const dim3 g(blocks_for(nr * lanes, 256)), b(256);
 #define LT_SPMV_CASE(L)                                                                                            \
     case L:                                                                                                        \
-        hipLaunchKernelGGL((k_spmv<L, T>), g, b, 0, stream(), A.nrows, A.rp.p, A.ci.p, va, x, y, alpha, beta_y,    \
+        hipLaunchKernelGGL((k_spmv<L, T>), g, b, 0, stream(), r0, r0 + nr, A.rp.p, A.ci.p, va, x, y, alpha, beta_y, \
                            skip, F);                                                                               \
         break;
-    switch (lanes) {
-        LT_SPMV_CASE(1)
-        LT_SPMV_CASE(2)
-        LT_SPMV_CASE(4)
-        LT_SPMV_CASE(8)
-        LT_SPMV_CASE(16)
-        LT_SPMV_CASE(32)
-        LT_SPMV_CASE(64)
-        default:
-            set_error("spmv: lanes per row must be a power of two <= 64");
-            return LT_ERR_ARG;
-    }
+        switch (lanes) {
+            LT_SPMV_CASE(1)
+            LT_SPMV_CASE(2)
+            LT_SPMV_CASE(4)
+            LT_SPMV_CASE(8)
+            LT_SPMV_CASE(16)
+            LT_SPMV_CASE(32)
+            LT_SPMV_CASE(64)
+            default:
+                set_error("spmv: lanes per row must be a power of two <= 64");
+                return LT_ERR_ARG;
+        }
 #undef LT_SPMV_CASE
-    LT_CHECK_LAUNCH();
+        LT_CHECK_LAUNCH();
+    }
     if (skip) {
         hipLaunchKernelGGL((k_spmv_chunks<T>), dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk,
                            A.chunk_beg.p, A.long_rows_of_chunk.p, A.rp.p, A.ci.p, va, x, chunk_sum, F);
