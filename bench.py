@@ -227,6 +227,102 @@ def batching_form(args):
     print(json.dumps(out), flush=True)
 
 
+def precision_report(args):
+    """Precision-form workload: errors of the FP32 / mixed paths against FP64.
+    (1) FP64 deflections against the oracle for q up to 0.999 and radii down
+    to 1e-4" (series convergence); (2) FP32 visibilities of a lensed image
+    (1024^2 image at 0.005", 5e7 visibilities, 2048 x 1025 grid) against the
+    FP64 plan on all visibilities and against the oracle on a subset; (3) the
+    precision-form solution against the FP64 solution for 8 lens samples with
+    256^2 sources.  Prints one JSON line."""
+    from lentils_amd import ImageSpace, VisibilitySpace
+    from lentils_amd import backend as B
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    from oracle import oracle as O
+    B.check(B.lib().lt_init(0))
+    out = {'metric': 'errors of the precision form against FP64 (precision workload)'}
+    # (1) deflections
+    g = np.random.Generator(np.random.PCG64(21))
+    worst, rows = 0.0, []
+    for q in (0.5, 0.7, 0.9, 0.99, 0.999):
+        for gam in (1.8, 2.0, 2.2):
+            b = g.uniform(0.5, 1.5)
+            lens = [O.pack_pemd(b=b, q=q, th=g.uniform(0, 180), qh=(gam - 1) / 2, x=0.013, y=-0.007),
+                    O.pack_shear(ss=0.03, sa=g.uniform(0, 180))]
+            r = np.exp(g.uniform(np.log(1e-4), np.log(3.0), 20000))
+            ph = g.uniform(0, 2 * np.pi, 20000)
+            pts = np.stack([0.013 + r * np.cos(ph), -0.007 + r * np.sin(ph)], 1)
+            arr = np.array(lens, dtype=O.MASS_MODEL_DTYPE)
+            glm = B.global_lens_model_ctype(2, 1, arr.ctypes.data_as(C.c_void_p))
+            got = np.zeros_like(pts)
+            B.check(B.lib().lt_deflect_points(C.byref(glm), B.ptr(pts), pts.shape[0], B.ptr(got)))
+            ref = O.deflect_points(lens, pts)
+            a_g, a_r = pts - got, pts - ref
+            e = float(np.max(np.linalg.norm(a_g - a_r, axis=1) / np.linalg.norm(a_r, axis=1)))
+            worst = max(worst, e)
+            rows.append({'q': q, 'gamma': gam, 'max_rel_err': e})
+    out['deflection'] = {'max_rel_err': worst, 'points_per_case': 20000, 'radii_arcsec': [1e-4, 3.0],
+                         'cases': rows}
+    log(f'deflections: max relative error {worst:.2e}')
+    # (2) visibilities
+    im = ImageSpace(shape=(1024, 1024), bounds=[(-2.56, 2.56), (-2.56, 2.56)])
+    src = ImageSpace(shape=(256, 256), bounds=[(-0.64, 0.64), (-0.64, 0.64)])
+    nvis = 50_000_000
+    uvw = gen_uv(nvis, seed=16)
+    vs = VisibilitySpace(channels=[1.0], uvcoords=uvw)
+    lenses = gen_lenses(8, seed=17)
+    lb = LensBatch(im, [src] * 8, lenses, z_src=2.0)
+    rp, ci, va = lb.csr(0)
+    x = np.ascontiguousarray(blob_source(src, seed=18).reshape(-1))
+    img = np.zeros(im.size)
+    B.check(B.lib().lt_csr_spmv_host(im.size, src.size, B.ptr(rp), B.ptr(ci), B.ptr(va), B.ptr(x), B.ptr(img)))
+    p64 = NUFFTPlan(vs, im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f64')
+    p32 = NUFFTPlan(vs, im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f32')
+    v64 = p64.forward(img)
+    v32 = p32.forward(img)
+    e_all = float(np.max(np.abs(v32 - v64)) / np.max(np.abs(v64)))
+    k = 200_000
+    og = O.Grid2D(1024, 1024, [-2.56, 2.56, -2.56, 2.56])
+    geo = O.NUFFTGeometry(og, pad=2, wsup=3, beta=14.04)
+    vo = O.nufft_forward(img.reshape(1024, 1024), uvw[:k], [1.0], geo, nthreads=len(os.sched_getaffinity(0)))
+    vo = vo.reshape(-1)
+    e_or32 = float(np.max(np.abs(v32[:k] - vo)) / np.max(np.abs(vo)))
+    e_or64 = float(np.max(np.abs(v64[:k] - vo)) / np.max(np.abs(vo)))
+    out['visibilities'] = {'nvis': nvis, 'f32_vs_f64_max_rel_err': e_all, 'oracle_subset': k,
+                           'f32_vs_oracle_max_rel_err': e_or32, 'f64_vs_oracle_max_rel_err': e_or64}
+    log(f'visibilities: FP32 vs FP64 {e_all:.2e}, FP32 vs oracle {e_or32:.2e}, FP64 vs oracle {e_or64:.2e}')
+    del p32, v32
+    # (3) solutions: one observation per sample
+    gn = np.random.Generator(np.random.PCG64(19))
+    w = np.full(nvis, 1e6)
+    dirty = []
+    for s_ in range(8):
+        rp, ci, va = lb.csr(s_)
+        xs = np.ascontiguousarray(blob_source(src, seed=30 + s_).reshape(-1))
+        im_s = np.zeros(im.size)
+        B.check(B.lib().lt_csr_spmv_host(im.size, src.size, B.ptr(rp), B.ptr(ci), B.ptr(va), B.ptr(xs),
+                                         B.ptr(im_s)))
+        vis = p64.forward(im_s).astype(np.complex64)
+        vis.real += gn.standard_normal(nvis, dtype=np.float32) * np.float32(1e-3)
+        vis.imag += gn.standard_normal(nvis, dtype=np.float32) * np.float32(1e-3)
+        dirty.append(p64.adjoint(w * vis.astype(np.complex128)))
+    inv = BatchedSourceInversion(lb, p64, weights=w, dirty_images=dirty, lambda_curvature=1e2)
+    t = time.perf_counter()
+    x64, it64, r64 = inv.solve(rtol=1e-6, maxiter=args.maxiter)
+    t64 = time.perf_counter() - t
+    t = time.perf_counter()
+    x32, it32, r32 = inv.solve(rtol=1e-6, maxiter=args.maxiter, precision='f32')
+    t32 = time.perf_counter() - t
+    dif = [float(np.linalg.norm(a - b_) / np.linalg.norm(b_)) for a, b_ in zip(x32, x64)]
+    out['solution'] = {'samples': 8, 'f64': {'seconds': t64, 'iters_median': float(np.median(it64)),
+                                             'true_rel_resid_max': float(r64.max())},
+                       'f32_form': {'seconds': t32, 'iters_median': float(np.median(it32)),
+                                    'true_rel_resid_max': float(r32.max())},
+                       'rel_diff_f32_vs_f64_max': max(dif)}
+    log(f'solutions: FP64 {t64:.2f}s, precision form {t32:.2f}s, max relative difference {max(dif):.2e}')
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(wl, gpu_iters):
     """Reference CPU path on a bounded sample: one lens sample's deflection +
     rasterization with the reference's own C (oracle/_ref, single thread)
@@ -306,6 +402,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--maxiter', type=int, default=20000)
     ap.add_argument('--precond', default='none', choices=['none', 'jacobi'])
+    ap.add_argument('--precision-report', action='store_true',
+                    help='errors of the FP32 paths against FP64 on the precision workload (1024^2 image, 5e7 vis)')
     ap.add_argument('--batching', action='store_true',
                     help='the batching workload (256 samples, uneven source grids, 1e6 visibilities) instead')
     ap.add_argument('--profile-degrid', action='store_true',
@@ -318,6 +416,9 @@ def main():
         return
     if args.batching:
         batching_form(args)
+        return
+    if args.precision_report:
+        precision_report(args)
         return
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (every solve of a batch is independent; '
