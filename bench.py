@@ -401,7 +401,7 @@ def main():
     ap.add_argument('--operator', default='nufft', choices=['nufft', 'bcb'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--maxiter', type=int, default=20000)
-    ap.add_argument('--precond', default='none', choices=['none', 'jacobi'])
+    ap.add_argument('--precond', default='none', choices=['none', 'jacobi', 'lens'])
     ap.add_argument('--precision-report', action='store_true',
                     help='errors of the FP32 paths against FP64 on the precision workload (1024^2 image, 5e7 vis)')
     ap.add_argument('--batching', action='store_true',
@@ -462,6 +462,20 @@ def main():
               'true_rel_resid_max': float(res32.max()), 'all_converged': bool(np.all(res32 <= 1e-6)),
               'dtype': 'f32 operator (SpMV, rocFFT, stencil), f64 CG vectors and inner products'}
         log(f'precision form: {t32:.2f}s iters median {np.median(its32):.0f} max res {res32.max():.2e}')
+    # the reference's preconditioner structure (lambda Cc^T Cc + c L_s^T L_s,
+    # three inner CG steps on it per outer iteration) on the same workload
+    lp = None
+    if args.operator == 'nufft' and args.precond == 'none':
+        for k in range(args.warmup):
+            wl.step(operator=args.operator, maxiter=args.maxiter, precond='lens')
+        t = time.perf_counter()
+        itsl, resl = wl.step(operator=args.operator, maxiter=args.maxiter, precond='lens')
+        tl = time.perf_counter() - t
+        lp = {'value': NLENS / tl, 'unit': 'source solves/s', 'ms_per_step': 1e3 * tl,
+              'iters_median': float(np.median(itsl)), 'iters_max': int(itsl.max()),
+              'true_rel_resid_max': float(resl.max()), 'all_converged': bool(np.all(resl <= 1e-6)),
+              'method': 'flexible CG, preconditioner: 3 Jacobi-PCG steps on the explicit lambda Cc^T Cc + c L_s^T L_s (L_s: rows of L with <= 64 entries)'}
+        log(f'lens preconditioner: {tl:.2f}s iters median {np.median(itsl):.0f} max res {resl.max():.2e}')
     rl = degrid_roofline(wl)
     log(f'degrid f32 {rl["ms"] * 1e3:.1f} us/launch, {rl["gbs"]:.0f} GB/s, {rl["gvis_s"]:.2f} Gvis/s')
     copy_gbs, _ = stream_copy_gbs()
@@ -483,6 +497,7 @@ def main():
         'solver': {'iters_median': float(np.median(its)), 'iters_max': int(its.max()),
                    'true_rel_resid_max': float(res.max()), 'all_converged': bool(np.all(res <= 1e-6))},
         'precision_form': pf,
+        'lens_preconditioner': lp,
         'degrid_gvis_s': rl['gvis_s'], 'degrid_gvis_s_f64': rl['gvis_s_f64'],
         'roofline': {'kernel': 'k_degrid<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS,

@@ -236,6 +236,9 @@ lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, do
 #define LT_PC_NONE 0
 #define LT_PC_JACOBI 1
 #define LT_PC_LEARNED 2 /* learned network (lt_solver_set_pcnet) in flexible CG, Jacobi as its guard */
+#define LT_PC_LENS 3    /* M = lambda_c Cc^T Cc + c L_s^T L_s (the reference's R + w L^T L preconditioner,
+                           test_all.py:63-65; L_s: L without rows above 256 entries), applied by a few
+                           Jacobi-PCG steps on M inside flexible CG (lt_solver_set_lens_pc) */
 lt_status lt_solver_create(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *data, double lambda_curv,
                            double lambda_tikh, int op_mode, lt_solver *out);
 /* As lt_solver_create, with the data given through their dirty images
@@ -257,6 +260,16 @@ lt_status lt_solver_rhs(lt_solver s, double *rhs);
  * rtol restarts from its x.  Needs the stencil operator (LT_OP_NUFFT with
  * an FP64 plan and kernel_support <= 4).  LT_F64 (default): all FP64. */
 lt_status lt_solver_set_precision(lt_solver s, int precision, int refine_every);
+/* LT_PC_LENS settings: inner Jacobi-PCG steps on M per application (default
+ * 3), the row-length limit of L_s (default 64) and the weight c of
+ * L_s^T L_s (<= 0: the data operator's response to a unit impulse at the
+ * image centre, the default). */
+lt_status lt_solver_set_lens_pc(lt_solver s, int inner_steps, int max_row, double c);
+/* The LT_PC_LENS matrix M = c L_s^T L_s + lambda Cc^T Cc (L_s: the rows of
+ * L with at most max_row entries), built on first use, as block-diagonal CSR
+ * over all samples' source pixels: nnz and the c used; row_ptr (ntot + 1),
+ * cols and vals (nnz) are written when non-NULL. */
+lt_status lt_solver_lens_pc_matrix(lt_solver s, long *nnz, double *c_used, int *row_ptr, int *cols, double *vals);
 /* replace the right-hand sides (all samples concatenated, host) */
 lt_status lt_solver_set_rhs(lt_solver s, const double *rhs);
 /* y = A x for all samples, in the precision set by lt_solver_set_precision */

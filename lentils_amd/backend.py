@@ -50,7 +50,7 @@ LT_OK = 0
 LT_ERR_ARG, LT_ERR_HIP, LT_ERR_NYQUIST, LT_ERR_UNSUPPORTED, LT_ERR_CAPACITY, LT_ERR_FFT, LT_ERR_NOT_CONVERGED = 1, 2, 3, 4, 5, 6, 7
 LT_F64, LT_F32 = 0, 1
 LT_OP_NUFFT, LT_OP_BCB, LT_OP_NUFFT_EXPLICIT = 0, 1, 2
-LT_PC_NONE, LT_PC_JACOBI, LT_PC_LEARNED = 0, 1, 2
+LT_PC_NONE, LT_PC_JACOBI, LT_PC_LEARNED, LT_PC_LENS = 0, 1, 2, 3
 RASTER_STACK_OVERFLOW, RASTER_FRAG_OVERFLOW, RASTER_POP_LIMIT, RASTER_BAD_LENS = 1, 2, 4, 8
 
 
@@ -122,6 +122,8 @@ def lib():
         'lt_solver_rhs': ([vp, vp], i),
         'lt_solver_set_rhs': ([vp, vp], i),
         'lt_solver_set_precision': ([vp, i, i], i),
+        'lt_solver_set_lens_pc': ([vp, i, i, d], i),
+        'lt_solver_lens_pc_matrix': ([vp, vp, vp, vp, vp, vp], i),
         'lt_solver_apply': ([vp, vp, vp], i),
         'lt_solver_destroy': ([vp], i),
         'lt_fft_r2c_host': ([i, i, i, vp, vp], i),
@@ -148,7 +150,7 @@ def exported_symbols():
             'lt_solver_create', 'lt_solver_create_dirty', 'lt_pcnet_param_count', 'lt_pcnet_create',
             'lt_pcnet_get_params', 'lt_pcnet_set_params', 'lt_pcnet_forward_host', 'lt_pcnet_grad_host',
             'lt_pcnet_train', 'lt_pcnet_eval', 'lt_pcnet_destroy', 'lt_solver_set_pcnet', 'lt_solver_diag', 'lt_solver_train_pcnet',
-            'lt_solver_pc_fallbacks', 'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_set_rhs', 'lt_solver_set_precision', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
+            'lt_solver_pc_fallbacks', 'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_set_rhs', 'lt_solver_set_precision', 'lt_solver_set_lens_pc', 'lt_solver_lens_pc_matrix', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
             'lt_fft_c2r_host', 'lt_csr_spmv_host', 'lt_stream_copy']
 
 

@@ -10,6 +10,10 @@
 // (LT_OP_BCB, data_base.py:115-129).  Inner products are FP64 per sample,
 // reduced in a fixed order inside one workgroup per sample; the solve stops
 // when the FP64 true residual ||b - A x|| / ||b|| <= rtol.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -287,6 +291,134 @@ __global__ void k_restart_p(long n, const int32_t *__restrict__ seg, const int *
     if (i < n && act[seg[i]]) p[i] = z[i];
 }
 
+// --- lens-structured preconditioner ---------------------------------------
+// M = lambda_c Cc^T Cc + c L_s^T L_s, L_s = L without its long rows (the
+// lens-centre pixels that touch most of the source plane): the reference's
+// preconditioner R + w L^T L (test_all.py:63-65, there factorised with
+// splu), applied here approximately by a few Jacobi-preconditioned CG steps
+// on the explicitly built M per outer iteration (flexible CG outside).
+
+// Explicit M = c L_s^T L_s + lambda_c Cc^T Cc as the sum over the rows r of
+// the stacked matrix [sqrt(c) L_s; sqrt(lambda_c) Cc] of their outer
+// products: row r with m entries gives m^2 (row, col, w v_a v_b) pairs
+__global__ void k_pair_count(long nl, const int32_t *__restrict__ lrp, int maxrow, long nc,
+                             const int32_t *__restrict__ crp, int64_t *__restrict__ cnt) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nl + nc) return;
+    long m;
+    if (r < nl) {
+        m = lrp[r + 1] - lrp[r];
+        if (m > maxrow) m = 0;
+    } else {
+        m = crp[r - nl + 1] - crp[r - nl];
+    }
+    cnt[r] = m * m;
+}
+
+// key = row << nb | col: both are source-pixel indices of the batch, < 2^nb
+__global__ void k_pair_emit(long nl, const int32_t *__restrict__ lrp, const int32_t *__restrict__ lci,
+                            const double *__restrict__ lva, int maxrow, double c, long nc, int nb,
+                            const int32_t *__restrict__ crp, const int32_t *__restrict__ cci,
+                            const double *__restrict__ cva, double lam, const int64_t *__restrict__ off,
+                            uint64_t *__restrict__ key, double *__restrict__ val) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nl + nc) return;
+    const int32_t *ci;
+    const double *va;
+    int b0, b1;
+    double w;
+    if (r < nl) {
+        b0 = lrp[r];
+        b1 = lrp[r + 1];
+        if (b1 - b0 > maxrow) return;
+        ci = lci;
+        va = lva;
+        w = c;
+    } else {
+        b0 = crp[r - nl];
+        b1 = crp[r - nl + 1];
+        ci = cci;
+        va = cva;
+        w = lam;
+    }
+    long o = off[r];
+    for (int a = b0; a < b1; ++a)
+        for (int b = b0; b < b1; ++b, ++o) {
+            key[o] = ((uint64_t)(uint32_t)ci[a] << nb) | (uint64_t)(uint32_t)ci[b];
+            val[o] = w * va[a] * va[b];
+        }
+}
+
+__global__ void k_key_flags(long n, const uint64_t *__restrict__ key, int64_t *__restrict__ flag) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+// one thread per distinct (row, col): the in-order sum of its pairs
+__global__ void k_key_sum(long n, int nb, const uint64_t *__restrict__ key, const double *__restrict__ val,
+                          const int64_t *__restrict__ gpos, int32_t *__restrict__ col, double *__restrict__ out,
+                          int32_t *__restrict__ rowcnt) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || gpos[i + 1] == gpos[i]) return;
+    const uint64_t k = key[i];
+    double s = 0.0;
+    for (long j = i; j < n && key[j] == k; ++j) s += val[j];
+    const long g = gpos[i];
+    col[g] = (int32_t)(k & ((1ull << nb) - 1));
+    out[g] = s;
+    atomicAdd(&rowcnt[(long)(k >> nb)], 1);  // a count: the same for any order
+}
+
+__global__ void k_i32_to_i64(long n, const int32_t *__restrict__ a, int64_t *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_i64_to_i32_rp(long n, const int64_t *__restrict__ a, int32_t *__restrict__ b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) b[i] = (int32_t)a[i];
+}
+
+__global__ void k_csr_diag(long n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                           const double *__restrict__ va, double *__restrict__ d) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double v = 0.0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k)
+        if (ci[k] == r) v = va[k];
+    d[r] = v > 0.0 ? v : 1.0;
+}
+
+// z = r / dM
+__global__ void k_divide(long n, const double *__restrict__ r, const double *__restrict__ dM, double *__restrict__ z) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = r[i] / dM[i];
+}
+
+// inner CG update: x += a_s p; r -= a_s Mp; z = r / dM
+__global__ void k_inner_xr(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
+                           const double *__restrict__ pMp, const double *__restrict__ p, const double *__restrict__ Mp,
+                           const double *__restrict__ dM, double *__restrict__ x, double *__restrict__ r,
+                           double *__restrict__ z) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const double a = pMp[s] != 0.0 ? rz[s] / pMp[s] : 0.0;
+    x[i] += a * p[i];
+    r[i] -= a * Mp[i];
+    z[i] = r[i] / dM[i];
+}
+
+// p = z + (rz_new / rz) p
+__global__ void k_inner_p(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
+                          const double *__restrict__ rz_new, const double *__restrict__ z, double *__restrict__ p) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    const double b = rz[s] != 0.0 ? rz_new[s] / rz[s] : 0.0;
+    p[i] = z[i] + b * p[i];
+}
+
 // samples that left the network (plain CG fallback) take z = r
 __global__ void k_pc_plain(long n, const int32_t *__restrict__ seg, const int *__restrict__ usenet,
                            const double *__restrict__ r, double *__restrict__ z) {
@@ -346,6 +478,18 @@ struct Solver {
     int refine_every = 20;
     DevBuf<float> x32, i1_32, i2_32, y32, t32, c32;
     DevBuf<double> Ax, xx;
+    // lens-structured preconditioner (LT_PC_LENS)
+    int lens_inner = 3;            // inner Jacobi-PCG steps on M per application
+    int lens_maxrow = 64;          // image rows with more entries are left out of L_s
+    double lens_c = 0.0;           // weight of L_s^T L_s (0: the data operator's impulse-response peak)
+    double lens_c_used = 0.0;
+    const int *cur_act = nullptr;  // active flags of the running solve (sparse products skip the rest)
+    DevBuf<double> dM, ix, ir, iz, ip, iMp, irz, irz2, ipMp;
+    bool lens_ready = false;
+    DevCSR Mpc;                    // explicit M (block-diagonal over samples)
+    lt_status lens_setup();
+    lt_status lens_build_matrix();
+    lt_status lens_apply(const int *af);  // z = approx M^-1 r
     PcNet *net = nullptr;          // learned preconditioner (LT_PC_LEARNED)
     DevBuf<float> dhat;            // diag / mean diag per sample, network input
     DevBuf<double> ainv, zap;      // 1 / mean diag; z . A p
@@ -553,8 +697,147 @@ lt_status Solver::jacobi_setup() {
     return sync();
 }
 
+lt_status Solver::lens_setup() {
+    if (lens_ready) return LT_OK;
+    const unsigned gb = blocks_for(ntot, 256);
+    for (auto *v : {&dM, &ix, &ir, &iz, &ip, &iMp}) LT_TRY(v->alloc(ntot));
+    for (auto *v : {&irz, &irz2, &ipMp}) LT_TRY(v->alloc(S));
+    double c = lens_c;
+    if (!(c > 0.0)) {
+        // impulse response of the data operator at the image centre (sample
+        // 0's slot): its peak is the data weight of one image pixel
+        const long ctr = (long)(lb->im.nx / 2) * lb->im.ny + lb->im.ny / 2;
+        LT_TRY(img1.zero());
+        const double one = 1.0;
+        LT_HIP(hipMemcpyAsync(img1.p + ctr, &one, sizeof(double), hipMemcpyHostToDevice, stream()));
+        if (mode != LT_OP_BCB)
+            LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, all_active.p, 1, 1.0, false, use_stencil ? &stencil : nullptr));
+        else
+            LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, all_active.p, 1, 1.0, false));
+        LT_TRY(img2.download(&c, 1, ctr));
+        LT_TRY(sync());
+        if (!(c > 0.0)) c = 1.0;
+    }
+    lens_c_used = c;
+    LT_TRY(lens_build_matrix());
+    hipLaunchKernelGGL(k_csr_diag, dim3(gb), dim3(256), 0, stream(), ntot, Mpc.rp.p, Mpc.ci.p, Mpc.va.p, dM.p);
+    LT_CHECK_LAUNCH();
+    lens_ready = true;
+    return sync();
+}
+
+lt_status Solver::lens_build_matrix() {
+    int nb = 1;
+    while ((1L << nb) < ntot) ++nb;  // ntot < 2^31, so 2 nb <= 62 key bits
+    const DevCSR &L = lb->L;
+    const long nl = L.nrows, nc = C.nrows, nr = nl + nc;
+    DevBuf<int64_t> cnt, off;
+    LT_TRY(cnt.alloc(nr));
+    LT_TRY(off.alloc(nr + 1));
+    hipLaunchKernelGGL(k_pair_count, dim3(blocks_for(nr, 256)), dim3(256), 0, stream(), nl, L.rp.p, lens_maxrow, nc,
+                       C.rp.p, cnt.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(cnt.p, off.p, nr));
+    int64_t np = 0;
+    LT_TRY(off.download(&np, 1, nr));
+    LT_TRY(sync());
+    if (np == 0) {
+        set_error("lens preconditioner: empty matrix");
+        return LT_ERR_ARG;
+    }
+    DevBuf<uint64_t> k1, k2;
+    DevBuf<double> v1, v2;
+    LT_TRY(k1.alloc(std::max<int64_t>(np, 1)));
+    LT_TRY(k2.alloc(std::max<int64_t>(np, 1)));
+    LT_TRY(v1.alloc(std::max<int64_t>(np, 1)));
+    LT_TRY(v2.alloc(std::max<int64_t>(np, 1)));
+    hipLaunchKernelGGL(k_pair_emit, dim3(blocks_for(nr, 256)), dim3(256), 0, stream(), nl, L.rp.p, L.ci.p, L.va.p,
+                       lens_maxrow, lens_c_used, nc, nb, C.rp.p, C.ci.p, C.va.p, lam_c, off.p, k1.p, v1.p);
+    LT_CHECK_LAUNCH();
+    {
+        size_t tb = 0;
+        LT_HIP(rocprim::radix_sort_pairs(nullptr, tb, k1.p, k2.p, v1.p, v2.p, (size_t)np, 0, 2 * nb, stream()));
+        DevBuf<uint8_t> tmpb;
+        LT_TRY(tmpb.alloc(std::max<size_t>(tb, 1)));
+        LT_HIP(rocprim::radix_sort_pairs(tmpb.p, tb, k1.p, k2.p, v1.p, v2.p, (size_t)np, 0, 2 * nb, stream()));
+    }
+    k1.release();
+    v1.release();
+    DevBuf<int64_t> flag, gpos;
+    LT_TRY(flag.alloc(std::max<int64_t>(np, 1)));
+    LT_TRY(gpos.alloc(np + 1));
+    hipLaunchKernelGGL(k_key_flags, dim3(blocks_for(np, 256)), dim3(256), 0, stream(), np, k2.p, flag.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(flag.p, gpos.p, np));
+    int64_t nz = 0;
+    LT_TRY(gpos.download(&nz, 1, np));
+    LT_TRY(sync());
+    if (nz >= INT32_MAX) {
+        set_error("lens preconditioner: matrix too large");
+        return LT_ERR_CAPACITY;
+    }
+    Mpc = DevCSR();
+    Mpc.nrows = Mpc.ncols = ntot;
+    Mpc.nnz = nz;
+    LT_TRY(Mpc.ci.alloc(std::max<int64_t>(nz, 1)));
+    LT_TRY(Mpc.va.alloc(std::max<int64_t>(nz, 1)));
+    LT_TRY(Mpc.rp.alloc(ntot + 1));
+    DevBuf<int32_t> rowcnt;
+    DevBuf<int64_t> rc64, rp64;
+    LT_TRY(rowcnt.alloc(ntot));
+    LT_TRY(rowcnt.zero());
+    hipLaunchKernelGGL(k_key_sum, dim3(blocks_for(np, 256)), dim3(256), 0, stream(), np, nb, k2.p, v2.p, gpos.p,
+                       Mpc.ci.p, Mpc.va.p, rowcnt.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(rc64.alloc(ntot));
+    LT_TRY(rp64.alloc(ntot + 1));
+    hipLaunchKernelGGL(k_i32_to_i64, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, rowcnt.p, rc64.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(rc64.p, rp64.p, ntot));
+    hipLaunchKernelGGL(k_i64_to_i32_rp, dim3(blocks_for(ntot + 1, 256)), dim3(256), 0, stream(), ntot, rp64.p,
+                       Mpc.rp.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(csr_prepare(Mpc));
+    LT_TRY(csr_autotune(Mpc, false));
+    return sync();
+}
+
+// z ~ M^-1 r: lens_inner steps of Jacobi-preconditioned CG on
+// M = lam_c Cc^T Cc + c L_s^T L_s from z = 0, every sample at once
+lt_status Solver::lens_apply(const int *af) {
+    const unsigned gb = blocks_for(ntot, 256);
+    LT_TRY(ix.zero());
+    hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, r.p, ir.p);
+    hipLaunchKernelGGL(k_divide, dim3(gb), dim3(256), 0, stream(), ntot, ir.p, dM.p, iz.p);
+    hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, iz.p, ip.p);
+    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, ir.p, iz.p, irz.p);
+    LT_CHECK_LAUNCH();
+    for (int k = 0; k < lens_inner; ++k) {
+        // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc
+        LT_TRY(csr_spmv(Mpc, ip.p, iMp.p, 1.0, 0.0, 0, af, 0, seg.p));
+        hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, ip.p, iMp.p, ipMp.p);
+        hipLaunchKernelGGL(k_inner_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, irz.p, ipMp.p, ip.p, iMp.p,
+                           dM.p, ix.p, ir.p, iz.p);
+        LT_CHECK_LAUNCH();
+        if (k + 1 < lens_inner) {
+            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, ir.p, iz.p, irz2.p);
+            hipLaunchKernelGGL(k_inner_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, irz.p, irz2.p, iz.p, ip.p);
+            hipLaunchKernelGGL(k_copy, dim3(blocks_for(S, 64)), dim3(64), 0, stream(), (long)S, irz2.p, irz.p);
+            LT_CHECK_LAUNCH();
+        }
+    }
+    if (lens_inner == 0) {
+        hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, iz.p, z.p);
+    } else {
+        hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, ix.p, z.p);
+    }
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
 lt_status Solver::precondition(int precond, const double *dinv_p) {
     const unsigned gb = blocks_for(ntot, 256);
+    if (precond == LT_PC_LENS) return lens_apply(cur_act);
     if (precond != LT_PC_LEARNED) {
         hipLaunchKernelGGL(k_precond, dim3(gb), dim3(256), 0, stream(), ntot, dinv_p, r.p, z.p);
         LT_CHECK_LAUNCH();
@@ -719,7 +1002,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     const unsigned gb = blocks_for(ntot, 256), gs = blocks_for(S, 64);
     const bool mixed = (prec_mode == LT_F32);
     if (mixed) LT_TRY(prepare32());
-    if (precond != LT_PC_NONE && precond != LT_PC_JACOBI && precond != LT_PC_LEARNED) {
+    if (precond != LT_PC_NONE && precond != LT_PC_JACOBI && precond != LT_PC_LEARNED && precond != LT_PC_LENS) {
         set_error("solver: unknown preconditioner %d", precond);
         return LT_ERR_ARG;
     }
@@ -729,8 +1012,12 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         if (!dinv.p) LT_TRY(jacobi_setup());
         dinv_p = dinv.p;
     }
-    const bool flexible = (precond == LT_PC_LEARNED);
-    if (flexible) {
+    const bool learned = (precond == LT_PC_LEARNED);
+    const bool flexible = learned || precond == LT_PC_LENS;
+    if (precond == LT_PC_LENS) LT_TRY(lens_setup());
+    if (flexible && zap.n < (size_t)S) LT_TRY(zap.alloc(S));  // z . A p of the flexible beta
+    cur_act = nullptr;
+    if (learned) {
         LT_TRY(learned_setup());
         LT_TRY(nfall.zero());
         usenet_host.assign(S, 1);
@@ -745,10 +1032,11 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     std::vector<int> hflag(S), reflist(S);
     std::vector<double> rr_ref(S);
     DevBuf<int> dflag, dref;
-    if (flexible || mixed) LT_TRY(dflag.alloc(S));
+    if (learned || mixed) LT_TRY(dflag.alloc(S));
     if (mixed) LT_TRY(dref.alloc(S));
     std::vector<int> ones(S, 1), zeros(S, 0);
     LT_TRY(act.upload(ones.data(), S));
+    cur_act = act.p;
     LT_TRY(iters.upload(zeros.data(), S));
     LT_TRY(x.zero());
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, b.p, r.p);
@@ -828,13 +1116,13 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         }
         if (outer % check_every == 0 || it == maxiter - 1) {
             LT_TRY(act.download(hact.data(), S));
-            if (flexible) LT_TRY(rr.download(hrr_chk.data(), S));
+            if (learned) LT_TRY(rr.download(hrr_chk.data(), S));
             LT_TRY(sync());
             nact = 0;
             for (int s = 0; s < S; ++s)
                 if (hact[s]) alist[nact++] = s;
             if (nact > 0) LT_TRY(dlist.upload(alist.data(), nact));
-            if (flexible) {
+            if (learned) {
                 const size_t back = kWindow / check_every;
                 int nsw = 0;
                 for (int s = 0; s < S; ++s) {
@@ -885,7 +1173,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(iters.download(hit.data(), S));
     if (xout) LT_TRY(x.download(xout, ntot));
     nfall_host.assign(S, 0);
-    if (flexible) LT_TRY(nfall.download(nfall_host.data(), S));
+    if (learned) LT_TRY(nfall.download(nfall_host.data(), S));
     LT_TRY(sync());
     for (int s = 0; s < S; ++s) {
         if (it_out) it_out[s] = hit[s];
@@ -1062,6 +1350,34 @@ lt_status lt_solver_set_precision(lt_solver h, int precision, int refine_every) 
     h->s.refine_every = refine_every;
     if (precision == LT_F32) return h->s.prepare32();
     return LT_OK;
+}
+
+lt_status lt_solver_set_lens_pc(lt_solver h, int inner_steps, int max_row, double c) {
+    if (!h || inner_steps < 0 || inner_steps > 64 || max_row < 1) {
+        set_error("lt_solver_set_lens_pc: bad arguments");
+        return LT_ERR_ARG;
+    }
+    Solver &S = h->s;
+    S.lens_inner = inner_steps;
+    if (S.lens_maxrow != max_row || S.lens_c != c) S.lens_ready = false;
+    S.lens_maxrow = max_row;
+    S.lens_c = c;
+    return LT_OK;
+}
+
+lt_status lt_solver_lens_pc_matrix(lt_solver h, long *nnz, double *c_used, int *row_ptr, int *cols, double *vals) {
+    if (!h || !nnz) {
+        set_error("lt_solver_lens_pc_matrix: bad arguments");
+        return LT_ERR_ARG;
+    }
+    Solver &S = h->s;
+    LT_TRY(S.lens_setup());
+    *nnz = (long)S.Mpc.nnz;
+    if (c_used) *c_used = S.lens_c_used;
+    if (row_ptr) LT_TRY(S.Mpc.rp.download(row_ptr, S.ntot + 1));
+    if (cols && S.Mpc.nnz > 0) LT_TRY(S.Mpc.ci.download(cols, S.Mpc.nnz));
+    if (vals && S.Mpc.nnz > 0) LT_TRY(S.Mpc.va.download(vals, S.Mpc.nnz));
+    return sync();
 }
 
 lt_status lt_solver_set_rhs(lt_solver h, const double *rhs) {

@@ -169,6 +169,22 @@ class BatchedSourceInversion:
         B.check(B.lib().lt_solver_rhs(self._handle, B.ptr(out)))
         return self._split(out)
 
+    def set_lens_preconditioner(self, inner_steps=3, max_row=64, c=0.0):
+        B.check(B.lib().lt_solver_set_lens_pc(self._handle, int(inner_steps), int(max_row), float(c)))
+
+    def lens_preconditioner_matrix(self):
+        """The lens preconditioner's M = c L_s^T L_s + lambda Cc^T Cc over all
+        samples' source pixels: (scipy CSR matrix, c)."""
+        import scipy.sparse as sp
+        nnz, c = C.c_long(0), C.c_double(0)
+        B.check(B.lib().lt_solver_lens_pc_matrix(self._handle, C.byref(nnz), C.byref(c), None, None, None))
+        n = int(sum(self._sizes))
+        rp = np.zeros(n + 1, dtype=np.int32)
+        ci = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        va = np.zeros(max(nnz.value, 1), dtype=np.float64)
+        B.check(B.lib().lt_solver_lens_pc_matrix(self._handle, C.byref(nnz), None, B.ptr(rp), B.ptr(ci), B.ptr(va)))
+        return sp.csr_matrix((va[:nnz.value], ci[:nnz.value], rp), shape=(n, n)), c.value
+
     def set_rhs(self, bs):
         """Replace the right-hand sides (per-sample arrays)."""
         b = np.ascontiguousarray(np.concatenate([np.asarray(v).reshape(-1) for v in bs]), dtype=np.float64)
@@ -231,12 +247,15 @@ class BatchedSourceInversion:
 
     def solve(self, rtol=1e-6, maxiter=5000, precond='none', precision='f64', refine_every=20):
         """Returns (sources, iterations, true relative residuals).
-        precond: 'none' (plain CG), 'jacobi', or 'learned' (flexible CG with
-        the network given to set_preconditioner).  precision 'f32': the
+        precond: 'none' (plain CG), 'jacobi', 'learned' (flexible CG with
+        the network given to set_preconditioner) or 'lens' (flexible CG with
+        a few inner CG steps on lambda Cc^T Cc + c L_s^T L_s, the reference's
+        R + w L^T L preconditioner; see set_lens_preconditioner).  precision 'f32': the
         operator runs in FP32 inside the FP64 CG, with the FP64 true residual
         refreshed every `refine_every` iterations (the residuals returned are
         always FP64 true residuals)."""
-        pc = {'none': B.LT_PC_NONE, 'jacobi': B.LT_PC_JACOBI, 'learned': B.LT_PC_LEARNED}[precond]
+        pc = {'none': B.LT_PC_NONE, 'jacobi': B.LT_PC_JACOBI, 'learned': B.LT_PC_LEARNED,
+              'lens': B.LT_PC_LENS}[precond]
         B.check(B.lib().lt_solver_set_precision(self._handle, {'f64': B.LT_F64, 'f32': B.LT_F32}[precision],
                                                 int(refine_every)))
         n = len(self._sizes)
