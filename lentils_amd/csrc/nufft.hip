@@ -240,6 +240,153 @@ __global__ void __launch_bounds__(kNThreads) k_degrid(Geo G, const T *__restrict
     }
 }
 
+// The 2W+1 taps from the W+1 polynomials of taps 0..W, split into even and
+// odd parts in f: P_k(f) = E_k(f^2) + f O_k(f^2).  The kernel is even and
+// tap k sits at argument f + W - k, so tap 2W-k at f equals tap k at -f:
+// P_{2W-k}(f) = E_k(f^2) - f O_k(f^2).  Half the polynomial work of kb_taps;
+// the fit error of tap k at -f bounds the error of tap 2W-k.  No zeroing.
+template <class T, int W>
+__device__ inline void kb_taps_sym(const T *__restrict__ kc, T f, T (&out)[2 * W + 1]) {
+    constexpr int D = TapDeg<T, W>::D;
+    constexpr int DE = D / 2 * 2;              // highest even power
+    constexpr int DO = (D % 2) ? D : D - 1;    // highest odd power
+    const T s = f * f;
+#pragma unroll
+    for (int k = 0; k <= W; ++k) {
+        const T *c = kc + k * (D + 1);
+        T e = c[DE];
+#pragma unroll
+        for (int m = DE - 2; m >= 0; m -= 2) e = e * s + c[m];
+        T o = c[DO];
+#pragma unroll
+        for (int m = DO - 2; m >= 1; m -= 2) o = o * s + c[m];
+        out[k] = e + f * o;
+        if (k < W) out[2 * W - k] = e - f * o;
+    }
+}
+
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+
+// Degridder for tiles of exactly 32 cells in v (the window row stride WV =
+// 32 + 2W is then a compile-time constant, so every window read of a
+// visibility is one LDS base address plus an immediate offset).  A
+// visibility with f != 0 on both axes has exactly 2W nonzero taps per axis
+// (tap 0 is outside the support for f > 0, tap 2W for f < 0): those are
+// selected and the 2W x 2W window product is formed.  Waves holding a
+// visibility with f == 0 on an axis (all 2W+1 taps inside the support) take
+// the full (2W+1)^2 product.  Same results as k_degrid up to rounding.
+template <class T, int W>
+__global__ void __launch_bounds__(kNThreads) k_degrid32(Geo G, const T *__restrict__ kc,
+                                                        const Cplx<T> *__restrict__ grid,
+                                                        const int32_t *__restrict__ ck_tile,
+                                                        const int32_t *__restrict__ ck_beg,
+                                                        const int32_t *__restrict__ ck_end,
+                                                        const int32_t *__restrict__ ngp, const T *__restrict__ frac,
+                                                        const T *__restrict__ ph, const double *__restrict__ wts,
+                                                        Cplx<T> *__restrict__ vis, int accumulate) {
+    extern __shared__ __align__(16) unsigned char smem_d32[];
+    Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem_d32);
+    // window reads as one aligned 2-vector (one LDS instruction with an
+    // immediate offset per tap)
+    using V = typename Vec2<T>::type;
+    const V *wv = reinterpret_cast<const V *>(smem_d32);
+    constexpr int NT = 2 * W + 1;
+    constexpr int NF = 2 * W;
+    constexpr int WV = 32 + 2 * W;
+    const int c = blockIdx.x;
+    const int tile = ck_tile[c];
+    const int tu = tile / G.ntv, tv = tile - tu * G.ntv;
+    const int a0 = tu * G.TU - W - G.cu, b0 = tv * 32 - W - G.cv;
+    const int nwin = G.WU * WV;
+    for (int i = threadIdx.x; i < nwin; i += kNThreads) {
+        const int wa = i / WV, wb = i - wa * WV;
+        win[i] = fetch_folded(G, grid, a0 + wa, b0 + wb);
+    }
+    __syncthreads();
+    const int beg = ck_beg[c], end = ck_end[c];
+    const T c0 = (T)(1.0 / (4.0 * W * W));
+    for (int p0 = beg; p0 < end; p0 += kNThreads) {
+        const int p = p0 + threadIdx.x;
+        const bool live = p < end;
+        const int q = live ? p : beg;  // idle lanes read a valid entry, store nothing
+        int na, nb;
+        unpack_ngp((uint32_t)ngp[q], na, nb);
+        const T fu = frac[2 * (long)q], fv = frac[2 * (long)q + 1];
+        T ku[NT], kv[NT];
+        kb_taps_sym<T, W>(kc, fu, ku);
+        kb_taps_sym<T, W>(kc, fv, kv);
+        const int r0 = na - a0 - W, s0 = nb - b0 - W;
+        T ar = 0, ai = 0;
+        if (__all(fu != (T)0 && fv != (T)0)) {
+            const int du = fu > (T)0 ? 1 : 0, dv = fv > (T)0 ? 1 : 0;
+            T ku6[NF], kv6[NF];
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                ku6[j] = du ? ku[j + 1] : ku[j];
+                kv6[j] = dv ? kv[j + 1] : kv[j];
+            }
+            const V *base = wv + (r0 + du) * WV + s0 + dv;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                T rr = 0, ri = 0;
+#pragma unroll
+                for (int l = 0; l < NF; ++l) {
+                    const V g = base[k * WV + l];
+                    rr += kv6[l] * g.x;
+                    ri += kv6[l] * g.y;
+                }
+                ar += ku6[k] * rr;
+                ai += ku6[k] * ri;
+            }
+        } else {
+            if (fu > (T)0) ku[0] = (T)0;
+            if (fu < (T)0) ku[NT - 1] = (T)0;
+            if (fv > (T)0) kv[0] = (T)0;
+            if (fv < (T)0) kv[NT - 1] = (T)0;
+            const V *base = wv + r0 * WV + s0;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                T rr = 0, ri = 0;
+#pragma unroll
+                for (int l = 0; l < NT; ++l) {
+                    const V g = base[k * WV + l];
+                    rr += kv[l] * g.x;
+                    ri += kv[l] * g.y;
+                }
+                ar += ku[k] * rr;
+                ai += ku[k] * ri;
+            }
+        }
+        if (!live) continue;
+        T cs = (T)1, sn = (T)0;
+        if (ph) {
+            cs = ph[2 * (long)p];
+            sn = ph[2 * (long)p + 1];
+        }
+        T outr = c0 * (cs * ar + sn * ai);
+        T outi = c0 * (-sn * ar + cs * ai);
+        if (wts) {
+            const T w = (T)wts[p];
+            outr *= w;
+            outi *= w;
+        }
+        if (accumulate) {
+            outr += vis[p].re;
+            outi += vis[p].im;
+        }
+        vis[p] = Cplx<T>{outr, outi};
+    }
+}
+
 __device__ inline int gcd_i(int a, int b) {
     while (b) {
         int t = a % b;
@@ -1146,9 +1293,14 @@ lt_status launch_degrid(NufftPlan &P, const void *grid, void *vis, const double 
     const T *fr = P.prec == LT_F64 ? (const T *)P.frac64.p : (const T *)P.frac32.p;
 #define LT_DEGRID_CASE(WW)                                                                                        \
     case WW:                                                                                                      \
-        hipLaunchKernelGGL((k_degrid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, kc, \
-                           (const Cplx<T> *)grid, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph, wts,       \
-                           (Cplx<T> *)vis, acc ? 1 : 0);                                                          \
+        if (P.TV == 32)                                                                                           \
+            hipLaunchKernelGGL((k_degrid32<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, \
+                               kc, (const Cplx<T> *)grid, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph,   \
+                               wts, (Cplx<T> *)vis, acc ? 1 : 0);                                                 \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_degrid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G,   \
+                               kc, (const Cplx<T> *)grid, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph,   \
+                               wts, (Cplx<T> *)vis, acc ? 1 : 0);                                                 \
         break;
     switch (P.W) {
         LT_DEGRID_CASE(1)

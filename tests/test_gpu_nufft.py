@@ -85,6 +85,34 @@ def test_grid_degrid_c1_sizes(hip, oracle, prec, tol):
     B.lib().lt_nufft_destroy(h)
 
 
+@pytest.mark.parametrize('prec,tol', [(0, 1e-12), (1, 1e-5)])
+def test_degrid_on_grid_points(hip, oracle, prec, tol):
+    """Visibilities exactly on grid lines (fractional offset 0 on an axis: all
+    2W+1 taps inside the support) mixed with general ones, in the same tiles
+    and waves: the degridder's f == 0 path and its 2W-tap path both match the
+    oracle.  The geometry is exact in binary (dx = 1/64, du = 1/4)."""
+    B, O = hip, oracle
+    g = rng(67)
+    n = 60000
+    uv = np.zeros((n, 3))
+    uv[:, :2] = g.uniform(-30.0, 30.0, size=(n, 2))
+    on_u = g.random(n) < 0.3
+    on_v = g.random(n) < 0.3
+    uv[on_u, 0] = np.round(uv[on_u, 0] * 4.0) / 4.0
+    uv[on_v, 1] = np.round(uv[on_v, 1] * 4.0) / 4.0
+    assert np.sum(on_u & on_v) > 1000
+    h, keep = _plan(B, uv, [1.0], 128, 128, [(-1.0, 1.0), (-1.0, 1.0)], 3, 14.04, prec)
+    geo = O.NUFFTGeometry(O.Grid2D(128, 128, [-1.0, 1.0, -1.0, 1.0]), pad=2, wsup=3, beta=14.04)
+    grid = np.ascontiguousarray(g.normal(size=(256, 129, 2)).view(np.complex128)[..., 0])
+    vis = np.zeros(n, dtype=np.complex128)
+    B.check(B.lib().lt_nufft_degrid_host(h, B.ptr(grid), B.ptr(vis)))
+    ref = O.grid(uv, [1.0], None, geo, 0, grid_in=grid, nthreads=8).reshape(-1)
+    assert O.max_relative_error(vis, ref) < tol
+    for sel in (on_u | on_v, ~(on_u | on_v)):
+        assert np.max(np.abs(vis[sel] - ref[sel])) < tol * np.max(np.abs(ref))
+    B.lib().lt_nufft_destroy(h)
+
+
 def test_grid_cpu_drop_in_symbol(hip, oracle):
     from lentils_amd import FourierSpace, ImageSpace, VisibilitySpace
     B, O = hip, oracle
