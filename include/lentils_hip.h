@@ -212,6 +212,18 @@ lt_status lt_nufft_info(lt_nufft h, int *nu, int *nv, int *half_nv, double *kb_b
  * with HIP events on the plan's stream; returns the mean per-launch time. */
 lt_status lt_nufft_time(lt_nufft h, int direction, int reps, double *ms_per_launch);
 lt_status lt_nufft_destroy(lt_nufft h);
+/* No reference counterpart (the reference passes the data object's sigma to
+ * every operator): keeps the per-visibility weights (nchannels*nrows
+ * float64, caller order) resident in the plan, and for FP64 plans with
+ * kernel_support <= 4 builds the normal stencil D^T diag(w) D from them
+ * once.  Solvers created with weights == NULL share both. */
+lt_status lt_nufft_set_weights(lt_nufft h, const double *weights);
+
+/* Device memory for resident inputs (plain hipMalloc'd bytes). */
+lt_status lt_device_alloc(long nbytes, void **dptr);
+lt_status lt_device_upload(void *dptr, const void *host, long nbytes);
+lt_status lt_device_download(const void *dptr, void *host, long nbytes);
+lt_status lt_device_free(void *dptr);
 
 /* No reference counterpart: the per-tap kernel polynomials the gridder
  * evaluates instead of kb_g (nufft.c:11-21).  Tap k of 2W+1 at fractional
@@ -237,8 +249,9 @@ lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, do
 #define LT_PC_JACOBI 1
 #define LT_PC_LEARNED 2 /* learned network (lt_solver_set_pcnet) in flexible CG, Jacobi as its guard */
 #define LT_PC_LENS 3    /* M = lambda_c Cc^T Cc + c L_s^T L_s (the reference's R + w L^T L preconditioner,
-                           test_all.py:63-65; L_s: L without rows above 256 entries), applied by a few
-                           Jacobi-PCG steps on M inside flexible CG (lt_solver_set_lens_pc) */
+                           test_all.py:63-65; L_s: L without its longest rows), built explicitly and
+                           applied by a few Jacobi-PCG steps on M inside flexible CG
+                           (lt_solver_set_lens_pc) */
 lt_status lt_solver_create(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *data, double lambda_curv,
                            double lambda_tikh, int op_mode, lt_solver *out);
 /* As lt_solver_create, with the data given through their dirty images
@@ -247,6 +260,14 @@ lt_status lt_solver_create(lt_lensbatch lb, lt_nufft nf, const double *weights, 
  * every sample its own (one observation per lens model, same uv coverage). */
 lt_status lt_solver_create_dirty(lt_lensbatch lb, lt_nufft nf, const double *weights, const double *dirty,
                                  int ndirty, double lambda_curv, double lambda_tikh, int op_mode, lt_solver *out);
+/* weights == NULL in lt_solver_create / lt_solver_create_dirty: the plan's
+ * resident weights and stencil (lt_nufft_set_weights; not for LT_OP_BCB).
+ * lt_solver_create_resident: as lt_solver_create_dirty with the plan's
+ * weights and the dirty images already in device memory (d_dirty: ndirty
+ * x nx*ny float64, e.g. from lt_device_alloc), so that nothing crosses
+ * PCIe but the lens parameters. */
+lt_status lt_solver_create_resident(lt_lensbatch lb, lt_nufft nf, const double *d_dirty, int ndirty,
+                                   double lambda_curv, double lambda_tikh, int op_mode, lt_solver *out);
 /* CG to ||b - A x|| / ||b|| <= rtol (FP64 true residual).  x: host array
  * of all samples' sources concatenated (may be NULL); iters/resid: per
  * sample (may be NULL). */

@@ -103,6 +103,12 @@ def lib():
         'lt_nufft_time': ([vp, i, i, dp], i),
         'lt_kb_taps': ([d, i, i, vp, dp], i),
         'lt_nufft_destroy': ([vp], i),
+        'lt_nufft_set_weights': ([vp, vp], i),
+        'lt_device_alloc': ([l, C.POINTER(vp)], i),
+        'lt_device_upload': ([vp, vp, l], i),
+        'lt_device_download': ([vp, vp, l], i),
+        'lt_device_free': ([vp], i),
+        'lt_solver_create_resident': ([vp, vp, vp, i, d, d, i, C.POINTER(vp)], i),
         'lt_solver_create': ([vp, vp, vp, vp, d, d, i, C.POINTER(vp)], i),
         'lt_solver_create_dirty': ([vp, vp, vp, vp, i, d, d, i, C.POINTER(vp)], i),
         'lt_pcnet_param_count': ([], l),
@@ -146,7 +152,9 @@ def exported_symbols():
             'lt_lens_matrix_nnz', 'lt_lens_matrix_csr', 'lt_deflect_lattice', 'lt_lens_matrix_from_lattice',
             'lt_lensbatch_create', 'lt_lensbatch_info', 'lt_lensbatch_download', 'lt_lensbatch_destroy',
             'lt_nufft_create', 'lt_nufft_forward_host', 'lt_nufft_adjoint_host', 'lt_nufft_degrid_host',
-            'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_kb_taps',
+            'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_nufft_set_weights',
+            'lt_device_alloc', 'lt_device_upload', 'lt_device_download', 'lt_device_free', 'lt_solver_create_resident',
+            'lt_kb_taps',
             'lt_solver_create', 'lt_solver_create_dirty', 'lt_pcnet_param_count', 'lt_pcnet_create',
             'lt_pcnet_get_params', 'lt_pcnet_set_params', 'lt_pcnet_forward_host', 'lt_pcnet_grad_host',
             'lt_pcnet_train', 'lt_pcnet_eval', 'lt_pcnet_destroy', 'lt_solver_set_pcnet', 'lt_solver_diag', 'lt_solver_train_pcnet',

@@ -15,6 +15,8 @@
 #include <rocprim/rocprim.hpp>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cmath>
 #include <memory>
 #include <random>
@@ -29,19 +31,29 @@ lt_status build_curvature(int nx, int ny, double dx, double dy, int32_t coloff, 
 LensBatch *lensbatch_of(lt_lensbatch h);
 NufftPlan *nufft_of(lt_nufft h);
 lt_status grid_c128_to_plan(NufftPlan &P, const double *src, void *dst);
+__global__ void k_gather_weights(long n, const int32_t *__restrict__ perm, const double *__restrict__ w_in,
+                                 double *__restrict__ w_out);
 
 namespace {
 
 constexpr int kDotThreads = 256;
 
-// out[s] = sum over segment s of a*b (a fixed-order block reduction)
-__global__ void __launch_bounds__(kDotThreads) k_seg_dot(const int64_t *__restrict__ off, const double *__restrict__ a,
-                                                          const double *__restrict__ b, double *__restrict__ out) {
+// Segmented dot products out[s] = sum over segment s of a*b in two passes:
+// gridDim.x parts per segment each reduce a contiguous slice to part[s][p]
+// (fixed-order block reduction), then one thread per segment adds its parts
+// in order.  The order depends only on the segment bounds and the part
+// count, so results are run-to-run identical.
+__global__ void __launch_bounds__(kDotThreads) k_seg_dot_part(const int64_t *__restrict__ off,
+                                                               const double *__restrict__ a,
+                                                               const double *__restrict__ b,
+                                                               double *__restrict__ part) {
     __shared__ double sh[kDotThreads / 64];
-    const int s = blockIdx.x;
-    const long b0 = off[s], b1 = off[s + 1];
+    const int s = blockIdx.y, P = gridDim.x, q = blockIdx.x;
+    const long b0 = off[s], len = off[s + 1] - b0;
+    const long per = (len + P - 1) / P;
+    const long lo = b0 + min(len, per * q), hi = b0 + min(len, per * (q + 1));
     double acc = 0.0;
-    for (long i = b0 + threadIdx.x; i < b1; i += kDotThreads) acc += a[i] * b[i];
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) acc += a[i] * b[i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
@@ -49,7 +61,28 @@ __global__ void __launch_bounds__(kDotThreads) k_seg_dot(const int64_t *__restri
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int k = 0; k < kDotThreads / 64; ++k) t += sh[k];
-        out[s] = t;
+        part[(long)s * P + q] = t;
+    }
+}
+
+__global__ void k_seg_dot_fin(int S, int P, const double *__restrict__ part, double *__restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    double t = 0.0;
+    for (int q = 0; q < P; ++q) t += part[(long)s * P + q];
+    out[s] = t;
+}
+
+// copy a CSR block into a block-diagonal matrix at (row0, col0), its
+// entries starting at nz0
+__global__ void k_csr_place(long nrows, long nnz, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                            const double *__restrict__ va, long row0, long nz0, int32_t col0,
+                            int32_t *__restrict__ orp, int32_t *__restrict__ oci, double *__restrict__ ova) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrows) orp[row0 + i] = (int32_t)(rp[i] + nz0);
+    if (i < nnz) {
+        oci[nz0 + i] = ci[i] + col0;
+        ova[nz0 + i] = va[i];
     }
 }
 
@@ -354,29 +387,41 @@ __global__ void k_key_flags(long n, const uint64_t *__restrict__ key, int64_t *_
     if (i < n) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
 }
 
+// position of the first pair of every distinct (row, col)
+__global__ void k_key_heads(long n, const int64_t *__restrict__ flag, const int64_t *__restrict__ gpos,
+                            int64_t *__restrict__ hpos) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) hpos[gpos[i]] = i;
+}
+
 // one thread per distinct (row, col): the in-order sum of its pairs
-__global__ void k_key_sum(long n, int nb, const uint64_t *__restrict__ key, const double *__restrict__ val,
-                          const int64_t *__restrict__ gpos, int32_t *__restrict__ col, double *__restrict__ out,
-                          int32_t *__restrict__ rowcnt) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || gpos[i + 1] == gpos[i]) return;
-    const uint64_t k = key[i];
+__global__ void k_group_pairs(long nz, int nb, const uint64_t *__restrict__ key, const double *__restrict__ val,
+                              const int64_t *__restrict__ hpos, int32_t *__restrict__ col, double *__restrict__ out,
+                              int32_t *__restrict__ urow) {
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nz) return;
+    const long h0 = hpos[g], h1 = hpos[g + 1];
     double s = 0.0;
-    for (long j = i; j < n && key[j] == k; ++j) s += val[j];
-    const long g = gpos[i];
+    for (long j = h0; j < h1; ++j) s += val[j];
+    const uint64_t k = key[h0];
     col[g] = (int32_t)(k & ((1ull << nb) - 1));
+    urow[g] = (int32_t)(k >> nb);
     out[g] = s;
-    atomicAdd(&rowcnt[(long)(k >> nb)], 1);  // a count: the same for any order
 }
 
-__global__ void k_i32_to_i64(long n, const int32_t *__restrict__ a, int64_t *__restrict__ b) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = a[i];
-}
-
-__global__ void k_i64_to_i32_rp(long n, const int64_t *__restrict__ a, int32_t *__restrict__ b) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= n) b[i] = (int32_t)a[i];
+// row pointers of the sorted distinct entries: rp[r] = first entry of row >= r
+__global__ void k_rowptr_search(long nrows, long nz, const int32_t *__restrict__ urow, int32_t *__restrict__ rp) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nrows) return;
+    long lo = 0, hi = nz;
+    while (lo < hi) {
+        const long mid = (lo + hi) >> 1;
+        if (urow[mid] < r)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    rp[r] = (int32_t)lo;
 }
 
 __global__ void k_csr_diag(long n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -462,7 +507,8 @@ struct Solver {
     DevBuf<int64_t> soff;
     DevBuf<int32_t> seg;
     DevCSR C;                 // block-diagonal curvature of every source grid
-    DevBuf<double> w;         // visibility weights in plan order
+    DevBuf<double> w;         // visibility weights in plan order (own copy)
+    const double *wp = nullptr;  // the weights used: w.p, or the plan's resident ones
     DevBuf<double> dirty;     // D^T C d_s (ndirty x nx*ny)
     int ndirty = 1;           // 1: one data set shared by every sample; S: one per sample
     DevBuf<double> beamspec;  // BCB multiplier, nu x half_nv complex
@@ -470,7 +516,8 @@ struct Solver {
     DevBuf<double> rz, rz_new, pAp, rr, bb, alpha, beta;
     DevBuf<int> act, iters, all_active;
     double beam_peak = 0.0;
-    NormalStencil stencil;
+    NormalStencil own_stencil;
+    NormalStencil *stp = nullptr;  // &own_stencil, or the plan's shared stencil
     bool use_stencil = false;
     // precision form: FP32 operator (SpMV, FFT, stencil) inside FP64 CG,
     // true FP64 residual refreshed every refine_every iterations
@@ -497,8 +544,21 @@ struct Solver {
     DevBuf<int> usenet;            // 1: network direction, 0: plain CG fallback
     std::vector<int> nfall_host, usenet_host;
 
+    DevBuf<double> dotpart;  // S x dot_parts partial sums of seg_dot
+    int dot_parts = 1;
+    // out[s] = a_s . b_s for every sample s (deterministic order)
+    lt_status seg_dot(const double *a, const double *b, double *out) {
+        hipLaunchKernelGGL(k_seg_dot_part, dim3(dot_parts, S), dim3(kDotThreads), 0, stream(), soff.p, a, b,
+                           dotpart.p);
+        LT_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_seg_dot_fin, dim3(blocks_for(S, 64)), dim3(64), 0, stream(), S, dot_parts, dotpart.p,
+                           out);
+        LT_CHECK_LAUNCH();
+        return LT_OK;
+    }
     lt_status create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data,
-                     const double *dirty_in, int ndirty_in, double lc, double lt, int op_mode);
+                     const double *dirty_in, int ndirty_in, double lc, double lt, int op_mode,
+                     bool dirty_on_device = false);
     // y = A x for the samples in active_list; with actflags (per-sample 0/1)
     // the sparse products skip the rows of inactive samples as well
     lt_status apply(const double *xin, double *yout, const int *active_list, int nactive,
@@ -524,7 +584,8 @@ lt_status beam_spectrum(Solver &S, const double *weights_caller_order);
 }  // namespace
 
 lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, const double *data,
-                         const double *dirty_in, int ndirty_in, double lc, double lt_, int op_mode) {
+                         const double *dirty_in, int ndirty_in, double lc, double lt_, int op_mode,
+                         bool dirty_on_device) {
     lb = lb_;
     nf = nf_;
     S = lb->nsamples;
@@ -548,61 +609,87 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     soff_host = lb->srcoff_host;
     ntot = soff_host[S];
     LT_TRY(soff.upload(soff_host.data(), S + 1));
+    dot_parts = std::max(1, std::min(64, (1024 + S - 1) / S));
+    LT_TRY(dotpart.alloc((long)S * dot_parts));
     LT_TRY(seg.alloc(ntot));
     hipLaunchKernelGGL(k_seg_ids, dim3(64, S), dim3(256), 0, stream(), soff.p, S, seg.p);
     LT_CHECK_LAUNCH();
     LT_TRY(lb->ensure_transpose());
-    // block-diagonal curvature matrix
+    // block-diagonal curvature matrix: one block per distinct source grid,
+    // placed at every sample's row / column offset
     {
-        std::vector<DevCSR> parts(S);
-        long nnz = 0;
+        std::map<std::tuple<int, int, double, double>, int> uniq;
+        std::vector<DevCSR> blocks;
+        std::vector<int> bsel(S);
+        std::vector<long> nzoff(S + 1, 0);
         for (int s = 0; s < S; ++s) {
             const SrcGrid &g = lb->src_host[s];
-            LT_TRY(build_curvature(g.nx, g.ny, g.dx, g.dy, (int32_t)soff_host[s], parts[s]));
-            nnz += parts[s].nnz;
+            const auto key = std::make_tuple(g.nx, g.ny, g.dx, g.dy);
+            auto it = uniq.find(key);
+            if (it == uniq.end()) {
+                blocks.emplace_back();
+                LT_TRY(build_curvature(g.nx, g.ny, g.dx, g.dy, 0, blocks.back()));
+                it = uniq.emplace(key, (int)blocks.size() - 1).first;
+            }
+            bsel[s] = it->second;
+            nzoff[s + 1] = nzoff[s] + blocks[bsel[s]].nnz;
         }
+        const long nnz = nzoff[S];
         C.nrows = C.ncols = ntot;
         C.nnz = nnz;
         LT_TRY(C.rp.alloc(ntot + 1));
-        LT_TRY(C.ci.alloc(nnz));
-        LT_TRY(C.va.alloc(nnz));
-        long ro = 0, no = 0;
+        LT_TRY(C.ci.alloc(std::max<long>(nnz, 1)));
+        LT_TRY(C.va.alloc(std::max<long>(nnz, 1)));
         for (int s = 0; s < S; ++s) {
-            std::vector<int32_t> rp(parts[s].nrows + 1);
-            LT_TRY(parts[s].rp.download(rp.data(), rp.size()));
-            LT_TRY(sync());
-            for (auto &v : rp) v += (int32_t)no;
-            LT_HIP(hipMemcpy(C.rp.p + ro, rp.data(), rp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-            LT_HIP(hipMemcpyAsync(C.ci.p + no, parts[s].ci.p, parts[s].nnz * sizeof(int32_t),
-                                  hipMemcpyDeviceToDevice, stream()));
-            LT_HIP(hipMemcpyAsync(C.va.p + no, parts[s].va.p, parts[s].nnz * sizeof(double), hipMemcpyDeviceToDevice,
-                                  stream()));
-            LT_TRY(sync());
-            ro += parts[s].nrows;
-            no += parts[s].nnz;
+            const DevCSR &B = blocks[bsel[s]];
+            const long n = std::max<long>(B.nrows, B.nnz);
+            if (n == 0) continue;
+            hipLaunchKernelGGL(k_csr_place, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), B.nrows, B.nnz, B.rp.p,
+                               B.ci.p, B.va.p, soff_host[s], nzoff[s], (int32_t)soff_host[s], C.rp.p, C.ci.p, C.va.p);
+            LT_CHECK_LAUNCH();
         }
+        const int32_t last = (int32_t)nnz;
+        LT_HIP(hipMemcpyAsync(C.rp.p + ntot, &last, sizeof(int32_t), hipMemcpyHostToDevice, stream()));
+        LT_TRY(sync());
         LT_TRY(csr_prepare(C));
     }
     // weights and data into plan order; dirty image D^T C d
     const long nv = nf->nvis;
+    const bool shared_w = (weights == nullptr);
+    if (shared_w && nv > 0 && nf->wplan.n < (size_t)nv) {
+        set_error("solver: no weights given and none resident in the NUFFT plan (lt_nufft_set_weights)");
+        return LT_ERR_ARG;
+    }
+    if (shared_w && mode == LT_OP_BCB) {
+        set_error("solver: the BCB operator needs the weights argument");
+        return LT_ERR_ARG;
+    }
     {
         DevBuf<double> dc;
-        LT_TRY(w.alloc(std::max<long>(nv, 1)));
-        if (nv) {
-            std::vector<int32_t> hp(nv);
-            LT_TRY(nf->perm.download(hp.data(), nv));
-            LT_TRY(sync());
-            std::vector<double> hs(nv);
-            for (long q = 0; q < nv; ++q) hs[q] = weights[hp[q]];
-            LT_TRY(w.upload(hs.data(), nv));
+        if (shared_w) {
+            wp = nf->wplan.p;
+        } else {
+            LT_TRY(w.alloc(std::max<long>(nv, 1)));
+            if (nv) {
+                DevBuf<double> raw;
+                LT_TRY(raw.upload(weights, nv));
+                hipLaunchKernelGGL(k_gather_weights, dim3(blocks_for(nv, 256)), dim3(256), 0, stream(), nv,
+                                   nf->perm.p, raw.p, w.p);
+                LT_CHECK_LAUNCH();
+                LT_TRY(sync());
+            }
+            wp = w.p;
         }
         LT_TRY(dirty.alloc(npix * ndirty));
-        if (dirty_in) {
+        if (dirty_in && dirty_on_device) {
+            LT_HIP(hipMemcpyAsync(dirty.p, dirty_in, npix * ndirty * sizeof(double), hipMemcpyDeviceToDevice,
+                                  stream()));
+        } else if (dirty_in) {
             LT_TRY(dirty.upload(dirty_in, npix * ndirty));
         } else if (nv) {
             LT_TRY(dc.upload(data, 2 * nv));
             LT_TRY(nf->to_sorted(dc.p, nf->visbuf.p));
-            LT_TRY(nf->adjoint(nf->visbuf.p, w.p, dirty.p, 1.0, false));
+            LT_TRY(nf->adjoint(nf->visbuf.p, wp, dirty.p, 1.0, false));
         } else {
             LT_TRY(dirty.zero());
         }
@@ -610,8 +697,13 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     LT_TRY(nf->ensure_batch(S));
     if (mode == LT_OP_BCB) LT_TRY(beam_spectrum(*this, weights));
     if (mode == LT_OP_NUFFT && nf->prec == LT_F64 && nf->W <= 4 && nv > 0) {
-        LT_TRY(stencil.build(*nf, w.p));
-        use_stencil = true;
+        if (shared_w) {
+            stp = nf->wstencil.get();
+        } else {
+            LT_TRY(own_stencil.build(*nf, w.p));
+            stp = &own_stencil;
+        }
+        use_stencil = stp != nullptr;
     }
     // work vectors
     LT_TRY(b.alloc(ntot));
@@ -634,9 +726,10 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     LT_TRY(csr_autotune(lb->L, false));
     LT_TRY(csr_autotune(lb->LT, false));
     LT_TRY(csr_autotune(C, false));
-    if (use_stencil) {
+    if (use_stencil && !(shared_w && nf->wstencil_tuned_batch == S)) {
         LT_TRY(img1.zero());
-        LT_TRY(stencil_autotune(*nf, stencil, img1.p, img2.p, all_active.p, S));
+        LT_TRY(stencil_autotune(*nf, *stp, img1.p, img2.p, all_active.p, S));
+        if (shared_w) nf->wstencil_tuned_batch = S;
     }
     // rhs b_s = L_s^T dirty_s (a shared dirty image is replicated)
     for (int s = 0; s < S; ++s)
@@ -649,7 +742,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
 lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact, const int *af) {
     LT_TRY(csr_spmv(lb->L, xin, img1.p, 1.0, 0.0, 0, af, npix, nullptr));
     if (mode != LT_OP_BCB)
-        LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, alist, nact, 1.0, false, use_stencil ? &stencil : nullptr));
+        LT_TRY(nf->normal_batched(img1.p, img2.p, wp, alist, nact, 1.0, false, use_stencil ? stp : nullptr));
     else
         LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, alist, nact, 1.0, false));
     LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0, af, 0, seg.p));
@@ -711,7 +804,7 @@ lt_status Solver::lens_setup() {
         const double one = 1.0;
         LT_HIP(hipMemcpyAsync(img1.p + ctr, &one, sizeof(double), hipMemcpyHostToDevice, stream()));
         if (mode != LT_OP_BCB)
-            LT_TRY(nf->normal_batched(img1.p, img2.p, w.p, all_active.p, 1, 1.0, false, use_stencil ? &stencil : nullptr));
+            LT_TRY(nf->normal_batched(img1.p, img2.p, wp, all_active.p, 1, 1.0, false, use_stencil ? stp : nullptr));
         else
             LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, all_active.p, 1, 1.0, false));
         LT_TRY(img2.download(&c, 1, ctr));
@@ -782,19 +875,17 @@ lt_status Solver::lens_build_matrix() {
     LT_TRY(Mpc.ci.alloc(std::max<int64_t>(nz, 1)));
     LT_TRY(Mpc.va.alloc(std::max<int64_t>(nz, 1)));
     LT_TRY(Mpc.rp.alloc(ntot + 1));
-    DevBuf<int32_t> rowcnt;
-    DevBuf<int64_t> rc64, rp64;
-    LT_TRY(rowcnt.alloc(ntot));
-    LT_TRY(rowcnt.zero());
-    hipLaunchKernelGGL(k_key_sum, dim3(blocks_for(np, 256)), dim3(256), 0, stream(), np, nb, k2.p, v2.p, gpos.p,
-                       Mpc.ci.p, Mpc.va.p, rowcnt.p);
+    DevBuf<int64_t> hpos;
+    DevBuf<int32_t> urow;
+    LT_TRY(hpos.alloc(nz + 1));
+    LT_TRY(urow.alloc(std::max<int64_t>(nz, 1)));
+    hipLaunchKernelGGL(k_key_heads, dim3(blocks_for(np, 256)), dim3(256), 0, stream(), np, flag.p, gpos.p, hpos.p);
     LT_CHECK_LAUNCH();
-    LT_TRY(rc64.alloc(ntot));
-    LT_TRY(rp64.alloc(ntot + 1));
-    hipLaunchKernelGGL(k_i32_to_i64, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, rowcnt.p, rc64.p);
+    LT_HIP(hipMemcpyAsync(hpos.p + nz, &np, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+    hipLaunchKernelGGL(k_group_pairs, dim3(blocks_for(nz, 256)), dim3(256), 0, stream(), nz, nb, k2.p, v2.p, hpos.p,
+                       Mpc.ci.p, Mpc.va.p, urow.p);
     LT_CHECK_LAUNCH();
-    LT_TRY(exclusive_scan_i64(rc64.p, rp64.p, ntot));
-    hipLaunchKernelGGL(k_i64_to_i32_rp, dim3(blocks_for(ntot + 1, 256)), dim3(256), 0, stream(), ntot, rp64.p,
+    hipLaunchKernelGGL(k_rowptr_search, dim3(blocks_for(ntot + 1, 256)), dim3(256), 0, stream(), ntot, nz, urow.p,
                        Mpc.rp.p);
     LT_CHECK_LAUNCH();
     LT_TRY(csr_prepare(Mpc));
@@ -810,17 +901,17 @@ lt_status Solver::lens_apply(const int *af) {
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, r.p, ir.p);
     hipLaunchKernelGGL(k_divide, dim3(gb), dim3(256), 0, stream(), ntot, ir.p, dM.p, iz.p);
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, iz.p, ip.p);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, ir.p, iz.p, irz.p);
+    LT_TRY(seg_dot(ir.p, iz.p, irz.p));
     LT_CHECK_LAUNCH();
     for (int k = 0; k < lens_inner; ++k) {
         // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc
         LT_TRY(csr_spmv(Mpc, ip.p, iMp.p, 1.0, 0.0, 0, af, 0, seg.p));
-        hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, ip.p, iMp.p, ipMp.p);
+        LT_TRY(seg_dot(ip.p, iMp.p, ipMp.p));
         hipLaunchKernelGGL(k_inner_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, irz.p, ipMp.p, ip.p, iMp.p,
                            dM.p, ix.p, ir.p, iz.p);
         LT_CHECK_LAUNCH();
         if (k + 1 < lens_inner) {
-            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, ir.p, iz.p, irz2.p);
+            LT_TRY(seg_dot(ir.p, iz.p, irz2.p));
             hipLaunchKernelGGL(k_inner_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, irz.p, irz2.p, iz.p, ip.p);
             hipLaunchKernelGGL(k_copy, dim3(blocks_for(S, 64)), dim3(64), 0, stream(), (long)S, irz2.p, irz.p);
             LT_CHECK_LAUNCH();
@@ -851,7 +942,7 @@ lt_status Solver::precondition(int precond, const double *dinv_p) {
     LT_TRY(net->forward(S));
     hipLaunchKernelGGL(k_pc_out, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, net->act[PcNet::Y].p, rr.p,
                        ainv.p, HW, z.p);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz_new.p);
+    LT_TRY(seg_dot(r.p, z.p, rz_new.p));
     hipLaunchKernelGGL(k_pc_guard, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rz_new.p, dinv_p, r.p, z.p,
                        nfall.p);
     hipLaunchKernelGGL(k_pc_plain, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, usenet.p, r.p, z.p);
@@ -886,7 +977,7 @@ lt_status Solver::learned_setup() {
     LT_TRY(usenet.alloc(S));
     hipLaunchKernelGGL(k_recip, dim3(gb), dim3(256), 0, stream(), ntot, dinv.p, tmp.p);  // diag
     hipLaunchKernelGGL(k_fill, dim3(gb), dim3(256), 0, stream(), ntot, 1.0, onesv.p);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, tmp.p, onesv.p, dsum.p);
+    LT_TRY(seg_dot(tmp.p, onesv.p, dsum.p));
     hipLaunchKernelGGL(k_pc_dhat, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, tmp.p, dsum.p, dhat.p);
     LT_CHECK_LAUNCH();
     std::vector<double> hs(S), hinv(S);
@@ -922,7 +1013,7 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
         double acc = 0.0;
         for (int q = 0; q < nres; ++q, ++it) {
             hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, pool.p + (long)order[q] * ntot, r.p);
-            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+            LT_TRY(seg_dot(r.p, r.p, rr.p));
             hipLaunchKernelGGL(k_pc_in, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, r.p, rr.p, dhat.p, HW,
                                net->act[PcNet::X0].p);
             LT_CHECK_LAUNCH();
@@ -931,8 +1022,8 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
                                net->act[PcNet::Y].p, rr.p, ainv.p, HW, z.p);
             LT_CHECK_LAUNCH();
             LT_TRY(apply(z.p, Ap.p, all_active.p, S));
-            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, z.p, Ap.p, zAz.p);
-            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, z.p, r.p, zr.p);
+            LT_TRY(seg_dot(z.p, Ap.p, zAz.p));
+            LT_TRY(seg_dot(z.p, r.p, zr.p));
             LT_CHECK_LAUNCH();
             if (epochs > 0) {
                 hipLaunchKernelGGL(k_energy_grad, dim3(gb), dim3(256), 0, stream(), ntot, S, seg.p, soff.p, Ap.p,
@@ -982,18 +1073,18 @@ lt_status Solver::prepare32() {
 
 lt_status Solver::apply32(const double *xin, double *yout, const int *alist, int nact, const int *af) {
     const unsigned gb = blocks_for(ntot, 256);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, xin, xin, xx.p);
+    LT_TRY(seg_dot(xin, xin, xx.p));
     hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, xin, x32.p);
     LT_CHECK_LAUNCH();
     LT_TRY(csr_spmv32(lb->L, x32.p, i1_32.p, 1.f, 0.f, af, npix, nullptr));
-    LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, stencil));
+    LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, *stp));
     LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p));
     if (lam_c != 0.0) {
         LT_TRY(csr_spmv32(C, x32.p, t32.p, 1.f, 0.f, af, 0, seg.p));
         LT_TRY(csr_spmv32(C, t32.p, c32.p, 1.f, 0.f, af, 0, seg.p));
     }
     hipLaunchKernelGGL(k_from_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, y32.p,
-                       stencil.coef32_scale, lam_c != 0.0 ? c32.p : nullptr, lam_c, lam_t, xin, yout);
+                       stp->coef32_scale, lam_c != 0.0 ? c32.p : nullptr, lam_c, lam_t, xin, yout);
     LT_CHECK_LAUNCH();
     return LT_OK;
 }
@@ -1040,13 +1131,13 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(iters.upload(zeros.data(), S));
     LT_TRY(x.zero());
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, b.p, r.p);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+    LT_TRY(seg_dot(r.p, r.p, rr.p));
     LT_CHECK_LAUNCH();
     LT_TRY(precondition(precond, dinv_p));
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, z.p, p.p);
     LT_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, b.p, b.p, bb.p);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz.p);
+    LT_TRY(seg_dot(b.p, b.p, bb.p));
+    LT_TRY(seg_dot(r.p, z.p, rz.p));
     LT_CHECK_LAUNCH();
     if (mixed) {
         LT_TRY(bb.download(rr_ref.data(), S));
@@ -1068,15 +1159,15 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             LT_TRY(apply32(p.p, Ap.p, dlist.p, nact, act.p));
         else
             LT_TRY(apply(p.p, Ap.p, dlist.p, nact, act.p));
-        hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, p.p, Ap.p, pAp.p);
+        LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
         hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
-        hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+        LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
         LT_TRY(precondition(precond, dinv_p));
-        hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz_new.p);
+        LT_TRY(seg_dot(r.p, z.p, rz_new.p));
         if (flexible) {
-            hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, z.p, Ap.p, zap.p);
+            LT_TRY(seg_dot(z.p, Ap.p, zap.p));
             hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
                                alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
         } else {
@@ -1106,10 +1197,10 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
                 LT_TRY(dref.upload(reflist.data(), nref));
                 LT_TRY(apply(x.p, Ax.p, dref.p, nref, dflag.p));
                 hipLaunchKernelGGL(k_resid, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, dflag.p, b.p, Ax.p, r.p);
-                hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+                LT_TRY(seg_dot(r.p, r.p, rr.p));
                 LT_CHECK_LAUNCH();
                 LT_TRY(precondition(precond, dinv_p));
-                hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz_new.p);
+                LT_TRY(seg_dot(r.p, z.p, rz_new.p));
                 hipLaunchKernelGGL(k_take_flagged, dim3(gs), dim3(64), 0, stream(), S, dflag.p, rz_new.p, rz.p);
                 LT_CHECK_LAUNCH();
             }
@@ -1149,7 +1240,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(apply(x.p, Ap.p, all_active.p, S));
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, b.p, r.p);
     hipLaunchKernelGGL(k_axpby, dim3(gb), dim3(256), 0, stream(), ntot, -1.0, Ap.p, 1.0, r.p);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, r.p, rr.p);
+    LT_TRY(seg_dot(r.p, r.p, rr.p));
     LT_CHECK_LAUNCH();
     LT_TRY(rr.download(hrr.data(), S));
     LT_TRY(bb.download(hbb.data(), S));
@@ -1166,7 +1257,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(dlist.upload(alist.data(), nact));
     LT_TRY(precondition(precond, dinv_p));
     hipLaunchKernelGGL(k_restart_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, z.p, p.p);
-    hipLaunchKernelGGL(k_seg_dot, dim3(S), dim3(kDotThreads), 0, stream(), soff.p, r.p, z.p, rz.p);
+    LT_TRY(seg_dot(r.p, z.p, rz.p));
     LT_CHECK_LAUNCH();
     }
     std::vector<int> hit(S);
@@ -1317,13 +1408,67 @@ lt_status lt_solver_create_dirty(lt_lensbatch lbh, lt_nufft nfh, const double *w
                                  int ndirty, double lambda_curv, double lambda_tikh, int op_mode, lt_solver *out) {
     LensBatch *lb = lensbatch_of(lbh);
     NufftPlan *nf = nufft_of(nfh);
-    if (!lb || !nf || !out || !dirty || (nf->nvis > 0 && !weights)) {
+    if (!lb || !nf || !out || !dirty) {
         set_error("lt_solver_create_dirty: bad arguments");
         return LT_ERR_ARG;
     }
     auto h = std::make_unique<lt_solver_s>();
     LT_TRY(h->s.create(lb, nf, weights, nullptr, dirty, ndirty, lambda_curv, lambda_tikh, op_mode));
     *out = h.release();
+    return LT_OK;
+}
+
+lt_status lt_solver_create_resident(lt_lensbatch lbh, lt_nufft nfh, const double *d_dirty, int ndirty,
+                                   double lambda_curv, double lambda_tikh, int op_mode, lt_solver *out) {
+    LensBatch *lb = lensbatch_of(lbh);
+    NufftPlan *nf = nufft_of(nfh);
+    if (!lb || !nf || !out || !d_dirty) {
+        set_error("lt_solver_create_resident: bad arguments");
+        return LT_ERR_ARG;
+    }
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, d_dirty) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        set_error("lt_solver_create_resident: d_dirty is not device memory");
+        return LT_ERR_ARG;
+    }
+    auto h = std::make_unique<lt_solver_s>();
+    LT_TRY(h->s.create(lb, nf, nullptr, nullptr, d_dirty, ndirty, lambda_curv, lambda_tikh, op_mode, true));
+    *out = h.release();
+    return LT_OK;
+}
+
+lt_status lt_nufft_set_weights(lt_nufft nfh, const double *weights) {
+    NufftPlan *nf = nufft_of(nfh);
+    if (!nf) {
+        set_error("lt_nufft_set_weights: bad arguments");
+        return LT_ERR_ARG;
+    }
+    return nf->set_weights(weights);
+}
+
+lt_status lt_device_alloc(long nbytes, void **dptr) {
+    if (!dptr || nbytes < 0) return LT_ERR_ARG;
+    *dptr = nullptr;
+    if (nbytes == 0) return LT_OK;
+    LT_HIP(hipMalloc(dptr, (size_t)nbytes));
+    return LT_OK;
+}
+
+lt_status lt_device_upload(void *dptr, const void *host, long nbytes) {
+    if ((!dptr || !host) && nbytes > 0) return LT_ERR_ARG;
+    if (nbytes > 0) LT_HIP(hipMemcpy(dptr, host, (size_t)nbytes, hipMemcpyHostToDevice));
+    return LT_OK;
+}
+
+lt_status lt_device_download(const void *dptr, void *host, long nbytes) {
+    if ((!dptr || !host) && nbytes > 0) return LT_ERR_ARG;
+    if (nbytes > 0) LT_HIP(hipMemcpy(host, dptr, (size_t)nbytes, hipMemcpyDeviceToHost));
+    return LT_OK;
+}
+
+lt_status lt_device_free(void *dptr) {
+    if (dptr) LT_HIP(hipFree(dptr));
     return LT_OK;
 }
 

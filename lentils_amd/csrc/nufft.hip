@@ -2199,6 +2199,36 @@ lt_status NormalStencil::apply32(NufftPlan &P, const void *spec_in, long sstride
     return stencil_apply_t<float, float>(*this, P, coef32.p, spec_in, sstride, spec_out, active, nactive);
 }
 
+__global__ void k_gather_weights(long n, const int32_t *__restrict__ perm, const double *__restrict__ w_in,
+                                 double *__restrict__ w_out) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) w_out[q] = w_in[perm[q]];
+}
+
+lt_status NufftPlan::set_weights(const double *weights) {
+    if (!weights && nvis > 0) {
+        set_error("set_weights: no weights");
+        return LT_ERR_ARG;
+    }
+    LT_TRY(wplan.alloc(std::max<long>(nvis, 1)));
+    if (nvis > 0) {
+        DevBuf<double> raw;
+        LT_TRY(raw.upload(weights, nvis));
+        hipLaunchKernelGGL(k_gather_weights, dim3(blocks_for(nvis, 256)), dim3(256), 0, stream(), nvis, perm.p, raw.p,
+                           wplan.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(sync());
+    }
+    wstencil.reset();
+    wstencil_tuned_batch = -1;
+    if (has_image && prec == LT_F64 && W <= 4 && nvis > 0) {
+        auto st = std::make_unique<NormalStencil>();
+        LT_TRY(st->build(*this, wplan.p));
+        wstencil = std::move(st);
+    }
+    return sync();
+}
+
 lt_status NufftPlan::to_sorted(const double *src, void *dst) {
     if (nvis == 0) return LT_OK;
     if (prec == LT_F64)

@@ -171,6 +171,14 @@ struct NufftPlan {
     // permute between caller order (complex128) and sorted complex T
     lt_status to_sorted(const double *vis_c128, void *sorted);
     lt_status from_sorted(const void *sorted, double *vis_c128, bool accumulate);
+
+    // resident per-visibility weights (plan order) and, for FP64 plans with
+    // W <= 4, the normal stencil D^T diag(w) D built from them once; solvers
+    // created without weights share both
+    DevBuf<double> wplan;
+    std::unique_ptr<NormalStencil> wstencil;
+    int wstencil_tuned_batch = -1;  // batch size the stencil was last autotuned for
+    lt_status set_weights(const double *weights_caller_order);
 };
 
 // pick the stencil's samples per workgroup by timing the normal operator

@@ -8,6 +8,11 @@
 
 #include "lt_internal.h"
 
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace lt {
 
 namespace {
@@ -298,14 +303,47 @@ static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, int &best_out)
     return LT_OK;
 }
 
+// Process-wide memo of tuned lane counts: matrices of the same shape class
+// (device, precision, row count, mean row length to a quarter octave,
+// presence of long rows) reuse the first measurement, so rebuilding lens
+// matrices of a new batch of the same geometry does not re-run the sweep.
+namespace {
+struct TuneKey {
+    int dev, fp32, lenq, has_long;
+    long nrows;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(dev, fp32, lenq, has_long, nrows) < std::tie(o.dev, o.fp32, o.lenq, o.has_long, o.nrows);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+}  // namespace
+
 lt_status csr_autotune(DevCSR &A, bool fp32) {
     if (A.nrows == 0 || A.nnz == 0) return LT_OK;
     if (!A.prepared) LT_TRY(csr_prepare(A));
-    if (fp32) {
-        LT_TRY(csr_make_f32(A));
-        return tune_lanes<float>(A, A.va32.p, A.chunk_sum32.p, A.lanes32);
+    if (fp32) LT_TRY(csr_make_f32(A));
+    int dev = 0;
+    LT_HIP(hipGetDevice(&dev));
+    const double mean = (double)A.nnz / (double)A.nrows;
+    const TuneKey key{dev, fp32 ? 1 : 0, (int)std::lround(4.0 * std::log2(mean)), A.nlong > 0 ? 1 : 0, A.nrows};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            (fp32 ? A.lanes32 : A.lanes) = it->second;
+            return LT_OK;
+        }
     }
-    return tune_lanes<double>(A, A.va.p, A.chunk_sum.p, A.lanes);
+    int best = 0;
+    if (fp32)
+        LT_TRY(tune_lanes<float>(A, A.va32.p, A.chunk_sum32.p, best));
+    else
+        LT_TRY(tune_lanes<double>(A, A.va.p, A.chunk_sum.p, best));
+    (fp32 ? A.lanes32 : A.lanes) = best;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = best;
+    return LT_OK;
 }
 
 // Transpose: counting sort of the nonzeros by column, then a segmented sort

@@ -27,6 +27,41 @@ def _image_array(spaces):
     return arr
 
 
+def _weights(weights, sigma, mask):
+    if weights is None:
+        s = np.asarray(sigma, dtype=np.float64).reshape(-1)
+        m = np.ones(s.shape, dtype=bool) if mask is None else np.asarray(mask, dtype=bool).reshape(-1)
+        w = np.zeros(s.shape)
+        w[m] = s[m] ** -2  # data_base.py:36-44
+    else:
+        w = np.asarray(weights, dtype=np.float64).reshape(-1)
+    return np.ascontiguousarray(w)
+
+
+class DeviceArray:
+    """float64 array resident in device memory (inputs that stay on the GPU
+    between solves, e.g. the dirty images of a batch of observations)."""
+
+    def __init__(self, array):
+        a = np.ascontiguousarray(array, dtype=np.float64)
+        self.shape, self.size = a.shape, a.size
+        p = C.c_void_p()
+        B.check(B.lib().lt_device_alloc(a.nbytes, C.byref(p)))
+        self.ptr = p
+        B.check(B.lib().lt_device_upload(p, B.ptr(a), a.nbytes))
+
+    def numpy(self):
+        out = np.zeros(self.shape)
+        B.check(B.lib().lt_device_download(self.ptr, B.ptr(out), out.nbytes))
+        return out
+
+    def __del__(self):
+        p = getattr(self, 'ptr', None)
+        if p is not None and B._lib is not None:
+            B._lib.lt_device_free(p)
+            self.ptr = None
+
+
 class LensBatch:
     """Lens matrices L_s (image -> source_spaces[s]) for many lens models,
     built in one GPU pass and kept on the device with their transposes."""
@@ -105,6 +140,21 @@ class NUFFTPlan:
         B.check(B.lib().lt_nufft_adjoint_host(self._handle, B.ptr(v), B.ptr(img)))
         return img.reshape(self.image_space.shape)
 
+    @property
+    def has_weights(self):
+        return getattr(self, '_has_weights', False)
+
+    def set_weights(self, weights=None, sigma=None, mask=None):
+        """Keep the per-visibility weights (or sigma^-2 on the unmasked
+        visibilities, data_base.py:36-44) resident in the plan, together with
+        the normal stencil built from them (FP64 plans, support <= 4); solvers
+        created without weights use both."""
+        w = _weights(weights, sigma, mask)
+        if w.size != self.vis_space.size:
+            raise ValueError(f'{w.size} weights for {self.vis_space.size} visibilities')
+        B.check(B.lib().lt_nufft_set_weights(self._handle, B.ptr(w)))
+        self._has_weights = True
+
     def time(self, direction, reps=20):
         """Mean device time (ms) of one degrid (0) or grid (1) launch."""
         ms = C.c_double(0)
@@ -132,20 +182,30 @@ class BatchedSourceInversion:
         if (data is None) == (dirty_images is None):
             raise ValueError('give either data or dirty_images')
         d = None if data is None else np.ascontiguousarray(data, dtype=np.complex128).reshape(-1)
-        if weights is None:
-            s = np.asarray(sigma, dtype=np.float64).reshape(-1)
-            m = np.ones(s.shape, dtype=bool) if mask is None else np.asarray(mask, dtype=bool).reshape(-1)
-            w = np.zeros(s.shape)
-            w[m] = s[m] ** -2  # data_base.py:36-44
+        if weights is None and sigma is None:
+            if not nufft_plan.has_weights:
+                raise ValueError('give weights or sigma, or make them resident with NUFFTPlan.set_weights')
+            self._w = None  # the plan's resident weights (and normal stencil)
         else:
-            w = np.asarray(weights, dtype=np.float64).reshape(-1)
-        self._w = np.ascontiguousarray(w)
+            self._w = _weights(weights, sigma, mask)
+        wptr = None if self._w is None else B.ptr(self._w)
         self._d = d
         mode = {'nufft': B.LT_OP_NUFFT, 'bcb': B.LT_OP_BCB, 'nufft_explicit': B.LT_OP_NUFFT_EXPLICIT}[operator]
         h = C.c_void_p()
         if d is not None:
-            B.check(B.lib().lt_solver_create(lens_batch._handle, nufft_plan._handle, B.ptr(self._w), B.ptr(self._d),
+            B.check(B.lib().lt_solver_create(lens_batch._handle, nufft_plan._handle, wptr, B.ptr(self._d),
                                              float(lambda_curvature), float(lambda_tikhonov), mode, C.byref(h)))
+        elif isinstance(dirty_images, DeviceArray):
+            npix = lens_batch.image_space.size
+            nd = dirty_images.size // npix
+            if self._w is not None:
+                raise ValueError('device-resident dirty images take the plan\'s resident weights')
+            if nd * npix != dirty_images.size or nd not in (1, lens_batch.nsamples):
+                raise ValueError('dirty_images: one image-space array, or one per sample')
+            self._dirty = dirty_images  # keeps the device memory alive
+            B.check(B.lib().lt_solver_create_resident(lens_batch._handle, nufft_plan._handle, dirty_images.ptr, nd,
+                                                      float(lambda_curvature), float(lambda_tikhonov), mode,
+                                                      C.byref(h)))
         else:
             dl = dirty_images if isinstance(dirty_images, (list, tuple)) else [dirty_images]
             npix = lens_batch.image_space.size
@@ -153,7 +213,7 @@ class BatchedSourceInversion:
                 raise ValueError('dirty_images: one image-space array, or one per sample')
             self._dirty = np.ascontiguousarray(np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1)
                                                                for a in dl]))
-            B.check(B.lib().lt_solver_create_dirty(lens_batch._handle, nufft_plan._handle, B.ptr(self._w),
+            B.check(B.lib().lt_solver_create_dirty(lens_batch._handle, nufft_plan._handle, wptr,
                                                    B.ptr(self._dirty), len(dl), float(lambda_curvature),
                                                    float(lambda_tikhonov), mode, C.byref(h)))
         self._handle = h

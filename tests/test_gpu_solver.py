@@ -127,6 +127,47 @@ def test_per_sample_dirty_images(hip, oracle):
         BatchedSourceInversion(lb, plan, weights=w, dirty_images=dirty[:2])
 
 
+def test_resident_inputs(hip, oracle):
+    """Weights resident in the plan (with the normal stencil built from them
+    once) and dirty images in device memory give the same right-hand sides,
+    operator and solutions as the host-input solver; a second solver on the
+    same plan reuses them; device-input misuse is rejected."""
+    from lentils_amd.solvers import BatchedSourceInversion, DeviceArray, LensBatch, NUFFTPlan
+    im, src, uv, vs, models, data = _problem(nlens=3)
+    lb = LensBatch(im, [src] * 3, models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = rng(90).uniform(0.5e6, 1.5e6, size=uv.shape[0])
+    datas = [data * (k + 1) for k in range(3)]
+    dirty = [plan.adjoint(w * d) for d in datas]
+    host = BatchedSourceInversion(lb, plan, weights=w, dirty_images=dirty, lambda_curvature=1e2)
+    with pytest.raises(ValueError):
+        BatchedSourceInversion(lb, plan, dirty_images=dirty, lambda_curvature=1e2)  # no weights anywhere
+    plan.set_weights(w)
+    dd = DeviceArray(np.stack(dirty))
+    assert np.array_equal(dd.numpy(), np.stack(dirty))
+    res = BatchedSourceInversion(lb, plan, dirty_images=dd, lambda_curvature=1e2)
+    for a, b in zip(host.rhs(), res.rhs()):
+        assert np.array_equal(a, b)
+    g = rng(91)
+    xs = [g.normal(size=src.shape) for _ in range(3)]
+    for a, b in zip(host.apply(xs), res.apply(xs)):
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a))
+    x0, it0, r0 = host.solve(rtol=1e-6, maxiter=20000)
+    x1, it1, r1 = res.solve(rtol=1e-6, maxiter=20000)
+    assert np.all(r1 <= 1e-6) and np.all(np.abs(np.asarray(it0) - np.asarray(it1)) <= 2)
+    for a, b in zip(x0, x1):
+        assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(a))
+    again = BatchedSourceInversion(lb, plan, dirty_images=dd, lambda_curvature=1e2)
+    x2, it2, r2 = again.solve(rtol=1e-6, maxiter=20000)
+    assert np.array_equal(it1, it2)
+    for a, b in zip(x1, x2):
+        assert np.array_equal(a, b)  # run-to-run identical
+    with pytest.raises(ValueError):
+        BatchedSourceInversion(lb, plan, weights=w, dirty_images=dd, lambda_curvature=1e2)
+    with pytest.raises(ValueError):
+        BatchedSourceInversion(lb, plan, dirty_images=DeviceArray(np.stack(dirty[:2])), lambda_curvature=1e2)
+
+
 def test_precision_form(hip, oracle):
     """FP32 operator inside FP64 CG with true-residual refresh: the FP32
     operator agrees with the FP64 one to FP32 rounding, every sample still
