@@ -170,6 +170,11 @@ lt_status lt_fft_c2r_host(int nx, int ny, int batch, const double *in, double *o
  * explicit matrix, operator_base.py:96-98). */
 lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                            const double *x, double *y);
+/* Same with the device kernel chosen: 0 the default, 1..64 (powers of two)
+ * the lane-group kernel with that many lanes per row, 128 the streaming
+ * (LDS-staged) kernel.  Every choice sums each row in a fixed order. */
+lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
+                                  const double *x, double *y, int kernel);
 
 /* No reference counterpart: a plain streaming copy of nbytes (multiple of
  * 32) between two device buffers, reps times; mean time per launch.  The

@@ -130,8 +130,14 @@ lt_status lt_stream_copy(long nbytes, int reps, double *ms_per_launch) {
 
 lt_status lt_csr_spmv_host(long nrows, long ncols, const int *rp, const int *ci, const double *va, const double *x,
                            double *y) {
+    return lt_csr_spmv_host_kernel(nrows, ncols, rp, ci, va, x, y, 0);
+}
+
+lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *rp, const int *ci, const double *va,
+                                  const double *x, double *y, int kernel) {
     LT_TRY(ensure_init());
-    if (nrows < 0 || ncols < 0 || !rp || !y || (ncols > 0 && !x)) {
+    const bool pow2 = kernel > 0 && kernel <= 64 && (kernel & (kernel - 1)) == 0;
+    if (nrows < 0 || ncols < 0 || !rp || !y || (ncols > 0 && !x) || !(kernel == 0 || pow2 || kernel == kLanesStream)) {
         set_error("lt_csr_spmv_host: bad arguments");
         return LT_ERR_ARG;
     }
@@ -146,7 +152,7 @@ lt_status lt_csr_spmv_host(long nrows, long ncols, const int *rp, const int *ci,
     DevBuf<double> dx, dy;
     LT_TRY(dx.upload(x, ncols));
     LT_TRY(dy.alloc(nrows > 0 ? nrows : 1));
-    LT_TRY(csr_spmv(A, dx.p, dy.p, 1.0, 0.0, 0));
+    LT_TRY(csr_spmv(A, dx.p, dy.p, 1.0, 0.0, kernel));
     LT_TRY(dy.download(y, nrows));
     return sync();
 }

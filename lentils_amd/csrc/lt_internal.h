@@ -126,13 +126,17 @@ struct DevCSR {
     // long-row plan (csr_prepare): rows above kLongRow nonzeros are summed
     // in fixed-size chunks by whole workgroups, then combined in order
     bool prepared = false;
-    int lanes = 1;
+    int lanes = 1;  // lane-group width of k_spmv, or kLanesStream for the streaming kernel
+    long nblk = 0;        // streaming-kernel workgroups (ceil(nnz / its nonzero range))
+    DevBuf<int32_t> rblk;  // first row of each streaming workgroup, nblk + 1
     long nlong = 0, nchunk = 0;
     DevBuf<int32_t> long_rows, long_chunk0, chunk_beg, long_rows_of_chunk;
     DevBuf<double> chunk_sum;
     DevBuf<float> va32, chunk_sum32;  // FP32 values (csr_make_f32)
     int lanes32 = 0;                  // lane-group width of the FP32 product (0: as lanes)
 };
+
+constexpr int kLanesStream = 128;  // pseudo lane count: the streaming (LDS-staged) SpMV kernel
 
 // analyse row lengths once (lane group size, long-row chunk plan)
 lt_status csr_prepare(DevCSR &A);

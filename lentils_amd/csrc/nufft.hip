@@ -1141,6 +1141,184 @@ __global__ void k_max_abs_b(long nrow, const int32_t *__restrict__ row_cell, int
     atomicMax(out, b < 0 ? -b : b);
 }
 
+// ---------------------------------------------------------------------------
+// Fused row transforms of the band normal operator.  Each workgroup takes
+// 2*PAIRS consecutive image rows of one sample; every pair (a, b) is
+// transformed as one complex sequence a + i b of length N = nyp (the rows
+// zero-padded along y) by an unnormalised mixed-radix (4, then 2) Stockham
+// FFT in LDS, and split with the two-real-sequences identity.
+//   forward: apodize + pad + r2c of the rows + pick the band's nbv columns,
+//            written as the band's column-major input [slot][v][u];
+//   inverse: Hermitian extension of the band rows + c2r + crop + apodize
+//            (+ alpha, accumulate), written to the output images.
+// They replace k_band_pad + rows r2c + k_band_gather and k_band_scatter +
+// rows c2r + k_band_crop with one pass over the data each way.
+// twd[e] = exp(-2 pi i e / N) (e < N); the inverse uses its conjugate.
+// ---------------------------------------------------------------------------
+template <class T, bool INV>
+__device__ inline void dft4(Cplx<T> (&v)[4]) {
+    const Cplx<T> t0{v[0].re + v[2].re, v[0].im + v[2].im}, t1{v[0].re - v[2].re, v[0].im - v[2].im};
+    const Cplx<T> t2{v[1].re + v[3].re, v[1].im + v[3].im}, t3{v[1].re - v[3].re, v[1].im - v[3].im};
+    v[0] = Cplx<T>{t0.re + t2.re, t0.im + t2.im};
+    v[2] = Cplx<T>{t0.re - t2.re, t0.im - t2.im};
+    if (!INV) {  // -i t3 = (t3.im, -t3.re)
+        v[1] = Cplx<T>{t1.re + t3.im, t1.im - t3.re};
+        v[3] = Cplx<T>{t1.re - t3.im, t1.im + t3.re};
+    } else {
+        v[1] = Cplx<T>{t1.re - t3.im, t1.im + t3.re};
+        v[3] = Cplx<T>{t1.re + t3.im, t1.im - t3.re};
+    }
+}
+
+// one Stockham stage of radix R over PAIRS sequences of length N in buf
+template <class T, int N, int R, int PAIRS, bool INV>
+__device__ inline void stockham_stage(Cplx<T> (*buf)[N], int Ns, const Cplx<T> *__restrict__ twd) {
+    constexpr int NB = N / R * PAIRS;  // butterflies of the stage over all sequences
+    constexpr int PER = (NB + 255) / 256;
+    Cplx<T> v[PER][R];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int g = threadIdx.x + 256 * q;
+        if (g < NB) {
+            const int pr = g / (N / R), j = g - pr * (N / R), k = j % Ns;
+            const int step = N / (Ns * R);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const Cplx<T> a = buf[pr][j + r * (N / R)];
+                const Cplx<T> w = twd[k * r * step];
+                const T wi = INV ? -w.im : w.im;
+                v[q][r] = Cplx<T>{a.re * w.re - a.im * wi, a.re * wi + a.im * w.re};
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int g = threadIdx.x + 256 * q;
+        if (g < NB) {
+            const int pr = g / (N / R), j = g - pr * (N / R), k = j % Ns;
+            if constexpr (R == 4) {
+                dft4<T, INV>(v[q]);
+                const int d = (j / Ns) * Ns * 4 + k;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) buf[pr][d + r * Ns] = v[q][r];
+            } else {
+                const Cplx<T> b0{v[q][0].re + v[q][1].re, v[q][0].im + v[q][1].im};
+                const Cplx<T> b1{v[q][0].re - v[q][1].re, v[q][0].im - v[q][1].im};
+                const int d = (j / Ns) * Ns * 2 + k;
+                buf[pr][d] = b0;
+                buf[pr][d + Ns] = b1;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <class T, int N, int PAIRS, bool INV>
+__device__ inline void fft_rows(Cplx<T> (*buf)[N], const Cplx<T> *__restrict__ twd) {
+    int Ns = 1;
+#pragma unroll
+    for (int n = N; n >= 4 && n % 4 == 0; n /= 4) {
+        stockham_stage<T, N, 4, PAIRS, INV>(buf, Ns, twd);
+        Ns *= 4;
+    }
+    if (Ns < N) stockham_stage<T, N, 2, PAIRS, INV>(buf, Ns, twd);
+}
+
+template <class T, int N, int PAIRS, class TI>
+__global__ void __launch_bounds__(256) k_rows_fwd(int nx, int ny, int pady, int padx, int nu, int nbv,
+                                                  const TI *__restrict__ img, const double *__restrict__ apod,
+                                                  const int *__restrict__ active, const Cplx<T> *__restrict__ twd,
+                                                  Cplx<T> *__restrict__ band) {
+    __shared__ Cplx<T> buf[PAIRS][N];
+    const int slot = blockIdx.y;
+    const int smp = active ? active[slot] : slot;
+    const int row0 = blockIdx.x * 2 * PAIRS;
+    const TI *im = img + (long)smp * nx * ny;
+    for (int q = threadIdx.x; q < PAIRS * N; q += 256) {
+        const int pr = q / N, k = q - pr * N, j = k - pady;
+        const int ia = row0 + 2 * pr, ib = ia + 1;
+        T re = (T)0, imv = (T)0;
+        if (j >= 0 && j < ny) {
+            if (ia < nx) re = (T)(im[(long)ia * ny + j] * (TI)apod[(long)ia * ny + j]);
+            if (ib < nx) imv = (T)(im[(long)ib * ny + j] * (TI)apod[(long)ib * ny + j]);
+        }
+        buf[pr][k] = Cplx<T>{re, imv};
+    }
+    __syncthreads();
+    fft_rows<T, N, PAIRS, false>(buf, twd);
+    Cplx<T> *B = band + (long)slot * nbv * nu;
+    const T h = (T)0.5;
+    for (int q = threadIdx.x; q < nbv * 2 * PAIRS; q += 256) {
+        const int v = q / (2 * PAIRS), rr = q - v * 2 * PAIRS, pr = rr >> 1;
+        if (row0 + rr >= nx) continue;
+        const Cplx<T> z = buf[pr][v], zm = buf[pr][(N - v) & (N - 1)];
+        const Cplx<T> x = (rr & 1) ? Cplx<T>{h * (z.im + zm.im), h * (zm.re - z.re)}
+                                   : Cplx<T>{h * (z.re + zm.re), h * (z.im - zm.im)};
+        B[(long)v * nu + padx + row0 + rr] = x;
+    }
+}
+
+// band input rows outside the image (u < padx or u >= padx + nx) are zero;
+// the in-place column transform overwrites them, so they are rewritten
+template <class T>
+__global__ void k_band_zero_rows(int nx, int padx, int nu, int nbv, Cplx<T> *__restrict__ band) {
+    const int slot = blockIdx.y;
+    const int nz = nu - nx;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)nbv * nz) return;
+    const int v = (int)(g / nz), z = (int)(g - (long)v * nz);
+    const int u = z < padx ? z : z + nx;
+    band[(long)slot * nbv * nu + (long)v * nu + u] = Cplx<T>{(T)0, (T)0};
+}
+
+template <class T, int N, int PAIRS, class TO>
+__global__ void __launch_bounds__(256) k_rows_inv(int nx, int ny, int pady, int padx, int nu, int nbv,
+                                                  const Cplx<T> *__restrict__ band, const double *__restrict__ apod,
+                                                  TO alpha, int accumulate, const int *__restrict__ active,
+                                                  const Cplx<T> *__restrict__ twd, TO *__restrict__ out) {
+    __shared__ Cplx<T> buf[PAIRS][N];
+    const int slot = blockIdx.y;
+    const int smp = active ? active[slot] : slot;
+    const int row0 = blockIdx.x * 2 * PAIRS;
+    const Cplx<T> *B = band + (long)slot * nbv * nu;
+    constexpr int H = N / 2;
+    // pass a: the a rows' Hermitian spectra; pass b: + i times the b rows'
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int q = threadIdx.x; q < (H + 1) * PAIRS; q += 256) {
+            const int v = q / PAIRS, pr = q - v * PAIRS;
+            const int row = row0 + 2 * pr + pass;
+            Cplx<T> c{(T)0, (T)0};
+            if (v < nbv && row < nx) c = B[(long)v * nu + padx + row];
+            if (v == 0 || v == H) c.im = (T)0;
+            // value at v and its conjugate at N - v
+            if (pass == 0) {
+                buf[pr][v] = c;
+                if (v > 0 && v < H) buf[pr][N - v] = Cplx<T>{c.re, -c.im};
+            } else {  // += i c at v, += i conj(c) at N - v
+                Cplx<T> &z = buf[pr][v];
+                z = Cplx<T>{z.re - c.im, z.im + c.re};
+                if (v > 0 && v < H) {
+                    Cplx<T> &zm = buf[pr][N - v];
+                    zm = Cplx<T>{zm.re + c.im, zm.im + c.re};
+                }
+            }
+        }
+        __syncthreads();
+    }
+    fft_rows<T, N, PAIRS, true>(buf, twd);
+    TO *o = out + (long)smp * nx * ny;
+    for (int q = threadIdx.x; q < 2 * PAIRS * ny; q += 256) {
+        const int rr = q / ny, j = q - rr * ny;
+        const int row = row0 + rr;
+        if (row >= nx) continue;
+        const Cplx<T> z = buf[rr >> 1][pady + j];
+        const long g = (long)row * ny + j;
+        const TO val = alpha * ((TO)((rr & 1) ? z.im : z.re) * (TO)apod[g]);
+        o[g] = accumulate ? o[g] + val : val;
+    }
+}
+
 Geo make_geo(const NufftPlan &P) {
     Geo G;
     G.nu = P.nu;
@@ -1832,6 +2010,107 @@ lt_status NufftPlan::band_fft(int n, int precision, int nbv, BandFFT *&out) {
     return LT_OK;
 }
 
+static lt_status ensure_twiddles(NufftPlan &P) {
+    const int N = P.nyp;
+    if (P.twd64.n == 2 * (size_t)N && P.twd32.n == 2 * (size_t)N) return LT_OK;
+    std::vector<double> t64(2 * (size_t)N);
+    std::vector<float> t32(2 * (size_t)N);
+    for (int e = 0; e < N; ++e) {
+        const double a = 2.0 * M_PI * (double)e / (double)N;
+        t64[2 * e] = std::cos(a);
+        t64[2 * e + 1] = -std::sin(a);
+        t32[2 * e] = (float)t64[2 * e];
+        t32[2 * e + 1] = (float)t64[2 * e + 1];
+    }
+    LT_TRY(P.twd64.upload(t64.data(), t64.size()));
+    LT_TRY(P.twd32.upload(t32.data(), t32.size()));
+    return sync();
+}
+
+static bool fused_rows_ok(const NufftPlan &P, int nbv) {
+    const int N = P.nyp;
+    return P.fused_rows && N >= 64 && N <= 2048 && (N & (N - 1)) == 0 && P.nv == N && P.half_nv == N / 2 + 1 &&
+           nbv >= 1 && nbv <= N / 2 + 1 && P.pady >= 0 && P.pady + P.ny <= N && P.padx >= 0 &&
+           P.padx + P.nx <= P.nu;
+}
+
+template <class T>
+static const Cplx<T> *twiddles_of(const NufftPlan &P);
+template <>
+const Cplx<double> *twiddles_of<double>(const NufftPlan &P) {
+    return (const Cplx<double> *)P.twd64.p;
+}
+template <>
+const Cplx<float> *twiddles_of<float>(const NufftPlan &P) {
+    return (const Cplx<float> *)P.twd32.p;
+}
+
+// sequence pairs per workgroup: up to 32 KB of LDS, at most 4
+template <class T, int N>
+constexpr int rows_pairs() {
+    constexpr int per = N * 2 * (int)sizeof(T);
+    return per >= 32768 ? 1 : (32768 / per > 4 ? 4 : 32768 / per);
+}
+
+template <class T, class TI>
+static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *active, int nactive, Cplx<T> *B) {
+    const Cplx<T> *tw = twiddles_of<T>(P);
+    if (P.nu > P.nx) {
+        hipLaunchKernelGGL((k_band_zero_rows<T>), dim3(blocks_for((long)nbv * (P.nu - P.nx), 256), nactive),
+                           dim3(256), 0, stream(), P.nx, P.padx, P.nu, nbv, B);
+        LT_CHECK_LAUNCH();
+    }
+#define LT_ROWS_FWD(NN)                                                                                            \
+    case NN: {                                                                                                     \
+        constexpr int PR = rows_pairs<T, NN>();                                                                    \
+        hipLaunchKernelGGL((k_rows_fwd<T, NN, PR, TI>), dim3((P.nx + 2 * PR - 1) / (2 * PR), nactive), dim3(256), \
+                           0, stream(), P.nx, P.ny, P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, tw, B);     \
+        break;                                                                                                     \
+    }
+    switch (P.nyp) {
+        LT_ROWS_FWD(64)
+        LT_ROWS_FWD(128)
+        LT_ROWS_FWD(256)
+        LT_ROWS_FWD(512)
+        LT_ROWS_FWD(1024)
+        LT_ROWS_FWD(2048)
+        default:
+            set_error("fused row transform: unsupported length %d", P.nyp);
+            return LT_ERR_ARG;
+    }
+#undef LT_ROWS_FWD
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+template <class T, class TO>
+static lt_status rows_inverse(NufftPlan &P, int nbv, const Cplx<T> *F, const int *active, int nactive, TO alpha,
+                              bool acc, TO *out) {
+    const Cplx<T> *tw = twiddles_of<T>(P);
+#define LT_ROWS_INV(NN)                                                                                            \
+    case NN: {                                                                                                     \
+        constexpr int PR = rows_pairs<T, NN>();                                                                    \
+        hipLaunchKernelGGL((k_rows_inv<T, NN, PR, TO>), dim3((P.nx + 2 * PR - 1) / (2 * PR), nactive), dim3(256), \
+                           0, stream(), P.nx, P.ny, P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0,    \
+                           active, tw, out);                                                                       \
+        break;                                                                                                     \
+    }
+    switch (P.nyp) {
+        LT_ROWS_INV(64)
+        LT_ROWS_INV(128)
+        LT_ROWS_INV(256)
+        LT_ROWS_INV(512)
+        LT_ROWS_INV(1024)
+        LT_ROWS_INV(2048)
+        default:
+            set_error("fused row transform: unsupported length %d", P.nyp);
+            return LT_ERR_ARG;
+    }
+#undef LT_ROWS_INV
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
 // D^T C D through the stencil with a band-pruned separable transform: rows of
 // the padded image are transformed along y (only the image's rows exist),
 // only the stencil's band of nbv spectral columns is carried to the
@@ -1853,23 +2132,33 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     const long rsz = (long)P.nx * P.nyp, csz = (long)P.nx * P.half_nv, bsz = (long)nbv * P.nu;
     const size_t need_r = (size_t)P.batch * rsz * sizeof(T), need_c = (size_t)P.batch * csz * 2 * sizeof(T),
                  need_b = (size_t)P.batch * bsz * 2 * sizeof(T);
-    if (P.bandR.n < need_r) LT_TRY(P.bandR.alloc(need_r));
-    if (P.bandC.n < need_c) LT_TRY(P.bandC.alloc(need_c));
-    if (P.bandB.n < need_b) LT_TRY(P.bandB.alloc(need_b));
-    if (P.bandF.n < need_b) LT_TRY(P.bandF.alloc(need_b));
+    const bool fused = fused_rows_ok(P, nbv);
+    if (!fused && P.bandR.n < need_r) LT_TRY(P.bandR.alloc(need_r));
+    if (!fused && P.bandC.n < need_c) LT_TRY(P.bandC.alloc(need_c));
+    if (P.bandB.n < need_b) {
+        LT_TRY(P.bandB.alloc(need_b));
+        LT_TRY(P.bandB.zero());
+    }
+    if (P.bandF.n < need_b) {
+        LT_TRY(P.bandF.alloc(need_b));
+        LT_TRY(P.bandF.zero());
+    }
     T *R = (T *)P.bandR.p;
     Cplx<T> *Cb = (Cplx<T> *)P.bandC.p, *Bb = (Cplx<T> *)P.bandB.p, *Fb = (Cplx<T> *)P.bandF.p;
-    hipLaunchKernelGGL((k_band_pad<T, TI>), dim3(blocks_for(rsz, 256), nactive), dim3(256), 0, stream(), P.nx, P.ny,
-                       P.pady, P.nyp, imgs, P.apod.p, active, R);
-    LT_CHECK_LAUNCH();
-    {
+    if (fused) {
+        LT_TRY(ensure_twiddles(P));
+        LT_TRY((rows_forward<T, TI>(P, nbv, imgs, active, nactive, Bb)));
+    } else {
+        hipLaunchKernelGGL((k_band_pad<T, TI>), dim3(blocks_for(rsz, 256), nactive), dim3(256), 0, stream(), P.nx,
+                           P.ny, P.pady, P.nyp, imgs, P.apod.p, active, R);
+        LT_CHECK_LAUNCH();
         void *in[1] = {R};
         void *o[1] = {Cb};
         LT_TRY(fft_check(rocfft_execute(bf->plan[0], in, o, bf->info[0]), "execute band rows r2c"));
+        hipLaunchKernelGGL((k_band_gather<T>), dim3((P.nu + 31) / 32, (nbv + 31) / 32, nactive), dim3(256), 0,
+                           stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Cb, Bb);
+        LT_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL((k_band_gather<T>), dim3((P.nu + 31) / 32, (nbv + 31) / 32, nactive), dim3(256), 0, stream(),
-                       P.nx, P.padx, P.nu, P.half_nv, nbv, Cb, Bb);
-    LT_CHECK_LAUNCH();
     {
         void *io[1] = {Bb};
         LT_TRY(fft_check(rocfft_execute(bf->plan[1], io, nullptr, bf->info[1]), "execute band columns forward"));
@@ -1888,6 +2177,7 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
         void *io[1] = {Fb};
         LT_TRY(fft_check(rocfft_execute(bf->plan[2], io, nullptr, bf->info[2]), "execute band columns inverse"));
     }
+    if (fused) return rows_inverse<T, TI>(P, nbv, Fb, active, nactive, alpha, acc, out);
     hipLaunchKernelGGL((k_band_scatter<T>), dim3((P.nx + 31) / 32, (P.half_nv + 31) / 32, nactive), dim3(256), 0,
                        stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Fb, Cb);
     LT_CHECK_LAUNCH();

@@ -139,6 +139,11 @@ struct NufftPlan {
     };
     std::vector<std::unique_ptr<BandFFT>> bandfft;
     DevBuf<uint8_t> bandR, bandC, bandB, bandF;
+    // fused row transforms (power-of-two nyp, 64 .. 2048): twiddle tables
+    // exp(-2 pi i e / nyp), e < nyp, complex FP64 and FP32
+    DevBuf<double> twd64;
+    DevBuf<float> twd32;
+    bool fused_rows = true;  // use them where nyp allows
     lt_status band_fft(int n, int precision, int nbv, BandFFT *&out);
     DevBuf<uint8_t> realB, specB, specB2, partialB;
     lt_status batch_fft(int n, BatchFFT *&out, int precision = -1);

@@ -53,6 +53,68 @@ __global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32
     if (lane == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
 }
 
+// Streaming SpMV (CSR-stream): workgroup b owns the nonzeros
+// [b*kStreamNnz, (b+1)*kStreamNnz) and the rows that start among them
+// (rblk[b] .. rblk[b+1]).  It first forms every product va[k] * x[ci[k]] of
+// its range with coalesced loads into LDS, then one thread per row adds the
+// row's products in order (a row running past the range reads its tail
+// from memory).  Long rows are left to the chunked path.  With a row filter,
+// workgroups whose rows all belong to inactive samples return at once.
+constexpr int kStreamNnz = 2048;
+
+__device__ inline long row_sample(const RowFilter &F, long row) {
+    return F.rowseg ? (long)F.rowseg[row] : row / F.rps;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__restrict__ rblk,
+                                                      const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                      const T *__restrict__ va, const T *__restrict__ x,
+                                                      T *__restrict__ y, T alpha, T beta, int skip_long, RowFilter F) {
+    __shared__ T prod[kStreamNnz];
+    const long b = blockIdx.x;
+    const long r0 = rblk[b], r1 = rblk[b + 1];
+    if (r0 >= r1) return;  // inside a long row: no row starts here
+    if (F.act) {
+        bool any = false;
+        for (long sm = row_sample(F, r0), s1 = row_sample(F, r1 - 1); sm <= s1 && !any; ++sm) any = F.act[sm] != 0;
+        if (!any) return;
+    }
+    const long k0 = b * kStreamNnz, k1 = min(k0 + kStreamNnz, nnz);
+    for (long k = k0 + threadIdx.x; k < k1; k += 256) prod[k - k0] = va[k] * x[ci[k]];
+    __syncthreads();
+    for (long r = r0 + threadIdx.x; r < r1; r += 256) {
+        if (F.skip(r)) continue;
+        const int rb = rp[r], re = rp[r + 1];
+        if (skip_long && re - rb > kLongRow) continue;
+        const int ein = (int)min((long)re, k1);
+        T s = (T)0;
+        for (int k = rb; k < ein; ++k) s += prod[k - k0];
+        for (int k = ein; k < re; ++k) s += va[k] * x[ci[k]];
+        y[r] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[r];
+    }
+}
+
+// first row starting at or after nonzero b * kStreamNnz, per workgroup
+__global__ void k_stream_blocks(long nblk, long nrows, const int32_t *__restrict__ rp, int32_t *__restrict__ rblk) {
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nblk) return;
+    if (b == nblk) {
+        rblk[b] = (int32_t)nrows;
+        return;
+    }
+    const long k0 = b * kStreamNnz;
+    long lo = 0, hi = nrows;
+    while (lo < hi) {
+        const long mid = (lo + hi) >> 1;
+        if (rp[mid] < k0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    rblk[b] = (int32_t)lo;
+}
+
 // partial sums of fixed-size chunks of the long rows, one workgroup each
 template <class T>
 __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t *__restrict__ chunk_beg,
@@ -199,6 +261,14 @@ lt_status csr_prepare(DevCSR &A) {
             A.long_rows_of_chunk = std::move(chunk_row);
         }
     }
+    A.nblk = 0;
+    if (A.nrows > 0 && A.nnz > 0) {
+        A.nblk = (A.nnz + kStreamNnz - 1) / kStreamNnz;
+        LT_TRY(A.rblk.alloc(A.nblk + 1));
+        hipLaunchKernelGGL(k_stream_blocks, dim3(blocks_for(A.nblk + 1, 256)), dim3(256), 0, stream(), A.nblk,
+                           A.nrows, A.rp.p, A.rblk.p);
+        LT_CHECK_LAUNCH();
+    }
     A.prepared = true;
     return sync();
 }
@@ -210,9 +280,15 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
     const RowFilter F{act, rows_per_sample > 0 ? rows_per_sample : 1, rowseg};
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
     const int skip = (A.prepared && A.nlong > 0) ? 1 : 0;
+    if (lanes == kLanesStream && !(A.prepared && A.nblk > 0)) lanes = csr_pick_lanes(A);
+    if (lanes == kLanesStream) {
+        hipLaunchKernelGGL((k_spmv_stream<T>), dim3((unsigned)A.nblk), dim3(256), 0, stream(), A.nnz, A.rblk.p,
+                           A.rp.p, A.ci.p, va, x, y, alpha, beta_y, skip, F);
+        LT_CHECK_LAUNCH();
+    }
     // a dispatch covers at most 2^30 work-items (the grid size is 32-bit)
-    const long rows_per_launch = (1L << 30) / lanes;
-    for (long r0 = 0; r0 < A.nrows; r0 += rows_per_launch) {
+    const long rows_per_launch = (1L << 30) / std::min(lanes, 64);
+    for (long r0 = 0; lanes != kLanesStream && r0 < A.nrows; r0 += rows_per_launch) {
         const long nr = std::min(rows_per_launch, A.nrows - r0);
         const dim3 g(blocks_for(nr * lanes, 256)), b(256);
 #define LT_SPMV_CASE(L)                                                                                            \
@@ -284,7 +360,8 @@ static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, int &best_out)
     LT_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int best_l = A.lanes;
-    for (int L = 1; L <= 64; L <<= 1) {
+    for (int L = 1; L <= kLanesStream; L <<= 1) {
+        if (L == kLanesStream && A.nblk == 0) break;
         LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr));  // warm-up
         LT_HIP(hipEventRecord(e0, stream()));
         for (int r = 0; r < 3; ++r) LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr));
