@@ -37,7 +37,7 @@ lt_status deflect_lattice_device(const DevLens *lens_dev, const int *lens_off_de
 
 namespace {
 
-constexpr int kShort = 32;  // fragments a short row may hold
+constexpr int kShort = 64;  // fragments a short row may hold
 constexpr int kRasterThreads = 128;
 
 struct LocalFrags {
@@ -65,6 +65,22 @@ struct CountEmit {
     __device__ void operator()(int, int, double) {}
 };
 
+// The count pass keeps the first kStage CSR entries of every short row here
+// (row r at r * kStage); the fill pass copies rows of at most kStage entries
+// instead of rasterizing them again.
+constexpr int kStage = 8;
+
+struct StageEmit {
+    int32_t *cols;
+    double *vals;
+    __device__ void operator()(int k, int col, double val) {
+        if (k < kStage) {
+            cols[k] = col;
+            vals[k] = val;
+        }
+    }
+};
+
 struct StoreEmit {
     int32_t *cols;
     double *vals;
@@ -86,7 +102,9 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_count(BatchGeom G, co
                                                                   int64_t *__restrict__ nnz_row,
                                                                   int32_t *__restrict__ nfrag,
                                                                   int32_t *__restrict__ minidx,
-                                                                  int32_t *__restrict__ lastne, int *__restrict__ status) {
+                                                                  int32_t *__restrict__ lastne, int *__restrict__ status,
+                                                                  int32_t *__restrict__ stcol,
+                                                                  double *__restrict__ stval) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= G.npix * G.nsamples) return;
     const int s = (int)(r / G.npix);
@@ -110,8 +128,8 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_count(BatchGeom G, co
         minidx[r] = -1;
         return;
     }
-    CountEmit ce;
-    int k = reduce_fragments(fr.area, fr.idx, fr.n, 0, ce);
+    StageEmit se{stcol + r * kStage, stval + r * kStage};
+    int k = reduce_fragments(fr.area, fr.idx, fr.n, 0, se);
     nnz_row[r] = k;
     minidx[r] = fr.n > 0 ? fr.idx[0] : -1;
 }
@@ -134,7 +152,7 @@ struct GlobalFrags {
 // Long rows with at most kMedium fragments are walked by one thread each
 // (sequentially, like short rows); longer ("huge") rows take the frontier
 // decomposition below.  This pass counts every long row up to kMedium + 1.
-constexpr long kMedium = 2048;
+constexpr long kMedium = 512;
 
 struct CappedCount {
     long n = 0;
@@ -208,9 +226,10 @@ __global__ void __launch_bounds__(kExpandThreads) k_long_expand(BatchGeom G, con
         __syncthreads();
         for (int q = threadIdx.x; q < n; q += kExpandThreads) {
             int ovf = 0;
-            cnt[q] = ritem_count(cur[q], g, &ovf);
+            const bool ex = ritem_splits(cur[q]);
+            cnt[q] = ex ? ritem_count(cur[q], g, &ovf) : 1;
             if (ovf) atomicOr(&sh_ovf, 1);
-            if (cur[q].kind == 0) atomicOr(&sh_sub, 1);
+            if (ex) atomicOr(&sh_sub, 1);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -225,7 +244,12 @@ __global__ void __launch_bounds__(kExpandThreads) k_long_expand(BatchGeom G, con
         }
         __syncthreads();
         if (sh_stop) break;
-        for (int q = threadIdx.x; q < n; q += kExpandThreads) ritem_write(cur[q], g, nxt + cnt[q]);
+        for (int q = threadIdx.x; q < n; q += kExpandThreads) {
+            if (ritem_splits(cur[q]))
+                ritem_write(cur[q], g, nxt + cnt[q]);
+            else
+                nxt[cnt[q]] = cur[q];
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             sh_n = cnt[n];
@@ -392,6 +416,8 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_fill(BatchGeom G, con
                                                                  const int32_t *__restrict__ nfrag,
                                                                  const int32_t *__restrict__ minidx,
                                                                  const int32_t *__restrict__ lastne,
+                                                                 const int32_t *__restrict__ stcol,
+                                                                 const double *__restrict__ stval,
                                                                  int32_t *__restrict__ cols, double *__restrict__ vals) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= G.npix * G.nsamples) return;
@@ -404,9 +430,18 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_fill(BatchGeom G, con
         const int32_t q = lastne[r];
         if (q >= 0 && (long)q >= (long)s * G.npix) stale = minidx[q];
     }
+    const long o = rowoff[r];
+    const int k = (int)(rowoff[r + 1] - o);
+    if (nfrag[r] > 0 && k <= kStage) {  // entries kept by the count pass
+        for (int e = 0; e < k; ++e) {
+            cols[o + e] = (int32_t)srcoff[s] + stcol[r * kStage + e];
+            vals[o + e] = stval[r * kStage + e];
+        }
+        return;
+    }
     LocalFrags fr;
     if (nfrag[r] > 0) raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], fr);
-    StoreEmit se{cols + rowoff[r], vals + rowoff[r], (int32_t)srcoff[s]};
+    StoreEmit se{cols + o, vals + o, (int32_t)srcoff[s]};
     reduce_fragments(fr.area, fr.idx, fr.n, stale, se);
 }
 
@@ -515,8 +550,12 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
     LT_TRY(dstatus.alloc(ns));
     LT_TRY(dstatus.zero());
     BatchGeom G{ns, im.nx, im.ny, lx, ly, npix};
+    DevBuf<int32_t> stcol;
+    DevBuf<double> stval;
+    LT_TRY(stcol.alloc(R * kStage));
+    LT_TRY(stval.alloc(R * kStage));
     hipLaunchKernelGGL(k_raster_count, dim3(blocks_for(R, kRasterThreads)), dim3(kRasterThreads), 0, stream(), G,
-                       src.p, lattice.p, mask.p, nnz_row.p, nfrag.p, minidx.p, lastne.p, dstatus.p);
+                       src.p, lattice.p, mask.p, nnz_row.p, nfrag.p, minidx.p, lastne.p, dstatus.p, stcol.p, stval.p);
     LT_CHECK_LAUNCH();
     // long rows: compacted on the device; their fragments are produced by the
     // frontier decomposition (k_long_expand / k_long_sub_*), sorted by
@@ -673,7 +712,8 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
     hipLaunchKernelGGL(k_rowptr, dim3(blocks_for(R + 1, 256)), dim3(256), 0, stream(), rowoff.p, R, L.rp.p);
     LT_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_raster_fill, dim3(blocks_for(R, kRasterThreads)), dim3(kRasterThreads), 0, stream(), G,
-                       src.p, srcoff.p, lattice.p, mask.p, rowoff.p, nfrag.p, minidx.p, lastne.p, L.ci.p, L.va.p);
+                       src.p, srcoff.p, lattice.p, mask.p, rowoff.p, nfrag.p, minidx.p, lastne.p, stcol.p, stval.p,
+                       L.ci.p, L.va.p);
     LT_CHECK_LAUNCH();
     if (nlong) {
         hipLaunchKernelGGL(k_long_copy, dim3(nlong), dim3(256), 0, stream(), drows.p, ebase.p, nlong, rowoff.p,

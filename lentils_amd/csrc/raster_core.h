@@ -220,13 +220,16 @@ __host__ __device__ int raster_triangle(const double *tv, const SrcGrid &g, Sink
 // much of the source plane) are rasterized in parallel without changing the
 // emission order: the split tree of each fan triangle is expanded breadth
 // first, keeping every level in the reference's depth-first order (children
-// last-pushed first, fragments in place), until the frontier would exceed
-// kLongFrontier items; each remaining subtree is then walked depth first by
-// its own thread, starting with the stack depth the reference would have at
-// that node.  Concatenating the frontier's outputs in order reproduces the
-// reference's fragment sequence exactly.
-constexpr int kLongFrontier = 512;
-constexpr int kLongLevels = 40;
+// last-pushed first, fragments in place) and splitting only the nodes that
+// still span more than kLongSplitCells source cells, until none does or the
+// frontier would exceed kLongFrontier items; each remaining subtree is then
+// walked depth first by its own thread, starting with the stack depth the
+// reference would have at that node.  Concatenating the frontier's outputs
+// in order reproduces the reference's fragment sequence exactly (any cut of
+// the split tree listed in depth-first order does).
+constexpr int kLongFrontier = 2048;
+constexpr int kLongLevels = 64;
+constexpr long kLongSplitCells = 32;
 
 struct RItem {
     RTri tri;
@@ -235,6 +238,11 @@ struct RItem {
     double area;
     int idx;
 };
+
+// whether the breadth-first expansion splits this frontier item further
+LT_HD bool ritem_splits(const RItem &it) {
+    return it.kind == 0 && (long)(it.tri.hi[0] - it.tri.lo[0]) * (long)(it.tri.hi[1] - it.tri.lo[1]) > kLongSplitCells;
+}
 
 // number of frontier items `it` turns into at the next level; sets *ovf
 // when expanding it would overflow the reference's stack

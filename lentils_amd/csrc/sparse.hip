@@ -34,13 +34,24 @@ struct RowFilter {
     }
 };
 
+// Workgroup order for the 8 XCDs (each with its own L2): the dispatcher
+// hands workgroup b to XCD b % 8, so renumbering the i-th workgroup of XCD x
+// to start_x + i gives every XCD one contiguous range of the rows (and of
+// the vector entries they gather), processed in increasing order.  A
+// bijection for any grid size.
+__device__ inline long xcd_block(long b, long nb) {
+    constexpr long X = 8;
+    const long x = b % X, i = b / X, q = nb / X, r = nb % X;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // T: value and vector type (double, or float for the precision form)
 template <int LANES, class T>
 __global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32_t *__restrict__ rp,
                                                const int32_t *__restrict__ ci, const T *__restrict__ va,
                                                const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
                                                int skip_long, RowFilter F) {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const long row = row0 + gid / LANES;
     const int lane = (int)(gid % LANES);
     if (row >= nrows || F.skip(row)) return;  // whole lane groups leave together
@@ -72,7 +83,8 @@ __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__
                                                       const T *__restrict__ va, const T *__restrict__ x,
                                                       T *__restrict__ y, T alpha, T beta, int skip_long, RowFilter F) {
     __shared__ T prod[kStreamNnz];
-    const long b = blockIdx.x;
+    __shared__ int32_t rps[kStreamNnz + 1];
+    const long b = xcd_block(blockIdx.x, gridDim.x);
     const long r0 = rblk[b], r1 = rblk[b + 1];
     if (r0 >= r1) return;  // inside a long row: no row starts here
     if (F.act) {
@@ -81,11 +93,31 @@ __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__
         if (!any) return;
     }
     const long k0 = b * kStreamNnz, k1 = min(k0 + kStreamNnz, nnz);
-    for (long k = k0 + threadIdx.x; k < k1; k += 256) prod[k - k0] = va[k] * x[ci[k]];
+    constexpr int PER = kStreamNnz / 256;
+    // the row pointers of the block's rows go to LDS alongside the products
+    const long nr = r1 - r0;
+    const bool rp_lds = nr < kStreamNnz;
+    if (rp_lds)
+        for (long r = threadIdx.x; r <= nr; r += 256) rps[r] = rp[r0 + r];
+    if (k1 - k0 == kStreamNnz) {  // full range: all loads of a thread in flight together
+        int c[PER];
+        T v[PER], xv[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            c[u] = ci[k0 + threadIdx.x + 256 * u];
+            v[u] = va[k0 + threadIdx.x + 256 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) prod[threadIdx.x + 256 * u] = v[u] * xv[u];
+    } else {
+        for (long k = k0 + threadIdx.x; k < k1; k += 256) prod[k - k0] = va[k] * x[ci[k]];
+    }
     __syncthreads();
     for (long r = r0 + threadIdx.x; r < r1; r += 256) {
         if (F.skip(r)) continue;
-        const int rb = rp[r], re = rp[r + 1];
+        const int rb = rp_lds ? rps[r - r0] : rp[r], re = rp_lds ? rps[r - r0 + 1] : rp[r + 1];
         if (skip_long && re - rb > kLongRow) continue;
         const int ein = (int)min((long)re, k1);
         T s = (T)0;

@@ -94,13 +94,19 @@ long hc_pixel_frags(const double *lat, int ny, int i, int j, int snx, int sny, d
                 std::vector<int> cnt(cur.size() + 1, 0);
                 int ovf = 0, any_sub = 0;
                 for (size_t q = 0; q < cur.size(); ++q) {
-                    cnt[q + 1] = cnt[q] + ritem_count(cur[q], g, &ovf);
-                    any_sub |= (cur[q].kind == 0);
+                    const bool ex = ritem_splits(cur[q]);
+                    cnt[q + 1] = cnt[q] + (ex ? ritem_count(cur[q], g, &ovf) : 1);
+                    any_sub |= ex;
                 }
                 if (ovf) return -1;
                 if (!any_sub || cnt.back() > front) break;
                 nxt.assign(cnt.back(), RItem{});
-                for (size_t q = 0; q < cur.size(); ++q) ritem_write(cur[q], g, nxt.data() + cnt[q]);
+                for (size_t q = 0; q < cur.size(); ++q) {
+                    if (ritem_splits(cur[q]))
+                        ritem_write(cur[q], g, nxt.data() + cnt[q]);
+                    else
+                        nxt[cnt[q]] = cur[q];
+                }
                 cur.swap(nxt);
             }
             if (level > levels_max) levels_max = level;

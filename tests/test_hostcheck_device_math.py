@@ -118,7 +118,7 @@ def test_long_row_decomposition_reproduces_emission_order(hc, oracle, case):
     for (i, j) in pix:
         ref, _ = _frags(hc, lat, n, i, j, src, 0, 0)
         longest = max(longest, ref[0].size)
-        for front in (1, 4, 16, 512):
+        for front in (1, 4, 16, 512, 2048):
             got, lev = _frags(hc, lat, n, i, j, src, 1, front)
             if got is None:  # the GPU would take the sequential path for this row
                 continue
