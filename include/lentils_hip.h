@@ -296,6 +296,12 @@ lt_status lt_solver_set_lens_pc(lt_solver s, int inner_steps, int max_row, doubl
  * over all samples' source pixels: nnz and the c used; row_ptr (ntot + 1),
  * cols and vals (nnz) are written when non-NULL. */
 lt_status lt_solver_lens_pc_matrix(lt_solver s, long *nnz, double *c_used, int *row_ptr, int *cols, double *vals);
+/* No reference counterpart: on (default), the FP64 CG (plain, Jacobi and
+ * lens-preconditioned; not the precision form or the learned preconditioner)
+ * runs its blocks of iterations between convergence checks as captured HIP
+ * graphs, one per launch width (active samples rounded up to 8); off: the
+ * same kernels launched one by one.  Results are identical. */
+lt_status lt_solver_set_graphs(lt_solver s, int on);
 /* replace the right-hand sides (all samples concatenated, host) */
 lt_status lt_solver_set_rhs(lt_solver s, const double *rhs);
 /* y = A x for all samples, in the precision set by lt_solver_set_precision */

@@ -544,6 +544,17 @@ struct Solver {
     DevBuf<int> usenet;            // 1: network direction, 0: plain CG fallback
     std::vector<int> nfall_host, usenet_host;
 
+    // CG blocks of check_every iterations captured as HIP graphs, one per
+    // launch width (active samples rounded up to a multiple of 8)
+    bool use_graphs = true;
+    std::map<int, hipGraphExec_t> graphs;
+    std::map<int, int> graph_warm;
+    void drop_graphs() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+        graph_warm.clear();
+    }
+    ~Solver() { drop_graphs(); }
     DevBuf<double> dotpart;  // S x dot_parts partial sums of seg_dot
     int dot_parts = 1;
     // out[s] = a_s . b_s for every sample s (deterministic order)
@@ -897,7 +908,7 @@ lt_status Solver::lens_build_matrix() {
 // M = lam_c Cc^T Cc + c L_s^T L_s from z = 0, every sample at once
 lt_status Solver::lens_apply(const int *af) {
     const unsigned gb = blocks_for(ntot, 256);
-    LT_TRY(ix.zero());
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(256), 0, stream(), ntot, 0.0, ix.p);  // a kernel: graph-safe
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, r.p, ir.p);
     hipLaunchKernelGGL(k_divide, dim3(gb), dim3(256), 0, stream(), ntot, ir.p, dM.p, iz.p);
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, iz.p, ip.p);
@@ -1147,14 +1158,99 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     std::vector<int> hact(S, 1), alist(S);
     int nact = S;
     for (int s = 0; s < S; ++s) alist[s] = s;
+    // the device list of samples to launch; fixed capacity S so that its
+    // address never changes (captured graphs refer to it)
     DevBuf<int> dlist;
     LT_TRY(dlist.upload(alist.data(), S));
+    auto set_list = [&](const int *h, int n) -> lt_status {
+        if (n > 0) LT_HIP(hipMemcpyAsync(dlist.p, h, (size_t)n * sizeof(int), hipMemcpyHostToDevice, stream()));
+        return LT_OK;
+    };
     const int check_every = 5;
     int outer = 0;
     int it = 0;
     std::vector<double> hrr(S), hbb(S);
+    // one FP64 CG iteration over the samples in `list` (nl entries; samples
+    // that are not active ride along with zero steps)
+    auto iter_body = [&](const int *list, int nl) -> lt_status {
+        LT_TRY(apply(p.p, Ap.p, list, nl, act.p));
+        LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
+        hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
+        hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
+        LT_TRY(seg_dot(r.p, r.p, rr.p));
+        LT_CHECK_LAUNCH();
+        LT_TRY(precondition(precond, dinv_p));
+        LT_TRY(seg_dot(r.p, z.p, rz_new.p));
+        if (flexible) {
+            LT_TRY(seg_dot(z.p, Ap.p, zap.p));
+            hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
+                               alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
+        } else {
+            hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, rr.p, bb.p,
+                               rtol2, beta.p, iters.p);
+        }
+        hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, z.p, p.p);
+        LT_CHECK_LAUNCH();
+        return LT_OK;
+    };
+    // Graph mode: blocks of check_every iterations between the host's
+    // convergence checks are captured once per launch width (active samples
+    // rounded up to 8, padded with inactive ones) and replayed; the first
+    // block at a new width runs uncaptured so that every lazily created
+    // buffer and FFT plan exists before capture.  Same arithmetic as the
+    // per-iteration loop.
+    drop_graphs();
+    const bool graph_mode = use_graphs && !mixed && !learned;
+    std::vector<int> padded(S);
     for (int round = 0;; ++round) {
-    for (; it < maxiter && nact > 0; ++it) {
+    while (graph_mode && it < maxiter && nact > 0) {
+        const int nb = std::min(check_every, maxiter - it);
+        const int nl = std::min(S, (nact + 7) / 8 * 8);
+        int q = 0;
+        for (int s = 0; s < S; ++s)
+            if (hact[s]) padded[q++] = s;
+        for (int s = 0; s < S && q < nl; ++s)
+            if (!hact[s]) padded[q++] = s;
+        LT_TRY(set_list(padded.data(), nl));
+        bool done = false;
+        if (nb == check_every && use_graphs) {
+            auto g = graphs.find(nl);
+            if (g == graphs.end() && graph_warm[nl] > 0) {
+                hipGraph_t gr = nullptr;
+                LT_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+                lt_status st = LT_OK;
+                for (int k = 0; k < nb && st == LT_OK; ++k) st = iter_body(dlist.p, nl);
+                const hipError_t e = hipStreamEndCapture(stream(), &gr);
+                hipGraphExec_t ge = nullptr;
+                if (st == LT_OK && e == hipSuccess && gr &&
+                    hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) == hipSuccess) {
+                    graphs[nl] = ge;
+                    g = graphs.find(nl);
+                } else {
+                    use_graphs = false;  // capture not possible here: plain launches from now on
+                    (void)hipGetLastError();
+                }
+                if (gr) (void)hipGraphDestroy(gr);
+            }
+            if (g != graphs.end()) {
+                LT_HIP(hipGraphLaunch(g->second, stream()));
+                done = true;
+            }
+        }
+        if (!done) {
+            for (int k = 0; k < nb; ++k) LT_TRY(iter_body(dlist.p, nl));
+            ++graph_warm[nl];
+        }
+        it += nb;
+        outer += nb;
+        LT_TRY(act.download(hact.data(), S));
+        LT_TRY(sync());
+        nact = 0;
+        for (int s = 0; s < S; ++s)
+            if (hact[s]) alist[nact++] = s;
+    }
+    if (graph_mode && nact > 0) LT_TRY(set_list(alist.data(), nact));
+    for (; !graph_mode && it < maxiter && nact > 0; ++it) {
         if (mixed)
             LT_TRY(apply32(p.p, Ap.p, dlist.p, nact, act.p));
         else
@@ -1212,7 +1308,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             nact = 0;
             for (int s = 0; s < S; ++s)
                 if (hact[s]) alist[nact++] = s;
-            if (nact > 0) LT_TRY(dlist.upload(alist.data(), nact));
+            if (nact > 0) LT_TRY(set_list(alist.data(), nact));
             if (learned) {
                 const size_t back = kWindow / check_every;
                 int nsw = 0;
@@ -1254,7 +1350,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     }
     if (nact == 0 || it >= maxiter || round >= 3) break;
     LT_TRY(act.upload(hact.data(), S));
-    LT_TRY(dlist.upload(alist.data(), nact));
+    LT_TRY(set_list(alist.data(), nact));
     LT_TRY(precondition(precond, dinv_p));
     hipLaunchKernelGGL(k_restart_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, z.p, p.p);
     LT_TRY(seg_dot(r.p, z.p, rz.p));
@@ -1523,6 +1619,12 @@ lt_status lt_solver_lens_pc_matrix(lt_solver h, long *nnz, double *c_used, int *
     if (cols && S.Mpc.nnz > 0) LT_TRY(S.Mpc.ci.download(cols, S.Mpc.nnz));
     if (vals && S.Mpc.nnz > 0) LT_TRY(S.Mpc.va.download(vals, S.Mpc.nnz));
     return sync();
+}
+
+lt_status lt_solver_set_graphs(lt_solver h, int on) {
+    if (!h) return LT_ERR_ARG;
+    h->s.use_graphs = on != 0;
+    return LT_OK;
 }
 
 lt_status lt_solver_set_rhs(lt_solver h, const double *rhs) {

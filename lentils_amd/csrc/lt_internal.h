@@ -97,9 +97,6 @@ inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads
 lt_status exclusive_scan_i64(const int64_t *in, int64_t *out, long n);
 // inclusive running max of int32 values (used for "last non-empty row")
 lt_status inclusive_max_i32(const int32_t *in, int32_t *out, long n);
-// ascending sort of keys[seg[s]..seg[s+1]) for each segment (seg_host: the
-// same offsets on the host)
-lt_status segmented_sort_u64(uint64_t *keys, const int64_t *seg, const std::vector<int64_t> &seg_host);
 
 // ---------------------------------------------------------------------------
 // per-lens constants, precomputed on the host with the host libm so that the

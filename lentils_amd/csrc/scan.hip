@@ -133,101 +133,5 @@ lt_status inclusive_max_i32(const int32_t *in, int32_t *out, long n) {
     return run_scan<OpMax, false>(in, out, n, nullptr);
 }
 
-// ---------------------------------------------------------------------------
-// segmented sort of uint64 keys
-// ---------------------------------------------------------------------------
-namespace {
-
-constexpr int kSortThreads = 256;
-constexpr int kSortLds = 4096;
-
-__global__ void k_sort_short(uint64_t *keys, const int64_t *seg, const int32_t *ids, int nseg) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    int id = ids ? ids[s] : s;
-    long b = seg[id], e = seg[id + 1];
-    if (e - b > 32) return;
-    for (long a = b + 1; a < e; ++a) {
-        uint64_t k = keys[a];
-        long c = a - 1;
-        while (c >= b && keys[c] > k) {
-            keys[c + 1] = keys[c];
-            --c;
-        }
-        keys[c + 1] = k;
-    }
-}
-
-__device__ void bitonic(uint64_t *a, int P) {
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P; i += blockDim.x) {
-                int l = i ^ j;
-                if (l > i) {
-                    uint64_t x = a[i], y = a[l];
-                    bool up = ((i & k) == 0);
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[l] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// one workgroup per long segment; scratch holds P(seg) keys for segments
-// above the LDS size (offsets in scr_off)
-__global__ void __launch_bounds__(kSortThreads) k_sort_long(uint64_t *keys, const int64_t *seg, const int32_t *ids,
-                                                             const int64_t *scr_off, uint64_t *scratch) {
-    __shared__ uint64_t sh[kSortLds];
-    const int id = ids[blockIdx.x];
-    const long b = seg[id], n = seg[id + 1] - b;
-    int P = 1;
-    while (P < n) P <<= 1;
-    uint64_t *a = (P <= kSortLds) ? sh : scratch + scr_off[blockIdx.x];
-    for (int i = threadIdx.x; i < P; i += blockDim.x) a[i] = (i < n) ? keys[b + i] : ~0ull;
-    __syncthreads();
-    bitonic(a, P);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) keys[b + i] = a[i];
-}
-
-}  // namespace
-
-// Sort keys[seg[s]..seg[s+1]) ascending for every segment s < nseg.
-// seg is a device array of nseg+1 offsets; seg_host is the same on the host
-// (needed to size the scratch of long segments).
-lt_status segmented_sort_u64(uint64_t *keys, const int64_t *seg, const std::vector<int64_t> &seg_host) {
-    const int nseg = (int)seg_host.size() - 1;
-    if (nseg <= 0) return LT_OK;
-    hipLaunchKernelGGL(k_sort_short, dim3(blocks_for(nseg, 256)), dim3(256), 0, stream(), keys, seg,
-                       (const int32_t *)nullptr, nseg);
-    LT_CHECK_LAUNCH();
-    std::vector<int32_t> longids;
-    std::vector<int64_t> scr;
-    int64_t scr_total = 0;
-    for (int s = 0; s < nseg; ++s) {
-        int64_t n = seg_host[s + 1] - seg_host[s];
-        if (n > 32) {
-            longids.push_back(s);
-            int64_t P = 1;
-            while (P < n) P <<= 1;
-            scr.push_back(scr_total);
-            if (P > kSortLds) scr_total += P;
-        }
-    }
-    if (longids.empty()) return LT_OK;
-    DevBuf<int32_t> dids;
-    DevBuf<int64_t> doff;
-    DevBuf<uint64_t> dscr;
-    LT_TRY(dids.upload(longids.data(), longids.size()));
-    LT_TRY(doff.upload(scr.data(), scr.size()));
-    LT_TRY(dscr.alloc(std::max<int64_t>(scr_total, 1)));
-    hipLaunchKernelGGL(k_sort_long, dim3((unsigned)longids.size()), dim3(kSortThreads), 0, stream(), keys, seg,
-                       dids.p, doff.p, dscr.p);
-    LT_CHECK_LAUNCH();
-    return sync();
-}
 
 }  // namespace lt

@@ -1,10 +1,16 @@
 // CSR sparse matrix kernels: SpMV and a deterministic transpose.
 //
-// SpMV assigns a power-of-two group of lanes to each row (chosen from the
-// mean row length), strides the group over the row and reduces with
-// wavefront shuffles, so every row is summed in a fixed order.  It is
-// HBM-bound: (4 + 8) B per nonzero + 4 B per row + x gathers.
+// SpMV either assigns a power-of-two group of lanes to each row, strides the
+// group over the row and reduces with wavefront shuffles, or streams fixed
+// ranges of nonzeros through LDS (k_spmv_stream); the form is timed per
+// matrix shape (csr_autotune).  Every row is summed in a fixed order.  Work:
+// (4 + 8) B per nonzero + 4 B per row + x gathers.  The transpose is one radix
+// sort of (column, position) keys.
 #include <algorithm>
+
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
 
 #include "lt_internal.h"
 
@@ -212,10 +218,6 @@ __global__ void k_long_rows_fill(long n, const int32_t *__restrict__ rp, const i
     }
 }
 
-__global__ void k_col_count(long nnz, const int32_t *__restrict__ ci, int64_t *__restrict__ cnt) {
-    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nnz) atomicAdd((unsigned long long *)&cnt[ci[k]], 1ull);
-}
 
 __global__ void k_row_of(long nrows, const int32_t *__restrict__ rp, int32_t *__restrict__ rowof) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,28 +225,37 @@ __global__ void k_row_of(long nrows, const int32_t *__restrict__ rp, int32_t *__
     for (int k = rp[r]; k < rp[r + 1]; ++k) rowof[k] = (int32_t)r;
 }
 
-__global__ void k_scatter_keys(long nnz, const int32_t *__restrict__ ci, const int64_t *__restrict__ off,
-                               int64_t *__restrict__ cursor, uint64_t *__restrict__ keys) {
+
+__global__ void k_col_keys(long nnz, const int32_t *__restrict__ ci, int kb, uint64_t *__restrict__ keys) {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    const int c = ci[k];
-    const long pos = off[c] + (long)atomicAdd((unsigned long long *)&cursor[c], 1ull);
-    keys[pos] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)k;
+    if (k < nnz) keys[k] = ((uint64_t)(uint32_t)ci[k] << kb) | (uint64_t)k;
+}
+
+__global__ void k_col_starts(long nc, long nnz, const uint64_t *__restrict__ keys, int kb, int32_t *__restrict__ rp) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > nc) return;
+    const uint64_t key = (uint64_t)c << kb;
+    long lo = 0, hi = nnz;
+    while (lo < hi) {
+        const long mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    rp[c] = (int32_t)lo;
 }
 
 __global__ void k_gather_t(long nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ rowof,
-                           const double *__restrict__ va, int32_t *__restrict__ tci, double *__restrict__ tva) {
+                           const double *__restrict__ va, int32_t *__restrict__ tci, double *__restrict__ tva,
+                           uint64_t kmask) {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nnz) return;
-    const uint32_t k = (uint32_t)keys[p];
+    const long k = (long)(keys[p] & kmask);
     tci[p] = rowof[k];
     tva[p] = va[k];
 }
 
-__global__ void k_i64_to_i32(long n, const int64_t *__restrict__ a, int32_t *__restrict__ b) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (int32_t)a[i];
-}
 
 }  // namespace
 
@@ -463,40 +474,38 @@ lt_status csr_transpose(const DevCSR &A, DevCSR &B) {
     B.nrows = nc;
     B.ncols = A.nrows;
     B.nnz = nnz;
-    DevBuf<int64_t> cnt, off, cursor;
-    DevBuf<int32_t> rowof;
-    DevBuf<uint64_t> keys;
-    LT_TRY(cnt.alloc(nc + 1));
-    LT_TRY(cnt.zero());
-    LT_TRY(off.alloc(nc + 1));
-    LT_TRY(cursor.alloc(nc + 1));
-    LT_TRY(cursor.zero());
-    LT_TRY(rowof.alloc(std::max<long>(nnz, 1)));
-    LT_TRY(keys.alloc(std::max<long>(nnz, 1)));
     LT_TRY(B.rp.alloc(nc + 1));
     LT_TRY(B.ci.alloc(std::max<long>(nnz, 1)));
     LT_TRY(B.va.alloc(std::max<long>(nnz, 1)));
-    if (nnz > 0) {
-        hipLaunchKernelGGL(k_col_count, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream(), nnz, A.ci.p, cnt.p);
-        LT_CHECK_LAUNCH();
+    if (nnz == 0) {
+        LT_TRY(B.rp.zero());
+        return sync();
     }
-    LT_TRY(exclusive_scan_i64(cnt.p, off.p, nc));
-    if (nnz > 0) {
-        hipLaunchKernelGGL(k_row_of, dim3(blocks_for(A.nrows, 256)), dim3(256), 0, stream(), A.nrows, A.rp.p,
-                           rowof.p);
-        LT_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_scatter_keys, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream(), nnz, A.ci.p, off.p,
-                           cursor.p, keys.p);
-        LT_CHECK_LAUNCH();
-        std::vector<int64_t> seg(nc + 1);
-        LT_TRY(off.download(seg.data(), nc + 1));
-        LT_TRY(sync());
-        LT_TRY(segmented_sort_u64(keys.p, off.p, seg));
-        hipLaunchKernelGGL(k_gather_t, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream(), nnz, keys.p, rowof.p,
-                           A.va.p, B.ci.p, B.va.p);
-        LT_CHECK_LAUNCH();
-    }
-    hipLaunchKernelGGL(k_i64_to_i32, dim3(blocks_for(nc + 1, 256)), dim3(256), 0, stream(), nc + 1, off.p, B.rp.p);
+    // one radix sort of (column, position) keys: ascending columns, and
+    // inside a column ascending source rows, independent of any atomics
+    int kb = 1, cb = 1;
+    while ((1L << kb) < nnz) ++kb;
+    while ((1L << cb) < nc) ++cb;
+    DevBuf<int32_t> rowof;
+    DevBuf<uint64_t> keys, keys2;
+    LT_TRY(rowof.alloc(nnz));
+    LT_TRY(keys.alloc(nnz));
+    LT_TRY(keys2.alloc(nnz));
+    hipLaunchKernelGGL(k_row_of, dim3(blocks_for(A.nrows, 256)), dim3(256), 0, stream(), A.nrows, A.rp.p, rowof.p);
+    LT_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_col_keys, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream(), nnz, A.ci.p, kb, keys.p);
+    LT_CHECK_LAUNCH();
+    size_t tb = 0;
+    LT_HIP(rocprim::radix_sort_keys(nullptr, tb, keys.p, keys2.p, (size_t)nnz, 0, kb + cb, stream()));
+    DevBuf<uint8_t> tmp;
+    LT_TRY(tmp.alloc(std::max<size_t>(tb, 1)));
+    LT_HIP(rocprim::radix_sort_keys(tmp.p, tb, keys.p, keys2.p, (size_t)nnz, 0, kb + cb, stream()));
+    hipLaunchKernelGGL(k_gather_t, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream(), nnz, keys2.p, rowof.p, A.va.p,
+                       B.ci.p, B.va.p, (1ull << kb) - 1);
+    LT_CHECK_LAUNCH();
+    // row pointers of the transpose: first sorted key of each column
+    hipLaunchKernelGGL(k_col_starts, dim3(blocks_for(nc + 1, 256)), dim3(256), 0, stream(), nc, nnz, keys2.p, kb,
+                       B.rp.p);
     LT_CHECK_LAUNCH();
     return sync();
 }

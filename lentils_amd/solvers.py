@@ -229,6 +229,11 @@ class BatchedSourceInversion:
         B.check(B.lib().lt_solver_rhs(self._handle, B.ptr(out)))
         return self._split(out)
 
+    def set_graphs(self, on=True):
+        """Run the FP64 CG's iteration blocks as captured HIP graphs (default)
+        or as individual launches; the results are identical."""
+        B.check(B.lib().lt_solver_set_graphs(self._handle, 1 if on else 0))
+
     def set_lens_preconditioner(self, inner_steps=3, max_row=64, c=0.0):
         B.check(B.lib().lt_solver_set_lens_pc(self._handle, int(inner_steps), int(max_row), float(c)))
 
