@@ -67,6 +67,9 @@ def blob_source(src, seed=14):
     return img
 
 
+GRAPHS = True  # CG iteration blocks as captured HIP graphs (bench --no-graphs turns them off)
+
+
 class Workload:
     """Inputs of one step, resident before timing.  Default: every lens
     sample has its own observation (its own Gaussian-blob source lensed by
@@ -132,6 +135,7 @@ class Workload:
         else:
             inv = BatchedSourceInversion(lb, self.plan, dirty_images=self.ddirty,
                                          lambda_curvature=self.lambda_curv, operator=operator)
+        inv.set_graphs(GRAPHS)
         x, its, res = inv.solve(rtol=rtol, maxiter=maxiter, precond=precond, precision=precision)
         return its, res
 
@@ -422,9 +426,13 @@ def main():
                     help='the batching workload (256 samples, uneven source grids, 1e6 visibilities) instead')
     ap.add_argument('--profile-degrid', action='store_true',
                     help='only run the degridder and the calibration copy (for rocprofv3 --pmc)')
+    ap.add_argument('--no-graphs', action='store_true',
+                    help='launch the CG kernels one by one instead of as captured HIP graphs (for kernel tracing)')
     ap.add_argument('--shared-data', action='store_true',
                     help='one observation for all samples (most samples then fit the data with the wrong lens)')
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     if args.profile_degrid:
         profile_degrid()
         return

@@ -272,10 +272,12 @@ __global__ void __launch_bounds__(kExpandThreads) k_long_expand(BatchGeom G, con
 // for the sequential path.
 __global__ void k_long_sub_count(const int64_t *__restrict__ rows, long npix, const SrcGrid *__restrict__ src,
                                  const RItem *__restrict__ front, const int32_t *__restrict__ nfront, int nb,
-                                 int *__restrict__ fallback, int64_t *__restrict__ cnt) {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                 const int32_t *__restrict__ order, int *__restrict__ fallback,
+                                 int64_t *__restrict__ cnt) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * kLongFrontier) return;
+    const long gid = order[t];  // items in decreasing size: similar walks share a wave
     const int b = (int)(gid / kLongFrontier), q = (int)(gid - (long)b * kLongFrontier);
-    if (b >= nb) return;
     if (q >= nfront[b]) {
         cnt[gid] = 0;
         return;
@@ -290,6 +292,26 @@ __global__ void k_long_sub_count(const int64_t *__restrict__ rows, long npix, co
     CountSink cs;
     if (raster_dfs(it.tri, it.depth, g, cs)) atomicOr(&fallback[l], 1);
     cnt[gid] = cs.n;
+}
+
+// walk-order keys of the frontier slots: larger source-cell extent first,
+// empty slots last
+__global__ void k_item_keys(long nitems, const RItem *__restrict__ front, const int32_t *__restrict__ nfront,
+                            uint32_t *__restrict__ keys, int32_t *__restrict__ ids) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nitems) return;
+    const int b = (int)(gid / kLongFrontier), q = (int)(gid - (long)b * kLongFrontier);
+    uint32_t k = 0xffffffffu;
+    if (q < nfront[b]) {
+        const RItem &it = front[(long)b * 2 * kLongFrontier + q];
+        long ext = 0;
+        if (it.kind == 0) ext = (long)(it.tri.hi[0] - it.tri.lo[0]) * (long)(it.tri.hi[1] - it.tri.lo[1]);
+        if (ext < 0) ext = 0;
+        if (ext > 0x7fffffffL) ext = 0x7fffffffL;
+        k = 0x7fffffffu - (uint32_t)ext;
+    }
+    keys[gid] = k;
+    ids[gid] = (int32_t)gid;
 }
 
 // fragment count of every huge row: from the frontier counts, or by the
@@ -323,12 +345,14 @@ __global__ void k_long_rowtot(BatchGeom G, const SrcGrid *__restrict__ src, cons
 // emission position
 __global__ void k_long_sub_emit(const int64_t *__restrict__ rows, const int32_t *__restrict__ hmap, long npix,
                                 const SrcGrid *__restrict__ src, const RItem *__restrict__ front,
-                                const int32_t *__restrict__ nfront, int nb, const int *__restrict__ fallback,
-                                const int64_t *__restrict__ pos, const int64_t *__restrict__ foff,
-                                double *__restrict__ area, uint64_t *__restrict__ key) {
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                const int32_t *__restrict__ nfront, int nb, const int32_t *__restrict__ order,
+                                const int *__restrict__ fallback, const int64_t *__restrict__ pos,
+                                const int64_t *__restrict__ foff, double *__restrict__ area,
+                                uint64_t *__restrict__ key) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * kLongFrontier) return;
+    const long gid = order[t];
     const int b = (int)(gid / kLongFrontier), q = (int)(gid - (long)b * kLongFrontier);
-    if (b >= nb) return;
     const int l = b >> 3;
     if (q >= nfront[b] || fallback[l]) return;
     const long n0 = pos[gid] - pos[(long)l * 8 * kLongFrontier];
@@ -605,6 +629,7 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
         DevBuf<int32_t> dhmap, nfront;
         DevBuf<int64_t> dhrow, cnt, pos, tot;
         DevBuf<RItem> front;
+        DevBuf<int32_t> order;  // walk order of the frontier slots
         DevBuf<int> hfb;
         const int nb = nhuge * 8;
         const long nitems = (long)nb * kLongFrontier;
@@ -621,8 +646,27 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
             hipLaunchKernelGGL(k_long_expand, dim3(nb), dim3(kExpandThreads), 0, stream(), G, src.p, lattice.p,
                                dhrow.p, nhuge, front.p, nfront.p, hfb.p);
             LT_CHECK_LAUNCH();
+            {  // walk order: items by decreasing extent (a permutation of the slots)
+                DevBuf<uint32_t> k1, k2;
+                DevBuf<int32_t> i1;
+                LT_TRY(k1.alloc(nitems));
+                LT_TRY(k2.alloc(nitems));
+                LT_TRY(i1.alloc(nitems));
+                LT_TRY(order.alloc(nitems));
+                hipLaunchKernelGGL(k_item_keys, dim3(blocks_for(nitems, 256)), dim3(256), 0, stream(), nitems,
+                                   front.p, nfront.p, k1.p, i1.p);
+                LT_CHECK_LAUNCH();
+                size_t tb = 0;
+                LT_HIP(rocprim::radix_sort_pairs(nullptr, tb, k1.p, k2.p, i1.p, order.p, (size_t)nitems, 0, 32,
+                                                 stream()));
+                DevBuf<uint8_t> tmp;
+                LT_TRY(tmp.alloc(std::max<size_t>(tb, 1)));
+                LT_HIP(rocprim::radix_sort_pairs(tmp.p, tb, k1.p, k2.p, i1.p, order.p, (size_t)nitems, 0, 32,
+                                                 stream()));
+                LT_TRY(sync());
+            }
             hipLaunchKernelGGL(k_long_sub_count, dim3(blocks_for(nitems, 256)), dim3(256), 0, stream(), dhrow.p,
-                               npix, src.p, front.p, nfront.p, nb, hfb.p, cnt.p);
+                               npix, src.p, front.p, nfront.p, nb, order.p, hfb.p, cnt.p);
             LT_CHECK_LAUNCH();
             LT_TRY(exclusive_scan_i64(cnt.p, pos.p, nitems));
             hipLaunchKernelGGL(k_long_rowtot, dim3(blocks_for(nhuge, 64)), dim3(64), 0, stream(), G, src.p,
@@ -654,7 +698,8 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
         LT_TRY(fkey2.alloc(F));
         if (nhuge) {
             hipLaunchKernelGGL(k_long_sub_emit, dim3(blocks_for(nitems, 256)), dim3(256), 0, stream(), dhrow.p,
-                               dhmap.p, npix, src.p, front.p, nfront.p, nb, hfb.p, pos.p, dfoff.p, farea.p, fkey.p);
+                               dhmap.p, npix, src.p, front.p, nfront.p, nb, order.p, hfb.p, pos.p, dfoff.p, farea.p,
+                               fkey.p);
             LT_CHECK_LAUNCH();
         }
         hipLaunchKernelGGL(k_long_emit, dim3(blocks_for(nlong, 64)), dim3(64), 0, stream(), G, src.p, lattice.p,
