@@ -496,7 +496,7 @@ def main():
         lp = {'value': NLENS / tl, 'unit': 'source solves/s', 'ms_per_step': 1e3 * tl,
               'iters_median': float(np.median(itsl)), 'iters_max': int(itsl.max()),
               'true_rel_resid_max': float(resl.max()), 'all_converged': bool(np.all(resl <= 1e-6)),
-              'method': 'flexible CG, preconditioner: 3 Jacobi-PCG steps on the explicit lambda Cc^T Cc + c L_s^T L_s (L_s: rows of L with <= 64 entries)'}
+              'method': 'flexible CG, preconditioner: 3 Jacobi-PCG steps on the explicit lambda Cc^T Cc + c L_s^T L_s (L_s: rows of L with <= 64 nonzero entries)'}
         log(f'lens preconditioner: {tl:.2f}s iters median {np.median(itsl):.0f} max res {resl.max():.2e}')
     rl = degrid_roofline(wl)
     log(f'degrid f32 {rl["ms"] * 1e3:.1f} us/launch, {rl["gbs"]:.0f} GB/s, {rl["gvis_s"]:.2f} Gvis/s')
@@ -521,7 +521,7 @@ def main():
         'precision_form': pf,
         'lens_preconditioner': lp,
         'degrid_gvis_s': rl['gvis_s'], 'degrid_gvis_s_f64': rl['gvis_s_f64'],
-        'roofline': {'kernel': 'k_degrid<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
+        'roofline': {'kernel': 'k_degrid32<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS,
                      'traffic': load_traffic(), 'algorithmic_bytes': rl['bytes'],
                      'stream_copy_gbs': copy_gbs},

@@ -292,7 +292,7 @@ lt_status lt_solver_set_precision(lt_solver s, int precision, int refine_every);
  * image centre, the default). */
 lt_status lt_solver_set_lens_pc(lt_solver s, int inner_steps, int max_row, double c);
 /* The LT_PC_LENS matrix M = c L_s^T L_s + lambda Cc^T Cc (L_s: the rows of
- * L with at most max_row entries), built on first use, as block-diagonal CSR
+ * L with at most max_row nonzero entries), built on first use, as block-diagonal CSR
  * over all samples' source pixels: nnz and the c used; row_ptr (ntot + 1),
  * cols and vals (nnz) are written when non-NULL. */
 lt_status lt_solver_lens_pc_matrix(lt_solver s, long *nnz, double *c_used, int *row_ptr, int *cols, double *vals);

@@ -734,7 +734,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     for (int s = 0; s < S; ++s) ids[s] = s;
     LT_TRY(all_active.upload(ids.data(), S));
     // autotuned launch shapes: SpMV lane groups of L, L^T, Cc; stencil batching
-    LT_TRY(csr_autotune(lb->L, false));
+    LT_TRY(csr_autotune(lb->Lp, false));
     LT_TRY(csr_autotune(lb->LT, false));
     LT_TRY(csr_autotune(C, false));
     if (use_stencil && !(shared_w && nf->wstencil_tuned_batch == S)) {
@@ -751,7 +751,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
 }
 
 lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact, const int *af) {
-    LT_TRY(csr_spmv(lb->L, xin, img1.p, 1.0, 0.0, 0, af, npix, nullptr));
+    LT_TRY(csr_spmv(lb->Lp, xin, img1.p, 1.0, 0.0, 0, af, npix, nullptr));
     if (mode != LT_OP_BCB)
         LT_TRY(nf->normal_batched(img1.p, img2.p, wp, alist, nact, 1.0, false, use_stencil ? stp : nullptr));
     else
@@ -833,7 +833,7 @@ lt_status Solver::lens_setup() {
 lt_status Solver::lens_build_matrix() {
     int nb = 1;
     while ((1L << nb) < ntot) ++nb;  // ntot < 2^31, so 2 nb <= 62 key bits
-    const DevCSR &L = lb->L;
+    const DevCSR &L = lb->Lp;
     const long nl = L.nrows, nc = C.nrows, nr = nl + nc;
     DevBuf<int64_t> cnt, off;
     LT_TRY(cnt.alloc(nr));
@@ -1068,7 +1068,7 @@ lt_status Solver::prepare32() {
         return LT_ERR_UNSUPPORTED;
     }
     if (x32.p) return LT_OK;
-    LT_TRY(csr_autotune(lb->L, true));
+    LT_TRY(csr_autotune(lb->Lp, true));
     LT_TRY(csr_autotune(lb->LT, true));
     LT_TRY(csr_autotune(C, true));
     LT_TRY(x32.alloc(ntot));
@@ -1087,7 +1087,7 @@ lt_status Solver::apply32(const double *xin, double *yout, const int *alist, int
     LT_TRY(seg_dot(xin, xin, xx.p));
     hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, xin, x32.p);
     LT_CHECK_LAUNCH();
-    LT_TRY(csr_spmv32(lb->L, x32.p, i1_32.p, 1.f, 0.f, af, npix, nullptr));
+    LT_TRY(csr_spmv32(lb->Lp, x32.p, i1_32.p, 1.f, 0.f, af, npix, nullptr));
     LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, *stp));
     LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p));
     if (lam_c != 0.0) {

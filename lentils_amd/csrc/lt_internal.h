@@ -151,5 +151,7 @@ lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, flo
                      long rows_per_sample = 0, const int32_t *rowseg = nullptr);
 // B = A^T with column order inside each row ascending (deterministic)
 lt_status csr_transpose(const DevCSR &A, DevCSR &B);
+// B = A without its entries whose value is exactly zero (order kept)
+lt_status csr_drop_zeros(const DevCSR &A, DevCSR &B);
 
 }  // namespace lt

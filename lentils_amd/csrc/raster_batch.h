@@ -20,15 +20,22 @@ struct LensBatch {
     DevBuf<int64_t> srcoff;
     DevBuf<double> lattice;  // deflected stencil lattices, ns x (2nx+1)(2ny+1) x 2
     DevCSR L;                // block diagonal: rows ns*npix, cols sum(nsrc)
-    DevCSR LT;               // transpose, built on demand
+    DevCSR Lp;               // L without its exact-zero entries (the solver's operator), built on demand
+    DevCSR LT;               // transpose of Lp, built on demand
 
     // lattice_host != nullptr: rasterize given lattices instead of deflecting
     lt_status build(const image_space &im, const image_space *srcs, int ns, const generic_mass_model *lenses,
                     const int *lens_counts, const double *lattice_host);
     lt_status download(int s, int *row_inds, int *cols, double *vals) const;
+    // Lp and its transpose.  Rows without fragments hold one explicit zero
+    // (the stale column, raster.c:388-391) and a dropped first fragment can
+    // leave zero-valued entries; they add nothing to L x but make the stale
+    // columns very long rows of the transpose.  L itself stays as built.
     lt_status ensure_transpose() {
         if (LT.nrows == 0 && L.nrows > 0) {
-            LT_TRY(csr_transpose(L, LT));
+            LT_TRY(csr_drop_zeros(L, Lp));
+            LT_TRY(csr_prepare(Lp));
+            LT_TRY(csr_transpose(Lp, LT));
             LT_TRY(csr_prepare(LT));
         }
         return LT_OK;

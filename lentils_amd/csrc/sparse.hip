@@ -226,6 +226,32 @@ __global__ void k_row_of(long nrows, const int32_t *__restrict__ rp, int32_t *__
 }
 
 
+__global__ void k_nonzero_flags(long nnz, const double *__restrict__ va, int64_t *__restrict__ flag) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) flag[k] = va[k] != 0.0 ? 1 : 0;
+}
+
+// kept entries of row r = pos[rp[r+1]] - pos[rp[r]] (pos: exclusive scan of the flags)
+__global__ void k_row_kept(long n, const int32_t *__restrict__ rp, const int64_t *__restrict__ pos,
+                           int64_t *__restrict__ cnt) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) cnt[r] = pos[rp[r + 1]] - pos[rp[r]];
+}
+
+__global__ void k_i64_rowptr(long n, const int64_t *__restrict__ off, int32_t *__restrict__ rp) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= n) rp[r] = (int32_t)off[r];
+}
+
+__global__ void k_keep_entries(long nnz, const int32_t *__restrict__ ci, const double *__restrict__ va,
+                               const int64_t *__restrict__ pos, int32_t *__restrict__ oci, double *__restrict__ ova) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz && pos[k + 1] != pos[k]) {
+        oci[pos[k]] = ci[k];
+        ova[pos[k]] = va[k];
+    }
+}
+
 __global__ void k_col_keys(long nnz, const int32_t *__restrict__ ci, int kb, uint64_t *__restrict__ keys) {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nnz) keys[k] = ((uint64_t)(uint32_t)ci[k] << kb) | (uint64_t)k;
@@ -469,6 +495,41 @@ lt_status csr_autotune(DevCSR &A, bool fp32) {
 // Transpose: counting sort of the nonzeros by column, then a segmented sort
 // inside each column on the original nonzero position, which orders every
 // output row by ascending source row -- independent of atomic order.
+lt_status csr_drop_zeros(const DevCSR &A, DevCSR &B) {
+    const long nnz = A.nnz, n = A.nrows;
+    B = DevCSR();
+    B.nrows = n;
+    B.ncols = A.ncols;
+    LT_TRY(B.rp.alloc(n + 1));
+    DevBuf<int64_t> flag, pos, cnt, off;
+    LT_TRY(flag.alloc(nnz + 1));
+    LT_TRY(pos.alloc(nnz + 1));
+    LT_TRY(cnt.alloc(n + 1));
+    LT_TRY(off.alloc(n + 1));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_nonzero_flags, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream(), nnz, A.va.p, flag.p);
+        LT_CHECK_LAUNCH();
+    }
+    LT_TRY(exclusive_scan_i64(flag.p, pos.p, nnz));
+    int64_t kept = 0;
+    LT_TRY(pos.download(&kept, 1, nnz));
+    hipLaunchKernelGGL(k_row_kept, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, pos.p, cnt.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(cnt.p, off.p, n));
+    hipLaunchKernelGGL(k_i64_rowptr, dim3(blocks_for(n + 1, 256)), dim3(256), 0, stream(), n, off.p, B.rp.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(sync());
+    B.nnz = kept;
+    LT_TRY(B.ci.alloc(std::max<long>(kept, 1)));
+    LT_TRY(B.va.alloc(std::max<long>(kept, 1)));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_keep_entries, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream(), nnz, A.ci.p, A.va.p,
+                           pos.p, B.ci.p, B.va.p);
+        LT_CHECK_LAUNCH();
+    }
+    return sync();
+}
+
 lt_status csr_transpose(const DevCSR &A, DevCSR &B) {
     const long nnz = A.nnz, nc = A.ncols;
     B.nrows = nc;
