@@ -172,7 +172,9 @@ lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int
                            const double *x, double *y);
 /* Same with the device kernel chosen: 0 the default, 1..64 (powers of two)
  * the lane-group kernel with that many lanes per row, 128 the streaming
- * (LDS-staged) kernel.  Every choice sums each row in a fixed order. */
+ * (LDS-staged) kernel, -1..-64 the lane-group kernel over the list of
+ * nonempty rows (the solver's L x; rows without entries give 0).  Every
+ * choice sums each row in a fixed order. */
 lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                                   const double *x, double *y, int kernel);
 

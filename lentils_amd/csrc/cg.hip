@@ -726,6 +726,8 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     LT_TRY(tmp.alloc(ntot));
     LT_TRY(img1.alloc(npix * S));
     LT_TRY(img2.alloc(npix * S));
+    // L x writes only the nonempty rows of L: the others stay zero here
+    LT_TRY(img1.zero());
     for (auto *v : {&rz, &rz_new, &pAp, &rr, &bb, &alpha, &beta}) LT_TRY(v->alloc(S));
     LT_TRY(act.alloc(S));
     LT_TRY(iters.alloc(S));
@@ -738,7 +740,6 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     LT_TRY(csr_autotune(lb->LT, false));
     LT_TRY(csr_autotune(C, false));
     if (use_stencil && !(shared_w && nf->wstencil_tuned_batch == S)) {
-        LT_TRY(img1.zero());
         LT_TRY(stencil_autotune(*nf, *stp, img1.p, img2.p, all_active.p, S));
         if (shared_w) nf->wstencil_tuned_batch = S;
     }
@@ -751,7 +752,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
 }
 
 lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact, const int *af) {
-    LT_TRY(csr_spmv(lb->Lp, xin, img1.p, 1.0, 0.0, 0, af, npix, nullptr));
+    LT_TRY(csr_spmv_nz(lb->Lp, xin, img1.p, af, npix));
     if (mode != LT_OP_BCB)
         LT_TRY(nf->normal_batched(img1.p, img2.p, wp, alist, nact, 1.0, false, use_stencil ? stp : nullptr));
     else
@@ -819,6 +820,7 @@ lt_status Solver::lens_setup() {
         else
             LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, all_active.p, 1, 1.0, false));
         LT_TRY(img2.download(&c, 1, ctr));
+        LT_TRY(img1.zero());  // L x relies on zero rows outside L's nonempty rows
         LT_TRY(sync());
         if (!(c > 0.0)) c = 1.0;
     }
@@ -1077,6 +1079,7 @@ lt_status Solver::prepare32() {
     LT_TRY(c32.alloc(ntot));
     LT_TRY(xx.alloc(S));
     LT_TRY(i1_32.alloc(npix * S));
+    LT_TRY(i1_32.zero());  // rows of L without entries are never written
     LT_TRY(i2_32.alloc(npix * S));
     LT_TRY(Ax.alloc(ntot));
     return LT_OK;
@@ -1087,7 +1090,7 @@ lt_status Solver::apply32(const double *xin, double *yout, const int *alist, int
     LT_TRY(seg_dot(xin, xin, xx.p));
     hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, xin, x32.p);
     LT_CHECK_LAUNCH();
-    LT_TRY(csr_spmv32(lb->Lp, x32.p, i1_32.p, 1.f, 0.f, af, npix, nullptr));
+    LT_TRY(csr_spmv32_nz(lb->Lp, x32.p, i1_32.p, af, npix));
     LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, *stp));
     LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p));
     if (lam_c != 0.0) {

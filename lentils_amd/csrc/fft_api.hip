@@ -136,7 +136,8 @@ lt_status lt_csr_spmv_host(long nrows, long ncols, const int *rp, const int *ci,
 lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *rp, const int *ci, const double *va,
                                   const double *x, double *y, int kernel) {
     LT_TRY(ensure_init());
-    const bool pow2 = kernel > 0 && kernel <= 64 && (kernel & (kernel - 1)) == 0;
+    const int kl = kernel < 0 ? -kernel : kernel;
+    const bool pow2 = kl > 0 && kl <= 64 && (kl & (kl - 1)) == 0;
     if (nrows < 0 || ncols < 0 || !rp || !y || (ncols > 0 && !x) || !(kernel == 0 || pow2 || kernel == kLanesStream)) {
         set_error("lt_csr_spmv_host: bad arguments");
         return LT_ERR_ARG;
@@ -152,7 +153,14 @@ lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *rp, const i
     DevBuf<double> dx, dy;
     LT_TRY(dx.upload(x, ncols));
     LT_TRY(dy.alloc(nrows > 0 ? nrows : 1));
-    LT_TRY(csr_spmv(A, dx.p, dy.p, 1.0, 0.0, kernel));
+    if (kernel < 0) {  // the product over the nonempty rows, the rest of y zero
+        LT_TRY(csr_nonempty_rows(A));
+        A.lanes_nz = kl;
+        LT_TRY(dy.zero());
+        LT_TRY(csr_spmv_nz(A, dx.p, dy.p));
+    } else {
+        LT_TRY(csr_spmv(A, dx.p, dy.p, 1.0, 0.0, kernel));
+    }
     LT_TRY(dy.download(y, nrows));
     return sync();
 }

@@ -131,6 +131,11 @@ struct DevCSR {
     DevBuf<double> chunk_sum;
     DevBuf<float> va32, chunk_sum32;  // FP32 values (csr_make_f32)
     int lanes32 = 0;                  // lane-group width of the FP32 product (0: as lanes)
+    // rows holding at least one entry (csr_nonempty_rows), for products that
+    // write those rows only
+    long nnzr = -1;
+    DevBuf<int32_t> nzr;
+    int lanes_nz = 0, lanes_nz32 = 0;
 };
 
 constexpr int kLanesStream = 128;  // pseudo lane count: the streaming (LDS-staged) SpMV kernel
@@ -153,5 +158,13 @@ lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, flo
 lt_status csr_transpose(const DevCSR &A, DevCSR &B);
 // B = A without its entries whose value is exactly zero (order kept)
 lt_status csr_drop_zeros(const DevCSR &A, DevCSR &B);
+// list of the rows with at least one entry (A.nzr, A.nnzr), after csr_prepare
+lt_status csr_nonempty_rows(DevCSR &A);
+// y = A x on the rows listed by csr_nonempty_rows only: the other rows of y
+// are not written (the caller keeps them zero).  Same filter as csr_spmv.
+lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act = nullptr,
+                      long rows_per_sample = 0);
+lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act = nullptr,
+                        long rows_per_sample = 0);
 
 }  // namespace lt

@@ -35,6 +35,7 @@ struct LensBatch {
         if (LT.nrows == 0 && L.nrows > 0) {
             LT_TRY(csr_drop_zeros(L, Lp));
             LT_TRY(csr_prepare(Lp));
+            LT_TRY(csr_nonempty_rows(Lp));
             LT_TRY(csr_transpose(Lp, LT));
             LT_TRY(csr_prepare(LT));
         }

@@ -70,6 +70,39 @@ __global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32
     if (lane == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
 }
 
+// The same product over a list of rows (the nonempty rows of a matrix whose
+// other rows the caller keeps at zero): lane group g takes row nzr[g].
+template <int LANES, class T>
+__global__ void __launch_bounds__(256) k_spmv_rows(long i0, long n, const int32_t *__restrict__ nzr,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const T *__restrict__ va, const T *__restrict__ x,
+                                                    T *__restrict__ y, int skip_long, RowFilter F) {
+    const long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long i = i0 + gid / LANES;
+    const int lane = (int)(gid % LANES);
+    if (i >= n) return;
+    const long row = nzr[i];
+    if (F.skip(row)) return;
+    T s = (T)0;
+    const int b = rp[row], e = rp[row + 1];
+    if (skip_long && e - b > kLongRow) return;
+    for (int k = b + lane; k < e; k += LANES) s += va[k] * x[ci[k]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
+    if (lane == 0) y[row] = s;
+}
+
+__global__ void k_row_nonempty(long n, const int32_t *__restrict__ rp, int64_t *__restrict__ flag) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) flag[r] = rp[r + 1] > rp[r] ? 1 : 0;
+}
+
+__global__ void k_row_list(long n, const int32_t *__restrict__ rp, const int64_t *__restrict__ pos,
+                           int32_t *__restrict__ nzr) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n && rp[r + 1] > rp[r]) nzr[pos[r]] = (int32_t)r;
+}
+
 // Streaming SpMV (CSR-stream): workgroup b owns the nonzeros
 // [b*kStreamNnz, (b+1)*kStreamNnz) and the rows that start among them
 // (rblk[b] .. rblk[b+1]).  It first forms every product va[k] * x[ci[k]] of
@@ -344,20 +377,50 @@ lt_status csr_prepare(DevCSR &A) {
 
 template <class T>
 static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, T *y, T alpha, T beta_y, int lanes,
-                        const int *act, long rows_per_sample, const int32_t *rowseg) {
+                        const int *act, long rows_per_sample, const int32_t *rowseg, bool nz_only = false) {
     if (A.nrows == 0) return LT_OK;
     const RowFilter F{act, rows_per_sample > 0 ? rows_per_sample : 1, rowseg};
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
     const int skip = (A.prepared && A.nlong > 0) ? 1 : 0;
-    if (lanes == kLanesStream && !(A.prepared && A.nblk > 0)) lanes = csr_pick_lanes(A);
-    if (lanes == kLanesStream) {
+    if (nz_only) {
+        if (A.nnzr < 0 || !A.prepared || alpha != (T)1 || beta_y != (T)0 || lanes > 64) {
+            set_error("spmv: nonempty-row product needs csr_nonempty_rows, alpha 1, beta 0, lanes <= 64");
+            return LT_ERR_ARG;
+        }
+        const long per = (1L << 30) / lanes;
+        for (long i0 = 0; i0 < A.nnzr; i0 += per) {
+            const long ni = std::min(per, A.nnzr - i0);
+            const dim3 g(blocks_for(ni * lanes, 256)), b(256);
+#define LT_SPMV_CASE(L)                                                                                            \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv_rows<L, T>), g, b, 0, stream(), i0, i0 + ni, A.nzr.p, A.rp.p, A.ci.p, va, x, y,  \
+                           skip, F);                                                                               \
+        break;
+            switch (lanes) {
+                LT_SPMV_CASE(1)
+                LT_SPMV_CASE(2)
+                LT_SPMV_CASE(4)
+                LT_SPMV_CASE(8)
+                LT_SPMV_CASE(16)
+                LT_SPMV_CASE(32)
+                LT_SPMV_CASE(64)
+                default:
+                    set_error("spmv: lanes per row must be a power of two <= 64");
+                    return LT_ERR_ARG;
+            }
+#undef LT_SPMV_CASE
+            LT_CHECK_LAUNCH();
+        }
+    }
+    if (!nz_only && lanes == kLanesStream && !(A.prepared && A.nblk > 0)) lanes = csr_pick_lanes(A);
+    if (!nz_only && lanes == kLanesStream) {
         hipLaunchKernelGGL((k_spmv_stream<T>), dim3((unsigned)A.nblk), dim3(256), 0, stream(), A.nnz, A.rblk.p,
                            A.rp.p, A.ci.p, va, x, y, alpha, beta_y, skip, F);
         LT_CHECK_LAUNCH();
     }
     // a dispatch covers at most 2^30 work-items (the grid size is 32-bit)
-    const long rows_per_launch = (1L << 30) / std::min(lanes, 64);
-    for (long r0 = 0; lanes != kLanesStream && r0 < A.nrows; r0 += rows_per_launch) {
+    const long rows_per_launch = (1L << 30) / std::max(1, std::min(lanes, 64));
+    for (long r0 = 0; !nz_only && lanes != kLanesStream && r0 < A.nrows; r0 += rows_per_launch) {
         const long nr = std::min(rows_per_launch, A.nrows - r0);
         const dim3 g(blocks_for(nr * lanes, 256)), b(256);
 #define LT_SPMV_CASE(L)                                                                                            \
@@ -396,6 +459,43 @@ lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, do
     return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, alpha, beta_y, lanes, act, rows_per_sample, rowseg);
 }
 
+lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act, long rows_per_sample) {
+    return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, 1.0, 0.0, A.lanes_nz > 0 ? A.lanes_nz : 4, act,
+                          rows_per_sample, nullptr, true);
+}
+
+lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act, long rows_per_sample) {
+    if (A.nnz > 0 && !A.va32.p) {
+        set_error("spmv: FP32 values not prepared");
+        return LT_ERR_ARG;
+    }
+    const int lanes = A.lanes_nz32 > 0 ? A.lanes_nz32 : (A.lanes_nz > 0 ? A.lanes_nz : 4);
+    return spmv_t<float>(A, A.va32.p, A.chunk_sum32.p, x, y, 1.f, 0.f, lanes, act, rows_per_sample, nullptr, true);
+}
+
+lt_status csr_nonempty_rows(DevCSR &A) {
+    if (A.nnzr >= 0) return LT_OK;
+    const long n = A.nrows;
+    DevBuf<int64_t> flag, pos;
+    LT_TRY(flag.alloc(n + 1));
+    LT_TRY(pos.alloc(n + 1));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_row_nonempty, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, flag.p);
+        LT_CHECK_LAUNCH();
+    }
+    LT_TRY(exclusive_scan_i64(flag.p, pos.p, n));
+    int64_t cnt = 0;
+    LT_TRY(pos.download(&cnt, 1, n));
+    LT_TRY(A.nzr.alloc(std::max<long>(cnt, 1)));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_row_list, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, pos.p, A.nzr.p);
+        LT_CHECK_LAUNCH();
+    }
+    LT_TRY(sync());
+    A.nnzr = cnt;
+    return LT_OK;
+}
+
 lt_status csr_make_f32(DevCSR &A) {
     if (A.va32.p || A.nnz == 0) return LT_OK;
     LT_TRY(A.va32.alloc(A.nnz));
@@ -419,7 +519,7 @@ lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, flo
 // row) on this matrix and keeps the fastest (A.lanes, or A.lanes32 for the
 // FP32 values).  A few milliseconds per matrix.
 template <class T>
-static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, int &best_out) {
+static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, bool nz, int &best_out) {
     DevBuf<T> x, y;
     LT_TRY(x.alloc(std::max<long>(A.ncols, 1)));
     LT_TRY(y.alloc(std::max<long>(A.nrows, 1)));
@@ -430,10 +530,11 @@ static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, int &best_out)
     float best = 1e30f;
     int best_l = A.lanes;
     for (int L = 1; L <= kLanesStream; L <<= 1) {
-        if (L == kLanesStream && A.nblk == 0) break;
-        LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr));  // warm-up
+        if (L == kLanesStream && (A.nblk == 0 || nz)) break;
+        LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr, nz));  // warm-up
         LT_HIP(hipEventRecord(e0, stream()));
-        for (int r = 0; r < 3; ++r) LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr));
+        for (int r = 0; r < 3; ++r)
+            LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr, nz));
         LT_HIP(hipEventRecord(e1, stream()));
         LT_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -455,10 +556,11 @@ static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, int &best_out)
 // matrices of a new batch of the same geometry does not re-run the sweep.
 namespace {
 struct TuneKey {
-    int dev, fp32, lenq, has_long;
+    int dev, fp32, lenq, has_long, nz;
     long nrows;
     bool operator<(const TuneKey &o) const {
-        return std::tie(dev, fp32, lenq, has_long, nrows) < std::tie(o.dev, o.fp32, o.lenq, o.has_long, o.nrows);
+        return std::tie(dev, fp32, lenq, has_long, nz, nrows) <
+               std::tie(o.dev, o.fp32, o.lenq, o.has_long, o.nz, o.nrows);
     }
 };
 std::mutex g_tune_mu;
@@ -471,24 +573,30 @@ lt_status csr_autotune(DevCSR &A, bool fp32) {
     if (fp32) LT_TRY(csr_make_f32(A));
     int dev = 0;
     LT_HIP(hipGetDevice(&dev));
-    const double mean = (double)A.nnz / (double)A.nrows;
-    const TuneKey key{dev, fp32 ? 1 : 0, (int)std::lround(4.0 * std::log2(mean)), A.nlong > 0 ? 1 : 0, A.nrows};
-    {
-        std::lock_guard<std::mutex> g(g_tune_mu);
-        auto it = g_tune.find(key);
-        if (it != g_tune.end()) {
-            (fp32 ? A.lanes32 : A.lanes) = it->second;
-            return LT_OK;
+    // the full product, then (when the nonempty-row list exists) the
+    // product over the listed rows, each with its own memo entry
+    for (int nz = 0; nz <= (A.nnzr >= 0 ? 1 : 0); ++nz) {
+        const double mean = (double)A.nnz / (double)(nz ? std::max(1L, A.nnzr) : A.nrows);
+        const TuneKey key{dev, fp32 ? 1 : 0, (int)std::lround(4.0 * std::log2(mean)), A.nlong > 0 ? 1 : 0, nz,
+                          nz ? A.nnzr : A.nrows};
+        int &slot = nz ? (fp32 ? A.lanes_nz32 : A.lanes_nz) : (fp32 ? A.lanes32 : A.lanes);
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) {
+                slot = it->second;
+                continue;
+            }
         }
+        int best = 0;
+        if (fp32)
+            LT_TRY(tune_lanes<float>(A, A.va32.p, A.chunk_sum32.p, nz != 0, best));
+        else
+            LT_TRY(tune_lanes<double>(A, A.va.p, A.chunk_sum.p, nz != 0, best));
+        slot = best;
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune[key] = best;
     }
-    int best = 0;
-    if (fp32)
-        LT_TRY(tune_lanes<float>(A, A.va32.p, A.chunk_sum32.p, best));
-    else
-        LT_TRY(tune_lanes<double>(A, A.va.p, A.chunk_sum.p, best));
-    (fp32 ? A.lanes32 : A.lanes) = best;
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = best;
     return LT_OK;
 }
 
