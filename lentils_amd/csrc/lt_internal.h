@@ -167,4 +167,14 @@ lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *ac
 lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act = nullptr,
                         long rows_per_sample = 0);
 
+// Workgroup order for the 8 XCDs (each with its own L2): the dispatcher
+// hands workgroup b to XCD b % 8, so renumbering the i-th workgroup of XCD x
+// to start_x + i gives every XCD one contiguous range of the logical
+// workgroups, processed in increasing order.  A bijection for any grid size.
+__device__ inline long xcd_block(long b, long nb) {
+    constexpr long X = 8;
+    const long x = b % X, i = b / X, q = nb / X, r = nb % X;
+    return x * q + (x < r ? x : r) + i;
+}
+
 }  // namespace lt

@@ -926,24 +926,29 @@ __global__ void k_compact(long n, const int64_t *__restrict__ off, int32_t *__re
 }
 
 // R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) for NB samples per
-// block; S windows of the tile staged in LDS with fold / wrap / sign
+// block; S windows of the tile staged in LDS with fold / wrap / sign.  The
+// workgroups of one tile (one per group of NB samples) run back to back on
+// one XCD, so the tile's coefficients come from that XCD's L2 after the
+// first group.
 template <class T, int NB, class TC, bool BAND = false>
 __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R, int ST, int ntv_st, const int32_t *__restrict__ tiles,
-                                                       const int32_t *__restrict__ rowmap, long nrow,
+                                                       int ngroup, const int32_t *__restrict__ rowmap, long nrow,
                                                        const TC *__restrict__ coef, const Cplx<T> *__restrict__ spec,
                                                        long sstride, Cplx<T> *__restrict__ rbuf,
                                                        const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
     const int WS = ST + 2 * R;
     Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem);
-    const int tile = tiles[blockIdx.x];
+    const long lb = xcd_block(blockIdx.x, gridDim.x);
+    const int grp = (int)(lb % ngroup);
+    const int tile = tiles[lb / ngroup];
     const int tu = tile / ntv_st, tv = tile - tu * ntv_st;
     const int u0 = tu * ST - R, v0 = tv * ST - R;  // centred coords of window (0, 0)
     int smp[NB];
     int ns = 0;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const int q = blockIdx.y * NB + k;
+        const int q = grp * NB + k;
         smp[k] = q < nactive ? (active ? active[q] : q) : -1;
         if (smp[k] >= 0) ns = k + 1;
     }
@@ -2405,14 +2410,15 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
 #define LT_STENCIL_NB(NBV)                                                                                        \
     {                                                                                                             \
         const size_t sh = (size_t)NBV * WS * WS * sizeof(Cplx<T>);                                                \
-        const dim3 grid((unsigned)St.ntile, (nactive + NBV - 1) / NBV);                                           \
+        const int ngroup = (nactive + NBV - 1) / NBV;                                                             \
+        const dim3 grid((unsigned)(St.ntile * ngroup));                                                           \
         if (band)                                                                                                 \
             hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, true>), grid, dim3(St.ST * St.ST), sh, stream(), G,   \
-                               St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p, St.nrow, coef,                       \
+                               St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
                                (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
         else                                                                                                      \
             hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, false>), grid, dim3(St.ST * St.ST), sh, stream(), G,  \
-                               St.R, St.ST, ntv_st, St.tiles.p, St.rowmap.p, St.nrow, coef,                       \
+                               St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
                                (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
     }
         if (St.nb_sel == 2)

@@ -40,17 +40,6 @@ struct RowFilter {
     }
 };
 
-// Workgroup order for the 8 XCDs (each with its own L2): the dispatcher
-// hands workgroup b to XCD b % 8, so renumbering the i-th workgroup of XCD x
-// to start_x + i gives every XCD one contiguous range of the rows (and of
-// the vector entries they gather), processed in increasing order.  A
-// bijection for any grid size.
-__device__ inline long xcd_block(long b, long nb) {
-    constexpr long X = 8;
-    const long x = b % X, i = b / X, q = nb / X, r = nb % X;
-    return x * q + (x < r ? x : r) + i;
-}
-
 // T: value and vector type (double, or float for the precision form)
 template <int LANES, class T>
 __global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32_t *__restrict__ rp,
