@@ -172,9 +172,10 @@ lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int
                            const double *x, double *y);
 /* Same with the device kernel chosen: 0 the default, 1..64 (powers of two)
  * the lane-group kernel with that many lanes per row, 128 the streaming
- * (LDS-staged) kernel, -1..-64 the lane-group kernel over the list of
- * nonempty rows (the solver's L x; rows without entries give 0).  Every
- * choice sums each row in a fixed order. */
+ * (LDS-staged) kernel, -1..-64 the solver's L x: rows of 1..4 entries from
+ * a padded 4-wide table, rows of 5..64 by that many lanes per row, longer
+ * rows in 2048-entry chunks, rows without entries 0.  Every choice sums
+ * each row in a fixed order. */
 lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                                   const double *x, double *y, int kernel);
 
@@ -225,6 +226,10 @@ lt_status lt_nufft_destroy(lt_nufft h);
  * kernel_support <= 4 builds the normal stencil D^T diag(w) D from them
  * once.  Solvers created with weights == NULL share both. */
 lt_status lt_nufft_set_weights(lt_nufft h, const double *weights);
+/* No reference counterpart: forces the normal stencil's samples per
+ * workgroup (1, 2, 4, 8) for this plan instead of the autotuned choice (0).
+ * Every choice gives the same bits. */
+lt_status lt_nufft_set_stencil_shape(lt_nufft h, int samples_per_group);
 
 /* Device memory for resident inputs (plain hipMalloc'd bytes). */
 lt_status lt_device_alloc(long nbytes, void **dptr);

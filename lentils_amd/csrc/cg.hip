@@ -1546,6 +1546,16 @@ lt_status lt_nufft_set_weights(lt_nufft nfh, const double *weights) {
     return nf->set_weights(weights);
 }
 
+lt_status lt_nufft_set_stencil_shape(lt_nufft nfh, int samples_per_group) {
+    NufftPlan *nf = nufft_of(nfh);
+    if (!nf || samples_per_group < 0 || samples_per_group > 8 || (samples_per_group & (samples_per_group - 1))) {
+        set_error("lt_nufft_set_stencil_shape: samples per group 0, 1, 2, 4 or 8");
+        return LT_ERR_ARG;
+    }
+    nf->force_nb = samples_per_group;
+    return LT_OK;
+}
+
 lt_status lt_device_alloc(long nbytes, void **dptr) {
     if (!dptr || nbytes < 0) return LT_ERR_ARG;
     *dptr = nullptr;

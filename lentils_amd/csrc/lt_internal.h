@@ -120,8 +120,9 @@ struct DevCSR {
     DevBuf<int32_t> rp;  // nrows + 1 (global offsets)
     DevBuf<int32_t> ci;  // global column indices
     DevBuf<double> va;
-    // long-row plan (csr_prepare): rows above kLongRow nonzeros are summed
+    // long-row plan (csr_prepare): rows above long_row nonzeros are summed
     // in fixed-size chunks by whole workgroups, then combined in order
+    int long_row = 2048;
     bool prepared = false;
     int lanes = 1;  // lane-group width of k_spmv, or kLanesStream for the streaming kernel
     long nblk = 0;        // streaming-kernel workgroups (ceil(nnz / its nonzero range))
@@ -131,10 +132,14 @@ struct DevCSR {
     DevBuf<double> chunk_sum;
     DevBuf<float> va32, chunk_sum32;  // FP32 values (csr_make_f32)
     int lanes32 = 0;                  // lane-group width of the FP32 product (0: as lanes)
-    // rows holding at least one entry (csr_nonempty_rows), for products that
-    // write those rows only
-    long nnzr = -1;
-    DevBuf<int32_t> nzr;
+    // nonempty rows (csr_nonempty_rows), for products that write those rows
+    // only: rows of 1..4 entries as a padded 4-wide table (ell_*: row, four
+    // columns, four values; padding value 0 at the row's first column), the
+    // longer ones listed in nzr
+    long nnzr = -1, nell = 0;
+    DevBuf<int32_t> nzr, ell_row, ell_ci;
+    DevBuf<double> ell_va;
+    DevBuf<float> ell_va32;
     int lanes_nz = 0, lanes_nz32 = 0;
 };
 
@@ -158,9 +163,10 @@ lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, flo
 lt_status csr_transpose(const DevCSR &A, DevCSR &B);
 // B = A without its entries whose value is exactly zero (order kept)
 lt_status csr_drop_zeros(const DevCSR &A, DevCSR &B);
-// list of the rows with at least one entry (A.nzr, A.nnzr), after csr_prepare
+// the nonempty rows of A, after csr_prepare: rows of 1..4 entries as the
+// 4-wide table (A.ell_*, A.nell), longer ones as a list (A.nzr, A.nnzr)
 lt_status csr_nonempty_rows(DevCSR &A);
-// y = A x on the rows listed by csr_nonempty_rows only: the other rows of y
+// y = A x on the nonempty rows only (csr_nonempty_rows): the other rows of y
 // are not written (the caller keeps them zero).  Same filter as csr_spmv.
 lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act = nullptr,
                       long rows_per_sample = 0);

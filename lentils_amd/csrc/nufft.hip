@@ -929,14 +929,16 @@ __global__ void k_compact(long n, const int64_t *__restrict__ off, int32_t *__re
 // block; S windows of the tile staged in LDS with fold / wrap / sign.  The
 // workgroups of one tile (one per group of NB samples) run back to back on
 // one XCD, so the tile's coefficients come from that XCD's L2 after the
-// first group.
-template <class T, int NB, class TC, bool BAND = false>
-__global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R, int ST, int ntv_st, const int32_t *__restrict__ tiles,
+// first group.  RT > 0: the radius R compiled in (the offset loop unrolled,
+// its coefficient loads issued together); RT = 0: R at run time.
+template <class T, int NB, class TC, bool BAND = false, int RT = 0>
+__global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, int ntv_st, const int32_t *__restrict__ tiles,
                                                        int ngroup, const int32_t *__restrict__ rowmap, long nrow,
                                                        const TC *__restrict__ coef, const Cplx<T> *__restrict__ spec,
                                                        long sstride, Cplx<T> *__restrict__ rbuf,
                                                        const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
+    const int R = RT > 0 ? RT : R_;
     const int WS = ST + 2 * R;
     Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem);
     const long lb = xcd_block(blockIdx.x, gridDim.x);
@@ -971,6 +973,7 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R, int ST, int
     for (int k = 0; k < NB; ++k) ar[k] = ai[k] = (T)0;
     const int NR = 2 * R + 1;
     for (int ou = 0; ou < NR; ++ou) {
+#pragma unroll
         for (int ov = 0; ov < NR; ++ov) {
             const T cf = (T)coef[(long)(ou * NR + ov) * nrow + row];
             const int wi = (ly + ou) * WS + (lx + ov);
@@ -2197,7 +2200,7 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     return LT_OK;
 }
 
-// Autotuner of the stencil's samples-per-workgroup (2, 4, 8): times the
+// Autotuner of the stencil's samples per workgroup (1, 2, 4, 8): times the
 // whole band normal operator at the full batch for each and keeps the
 // fastest.  imgs / out: nb images of scratch.
 lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, double *out, const int *all, int nb) {
@@ -2207,7 +2210,7 @@ lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, 
     LT_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int best_nb = 4;
-    for (int cand : {2, 4, 8}) {
+    for (int cand : {1, 2, 4, 8}) {
         St.nb_sel = cand;
         LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));  // warm-up
         LT_HIP(hipEventRecord(e0, stream()));
@@ -2405,14 +2408,19 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
     if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
     const int WS = St.ST + 2 * St.R;
     const int ntv_st = (P.nv + St.ST - 1) / St.ST;
+    const int nb_sel = P.force_nb > 0 ? P.force_nb : St.nb_sel;
     if (St.ntile > 0 && nactive > 0) {
-        // NB samples share one staged coefficient read (autotuned: 2, 4 or 8)
+        // NB samples share one staged coefficient read (autotuned: 1, 2, 4 or 8)
 #define LT_STENCIL_NB(NBV)                                                                                        \
     {                                                                                                             \
         const size_t sh = (size_t)NBV * WS * WS * sizeof(Cplx<T>);                                                \
         const int ngroup = (nactive + NBV - 1) / NBV;                                                             \
         const dim3 grid((unsigned)(St.ntile * ngroup));                                                           \
-        if (band)                                                                                                 \
+        if (band && St.R == 6)                                                                                    \
+            hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, true, 6>), grid, dim3(St.ST * St.ST), sh, stream(), G, \
+                               St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
+                               (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
+        else if (band)                                                                                            \
             hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, true>), grid, dim3(St.ST * St.ST), sh, stream(), G,   \
                                St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
                                (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
@@ -2421,9 +2429,11 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
                                St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
                                (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
     }
-        if (St.nb_sel == 2)
+        if (nb_sel == 1)
+            LT_STENCIL_NB(1)
+        else if (nb_sel == 2)
             LT_STENCIL_NB(2)
-        else if (St.nb_sel == 8)
+        else if (nb_sel == 8)
             LT_STENCIL_NB(8)
         else
             LT_STENCIL_NB(4)

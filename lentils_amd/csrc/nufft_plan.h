@@ -144,6 +144,7 @@ struct NufftPlan {
     DevBuf<double> twd64;
     DevBuf<float> twd32;
     bool fused_rows = true;  // use them where nyp allows
+    int force_nb = 0;  // stencil samples per workgroup when > 0 (else the autotuned one)
     lt_status band_fft(int n, int precision, int nbv, BandFFT *&out);
     DevBuf<uint8_t> realB, specB, specB2, partialB;
     lt_status batch_fft(int n, BatchFFT *&out, int precision = -1);

@@ -34,6 +34,7 @@ struct LensBatch {
     lt_status ensure_transpose() {
         if (LT.nrows == 0 && L.nrows > 0) {
             LT_TRY(csr_drop_zeros(L, Lp));
+            Lp.long_row = 64;  // the few lens-centre rows beyond this: one workgroup per 2048-entry chunk
             LT_TRY(csr_prepare(Lp));
             LT_TRY(csr_nonempty_rows(Lp));
             LT_TRY(csr_transpose(Lp, LT));

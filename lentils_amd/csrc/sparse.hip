@@ -18,12 +18,12 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 namespace lt {
 
 namespace {
 
-constexpr int kLongRow = 2048;  // rows longer than this go to the chunked path
 constexpr int kChunk = 2048;
 
 // Rows of a block-diagonal batch belong to samples: sample = row / rps, or
@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32
     if (row >= nrows || F.skip(row)) return;  // whole lane groups leave together
     T s = (T)0;
     const int b = rp[row], e = rp[row + 1];
-    if (skip_long && e - b > kLongRow) return;  // summed by k_spmv_chunks / k_spmv_long
+    if (skip_long > 0 && e - b > skip_long) return;  // summed by k_spmv_chunks / k_spmv_long
     for (int k = b + lane; k < e; k += LANES) s += va[k] * x[ci[k]];
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
@@ -74,22 +74,59 @@ __global__ void __launch_bounds__(256) k_spmv_rows(long i0, long n, const int32_
     if (F.skip(row)) return;
     T s = (T)0;
     const int b = rp[row], e = rp[row + 1];
-    if (skip_long && e - b > kLongRow) return;
+    if (skip_long > 0 && e - b > skip_long) return;
     for (int k = b + lane; k < e; k += LANES) s += va[k] * x[ci[k]];
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
     if (lane == 0) y[row] = s;
 }
 
-__global__ void k_row_nonempty(long n, const int32_t *__restrict__ rp, int64_t *__restrict__ flag) {
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) flag[r] = rp[r + 1] > rp[r] ? 1 : 0;
+// Rows of 1..4 entries, one thread per row of the padded table: the row
+// index, the four columns and the four values are independent aligned
+// loads, then four gathers.  Terms are added in the row's order.
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_ell4(long n, const int32_t *__restrict__ erow,
+                                                    const int4 *__restrict__ eci, const T *__restrict__ eva,
+                                                    const T *__restrict__ x, T *__restrict__ y, RowFilter F) {
+    const long i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long row = erow[i];
+    const int4 c = eci[i];
+    const T v0 = eva[4 * i], v1 = eva[4 * i + 1], v2 = eva[4 * i + 2], v3 = eva[4 * i + 3];
+    if (F.skip(row)) return;
+    T s = v0 * x[c.x];
+    s += v1 * x[c.y];
+    s += v2 * x[c.z];
+    s += v3 * x[c.w];
+    y[row] = s;
 }
 
-__global__ void k_row_list(long n, const int32_t *__restrict__ rp, const int64_t *__restrict__ pos,
-                           int32_t *__restrict__ nzr) {
+__global__ void k_row_class(long n, const int32_t *__restrict__ rp, int64_t *__restrict__ fe,
+                            int64_t *__restrict__ fo) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n && rp[r + 1] > rp[r]) nzr[pos[r]] = (int32_t)r;
+    if (r >= n) return;
+    const int len = rp[r + 1] - rp[r];
+    fe[r] = (len >= 1 && len <= 4) ? 1 : 0;
+    fo[r] = len > 4 ? 1 : 0;
+}
+
+__global__ void k_row_split(long n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                            const double *__restrict__ va, const int64_t *__restrict__ pe,
+                            const int64_t *__restrict__ po, int32_t *__restrict__ erow, int32_t *__restrict__ eci,
+                            double *__restrict__ eva, int32_t *__restrict__ nzr) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int b = rp[r], e = rp[r + 1];
+    if (e - b > 4) {
+        nzr[po[r]] = (int32_t)r;
+    } else if (e > b) {
+        const long i = pe[r];
+        erow[i] = (int32_t)r;
+        for (int j = 0; j < 4; ++j) {
+            eci[4 * i + j] = b + j < e ? ci[b + j] : ci[b];
+            eva[4 * i + j] = b + j < e ? va[b + j] : 0.0;
+        }
+    }
 }
 
 // Streaming SpMV (CSR-stream): workgroup b owns the nonzeros
@@ -146,7 +183,7 @@ __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__
     for (long r = r0 + threadIdx.x; r < r1; r += 256) {
         if (F.skip(r)) continue;
         const int rb = rp_lds ? rps[r - r0] : rp[r], re = rp_lds ? rps[r - r0 + 1] : rp[r + 1];
-        if (skip_long && re - rb > kLongRow) continue;
+        if (skip_long > 0 && re - rb > skip_long) continue;
         const int ein = (int)min((long)re, k1);
         T s = (T)0;
         for (int k = rb; k < ein; ++k) s += prod[k - k0];
@@ -213,12 +250,12 @@ __global__ void k_d2f_sp(long n, const double *__restrict__ a, float *__restrict
     if (i < n) b[i] = (float)a[i];
 }
 
-__global__ void k_long_rows_flag(long n, const int32_t *__restrict__ rp, int64_t *__restrict__ flag,
+__global__ void k_long_rows_flag(long n, const int32_t *__restrict__ rp, int long_row, int64_t *__restrict__ flag,
                                  int64_t *__restrict__ nch) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int len = rp[r + 1] - rp[r];
-    const bool lg = len > kLongRow;
+    const bool lg = len > long_row;
     flag[r] = lg ? 1 : 0;
     nch[r] = lg ? (len + kChunk - 1) / kChunk : 0;
 }
@@ -325,7 +362,8 @@ lt_status csr_prepare(DevCSR &A) {
         LT_TRY(pos.alloc(n + 1));
         LT_TRY(nch.alloc(n + 1));
         LT_TRY(choff.alloc(n + 1));
-        hipLaunchKernelGGL(k_long_rows_flag, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, flag.p,
+        hipLaunchKernelGGL(k_long_rows_flag, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, A.long_row,
+                           flag.p,
                            nch.p);
         LT_CHECK_LAUNCH();
         LT_TRY(exclusive_scan_i64(flag.p, pos.p, n));
@@ -370,11 +408,17 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
     if (A.nrows == 0) return LT_OK;
     const RowFilter F{act, rows_per_sample > 0 ? rows_per_sample : 1, rowseg};
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
-    const int skip = (A.prepared && A.nlong > 0) ? 1 : 0;
+    const int skip = (A.prepared && A.nlong > 0) ? A.long_row : 0;  // row length above which the chunked path sums
     if (nz_only) {
         if (A.nnzr < 0 || !A.prepared || alpha != (T)1 || beta_y != (T)0 || lanes > 64) {
             set_error("spmv: nonempty-row product needs csr_nonempty_rows, alpha 1, beta 0, lanes <= 64");
             return LT_ERR_ARG;
+        }
+        if (A.nell > 0) {
+            const T *eva = std::is_same<T, float>::value ? (const T *)A.ell_va32.p : (const T *)A.ell_va.p;
+            hipLaunchKernelGGL((k_spmv_ell4<T>), dim3(blocks_for(A.nell, 256)), dim3(256), 0, stream(), A.nell,
+                               A.ell_row.p, (const int4 *)A.ell_ci.p, eva, x, y, F);
+            LT_CHECK_LAUNCH();
         }
         const long per = (1L << 30) / lanes;
         for (long i0 = 0; i0 < A.nnzr; i0 += per) {
@@ -454,7 +498,7 @@ lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *ac
 }
 
 lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act, long rows_per_sample) {
-    if (A.nnz > 0 && !A.va32.p) {
+    if ((A.nnz > 0 && !A.va32.p) || (A.nell > 0 && !A.ell_va32.p)) {
         set_error("spmv: FP32 values not prepared");
         return LT_ERR_ARG;
     }
@@ -465,27 +509,43 @@ lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *ac
 lt_status csr_nonempty_rows(DevCSR &A) {
     if (A.nnzr >= 0) return LT_OK;
     const long n = A.nrows;
-    DevBuf<int64_t> flag, pos;
-    LT_TRY(flag.alloc(n + 1));
-    LT_TRY(pos.alloc(n + 1));
+    DevBuf<int64_t> fe, fo, pe, po;
+    LT_TRY(fe.alloc(n + 1));
+    LT_TRY(fo.alloc(n + 1));
+    LT_TRY(pe.alloc(n + 1));
+    LT_TRY(po.alloc(n + 1));
     if (n > 0) {
-        hipLaunchKernelGGL(k_row_nonempty, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, flag.p);
+        hipLaunchKernelGGL(k_row_class, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, fe.p, fo.p);
         LT_CHECK_LAUNCH();
     }
-    LT_TRY(exclusive_scan_i64(flag.p, pos.p, n));
-    int64_t cnt = 0;
-    LT_TRY(pos.download(&cnt, 1, n));
-    LT_TRY(A.nzr.alloc(std::max<long>(cnt, 1)));
+    LT_TRY(exclusive_scan_i64(fe.p, pe.p, n));
+    LT_TRY(exclusive_scan_i64(fo.p, po.p, n));
+    int64_t ne = 0, no = 0;
+    LT_TRY(pe.download(&ne, 1, n));
+    LT_TRY(po.download(&no, 1, n));
+    LT_TRY(A.nzr.alloc(std::max<long>(no, 1)));
+    LT_TRY(A.ell_row.alloc(std::max<long>(ne, 1)));
+    LT_TRY(A.ell_ci.alloc(4 * std::max<long>(ne, 1)));
+    LT_TRY(A.ell_va.alloc(4 * std::max<long>(ne, 1)));
     if (n > 0) {
-        hipLaunchKernelGGL(k_row_list, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, pos.p, A.nzr.p);
+        hipLaunchKernelGGL(k_row_split, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, A.ci.p, A.va.p,
+                           pe.p, po.p, A.ell_row.p, A.ell_ci.p, A.ell_va.p, A.nzr.p);
         LT_CHECK_LAUNCH();
     }
     LT_TRY(sync());
-    A.nnzr = cnt;
+    A.nnzr = no;
+    A.nell = ne;
+    if (A.va32.p) LT_TRY(csr_make_f32(A));  // the table's FP32 values
     return LT_OK;
 }
 
 lt_status csr_make_f32(DevCSR &A) {
+    if (A.nell > 0 && !A.ell_va32.p) {
+        LT_TRY(A.ell_va32.alloc(4 * A.nell));
+        hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(4 * A.nell, 256)), dim3(256), 0, stream(), 4 * A.nell,
+                           A.ell_va.p, A.ell_va32.p);
+        LT_CHECK_LAUNCH();
+    }
     if (A.va32.p || A.nnz == 0) return LT_OK;
     LT_TRY(A.va32.alloc(A.nnz));
     hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(A.nnz, 256)), dim3(256), 0, stream(), A.nnz, A.va.p, A.va32.p);

@@ -155,6 +155,11 @@ class NUFFTPlan:
         B.check(B.lib().lt_nufft_set_weights(self._handle, B.ptr(w)))
         self._has_weights = True
 
+    def set_stencil_shape(self, samples_per_group=0):
+        """Forces the normal stencil's samples per workgroup (1, 2, 4, 8; 0:
+        autotuned); every choice gives the same bits."""
+        B.check(B.lib().lt_nufft_set_stencil_shape(self._handle, int(samples_per_group)))
+
     def time(self, direction, reps=20):
         """Mean device time (ms) of one degrid (0) or grid (1) launch."""
         ms = C.c_double(0)

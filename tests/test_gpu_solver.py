@@ -63,6 +63,29 @@ def test_solver_operator_matches_oracle(hip, oracle, operator):
         assert O.max_relative_error(ys[k].reshape(-1), ref) < 1e-12
 
 
+def test_stencil_shapes_same_bits(hip, oracle):
+    """Every samples-per-workgroup choice of the normal stencil gives the
+    same operator bits; one of them against the oracle's operator."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    im, src, uv, vs, models, data = _problem()
+    lb = LensBatch(im, [src] * len(models), models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    xs = [rng(74 + k).normal(size=src.size) for k in range(len(models))]
+    base = None
+    for nb in (1, 2, 4, 8):
+        plan.set_stencil_shape(nb)
+        inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+        ys = np.stack([y.reshape(-1) for y in inv.apply(xs)])
+        if base is None:
+            base = ys
+            ref, _ = _oracle_apply(O, im, src, uv, lb.csr(0), xs[0], w, 1e2)
+            assert O.max_relative_error(ys[0], ref) < 1e-12
+        assert np.array_equal(ys, base), nb
+    plan.set_stencil_shape(0)
+
+
 def test_cg_converges_and_oracle_confirms(hip, oracle):
     from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
     O = oracle

@@ -33,6 +33,11 @@ def _cases():
     lens = g.choice([0, 1, 2, 3, 4], size=n, p=[0.2, 0.3, 0.3, 0.1, 0.1])
     lens[-300:] = 0
     out.append(('short', n, 4000, lens))
+    # L's full mix: empty, 1..4 (the 4-wide table), 5..64 (lane groups), longer (chunks)
+    n = 40000
+    lens = g.choice([0, 1, 2, 3, 4, 5, 8, 17, 64, 65, 300], size=n,
+                    p=[0.5, 0.01, 0.15, 0.15, 0.1, 0.04, 0.03, 0.01, 0.005, 0.003, 0.002])
+    out.append(('lens mix', n, 16384, lens))
     # rows like L^T / M (10..60 entries) with a few long rows (chunked path)
     n = 6000
     lens = g.integers(10, 60, size=n)
