@@ -536,7 +536,7 @@ struct Solver {
     DevCSR Mpc;                    // explicit M (block-diagonal over samples)
     lt_status lens_setup();
     lt_status lens_build_matrix();
-    lt_status lens_apply(const int *af);  // z = approx M^-1 r
+    lt_status lens_apply(const int *af, const int *alist, int nact);  // z = approx M^-1 r
     PcNet *net = nullptr;          // learned preconditioner (LT_PC_LEARNED)
     DevBuf<float> dhat;            // diag / mean diag per sample, network input
     DevBuf<double> ainv, zap;      // 1 / mean diag; z . A p
@@ -583,7 +583,8 @@ struct Solver {
     // (nres x ntot, host) with the energy loss of this solver's operators
     lt_status train_pcnet(const double *R, int nres, int epochs, float lr, float lr_final, unsigned seed,
                           double *loss_hist);
-    lt_status precondition(int precond, const double *dinv_p);  // z from r
+    // z from r; alist / nact: the samples launched for (all when null)
+    lt_status precondition(int precond, const double *dinv_p, const int *alist = nullptr, int nact = 0);
 };
 
 fourier_space padded_fourier_space(const image_space &im, int pad);
@@ -739,6 +740,14 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     LT_TRY(csr_autotune(lb->Lp, false));
     LT_TRY(csr_autotune(lb->LT, false));
     LT_TRY(csr_autotune(C, false));
+    // sample partitions: products over some of the samples launch only their rows
+    {
+        std::vector<int64_t> poff(S + 1);
+        for (int s = 0; s <= S; ++s) poff[s] = (int64_t)s * npix;
+        LT_TRY(csr_set_samples(lb->Lp, poff.data(), S));
+        LT_TRY(csr_set_samples(lb->LT, soff_host.data(), S));
+        LT_TRY(csr_set_samples(C, soff_host.data(), S));
+    }
     if (use_stencil && !(shared_w && nf->wstencil_tuned_batch == S)) {
         LT_TRY(stencil_autotune(*nf, *stp, img1.p, img2.p, all_active.p, S));
         if (shared_w) nf->wstencil_tuned_batch = S;
@@ -752,15 +761,15 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
 }
 
 lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact, const int *af) {
-    LT_TRY(csr_spmv_nz(lb->Lp, xin, img1.p, af, npix));
+    LT_TRY(csr_spmv_nz(lb->Lp, xin, img1.p, af, npix, alist, nact));
     if (mode != LT_OP_BCB)
         LT_TRY(nf->normal_batched(img1.p, img2.p, wp, alist, nact, 1.0, false, use_stencil ? stp : nullptr));
     else
         LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, alist, nact, 1.0, false));
-    LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0, af, 0, seg.p));
+    LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
     if (lam_c != 0.0) {
-        LT_TRY(csr_spmv(C, xin, tmp.p, 1.0, 0.0, 0, af, 0, seg.p));
-        LT_TRY(csr_spmv(C, tmp.p, yout, lam_c, 1.0, 0, af, 0, seg.p));
+        LT_TRY(csr_spmv(C, xin, tmp.p, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
+        LT_TRY(csr_spmv(C, tmp.p, yout, lam_c, 1.0, 0, af, 0, seg.p, alist, nact));
     }
     if (lam_t != 0.0) {
         hipLaunchKernelGGL(k_axpby, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, lam_t, xin, 1.0, yout);
@@ -903,12 +912,13 @@ lt_status Solver::lens_build_matrix() {
     LT_CHECK_LAUNCH();
     LT_TRY(csr_prepare(Mpc));
     LT_TRY(csr_autotune(Mpc, false));
+    LT_TRY(csr_set_samples(Mpc, soff_host.data(), S));
     return sync();
 }
 
 // z ~ M^-1 r: lens_inner steps of Jacobi-preconditioned CG on
 // M = lam_c Cc^T Cc + c L_s^T L_s from z = 0, every sample at once
-lt_status Solver::lens_apply(const int *af) {
+lt_status Solver::lens_apply(const int *af, const int *alist, int nact) {
     const unsigned gb = blocks_for(ntot, 256);
     hipLaunchKernelGGL(k_fill, dim3(gb), dim3(256), 0, stream(), ntot, 0.0, ix.p);  // a kernel: graph-safe
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, r.p, ir.p);
@@ -918,7 +928,7 @@ lt_status Solver::lens_apply(const int *af) {
     LT_CHECK_LAUNCH();
     for (int k = 0; k < lens_inner; ++k) {
         // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc
-        LT_TRY(csr_spmv(Mpc, ip.p, iMp.p, 1.0, 0.0, 0, af, 0, seg.p));
+        LT_TRY(csr_spmv(Mpc, ip.p, iMp.p, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
         LT_TRY(seg_dot(ip.p, iMp.p, ipMp.p));
         hipLaunchKernelGGL(k_inner_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, irz.p, ipMp.p, ip.p, iMp.p,
                            dM.p, ix.p, ir.p, iz.p);
@@ -939,9 +949,9 @@ lt_status Solver::lens_apply(const int *af) {
     return LT_OK;
 }
 
-lt_status Solver::precondition(int precond, const double *dinv_p) {
+lt_status Solver::precondition(int precond, const double *dinv_p, const int *alist, int nact) {
     const unsigned gb = blocks_for(ntot, 256);
-    if (precond == LT_PC_LENS) return lens_apply(cur_act);
+    if (precond == LT_PC_LENS) return lens_apply(cur_act, alist, nact);
     if (precond != LT_PC_LEARNED) {
         hipLaunchKernelGGL(k_precond, dim3(gb), dim3(256), 0, stream(), ntot, dinv_p, r.p, z.p);
         LT_CHECK_LAUNCH();
@@ -1090,12 +1100,12 @@ lt_status Solver::apply32(const double *xin, double *yout, const int *alist, int
     LT_TRY(seg_dot(xin, xin, xx.p));
     hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, xin, x32.p);
     LT_CHECK_LAUNCH();
-    LT_TRY(csr_spmv32_nz(lb->Lp, x32.p, i1_32.p, af, npix));
+    LT_TRY(csr_spmv32_nz(lb->Lp, x32.p, i1_32.p, af, npix, alist, nact));
     LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, *stp));
-    LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p));
+    LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));
     if (lam_c != 0.0) {
-        LT_TRY(csr_spmv32(C, x32.p, t32.p, 1.f, 0.f, af, 0, seg.p));
-        LT_TRY(csr_spmv32(C, t32.p, c32.p, 1.f, 0.f, af, 0, seg.p));
+        LT_TRY(csr_spmv32(C, x32.p, t32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));
+        LT_TRY(csr_spmv32(C, t32.p, c32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));
     }
     hipLaunchKernelGGL(k_from_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, y32.p,
                        stp->coef32_scale, lam_c != 0.0 ? c32.p : nullptr, lam_c, lam_t, xin, yout);
@@ -1182,7 +1192,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
         LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
-        LT_TRY(precondition(precond, dinv_p));
+        LT_TRY(precondition(precond, dinv_p, list, nl));
         LT_TRY(seg_dot(r.p, z.p, rz_new.p));
         if (flexible) {
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));
@@ -1263,7 +1273,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
         LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
-        LT_TRY(precondition(precond, dinv_p));
+        LT_TRY(precondition(precond, dinv_p, dlist.p, nact));
         LT_TRY(seg_dot(r.p, z.p, rz_new.p));
         if (flexible) {
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));

@@ -141,6 +141,12 @@ struct DevCSR {
     DevBuf<double> ell_va;
     DevBuf<float> ell_va32;
     int lanes_nz = 0, lanes_nz32 = 0;
+    // sample partition (csr_set_samples): sample s owns rows [srow[s],
+    // srow[s+1]) and the table / list entries [sell[s], sell[s+1]) and
+    // [snzr[s], snzr[s+1]); *_max: the largest per-sample extent
+    int ns = 0;
+    DevBuf<int64_t> srow, sell, snzr;
+    long srow_max = 0, sell_max = 0, snzr_max = 0;
 };
 
 constexpr int kLanesStream = 128;  // pseudo lane count: the streaming (LDS-staged) SpMV kernel
@@ -150,15 +156,19 @@ lt_status csr_prepare(DevCSR &A);
 // y = alpha A x + beta y (FP64); lanes_per_row <= 0 uses the prepared choice
 // act (device, per sample): skip the rows of samples with act == 0; a row's
 // sample is row / rows_per_sample, or rowseg[row] when rowseg is given
+// alist / nact: the device list of samples the caller launches for (with a
+// sample partition and nact below the sample count, only their rows run)
 lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes_per_row,
-                   const int *act = nullptr, long rows_per_sample = 0, const int32_t *rowseg = nullptr);
+                   const int *act = nullptr, long rows_per_sample = 0, const int32_t *rowseg = nullptr,
+                   const int *alist = nullptr, int nact = 0);
 int csr_pick_lanes(const DevCSR &A);
 // FP32 copy of the values (after csr_prepare) and y = alpha A x + beta y in FP32
 lt_status csr_make_f32(DevCSR &A);
 // pick the fastest lane-group width for this matrix by timing (FP64 or FP32)
 lt_status csr_autotune(DevCSR &A, bool fp32);
 lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, float beta_y, const int *act = nullptr,
-                     long rows_per_sample = 0, const int32_t *rowseg = nullptr);
+                     long rows_per_sample = 0, const int32_t *rowseg = nullptr, const int *alist = nullptr,
+                     int nact = 0);
 // B = A^T with column order inside each row ascending (deterministic)
 lt_status csr_transpose(const DevCSR &A, DevCSR &B);
 // B = A without its entries whose value is exactly zero (order kept)
@@ -169,9 +179,12 @@ lt_status csr_nonempty_rows(DevCSR &A);
 // y = A x on the nonempty rows only (csr_nonempty_rows): the other rows of y
 // are not written (the caller keeps them zero).  Same filter as csr_spmv.
 lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act = nullptr,
-                      long rows_per_sample = 0);
+                      long rows_per_sample = 0, const int *alist = nullptr, int nact = 0);
 lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act = nullptr,
-                        long rows_per_sample = 0);
+                        long rows_per_sample = 0, const int *alist = nullptr, int nact = 0);
+// sample partition of A's rows (row_off: S + 1 host offsets, ascending);
+// after csr_nonempty_rows when that is used
+lt_status csr_set_samples(DevCSR &A, const int64_t *row_off, int S);
 
 // Workgroup order for the 8 XCDs (each with its own L2): the dispatcher
 // hands workgroup b to XCD b % 8, so renumbering the i-th workgroup of XCD x

@@ -40,16 +40,13 @@ struct RowFilter {
     }
 };
 
-// T: value and vector type (double, or float for the precision form)
+// T: value and vector type (double, or float for the precision form).
+// One row's product by a group of LANES lanes; lane 0 writes.  Rows above
+// skip_long entries (> 0) are left to the chunked path.
 template <int LANES, class T>
-__global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32_t *__restrict__ rp,
-                                               const int32_t *__restrict__ ci, const T *__restrict__ va,
-                                               const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
-                                               int skip_long, RowFilter F) {
-    const long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const long row = row0 + gid / LANES;
-    const int lane = (int)(gid % LANES);
-    if (row >= nrows || F.skip(row)) return;  // whole lane groups leave together
+__device__ inline void row_spmv(long row, int lane, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                const T *__restrict__ va, const T *__restrict__ x, T *__restrict__ y, T alpha,
+                                T beta, int skip_long) {
     T s = (T)0;
     const int b = rp[row], e = rp[row + 1];
     if (skip_long > 0 && e - b > skip_long) return;  // summed by k_spmv_chunks / k_spmv_long
@@ -57,6 +54,34 @@ __global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
     if (lane == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
+}
+
+template <int LANES, class T>
+__global__ void __launch_bounds__(256) k_spmv(long row0, long nrows, const int32_t *__restrict__ rp,
+                                               const int32_t *__restrict__ ci, const T *__restrict__ va,
+                                               const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+                                               int skip_long, RowFilter F) {
+    const long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long row = row0 + gid / LANES;
+    if (row >= nrows || F.skip(row)) return;  // whole lane groups leave together
+    row_spmv<LANES, T>(row, (int)(gid % LANES), rp, ci, va, x, y, alpha, beta, skip_long);
+}
+
+// Sample grid: workgroup row y takes sample alist[y] (skipped when its act
+// flag is 0) and x runs over that sample's rows [srow[s], srow[s+1]), so a
+// product over a few samples launches only their rows.
+template <int LANES, class T>
+__global__ void __launch_bounds__(256) k_spmv_sg(const int *__restrict__ alist, const int *__restrict__ act,
+                                                  const int64_t *__restrict__ srow, const int32_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ ci, const T *__restrict__ va,
+                                                  const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+                                                  int skip_long) {
+    const int s = alist[blockIdx.y];
+    if (act && act[s] == 0) return;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long row = srow[s] + gid / LANES;
+    if (row >= srow[s + 1]) return;
+    row_spmv<LANES, T>(row, (int)(gid % LANES), rp, ci, va, x, y, alpha, beta, skip_long);
 }
 
 // The same product over a list of rows (the nonempty rows of a matrix whose
@@ -68,22 +93,42 @@ __global__ void __launch_bounds__(256) k_spmv_rows(long i0, long n, const int32_
                                                     T *__restrict__ y, int skip_long, RowFilter F) {
     const long gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const long i = i0 + gid / LANES;
-    const int lane = (int)(gid % LANES);
     if (i >= n) return;
     const long row = nzr[i];
     if (F.skip(row)) return;
-    T s = (T)0;
-    const int b = rp[row], e = rp[row + 1];
-    if (skip_long > 0 && e - b > skip_long) return;
-    for (int k = b + lane; k < e; k += LANES) s += va[k] * x[ci[k]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
-    if (lane == 0) y[row] = s;
+    row_spmv<LANES, T>(row, (int)(gid % LANES), rp, ci, va, x, y, (T)1, (T)0, skip_long);
+}
+
+// list entries of sample alist[y]: [snzr[s], snzr[s+1])
+template <int LANES, class T>
+__global__ void __launch_bounds__(256) k_spmv_rows_sg(const int *__restrict__ alist, const int *__restrict__ act,
+                                                       const int64_t *__restrict__ snzr,
+                                                       const int32_t *__restrict__ nzr, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci, const T *__restrict__ va,
+                                                       const T *__restrict__ x, T *__restrict__ y, int skip_long) {
+    const int s = alist[blockIdx.y];
+    if (act && act[s] == 0) return;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long i = snzr[s] + gid / LANES;
+    if (i >= snzr[s + 1]) return;
+    row_spmv<LANES, T>(nzr[i], (int)(gid % LANES), rp, ci, va, x, y, (T)1, (T)0, skip_long);
 }
 
 // Rows of 1..4 entries, one thread per row of the padded table: the row
 // index, the four columns and the four values are independent aligned
 // loads, then four gathers.  Terms are added in the row's order.
+template <class T>
+__device__ inline void ell4_row(long i, long row, const int4 *__restrict__ eci, const T *__restrict__ eva,
+                                const T *__restrict__ x, T *__restrict__ y) {
+    const int4 c = eci[i];
+    const T v0 = eva[4 * i], v1 = eva[4 * i + 1], v2 = eva[4 * i + 2], v3 = eva[4 * i + 3];
+    T s = v0 * x[c.x];
+    s += v1 * x[c.y];
+    s += v2 * x[c.z];
+    s += v3 * x[c.w];
+    y[row] = s;
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_spmv_ell4(long n, const int32_t *__restrict__ erow,
                                                     const int4 *__restrict__ eci, const T *__restrict__ eva,
@@ -91,14 +136,38 @@ __global__ void __launch_bounds__(256) k_spmv_ell4(long n, const int32_t *__rest
     const long i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const long row = erow[i];
-    const int4 c = eci[i];
-    const T v0 = eva[4 * i], v1 = eva[4 * i + 1], v2 = eva[4 * i + 2], v3 = eva[4 * i + 3];
     if (F.skip(row)) return;
-    T s = v0 * x[c.x];
-    s += v1 * x[c.y];
-    s += v2 * x[c.z];
-    s += v3 * x[c.w];
-    y[row] = s;
+    ell4_row<T>(i, row, eci, eva, x, y);
+}
+
+// table entries of sample alist[y]: [sell[s], sell[s+1])
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_ell4_sg(const int *__restrict__ alist, const int *__restrict__ act,
+                                                       const int64_t *__restrict__ sell,
+                                                       const int32_t *__restrict__ erow, const int4 *__restrict__ eci,
+                                                       const T *__restrict__ eva, const T *__restrict__ x,
+                                                       T *__restrict__ y) {
+    const int s = alist[blockIdx.y];
+    if (act && act[s] == 0) return;
+    const long i = sell[s] + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sell[s + 1]) return;
+    ell4_row<T>(i, erow[i], eci, eva, x, y);
+}
+
+// first position of a sorted list at or above each key
+__global__ void k_lower_bound(int nkey, const int64_t *__restrict__ key, const int32_t *__restrict__ list, long n,
+                              int64_t *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nkey) return;
+    long lo = 0, hi = n;
+    while (lo < hi) {
+        const long mid = (lo + hi) / 2;
+        if (list[mid] < key[t])
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[t] = lo;
 }
 
 __global__ void k_row_class(long n, const int32_t *__restrict__ rp, int64_t *__restrict__ fe,
@@ -402,11 +471,37 @@ lt_status csr_prepare(DevCSR &A) {
     return sync();
 }
 
+// calls f(std::integral_constant<int, L>) for a lane-group width L
+template <class Fn>
+static lt_status with_lanes(int lanes, Fn &&f) {
+    switch (lanes) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        case 16: f(std::integral_constant<int, 16>{}); break;
+        case 32: f(std::integral_constant<int, 32>{}); break;
+        case 64: f(std::integral_constant<int, 64>{}); break;
+        default:
+            set_error("spmv: lanes per row must be a power of two <= 64");
+            return LT_ERR_ARG;
+    }
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+// alist / nact (device list, its length): when the matrix has a sample
+// partition and the list holds fewer than all samples, the row kernels
+// launch a sample grid over the listed samples only (same row products,
+// same bits); the streaming and long-row kernels keep their filters.
 template <class T>
 static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, T *y, T alpha, T beta_y, int lanes,
-                        const int *act, long rows_per_sample, const int32_t *rowseg, bool nz_only = false) {
+                        const int *act, long rows_per_sample, const int32_t *rowseg, bool nz_only = false,
+                        const int *alist = nullptr, int nact = 0) {
     if (A.nrows == 0) return LT_OK;
     const RowFilter F{act, rows_per_sample > 0 ? rows_per_sample : 1, rowseg};
+    const bool sg = alist && A.ns > 0 && nact < A.ns;
+    if (sg && nact <= 0) return LT_OK;
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
     const int skip = (A.prepared && A.nlong > 0) ? A.long_row : 0;  // row length above which the chunked path sums
     if (nz_only) {
@@ -414,67 +509,61 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
             set_error("spmv: nonempty-row product needs csr_nonempty_rows, alpha 1, beta 0, lanes <= 64");
             return LT_ERR_ARG;
         }
-        if (A.nell > 0) {
-            const T *eva = std::is_same<T, float>::value ? (const T *)A.ell_va32.p : (const T *)A.ell_va.p;
-            hipLaunchKernelGGL((k_spmv_ell4<T>), dim3(blocks_for(A.nell, 256)), dim3(256), 0, stream(), A.nell,
-                               A.ell_row.p, (const int4 *)A.ell_ci.p, eva, x, y, F);
-            LT_CHECK_LAUNCH();
-        }
-        const long per = (1L << 30) / lanes;
-        for (long i0 = 0; i0 < A.nnzr; i0 += per) {
-            const long ni = std::min(per, A.nnzr - i0);
-            const dim3 g(blocks_for(ni * lanes, 256)), b(256);
-#define LT_SPMV_CASE(L)                                                                                            \
-    case L:                                                                                                        \
-        hipLaunchKernelGGL((k_spmv_rows<L, T>), g, b, 0, stream(), i0, i0 + ni, A.nzr.p, A.rp.p, A.ci.p, va, x, y,  \
-                           skip, F);                                                                               \
-        break;
-            switch (lanes) {
-                LT_SPMV_CASE(1)
-                LT_SPMV_CASE(2)
-                LT_SPMV_CASE(4)
-                LT_SPMV_CASE(8)
-                LT_SPMV_CASE(16)
-                LT_SPMV_CASE(32)
-                LT_SPMV_CASE(64)
-                default:
-                    set_error("spmv: lanes per row must be a power of two <= 64");
-                    return LT_ERR_ARG;
+        const T *eva = std::is_same<T, float>::value ? (const T *)A.ell_va32.p : (const T *)A.ell_va.p;
+        if (sg) {
+            if (A.sell_max > 0) {
+                hipLaunchKernelGGL((k_spmv_ell4_sg<T>), dim3(blocks_for(A.sell_max, 256), nact), dim3(256), 0,
+                                   stream(), alist, act, A.sell.p, A.ell_row.p, (const int4 *)A.ell_ci.p, eva, x, y);
+                LT_CHECK_LAUNCH();
             }
-#undef LT_SPMV_CASE
-            LT_CHECK_LAUNCH();
+            if (A.snzr_max > 0)
+                LT_TRY(with_lanes(lanes, [&](auto LC) {
+                    constexpr int L = decltype(LC)::value;
+                    hipLaunchKernelGGL((k_spmv_rows_sg<L, T>), dim3(blocks_for(A.snzr_max * L, 256), nact),
+                                       dim3(256), 0, stream(), alist, act, A.snzr.p, A.nzr.p, A.rp.p, A.ci.p, va, x,
+                                       y, skip);
+                }));
+        } else {
+            if (A.nell > 0) {
+                hipLaunchKernelGGL((k_spmv_ell4<T>), dim3(blocks_for(A.nell, 256)), dim3(256), 0, stream(), A.nell,
+                                   A.ell_row.p, (const int4 *)A.ell_ci.p, eva, x, y, F);
+                LT_CHECK_LAUNCH();
+            }
+            // a dispatch covers at most 2^30 work-items (the grid size is 32-bit)
+            const long per = (1L << 30) / lanes;
+            for (long i0 = 0; i0 < A.nnzr; i0 += per) {
+                const long ni = std::min(per, A.nnzr - i0);
+                LT_TRY(with_lanes(lanes, [&](auto LC) {
+                    constexpr int L = decltype(LC)::value;
+                    hipLaunchKernelGGL((k_spmv_rows<L, T>), dim3(blocks_for(ni * L, 256)), dim3(256), 0, stream(),
+                                       i0, i0 + ni, A.nzr.p, A.rp.p, A.ci.p, va, x, y, skip, F);
+                }));
+            }
         }
-    }
-    if (!nz_only && lanes == kLanesStream && !(A.prepared && A.nblk > 0)) lanes = csr_pick_lanes(A);
-    if (!nz_only && lanes == kLanesStream) {
+    } else if (lanes == kLanesStream && A.prepared && A.nblk > 0) {
         hipLaunchKernelGGL((k_spmv_stream<T>), dim3((unsigned)A.nblk), dim3(256), 0, stream(), A.nnz, A.rblk.p,
                            A.rp.p, A.ci.p, va, x, y, alpha, beta_y, skip, F);
         LT_CHECK_LAUNCH();
-    }
-    // a dispatch covers at most 2^30 work-items (the grid size is 32-bit)
-    const long rows_per_launch = (1L << 30) / std::max(1, std::min(lanes, 64));
-    for (long r0 = 0; !nz_only && lanes != kLanesStream && r0 < A.nrows; r0 += rows_per_launch) {
-        const long nr = std::min(rows_per_launch, A.nrows - r0);
-        const dim3 g(blocks_for(nr * lanes, 256)), b(256);
-#define LT_SPMV_CASE(L)                                                                                            \
-    case L:                                                                                                        \
-        hipLaunchKernelGGL((k_spmv<L, T>), g, b, 0, stream(), r0, r0 + nr, A.rp.p, A.ci.p, va, x, y, alpha, beta_y, \
-                           skip, F);                                                                               \
-        break;
-        switch (lanes) {
-            LT_SPMV_CASE(1)
-            LT_SPMV_CASE(2)
-            LT_SPMV_CASE(4)
-            LT_SPMV_CASE(8)
-            LT_SPMV_CASE(16)
-            LT_SPMV_CASE(32)
-            LT_SPMV_CASE(64)
-            default:
-                set_error("spmv: lanes per row must be a power of two <= 64");
-                return LT_ERR_ARG;
+    } else {
+        if (lanes == kLanesStream) lanes = csr_pick_lanes(A);
+        if (sg) {
+            if (A.srow_max > 0)
+                LT_TRY(with_lanes(lanes, [&](auto LC) {
+                    constexpr int L = decltype(LC)::value;
+                    hipLaunchKernelGGL((k_spmv_sg<L, T>), dim3(blocks_for(A.srow_max * L, 256), nact), dim3(256), 0,
+                                       stream(), alist, act, A.srow.p, A.rp.p, A.ci.p, va, x, y, alpha, beta_y, skip);
+                }));
+        } else {
+            const long rows_per_launch = (1L << 30) / std::min(lanes, 64);
+            for (long r0 = 0; r0 < A.nrows; r0 += rows_per_launch) {
+                const long nr = std::min(rows_per_launch, A.nrows - r0);
+                LT_TRY(with_lanes(lanes, [&](auto LC) {
+                    constexpr int L = decltype(LC)::value;
+                    hipLaunchKernelGGL((k_spmv<L, T>), dim3(blocks_for(nr * L, 256)), dim3(256), 0, stream(), r0,
+                                       r0 + nr, A.rp.p, A.ci.p, va, x, y, alpha, beta_y, skip, F);
+                }));
+            }
         }
-#undef LT_SPMV_CASE
-        LT_CHECK_LAUNCH();
     }
     if (skip) {
         hipLaunchKernelGGL((k_spmv_chunks<T>), dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk,
@@ -487,23 +576,57 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
     return LT_OK;
 }
 
+lt_status csr_set_samples(DevCSR &A, const int64_t *row_off, int S) {
+    if (S <= 0 || row_off[S] != A.nrows) {
+        set_error("spmv: sample partition does not cover the matrix rows");
+        return LT_ERR_ARG;
+    }
+    LT_TRY(A.srow.upload(row_off, S + 1));
+    A.srow_max = 0;
+    for (int s = 0; s < S; ++s) A.srow_max = std::max<long>(A.srow_max, row_off[s + 1] - row_off[s]);
+    if (A.nnzr >= 0) {
+        std::vector<int64_t> he(S + 1), hn(S + 1);
+        LT_TRY(A.sell.alloc(S + 1));
+        LT_TRY(A.snzr.alloc(S + 1));
+        hipLaunchKernelGGL(k_lower_bound, dim3(blocks_for(S + 1, 64)), dim3(64), 0, stream(), S + 1, A.srow.p,
+                           A.ell_row.p, A.nell, A.sell.p);
+        hipLaunchKernelGGL(k_lower_bound, dim3(blocks_for(S + 1, 64)), dim3(64), 0, stream(), S + 1, A.srow.p,
+                           A.nzr.p, A.nnzr, A.snzr.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(A.sell.download(he.data(), S + 1));
+        LT_TRY(A.snzr.download(hn.data(), S + 1));
+        LT_TRY(sync());
+        A.sell_max = A.snzr_max = 0;
+        for (int s = 0; s < S; ++s) {
+            A.sell_max = std::max<long>(A.sell_max, he[s + 1] - he[s]);
+            A.snzr_max = std::max<long>(A.snzr_max, hn[s + 1] - hn[s]);
+        }
+    }
+    A.ns = S;
+    return sync();
+}
+
 lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, double beta_y, int lanes,
-                   const int *act, long rows_per_sample, const int32_t *rowseg) {
-    return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, alpha, beta_y, lanes, act, rows_per_sample, rowseg);
+                   const int *act, long rows_per_sample, const int32_t *rowseg, const int *alist, int nact) {
+    return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, alpha, beta_y, lanes, act, rows_per_sample, rowseg, false,
+                          alist, nact);
 }
 
-lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act, long rows_per_sample) {
+lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act, long rows_per_sample,
+                      const int *alist, int nact) {
     return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, 1.0, 0.0, A.lanes_nz > 0 ? A.lanes_nz : 4, act,
-                          rows_per_sample, nullptr, true);
+                          rows_per_sample, nullptr, true, alist, nact);
 }
 
-lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act, long rows_per_sample) {
+lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act, long rows_per_sample,
+                        const int *alist, int nact) {
     if ((A.nnz > 0 && !A.va32.p) || (A.nell > 0 && !A.ell_va32.p)) {
         set_error("spmv: FP32 values not prepared");
         return LT_ERR_ARG;
     }
     const int lanes = A.lanes_nz32 > 0 ? A.lanes_nz32 : (A.lanes_nz > 0 ? A.lanes_nz : 4);
-    return spmv_t<float>(A, A.va32.p, A.chunk_sum32.p, x, y, 1.f, 0.f, lanes, act, rows_per_sample, nullptr, true);
+    return spmv_t<float>(A, A.va32.p, A.chunk_sum32.p, x, y, 1.f, 0.f, lanes, act, rows_per_sample, nullptr, true,
+                         alist, nact);
 }
 
 lt_status csr_nonempty_rows(DevCSR &A) {
@@ -555,13 +678,13 @@ lt_status csr_make_f32(DevCSR &A) {
 }
 
 lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, float beta_y, const int *act,
-                     long rows_per_sample, const int32_t *rowseg) {
+                     long rows_per_sample, const int32_t *rowseg, const int *alist, int nact) {
     if (A.nnz > 0 && !A.va32.p) {
         set_error("spmv: FP32 values not prepared");
         return LT_ERR_ARG;
     }
     return spmv_t<float>(A, A.va32.p, A.chunk_sum32.p, x, y, alpha, beta_y, A.lanes32, act, rows_per_sample,
-                         rowseg);
+                         rowseg, false, alist, nact);
 }
 
 // Autotuner: times the product with every lane-group width (1..64 lanes per
