@@ -1019,34 +1019,58 @@ __global__ void k_stencil_fold(Geo G, const int32_t *__restrict__ rowmap, long n
     spec[(long)smp * sstride + g] = tot;
 }
 
-// The same fold into a band stored [v][u] (v < nbv): unfolded (a, v) plus
-// the conjugate of (-a, -v), column 0 doubled
-template <class T>
-__global__ void k_stencil_fold_band(Geo G, const int32_t *__restrict__ rowmap, long nrow,
-                                    const Cplx<T> *__restrict__ rbuf, Cplx<T> *__restrict__ band, int nbv) {
-    const int slot = blockIdx.y;
+// the fold's cells: row v = 0 whole (the Hermitian step below writes any
+// of it), other rows where a stencil row or its mirror exists; r1 / r2 are
+// the stencil rows of (a, v) and of (-a, -v) (-1: none)
+__global__ void k_fold_cells(Geo G, const int32_t *__restrict__ rowmap, int nbv, int64_t *__restrict__ flag,
+                             int32_t *__restrict__ r1o, int32_t *__restrict__ r2o) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)nbv * G.nu) return;
     const int iv = (int)(g / G.nu), iu = (int)(g - (long)iv * G.nu);
     const int a = iu <= G.bu ? iu : iu - G.nu;
-    Cplx<T> tot{(T)0, (T)0};
+    int r1 = -1, r2 = -1;
     if (abs(a) <= G.bu && iv <= G.bv) {
-        const int r1 = rowmap[(long)(a + G.cu) * G.nv + (iv + G.cv)];
-        if (r1 >= 0) tot = rbuf[(long)slot * nrow + r1];
-        if (iv >= 1) {
-            const int r2 = rowmap[(long)(-a + G.cu) * G.nv + (-iv + G.cv)];
-            if (r2 >= 0) {
-                const Cplx<T> f = rbuf[(long)slot * nrow + r2];
-                tot.re += f.re;
-                tot.im -= f.im;
-            }
-        }
-        if (iv == 0) {
-            tot.re *= (T)2;
-            tot.im *= (T)2;
-        }
+        r1 = rowmap[(long)(a + G.cu) * G.nv + (iv + G.cv)];
+        if (iv >= 1) r2 = rowmap[(long)(-a + G.cu) * G.nv + (-iv + G.cv)];
     }
-    band[(long)slot * nbv * G.nu + g] = tot;
+    r1o[g] = r1;
+    r2o[g] = r2;
+    flag[g] = (iv == 0 || r1 >= 0 || r2 >= 0) ? 1 : 0;
+}
+
+__global__ void k_fold_compact(long n, const int64_t *__restrict__ pos, const int32_t *__restrict__ r1,
+                               const int32_t *__restrict__ r2, int32_t *__restrict__ cell, int32_t *__restrict__ o1,
+                               int32_t *__restrict__ o2) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n || pos[g + 1] == pos[g]) return;
+    cell[pos[g]] = (int32_t)g;
+    o1[pos[g]] = r1[g];
+    o2[pos[g]] = r2[g];
+}
+
+// The fold into a band stored [v][u] (v < nbv) over the fold's cells only
+// (the rest of the band is zero and stays so): unfolded (a, v) plus the
+// conjugate of (-a, -v), row v = 0 doubled
+template <class T>
+__global__ void k_stencil_fold_list(long nfold, int nu, const int32_t *__restrict__ cell,
+                                    const int32_t *__restrict__ r1, const int32_t *__restrict__ r2, long nrow,
+                                    const Cplx<T> *__restrict__ rbuf, Cplx<T> *__restrict__ band, long bsz) {
+    const int slot = blockIdx.y;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nfold) return;
+    const int g = cell[i], a1 = r1[i], a2 = r2[i];
+    Cplx<T> tot{(T)0, (T)0};
+    if (a1 >= 0) tot = rbuf[(long)slot * nrow + a1];
+    if (a2 >= 0) {
+        const Cplx<T> f = rbuf[(long)slot * nrow + a2];
+        tot.re += f.re;
+        tot.im -= f.im;
+    }
+    if (g < nu) {  // row v = 0
+        tot.re *= (T)2;
+        tot.im *= (T)2;
+    }
+    band[(long)slot * bsz + g] = tot;
 }
 
 // Hermitian symmetrisation of band row v = 0 (what c2r assumes of column 0)
@@ -1265,19 +1289,6 @@ __global__ void __launch_bounds__(256) k_rows_fwd(int nx, int ny, int pady, int 
                                    : Cplx<T>{h * (z.re + zm.re), h * (z.im - zm.im)};
         B[(long)v * nu + padx + row0 + rr] = x;
     }
-}
-
-// band input rows outside the image (u < padx or u >= padx + nx) are zero;
-// the in-place column transform overwrites them, so they are rewritten
-template <class T>
-__global__ void k_band_zero_rows(int nx, int padx, int nu, int nbv, Cplx<T> *__restrict__ band) {
-    const int slot = blockIdx.y;
-    const int nz = nu - nx;
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long)nbv * nz) return;
-    const int v = (int)(g / nz), z = (int)(g - (long)v * nz);
-    const int u = z < padx ? z : z + nx;
-    band[(long)slot * nbv * nu + (long)v * nu + u] = Cplx<T>{(T)0, (T)0};
 }
 
 template <class T, int N, int PAIRS, class TO>
@@ -1990,14 +2001,14 @@ lt_status NufftPlan::band_fft(int n, int precision, int nbv, BandFFT *&out) {
     b->nbv = nbv;
     const rocfft_precision rp = precision == LT_F64 ? rocfft_precision_double : rocfft_precision_single;
     const size_t ly = (size_t)nyp, lu = (size_t)nu;
-    // rows along y (r2c / c2r, out of place), columns along u (c2c, in place)
+    // rows along y (r2c / c2r) and columns along u (c2c), all out of place
     LT_TRY(fft_check(rocfft_plan_create(&b->plan[0], rocfft_placement_notinplace, rocfft_transform_type_real_forward,
                                         rp, 1, &ly, (size_t)n * nx, nullptr),
                      "band plan (rows r2c)"));
-    LT_TRY(fft_check(rocfft_plan_create(&b->plan[1], rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+    LT_TRY(fft_check(rocfft_plan_create(&b->plan[1], rocfft_placement_notinplace, rocfft_transform_type_complex_forward,
                                         rp, 1, &lu, (size_t)n * nbv, nullptr),
                      "band plan (columns forward)"));
-    LT_TRY(fft_check(rocfft_plan_create(&b->plan[2], rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+    LT_TRY(fft_check(rocfft_plan_create(&b->plan[2], rocfft_placement_notinplace, rocfft_transform_type_complex_inverse,
                                         rp, 1, &lu, (size_t)n * nbv, nullptr),
                      "band plan (columns inverse)"));
     LT_TRY(fft_check(rocfft_plan_create(&b->plan[3], rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
@@ -2060,14 +2071,10 @@ constexpr int rows_pairs() {
     return per >= 32768 ? 1 : (32768 / per > 4 ? 4 : 32768 / per);
 }
 
+// writes the image rows of B; its padding rows stay as they are (zero)
 template <class T, class TI>
 static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *active, int nactive, Cplx<T> *B) {
     const Cplx<T> *tw = twiddles_of<T>(P);
-    if (P.nu > P.nx) {
-        hipLaunchKernelGGL((k_band_zero_rows<T>), dim3(blocks_for((long)nbv * (P.nu - P.nx), 256), nactive),
-                           dim3(256), 0, stream(), P.nx, P.padx, P.nu, nbv, B);
-        LT_CHECK_LAUNCH();
-    }
 #define LT_ROWS_FWD(NN)                                                                                            \
     case NN: {                                                                                                     \
         constexpr int PR = rows_pairs<T, NN>();                                                                    \
@@ -2143,19 +2150,28 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     const bool fused = fused_rows_ok(P, nbv);
     if (!fused && P.bandR.n < need_r) LT_TRY(P.bandR.alloc(need_r));
     if (!fused && P.bandC.n < need_c) LT_TRY(P.bandC.alloc(need_c));
-    if (P.bandB.n < need_b) {
-        LT_TRY(P.bandB.alloc(need_b));
-        LT_TRY(P.bandB.zero());
+    // the band buffers the row transforms and the fold write hold zeros
+    // outside the cells they write (from allocation on); the column
+    // transforms write their outputs to the plan's scratch bandB / bandF
+    const int pi = prec == LT_F64 ? 0 : 1;
+    DevBuf<uint8_t> &bin = St.band_in[pi], &bout = St.band_out[pi];
+    if (bin.n < need_b) {
+        LT_TRY(bin.alloc(need_b));
+        LT_TRY(bin.zero());
     }
-    if (P.bandF.n < need_b) {
-        LT_TRY(P.bandF.alloc(need_b));
-        LT_TRY(P.bandF.zero());
+    if (bout.n < need_b) {
+        LT_TRY(bout.alloc(need_b));
+        LT_TRY(bout.zero());
     }
+    if (P.bandB.n < need_b) LT_TRY(P.bandB.alloc(need_b));
+    if (P.bandF.n < need_b) LT_TRY(P.bandF.alloc(need_b));
+    LT_TRY(St.ensure_fold(P));
     T *R = (T *)P.bandR.p;
-    Cplx<T> *Cb = (Cplx<T> *)P.bandC.p, *Bb = (Cplx<T> *)P.bandB.p, *Fb = (Cplx<T> *)P.bandF.p;
+    Cplx<T> *Cb = (Cplx<T> *)P.bandC.p, *Bin = (Cplx<T> *)bin.p, *Bf = (Cplx<T> *)P.bandB.p,
+            *Fin = (Cplx<T> *)bout.p, *Fo = (Cplx<T> *)P.bandF.p;
     if (fused) {
         LT_TRY(ensure_twiddles(P));
-        LT_TRY((rows_forward<T, TI>(P, nbv, imgs, active, nactive, Bb)));
+        LT_TRY((rows_forward<T, TI>(P, nbv, imgs, active, nactive, Bin)));
     } else {
         hipLaunchKernelGGL((k_band_pad<T, TI>), dim3(blocks_for(rsz, 256), nactive), dim3(256), 0, stream(), P.nx,
                            P.ny, P.pady, P.nyp, imgs, P.apod.p, active, R);
@@ -2164,30 +2180,35 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
         void *o[1] = {Cb};
         LT_TRY(fft_check(rocfft_execute(bf->plan[0], in, o, bf->info[0]), "execute band rows r2c"));
         hipLaunchKernelGGL((k_band_gather<T>), dim3((P.nu + 31) / 32, (nbv + 31) / 32, nactive), dim3(256), 0,
-                           stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Cb, Bb);
+                           stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Cb, Bin);
         LT_CHECK_LAUNCH();
     }
     {
-        void *io[1] = {Bb};
-        LT_TRY(fft_check(rocfft_execute(bf->plan[1], io, nullptr, bf->info[1]), "execute band columns forward"));
+        void *in[1] = {Bin};
+        void *o[1] = {Bf};
+        LT_TRY(fft_check(rocfft_execute(bf->plan[1], in, o, bf->info[1]), "execute band columns forward"));
     }
     if (prec == LT_F64)
-        LT_TRY((stencil_apply_t<double, double>(St, P, St.coef.p, Bb, bsz, nullptr, nullptr, nactive, true)));
+        LT_TRY((stencil_apply_t<double, double>(St, P, St.coef.p, Bf, bsz, nullptr, nullptr, nactive, true)));
     else
-        LT_TRY((stencil_apply_t<float, float>(St, P, St.coef32.p, Bb, bsz, nullptr, nullptr, nactive, true)));
-    Geo G = make_geo(P);
-    hipLaunchKernelGGL((k_stencil_fold_band<T>), dim3(blocks_for(bsz, 256), nactive), dim3(256), 0, stream(), G,
-                       St.rowmap.p, St.nrow, (const Cplx<T> *)St.rbuf.p, Fb, nbv);
+        LT_TRY((stencil_apply_t<float, float>(St, P, St.coef32.p, Bf, bsz, nullptr, nullptr, nactive, true)));
+    if (St.nfold > 0) {
+        hipLaunchKernelGGL((k_stencil_fold_list<T>), dim3(blocks_for(St.nfold, 256), nactive), dim3(256), 0, stream(),
+                           St.nfold, P.nu, St.fold_cell.p, St.fold_r1.p, St.fold_r2.p, St.nrow,
+                           (const Cplx<T> *)St.rbuf.p, Fin, bsz);
+        LT_CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL((k_herm_band0<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0, stream(), P.nu,
-                       bsz, Fb);
+                       bsz, Fin);
     LT_CHECK_LAUNCH();
     {
-        void *io[1] = {Fb};
-        LT_TRY(fft_check(rocfft_execute(bf->plan[2], io, nullptr, bf->info[2]), "execute band columns inverse"));
+        void *in[1] = {Fin};
+        void *o[1] = {Fo};
+        LT_TRY(fft_check(rocfft_execute(bf->plan[2], in, o, bf->info[2]), "execute band columns inverse"));
     }
-    if (fused) return rows_inverse<T, TI>(P, nbv, Fb, active, nactive, alpha, acc, out);
+    if (fused) return rows_inverse<T, TI>(P, nbv, Fo, active, nactive, alpha, acc, out);
     hipLaunchKernelGGL((k_band_scatter<T>), dim3((P.nx + 31) / 32, (P.half_nv + 31) / 32, nactive), dim3(256), 0,
-                       stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Fb, Cb);
+                       stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Fo, Cb);
     LT_CHECK_LAUNCH();
     {
         void *in[1] = {Cb};
@@ -2197,6 +2218,37 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     hipLaunchKernelGGL((k_band_crop<T, TI>), dim3(blocks_for((long)P.nx * P.ny, 256), nactive), dim3(256), 0, stream(),
                        P.nx, P.ny, P.pady, P.nyp, R, P.apod.p, alpha, acc ? 1 : 0, active, out);
     LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
+lt_status NormalStencil::ensure_fold(NufftPlan &P) {
+    if (nfold >= 0) return LT_OK;
+    const Geo G = make_geo(P);
+    const long n = (long)nbv * P.nu;
+    DevBuf<int64_t> flag, pos;
+    DevBuf<int32_t> r1, r2;
+    LT_TRY(flag.alloc(n + 1));
+    LT_TRY(pos.alloc(n + 1));
+    LT_TRY(r1.alloc(std::max<long>(n, 1)));
+    LT_TRY(r2.alloc(std::max<long>(n, 1)));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_fold_cells, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), G, rowmap.p, nbv, flag.p,
+                           r1.p, r2.p);
+        LT_CHECK_LAUNCH();
+    }
+    LT_TRY(exclusive_scan_i64(flag.p, pos.p, n));
+    int64_t cnt = 0;
+    LT_TRY(pos.download(&cnt, 1, n));
+    LT_TRY(fold_cell.alloc(std::max<long>(cnt, 1)));
+    LT_TRY(fold_r1.alloc(std::max<long>(cnt, 1)));
+    LT_TRY(fold_r2.alloc(std::max<long>(cnt, 1)));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_fold_compact, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, pos.p, r1.p, r2.p,
+                           fold_cell.p, fold_r1.p, fold_r2.p);
+        LT_CHECK_LAUNCH();
+    }
+    LT_TRY(sync());
+    nfold = cnt;
     return LT_OK;
 }
 

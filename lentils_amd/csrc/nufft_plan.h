@@ -69,8 +69,18 @@ struct NormalStencil {
     bool use_band = true;      // band-pruned separable transform instead of the full 2-D one
     int nb_sel = 4;            // samples per stencil workgroup (stencil_autotune)
     DevBuf<uint8_t> rbuf;      // per-sample stencil output, [batch][nrow] complex T
+    // band form: the [v][u] band cells the fold can make nonzero (row v = 0
+    // whole, the others where a stencil row or its mirror exists) with
+    // their two stencil rows (-1: none); every other cell stays zero
+    long nfold = -1;
+    DevBuf<int32_t> fold_cell, fold_r1, fold_r2;
+    // band spectra of the row transforms (band_in) and of the fold
+    // (band_out), per precision: written only on the cells above and the
+    // image rows, zero elsewhere from allocation on
+    DevBuf<uint8_t> band_in[2], band_out[2];
 
     lt_status build(NufftPlan &P, const double *w_sorted);
+    lt_status ensure_fold(NufftPlan &P);  // fold_* (band form)
     lt_status apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active, int nactive);
     lt_status apply32(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
                       int nactive);  // FP32 spectra
@@ -138,7 +148,7 @@ struct NufftPlan {
         ~BandFFT();
     };
     std::vector<std::unique_ptr<BandFFT>> bandfft;
-    DevBuf<uint8_t> bandR, bandC, bandB, bandF;
+    DevBuf<uint8_t> bandR, bandC, bandB, bandF;  // bandB / bandF: outputs of the column transforms
     // fused row transforms (power-of-two nyp, 64 .. 2048): twiddle tables
     // exp(-2 pi i e / nyp), e < nyp, complex FP64 and FP32
     DevBuf<double> twd64;
