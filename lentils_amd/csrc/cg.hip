@@ -65,6 +65,27 @@ __global__ void __launch_bounds__(kDotThreads) k_seg_dot_part(const int64_t *__r
     }
 }
 
+// one workgroup per sample (many samples): the whole dot in one launch
+constexpr int kDotOne = 1024;
+__global__ void __launch_bounds__(kDotOne) k_seg_dot_one(const int64_t *__restrict__ off,
+                                                          const double *__restrict__ a,
+                                                          const double *__restrict__ b, double *__restrict__ out) {
+    __shared__ double sh[kDotOne / 64];
+    const int s = blockIdx.x;
+    const long lo = off[s], hi = off[s + 1];
+    double acc = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) acc += a[i] * b[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kDotOne / 64; ++k) t += sh[k];
+        out[s] = t;
+    }
+}
+
 __global__ void k_seg_dot_fin(int S, int P, const double *__restrict__ part, double *__restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
@@ -557,8 +578,15 @@ struct Solver {
     ~Solver() { drop_graphs(); }
     DevBuf<double> dotpart;  // S x dot_parts partial sums of seg_dot
     int dot_parts = 1;
-    // out[s] = a_s . b_s for every sample s (deterministic order)
+    // out[s] = a_s . b_s for every sample s (deterministic order): one
+    // workgroup per sample when there are many samples, else partial sums
+    // over dot_parts workgroups per sample and a second pass
     lt_status seg_dot(const double *a, const double *b, double *out) {
+        if (dot_parts == 1) {
+            hipLaunchKernelGGL(k_seg_dot_one, dim3(S), dim3(kDotOne), 0, stream(), soff.p, a, b, out);
+            LT_CHECK_LAUNCH();
+            return LT_OK;
+        }
         hipLaunchKernelGGL(k_seg_dot_part, dim3(dot_parts, S), dim3(kDotThreads), 0, stream(), soff.p, a, b,
                            dotpart.p);
         LT_CHECK_LAUNCH();
@@ -621,7 +649,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     soff_host = lb->srcoff_host;
     ntot = soff_host[S];
     LT_TRY(soff.upload(soff_host.data(), S + 1));
-    dot_parts = std::max(1, std::min(64, (1024 + S - 1) / S));
+    dot_parts = S >= 32 ? 1 : std::max(1, std::min(64, (1024 + S - 1) / S));
     LT_TRY(dotpart.alloc((long)S * dot_parts));
     LT_TRY(seg.alloc(ntot));
     hipLaunchKernelGGL(k_seg_ids, dim3(64, S), dim3(256), 0, stream(), soff.p, S, seg.p);

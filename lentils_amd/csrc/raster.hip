@@ -129,9 +129,10 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_count(BatchGeom G, co
         return;
     }
     StageEmit se{stcol + r * kStage, stval + r * kStage};
-    int k = reduce_fragments(fr.area, fr.idx, fr.n, 0, se);
+    int first = 0;
+    int k = reduce_fragments(fr.area, fr.idx, fr.n, 0, se, &first);
     nnz_row[r] = k;
-    minidx[r] = fr.n > 0 ? fr.idx[0] : -1;
+    minidx[r] = fr.n > 0 ? first : -1;
 }
 
 // fragment n of a long row goes to base + n; its sort key is
@@ -466,7 +467,8 @@ __global__ void __launch_bounds__(kRasterThreads) k_raster_fill(BatchGeom G, con
     LocalFrags fr;
     if (nfrag[r] > 0) raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], fr);
     StoreEmit se{cols + o, vals + o, (int32_t)srcoff[s]};
-    reduce_fragments(fr.area, fr.idx, fr.n, stale, se);
+    int first = 0;
+    reduce_fragments(fr.area, fr.idx, fr.n, stale, se, &first);
 }
 
 __global__ void k_long_copy(const int64_t *__restrict__ rows, const int64_t *__restrict__ ebase, int nlong,

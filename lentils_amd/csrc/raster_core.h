@@ -332,12 +332,49 @@ LT_HD void fan_vertices(const double *lat, int ly, int i, int j, int t, double t
     }
 }
 
-// Stable insertion sort of a short fragment list by source index followed
-// by the reference's reduction (raster.c:372-391).  Calls emit(col, val)
-// per CSR entry; returns the number of entries.  `stale` is the column a
-// row without fragments reports.
+// The reference's reduction of a row's fragments (raster.c:372-391): a
+// stable sort by source index, then one CSR entry per distinct index in
+// ascending order, its value the in-order sum of the group's areas -- all
+// of them for the first group, all but the first for the others (the
+// reference starts a new group without adding its first area).  A row
+// without fragments gives one entry (stale, 0).  Calls emit(k, col, val)
+// per entry and returns the entry count; *first_col is the first entry's
+// column.  Rows of up to 64 fragments take one pass over the list per
+// entry (the next larger index found while summing the current one), so
+// the lists are only read; longer ones are sorted in place first.
 template <class Emit>
-__host__ __device__ int reduce_fragments(double *area, int *idx, int n, int stale, Emit &emit) {
+__host__ __device__ int reduce_fragments(double *area, int *idx, int n, int stale, Emit &emit, int *first_col) {
+    if (n == 0) {
+        emit(0, stale, 0.0);
+        *first_col = stale;
+        return 1;
+    }
+    if (n <= 64) {
+        int col = idx[0];
+        for (int f = 1; f < n; ++f) col = idx[f] < col ? idx[f] : col;
+        *first_col = col;
+        int k = 0;
+        for (bool first = true;; first = false) {
+            double val = 0.0;
+            bool skip = !first, more = false;
+            int next = 0;
+            for (int f = 0; f < n; ++f) {
+                const int id = idx[f];
+                if (id == col) {
+                    if (skip)
+                        skip = false;
+                    else
+                        val += area[f];
+                } else if (id > col && (!more || id < next)) {
+                    next = id;
+                    more = true;
+                }
+            }
+            emit(k++, col, val);
+            if (!more) return k;
+            col = next;
+        }
+    }
     for (int a = 1; a < n; ++a) {
         const double ar = area[a];
         const int id = idx[a];
@@ -350,8 +387,9 @@ __host__ __device__ int reduce_fragments(double *area, int *idx, int n, int stal
         area[b + 1] = ar;
         idx[b + 1] = id;
     }
+    *first_col = idx[0];
     double val = 0.0;
-    int col = n > 0 ? idx[0] : stale;
+    int col = idx[0];
     int k = 0;
     for (int f = 0; f < n; ++f) {
         if (idx[f] != col) {

@@ -59,8 +59,9 @@ long hc_lens_matrix_from_lattice(int nx, int ny, const char *mask, int snx, int 
             st |= raster_pixel(lat, ly, i, j, g, s);
             StoreEmit e{cols, vals, k};
             const int n = (int)s.area.size();
-            k += reduce_fragments(s.area.data(), s.idx.data(), n, stale, e);
-            if (n > 0) stale = s.idx[0];
+            int first = 0;
+            k += reduce_fragments(s.area.data(), s.idx.data(), n, stale, e, &first);
+            if (n > 0) stale = first;
         }
     rp[(long)nx * ny] = (int)k;
     if (status) *status = st;

@@ -103,6 +103,28 @@ def test_cg_converges_and_oracle_confirms(hip, oracle):
         assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 2e-6
 
 
+def test_many_samples_cg(hip, oracle):
+    """40 samples (one-workgroup-per-sample dots, sample-grid products as
+    samples converge): every sample reaches the FP64 true residual and the
+    oracle's operator confirms it for three of them."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    n = 40
+    im, src, uv, vs, models, data = _problem(nlens=n, seed=77)
+    lb = LensBatch(im, [src] * n, models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    xs, its, res = inv.solve(rtol=1e-6, maxiter=20000)
+    assert np.all(res <= 1e-6), (its, res)
+    assert its.min() < its.max()  # samples leave the active set at different iterations
+    for k in (0, 17, n - 1):
+        x = xs[k].reshape(-1)
+        Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x, w, 1e2)
+        b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 2e-6
+
+
 def test_bcb_matches_explicit(hip, oracle):
     from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
     O = oracle
