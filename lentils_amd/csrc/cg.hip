@@ -65,6 +65,13 @@ __global__ void __launch_bounds__(kDotThreads) k_seg_dot_part(const int64_t *__r
     }
 }
 
+// image rows (per sample, x index) holding a nonempty row of L: the others
+// are zero in L x and not read by L^T
+__global__ void k_row_mask(long n, const int32_t *__restrict__ rp, int ny, uint8_t *__restrict__ mask) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n && rp[r + 1] > rp[r]) mask[r / ny] = 1;
+}
+
 // one workgroup per sample (many samples): the whole dot in one launch
 constexpr int kDotOne = 1024;
 __global__ void __launch_bounds__(kDotOne) k_seg_dot_one(const int64_t *__restrict__ off,
@@ -534,6 +541,7 @@ struct Solver {
     int ndirty = 1;           // 1: one data set shared by every sample; S: one per sample
     DevBuf<double> beamspec;  // BCB multiplier, nu x half_nv complex
     DevBuf<double> b, x, r, z, p, Ap, img1, img2, tmp, dinv;
+    DevBuf<uint8_t> rowmask;  // S x nx: image rows L reaches (k_row_mask)
     DevBuf<double> rz, rz_new, pAp, rr, bb, alpha, beta;
     DevBuf<int> act, iters, all_active;
     double beam_peak = 0.0;
@@ -757,6 +765,11 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     LT_TRY(img2.alloc(npix * S));
     // L x writes only the nonempty rows of L: the others stay zero here
     LT_TRY(img1.zero());
+    LT_TRY(rowmask.alloc((long)S * lb->im.nx));
+    LT_TRY(rowmask.zero());
+    hipLaunchKernelGGL(k_row_mask, dim3(blocks_for(lb->Lp.nrows, 256)), dim3(256), 0, stream(), lb->Lp.nrows,
+                       lb->Lp.rp.p, lb->im.ny, rowmask.p);
+    LT_CHECK_LAUNCH();
     for (auto *v : {&rz, &rz_new, &pAp, &rr, &bb, &alpha, &beta}) LT_TRY(v->alloc(S));
     LT_TRY(act.alloc(S));
     LT_TRY(iters.alloc(S));
@@ -791,7 +804,8 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
 lt_status Solver::apply(const double *xin, double *yout, const int *alist, int nact, const int *af) {
     LT_TRY(csr_spmv_nz(lb->Lp, xin, img1.p, af, npix, alist, nact));
     if (mode != LT_OP_BCB)
-        LT_TRY(nf->normal_batched(img1.p, img2.p, wp, alist, nact, 1.0, false, use_stencil ? stp : nullptr));
+        LT_TRY(nf->normal_batched(img1.p, img2.p, wp, alist, nact, 1.0, false, use_stencil ? stp : nullptr,
+                                  rowmask.p));
     else
         LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, alist, nact, 1.0, false));
     LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
@@ -1129,7 +1143,7 @@ lt_status Solver::apply32(const double *xin, double *yout, const int *alist, int
     hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, xin, x32.p);
     LT_CHECK_LAUNCH();
     LT_TRY(csr_spmv32_nz(lb->Lp, x32.p, i1_32.p, af, npix, alist, nact));
-    LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, *stp));
+    LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, *stp, rowmask.p));
     LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));
     if (lam_c != 0.0) {
         LT_TRY(csr_spmv32(C, x32.p, t32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));

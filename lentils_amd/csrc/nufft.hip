@@ -1260,26 +1260,38 @@ __device__ inline void fft_rows(Cplx<T> (*buf)[N], const Cplx<T> *__restrict__ t
 template <class T, int N, int PAIRS, class TI>
 __global__ void __launch_bounds__(256) k_rows_fwd(int nx, int ny, int pady, int padx, int nu, int nbv,
                                                   const TI *__restrict__ img, const double *__restrict__ apod,
-                                                  const int *__restrict__ active, const Cplx<T> *__restrict__ twd,
-                                                  Cplx<T> *__restrict__ band) {
+                                                  const int *__restrict__ active, const uint8_t *__restrict__ rowmask,
+                                                  const Cplx<T> *__restrict__ twd, Cplx<T> *__restrict__ band) {
     __shared__ Cplx<T> buf[PAIRS][N];
     const int slot = blockIdx.y;
     const int smp = active ? active[slot] : slot;
     const int row0 = blockIdx.x * 2 * PAIRS;
     const TI *im = img + (long)smp * nx * ny;
+    // rows known to be zero (rowmask 0) are not read; a block of such rows
+    // only writes its zero band columns
+    const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
+    bool any = false;
+    for (int rr = 0; rr < 2 * PAIRS; ++rr) any = any || (row0 + rr < nx && (!rm || rm[row0 + rr]));
+    Cplx<T> *B = band + (long)slot * nbv * nu;
+    if (!any) {
+        for (int q = threadIdx.x; q < nbv * 2 * PAIRS; q += 256) {
+            const int v = q / (2 * PAIRS), rr = q - v * 2 * PAIRS;
+            if (row0 + rr < nx) B[(long)v * nu + padx + row0 + rr] = Cplx<T>{(T)0, (T)0};
+        }
+        return;
+    }
     for (int q = threadIdx.x; q < PAIRS * N; q += 256) {
         const int pr = q / N, k = q - pr * N, j = k - pady;
         const int ia = row0 + 2 * pr, ib = ia + 1;
         T re = (T)0, imv = (T)0;
         if (j >= 0 && j < ny) {
-            if (ia < nx) re = (T)(im[(long)ia * ny + j] * (TI)apod[(long)ia * ny + j]);
-            if (ib < nx) imv = (T)(im[(long)ib * ny + j] * (TI)apod[(long)ib * ny + j]);
+            if (ia < nx && (!rm || rm[ia])) re = (T)(im[(long)ia * ny + j] * (TI)apod[(long)ia * ny + j]);
+            if (ib < nx && (!rm || rm[ib])) imv = (T)(im[(long)ib * ny + j] * (TI)apod[(long)ib * ny + j]);
         }
         buf[pr][k] = Cplx<T>{re, imv};
     }
     __syncthreads();
     fft_rows<T, N, PAIRS, false>(buf, twd);
-    Cplx<T> *B = band + (long)slot * nbv * nu;
     const T h = (T)0.5;
     for (int q = threadIdx.x; q < nbv * 2 * PAIRS; q += 256) {
         const int v = q / (2 * PAIRS), rr = q - v * 2 * PAIRS, pr = rr >> 1;
@@ -1295,11 +1307,18 @@ template <class T, int N, int PAIRS, class TO>
 __global__ void __launch_bounds__(256) k_rows_inv(int nx, int ny, int pady, int padx, int nu, int nbv,
                                                   const Cplx<T> *__restrict__ band, const double *__restrict__ apod,
                                                   TO alpha, int accumulate, const int *__restrict__ active,
+                                                  const uint8_t *__restrict__ rowmask,
                                                   const Cplx<T> *__restrict__ twd, TO *__restrict__ out) {
     __shared__ Cplx<T> buf[PAIRS][N];
     const int slot = blockIdx.y;
     const int smp = active ? active[slot] : slot;
     const int row0 = blockIdx.x * 2 * PAIRS;
+    // output rows with rowmask 0 are not needed: not written, and a block of
+    // only such rows does nothing
+    const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
+    bool any = false;
+    for (int rr = 0; rr < 2 * PAIRS; ++rr) any = any || (row0 + rr < nx && (!rm || rm[row0 + rr]));
+    if (!any) return;
     const Cplx<T> *B = band + (long)slot * nbv * nu;
     constexpr int H = N / 2;
     // pass a: the a rows' Hermitian spectra; pass b: + i times the b rows'
@@ -1330,7 +1349,7 @@ __global__ void __launch_bounds__(256) k_rows_inv(int nx, int ny, int pady, int 
     for (int q = threadIdx.x; q < 2 * PAIRS * ny; q += 256) {
         const int rr = q / ny, j = q - rr * ny;
         const int row = row0 + rr;
-        if (row >= nx) continue;
+        if (row >= nx || (rm && !rm[row])) continue;
         const Cplx<T> z = buf[rr >> 1][pady + j];
         const long g = (long)row * ny + j;
         const TO val = alpha * ((TO)((rr & 1) ? z.im : z.re) * (TO)apod[g]);
@@ -2073,13 +2092,15 @@ constexpr int rows_pairs() {
 
 // writes the image rows of B; its padding rows stay as they are (zero)
 template <class T, class TI>
-static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *active, int nactive, Cplx<T> *B) {
+static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *active, int nactive, Cplx<T> *B,
+                              const uint8_t *rowmask) {
     const Cplx<T> *tw = twiddles_of<T>(P);
 #define LT_ROWS_FWD(NN)                                                                                            \
     case NN: {                                                                                                     \
         constexpr int PR = rows_pairs<T, NN>();                                                                    \
         hipLaunchKernelGGL((k_rows_fwd<T, NN, PR, TI>), dim3((P.nx + 2 * PR - 1) / (2 * PR), nactive), dim3(256), \
-                           0, stream(), P.nx, P.ny, P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, tw, B);     \
+                           0, stream(), P.nx, P.ny, P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, rowmask, tw, \
+                           B);                                                                                     \
         break;                                                                                                     \
     }
     switch (P.nyp) {
@@ -2100,14 +2121,14 @@ static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *
 
 template <class T, class TO>
 static lt_status rows_inverse(NufftPlan &P, int nbv, const Cplx<T> *F, const int *active, int nactive, TO alpha,
-                              bool acc, TO *out) {
+                              bool acc, TO *out, const uint8_t *rowmask) {
     const Cplx<T> *tw = twiddles_of<T>(P);
 #define LT_ROWS_INV(NN)                                                                                            \
     case NN: {                                                                                                     \
         constexpr int PR = rows_pairs<T, NN>();                                                                    \
         hipLaunchKernelGGL((k_rows_inv<T, NN, PR, TO>), dim3((P.nx + 2 * PR - 1) / (2 * PR), nactive), dim3(256), \
                            0, stream(), P.nx, P.ny, P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0,    \
-                           active, tw, out);                                                                       \
+                           active, rowmask, tw, out);                                                              \
         break;                                                                                                     \
     }
     switch (P.nyp) {
@@ -2132,7 +2153,7 @@ static lt_status rows_inverse(NufftPlan &P, int nbv, const Cplx<T> *F, const int
 // transform along u, and back the same way.  Equals the full 2-D path.
 template <class T, class TI>
 static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *imgs, TI *out, const int *active,
-                                  int nactive, TI alpha, bool acc) {
+                                  int nactive, TI alpha, bool acc, const uint8_t *rowmask = nullptr) {
     if (nactive <= 0) return LT_OK;
     if (nactive > P.batch) {
         set_error("normal operator: %d samples exceed the batch %d", nactive, P.batch);
@@ -2171,7 +2192,7 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
             *Fin = (Cplx<T> *)bout.p, *Fo = (Cplx<T> *)P.bandF.p;
     if (fused) {
         LT_TRY(ensure_twiddles(P));
-        LT_TRY((rows_forward<T, TI>(P, nbv, imgs, active, nactive, Bin)));
+        LT_TRY((rows_forward<T, TI>(P, nbv, imgs, active, nactive, Bin, rowmask)));
     } else {
         hipLaunchKernelGGL((k_band_pad<T, TI>), dim3(blocks_for(rsz, 256), nactive), dim3(256), 0, stream(), P.nx,
                            P.ny, P.pady, P.nyp, imgs, P.apod.p, active, R);
@@ -2206,7 +2227,7 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
         void *o[1] = {Fo};
         LT_TRY(fft_check(rocfft_execute(bf->plan[2], in, o, bf->info[2]), "execute band columns inverse"));
     }
-    if (fused) return rows_inverse<T, TI>(P, nbv, Fo, active, nactive, alpha, acc, out);
+    if (fused) return rows_inverse<T, TI>(P, nbv, Fo, active, nactive, alpha, acc, out, rowmask);
     hipLaunchKernelGGL((k_band_scatter<T>), dim3((P.nx + 31) / 32, (P.half_nv + 31) / 32, nactive), dim3(256), 0,
                        stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Fo, Cb);
     LT_CHECK_LAUNCH();
@@ -2283,10 +2304,10 @@ lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, 
 }
 
 lt_status NufftPlan::normal_batched32(const float *imgs, float *out, const int *active, int nactive, float alpha,
-                                      bool acc, NormalStencil &st) {
+                                      bool acc, NormalStencil &st, const uint8_t *rowmask) {
     if (st.use_band) {
         if (!st.coef32.p) LT_TRY(st.apply32(*this, nullptr, 0, nullptr, nullptr, 0));  // makes coef32
-        return normal_band_impl<float, float>(*this, st, imgs, out, active, nactive, alpha, acc);
+        return normal_band_impl<float, float>(*this, st, imgs, out, active, nactive, alpha, acc, rowmask);
     }
     if (nactive <= 0) return LT_OK;
     if (nactive > batch) {
@@ -2321,9 +2342,9 @@ lt_status NufftPlan::normal_batched32(const float *imgs, float *out, const int *
 }
 
 lt_status NufftPlan::normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
-                                    double alpha, bool acc, NormalStencil *st) {
+                                    double alpha, bool acc, NormalStencil *st, const uint8_t *rowmask) {
     if (st && st->use_band && prec == LT_F64)
-        return normal_band_impl<double, double>(*this, *st, imgs, out, active, nactive, alpha, acc);
+        return normal_band_impl<double, double>(*this, *st, imgs, out, active, nactive, alpha, acc, rowmask);
     return prec == LT_F64 ? normal_impl<double>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr, st)
                           : normal_impl<float>(*this, imgs, out, w, active, nactive, alpha, acc, nullptr, st);
 }

@@ -173,8 +173,12 @@ struct NufftPlan {
     // out_s = alpha * D^T diag(w) D img_s (+ out_s) for the samples in
     // `active` (device list of sample ids); images are nx*ny float64 each
     lt_status ensure_batch(int nb);
+    // rowmask (band stencil form only; nullable): per (sample, image row x
+    // index) 0 when that row of the input image is zero and its output is
+    // not needed -- the fused row transforms then neither read nor write it
     lt_status normal_batched(const double *imgs, double *out, const double *w, const int *active, int nactive,
-                             double alpha, bool accumulate, NormalStencil *stencil = nullptr);
+                             double alpha, bool accumulate, NormalStencil *stencil = nullptr,
+                             const uint8_t *rowmask = nullptr);
     // out_s = crop(irfft2(spec_mul * rfft2(zpad(img_s)))) (BCB), spec_mul is
     // nu x half_nv complex128
     lt_status convolve_batched(const double *imgs, double *out, const double *spec_mul, const int *active,
@@ -183,7 +187,7 @@ struct NufftPlan {
     // solver): FP32 images, spectra, rocFFT and stencil coefficients; the
     // result is D^T C D img / stencil.coef32_scale
     lt_status normal_batched32(const float *imgs, float *out, const int *active, int nactive, float alpha,
-                               bool accumulate, NormalStencil &stencil);
+                               bool accumulate, NormalStencil &stencil, const uint8_t *rowmask = nullptr);
     // permute between caller order (complex128) and sorted complex T
     lt_status to_sorted(const double *vis_c128, void *sorted);
     lt_status from_sorted(const void *sorted, double *vis_c128, bool accumulate);
