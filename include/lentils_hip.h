@@ -172,7 +172,8 @@ lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int
                            const double *x, double *y);
 /* Same with the device kernel chosen: 0 the default, 1..64 (powers of two)
  * the lane-group kernel with that many lanes per row, 128 the streaming
- * (LDS-staged) kernel, -1..-64 the solver's L x: rows of 1..4 entries from
+ * (LDS-staged) kernel, 256 one thread per row over a sliced copy (64-row
+ * slices, entries interleaved), -1..-64 the solver's L x: rows of 1..4 entries from
  * a padded 4-wide table, rows of 5..64 by that many lanes per row, longer
  * rows in 2048-entry chunks, rows without entries 0.  Every choice sums
  * each row in a fixed order. */

@@ -952,6 +952,7 @@ lt_status Solver::lens_build_matrix() {
     hipLaunchKernelGGL(k_rowptr_search, dim3(blocks_for(ntot + 1, 256)), dim3(256), 0, stream(), ntot, nz, urow.p,
                        Mpc.rp.p);
     LT_CHECK_LAUNCH();
+    Mpc.long_row = 64;  // rows beyond 64 entries summed in chunks
     LT_TRY(csr_prepare(Mpc));
     LT_TRY(csr_autotune(Mpc, false));
     LT_TRY(csr_set_samples(Mpc, soff_host.data(), S));

@@ -138,7 +138,8 @@ lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *rp, const i
     LT_TRY(ensure_init());
     const int kl = kernel < 0 ? -kernel : kernel;
     const bool pow2 = kl > 0 && kl <= 64 && (kl & (kl - 1)) == 0;
-    if (nrows < 0 || ncols < 0 || !rp || !y || (ncols > 0 && !x) || !(kernel == 0 || pow2 || kernel == kLanesStream)) {
+    if (nrows < 0 || ncols < 0 || !rp || !y || (ncols > 0 && !x) ||
+        !(kernel == 0 || pow2 || kernel == kLanesStream || kernel == kLanesSlice)) {
         set_error("lt_csr_spmv_host: bad arguments");
         return LT_ERR_ARG;
     }
@@ -160,6 +161,7 @@ lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *rp, const i
         LT_TRY(dy.zero());
         LT_TRY(csr_spmv_nz(A, dx.p, dy.p));
     } else {
+        if (kernel == kLanesSlice) LT_TRY(csr_make_slices(A));
         LT_TRY(csr_spmv(A, dx.p, dy.p, 1.0, 0.0, kernel));
     }
     LT_TRY(dy.download(y, nrows));

@@ -124,7 +124,7 @@ struct DevCSR {
     // in fixed-size chunks by whole workgroups, then combined in order
     int long_row = 2048;
     bool prepared = false;
-    int lanes = 1;  // lane-group width of k_spmv, or kLanesStream for the streaming kernel
+    int lanes = 1;  // lane-group width of k_spmv, kLanesStream (streaming kernel) or kLanesSlice (sliced copy)
     long nblk = 0;        // streaming-kernel workgroups (ceil(nnz / its nonzero range))
     DevBuf<int32_t> rblk;  // first row of each streaming workgroup, nblk + 1
     long nlong = 0, nchunk = 0;
@@ -147,9 +147,19 @@ struct DevCSR {
     int ns = 0;
     DevBuf<int64_t> srow, sell, snzr;
     long srow_max = 0, sell_max = 0, snzr_max = 0;
+    // sliced copy (csr_make_slices): slice q = rows 64q .. 64q+63, entry j of
+    // its row t at sl_ptr[q] + 64 j + t (slice width = its longest row that
+    // is not left to the chunked path), for one thread per row with
+    // coalesced loads
+    long nslice = 0;
+    DevBuf<int64_t> sl_ptr;
+    DevBuf<int32_t> sl_ci;
+    DevBuf<double> sl_va;
+    DevBuf<float> sl_va32;
 };
 
 constexpr int kLanesStream = 128;  // pseudo lane count: the streaming (LDS-staged) SpMV kernel
+constexpr int kLanesSlice = 256;   // pseudo lane count: one thread per row over the sliced copy
 
 // analyse row lengths once (lane group size, long-row chunk plan)
 lt_status csr_prepare(DevCSR &A);
@@ -182,6 +192,8 @@ lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *ac
                       long rows_per_sample = 0, const int *alist = nullptr, int nact = 0);
 lt_status csr_spmv32_nz(const DevCSR &A, const float *x, float *y, const int *act = nullptr,
                         long rows_per_sample = 0, const int *alist = nullptr, int nact = 0);
+// the sliced copy of A (after csr_prepare; FP32 values too when present)
+lt_status csr_make_slices(DevCSR &A);
 // sample partition of A's rows (row_off: S + 1 host offsets, ascending);
 // after csr_nonempty_rows when that is used
 lt_status csr_set_samples(DevCSR &A, const int64_t *row_off, int S);

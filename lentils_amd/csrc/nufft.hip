@@ -957,9 +957,13 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, in
     for (int k = 0; k < ns; ++k) {
         const Cplx<T> *grid = spec + (long)smp[k] * sstride;
         for (int i = threadIdx.x; i < WS * WS; i += blockDim.x) {
-            const int wa = i / WS, wb = i - wa * WS;
-            win[k * WS * WS + i] = BAND ? fetch_band(G, grid, u0 + wa - G.cu, v0 + wb - G.cv, nbv)
-                                        : fetch_folded(G, grid, u0 + wa - G.cu, v0 + wb - G.cv);
+            if (BAND) {  // the band is stored [v][u]: consecutive threads take consecutive u
+                const int wb = i / WS, wa = i - wb * WS;
+                win[k * WS * WS + wa * WS + wb] = fetch_band(G, grid, u0 + wa - G.cu, v0 + wb - G.cv, nbv);
+            } else {
+                const int wa = i / WS, wb = i - wa * WS;
+                win[k * WS * WS + i] = fetch_folded(G, grid, u0 + wa - G.cu, v0 + wb - G.cv);
+            }
         }
     }
     __syncthreads();

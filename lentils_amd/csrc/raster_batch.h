@@ -38,6 +38,7 @@ struct LensBatch {
             LT_TRY(csr_prepare(Lp));
             LT_TRY(csr_nonempty_rows(Lp));
             LT_TRY(csr_transpose(Lp, LT));
+            LT.long_row = 64;  // one thread or lane group per row up to 64 entries, chunks beyond
             LT_TRY(csr_prepare(LT));
         }
         return LT_OK;

@@ -154,6 +154,75 @@ __global__ void __launch_bounds__(256) k_spmv_ell4_sg(const int *__restrict__ al
     ell4_row<T>(i, erow[i], eci, eva, x, y);
 }
 
+// One thread per row over the sliced copy: entry j of the row's slice is
+// read by the 64 rows of the slice together (coalesced); the row's terms
+// are added in order.
+template <class T>
+__device__ inline void slice_row(long row, const int32_t *__restrict__ rp, const int64_t *__restrict__ sl_ptr,
+                                 const int32_t *__restrict__ sci, const T *__restrict__ sva, const T *__restrict__ x,
+                                 T *__restrict__ y, T alpha, T beta, int skip_long) {
+    const int len = rp[row + 1] - rp[row];
+    if (skip_long > 0 && len > skip_long) return;  // summed by k_spmv_chunks / k_spmv_long
+    const long base = sl_ptr[row >> 6] + (row & 63);
+    T s = (T)0;
+    for (int j = 0; j < len; ++j) s += sva[base + 64L * j] * x[sci[base + 64L * j]];
+    y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_slice(long nrows, const int32_t *__restrict__ rp,
+                                                     const int64_t *__restrict__ sl_ptr,
+                                                     const int32_t *__restrict__ sci, const T *__restrict__ sva,
+                                                     const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+                                                     int skip_long, RowFilter F) {
+    const long row = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (row >= nrows || F.skip(row)) return;
+    slice_row<T>(row, rp, sl_ptr, sci, sva, x, y, alpha, beta, skip_long);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_slice_sg(const int *__restrict__ alist, const int *__restrict__ act,
+                                                        const int64_t *__restrict__ srow,
+                                                        const int32_t *__restrict__ rp,
+                                                        const int64_t *__restrict__ sl_ptr,
+                                                        const int32_t *__restrict__ sci, const T *__restrict__ sva,
+                                                        const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+                                                        int skip_long) {
+    const int s = alist[blockIdx.y];
+    if (act && act[s] == 0) return;
+    const long row = srow[s] + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= srow[s + 1]) return;
+    slice_row<T>(row, rp, sl_ptr, sci, sva, x, y, alpha, beta, skip_long);
+}
+
+// slice widths: 64 x the longest row of each slice (rows above long_row
+// excluded when the chunked path sums them)
+__global__ void k_slice_width(long nslice, long nrows, const int32_t *__restrict__ rp, int skip_long,
+                              int64_t *__restrict__ width) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nslice) return;
+    int w = 0;
+    for (long r = q * 64; r < min(nrows, q * 64 + 64); ++r) {
+        const int len = rp[r + 1] - rp[r];
+        if (!(skip_long > 0 && len > skip_long)) w = max(w, len);
+    }
+    width[q] = 64L * w;
+}
+
+__global__ void k_slice_fill(long nrows, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                             const double *__restrict__ va, const int64_t *__restrict__ sl_ptr, int skip_long,
+                             int32_t *__restrict__ sci, double *__restrict__ sva) {
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const int b = rp[row], len = rp[row + 1] - b;
+    if (skip_long > 0 && len > skip_long) return;
+    const long base = sl_ptr[row >> 6] + (row & 63);
+    for (int j = 0; j < len; ++j) {
+        sci[base + 64L * j] = ci[b + j];
+        sva[base + 64L * j] = va[b + j];
+    }
+}
+
 // first position of a sorted list at or above each key
 __global__ void k_lower_bound(int nkey, const int64_t *__restrict__ key, const int32_t *__restrict__ list, long n,
                               int64_t *__restrict__ out) {
@@ -540,12 +609,26 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
                 }));
             }
         }
+    } else if (lanes == kLanesSlice && A.nslice > 0) {
+        const T *sva = std::is_same<T, float>::value ? (const T *)A.sl_va32.p : (const T *)A.sl_va.p;
+        if (sg) {
+            if (A.srow_max > 0) {
+                hipLaunchKernelGGL((k_spmv_slice_sg<T>), dim3(blocks_for(A.srow_max, 256), nact), dim3(256), 0,
+                                   stream(), alist, act, A.srow.p, A.rp.p, A.sl_ptr.p, A.sl_ci.p, sva, x, y, alpha,
+                                   beta_y, skip);
+                LT_CHECK_LAUNCH();
+            }
+        } else {
+            hipLaunchKernelGGL((k_spmv_slice<T>), dim3(blocks_for(A.nrows, 256)), dim3(256), 0, stream(), A.nrows,
+                               A.rp.p, A.sl_ptr.p, A.sl_ci.p, sva, x, y, alpha, beta_y, skip, F);
+            LT_CHECK_LAUNCH();
+        }
     } else if (lanes == kLanesStream && A.prepared && A.nblk > 0) {
         hipLaunchKernelGGL((k_spmv_stream<T>), dim3((unsigned)A.nblk), dim3(256), 0, stream(), A.nnz, A.rblk.p,
                            A.rp.p, A.ci.p, va, x, y, alpha, beta_y, skip, F);
         LT_CHECK_LAUNCH();
     } else {
-        if (lanes == kLanesStream) lanes = csr_pick_lanes(A);
+        if (lanes == kLanesStream || lanes == kLanesSlice) lanes = csr_pick_lanes(A);
         if (sg) {
             if (A.srow_max > 0)
                 LT_TRY(with_lanes(lanes, [&](auto LC) {
@@ -573,6 +656,36 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
                            A.long_rows.p, A.long_chunk0.p, chunk_sum, y, alpha, beta_y, F);
         LT_CHECK_LAUNCH();
     }
+    return LT_OK;
+}
+
+lt_status csr_make_slices(DevCSR &A) {
+    if (A.nslice > 0 || A.nrows == 0 || A.nnz == 0) return LT_OK;
+    if (!A.prepared) LT_TRY(csr_prepare(A));
+    const long nsl = (A.nrows + 63) / 64;
+    const int skip = A.nlong > 0 ? A.long_row : 0;
+    DevBuf<int64_t> width;
+    LT_TRY(width.alloc(nsl + 1));
+    LT_TRY(A.sl_ptr.alloc(nsl + 1));
+    hipLaunchKernelGGL(k_slice_width, dim3(blocks_for(nsl, 256)), dim3(256), 0, stream(), nsl, A.nrows, A.rp.p, skip,
+                       width.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(width.p, A.sl_ptr.p, nsl));
+    int64_t tot = 0;
+    LT_TRY(A.sl_ptr.download(&tot, 1, nsl));
+    LT_TRY(A.sl_ci.alloc(std::max<long>(tot, 1)));
+    LT_TRY(A.sl_va.alloc(std::max<long>(tot, 1)));
+    hipLaunchKernelGGL(k_slice_fill, dim3(blocks_for(A.nrows, 256)), dim3(256), 0, stream(), A.nrows, A.rp.p, A.ci.p,
+                       A.va.p, A.sl_ptr.p, skip, A.sl_ci.p, A.sl_va.p);
+    LT_CHECK_LAUNCH();
+    if (A.va32.p) {
+        LT_TRY(A.sl_va32.alloc(std::max<long>(tot, 1)));
+        hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(tot, 256)), dim3(256), 0, stream(), tot, A.sl_va.p,
+                           A.sl_va32.p);
+        LT_CHECK_LAUNCH();
+    }
+    LT_TRY(sync());
+    A.nslice = nsl;
     return LT_OK;
 }
 
@@ -663,6 +776,15 @@ lt_status csr_nonempty_rows(DevCSR &A) {
 }
 
 lt_status csr_make_f32(DevCSR &A) {
+    if (A.nslice > 0 && !A.sl_va32.p) {
+        int64_t tot = 0;
+        LT_TRY(A.sl_ptr.download(&tot, 1, A.nslice));
+        LT_TRY(sync());
+        LT_TRY(A.sl_va32.alloc(std::max<long>(tot, 1)));
+        hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(tot, 256)), dim3(256), 0, stream(), tot, A.sl_va.p,
+                           A.sl_va32.p);
+        LT_CHECK_LAUNCH();
+    }
     if (A.nell > 0 && !A.ell_va32.p) {
         LT_TRY(A.ell_va32.alloc(4 * A.nell));
         hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(4 * A.nell, 256)), dim3(256), 0, stream(), 4 * A.nell,
@@ -701,8 +823,9 @@ static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, bool nz, int &
     LT_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int best_l = A.lanes;
-    for (int L = 1; L <= kLanesStream; L <<= 1) {
-        if (L == kLanesStream && (A.nblk == 0 || nz)) break;
+    for (int L = 1; L <= kLanesSlice; L <<= 1) {
+        if (L == kLanesStream && (A.nblk == 0 || nz)) continue;
+        if (L == kLanesSlice && (A.nslice == 0 || nz)) continue;
         LT_TRY(spmv_t<T>(A, va, chunk_sum, x.p, y.p, (T)1, (T)0, L, nullptr, 0, nullptr, nz));  // warm-up
         LT_HIP(hipEventRecord(e0, stream()));
         for (int r = 0; r < 3; ++r)
@@ -742,6 +865,9 @@ std::map<TuneKey, int> g_tune;
 lt_status csr_autotune(DevCSR &A, bool fp32) {
     if (A.nrows == 0 || A.nnz == 0) return LT_OK;
     if (!A.prepared) LT_TRY(csr_prepare(A));
+    // the sliced copy is a candidate for matrices of several entries per row
+    // (not for those multiplied over their nonempty rows)
+    if (A.nnzr < 0 && (double)A.nnz >= 4.0 * (double)A.nrows) LT_TRY(csr_make_slices(A));
     if (fp32) LT_TRY(csr_make_f32(A));
     int dev = 0;
     LT_HIP(hipGetDevice(&dev));

@@ -1,9 +1,10 @@
 """GPU SpMV kernels against scipy: every lane-group width, the streaming
-(LDS-staged) kernel and the product over the list of nonempty rows, on
-matrices with the row patterns of the lens matrices and their transposes
-(empty rows, trailing empty rows, rows that run across the streaming
-kernel's 2048-nonzero ranges, rows long enough for the chunked path).  Every kernel sums a row in a fixed order, so
-repeated products are bit-identical."""
+(LDS-staged) kernel, the sliced copy and the product over the list of
+nonempty rows, on matrices with the row patterns of the lens matrices and
+their transposes (empty rows, trailing empty rows, rows that run across the
+streaming kernel's 2048-nonzero ranges, rows long enough for the chunked
+path).  Every kernel sums a row in a fixed order, so repeated products are
+bit-identical."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -12,8 +13,8 @@ from helpers import rng
 
 pytestmark = pytest.mark.gpu
 
-# 0 default, 1..64 lanes per row, 128 streaming, -1..-64 nonempty-row list
-KERNELS = [0, 1, 2, 4, 8, 16, 32, 64, 128, -1, -4, -16, -64]
+# 0 default, 1..64 lanes per row, 128 streaming, 256 sliced copy, -1..-64 nonempty-row list
+KERNELS = [0, 1, 2, 4, 8, 16, 32, 64, 128, 256, -1, -4, -16, -64]
 
 
 def _matrix(g, nrows, ncols, lengths):
@@ -69,7 +70,7 @@ def test_spmv_kernels_match_scipy(hip, case):
         assert np.all(np.isfinite(y)), (name, k)  # every row written
         assert np.max(np.abs(y - ref)) <= 1e-13 * scale, (name, k)
         got[k] = y
-    for k in (1, 128, -4):
+    for k in (1, 128, 256, -4):
         y = np.full(nrows, np.nan)
         B.check(B.lib().lt_csr_spmv_host_kernel(nrows, ncols, B.ptr(rp), B.ptr(ci), B.ptr(va), B.ptr(x),
                                                  B.ptr(y), k))
