@@ -171,7 +171,7 @@ __global__ void k_long_count_capped(BatchGeom G, const SrcGrid *__restrict__ src
     const long p = r - (long)s * G.npix;
     const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
     CappedCount cc;
-    const int st = raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cc);
+    const int st = raster_pixel<false>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cc);
     if (!cc.stop() && st) atomicOr(&status[s], st);
     cnt[l] = cc.n;
 }
@@ -291,7 +291,7 @@ __global__ void k_long_sub_count(const int64_t *__restrict__ rows, long npix, co
     }
     const SrcGrid g = src[(int)(rows[l] / npix)];
     CountSink cs;
-    if (raster_dfs(it.tri, it.depth, g, cs)) atomicOr(&fallback[l], 1);
+    if (raster_dfs<false>(it.tri, it.depth, g, cs)) atomicOr(&fallback[l], 1);
     cnt[gid] = cs.n;
 }
 
@@ -331,7 +331,7 @@ __global__ void k_long_rowtot(BatchGeom G, const SrcGrid *__restrict__ src, cons
         const long p = r - (long)s * G.npix;
         const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
         CountSink cs;
-        const int st = raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cs);
+        const int st = raster_pixel<false>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cs);
         if (st) atomicOr(&status[s], st);
         n = cs.n;
     }
@@ -364,7 +364,7 @@ __global__ void k_long_sub_emit(const int64_t *__restrict__ rows, const int32_t 
         return;
     }
     const SrcGrid g = src[(int)(rows[l] / npix)];
-    raster_dfs(it.tri, it.depth, g, gf);
+    raster_dfs<false>(it.tri, it.depth, g, gf);
 }
 
 // sequential emission of the long rows flagged for it (medium rows and huge
@@ -379,7 +379,7 @@ __global__ void k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src, const 
     const long p = r - (long)s * G.npix;
     const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
     GlobalFrags gf{area, key, foff[l]};
-    raster_pixel(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], gf);
+    raster_pixel<false>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], gf);
 }
 
 // Reduction of the sorted long rows (raster.c:375-391) in parallel over

@@ -74,6 +74,45 @@ __host__ __device__ inline const CutTables &cut_tables() {
 #endif
 }
 
+// The tables packed, per fixed (cut, child, vertex) position, into one
+// integer over the 8 masks, so that a lookup by a per-thread mask is a shift
+// of an immediate (no memory access).
+constexpr CutTables kCutTablesCx = LT_CUT_TABLES_INIT;
+constexpr unsigned cut_pack_edge(int k, int j) {
+    unsigned r = 0;
+    for (int m = 0; m < 8; ++m) r |= (unsigned)kCutTablesCx.edge[m][k][j] << (2 * m);
+    return r;
+}
+constexpr unsigned cut_pack_vert(int c, int v) {
+    unsigned r = 0;
+    for (int m = 0; m < 8; ++m) r |= (unsigned)kCutTablesCx.vert[m][c][v] << (3 * m);
+    return r;
+}
+constexpr unsigned cut_pack_side(int c) {
+    unsigned r = 0;
+    for (int m = 0; m < 8; ++m) r |= (unsigned)kCutTablesCx.side[m][c] << m;
+    return r;
+}
+constexpr unsigned cut_pack_ncut() {
+    unsigned r = 0;
+    for (int m = 0; m < 8; ++m) r |= (unsigned)kCutTablesCx.ncut[m] << (2 * m);
+    return r;
+}
+constexpr unsigned cut_pack_nchild() {
+    unsigned r = 0;
+    for (int m = 0; m < 8; ++m) r |= (unsigned)kCutTablesCx.nchild[m] << (2 * m);
+    return r;
+}
+
+// a[s] for s in 0..N-1 without indexing the array by a per-thread value
+template <int N>
+LT_HD double pick(const double (&a)[N], int s) {
+    double r = a[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) r = s == i ? a[i] : r;
+    return r;
+}
+
 // Root node of one fan triangle (raster.c:144-170): its three source-plane
 // vertices, barycentric corners (0,0), (0,1), (1,0) and the index range of
 // the source cells it touches.
@@ -100,9 +139,108 @@ LT_HD void raster_root(const double *tv, const SrcGrid &g, RTri &tri) {
 // What the reference does with a node it pops (raster.c:180-268).  A node of
 // extent 1 is a leaf: returns -1 and sets emit (inside the grid) with the
 // fragment's area and source index.  Otherwise the node is cut at the middle
-// grid line of its larger extent (ties -> x) and the pieces that overlap the
-// grid are written to kids[0..m-1] in the order the reference pushes them;
-// returns m (the reference processes them last to first).
+// grid line of its larger extent (ties -> x); the pieces that overlap the
+// grid, in the order the reference pushes them, are the kids[c] with ok[c]
+// set; returns their number (the reference processes them last to first).
+// Every table lookup and intermediate point stays in registers (lookups are
+// shifts of packed immediates, the cut points are picked by selects): same
+// arithmetic as raster_expand below, fewer memory round trips (for walks of
+// many short rows side by side) at the price of more instructions and
+// registers (raster_expand serves the long serial walks).
+LT_HD int expand_node(const RTri &tri, const SrcGrid &g, RTri (&kids)[3], bool (&ok)[3], double &area, int &idx,
+                      bool &emit) {
+    int ax = 0, ext = 0;
+    for (int a = 0; a < 2; ++a) {
+        int s = tri.hi[a] - tri.lo[a];
+        if (s > ext) {
+            ext = s;
+            ax = a;
+        }
+    }
+    const int lo_ax = ax ? tri.lo[1] : tri.lo[0], hi_ax = ax ? tri.hi[1] : tri.hi[0];
+    const int cut = lo_ax + ext / 2;
+    const double cell = ax ? g.dy : g.dx;
+    const double org = ax ? g.ymin : g.xmin;
+    const double cutc = cut * cell + org;
+    ok[0] = ok[1] = ok[2] = false;
+    if (ext == 1) {
+        const int i = tri.lo[0], j = tri.lo[1];
+        emit = !(i < 0 || i >= g.nx || j < 0 || j >= g.ny);
+        if (emit) {
+            const double x1 = tri.b[0][0], y1 = tri.b[0][1];
+            const double x2 = tri.b[1][0], y2 = tri.b[1][1];
+            const double x3 = tri.b[2][0], y3 = tri.b[2][1];
+            area = -0.25 * (x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2));
+            idx = g.ny * i + j;
+        }
+        return -1;
+    }
+    double px[5], py[5], bx[5], by[5], dist[3];
+    int mask = 0;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        px[v] = tri.v[v][0];
+        py[v] = tri.v[v][1];
+        bx[v] = tri.b[v][0];
+        by[v] = tri.b[v][1];
+        dist[v] = (ax ? py[v] : px[v]) - cutc;
+        mask |= ((dist[v] > 0.) << v);
+    }
+    px[3] = px[4] = py[3] = py[4] = bx[3] = bx[4] = by[3] = by[4] = 0.0;
+    const int ncut = (int)((cut_pack_ncut() >> (2 * mask)) & 3u);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (k < ncut) {
+            const int e0 = (int)((cut_pack_edge(k, 0) >> (2 * mask)) & 3u);
+            const int e1 = (int)((cut_pack_edge(k, 1) >> (2 * mask)) & 3u);
+            const double w0 = pick(dist, e1), w1 = -pick(dist, e0);
+            const double wn = 1.0 / (w0 + w1);
+            const double a0 = w0 * wn, a1 = w1 * wn;
+            px[3 + k] = a0 * pick(px, e0) + a1 * pick(px, e1);
+            py[3 + k] = a0 * pick(py, e0) + a1 * pick(py, e1);
+            bx[3 + k] = a0 * pick(bx, e0) + a1 * pick(bx, e1);
+            by[3 + k] = a0 * pick(by, e0) + a1 * pick(by, e1);
+        }
+    }
+    const int nchild = (int)((cut_pack_nchild() >> (2 * mask)) & 3u);
+    int m = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= nchild) continue;
+        const int s = (int)((cut_pack_side(c) >> mask) & 1u);
+        int lo0 = tri.lo[0], lo1 = tri.lo[1], hi0 = tri.hi[0], hi1 = tri.hi[1];
+        const int lo_c = s ? cut : lo_ax, hi_c = s ? hi_ax : cut;
+        if (ax) {
+            lo1 = lo_c;
+            hi1 = hi_c;
+        } else {
+            lo0 = lo_c;
+            hi0 = hi_c;
+        }
+        if (hi0 <= 0 || lo0 >= g.nx) continue;
+        if (hi1 <= 0 || lo1 >= g.ny) continue;
+        RTri &k = kids[c];
+        k.lo[0] = lo0;
+        k.lo[1] = lo1;
+        k.hi[0] = hi0;
+        k.hi[1] = hi1;
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const int src = (int)((cut_pack_vert(c, v) >> (3 * mask)) & 7u);
+            k.v[v][0] = pick(px, src);
+            k.v[v][1] = pick(py, src);
+            k.b[v][0] = pick(bx, src);
+            k.b[v][1] = pick(by, src);
+        }
+        ok[c] = true;
+        ++m;
+    }
+    return m;
+}
+
+// What the reference does with a node it pops, with the tables in constant
+// memory and the points in a local array, writing the children straight to
+// kids[0..m-1] (onto the walk's stack in raster_dfs<false>).
 LT_HD int raster_expand(const RTri &tri, const SrcGrid &g, RTri *kids, double &area, int &idx, bool &emit) {
     const CutTables &T = cut_tables();
     int ax = 0, ext = 0;
@@ -180,39 +318,84 @@ LT_HD int raster_expand(const RTri &tri, const SrcGrid &g, RTri *kids, double &a
 // stop() turns true ends the walk early (count passes that only need to know
 // that a row is long).  Returns LT_RASTER_* status bits: the reference gives
 // up on the triangle when a push fills its stack (raster.c:262-266).
-template <class Sink>
+// REG (many short walks side by side): expand_node in registers and the
+// last child kept as the current node instead of pushed and popped; else
+// (long serial walks, where registers cost occupancy) raster_expand onto the
+// stack.  Same nodes in the same order either way.
+template <bool REG = true, class Sink>
 __host__ __device__ int raster_dfs(const RTri &root, int base, const SrcGrid &g, Sink &sink) {
-    RTri stk[kRasterStack + 3];
-    int ns = 0, pops = 0;
-    stk[ns++] = root;
-    while (ns > 0) {
-        if (++pops > kRasterPopLimit) return 4;  // LT_RASTER_POP_LIMIT
-        const RTri tri = stk[--ns];
-        double area = 0.0;
-        int idx = 0;
-        bool emit = false;
-        const int m = raster_expand(tri, g, &stk[ns], area, idx, emit);
-        if (m < 0) {
-            if (emit) {
-                sink(area, idx);
-                if (sink.stop()) return 0;
+    if constexpr (!REG) {
+        RTri stk[kRasterStack + 3];
+        int ns = 0, pops = 0;
+        stk[ns++] = root;
+        while (ns > 0) {
+            if (++pops > kRasterPopLimit) return 4;  // LT_RASTER_POP_LIMIT
+            const RTri tri = stk[--ns];
+            double area = 0.0;
+            int idx = 0;
+            bool emit = false;
+            const int m = raster_expand(tri, g, &stk[ns], area, idx, emit);
+            if (m < 0) {
+                if (emit) {
+                    sink(area, idx);
+                    if (sink.stop()) return 0;
+                }
+                continue;
             }
-            continue;
+            // the reference pushes one piece at a time and stops as soon as its
+            // stack holds kRasterStack entries
+            if (m > 0 && ns + base + m >= kRasterStack) return 1;  // LT_RASTER_STACK_OVERFLOW
+            ns += m;
         }
-        // the reference pushes one piece at a time and stops as soon as its
-        // stack holds kRasterStack entries
-        if (m > 0 && ns + base + m >= kRasterStack) return 1;  // LT_RASTER_STACK_OVERFLOW
-        ns += m;
+        return 0;
+    } else {
+        // the reference pushes every child and pops the last one next; here the
+        // last child is kept as the current node and only the others go to the
+        // stack (same order, same stack depths)
+        RTri stk[kRasterStack + 3];
+        int ns = 0, pops = 0;
+        RTri cur = root;
+        for (;;) {
+            if (++pops > kRasterPopLimit) return 4;  // LT_RASTER_POP_LIMIT
+            RTri kids[3];
+            bool ok[3];
+            double area = 0.0;
+            int idx = 0;
+            bool emit = false;
+            const int m = expand_node(cur, g, kids, ok, area, idx, emit);
+            if (m < 0) {
+                if (emit) {
+                    sink(area, idx);
+                    if (sink.stop()) return 0;
+                }
+            } else if (m > 0) {
+                // the reference pushes one piece at a time and stops as soon as
+                // its stack holds kRasterStack entries
+                if (ns + base + m >= kRasterStack) return 1;  // LT_RASTER_STACK_OVERFLOW
+                int left = m;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    if (!ok[c]) continue;
+                    if (--left == 0)
+                        cur = kids[c];
+                    else
+                        stk[ns++] = kids[c];
+                }
+                continue;
+            }
+            if (ns == 0) break;
+            cur = stk[--ns];
+        }
+        return 0;
     }
-    return 0;
 }
 
 // Rasterize one triangle (vertices tv[0..5]).
-template <class Sink>
+template <bool REG = true, class Sink>
 __host__ __device__ int raster_triangle(const double *tv, const SrcGrid &g, Sink &sink) {
     RTri tri;
     raster_root(tv, g, tri);
-    return raster_dfs(tri, 0, g, sink);
+    return raster_dfs<REG>(tri, 0, g, sink);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,7 +483,7 @@ LT_HD void stencil_offsets(int k, int &ox, int &oy) {
 // All fragments of image pixel (i, j) from the deflected lattice (x-major,
 // ly = 2 ny + 1 points per column): 8 fan triangles
 // (p[t+1], p[t], p[8]).  raster.c:351-363
-template <class Sink>
+template <bool REG = true, class Sink>
 __host__ __device__ int raster_pixel(const double *lat, int ly, int i, int j, const SrcGrid &g, Sink &sink) {
     double sp[9][2];
     for (int k = 0; k < 9; ++k) {
@@ -314,7 +497,7 @@ __host__ __device__ int raster_pixel(const double *lat, int ly, int i, int j, co
     for (int t = 0; t < 8 && !sink.stop(); ++t) {
         const int t1 = (t + 1) & 7;
         const double tv[6] = {sp[t1][0], sp[t1][1], sp[t][0], sp[t][1], sp[8][0], sp[8][1]};
-        st |= raster_triangle(tv, g, sink);
+        st |= raster_triangle<REG>(tv, g, sink);
     }
     return st;
 }
