@@ -1172,6 +1172,11 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     }
     const bool learned = (precond == LT_PC_LEARNED);
     const bool flexible = learned || precond == LT_PC_LENS;
+    // plain CG: z = r, so r . z is the r . r just formed; the iterations use
+    // r and rr in place of z and rz_new (the same values, two launches fewer)
+    const bool plain = (precond == LT_PC_NONE);
+    const double *zz = plain ? r.p : z.p;
+    const double *rzn = plain ? rr.p : rz_new.p;
     if (precond == LT_PC_LENS) LT_TRY(lens_setup());
     if (flexible && zap.n < (size_t)S) LT_TRY(zap.alloc(S));  // z . A p of the flexible beta
     cur_act = nullptr;
@@ -1235,17 +1240,19 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
         LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
-        LT_TRY(precondition(precond, dinv_p, list, nl));
-        LT_TRY(seg_dot(r.p, z.p, rz_new.p));
+        if (!plain) {
+            LT_TRY(precondition(precond, dinv_p, list, nl));
+            LT_TRY(seg_dot(r.p, z.p, rz_new.p));
+        }
         if (flexible) {
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));
             hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
                                alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
         } else {
-            hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, rr.p, bb.p,
-                               rtol2, beta.p, iters.p);
+            hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rzn, rr.p, bb.p, rtol2,
+                               beta.p, iters.p);
         }
-        hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, z.p, p.p);
+        hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, zz, p.p);
         LT_CHECK_LAUNCH();
         return LT_OK;
     };
@@ -1316,17 +1323,19 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
         LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
-        LT_TRY(precondition(precond, dinv_p, dlist.p, nact));
-        LT_TRY(seg_dot(r.p, z.p, rz_new.p));
+        if (!plain) {
+            LT_TRY(precondition(precond, dinv_p, dlist.p, nact));
+            LT_TRY(seg_dot(r.p, z.p, rz_new.p));
+        }
         if (flexible) {
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));
             hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
                                alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
         } else {
-            hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, rr.p, bb.p,
-                               rtol2, beta.p, iters.p);
+            hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rzn, rr.p, bb.p, rtol2,
+                               beta.p, iters.p);
         }
-        hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, z.p, p.p);
+        hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, zz, p.p);
         LT_CHECK_LAUNCH();
         ++outer;
         if (mixed && refine_every > 0 && outer % refine_every == 0) {
