@@ -93,6 +93,39 @@ __global__ void __launch_bounds__(kDotOne) k_seg_dot_one(const int64_t *__restri
     }
 }
 
+// The same dot followed by the CG scalar step of its sample (plain CG, one
+// workgroup per sample): step 1, after p . Ap: alpha = rz / pAp (as
+// k_cg_alpha); step 2, after r . r: beta = rr / rz, rz = rr, iteration count
+// and convergence flag (as k_cg_beta with rz_new = rr).
+__global__ void __launch_bounds__(kDotOne) k_seg_dot_cg(const int64_t *__restrict__ off,
+                                                         const double *__restrict__ a,
+                                                         const double *__restrict__ b, double *__restrict__ out,
+                                                         int step, int *__restrict__ act, double *__restrict__ rz,
+                                                         double *__restrict__ ab, const double *__restrict__ bb,
+                                                         double rtol2, int *__restrict__ iters) {
+    __shared__ double sh[kDotOne / 64];
+    const int s = blockIdx.x;
+    const long lo = off[s], hi = off[s + 1];
+    double acc = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) acc += a[i] * b[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double t = 0.0;
+    for (int k = 0; k < kDotOne / 64; ++k) t += sh[k];
+    out[s] = t;
+    if (step == 1) {
+        ab[s] = (act[s] && t != 0.0) ? rz[s] / t : 0.0;
+    } else if (act[s]) {
+        iters[s] += 1;
+        ab[s] = rz[s] != 0.0 ? t / rz[s] : 0.0;
+        rz[s] = t;
+        if (!(t > rtol2 * bb[s])) act[s] = 0;  // also stops on NaN
+    }
+}
+
 __global__ void k_seg_dot_fin(int S, int P, const double *__restrict__ part, double *__restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
@@ -1177,6 +1210,9 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     const bool plain = (precond == LT_PC_NONE);
     const double *zz = plain ? r.p : z.p;
     const double *rzn = plain ? rr.p : rz_new.p;
+    // plain CG with one workgroup per sample's dot: alpha and the beta step
+    // are formed by the dot kernels themselves (two launches fewer)
+    const bool fused = plain && dot_parts == 1;
     if (precond == LT_PC_LENS) LT_TRY(lens_setup());
     if (flexible && zap.n < (size_t)S) LT_TRY(zap.alloc(S));  // z . A p of the flexible beta
     cur_act = nullptr;
@@ -1235,10 +1271,19 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     // that are not active ride along with zero steps)
     auto iter_body = [&](const int *list, int nl) -> lt_status {
         LT_TRY(apply(p.p, Ap.p, list, nl, act.p));
-        LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
-        hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
+        if (fused) {
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, p.p, Ap.p, pAp.p, 1, act.p,
+                               rz.p, alpha.p, bb.p, rtol2, iters.p);
+        } else {
+            LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
+            hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
+        }
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
-        LT_TRY(seg_dot(r.p, r.p, rr.p));
+        if (fused)
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, r.p, r.p, rr.p, 2, act.p,
+                               rz.p, beta.p, bb.p, rtol2, iters.p);
+        else
+            LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
         if (!plain) {
             LT_TRY(precondition(precond, dinv_p, list, nl));
@@ -1248,7 +1293,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));
             hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
                                alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
-        } else {
+        } else if (!fused) {
             hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rzn, rr.p, bb.p, rtol2,
                                beta.p, iters.p);
         }
@@ -1318,10 +1363,19 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             LT_TRY(apply32(p.p, Ap.p, dlist.p, nact, act.p));
         else
             LT_TRY(apply(p.p, Ap.p, dlist.p, nact, act.p));
-        LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
-        hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
+        if (fused) {
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, p.p, Ap.p, pAp.p, 1, act.p,
+                               rz.p, alpha.p, bb.p, rtol2, iters.p);
+        } else {
+            LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
+            hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
+        }
         hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
-        LT_TRY(seg_dot(r.p, r.p, rr.p));
+        if (fused)
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, r.p, r.p, rr.p, 2, act.p,
+                               rz.p, beta.p, bb.p, rtol2, iters.p);
+        else
+            LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
         if (!plain) {
             LT_TRY(precondition(precond, dinv_p, dlist.p, nact));
@@ -1331,7 +1385,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));
             hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
                                alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
-        } else {
+        } else if (!fused) {
             hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rzn, rr.p, bb.p, rtol2,
                                beta.p, iters.p);
         }
