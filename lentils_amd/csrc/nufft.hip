@@ -2087,11 +2087,11 @@ const Cplx<float> *twiddles_of<float>(const NufftPlan &P) {
     return (const Cplx<float> *)P.twd32.p;
 }
 
-// sequence pairs per workgroup: up to 32 KB of LDS, at most 4
+// sequence pairs per workgroup: up to 16 KB of LDS, at most 4
 template <class T, int N>
 constexpr int rows_pairs() {
     constexpr int per = N * 2 * (int)sizeof(T);
-    return per >= 32768 ? 1 : (32768 / per > 4 ? 4 : 32768 / per);
+    return per >= 16384 ? 1 : (16384 / per > 4 ? 4 : 16384 / per);
 }
 
 // writes the image rows of B; its padding rows stay as they are (zero)
@@ -2291,7 +2291,7 @@ lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, 
         St.nb_sel = cand;
         LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));  // warm-up
         LT_HIP(hipEventRecord(e0, stream()));
-        for (int r = 0; r < 2; ++r) LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));
+        for (int r = 0; r < 6; ++r) LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));
         LT_HIP(hipEventRecord(e1, stream()));
         LT_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
