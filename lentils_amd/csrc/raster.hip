@@ -271,10 +271,13 @@ __global__ void __launch_bounds__(kExpandThreads) k_long_expand(BatchGeom G, con
 // walked depth first from the stack depth the reference has at its root.
 // A subtree that overflows that stack or hits the pop limit flags its row
 // for the sequential path.
-__global__ void k_long_sub_count(const int64_t *__restrict__ rows, long npix, const SrcGrid *__restrict__ src,
-                                 const RItem *__restrict__ front, const int32_t *__restrict__ nfront, int nb,
-                                 const int32_t *__restrict__ order, int *__restrict__ fallback,
-                                 int64_t *__restrict__ cnt) {
+__global__ void __launch_bounds__(kRasterThreads) k_long_sub_count(const int64_t *__restrict__ rows, long npix,
+                                                                   const SrcGrid *__restrict__ src,
+                                                                   const RItem *__restrict__ front,
+                                                                   const int32_t *__restrict__ nfront, int nb,
+                                                                   const int32_t *__restrict__ order,
+                                                                   int *__restrict__ fallback,
+                                                                   int64_t *__restrict__ cnt) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * kLongFrontier) return;
     const long gid = order[t];  // items in decreasing size: similar walks share a wave
@@ -291,7 +294,7 @@ __global__ void k_long_sub_count(const int64_t *__restrict__ rows, long npix, co
     }
     const SrcGrid g = src[(int)(rows[l] / npix)];
     CountSink cs;
-    if (raster_dfs<false>(it.tri, it.depth, g, cs)) atomicOr(&fallback[l], 1);
+    if (raster_dfs<true>(it.tri, it.depth, g, cs)) atomicOr(&fallback[l], 1);
     cnt[gid] = cs.n;
 }
 
@@ -344,12 +347,17 @@ __global__ void k_long_rowtot(BatchGeom G, const SrcGrid *__restrict__ src, cons
 // Huge rows, step 3 (emit): every frontier item writes its fragments at its
 // row's staging offset (foff, indexed by long-row number hmap[h]) + its
 // emission position
-__global__ void k_long_sub_emit(const int64_t *__restrict__ rows, const int32_t *__restrict__ hmap, long npix,
-                                const SrcGrid *__restrict__ src, const RItem *__restrict__ front,
-                                const int32_t *__restrict__ nfront, int nb, const int32_t *__restrict__ order,
-                                const int *__restrict__ fallback, const int64_t *__restrict__ pos,
-                                const int64_t *__restrict__ foff, double *__restrict__ area,
-                                uint64_t *__restrict__ key) {
+__global__ void __launch_bounds__(kRasterThreads) k_long_sub_emit(const int64_t *__restrict__ rows,
+                                                                  const int32_t *__restrict__ hmap, long npix,
+                                                                  const SrcGrid *__restrict__ src,
+                                                                  const RItem *__restrict__ front,
+                                                                  const int32_t *__restrict__ nfront, int nb,
+                                                                  const int32_t *__restrict__ order,
+                                                                  const int *__restrict__ fallback,
+                                                                  const int64_t *__restrict__ pos,
+                                                                  const int64_t *__restrict__ foff,
+                                                                  double *__restrict__ area,
+                                                                  uint64_t *__restrict__ key) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * kLongFrontier) return;
     const long gid = order[t];
@@ -364,7 +372,7 @@ __global__ void k_long_sub_emit(const int64_t *__restrict__ rows, const int32_t 
         return;
     }
     const SrcGrid g = src[(int)(rows[l] / npix)];
-    raster_dfs<false>(it.tri, it.depth, g, gf);
+    raster_dfs<true>(it.tri, it.depth, g, gf);
 }
 
 // sequential emission of the long rows flagged for it (medium rows and huge
@@ -667,7 +675,8 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
                                                  stream()));
                 LT_TRY(sync());
             }
-            hipLaunchKernelGGL(k_long_sub_count, dim3(blocks_for(nitems, 256)), dim3(256), 0, stream(), dhrow.p,
+            hipLaunchKernelGGL(k_long_sub_count, dim3(blocks_for(nitems, kRasterThreads)), dim3(kRasterThreads), 0,
+                               stream(), dhrow.p,
                                npix, src.p, front.p, nfront.p, nb, order.p, hfb.p, cnt.p);
             LT_CHECK_LAUNCH();
             LT_TRY(exclusive_scan_i64(cnt.p, pos.p, nitems));
@@ -699,7 +708,8 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
         LT_TRY(fkey.alloc(F));
         LT_TRY(fkey2.alloc(F));
         if (nhuge) {
-            hipLaunchKernelGGL(k_long_sub_emit, dim3(blocks_for(nitems, 256)), dim3(256), 0, stream(), dhrow.p,
+            hipLaunchKernelGGL(k_long_sub_emit, dim3(blocks_for(nitems, kRasterThreads)), dim3(kRasterThreads), 0,
+                               stream(), dhrow.p,
                                dhmap.p, npix, src.p, front.p, nfront.p, nb, order.p, hfb.p, pos.p, dfoff.p, farea.p,
                                fkey.p);
             LT_CHECK_LAUNCH();

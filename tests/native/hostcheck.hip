@@ -114,7 +114,7 @@ long hc_pixel_frags(const double *lat, int ny, int i, int j, int snx, int sny, d
             for (const RItem &it : cur) {
                 if (it.kind == 1) {
                     s(it.area, it.idx);
-                } else if (raster_dfs<false>(it.tri, it.depth, g, s)) {
+                } else if (raster_dfs<true>(it.tri, it.depth, g, s)) {
                     return -1;
                 }
             }
