@@ -161,9 +161,10 @@ struct CappedCount {
     __device__ bool stop() const { return n > kMedium; }
 };
 
-__global__ void k_long_count_capped(BatchGeom G, const SrcGrid *__restrict__ src, const double *__restrict__ lat,
-                                    const int64_t *__restrict__ rows, int nlong, int64_t *__restrict__ cnt,
-                                    int *__restrict__ status) {
+__global__ void __launch_bounds__(64) k_long_count_capped(BatchGeom G, const SrcGrid *__restrict__ src,
+                                                          const double *__restrict__ lat,
+                                                          const int64_t *__restrict__ rows, int nlong,
+                                                          int64_t *__restrict__ cnt, int *__restrict__ status) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlong) return;
     const long r = rows[l];
@@ -171,7 +172,7 @@ __global__ void k_long_count_capped(BatchGeom G, const SrcGrid *__restrict__ src
     const long p = r - (long)s * G.npix;
     const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
     CappedCount cc;
-    const int st = raster_pixel<false>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cc);
+    const int st = raster_pixel<true>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cc);
     if (!cc.stop() && st) atomicOr(&status[s], st);
     cnt[l] = cc.n;
 }
@@ -377,9 +378,11 @@ __global__ void __launch_bounds__(kRasterThreads) k_long_sub_emit(const int64_t 
 
 // sequential emission of the long rows flagged for it (medium rows and huge
 // rows the frontier path gave back)
-__global__ void k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src, const double *__restrict__ lat,
-                            const int64_t *__restrict__ rows, const int64_t *__restrict__ foff, int nlong,
-                            const int *__restrict__ seq, double *__restrict__ area, uint64_t *__restrict__ key) {
+__global__ void __launch_bounds__(64) k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src,
+                                                  const double *__restrict__ lat, const int64_t *__restrict__ rows,
+                                                  const int64_t *__restrict__ foff, int nlong,
+                                                  const int *__restrict__ seq, double *__restrict__ area,
+                                                  uint64_t *__restrict__ key) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlong || !seq[l]) return;
     const long r = rows[l];
@@ -387,7 +390,7 @@ __global__ void k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src, const 
     const long p = r - (long)s * G.npix;
     const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
     GlobalFrags gf{area, key, foff[l]};
-    raster_pixel<false>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], gf);
+    raster_pixel<true>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], gf);
 }
 
 // Reduction of the sorted long rows (raster.c:375-391) in parallel over
