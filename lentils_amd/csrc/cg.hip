@@ -501,6 +501,19 @@ __global__ void k_divide(long n, const double *__restrict__ r, const double *__r
     if (i < n) z[i] = r[i] / dM[i];
 }
 
+// inner CG start: x = 0, r = the outer residual, z = p = r / dM
+__global__ void k_inner_start(long n, const double *__restrict__ r0, const double *__restrict__ dM,
+                              double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+                              double *__restrict__ p) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ri = r0[i], zi = ri / dM[i];
+    x[i] = 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+}
+
 // inner CG update: x += a_s p; r -= a_s Mp; z = r / dM
 __global__ void k_inner_xr(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
                            const double *__restrict__ pMp, const double *__restrict__ p, const double *__restrict__ Mp,
@@ -985,7 +998,6 @@ lt_status Solver::lens_build_matrix() {
     hipLaunchKernelGGL(k_rowptr_search, dim3(blocks_for(ntot + 1, 256)), dim3(256), 0, stream(), ntot, nz, urow.p,
                        Mpc.rp.p);
     LT_CHECK_LAUNCH();
-    Mpc.long_row = 64;  // rows beyond 64 entries summed in chunks
     LT_TRY(csr_prepare(Mpc));
     LT_TRY(csr_autotune(Mpc, false));
     LT_TRY(csr_set_samples(Mpc, soff_host.data(), S));
@@ -996,32 +1008,31 @@ lt_status Solver::lens_build_matrix() {
 // M = lam_c Cc^T Cc + c L_s^T L_s from z = 0, every sample at once
 lt_status Solver::lens_apply(const int *af, const int *alist, int nact) {
     const unsigned gb = blocks_for(ntot, 256);
-    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(256), 0, stream(), ntot, 0.0, ix.p);  // a kernel: graph-safe
-    hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, r.p, ir.p);
-    hipLaunchKernelGGL(k_divide, dim3(gb), dim3(256), 0, stream(), ntot, ir.p, dM.p, iz.p);
-    hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, iz.p, ip.p);
-    LT_TRY(seg_dot(ir.p, iz.p, irz.p));
+    // the inner iterate is z itself (no final copy); with no inner steps z is
+    // the Jacobi step r / dM
+    double *xi = lens_inner > 0 ? z.p : ix.p;
+    hipLaunchKernelGGL(k_inner_start, dim3(gb), dim3(256), 0, stream(), ntot, r.p, dM.p, xi, ir.p, iz.p, ip.p);
+    double *rzc = irz.p, *rzn = irz2.p;  // r . z of the current and the next inner step (swapped, not copied)
+    LT_TRY(seg_dot(ir.p, iz.p, rzc));
     LT_CHECK_LAUNCH();
     for (int k = 0; k < lens_inner; ++k) {
         // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc
         LT_TRY(csr_spmv(Mpc, ip.p, iMp.p, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
         LT_TRY(seg_dot(ip.p, iMp.p, ipMp.p));
-        hipLaunchKernelGGL(k_inner_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, irz.p, ipMp.p, ip.p, iMp.p,
-                           dM.p, ix.p, ir.p, iz.p);
+        hipLaunchKernelGGL(k_inner_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rzc, ipMp.p, ip.p, iMp.p,
+                           dM.p, xi, ir.p, iz.p);
         LT_CHECK_LAUNCH();
         if (k + 1 < lens_inner) {
-            LT_TRY(seg_dot(ir.p, iz.p, irz2.p));
-            hipLaunchKernelGGL(k_inner_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, irz.p, irz2.p, iz.p, ip.p);
-            hipLaunchKernelGGL(k_copy, dim3(blocks_for(S, 64)), dim3(64), 0, stream(), (long)S, irz2.p, irz.p);
+            LT_TRY(seg_dot(ir.p, iz.p, rzn));
+            hipLaunchKernelGGL(k_inner_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rzc, rzn, iz.p, ip.p);
             LT_CHECK_LAUNCH();
+            std::swap(rzc, rzn);
         }
     }
     if (lens_inner == 0) {
         hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, iz.p, z.p);
-    } else {
-        hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, ix.p, z.p);
+        LT_CHECK_LAUNCH();
     }
-    LT_CHECK_LAUNCH();
     return LT_OK;
 }
 
