@@ -2,6 +2,7 @@
 // reference's signatures, and the lt_* deflection / lens-matrix entry points.
 #include <cstdarg>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 #include "lt_internal.h"
@@ -60,6 +61,61 @@ lt_status ensure_init() {
 
 hipStream_t stream() { return g_stream; }
 int device_cu_count() { return g_cus; }
+
+// The pool: hipMalloc / hipFree cost tens of microseconds each and hipFree
+// synchronizes the device, while a lens batch and its solver allocate a few
+// hundred buffers per step, of sizes that recur from step to step (up to
+// the nonzero counts: hence size classes of a quarter octave).  All device
+// work of the library runs on its one stream, so a block's earlier users
+// are ordered before its next ones.  Blocks beyond kPoolCap pooled bytes
+// are freed.
+namespace {
+std::mutex g_pool_mu;
+std::map<std::pair<int, size_t>, std::vector<void *>> g_pool;
+size_t g_pool_bytes = 0;
+constexpr size_t kPoolCap = size_t(64) << 30;
+
+size_t pool_class(size_t bytes) {
+    if (bytes <= 4096) return (bytes + 255) / 256 * 256;
+    int k = 63 - __builtin_clzll((unsigned long long)bytes);
+    const size_t step = (size_t)1 << (k - 2);
+    return (bytes + step - 1) / step * step;
+}
+}  // namespace
+
+hipError_t pool_get(void **p, size_t bytes) {
+    const size_t cls = pool_class(bytes);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_pool.find({dev, cls});
+        if (it != g_pool.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            g_pool_bytes -= cls;
+            return hipSuccess;
+        }
+    }
+    return hipMalloc(p, cls);
+}
+
+void pool_put(void *p, size_t bytes) {
+    const size_t cls = pool_class(bytes);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipFree(p);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (g_pool_bytes + cls > kPoolCap) {
+        (void)hipFree(p);
+        return;
+    }
+    g_pool[{dev, cls}].push_back(p);
+    g_pool_bytes += cls;
+}
 
 lt_status build_curvature(int nx, int ny, double dx, double dy, int32_t coloff, DevCSR &C);
 lt_status build_gradient(int nx, int ny, double dx, double dy, DevCSR &G);

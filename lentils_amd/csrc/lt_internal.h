@@ -41,6 +41,11 @@ lt_status ensure_init();
 hipStream_t stream();
 int device_cu_count();
 
+// Device memory pool behind DevBuf (capi.hip): freed blocks are kept per
+// device and size class and handed to later allocations of the same class.
+hipError_t pool_get(void **p, size_t bytes);
+void pool_put(void *p, size_t bytes);
+
 // owning device buffer
 template <class T>
 struct DevBuf {
@@ -56,7 +61,7 @@ struct DevBuf {
     }
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) pool_put(p, n * sizeof(T));
         p = nullptr;
         n = 0;
     }
@@ -64,7 +69,7 @@ struct DevBuf {
         if (count == n && p) return LT_OK;
         release();
         if (count == 0) return LT_OK;
-        LT_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        LT_HIP(pool_get((void **)&p, count * sizeof(T)));
         n = count;
         return LT_OK;
     }
