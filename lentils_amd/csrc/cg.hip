@@ -167,11 +167,6 @@ __global__ void k_copy(long n, const double *__restrict__ x, double *__restrict_
     if (i < n) y[i] = x[i];
 }
 
-__global__ void k_mul(long n, const double *__restrict__ m, double *__restrict__ x) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] *= m[i];
-}
-
 // z = M^{-1} r (Jacobi) or z = r
 __global__ void k_precond(long n, const double *__restrict__ dinv, const double *__restrict__ r,
                           double *__restrict__ z) {
@@ -493,12 +488,6 @@ __global__ void k_csr_diag(long n, const int32_t *__restrict__ rp, const int32_t
     for (int k = rp[r]; k < rp[r + 1]; ++k)
         if (ci[k] == r) v = va[k];
     d[r] = v > 0.0 ? v : 1.0;
-}
-
-// z = r / dM
-__global__ void k_divide(long n, const double *__restrict__ r, const double *__restrict__ dM, double *__restrict__ z) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) z[i] = r[i] / dM[i];
 }
 
 // inner CG start: x = 0, r = the outer residual, z = p = r / dM
