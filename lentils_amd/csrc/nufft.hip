@@ -1331,7 +1331,8 @@ __global__ void __launch_bounds__(256) k_rows_inv(int nx, int ny, int pady, int 
             const int v = q / PAIRS, pr = q - v * PAIRS;
             const int row = row0 + 2 * pr + pass;
             Cplx<T> c{(T)0, (T)0};
-            if (v < nbv && row < nx) c = B[(long)v * nu + padx + row];
+            // rows not needed are taken as zero (not read)
+            if (v < nbv && row < nx && (!rm || rm[row])) c = B[(long)v * nu + padx + row];
             if (v == 0 || v == H) c.im = (T)0;
             // value at v and its conjugate at N - v
             if (pass == 0) {
@@ -2285,25 +2286,29 @@ lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, 
     hipEvent_t e0, e1;
     LT_HIP(hipEventCreate(&e0));
     LT_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    int best_nb = 4;
-    for (int cand : {1, 2, 4, 8}) {
-        St.nb_sel = cand;
-        LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));  // warm-up
-        LT_HIP(hipEventRecord(e0, stream()));
-        for (int r = 0; r < 6; ++r) LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));
-        LT_HIP(hipEventRecord(e1, stream()));
-        LT_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        LT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            best_nb = cand;
+    // three rounds over the candidates, each candidate's best round kept (so
+    // that clocks settling during the first timings favour none of them)
+    const int cands[4] = {1, 2, 4, 8};
+    float tmin[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    for (int round = 0; round < 3; ++round)
+        for (int c = 0; c < 4; ++c) {
+            St.nb_sel = cands[c];
+            LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));  // warm-up
+            LT_HIP(hipEventRecord(e0, stream()));
+            for (int r = 0; r < 3; ++r)
+                LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));
+            LT_HIP(hipEventRecord(e1, stream()));
+            LT_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            LT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            tmin[c] = std::min(tmin[c], ms);
         }
-    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    St.nb_sel = best_nb;
+    int best = 0;
+    for (int c = 1; c < 4; ++c)
+        if (tmin[c] < tmin[best]) best = c;
+    St.nb_sel = cands[best];
     return LT_OK;
 }
 
