@@ -94,17 +94,17 @@ __global__ void __launch_bounds__(kDotOne) k_seg_dot_one(const int64_t *__restri
 }
 
 // The same dot followed by the CG scalar step of its sample (plain CG, one
-// workgroup per sample): step 1, after p . Ap: alpha = rz / pAp (as
+// workgroup per sample, or per listed sample): step 1, after p . Ap: alpha = rz / pAp (as
 // k_cg_alpha); step 2, after r . r: beta = rr / rz, rz = rr, iteration count
 // and convergence flag (as k_cg_beta with rz_new = rr).
-__global__ void __launch_bounds__(kDotOne) k_seg_dot_cg(const int64_t *__restrict__ off,
+__global__ void __launch_bounds__(kDotOne) k_seg_dot_cg(const int64_t *__restrict__ off, const int *__restrict__ list,
                                                          const double *__restrict__ a,
                                                          const double *__restrict__ b, double *__restrict__ out,
                                                          int step, int *__restrict__ act, double *__restrict__ rz,
                                                          double *__restrict__ ab, const double *__restrict__ bb,
                                                          double rtol2, int *__restrict__ iters) {
     __shared__ double sh[kDotOne / 64];
-    const int s = blockIdx.x;
+    const int s = list ? list[blockIdx.x] : (int)blockIdx.x;  // with a list: its samples only
     const long lo = off[s], hi = off[s + 1];
     double acc = 0.0;
     for (long i = lo + threadIdx.x; i < hi; i += kDotOne) acc += a[i] * b[i];
@@ -185,6 +185,28 @@ __global__ void k_cg_xr(long n, const int32_t *__restrict__ seg, const int *__re
     const double a = alpha[s];
     x[i] += a * p[i];
     r[i] -= a * Ap[i];
+}
+
+// the same two updates over the listed samples only (grid y: the list)
+__global__ void k_cg_xr_sg(const int *__restrict__ list, const int64_t *__restrict__ off, const int *__restrict__ act,
+                           const double *__restrict__ alpha, const double *__restrict__ p,
+                           const double *__restrict__ Ap, double *__restrict__ x, double *__restrict__ r) {
+    const int s = list[blockIdx.y];
+    if (!act[s]) return;
+    const long i = off[s] + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= off[s + 1]) return;
+    const double a = alpha[s];
+    x[i] += a * p[i];
+    r[i] -= a * Ap[i];
+}
+
+__global__ void k_cg_p_sg(const int *__restrict__ list, const int64_t *__restrict__ off, const int *__restrict__ act,
+                          const double *__restrict__ beta, const double *__restrict__ z, double *__restrict__ p) {
+    const int s = list[blockIdx.y];
+    if (!act[s]) return;
+    const long i = off[s] + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= off[s + 1]) return;
+    p[i] = z[i] + beta[s] * p[i];
 }
 
 // p = z + beta_s p on active samples
@@ -1213,6 +1235,9 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     // plain CG with one workgroup per sample's dot: alpha and the beta step
     // are formed by the dot kernels themselves (two launches fewer)
     const bool fused = plain && dot_parts == 1;
+    long smax = 0;  // the largest source, for launches over listed samples
+    for (int q = 0; q < S; ++q) smax = std::max<long>(smax, soff_host[q + 1] - soff_host[q]);
+    const unsigned gsmax = blocks_for(std::max<long>(smax, 1), 256);
     if (precond == LT_PC_LENS) LT_TRY(lens_setup());
     if (flexible && zap.n < (size_t)S) LT_TRY(zap.alloc(S));  // z . A p of the flexible beta
     cur_act = nullptr;
@@ -1271,17 +1296,26 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     // that are not active ride along with zero steps)
     auto iter_body = [&](const int *list, int nl) -> lt_status {
         LT_TRY(apply(p.p, Ap.p, list, nl, act.p));
+        // with fewer listed samples than samples, the updates and dots run
+        // over the list only (same values)
+        const bool part = nl < S;
+        const int *dl = (fused && part) ? list : nullptr;
         if (fused) {
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, p.p, Ap.p, pAp.p, 1, act.p,
-                               rz.p, alpha.p, bb.p, rtol2, iters.p);
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nl : S), dim3(kDotOne), 0, stream(), soff.p, dl, p.p, Ap.p,
+                               pAp.p, 1, act.p, rz.p, alpha.p, bb.p, rtol2, iters.p);
         } else {
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
             hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         }
-        hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
+        if (part)
+            hipLaunchKernelGGL(k_cg_xr_sg, dim3(gsmax, nl), dim3(256), 0, stream(), list, soff.p, act.p, alpha.p,
+                               p.p, Ap.p, x.p, r.p);
+        else
+            hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p,
+                               r.p);
         if (fused)
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, r.p, r.p, rr.p, 2, act.p,
-                               rz.p, beta.p, bb.p, rtol2, iters.p);
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nl : S), dim3(kDotOne), 0, stream(), soff.p, dl, r.p, r.p,
+                               rr.p, 2, act.p, rz.p, beta.p, bb.p, rtol2, iters.p);
         else
             LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
@@ -1297,7 +1331,15 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rzn, rr.p, bb.p, rtol2,
                                beta.p, iters.p);
         }
-        hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, zz, p.p);
+        if (part)
+            hipLaunchKernelGGL(k_cg_p_sg, dim3(gsmax, nl), dim3(256), 0, stream(), list, soff.p, act.p, beta.p, zz,
+                               p.p);
+        else
+            if (part)
+            hipLaunchKernelGGL(k_cg_p_sg, dim3(gsmax, nact), dim3(256), 0, stream(), dlist.p, soff.p, act.p, beta.p,
+                               zz, p.p);
+        else
+            hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, zz, p.p);
         LT_CHECK_LAUNCH();
         return LT_OK;
     };
@@ -1363,17 +1405,26 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             LT_TRY(apply32(p.p, Ap.p, dlist.p, nact, act.p));
         else
             LT_TRY(apply(p.p, Ap.p, dlist.p, nact, act.p));
+        // with fewer listed samples than samples, the updates and dots run
+        // over the list only (same values)
+        const bool part = nact < S;
+        const int *dl = (fused && part) ? dlist.p : nullptr;
         if (fused) {
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, p.p, Ap.p, pAp.p, 1, act.p,
-                               rz.p, alpha.p, bb.p, rtol2, iters.p);
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nact : S), dim3(kDotOne), 0, stream(), soff.p, dl, p.p, Ap.p,
+                               pAp.p, 1, act.p, rz.p, alpha.p, bb.p, rtol2, iters.p);
         } else {
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
             hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         }
-        hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p, r.p);
+        if (part)
+            hipLaunchKernelGGL(k_cg_xr_sg, dim3(gsmax, nact), dim3(256), 0, stream(), dlist.p, soff.p, act.p, alpha.p,
+                               p.p, Ap.p, x.p, r.p);
+        else
+            hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p,
+                               r.p);
         if (fused)
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(S), dim3(kDotOne), 0, stream(), soff.p, r.p, r.p, rr.p, 2, act.p,
-                               rz.p, beta.p, bb.p, rtol2, iters.p);
+            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nact : S), dim3(kDotOne), 0, stream(), soff.p, dl, r.p, r.p,
+                               rr.p, 2, act.p, rz.p, beta.p, bb.p, rtol2, iters.p);
         else
             LT_TRY(seg_dot(r.p, r.p, rr.p));
         LT_CHECK_LAUNCH();
