@@ -154,6 +154,7 @@ struct NufftPlan {
     DevBuf<double> twd64;
     DevBuf<float> twd32;
     bool fused_rows = true;  // use them where nyp allows
+    bool wave_rows = true;   // FP32 with nyp = 1024: the one-wave transform (k_rows_fwd_w / k_rows_inv_w)
     int force_nb = 0;  // stencil samples per workgroup when > 0 (else the autotuned one)
     lt_status band_fft(int n, int precision, int nbv, BandFFT *&out);
     DevBuf<uint8_t> realB, specB, specB2, partialB;

@@ -680,9 +680,11 @@ lt_status csr_make_slices(DevCSR &A) {
     LT_CHECK_LAUNCH();
     if (A.va32.p) {
         LT_TRY(A.sl_va32.alloc(std::max<long>(tot, 1)));
-        hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(tot, 256)), dim3(256), 0, stream(), tot, A.sl_va.p,
-                           A.sl_va32.p);
-        LT_CHECK_LAUNCH();
+        if (tot > 0) {  // no entries when every row is long (chunked) or empty
+            hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(tot, 256)), dim3(256), 0, stream(), tot, A.sl_va.p,
+                               A.sl_va32.p);
+            LT_CHECK_LAUNCH();
+        }
     }
     LT_TRY(sync());
     A.nslice = nsl;
@@ -781,9 +783,11 @@ lt_status csr_make_f32(DevCSR &A) {
         LT_TRY(A.sl_ptr.download(&tot, 1, A.nslice));
         LT_TRY(sync());
         LT_TRY(A.sl_va32.alloc(std::max<long>(tot, 1)));
-        hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(tot, 256)), dim3(256), 0, stream(), tot, A.sl_va.p,
-                           A.sl_va32.p);
-        LT_CHECK_LAUNCH();
+        if (tot > 0) {  // no entries when every row is long (chunked) or empty
+            hipLaunchKernelGGL(k_d2f_sp, dim3(blocks_for(tot, 256)), dim3(256), 0, stream(), tot, A.sl_va.p,
+                               A.sl_va32.p);
+            LT_CHECK_LAUNCH();
+        }
     }
     if (A.nell > 0 && !A.ell_va32.p) {
         LT_TRY(A.ell_va32.alloc(4 * A.nell));

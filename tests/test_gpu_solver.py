@@ -63,6 +63,29 @@ def test_solver_operator_matches_oracle(hip, oracle, operator):
         assert O.max_relative_error(ys[k].reshape(-1), ref) < 1e-12
 
 
+def test_operator_rows_of_1024(hip, oracle):
+    """512^2 image (padded rows of 1024, the one-wave row transforms): the
+    FP64 operator against the oracle's, the FP32 operator against the FP64
+    one."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    im, src, uv, vs, models, data = _problem(n_img=512, nlens=2)
+    lb = LensBatch(im, [src] * len(models), models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    xs = [rng(76 + k).normal(size=src.size) for k in range(len(models))]
+    ys = inv.apply(xs)
+    y32 = inv.apply(xs, precision='f32')
+    for k in range(len(models)):
+        ref, _ = _oracle_apply(O, im, src, uv, lb.csr(k), xs[k], w, 1e2)
+        err = O.max_relative_error(ys[k].reshape(-1), ref)
+        err32 = O.max_relative_error(y32[k].reshape(-1), ys[k].reshape(-1))
+        print(f'sample {k}: FP64 vs oracle {err:.2e}, FP32 vs FP64 {err32:.2e}')
+        assert err < 1e-12
+        assert err32 < 1e-5
+
+
 def test_stencil_shapes_same_bits(hip, oracle):
     """Every samples-per-workgroup choice of the normal stencil gives the
     same operator bits; one of them against the oracle's operator."""
