@@ -238,6 +238,22 @@ lt_status lt_device_upload(void *dptr, const void *host, long nbytes);
 lt_status lt_device_download(const void *dptr, void *host, long nbytes);
 lt_status lt_device_free(void *dptr);
 
+/* Replaces RadioDataset.dirty_beam (lentils/common/data_base.py:163-172,
+ * beam space data_base.py:148-160): beam = N_b^T C 1 on the 2nx x 2ny beam
+ * space with bounds +-dx*n (the x pixel size on both axes), N_b the NUFFT
+ * with pad 2 and, for wsup <= 0 / kb_beta <= 0, the reference's defaults
+ * (kernel support 4, Beatty's beta, fourier_operators.py:98,108).  weights:
+ * nchannels*nrows float64 (sigma^-2 on unmasked visibilities, 0 elsewhere);
+ * beam: 4*nx*ny float64 (x-major); peak (may be NULL): its maximum, which
+ * equals the weight sum. */
+lt_status lt_dirty_beam_host(visibility_space visspace, image_space imspace, const double *weights, int wsup,
+                             double kb_beta, double *beam, double *peak);
+/* Replaces RadioDataset.blurred_covariance_operator applied to images
+ * (data_base.py:115-129): out_b = zpad^T irfft2(rfft2(roll(beam, (nx, ny)))
+ * / (4 nx ny) * rfft2(zpad img_b)) for nimg images of nx x ny (even sizes),
+ * beam from lt_dirty_beam_host.  Host pointers. */
+lt_status lt_bcb_apply_host(image_space imspace, const double *beam, int nimg, const double *img, double *out);
+
 /* No reference counterpart: the per-tap kernel polynomials the gridder
  * evaluates instead of kb_g (nufft.c:11-21).  Tap k of 2W+1 at fractional
  * offset f in [-1/2, 1/2] (distance f + W - k) has value
@@ -254,7 +270,9 @@ lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, do
  * kernel_support <= 4 through a grid-space stencil precomputed here from the
  * visibilities and weights, otherwise through a fused per-visibility
  * degrid-weight-grid pass (LT_OP_NUFFT_EXPLICIT forces the latter);
- * LT_OP_BCB uses the dirty-beam convolution (data_base.py:115-129). */
+ * LT_OP_BCB uses the dirty-beam convolution (data_base.py:115-129) with the
+ * beam of lt_dirty_beam_host at the reference's default NUFFT (kernel
+ * support 4, Beatty's beta), whatever the plan's own support. */
 #define LT_OP_NUFFT 0          /* exact D^T C D, precomputed grid stencil when possible */
 #define LT_OP_BCB 1            /* dirty-beam convolution */
 #define LT_OP_NUFFT_EXPLICIT 2 /* exact D^T C D, per-visibility degrid-weight-grid each apply */

@@ -11,5 +11,6 @@ from .operators import (Operator, CompositeOperatorProduct, CompositeOperatorSum
                         PriorCovarianceOperator, ManifoldLensOperator, FFTOperator, ZeroPaddingOperator,
                         ApodizationCorrectionOperator, GriddingOperator, NUFFTOperator, DFTOperator)
 from .solvers import LensBatch, NUFFTPlan, BatchedSourceInversion  # noqa: F401
+from .datasets import Dataset, RadioDataset, BlurredCovarianceOperator  # noqa: F401
 
 __version__ = '0.1.0'

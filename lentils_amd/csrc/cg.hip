@@ -633,6 +633,7 @@ struct Solver {
     // CG blocks of check_every iterations captured as HIP graphs, one per
     // launch width (active samples rounded up to a multiple of 8)
     bool use_graphs = true;
+    DevBuf<int> dlist;  // samples launched by the running solve (captured graphs read it)
     std::map<int, hipGraphExec_t> graphs;
     std::map<int, int> graph_warm;
     void drop_graphs() {
@@ -681,6 +682,9 @@ struct Solver {
 };
 
 fourier_space padded_fourier_space(const image_space &im, int pad);
+lt_status dirty_beam_dev(const visibility_space &vs, const image_space &im, const double *weights, int wsup,
+                         double beta, double *beam, double *peak);
+lt_status beam_multiplier(int nxp, int nyp, const double *beam, DevBuf<double> &spec);
 
 namespace {
 
@@ -1280,10 +1284,11 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     std::vector<int> hact(S, 1), alist(S);
     int nact = S;
     for (int s = 0; s < S; ++s) alist[s] = s;
-    // the device list of samples to launch; fixed capacity S so that its
-    // address never changes (captured graphs refer to it)
-    DevBuf<int> dlist;
-    LT_TRY(dlist.upload(alist.data(), S));
+    // the device list of samples to launch: a member of fixed capacity S,
+    // allocated once, so that its address never changes while a captured
+    // graph refers to it
+    if (dlist.n != (size_t)S) LT_TRY(dlist.alloc(S));
+    LT_HIP(hipMemcpyAsync(dlist.p, alist.data(), (size_t)S * sizeof(int), hipMemcpyHostToDevice, stream()));
     auto set_list = [&](const int *h, int n) -> lt_status {
         if (n > 0) LT_HIP(hipMemcpyAsync(dlist.p, h, (size_t)n * sizeof(int), hipMemcpyHostToDevice, stream()));
         return LT_OK;
@@ -1334,10 +1339,6 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         if (part)
             hipLaunchKernelGGL(k_cg_p_sg, dim3(gsmax, nl), dim3(256), 0, stream(), list, soff.p, act.p, beta.p, zz,
                                p.p);
-        else
-            if (part)
-            hipLaunchKernelGGL(k_cg_p_sg, dim3(gsmax, nact), dim3(256), 0, stream(), dlist.p, soff.p, act.p, beta.p,
-                               zz, p.p);
         else
             hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, zz, p.p);
         LT_CHECK_LAUNCH();
@@ -1526,6 +1527,9 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(seg_dot(r.p, z.p, rz.p));
     LT_CHECK_LAUNCH();
     }
+    // the captured graphs are tied to this solve's launch widths and
+    // buffers: none outlives it
+    drop_graphs();
     std::vector<int> hit(S);
     LT_TRY(iters.download(hit.data(), S));
     if (xout) LT_TRY(x.download(xout, ntot));
@@ -1541,106 +1545,26 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
 
 namespace {
 
-__global__ void k_roll(int nx, int ny, int sx, int sy, const double *__restrict__ in, double *__restrict__ out) {
-    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long)nx * ny) return;
-    const int i = (int)(g / ny), j = (int)(g % ny);
-    const int ti = (i + sx) % nx, tj = (j + sy) % ny;
-    out[(long)ti * ny + tj] = in[g];
-}
-
-__global__ void k_fill_c128(long n, double re, double im, double *__restrict__ v) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        v[2 * i] = re;
-        v[2 * i + 1] = im;
-    }
-}
-
-__global__ void k_scale(long n, double a, double *__restrict__ v) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) v[i] *= a;
-}
-
-// Dirty beam D_b^T C 1 on the (2nx, 2ny) beam space (data_base.py:149-172),
-// rolled by half the padded grid and transformed:
-// beamspec = rfft2(roll(beam)) / (nxp nyp)   (data_base.py:115-129).
-// The beam plan uses the same kernel support and beta as the main plan.
+// BCB multiplier of the reference's RadioDataset (data_base.py:115-129,
+// 163-172): the dirty beam N_b^T C 1 on the (2nx, 2ny) beam space with the
+// reference's default NUFFT (kernel support 4, Beatty's beta), rolled by half
+// the padded grid and transformed, rfft2(roll(beam)) / (nxp nyp).
 lt_status beam_spectrum(Solver &S, const double *weights) {
     NufftPlan &M = *S.nf;
-    if (M.prec != LT_F64) {
-        set_error("BCB mode needs an FP64 NUFFT plan");
-        return LT_ERR_UNSUPPORTED;
-    }
     if (M.nxp != 2 * M.nx || M.nyp != 2 * M.ny) {
         set_error("BCB mode needs the padded grid to be exactly twice the image (pad 2, even sizes)");
         return LT_ERR_UNSUPPORTED;
     }
-    const image_space &im = S.lb->im;
-    image_space bs{};
-    bs.nx = 2 * im.nx;
-    bs.ny = 2 * im.ny;
-    const double rx = im.dx * im.nx, ry = im.dx * im.ny;  // the reference uses dx for both axes
-    bs.xmin = -rx;
-    bs.xmax = rx;
-    bs.ymin = -ry;
-    bs.ymax = ry;
-    bs.dx = (bs.xmax - bs.xmin) / bs.nx;
-    bs.dy = (bs.ymax - bs.ymin) / bs.ny;
     visibility_space vs{};
     vs.nrows = M.nrows;
     vs.nchannels = M.nch;
     vs.nstokes = 1;
     vs.uv = M.uvw_host.data();
     vs.channels = M.chan_host.data();
-    NufftPlan B;
-    LT_TRY(B.create(vs, padded_fourier_space(bs, 2), M.W, M.beta, LT_F64));
-    LT_TRY(B.attach_image(bs, 2));
-    const long nv = B.nvis;
-    DevBuf<double> wB, beam, rolled;
-    {
-        std::vector<int32_t> hp(std::max<long>(nv, 1));
-        if (nv) LT_TRY(B.perm.download(hp.data(), nv));
-        LT_TRY(sync());
-        std::vector<double> hs(std::max<long>(nv, 1));
-        for (long q = 0; q < nv; ++q) hs[q] = weights[hp[q]];
-        LT_TRY(wB.upload(hs.data(), std::max<long>(nv, 1)));
-    }
-    const long nb = (long)bs.nx * bs.ny;
-    LT_TRY(beam.alloc(nb));
-    LT_TRY(rolled.alloc(nb));
-    if (nv) {
-        hipLaunchKernelGGL(k_fill_c128, dim3(blocks_for(nv, 256)), dim3(256), 0, stream(), nv, 1.0, 0.0,
-                           (double *)B.visbuf.p);
-        LT_CHECK_LAUNCH();
-        LT_TRY(B.adjoint(B.visbuf.p, wB.p, beam.p, 1.0, false));
-    } else {
-        LT_TRY(beam.zero());
-    }
-    hipLaunchKernelGGL(k_roll, dim3(blocks_for(nb, 256)), dim3(256), 0, stream(), bs.nx, bs.ny, M.nxp / 2, M.nyp / 2,
-                       beam.p, rolled.p);
-    LT_CHECK_LAUNCH();
-    // rfft2 on the main plan's transform (same nxp x nyp shape)
-    LT_HIP(hipMemcpyAsync(M.realbuf.p, rolled.p, nb * sizeof(double), hipMemcpyDeviceToDevice, stream()));
-    void *in[1] = {M.realbuf.p};
-    void *out[1] = {M.spec.p};
-    if (rocfft_execute(M.fwd, in, out, M.fwd_info) != rocfft_status_success) {
-        set_error("rocFFT execute (beam) failed");
-        return LT_ERR_FFT;
-    }
-    const long ns = 2L * M.nu * M.half_nv;
-    LT_TRY(S.beamspec.alloc(ns));
-    LT_HIP(hipMemcpyAsync(S.beamspec.p, M.spec.p, ns * sizeof(double), hipMemcpyDeviceToDevice, stream()));
-    hipLaunchKernelGGL(k_scale, dim3(blocks_for(ns, 256)), dim3(256), 0, stream(), ns, 1.0 / (double)nb,
-                       S.beamspec.p);
-    LT_CHECK_LAUNCH();
-    S.beam_peak = 0.0;
-    {
-        std::vector<double> hb(nb);
-        LT_TRY(beam.download(hb.data(), nb));
-        LT_TRY(sync());
-        for (double v : hb) S.beam_peak = std::max(S.beam_peak, v);
-    }
+    DevBuf<double> beam;
+    LT_TRY(beam.alloc(4L * M.nx * M.ny));
+    LT_TRY(dirty_beam_dev(vs, S.lb->im, weights, 0, 0.0, beam.p, &S.beam_peak));
+    LT_TRY(beam_multiplier(M.nxp, M.nyp, beam.p, S.beamspec));
     return sync();
 }
 

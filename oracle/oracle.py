@@ -325,6 +325,37 @@ def nufft_adjoint(vis, uvw, channels, geo, nthreads=1):
     return x * geo.apod
 
 
+def beam_grid(im):
+    """The dirty beam's image space of RadioDataset._make_beam_space
+    (data_base.py:148-160): twice the image in each direction, bounds
+    +-dx*n with the x pixel size for both axes."""
+    rx = im.dx * im.nx
+    ry = im.dx * im.ny
+    return Grid2D(2 * im.nx, 2 * im.ny, (-rx, rx, -ry, ry))
+
+
+def dirty_beam(uvw, channels, im, weights, wsup=4, beta=None, nthreads=1):
+    """RadioDataset.dirty_beam (data_base.py:163-172): the NUFFT adjoint
+    (pad 2; the reference's defaults W = 4 and Beatty's beta) of C 1 on the
+    beam space."""
+    bs = beam_grid(im)
+    geo = NUFFTGeometry(bs, pad=2, wsup=wsup, beta=beta)
+    c1 = np.asarray(weights, dtype=np.float64).reshape(1, -1).astype(np.complex128)
+    return nufft_adjoint(c1, uvw, channels, geo, nthreads=nthreads)
+
+
+def bcb_apply(img, beam, geo):
+    """RadioDataset.blurred_covariance_operator (data_base.py:115-129):
+    zpad^T fft^T diag(rfft2(roll(beam, (nxp/2, nyp/2))) / N) fft zpad, with
+    geo the main NUFFT's geometry (its padded grid must be the beam's)."""
+    db = np.roll(beam, [geo.nxp // 2, geo.nyp // 2], axis=[0, 1])
+    dbf = np.fft.rfft2(db) / db.size
+    p = np.zeros((geo.nxp, geo.nyp))
+    p[geo.padx:geo.padx + geo.im.nx, geo.pady:geo.pady + geo.im.ny] = np.asarray(img).reshape(geo.im.nx, geo.im.ny)
+    q = np.fft.irfft2(dbf * np.fft.rfft2(p), s=p.shape, norm='forward')
+    return q[geo.padx:geo.padx + geo.im.nx, geo.pady:geo.pady + geo.im.ny]
+
+
 def dft_matrix(uvw, channels, im):
     vs, keep = vis_struct(uvw, channels)
     nm = int(im.mask.sum())

@@ -177,3 +177,59 @@ def test_regularizer_and_dft_builders(hip, oracle):
     B.lib().grad_matrix_findiff_csr(_space(B, g), B.ptr(rp), B.ptr(ci), B.ptr(va))
     orp, oci, ova = O.grad_matrix(g)
     assert np.array_equal(rp, orp) and np.array_equal(ci[:rp[-1]], oci) and np.array_equal(va[:rp[-1]], ova)
+
+
+def test_drop_in_manifold_lens_matrix_csr(hip, oracle):
+    """The reference's symbol itself (raster.c:272-273, bound at
+    load_backend.py:122-126): caller-owned buffers sized by the reference's
+    50-per-row guess (lens_operators.py:30-34), trimmed through row_inds[-1];
+    structure identical to the oracle, values to the end-to-end bar; the
+    Python ManifoldLensOperator gives the same matrix and applies it."""
+    B, O = hip, oracle
+    from lentils_amd import GlobalLensModel, ImageSpace, ManifoldLensOperator, PowerLawEllipsoid, ExternalPotential
+    im = O.Grid2D(64, 64, [-1.28, 1.28, -1.28, 1.28])
+    src = O.Grid2D(32, 32, [-0.64, 0.64, -0.64, 0.64])
+    lenses = c1_lens(O)
+    glm, keep = _glm(B, lenses, O)
+    n = im.nx * im.ny
+    rp = np.zeros(n + 1, dtype=np.int32)
+    ci = np.zeros(50 * n, dtype=np.int32)
+    va = np.zeros(50 * n)
+    B.lib().manifold_lens_matrix_csr(_space(B, im), _space(B, src), glm, B.ptr(rp), B.ptr(ci), B.ptr(va))
+    orp, oci, ova, _ = O.lens_matrix_csr(im, src, lenses)
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(ci[:rp[-1]], oci)
+    _assert_rare_value_rows(rp, va[:rp[-1]], ova, n)
+    ims = ImageSpace(shape=(64, 64), bounds=[(-1.28, 1.28), (-1.28, 1.28)])
+    srs = ImageSpace(shape=(32, 32), bounds=[(-0.64, 0.64), (-0.64, 0.64)])
+    lm = GlobalLensModel([PowerLawEllipsoid(b=1.0, f=0.8, th=30.0, qh=0.5, x=0.02, y=-0.01, rc=0.0, z=0.5),
+                          ExternalPotential(ss=0.03, sa=45.0, x=0.02, y=-0.01, z=0.5)])
+    op = ManifoldLensOperator(ims, srs, lm, z_src=2.0)
+    M = op.matrix.tocsr()
+    assert np.array_equal(M.indptr, rp) and np.array_equal(M.indices, ci[:rp[-1]])
+    assert np.array_equal(M.data, va[:rp[-1]])
+    x = rng(44).normal(size=srs.shape)
+    y = op * x
+    assert O.max_relative_error(y.reshape(-1), M @ x.reshape(-1)) < 1e-13
+    yt = op.T * y
+    assert O.max_relative_error(yt.reshape(-1), M.T @ y.reshape(-1)) < 1e-13
+
+
+def test_prior_covariance_operators(hip, oracle):
+    """PriorCovarianceOperator (operator_base.py:200-251) on the GPU-built
+    finite-difference matrices: lambda R^T R against the oracle's matrices."""
+    O = oracle
+    import scipy.sparse as sp
+    from lentils_amd import ImageSpace, PriorCovarianceOperator
+    sp_ = ImageSpace(shape=(24, 20), bounds=[(-0.5, 0.7), (-0.4, 0.6)])
+    og = O.Grid2D(24, 20, sp_.bounds)
+    x = rng(45).normal(size=sp_.shape)
+    for kind, fn in (('curvature', O.curv_matrix), ('gradient', O.grad_matrix)):
+        op = PriorCovarianceOperator(sp_, type=kind, strength=3.5)
+        rp, ci, va = fn(og)
+        R = sp.csr_matrix((va, ci, rp), shape=(rp.size - 1, og.nx * og.ny))
+        ref = 3.5 * (R.T @ (R @ x.reshape(-1)))
+        assert O.max_relative_error((op * x).reshape(-1), ref) < 1e-12
+        assert op.T is op
+    tk = PriorCovarianceOperator(sp_, type='tikhonov', strength=2.0)
+    assert O.max_relative_error((tk * x).reshape(-1), 2.0 * x.reshape(-1)) < 1e-15

@@ -148,3 +148,23 @@ def test_nufft_adjointness_gpu(hip, oracle):
     rhs = np.dot(x, NTy)
     assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), abs(rhs))
     B.lib().lt_nufft_destroy(h)
+
+
+def test_fft_operator(hip, oracle):
+    """FFTOperator (fourier_operators.py:12-24) on rocFFT: forward equals
+    numpy's rfft2 (unnormalised), transpose numpy's irfft2 with
+    norm='forward', including spectra that are not Hermitian in columns 0
+    and ny/2 (numpy reads only their Hermitian part); odd and even sizes."""
+    O = oracle
+    from lentils_amd import FFTOperator, ImageSpace
+    for nx, ny in ((64, 48), (33, 50), (40, 31)):
+        im = ImageSpace(shape=(nx, ny), bounds=[(-1, 1), (-1, 1)])
+        op = FFTOperator(im)
+        x = rng(46).normal(size=(nx, ny))
+        X = op * x
+        ref = np.fft.rfft2(x, norm='backward')
+        assert O.max_relative_error(X.reshape(nx, ny // 2 + 1), ref) < 1e-13
+        g = rng(47).normal(size=(nx, ny // 2 + 1, 2)).view(np.complex128)[..., 0]
+        y = op.T * g
+        refT = np.fft.irfft2(g, s=(nx, ny), norm='forward')
+        assert O.max_relative_error(y.reshape(nx, ny), refT) < 1e-13

@@ -4,9 +4,7 @@
   oracle's composition of the same operators (max-relative 1e-12);
 * the precomputed grid stencil and the per-visibility explicit pass agree;
 * CG reaches the FP64 true relative residual it reports, and the oracle's
-  operator confirms it;
-* BCB (dirty-beam convolution) matches N^T C N to the reference's 1e-6
-  (nufft_tests.py:92-93)."""
+  operator confirms it.  The BCB operator's tests are in test_gpu_bcb.py."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -14,6 +12,11 @@ import scipy.sparse as sp
 from helpers import annulus_uv, rng
 
 pytestmark = pytest.mark.gpu
+
+# the oracle's FP64 operator must confirm the 1e-6 true relative residual the
+# GPU reports; the two operators agree to ~1e-12 of |A x|, which moves a
+# residual of 1e-6 |b| by ~1e-6 of itself
+RESID_BAR = 1e-6 * (1 + 1e-6)
 
 
 def _problem(nv=30000, n_img=64, n_src=32, nlens=3, seed=71):
@@ -123,7 +126,7 @@ def test_cg_converges_and_oracle_confirms(hip, oracle):
         x = xs[k].reshape(-1)
         Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x, w, 1e2)
         b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
-        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 2e-6
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= RESID_BAR
 
 
 def test_many_samples_cg(hip, oracle):
@@ -145,22 +148,7 @@ def test_many_samples_cg(hip, oracle):
         x = xs[k].reshape(-1)
         Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x, w, 1e2)
         b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
-        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 2e-6
-
-
-def test_bcb_matches_explicit(hip, oracle):
-    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
-    O = oracle
-    im, src, uv, vs, models, data = _problem(nlens=1, n_img=64)
-    lb = LensBatch(im, [src], models, z_src=2.0)
-    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
-    w = np.full(uv.shape[0], 1e6)
-    x = [rng(73).normal(size=src.size)]
-    y_exp = BatchedSourceInversion(lb, plan, data, weights=w, operator='nufft').apply(x)[0]
-    y_bcb = BatchedSourceInversion(lb, plan, data, weights=w, operator='bcb').apply(x)[0]
-    err = O.max_relative_error(y_bcb, y_exp)
-    print(f'BCB vs N^T C N: max-relative {err:.3e}')
-    assert err < 1e-2  # BCB omits the apodization correction and truncates the beam
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= RESID_BAR
 
 
 def test_per_sample_dirty_images(hip, oracle):
@@ -294,7 +282,7 @@ def test_precision_form(hip, oracle):
         x = x32[k].reshape(-1)
         Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x, w, 1e2)
         b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
-        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 2e-6
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= RESID_BAR
         d = np.linalg.norm(x - x64[k].reshape(-1)) / np.linalg.norm(x64[k])
         print(f'sample {k}: |x32 - x64| / |x64| = {d:.2e}')
 
@@ -320,7 +308,7 @@ def test_lens_preconditioner(hip, oracle):
         x = x1[k].reshape(-1)
         Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x, w, 1e2)
         b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
-        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) < 2e-6
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= RESID_BAR
     # the explicit M against scipy's c L_s^T L_s + lambda Cc^T Cc, for the
     # default row limit and for one that leaves rows out of L_s
     for max_row in (256, 4):

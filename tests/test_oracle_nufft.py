@@ -78,3 +78,67 @@ def test_curvature_matrix_is_laplacian(oracle):
     lap[:, 1:] += x[:, :-1]
     lap[:, :-1] += x[:, 1:]
     assert np.allclose(M @ x.reshape(-1), h * lap.reshape(-1))
+
+
+def test_gradient_matrix_is_forward_difference(oracle):
+    """grad_matrix_findiff_csr (nufft.c:163-194): x-differences then
+    y-differences, 1/dx and 1/dy, no entries on the last row / column; a
+    linear image has the exact constant gradient."""
+    O = oracle
+    im = O.Grid2D(6, 5, [-1.2, 0.9, -0.7, 0.8])
+    rp, ci, va = O.grad_matrix(im)
+    G = sp.csr_matrix((va, ci, rp), shape=(60, 30))
+    x = rng(4).normal(size=(6, 5))
+    gx = np.zeros((6, 5))
+    gy = np.zeros((6, 5))
+    gx[:-1] = (x[1:] - x[:-1]) / im.dx
+    gy[:, :-1] = (x[:, 1:] - x[:, :-1]) / im.dy
+    assert np.allclose(G @ x.reshape(-1), np.concatenate([gx.reshape(-1), gy.reshape(-1)]), rtol=0, atol=1e-12)
+    a, b = 0.7, -1.3
+    lin = a * im.dx * np.arange(6)[:, None] + b * im.dy * np.arange(5)[None, :]
+    d = (G @ lin.reshape(-1)).reshape(2, 6, 5)
+    assert np.allclose(d[0, :-1], a, atol=1e-12) and np.all(d[0, -1] == 0)
+    assert np.allclose(d[1, :, :-1], b, atol=1e-12) and np.all(d[1, :, -1] == 0)
+    assert rp[-1] == 2 * (5 * 5 + 6 * 4)
+
+
+def _bcb_problem(O, n=32, nv=3000, seed=3):
+    from helpers import annulus_uv
+    im = O.Grid2D(n, n, (-0.64, 0.64, -0.64, 0.64))
+    uvw = annulus_uv(nv, r2=1.5e6, seed=seed)
+    w = rng(seed + 1).uniform(0.5e6, 1.5e6, nv)
+    return im, uvw, w
+
+
+def test_bcb_oracle_matches_explicit_and_dft(oracle):
+    """The dirty-beam convolution (data_base.py:115-129,163-172) equals the
+    explicit N^T C N and the DFT's D^T C D to the reference's 1e-6 at its
+    default kernel support 4 (nufft_tests.py:87-97); at support 6 both are
+    exact to ~1e-11, which a wrong roll, a missing 1/N or an apodization
+    term would break."""
+    O = oracle
+    im, uvw, w = _bcb_problem(O)
+    rp, ci, va = O.dft_matrix(uvw, [1.0], im)
+    D = sp.csr_matrix((va, ci, rp), shape=(2 * uvw.shape[0], im.nx * im.ny))
+    x = rng(5).normal(size=(im.nx, im.ny))
+    dft = (D.T @ (np.repeat(w, 2) * (D @ x.reshape(-1)))).reshape(im.nx, im.ny)
+    for wsup, bar in ((4, 1e-6), (6, 1e-10)):
+        geo = O.NUFFTGeometry(im, pad=2, wsup=wsup)
+        beam = O.dirty_beam(uvw, [1.0], im, w, wsup=wsup)
+        bcb = O.bcb_apply(x, beam, geo)
+        vis = O.nufft_forward(x, uvw, [1.0], geo).reshape(-1) * w
+        expl = O.nufft_adjoint(vis.reshape(1, -1), uvw, [1.0], geo)
+        assert O.max_relative_error(bcb, expl) < bar
+        assert O.max_relative_error(bcb, dft) < bar
+
+
+def test_dirty_beam_peak_is_weight_sum(oracle):
+    """nufft_tests.py:116-118: max(beam) / sum(w) = 1 within 1e-6; the peak
+    sits at the beam space's centre pixel (offset 0) and the beam is even."""
+    O = oracle
+    im, uvw, w = _bcb_problem(O, n=24)
+    beam = O.dirty_beam(uvw, [1.0], im, w)
+    assert abs(beam.max() / w.sum() - 1.0) < 1e-6
+    assert np.unravel_index(np.argmax(beam), beam.shape) == (im.nx, im.ny)
+    inner = beam[1:, 1:]
+    assert O.max_relative_error(inner, inner[::-1, ::-1]) < 1e-6
