@@ -180,6 +180,20 @@ lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int
 lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                                   const double *x, double *y, int kernel);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  y = A x with the CSR arrays (row_ptr nrows+1, cols and vals
+ * nnz) and x, y already in device memory: the same kernels and summation
+ * order as lt_csr_spmv_host (lane groups from the mean row length, rows of
+ * more than 2048 entries in 2048-entry chunks added in order), so the same
+ * bits.  The long-row plan and chunk sums live in a fixed 1.4 MiB device
+ * array per GPU, set up by lt_init: at most 65536 chunks, which refuses
+ * nnz above about 67M with LT_ERR_CAPACITY before touching the device.
+ * No hipMalloc, hipFree or allocating library call; waits for the result.
+ * Calls that overlap on one GPU race on the shared array. */
+lt_status lt_csr_spmv_device_noalloc(long nrows, long ncols, long nnz, const int *d_row_ptr, const int *d_cols,
+                                     const double *d_vals, const double *d_x, double *d_y);
+
 /* No reference counterpart: a plain streaming copy of nbytes (multiple of
  * 32) between two device buffers, reps times; mean time per launch.  The
  * achievable-bandwidth ceiling and the calibration of HBM traffic counters. */

@@ -140,6 +140,7 @@ def lib():
         'lt_fft_c2r_host': ([i, i, i, vp, vp], i),
         'lt_csr_spmv_host': ([l, l, vp, vp, vp, vp, vp], i),
         'lt_csr_spmv_host_kernel': ([l, l, vp, vp, vp, vp, vp, i], i),
+        'lt_csr_spmv_device_noalloc': ([l, l, l, vp, vp, vp, vp, vp], i),
         'lt_stream_copy': ([l, i, dp], i),
     }
     for name, (args, res) in sig.items():
@@ -165,7 +166,7 @@ def exported_symbols():
             'lt_pcnet_get_params', 'lt_pcnet_set_params', 'lt_pcnet_forward_host', 'lt_pcnet_grad_host',
             'lt_pcnet_train', 'lt_pcnet_eval', 'lt_pcnet_destroy', 'lt_solver_set_pcnet', 'lt_solver_diag', 'lt_solver_train_pcnet',
             'lt_solver_pc_fallbacks', 'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_set_rhs', 'lt_solver_set_precision', 'lt_solver_set_lens_pc', 'lt_solver_set_graphs', 'lt_solver_lens_pc_matrix', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
-            'lt_fft_c2r_host', 'lt_csr_spmv_host', 'lt_csr_spmv_host_kernel', 'lt_stream_copy']
+            'lt_fft_c2r_host', 'lt_csr_spmv_host', 'lt_csr_spmv_host_kernel', 'lt_csr_spmv_device_noalloc', 'lt_stream_copy']
 
 
 def check(status):

@@ -47,6 +47,7 @@ lt_status init_device(int device) {
     hipDeviceProp_t prop;
     LT_HIP(hipGetDeviceProperties(&prop, device));
     g_cus = prop.multiProcessorCount;
+    LT_TRY(noalloc_setup(device));
     g_device = device;
     return LT_OK;
 }

@@ -38,6 +38,9 @@ const char *last_error();
 
 // library-wide device and stream (default priority; one process per GPU)
 lt_status ensure_init();
+// device set-up of the allocation-free SpMV's fixed scratch (sparse.hip);
+// called with `device` current
+lt_status noalloc_setup(int device);
 hipStream_t stream();
 int device_cu_count();
 

@@ -350,6 +350,22 @@ __global__ void k_stream_blocks(long nblk, long nrows, const int32_t *__restrict
     rblk[b] = (int32_t)lo;
 }
 
+// sum of one chunk [b, min(b + kChunk, row end)) of a long row by a whole
+// 256-thread workgroup, in a fixed order; thread 0 writes *out
+template <class T>
+__device__ inline void chunk_sum(int b, int row, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                 const T *__restrict__ va, const T *__restrict__ x, T *__restrict__ out) {
+    __shared__ T sh[4];
+    const int e = min(b + kChunk, rp[row + 1]);
+    T s = (T)0;
+    for (int k = b + threadIdx.x; k < e; k += 256) s += va[k] * x[ci[k]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
 // partial sums of fixed-size chunks of the long rows, one workgroup each
 template <class T>
 __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t *__restrict__ chunk_beg,
@@ -357,18 +373,9 @@ __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t 
                                                       const int32_t *__restrict__ rp,
                                                       const int32_t *__restrict__ ci, const T *__restrict__ va,
                                                       const T *__restrict__ x, T *__restrict__ out, RowFilter F) {
-    __shared__ T sh[4];
     const long c = blockIdx.x;
     if (F.skip(chunk_end_row[c])) return;
-    const int b = chunk_beg[c];
-    const int e = min(b + kChunk, rp[chunk_end_row[c] + 1]);
-    T s = (T)0;
-    for (int k = b + threadIdx.x; k < e; k += 256) s += va[k] * x[ci[k]];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) out[c] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    chunk_sum<T>(chunk_beg[c], chunk_end_row[c], rp, ci, va, x, out + c);
 }
 
 // long rows: sum their chunks in order
@@ -981,4 +988,179 @@ lt_status csr_transpose(const DevCSR &A, DevCSR &B) {
     return sync();
 }
 
+// ---------------------------------------------------------------------------
+// SpMV that allocates nothing (lt_csr_spmv_device_noalloc): the long-row plan
+// and its chunk sums live in `scratch`, one fixed array per GPU, instead of
+// pool memory.  Same kernels and summation order as lt_csr_spmv_host.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr long kNaMaxChunks = 65536;  // chunk sums of rows above kNaLongRow entries
+constexpr long kNaMaxLong = 32768;    // rows above kNaLongRow entries
+constexpr int kNaLongRow = 2048;      // DevCSR::long_row default
+// byte offsets into scratch: two counters, then the plan and the sums
+constexpr size_t kNaRows = 256;
+constexpr size_t kNaChunk0 = kNaRows + 4 * kNaMaxLong;
+constexpr size_t kNaNch = kNaChunk0 + 4 * kNaMaxLong;
+constexpr size_t kNaBeg = kNaNch + 4 * kNaMaxLong;
+constexpr size_t kNaRow = kNaBeg + 4 * kNaMaxChunks;
+constexpr size_t kNaSum = kNaRow + 4 * kNaMaxChunks;
+constexpr size_t kNaBytes = kNaSum + 8 * kNaMaxChunks;  // 1.38 MiB
+
+// per-GPU device array; host code uses its address from hipGetSymbolAddress
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+struct NaPlan {
+    int *cnt;  // [0] long rows, [1] chunks
+    int32_t *rows, *chunk0, *nch, *beg, *row;
+    double *sum;
+};
+
+NaPlan na_plan(void *base) {
+    unsigned char *b = (unsigned char *)base;
+    return NaPlan{(int *)b,
+                  (int32_t *)(b + kNaRows),
+                  (int32_t *)(b + kNaChunk0),
+                  (int32_t *)(b + kNaNch),
+                  (int32_t *)(b + kNaBeg),
+                  (int32_t *)(b + kNaRow),
+                  (double *)(b + kNaSum)};
+}
+
+// counters to zero by a kernel (no memset node if a caller captures the work)
+__global__ void k_na_reset(int *cnt) {
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+}
+
+// every row above long_row entries takes a slot and a contiguous range of
+// chunk slots; the slots' order varies from run to run, a row's sum does not
+// (its chunks are summed in row order by k_na_long)
+__global__ void k_na_plan(long nrows, const int32_t *__restrict__ rp, NaPlan P) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int b = rp[r], e = rp[r + 1];
+    if (e - b <= kNaLongRow) return;
+    const int nc = (e - b + kChunk - 1) / kChunk;
+    const int l = atomicAdd(&P.cnt[0], 1);
+    const int c0 = atomicAdd(&P.cnt[1], nc);
+    if (l >= kNaMaxLong || c0 + nc > kNaMaxChunks) return;  // excluded by the host's size check
+    P.rows[l] = (int32_t)r;
+    P.chunk0[l] = c0;
+    P.nch[l] = nc;
+    for (int k = 0; k < nc; ++k) {
+        P.beg[c0 + k] = b + k * kChunk;
+        P.row[c0 + k] = (int32_t)r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_na_chunks(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ va, const double *__restrict__ x,
+                                                    NaPlan P) {
+    const int c = blockIdx.x;
+    if (c >= P.cnt[1]) return;  // whole workgroups leave together
+    chunk_sum<double>(P.beg[c], P.row[c], rp, ci, va, x, P.sum + c);
+}
+
+__global__ void k_na_long(NaPlan P, double *__restrict__ y) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= P.cnt[0]) return;
+    double s = 0.0;
+    for (int c = P.chunk0[l]; c < P.chunk0[l] + P.nch[l]; ++c) s += P.sum[c];
+    y[P.rows[l]] = 1.0 * s;
+}
+
+std::mutex g_na_mu;
+void *g_na_ptr[64] = {};
+
+}  // namespace
+
+// Called by the library's device set-up with `device` current: the device
+// address of this GPU's copy of scratch.
+lt_status noalloc_setup(int device) {
+    if (device < 0 || device >= 64) {
+        set_error("noalloc: device %d out of range", device);
+        return LT_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (g_na_ptr[device]) return LT_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error("noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+        return LT_ERR_HIP;
+    }
+    g_na_ptr[device] = p;
+    return LT_OK;
+}
+
 }  // namespace lt
+
+using namespace lt;
+
+extern "C" {
+
+lt_status lt_csr_spmv_device_noalloc(long nrows, long ncols, long nnz, const int *d_row_ptr, const int *d_cols,
+                                     const double *d_vals, const double *d_x, double *d_y) {
+    LT_TRY(ensure_init());
+    if (nrows < 0 || ncols < 0 || nnz < 0 || !d_row_ptr || (nrows > 0 && !d_y) || (nnz > 0 && (!d_cols || !d_vals)) ||
+        (ncols > 0 && nnz > 0 && !d_x) || nrows > (1L << 31) - 2 || nnz > (1L << 31) - 1) {
+        set_error("lt_csr_spmv_device_noalloc: bad arguments");
+        return LT_ERR_ARG;
+    }
+    // every row above kNaLongRow entries has at least kNaLongRow + 1, so at
+    // most nnz / (kNaLongRow + 1) such rows and nnz / kChunk + that many chunks
+    const long max_long = nnz / (kNaLongRow + 1), max_chunks = nnz / kChunk + max_long;
+    if (max_long > kNaMaxLong || max_chunks > kNaMaxChunks) {
+        set_error("lt_csr_spmv_device_noalloc: %ld nonzeros exceed the fixed scratch (at most %ld chunks of %d "
+                  "entries for rows longer than %d)", nnz, kNaMaxChunks, kChunk, kNaLongRow);
+        return LT_ERR_CAPACITY;
+    }
+    int dev = 0;
+    LT_HIP(hipGetDevice(&dev));
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_na_mu);
+        base = (dev >= 0 && dev < 64) ? g_na_ptr[dev] : nullptr;
+    }
+    if (!base) {
+        set_error("lt_csr_spmv_device_noalloc: scratch not set up on device %d", dev);
+        return LT_ERR_HIP;
+    }
+    if (nrows == 0) return LT_OK;
+    int32_t last = 0;
+    LT_HIP(hipMemcpyAsync(&last, d_row_ptr + nrows, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
+    LT_TRY(sync());
+    if ((long)last != nnz) {
+        set_error("lt_csr_spmv_device_noalloc: row_ptr[nrows] = %d but nnz = %ld", last, nnz);
+        return LT_ERR_ARG;
+    }
+    const NaPlan P = na_plan(base);
+    hipLaunchKernelGGL(k_na_reset, dim3(1), dim3(64), 0, stream(), P.cnt);
+    hipLaunchKernelGGL(k_na_plan, dim3(blocks_for(nrows, 256)), dim3(256), 0, stream(), nrows, d_row_ptr, P);
+    LT_CHECK_LAUNCH();
+    // rows up to kNaLongRow entries: the lane-group product lt_csr_spmv_host
+    // picks from the mean row length (csr_pick_lanes)
+    double mean = (double)nnz / (double)nrows;
+    int lanes = 1;
+    while (lanes < 64 && lanes < mean) lanes <<= 1;
+    const RowFilter F{nullptr, 1, nullptr};
+    const long rows_per_launch = (1L << 30) / lanes;
+    for (long r0 = 0; r0 < nrows; r0 += rows_per_launch) {
+        const long nr = std::min(rows_per_launch, nrows - r0);
+        LT_TRY(with_lanes(lanes, [&](auto LC) {
+            constexpr int L = decltype(LC)::value;
+            hipLaunchKernelGGL((k_spmv<L, double>), dim3(blocks_for(nr * L, 256)), dim3(256), 0, stream(), r0,
+                               r0 + nr, d_row_ptr, d_cols, d_vals, d_x, d_y, 1.0, 0.0, kNaLongRow, F);
+        }));
+    }
+    if (max_chunks > 0) {
+        hipLaunchKernelGGL(k_na_chunks, dim3((unsigned)max_chunks), dim3(256), 0, stream(), d_row_ptr, d_cols, d_vals,
+                           d_x, P);
+        hipLaunchKernelGGL(k_na_long, dim3(blocks_for(std::max(max_long, 1L), 256)), dim3(256), 0, stream(), P, d_y);
+        LT_CHECK_LAUNCH();
+    }
+    return sync();
+}
+
+}  // extern "C"
