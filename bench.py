@@ -155,6 +155,27 @@ def degrid_roofline(wl, reps=20):
             'gvis_s_f64': NVIS / (ms64 * 1e-3) / 1e9}
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def stencil_roofline(wl, nb=NLENS, reps=20):
+    """The timed step's dominant kernel: the normal stencil k_stencil_apply
+    (band form, the plan's resident stencil, nb samples), event-timed on the
+    library stream.  Algorithmic work per launch (lt_nufft_stencil_time): 8 B
+    per FP64 coefficient (169 per stencil row), 16 B per sample per row
+    written and per distinct band value read; 4 flops per row, offset and
+    sample.  Its intensity (~12 flop/B) is above the FP64 ridge
+    (78.6 TFLOP/s / 8 TB/s), so FP64 throughput is the bound."""
+    B = wl.B
+    ms, by, fl = C.c_double(0), C.c_double(0), C.c_double(0)
+    rows, win = C.c_long(0), C.c_long(0)
+    B.check(B.lib().lt_nufft_stencil_time(wl.plan._handle, nb, reps, C.byref(ms), C.byref(by), C.byref(fl),
+                                          C.byref(rows), C.byref(win)))
+    t = ms.value * 1e-3
+    return {'ms': ms.value, 'bytes': by.value, 'flops': fl.value, 'rows': rows.value, 'win_cells': win.value,
+            'tflops': fl.value / t / 1e12, 'gbs': by.value / t / 1e9}
+
+
 def stream_copy_gbs(nbytes=1 << 30, reps=20):
     """Achievable HBM bandwidth: a plain device-to-device streaming copy."""
     from lentils_amd import backend as B
@@ -182,10 +203,37 @@ def profile_degrid(reps=10):
                       'copy_gbs': gbs, 'copy_launches': reps + 1}), flush=True)
 
 
-def load_traffic():
-    """HBM bytes per degrid launch from the committed counter summary
-    (profiles/degrid_f32_c2_pmc.json), or None."""
-    path = os.path.join(ROOT, 'profiles', 'degrid_f32_c2_pmc.json')
+def profile_kernels(reps=10):
+    """Workload for the counter runs (rocprofv3 --pmc): one lens batch of the
+    plain GPU workload (k_deflect_lattice over 64 lenses and the raster
+    kernels), the normal stencil at 64 samples (reps launches), the FP32
+    degridder of the 1e7-visibility uv set (reps launches), the learned
+    preconditioner's forward pass on 64 128^2 inputs (k_conv), and a 256 MiB
+    streaming copy reps times (the calibration kernel for FETCH_SIZE /
+    WRITE_SIZE)."""
+    from lentils_amd import backend as B
+    from lentils_amd.learned import PCNet
+    from lentils_amd.solvers import LensBatch
+    B.check(B.lib().lt_init(0))
+    wl = Workload(nlens=2)
+    lb = LensBatch(wl.im, [wl.src] * NLENS, gen_lenses(NLENS), z_src=2.0)
+    del lb
+    st = stencil_roofline(wl, reps=reps)
+    rl = degrid_roofline(wl, reps=reps)
+    net = PCNet(128, 128, batch=64, seed=3)
+    x0 = np.random.default_rng(4).normal(size=(64, 2, 128, 128)).astype(np.float32)
+    for _ in range(reps):
+        net.forward(x0)
+    nbytes = 256 << 20
+    gbs, cms = stream_copy_gbs(nbytes, reps)
+    print(json.dumps({'stencil': st, 'stencil_launches': reps + 1, 'degrid_ms': rl['ms'], 'degrid_launches': reps + 1,
+                      'copy_bytes': nbytes, 'copy_ms': cms, 'copy_gbs': gbs, 'copy_launches': reps + 1}), flush=True)
+
+
+def load_traffic(name):
+    """HBM bytes per launch from a committed counter summary under profiles/,
+    or None."""
+    path = os.path.join(ROOT, 'profiles', name)
     try:
         with open(path) as f:
             return json.load(f).get('traffic_bytes_per_launch')
@@ -341,74 +389,124 @@ def precision_report(args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(wl, gpu_iters):
-    """Reference CPU path on a bounded sample: one lens sample's deflection +
-    rasterization with the reference's own C (oracle/_ref, single thread)
-    or the oracle port, plus two CG iterations of the same operator with the
-    oracle's FP64 gridder (OpenMP over visibilities, all cores) and numpy
-    FFT / scipy SpMV; the solve time is extrapolated with the GPU's median
-    CG iteration count."""
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(wl, runs=5):
+    """The reference's radio solve (test_all.py:57-72) on the host for one of
+    the 64 lens samples (sample 0), timed as the GPU step is: lens matrix +
+    preconditioner + CG to a 1e-6 relative residual per source solve.
+
+    * lens matrix: the reference's own raster.c + deflect.c (oracle/_ref,
+      single thread) in a child process (its fragment buffer can overflow on
+      the lens-centre pixel, raster.c:204-212); the oracle port if it dies;
+    * operator: L^T BCB L + lambda Cc^T Cc, BCB the dirty-beam convolution of
+      data_base.py:115-129 with scipy.fft (single thread, as
+      fourier_operators.py:19,23 call it);
+    * preconditioner: scipy splu of lambda Cc^T Cc + wtsum L_s^T L_s
+      (test_all.py:63-65).  L_s is L without its rows of more than 64 entries:
+      with the reference's full L the unmasked lens-centre pixels (rows of up
+      to ~15k entries) make L^T L nearly dense and SuperLU runs out of memory;
+      the reference's own test masks those pixels out;
+    * CG: scipy.sparse.linalg.cg(rtol=1e-6).
+    Set-up excluded from the timing (as the GPU's resident inputs are): the
+    dirty beam (oracle gridder on all cores) and the right-hand side.  One
+    warm-up, then the median of `runs` timed runs of each part."""
     from oracle import oracle as O
+    import scipy.fft as sfft
     import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
     cores = len(os.sched_getaffinity(0))
     im = O.Grid2D(512, 512, [-2.56, 2.56, -2.56, 2.56])
     src = O.Grid2D(128, 128, [-0.64, 0.64, -0.64, 0.64])
     lm = wl.lenses[0]
     lm.setup_raytracing(2.0)
-    # The reference's raster.c keeps at most 128000 fragments per pixel and
-    # writes past its buffer beyond that (raster.c:204-212); the lens-centre
-    # pixel of this workload can exceed it, so the reference build runs in a
-    # child process and the oracle port stands in if the child dies.
-    kind = 'port'
-    ref_note = ''
+    kind_raster = 'port'
+    note = ''
+    t_ras = []
     if O.ref_lib() is not None:
         import subprocess
-        code = ('import sys,time,numpy as np; sys.path.insert(0, %r); from oracle import oracle as O; '
-                'L=np.frombuffer(bytes.fromhex(sys.argv[1]), dtype=O.MASS_MODEL_DTYPE); '
-                'im=O.Grid2D(512,512,[-2.56,2.56,-2.56,2.56]); src=O.Grid2D(128,128,[-0.64,0.64,-0.64,0.64]); '
-                't=time.perf_counter(); r=O.ref_lens_matrix_csr(im,src,L); print(time.perf_counter()-t)') % ROOT
+        code = ('import sys, time\n'
+                'import numpy as np\n'
+                'sys.path.insert(0, %r)\n'
+                'from oracle import oracle as O\n'
+                'L = np.frombuffer(bytes.fromhex(sys.argv[1]), dtype=O.MASS_MODEL_DTYPE)\n'
+                'im = O.Grid2D(512, 512, [-2.56, 2.56, -2.56, 2.56])\n'
+                'src = O.Grid2D(128, 128, [-0.64, 0.64, -0.64, 0.64])\n'
+                'for _ in range(%d):\n'
+                '    t = time.perf_counter()\n'
+                '    O.ref_lens_matrix_csr(im, src, L)\n'
+                '    print(time.perf_counter() - t, flush=True)\n') % (ROOT, runs + 1)
         r = subprocess.run([sys.executable, '-c', code, lm.lenses.tobytes().hex()], capture_output=True, text=True)
         if r.returncode == 0:
-            kind = 'reference'
-            t_raster = float(r.stdout.strip().splitlines()[-1])
+            kind_raster = 'reference'
+            t_ras = [float(v) for v in r.stdout.split()][1:]
         else:
-            ref_note = (f' (the reference raster.c build exited with status {r.returncode} on this lens: '
-                        f'fragment-buffer overflow at the lens-centre pixel; the oracle port was timed instead)')
-    if kind == 'port':
-        t0 = time.perf_counter()
-        rp, ci, va, _ = O.lens_matrix_csr(im, src, lm.lenses)
-        t_raster = time.perf_counter() - t0
+            note = (f'; the reference raster.c build exited with status {r.returncode} on this lens (fragment-buffer '
+                    f'overflow at the lens-centre pixel), so the oracle port was timed')
     rp, ci, va, _ = O.lens_matrix_csr(im, src, lm.lenses)
-    Lm = sp.csr_matrix((va, ci, rp), shape=(512 * 512, 128 * 128))
+    if not t_ras:
+        for k in range(runs + 1):
+            t0 = time.perf_counter()
+            O.lens_matrix_csr(im, src, lm.lenses)
+            if k:
+                t_ras.append(time.perf_counter() - t0)
+    L = sp.csr_matrix((va, ci, rp), shape=(512 * 512, 128 * 128))
     crp, cci, cva = O.curv_matrix(src)
     Cm = sp.csr_matrix((cva, cci, crp), shape=(128 * 128, 128 * 128))
-    geo = O.NUFFTGeometry(im, pad=2, wsup=3, beta=14.04)
-    x = np.random.default_rng(1).normal(size=128 * 128)
-    iters = 1
+    R = wl.lambda_curv * (Cm.T @ Cm)
+    # set-up: the dirty beam with the reference's default NUFFT, the BCB multiplier, the right-hand side
     t0 = time.perf_counter()
-    for _ in range(iters):
-        img = (Lm @ x).reshape(512, 512)
-        vis = O.nufft_forward(img, wl.uvw, [1.0], geo, nthreads=cores).reshape(-1) * wl.weights
-        back = O.nufft_adjoint(vis.reshape(1, -1), wl.uvw, [1.0], geo, nthreads=cores).reshape(-1)
-        y = Lm.T @ back + wl.lambda_curv * (Cm.T @ (Cm @ x))
-        x = y / np.linalg.norm(y)
-    t_iter = (time.perf_counter() - t0) / iters
-    solve_s = t_raster + float(np.median(gpu_iters)) * t_iter
-    cpu = ''
-    try:
-        with open('/proc/cpuinfo') as f:
-            for line in f:
-                if line.startswith('model name'):
-                    cpu = line.split(':', 1)[1].strip()
-                    break
-    except OSError:
-        cpu = platform.processor()
-    return {'value': 1.0 / solve_s, 'unit': 'source solves/s', 'cores': cores, 'kind': kind,
-            'sample': (f'1 of the 64 lens samples: deflect+raster {t_raster:.2f}s '
-                       f'({"reference raster.c/deflect.c" if kind == "reference" else "oracle port"}, 1 thread) + '
-                       f'{iters} CG iterations of the explicit N^T C N operator on {cores} threads '
-                       f'({t_iter:.2f}s each, oracle FP64 gridder + numpy FFT + scipy SpMV), extrapolated to '
-                       f'{float(np.median(gpu_iters)):.0f} iterations (GPU median); CPU: {cpu}{ref_note}')}
+    beam = O.dirty_beam(wl.uvw, [1.0], im, wl.weights, nthreads=cores)
+    geo = O.NUFFTGeometry(im, pad=2, wsup=4)
+    dbf = sfft.rfft2(np.roll(beam, [geo.nxp // 2, geo.nyp // 2], axis=[0, 1])) / beam.size
+    rhs = L.T @ wl.dirty[0].reshape(-1)
+    wtsum = float(np.sum(wl.weights))
+    t_setup = time.perf_counter() - t0
+
+    def bcb(x):
+        p = np.zeros((geo.nxp, geo.nyp))
+        p[geo.padx:geo.padx + 512, geo.pady:geo.pady + 512] = x.reshape(512, 512)
+        q = sfft.irfft2(dbf * sfft.rfft2(p), s=p.shape, norm='forward')
+        return q[geo.padx:geo.padx + 512, geo.pady:geo.pady + 512].reshape(-1)
+
+    A = spl.LinearOperator((src.nx * src.ny,) * 2, matvec=lambda x: L.T @ bcb(L @ x) + R @ x)
+    keep = (np.diff(L.indptr) <= 64).astype(np.float64)
+    Ls = sp.diags(keep) @ L
+    t_solve, iters, resid = [], [], []
+    for k in range(runs + 1):
+        t0 = time.perf_counter()
+        lu = spl.splu((R + wtsum * (Ls.T @ Ls)).tocsc())
+        M = spl.LinearOperator(A.shape, matvec=lu.solve)
+        n = [0]
+        x, info = spl.cg(A, rhs, rtol=1e-6, atol=0.0, M=M, maxiter=5000,
+                         callback=lambda xk: n.__setitem__(0, n[0] + 1))
+        dt = time.perf_counter() - t0
+        if k:
+            t_solve.append(dt)
+            iters.append(n[0])
+            resid.append(float(np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs)))
+    solve_s = float(np.median(t_ras) + np.median(t_solve))
+    return {'value': 1.0 / solve_s, 'unit': 'source solves/s', 'cores': 1, 'kind': 'port',
+            'sample': (f'1 of the 64 lens samples (sample 0), the reference\'s radio solve structure '
+                       f'(test_all.py:57-72): lens matrix {np.median(t_ras):.2f}s ('
+                       f'{"reference raster.c/deflect.c via oracle/_ref" if kind_raster == "reference" else "oracle port"}'
+                       f', 1 thread) + splu(R + wtsum L_s^T L_s) and scipy CG with the BCB operator (scipy.fft, '
+                       f'1 thread) to rtol 1e-6: {np.median(t_solve):.2f}s, {int(np.median(iters))} CG iterations, '
+                       f'true relative residual {max(resid):.2e}; medians of {runs} runs after a warm-up; set-up '
+                       f'excluded: dirty beam + BCB multiplier + rhs {t_setup:.1f}s on {cores} threads; '
+                       f'L_s = L without rows of more than 64 entries (the full L^T L does not fit SuperLU); '
+                       f'CPU: {_cpu_model()}{note}'),
+            'cg_iters': int(np.median(iters)), 'true_rel_resid_max': max(resid),
+            'lens_matrix_s': float(np.median(t_ras)), 'solve_s': float(np.median(t_solve)), 'setup_s': t_setup}
 
 
 def main():
@@ -426,6 +524,8 @@ def main():
                     help='the batching workload (256 samples, uneven source grids, 1e6 visibilities) instead')
     ap.add_argument('--profile-degrid', action='store_true',
                     help='only run the degridder and the calibration copy (for rocprofv3 --pmc)')
+    ap.add_argument('--profile-kernels', action='store_true',
+                    help='only run the hot kernels once each batch and the calibration copy (for rocprofv3 --pmc)')
     ap.add_argument('--no-graphs', action='store_true',
                     help='launch the CG kernels one by one instead of as captured HIP graphs (for kernel tracing)')
     ap.add_argument('--shared-data', action='store_true',
@@ -435,6 +535,9 @@ def main():
     GRAPHS = not args.no_graphs
     if args.profile_degrid:
         profile_degrid()
+        return
+    if args.profile_kernels:
+        profile_kernels()
         return
     if args.batching:
         batching_form(args)
@@ -498,6 +601,9 @@ def main():
               'true_rel_resid_max': float(resl.max()), 'all_converged': bool(np.all(resl <= 1e-6)),
               'method': 'flexible CG, preconditioner: 3 Jacobi-PCG steps on the explicit lambda Cc^T Cc + c L_s^T L_s (L_s: rows of L with <= 64 nonzero entries)'}
         log(f'lens preconditioner: {tl:.2f}s iters median {np.median(itsl):.0f} max res {resl.max():.2e}')
+    st = stencil_roofline(wl)
+    log(f'stencil {st["ms"] * 1e3:.1f} us/launch at {NLENS} samples: {st["tflops"]:.2f} TFLOP/s, {st["gbs"]:.0f} GB/s '
+        f'({st["rows"]} rows, {st["win_cells"]} window cells)')
     rl = degrid_roofline(wl)
     log(f'degrid f32 {rl["ms"] * 1e3:.1f} us/launch, {rl["gbs"]:.0f} GB/s, {rl["gvis_s"]:.2f} Gvis/s')
     copy_gbs, _ = stream_copy_gbs()
@@ -521,14 +627,21 @@ def main():
         'precision_form': pf,
         'lens_preconditioner': lp,
         'degrid_gvis_s': rl['gvis_s'], 'degrid_gvis_s_f64': rl['gvis_s_f64'],
-        'roofline': {'kernel': 'k_degrid32<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS,
-                     'traffic': load_traffic(), 'algorithmic_bytes': rl['bytes'],
-                     'stream_copy_gbs': copy_gbs},
+        'roofline': {'kernel': f'k_stencil_apply (FP64 normal stencil N^T C N, band form, {NLENS} samples per launch)',
+                     'bound': 'mfma', 'achieved': st['tflops'], 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': st['tflops'] / FP64_PEAK_TFLOPS, 'traffic': load_traffic('stencil_c2_pmc.json'),
+                     'algorithmic_flops': st['flops'], 'algorithmic_bytes': st['bytes'],
+                     'achieved_gbs': st['gbs'], 'us_per_launch': st['ms'] * 1e3,
+                     'compute_unit': 'FP64 (VALU FMA; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); '
+                                     'intensity above the FP64 ridge, so FP64 throughput bounds it'},
+        'roofline_degrid': {'kernel': 'k_degrid32<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
+                            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS,
+                            'traffic': load_traffic('degrid_f32_c2_pmc.json'), 'algorithmic_bytes': rl['bytes'],
+                            'us_per_launch': rl['ms'] * 1e3, 'stream_copy_gbs': copy_gbs},
     }
     if not args.no_cpu_baseline:
         t = time.perf_counter()
-        out['cpu_baseline'] = cpu_baseline(wl, its)
+        out['cpu_baseline'] = cpu_baseline(wl)
         log(f'cpu baseline {time.perf_counter() - t:.1f}s')
     print(json.dumps(out), flush=True)
 

@@ -245,6 +245,14 @@ lt_status lt_nufft_set_weights(lt_nufft h, const double *weights);
  * workgroup (1, 2, 4, 8) for this plan instead of the autotuned choice (0).
  * Every choice gives the same bits. */
 lt_status lt_nufft_set_stencil_shape(lt_nufft h, int samples_per_group);
+/* No reference counterpart: times the resident normal stencil's kernel as
+ * the solver launches it for nb samples (band form, tuned shape) with HIP
+ * events on the library stream; mean per-launch time, algorithmic bytes
+ * (8 B per stencil coefficient, 16 B per sample per row written and per
+ * distinct band value read) and flops (4 per row, offset and sample) per
+ * launch, the stencil's rows and distinct window cells (any may be NULL). */
+lt_status lt_nufft_stencil_time(lt_nufft h, int nb, int reps, double *ms_per_launch, double *bytes, double *flops,
+                                long *rows, long *win_cells);
 
 /* Device memory for resident inputs (plain hipMalloc'd bytes). */
 lt_status lt_device_alloc(long nbytes, void **dptr);

@@ -940,7 +940,12 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, in
     extern __shared__ unsigned char smem[];
     const int R = RT > 0 ? RT : R_;
     const int WS = ST + 2 * R;
-    Cplx<T> *win = reinterpret_cast<Cplx<T> *>(smem);
+    // the window as separate real and imaginary planes of NB x WS rows of
+    // WP (odd) values: the staging writes (stride WP) and the reads (16
+    // consecutive values per lane row) are free of LDS bank conflicts
+    const int WP = WS | 1;
+    T *wre = reinterpret_cast<T *>(smem);
+    T *wim = wre + NB * WS * WP;
     const long lb = xcd_block(blockIdx.x, gridDim.x);
     const int grp = (int)(lb % ngroup);
     const int tile = tiles[lb / ngroup];
@@ -957,13 +962,19 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, in
     for (int k = 0; k < ns; ++k) {
         const Cplx<T> *grid = spec + (long)smp[k] * sstride;
         for (int i = threadIdx.x; i < WS * WS; i += blockDim.x) {
+            int wa, wb;
+            Cplx<T> c;
             if (BAND) {  // the band is stored [v][u]: consecutive threads take consecutive u
-                const int wb = i / WS, wa = i - wb * WS;
-                win[k * WS * WS + wa * WS + wb] = fetch_band(G, grid, u0 + wa - G.cu, v0 + wb - G.cv, nbv);
+                wb = i / WS;
+                wa = i - wb * WS;
+                c = fetch_band(G, grid, u0 + wa - G.cu, v0 + wb - G.cv, nbv);
             } else {
-                const int wa = i / WS, wb = i - wa * WS;
-                win[k * WS * WS + i] = fetch_folded(G, grid, u0 + wa - G.cu, v0 + wb - G.cv);
+                wa = i / WS;
+                wb = i - wa * WS;
+                c = fetch_folded(G, grid, u0 + wa - G.cu, v0 + wb - G.cv);
             }
+            wre[k * WS * WP + wa * WP + wb] = c.re;
+            wim[k * WS * WP + wa * WP + wb] = c.im;
         }
     }
     __syncthreads();
@@ -980,12 +991,11 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, in
 #pragma unroll
         for (int ov = 0; ov < NR; ++ov) {
             const T cf = (T)coef[(long)(ou * NR + ov) * nrow + row];
-            const int wi = (ly + ou) * WS + (lx + ov);
+            const int wi = (ly + ou) * WP + (lx + ov);
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const Cplx<T> sv = win[k * WS * WS + wi];
-                ar[k] += cf * sv.re;
-                ai[k] += cf * sv.im;
+                ar[k] += cf * wre[k * WS * WP + wi];
+                ai[k] += cf * wim[k * WS * WP + wi];
             }
         }
     }
@@ -2691,7 +2701,7 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
         // NB samples share one staged coefficient read (autotuned: 1, 2, 4 or 8)
 #define LT_STENCIL_NB(NBV)                                                                                        \
     {                                                                                                             \
-        const size_t sh = (size_t)NBV * WS * WS * sizeof(Cplx<T>);                                                \
+        const size_t sh = (size_t)NBV * WS * (WS | 1) * sizeof(Cplx<T>);                                          \
         const int ngroup = (nactive + NBV - 1) / NBV;                                                             \
         const dim3 grid((unsigned)(St.ntile * ngroup));                                                           \
         if (band && St.R == 6)                                                                                    \
@@ -2787,6 +2797,91 @@ __global__ void k_gather_weights(long n, const int32_t *__restrict__ perm, const
                                  double *__restrict__ w_out) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) w_out[q] = w_in[perm[q]];
+}
+
+// Times the normal stencil's kernel as the solver launches it (band form,
+// the plan's resident stencil, nb samples, the tuned samples per workgroup)
+// with HIP events on the library stream: mean duration per launch over reps
+// launches.  Algorithmic work per launch: every stencil row's NO FP64
+// coefficients once (8 B each), and per sample its output (16 B per row) and
+// the distinct complex band values its windows read (16 B each: raw cells
+// within R of a row, a cell and its Hermitian mirror counted once); 4 flops
+// (a complex value times a real coefficient, added) per row, offset and
+// sample.
+lt_status stencil_time(NufftPlan &P, int nb, int reps, double *ms, double *bytes, double *flops, long *rows,
+                       long *win_cells) {
+    NormalStencil *St = P.wstencil.get();
+    if (!St || !St->use_band || P.prec != LT_F64) {
+        set_error("stencil timing: the plan has no resident FP64 band stencil (lt_nufft_set_weights)");
+        return LT_ERR_ARG;
+    }
+    if (nb <= 0 || reps <= 0) {
+        set_error("stencil timing: bad batch or repetition count");
+        return LT_ERR_ARG;
+    }
+    LT_TRY(P.ensure_batch(nb));
+    const long npix = (long)P.nx * P.ny;
+    DevBuf<double> imgs, out;
+    DevBuf<int> all;
+    LT_TRY(imgs.alloc(npix * nb));
+    LT_TRY(out.alloc(npix * nb));
+    LT_TRY(imgs.zero());
+    std::vector<int> ids(nb);
+    for (int k = 0; k < nb; ++k) ids[k] = k;
+    LT_TRY(all.upload(ids.data(), nb));
+    // first application allocates the band buffers; the tuner picks the shape
+    LT_TRY((normal_band_impl<double, double>(P, *St, imgs.p, out.p, all.p, nb, 1.0, false)));
+    if (P.wstencil_tuned_batch != nb) {
+        LT_TRY(stencil_autotune(P, *St, imgs.p, out.p, all.p, nb));
+        P.wstencil_tuned_batch = nb;
+    }
+    const long bsz = (long)St->nbv * P.nu;
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    LT_TRY((stencil_apply_t<double, double>(*St, P, St->coef.p, P.bandB.p, bsz, nullptr, nullptr, nb, true)));
+    LT_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r)
+        LT_TRY((stencil_apply_t<double, double>(*St, P, St->coef.p, P.bandB.p, bsz, nullptr, nullptr, nb, true)));
+    LT_HIP(hipEventRecord(e1, stream()));
+    LT_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    LT_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = (double)t / reps;
+    // distinct band values the windows read
+    const Geo G = make_geo(P);
+    const long ng = (long)P.nu * P.nv;
+    std::vector<int32_t> rm(ng);
+    LT_TRY(St->rowmap.download(rm.data(), ng));
+    LT_TRY(sync());
+    std::vector<uint8_t> m(ng, 0), mu(ng, 0);
+    const int R = St->R;
+    for (long g = 0; g < ng; ++g)  // dilate along v
+        if (rm[g] >= 0) {
+            const long a = g / P.nv, b = g % P.nv;
+            for (long q = std::max(0L, b - R); q <= std::min((long)P.nv - 1, b + R); ++q) mu[a * P.nv + q] = 1;
+        }
+    for (long g = 0; g < ng; ++g)  // then along u
+        if (mu[g]) {
+            const long a = g / P.nv, b = g % P.nv;
+            for (long q = std::max(0L, a - R); q <= std::min((long)P.nu - 1, a + R); ++q) m[q * P.nv + b] = 1;
+        }
+    long nwin = 0;
+    for (long g = 0; g < ng; ++g) {
+        if (!m[g]) continue;
+        const long a = g / P.nv - G.cu, b = g % P.nv - G.cv;  // centred raw coordinates
+        const long ma = -a + G.cu, mb = -b + G.cv;
+        const bool mirror_in = ma >= 0 && ma < P.nu && mb >= 0 && mb < P.nv && m[ma * P.nv + mb];
+        // a pair counts once: keep the member with the larger index
+        if (!mirror_in || g >= ma * P.nv + mb) ++nwin;
+    }
+    if (rows) *rows = St->nrow;
+    if (win_cells) *win_cells = nwin;
+    if (bytes) *bytes = 8.0 * St->NO * St->nrow + 16.0 * nb * (double)(St->nrow + nwin);
+    if (flops) *flops = 4.0 * St->NO * (double)St->nrow * nb;
+    return LT_OK;
 }
 
 lt_status NufftPlan::set_weights(const double *weights) {

@@ -44,6 +44,8 @@ struct lt_nufft_s {
 
 namespace lt {
 NufftPlan *nufft_of(lt_nufft h) { return h ? &h->plan : nullptr; }
+lt_status stencil_time(NufftPlan &P, int nb, int reps, double *ms, double *bytes, double *flops, long *rows,
+                       long *win_cells);
 }  // namespace lt
 
 extern "C" {
@@ -163,6 +165,13 @@ lt_status lt_nufft_time(lt_nufft h, int direction, int reps, double *ms) {
     (void)hipEventDestroy(e1);
     *ms = (double)t / reps;
     return LT_OK;
+}
+
+lt_status lt_nufft_stencil_time(lt_nufft h, int nb, int reps, double *ms, double *bytes, double *flops, long *rows,
+                                long *win_cells) {
+    if (!h) return LT_ERR_ARG;
+    LT_TRY(ensure_init());
+    return stencil_time(h->plan, nb, reps, ms, bytes, flops, rows, win_cells);
 }
 
 lt_status lt_nufft_destroy(lt_nufft h) {
