@@ -71,8 +71,11 @@ typedef struct {
 
 /* Replaces deflect_points, lentils/backend/deflect.c:60-66 (bound at
  * load_backend.py:59-61).  p_out = p_in - sum of lens deflections, FP64.
- * want_deriv != 0 is not supported: p_out is left untouched and an error is
- * recorded (lt_last_error_message). */
+ * want_deriv != 0: as the reference, every point's PEMD parameter
+ * derivatives d alpha / d(b_eff, qh, f, x, y, th) (deflect.c:304-338) are
+ * added into the same deriv[0..11] as (x, y) pairs, points and components
+ * in order; deriv must hold 12 doubles (the reference's caller passes a
+ * larger zeroed array). */
 void deflect_points(global_lens_model glm, double *p_in, int npoints, double *p_out, int want_deriv,
                     double *deriv);
 
@@ -135,6 +138,16 @@ const char *lt_version(void);
 /* Host-pointer deflection with a status code; same arithmetic as
  * deflect_points. */
 lt_status lt_deflect_points(const global_lens_model *glm, const double *p_in, long npoints, double *p_out);
+
+/* No reference counterpart (the reference only sums derivatives over all
+ * points, deflect.c:60-66): per-point PEMD parameter derivatives.  dalpha
+ * receives, per point p and per PEMD component c of the model (in order),
+ * the 12 values d alpha / d(b_eff, qh, f, x, y, th) as (x, y) pairs at
+ * dalpha[(p * n_pemd + c) * 12 + k], deflect.c:243-338 (b_eff the internal
+ * radius of deflect.c:284, held fixed for the other five; th in degrees);
+ * p_out as lt_deflect_points. */
+lt_status lt_deflect_points_deriv(const global_lens_model *glm, const double *p_in, long npoints, double *p_out,
+                                  double *dalpha);
 
 /* Capacity query for manifold_lens_matrix_csr (replaces the 50-per-row guess
  * of lens_operators.py:30-34).  raster_status receives LT_RASTER_* bits. */

@@ -85,6 +85,7 @@ def lib():
         'curv_matrix_findiff_csr': ([image_space_ctype, vp, vp, vp], None),
         'grad_matrix_findiff_csr': ([image_space_ctype, vp, vp, vp], None),
         'lt_deflect_points': ([C.POINTER(global_lens_model_ctype), vp, l, vp], i),
+        'lt_deflect_points_deriv': ([C.POINTER(global_lens_model_ctype), vp, l, vp, vp], i),
         'lt_lens_matrix_nnz': ([image_space_ctype, image_space_ctype, global_lens_model_ctype, lp, ip], i),
         'lt_lens_matrix_csr': ([image_space_ctype, image_space_ctype, global_lens_model_ctype, vp, vp, vp, l, lp,
                                 ip], i),
@@ -155,7 +156,7 @@ def lib():
 def exported_symbols():
     """Names every build of the library must export (include/lentils_hip.h)."""
     return ['lt_init', 'lt_last_error_message', 'lt_version', 'deflect_points', 'manifold_lens_matrix_csr',
-            'grid_cpu', 'dft_matrix_csr', 'curv_matrix_findiff_csr', 'grad_matrix_findiff_csr', 'lt_deflect_points',
+            'grid_cpu', 'dft_matrix_csr', 'curv_matrix_findiff_csr', 'grad_matrix_findiff_csr', 'lt_deflect_points', 'lt_deflect_points_deriv',
             'lt_lens_matrix_nnz', 'lt_lens_matrix_csr', 'lt_deflect_lattice', 'lt_lens_matrix_from_lattice',
             'lt_lensbatch_create', 'lt_lensbatch_info', 'lt_lensbatch_download', 'lt_lensbatch_destroy',
             'lt_nufft_create', 'lt_nufft_forward_host', 'lt_nufft_adjoint_host', 'lt_nufft_degrid_host',

@@ -163,14 +163,56 @@ lt_status lt_deflect_points(const global_lens_model *glm, const double *p_in, lo
     return sync();
 }
 
+lt_status lt_deflect_points_deriv(const global_lens_model *glm, const double *p_in, long npoints, double *p_out,
+                                  double *dalpha) {
+    LT_TRY(ensure_init());
+    if (!glm || npoints < 0 || (npoints > 0 && (!p_in || !p_out))) {
+        set_error("lt_deflect_points_deriv: bad arguments");
+        return LT_ERR_ARG;
+    }
+    std::vector<DevLens> lens;
+    LT_TRY(pack_lenses(glm->lenses, glm->num_lenses, lens));
+    long npemd = 0;
+    for (const DevLens &L : lens) npemd += (L.type == 0);
+    if (npoints == 0) return LT_OK;
+    if (npemd > 0 && !dalpha) {
+        set_error("lt_deflect_points_deriv: dalpha is NULL");
+        return LT_ERR_ARG;
+    }
+    DevBuf<double> din, dout, dd;
+    LT_TRY(din.upload(p_in, 2 * npoints));
+    LT_TRY(dout.alloc(2 * npoints));
+    LT_TRY(dd.alloc(std::max(1L, 12 * npemd * npoints)));
+    LT_TRY(deflect_points_deriv_device(lens, din.p, dout.p, dd.p, npoints));
+    LT_TRY(dout.download(p_out, 2 * npoints));
+    if (npemd > 0) LT_TRY(dd.download(dalpha, 12 * npemd * npoints));
+    return sync();
+}
+
+// want_deriv != 0: the reference adds every point's PEMD derivatives into
+// the same deriv[0..11] (deflect.c:26-66 passes one pointer to every point),
+// points in order and components in order; the per-point values come from
+// the GPU in chunks and are added here in that order
 void deflect_points(global_lens_model glm, double *p_in, int npoints, double *p_out, int want_deriv, double *deriv) {
-    (void)deriv;
-    if (want_deriv) {
-        set_error("deflect_points: derivatives (want_deriv) are not provided by this library");
-        fprintf(stderr, "lentils_hip: %s\n", last_error());
+    if (!want_deriv) {
+        if (lt_deflect_points(&glm, p_in, npoints, p_out) != LT_OK)
+            fprintf(stderr, "lentils_hip: %s\n", last_error());
         return;
     }
-    if (lt_deflect_points(&glm, p_in, npoints, p_out) != LT_OK) fprintf(stderr, "lentils_hip: %s\n", last_error());
+    long npemd = 0;
+    for (int i = 0; i < glm.num_lenses; ++i) npemd += (glm.lenses[i].type == 0);
+    const long chunk = 1L << 20;
+    std::vector<double> dv((size_t)std::max(1L, 12 * npemd * std::min<long>(chunk, std::max(npoints, 1))));
+    for (long p0 = 0; p0 < npoints; p0 += chunk) {
+        const long n = std::min<long>(chunk, npoints - p0);
+        if (lt_deflect_points_deriv(&glm, p_in + 2 * p0, n, p_out + 2 * p0, dv.data()) != LT_OK) {
+            fprintf(stderr, "lentils_hip: %s\n", last_error());
+            return;
+        }
+        if (!deriv) continue;
+        for (long q = 0; q < n * npemd; ++q)
+            for (int k = 0; k < 12; ++k) deriv[k] += dv[q * 12 + k];
+    }
 }
 
 lt_status lt_lens_matrix_nnz(image_space im, image_space src, global_lens_model glm, long *nnz, int *status) {

@@ -37,6 +37,55 @@ __global__ void __launch_bounds__(256) k_deflect_points(const DevLens *__restric
     pout[2 * p + 1] = oy;
 }
 
+// points with their PEMD parameter derivatives: the deflected point, and per
+// PEMD component c (in the model's order) the 12 values
+// d alpha / d(b_eff, qh, q, x, y, th) at dv[(p * npemd + c) * 12 + k]
+// (deflect.c:243-338); external shear adds its deflection only (its
+// derivative block is disabled in the reference, deflect.c:111-122)
+__global__ void __launch_bounds__(256) k_deflect_points_deriv(const DevLens *__restrict__ lenses, int nl, int npemd,
+                                                               const double *__restrict__ pin,
+                                                               double *__restrict__ pout, double *__restrict__ dv,
+                                                               long n) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double px = pin[2 * p], py = pin[2 * p + 1];
+    double ox = px, oy = py;
+    int c = 0;
+    for (int i = 0; i < nl; ++i) {
+        double ax = 0.0, ay = 0.0;
+        if (lenses[i].type == 0) {
+            double d[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) d[k] = 0.0;
+            pemd_deflection_d(lenses[i], px, py, ax, ay, d);
+            double *o = dv + (p * npemd + c) * 12;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) o[k] = d[k];
+            ++c;
+        } else {
+            shear_deflection(lenses[i], px, py, ax, ay);
+        }
+        ox -= ax;
+        oy -= ay;
+    }
+    pout[2 * p] = ox;
+    pout[2 * p + 1] = oy;
+}
+
+lt_status deflect_points_deriv_device(const std::vector<DevLens> &lens, const double *pin_dev, double *pout_dev,
+                                      double *dv_dev, long n) {
+    int npemd = 0;
+    for (const DevLens &L : lens) npemd += (L.type == 0);
+    DevBuf<DevLens> dl;
+    LT_TRY(dl.upload(lens.data(), lens.size()));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_deflect_points_deriv, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), dl.p,
+                           (int)lens.size(), npemd, pin_dev, pout_dev, dv_dev, n);
+        LT_CHECK_LAUNCH();
+    }
+    return sync();
+}
+
 // lattice point (a, b) of sample s; lattice is x-major, (2nx+1)*(2ny+1)
 // points per sample, 2 doubles each
 __global__ void __launch_bounds__(256) k_deflect_lattice(const DevLens *__restrict__ lenses,

@@ -181,6 +181,130 @@ LT_HD void pemd_deflection(const DevLens &L, double px, double py, double &ax, d
     ay += -hi;
 }
 
+// The series with its derivatives (the want_deriv branch of
+// deflect.c:179-239): F, and d[0..7] = dF/dt, dF/dq, dF/dx, dF/dy (complex,
+// interleaved; x, y the rotated, centred coordinates).  Mixed real/complex
+// products componentwise, complex quotients by cdiv, as gcc evaluates the
+// reference.
+LT_HD void hyp2f1_series_d(const DevLens &L, double zr, double zi, double &fr, double &fi, double d[8]) {
+    const double x = zr, y = zi, q = L.q, t = L.t, qt = L.qt;
+    double r2 = L.qq * x * x + y * y;
+    double zzr, zzi, w2r, w2i;
+    cmul(x, y, x, y, zzr, zzi);
+    cdiv(qt * r2, 0.0, zzr, zzi, w2r, w2i);
+    double sr, si;
+    csqrt_d(1.0 - w2r, -w2i, sr, si);
+    double ur = 0.5 * (1.0 - sr);
+    double ui = 0.5 * (-si);
+    double acc_r = 0.0, acc_i = 0.0, pr = 1.0, pi = 0.0, an = 1.0;
+    double dtr = 0.0, dti = 0.0, dur = 0.0, dui = 0.0, da = 0.0, p1r = 0.0, p1i = 0.0;
+#pragma unroll
+    for (int n = 0; n < 20; n++) {
+        acc_r += an * pr;
+        acc_i += an * pi;
+        double den = (double)(2 * n + 4) - t;
+        dtr += da * pr;  // df_dt += da_dt u_n
+        dti += da * pi;
+        const double ann = an * (double)n;  // df_du += a_n n u_{n-1}
+        dur += ann * p1r;
+        dui += ann * p1i;
+        p1r = pr;
+        p1i = pi;
+        da = (double)(-2 * (2 + n)) / (den * den) * an + 2.0 * ((double)(2 + n) - t) / den * da;
+        double nr, ni;
+        cmul(pr, pi, ur, ui, nr, ni);
+        pr = nr;
+        pi = ni;
+        an *= (den - t) / den;
+    }
+    fr = acc_r;
+    fi = acc_i;
+    double qr, qi, gr, gi;
+    cdiv(1.0, 0.0, 4.0 * sr, 4.0 * si, qr, qi);  // du/dw2 = 1 / (4 sqrt(1 - w2))
+    cmul(dur, dui, qr, qi, gr, gi);              // dF/dw2
+    const double q3 = q * q * q;
+    double ar, ai, br, bi;
+    cmul(q3 * zr, q3 * zi, zr, zi, ar, ai);
+    cdiv(-2.0 * (q * q * q * q * x * x + y * y), 0.0, ar, ai, br, bi);  // dw2/dq
+    double z3r, z3i;
+    cmul(zzr, zzi, zr, zi, z3r, z3i);
+    const double c1 = -2.0 * qt * y, c2 = -2.0 * qt * x;
+    double xr_, xi_, yr_, yi_;
+    cdiv(c1 * y, c1 * (-q * q * x), z3r, z3i, xr_, xi_);         // dw2/dx
+    cdiv(-(y * c2), (q * q * x) * c2, z3r, z3i, yr_, yi_);       // dw2/dy
+    d[0] = dtr;
+    d[1] = dti;
+    cmul(gr, gi, br, bi, d[2], d[3]);
+    cmul(gr, gi, xr_, xi_, d[4], d[5]);
+    cmul(gr, gi, yr_, yi_, d[6], d[7]);
+}
+
+// PEMD deflection accumulated into (ax, ay) and its derivatives
+// d alpha / d(b_eff, qh, q, x, y, th) as (x, y) pairs into dv[12]
+// (deflect.c:243-338; b_eff the internal radius of deflect.c:284, held fixed
+// for the other five, th in degrees).
+LT_HD void pemd_deflection_d(const DevLens &L, double px, double py, double &ax, double &ay, double dv[12]) {
+    const double q = L.q, t = L.t, b = L.b, cs = L.cr, sn = -L.ci;
+    double zr, zi;
+    cmul(px - L.x, py - L.y, L.cr, L.ci, zr, zi);
+    const double x = zr, y = zi;
+    double rs2 = L.qq * zr * zr + zi * zi;
+    double rs = sqrt(rs2);
+    double dr, di;
+    cdiv(L.bb, 0.0, q * zr, q * zi, dr, di);
+    double pw = pow_radial(b / rs, L.tm2);
+    double Ar = dr * pw, Ai = di * pw;
+    double Fr, Fi, fd[8];
+    hyp2f1_series_d(L, zr, zi, Fr, Fi, fd);
+    double gr, gi, hr, hi;
+    cmul(Ar, Ai, Fr, Fi, gr, gi);
+    cmul(gr, gi, L.cr, L.ci, hr, hi);
+    const double alr = hr, ali = -hi;
+    ax += alr;
+    ay += ali;
+    const double Abr = Ar * t / b, Abi = Ai * t / b;
+    const double lg = log(b / rs);
+    const double Atr = Ar * lg, Ati = Ai * lg;
+    const double kq = (t - 1.0) * q * q * x * x + y * y, qrs = q * rs2;
+    const double Aqr = -Ar * kq / qrs, Aqi = -Ai * kq / qrs;
+    const double zrsr = zr * rs2, zrsi = zi * rs2, qqx = q * q * x;
+    double nr_, ni_, Axr, Axi, Ayr, Ayi;
+    cmul(-Ar, -Ai, y * y + qqx * (x * (t - 1.0)), qqx * ((t - 2.0) * y), nr_, ni_);
+    cdiv(nr_, ni_, zrsr, zrsi, Axr, Axi);
+    cmul(-Ar, -Ai, y * ((t - 2.0) * x), q * q * x * x + y * ((t - 1.0) * y), nr_, ni_);
+    cdiv(nr_, ni_, zrsr, zrsi, Ayr, Ayi);
+    double ur, ui, vr, vi, wr, wi;
+    cmul(Fr, Fi, Abr, Abi, ur, ui);
+    cmul(ur, ui, L.cr, L.ci, wr, wi);
+    dv[0] += wr;
+    dv[1] += -wi;
+    cmul(Fr, Fi, Atr, Ati, ur, ui);
+    cmul(Ar, Ai, fd[0], fd[1], vr, vi);
+    cmul(ur + vr, ui + vi, L.cr, L.ci, wr, wi);
+    dv[2] += 2.0 * wr;
+    dv[3] += 2.0 * (-wi);
+    cmul(Fr, Fi, Aqr, Aqi, ur, ui);
+    cmul(Ar, Ai, fd[2], fd[3], vr, vi);
+    cmul(ur + vr, ui + vi, L.cr, L.ci, wr, wi);
+    dv[4] += wr;
+    dv[5] += -wi;
+    cmul(Fr, Fi, Axr, Axi, ur, ui);
+    cmul(Ar, Ai, fd[4], fd[5], vr, vi);
+    cmul(ur + vr, ui + vi, L.cr, L.ci, wr, wi);
+    const double dxsr = wr, dxsi = -wi;
+    cmul(Fr, Fi, Ayr, Ayi, ur, ui);
+    cmul(Ar, Ai, fd[6], fd[7], vr, vi);
+    cmul(ur + vr, ui + vi, L.cr, L.ci, wr, wi);
+    const double dysr = wr, dysi = -wi;
+    dv[6] += -(dxsr * cs - dysr * sn);
+    dv[7] += -(dxsi * cs - dysi * sn);
+    dv[8] += -(dxsr * sn + dysr * cs);
+    dv[9] += -(dxsi * sn + dysi * cs);
+    const double k_deg = M_PI / 180;
+    dv[10] += k_deg * (dxsr * y - dysr * x + -ali);
+    dv[11] += k_deg * (dxsi * y - dysi * x + alr);
+}
+
 // external shear + Keeton (2001) convergence/shear gradient terms.
 // deflect.c:70-108
 LT_HD void shear_deflection(const DevLens &L, double px, double py, double &ax, double &ay) {

@@ -121,6 +121,10 @@ struct DevLens {
 };
 
 lt_status pack_lenses(const generic_mass_model *in, int n, std::vector<DevLens> &out);
+// deflected points and per (point, PEMD component) the 12 parameter
+// derivatives at dv[(p * npemd + c) * 12 + k] (deflect.hip)
+lt_status deflect_points_deriv_device(const std::vector<DevLens> &lens, const double *pin_dev, double *pout_dev,
+                                      double *dv_dev, long n);
 
 // CSR on device (block-diagonal batch of matrices in one array)
 struct DevCSR {

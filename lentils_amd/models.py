@@ -41,15 +41,40 @@ class GlobalLensModel(global_lens_model_ctype):
         self._c_lenses = self.lenses.ctypes.data_as(C.c_void_p)
 
     def deflect(self, points, z_s, deriv=False):
-        """Source-plane positions of image-plane points (..., 2)."""
-        if deriv:
-            raise NotImplementedError('lens-parameter derivatives are not provided by this backend')
+        """Source-plane positions of image-plane points (..., 2).
+
+        deriv=True returns (deflected, gradients) as lens_models.py:64-77
+        does: gradients has shape (8,) + points.shape and, through the
+        reference's deflect_points(want_deriv=1), receives in its first 12
+        values the derivatives d alpha / d(b_eff, qh, f, x, y, th) of every
+        PEMD component summed over all points (deflect.c:304-338); the
+        per-point values are deflect_gradients()."""
         self.setup_raytracing(z_s)
         pts = np.ascontiguousarray(points, dtype=np.float64)
         out = np.zeros_like(pts)
         n = pts.size // 2
+        if deriv:
+            gradients = np.zeros((8,) + pts.shape)
+            B.lib().deflect_points(self, B.ptr(pts), n, B.ptr(out), 1, B.ptr(gradients))
+            return out, gradients
         B.check(B.lib().lt_deflect_points(C.byref(self), B.ptr(pts), n, B.ptr(out)))
         return out
+
+    def deflect_gradients(self, points, z_s):
+        """(deflected, dalpha): dalpha has shape (n_pemd, 6) + points.shape,
+        the derivatives of each PEMD component's deflection with respect to
+        its (b_eff, qh, f, x, y, th) at every point (b_eff the internal
+        Einstein-radius parameter of deflect.c:284, held fixed for the other
+        five; th in degrees)."""
+        self.setup_raytracing(z_s)
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        out = np.zeros_like(pts)
+        n = pts.size // 2
+        npemd = int(np.sum(self.lenses['type'] == 0))
+        dv = np.zeros((n, max(npemd, 1), 12))
+        B.check(B.lib().lt_deflect_points_deriv(C.byref(self), B.ptr(pts), n, B.ptr(out), B.ptr(dv)))
+        dv = dv[:, :npemd].reshape(n, npemd, 6, 2).transpose(1, 2, 0, 3)
+        return out, np.ascontiguousarray(dv).reshape((npemd, 6) + pts.shape)
 
 
 class MassModel:

@@ -146,6 +146,160 @@ static void or_pemd(const or_mass_model *m, const double *p, double *a) {
     a[1] += -hi;
 }
 
+/* The same series with its derivatives (the want_deriv branch of
+ * deflect.c:179-239): F, and d[0..7] = dF/dt, dF/dq, dF/dx, dF/dy (complex,
+ * interleaved), where x, y are the rotated, centred coordinates.  The
+ * recurrences follow deflect.c:213-224 term by term; the chain rule
+ * deflect.c:227-236, with the real/complex mixed products the way gcc
+ * evaluates them (componentwise) and every complex quotient by Smith's
+ * method. */
+static void or_hyp2f1_d(double t, double q, double zr, double zi, double *fr, double *fi, double *d) {
+    double x = zr, y = zi;
+    double r2 = q * q * x * x + y * y;
+    double qt = (1 - q * q) / (q * q);
+    double zzr, zzi, w2r, w2i;
+    or_cmul(x, y, x, y, &zzr, &zzi);
+    or_cdiv(qt * r2, 0.0, zzr, zzi, &w2r, &w2i);
+    double complex s = csqrt((1.0 - w2r) + I * (-w2i));
+    double ur = 0.5 * (1.0 - creal(s));
+    double ui = 0.5 * (-cimag(s));
+    double sr = 0.0, si = 0.0, pr = 1.0, pi = 0.0, an = 1.0;
+    double dtr = 0.0, dti = 0.0, dur = 0.0, dui = 0.0, da = 0.0, p1r = 0.0, p1i = 0.0;
+    for (int n = 0; n < 20; n++) {
+        sr += an * pr;
+        si += an * pi;
+        double den = 2 * n + 4 - t;
+        dtr += da * pr; /* df_dt += da_dt * u_n */
+        dti += da * pi;
+        double ann = an * n; /* df_du += a_n * n * u_{n-1} */
+        dur += ann * p1r;
+        dui += ann * p1i;
+        p1r = pr;
+        p1i = pi;
+        da = -2 * (2 + n) / (den * den) * an + 2 * (2 + n - t) / den * da;
+        double nr, ni;
+        or_cmul(pr, pi, ur, ui, &nr, &ni);
+        pr = nr;
+        pi = ni;
+        an *= (den - t) / den;
+    }
+    *fr = sr;
+    *fi = si;
+    /* du/dw2 = 1 / (4 sqrt(1 - w2)); dF/dw2 = dF/du du/dw2 */
+    double qr, qi, gr, gi;
+    or_cdiv(1.0, 0.0, 4 * creal(s), 4 * cimag(s), &qr, &qi);
+    or_cmul(dur, dui, qr, qi, &gr, &gi);
+    /* dw2/dq = -2 (q^4 x^2 + y^2) / (q^3 z^2) */
+    double q3 = q * q * q, ar, ai, br, bi, cr, ci;
+    or_cmul(q3 * zr, q3 * zi, zr, zi, &ar, &ai);
+    or_cdiv(-2 * (q * q * q * q * x * x + y * y), 0.0, ar, ai, &br, &bi);
+    /* z^3 */
+    double z3r, z3i;
+    or_cmul(zzr, zzi, zr, zi, &z3r, &z3i);
+    /* dw2/dx = -2 q_ y (y - i q^2 x) / z^3 */
+    double c1 = -2 * qt * y;
+    double xr_, xi_;
+    or_cdiv(c1 * y, c1 * (-q * q * x), z3r, z3i, &xr_, &xi_);
+    /* dw2/dy = -2 q_ i x (q^2 x + i y) / z^3: (i c2) (q^2 x + i y) = (-c2 y, c2 q^2 x) */
+    double c2 = -2 * qt * x;
+    double yr_, yi_;
+    or_cdiv(-(y * c2), (q * q * x) * c2, z3r, z3i, &yr_, &yi_);
+    d[0] = dtr;
+    d[1] = dti;
+    or_cmul(gr, gi, br, bi, &cr, &ci);
+    d[2] = cr;
+    d[3] = ci;
+    or_cmul(gr, gi, xr_, xi_, &cr, &ci);
+    d[4] = cr;
+    d[5] = ci;
+    or_cmul(gr, gi, yr_, yi_, &cr, &ci);
+    d[6] = cr;
+    d[7] = ci;
+}
+
+/* PEMD deflection added into a[2] and its parameter derivatives added into
+ * dv[12] = d alpha / d(b, qh, f, x, y, th) as (x, y) pairs, the want_deriv
+ * branch of deflect.c:243-338 (b the record's Einstein-radius parameter, qh
+ * the slope (t = 2 qh), f the axis ratio, x/y the lens centre, th in
+ * degrees). */
+static void or_pemd_d(const or_mass_model *m, const double *p, double *a, double *dv) {
+    const double *f = m->fpars;
+    double lx = f[0], ly = f[1], q = f[2], sn = f[4], cs = f[5], bb = f[6], qh = f[7];
+    double t = 2 * qh;
+    double b = pow(0.5 * bb * pow(q, t - 0.5) * (3.0 - t) / (2.0 - t), 1.0 / t);
+    double cr = cs, ci = -sn;
+    double zr, zi;
+    or_cmul(p[0] - lx, p[1] - ly, cr, ci, &zr, &zi);
+    double x = zr, y = zi;
+    double rs2 = q * q * x * x + y * y;
+    double rs = sqrt(rs2);
+    double dr, di;
+    or_cdiv(b * b, 0.0, q * zr, q * zi, &dr, &di);
+    double pw = pow(b / rs, t - 2);
+    double Ar = dr * pw, Ai = di * pw;
+    double Fr, Fi, fd[8];
+    or_hyp2f1_d(t, q, zr, zi, &Fr, &Fi, fd);
+    double gr, gi, hr, hi;
+    or_cmul(Ar, Ai, Fr, Fi, &gr, &gi);
+    or_cmul(gr, gi, cr, ci, &hr, &hi);
+    double alr = hr, ali = -hi; /* alpha = conj(A F crot) */
+    a[0] += alr;
+    a[1] += ali;
+    /* dA/db = A t / b, dA/dt = A log(b / rs) */
+    double Abr = Ar * t / b, Abi = Ai * t / b;
+    double lg = log(b / rs);
+    double Atr = Ar * lg, Ati = Ai * lg;
+    /* dA/dq = -A ((t-1) q^2 x^2 + y^2) / (q rs2) */
+    double kq = (t - 1) * q * q * x * x + y * y, qrs = q * rs2;
+    double Aqr = -Ar * kq / qrs, Aqi = -Ai * kq / qrs;
+    /* dA/dx = -A (y^2 + q^2 x (x (t-1) + i (t-2) y)) / (z rs2) */
+    double zrsr = zr * rs2, zrsi = zi * rs2, nr_, ni_, Axr, Axi, Ayr, Ayi;
+    double qqx = q * q * x;
+    or_cmul(-Ar, -Ai, y * y + qqx * (x * (t - 1)), qqx * ((t - 2) * y), &nr_, &ni_);
+    or_cdiv(nr_, ni_, zrsr, zrsi, &Axr, &Axi);
+    /* dA/dy = -A (i q^2 x^2 + y ((t-2) x + i (t-1) y)) / (z rs2) */
+    or_cmul(-Ar, -Ai, y * ((t - 2) * x), q * q * x * x + y * ((t - 1) * y), &nr_, &ni_);
+    or_cdiv(nr_, ni_, zrsr, zrsi, &Ayr, &Ayi);
+    /* chain rule, conjugate, rotation back */
+    double ur, ui, vr, vi, wr, wi;
+    or_cmul(Fr, Fi, Abr, Abi, &ur, &ui);
+    or_cmul(ur, ui, cr, ci, &wr, &wi);
+    double dbr = wr, dbi = -wi;
+    or_cmul(Fr, Fi, Atr, Ati, &ur, &ui);
+    or_cmul(Ar, Ai, fd[0], fd[1], &vr, &vi);
+    or_cmul(ur + vr, ui + vi, cr, ci, &wr, &wi);
+    double dtr = 2 * wr, dti = 2 * (-wi);
+    or_cmul(Fr, Fi, Aqr, Aqi, &ur, &ui);
+    or_cmul(Ar, Ai, fd[2], fd[3], &vr, &vi);
+    or_cmul(ur + vr, ui + vi, cr, ci, &wr, &wi);
+    double dqr = wr, dqi = -wi;
+    or_cmul(Fr, Fi, Axr, Axi, &ur, &ui);
+    or_cmul(Ar, Ai, fd[4], fd[5], &vr, &vi);
+    or_cmul(ur + vr, ui + vi, cr, ci, &wr, &wi);
+    double dxsr = wr, dxsi = -wi;
+    or_cmul(Fr, Fi, Ayr, Ayi, &ur, &ui);
+    or_cmul(Ar, Ai, fd[6], fd[7], &vr, &vi);
+    or_cmul(ur + vr, ui + vi, cr, ci, &wr, &wi);
+    double dysr = wr, dysi = -wi;
+    double dxr = -(dxsr * cs - dysr * sn), dxi = -(dxsi * cs - dysi * sn);
+    double dyr = -(dxsr * sn + dysr * cs), dyi = -(dxsi * sn + dysi * cs);
+    /* d/dth = pi/180 (dalpha/dxs y - dalpha/dys x + i alpha) */
+    double thr = M_PI / 180 * (dxsr * y - dysr * x + -ali);
+    double thi = M_PI / 180 * (dxsi * y - dysi * x + alr);
+    dv[0] += dbr;
+    dv[1] += dbi;
+    dv[2] += dtr;
+    dv[3] += dti;
+    dv[4] += dqr;
+    dv[5] += dqi;
+    dv[6] += dxr;
+    dv[7] += dxi;
+    dv[8] += dyr;
+    dv[9] += dyi;
+    dv[10] += thr;
+    dv[11] += thi;
+}
+
 /* external shear plus Keeton-2001 gradient terms.  deflect.c:70-108 */
 static void or_shear(const or_mass_model *m, const double *p, double *a) {
     const double *f = m->fpars;
@@ -194,6 +348,44 @@ static int or_deflect_one(or_lens_model glm, const double *xin, double *xout) {
 int oracle_deflect_points(or_lens_model glm, const double *pin, int n, double *pout) {
     int bad = 0;
     for (int p = 0; p < n; ++p) bad |= or_deflect_one(glm, &pin[2 * p], &pout[2 * p]);
+    return bad;
+}
+
+/* deflect_points with want_deriv = 1 (deflect.c:26-66, 304-338): every
+ * point's PEMD derivatives are added into the same deriv[0..11] (points in
+ * order, components in order), as the reference does; per_point (may be
+ * NULL) receives each point's own sums over the PEMD components,
+ * [point][12].  The shear's derivative block is disabled in the reference
+ * (deflect.c:111-122), so shears add nothing. */
+int oracle_deflect_points_deriv(or_lens_model glm, const double *pin, int n, double *pout, double *deriv,
+                                double *per_point) {
+    int bad = 0;
+    for (int p = 0; p < n; ++p) {
+        const double *xin = &pin[2 * p];
+        double *xout = &pout[2 * p];
+        double own[12] = {0};
+        xout[0] = xin[0];
+        xout[1] = xin[1];
+        for (int i = 0; i < glm.num_lenses; ++i) {
+            double a[2] = {0.0, 0.0};
+            const or_mass_model *m = &glm.lenses[i];
+            if (m->type == 0) {
+                double dv[12] = {0};
+                or_pemd_d(m, xin, a, dv);
+                for (int k = 0; k < 12; ++k) {
+                    deriv[k] += dv[k];
+                    own[k] += dv[k];
+                }
+            } else if (m->type == 1) {
+                or_shear(m, xin, a);
+            } else {
+                bad = 1;
+            }
+            xout[0] -= a[0];
+            xout[1] -= a[1];
+        }
+        if (per_point) memcpy(&per_point[12 * p], own, sizeof(own));
+    }
     return bad;
 }
 

@@ -97,6 +97,9 @@ def lib():
         L = _lib
         L.oracle_deflect_points.argtypes = [CLensModel, C.c_void_p, C.c_int, C.c_void_p]
         L.oracle_deflect_points.restype = C.c_int
+        L.oracle_deflect_points_deriv.argtypes = [CLensModel, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p]
+        L.oracle_deflect_points_deriv.restype = C.c_int
         L.oracle_lens_matrix_csr.argtypes = [CImageSpace, CImageSpace, CLensModel, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.POINTER(C.c_long)]
         L.oracle_lens_matrix_csr.restype = C.c_int
@@ -184,6 +187,27 @@ def deflect_points(lenses, pts, use_ref=False):
         if lib().oracle_deflect_points(glm, _ptr(pts), n, _ptr(out)):
             raise ValueError('unsupported lens type')
     return out
+
+
+def deflect_points_deriv(lenses, pts, use_ref=False):
+    """deflect_points with want_deriv = 1 (deflect.c:60-66, 304-338):
+    (deflected points, deriv[12], per_point) where deriv holds the
+    derivatives d alpha / d(b, qh, f, x, y, th) as (x, y) pairs summed over
+    every point and PEMD component in order (the reference adds them all
+    into one array), and per_point [n][12] each point's own sums (None for
+    the reference build)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    out = np.zeros_like(pts)
+    glm, keep = _lens_struct(lenses)
+    n = pts.size // 2
+    deriv = np.zeros(16)
+    if use_ref:
+        ref_lib().deflect_points(glm, _ptr(pts), n, _ptr(out), 1, _ptr(deriv))
+        return out, deriv[:12].copy(), None
+    per = np.zeros((n, 12))
+    if lib().oracle_deflect_points_deriv(glm, _ptr(pts), n, _ptr(out), _ptr(deriv), _ptr(per)):
+        raise ValueError('unsupported lens type')
+    return out, deriv[:12].copy(), per
 
 
 def stencil_lattice(im):

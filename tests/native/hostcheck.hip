@@ -41,6 +41,36 @@ int hc_deflect_points(const generic_mass_model *lenses, int nl, const double *pi
     return 0;
 }
 
+// deflection with the PEMD parameter derivatives, per point summed over
+// the PEMD components (the device kernel's per-component values, added in
+// component order)
+int hc_deflect_points_deriv(const generic_mass_model *lenses, int nl, const double *pin, long n, double *pout,
+                            double *dv) {
+    std::vector<DevLens> L(nl);
+    for (int k = 0; k < nl; ++k)
+        if (!pack_lens_one(lenses[k], L[k])) return 1;
+    for (long p = 0; p < n; ++p) {
+        const double px = pin[2 * p], py = pin[2 * p + 1];
+        double ox = px, oy = py, acc[12] = {0};
+        for (int i = 0; i < nl; ++i) {
+            double ax = 0.0, ay = 0.0;
+            if (L[i].type == 0) {
+                double d[12] = {0};
+                pemd_deflection_d(L[i], px, py, ax, ay, d);
+                for (int k = 0; k < 12; ++k) acc[k] += d[k];
+            } else {
+                shear_deflection(L[i], px, py, ax, ay);
+            }
+            ox -= ax;
+            oy -= ay;
+        }
+        pout[2 * p] = ox;
+        pout[2 * p + 1] = oy;
+        for (int k = 0; k < 12; ++k) dv[12 * p + k] = acc[k];
+    }
+    return 0;
+}
+
 // lens matrix from a lattice with the kernels' per-row code path; cols /
 // vals may be NULL (count only)
 long hc_lens_matrix_from_lattice(int nx, int ny, const char *mask, int snx, int sny, double sxmin, double symin,

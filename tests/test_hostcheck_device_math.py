@@ -35,6 +35,8 @@ def hc():
     L = C.CDLL(OUT)
     L.hc_deflect_points.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_long, C.c_void_p]
     L.hc_deflect_points.restype = C.c_int
+    L.hc_deflect_points_deriv.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
+    L.hc_deflect_points_deriv.restype = C.c_int
     L.hc_lens_matrix_from_lattice.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double,
                                               C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
@@ -126,3 +128,19 @@ def test_long_row_decomposition_reproduces_emission_order(hc, oracle, case):
             assert np.array_equal(got[0].view(np.int64), ref[0].view(np.int64)), (i, j, front)
             compared += ref[0].size > 1000
     assert longest > 1000 and compared >= 4
+
+
+def test_device_derivative_math_bitwise_on_host(hc, oracle):
+    """The kernels' derivative arithmetic (pemd_deflection_d, want_deriv
+    branch of deflect.c:179-338) built for the host equals the oracle bit
+    for bit, per point (and so the reference build's sums)."""
+    O = oracle
+    pts = rng(32).uniform(-1.8, 1.8, size=(20000, 2))
+    for lenses in [c1_lens(O)] + random_lenses(O, 6):
+        arr = np.array(lenses, dtype=O.MASS_MODEL_DTYPE)
+        out = np.zeros_like(pts)
+        dv = np.zeros((pts.shape[0], 12))
+        assert hc.hc_deflect_points_deriv(_p(arr), len(arr), _p(pts), pts.shape[0], _p(out), _p(dv)) == 0
+        ref, _, per = O.deflect_points_deriv(lenses, pts)
+        assert np.array_equal(out, ref)
+        assert np.array_equal(dv, per)

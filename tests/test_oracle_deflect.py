@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import scipy.special as sps
 
-from helpers import c1_lens, rng
+from helpers import c1_lens, random_lenses, rng
 
 
 def test_oracle_matches_reference_build_bitwise(oracle):
@@ -120,3 +120,61 @@ def test_components_add_linearly(oracle):
     S = O.pack_shear(ss=0.03, sa=45.0, x=0.01, y=0.02)
     a = _alpha(O, [P, S], pts)
     assert np.allclose(a, _alpha(O, [P], pts) + _alpha(O, [S], pts), rtol=1e-13, atol=1e-15)
+
+
+def _alpha(O, lenses, pts):
+    return pts - O.deflect_points(lenses, pts)
+
+
+def test_deflection_derivatives_match_reference_build(oracle):
+    """want_deriv = 1 (deflect.c:179-239, 243-338): the oracle's sums equal
+    the reference's own deflect.c (oracle/_ref) bit for bit, and so do the
+    deflected points."""
+    O = oracle
+    if O.ref_lib() is None:
+        pytest.skip('reference sources absent: oracle/_ref not built')
+    for k, lenses in enumerate([c1_lens(O)] + random_lenses(O, 4)):
+        pts = rng(50 + k).uniform(-2.2, 2.2, size=(4000, 2))
+        a, d, per = O.deflect_points_deriv(lenses, pts)
+        r, dr, _ = O.deflect_points_deriv(lenses, pts, use_ref=True)
+        assert np.array_equal(a, r)
+        assert np.array_equal(d, dr)
+        assert np.all(np.isfinite(per))
+        assert np.array_equal(a, O.deflect_points(lenses, pts))
+
+
+def _pemd_beff(O, beff, q, th, qh, x, y):
+    """A PEMD record whose internal radius (deflect.c:284) is beff: the
+    reference differentiates in (b_eff, qh, q, x, y, th) with b_eff held
+    fixed (dA_db = A t / b, deflect.c:316)."""
+    t = 2 * qh
+    b = 2 * beff ** t * (2 - t) / (q ** (t - 0.5) * (3 - t))
+    return O.pack_pemd(b=b, q=q, th=th, qh=qh, x=x, y=y)
+
+
+def test_deflection_derivatives_finite_differences(oracle):
+    """Each point's d alpha / d(b_eff, qh, q, x, y, th) against central
+    finite differences of the deflection (th in degrees, qh the slope with
+    t = 2 qh, b_eff the internal radius held fixed while q and qh move): a
+    dropped term, a wrong sign or a swapped component fails this."""
+    O = oracle
+    g = rng(60)
+    for case in range(3):
+        beff, q, th, qh = g.uniform(0.6, 1.4), g.uniform(0.6, 0.95), g.uniform(0, 180), g.uniform(0.4, 0.6)
+        x0, y0 = g.uniform(-0.05, 0.05, 2)
+        shear = O.pack_shear(ss=0.03, sa=g.uniform(0, 180), x=x0, y=y0)
+        pts = g.uniform(-2.0, 2.0, size=(600, 2))
+        pts = pts[np.hypot(pts[:, 0] - x0, pts[:, 1] - y0) > 0.1]
+        base = dict(beff=beff, q=q, th=th, qh=qh, x=x0, y=y0)
+        _, _, per = O.deflect_points_deriv([_pemd_beff(O, **base), shear], pts)
+        steps = {'beff': 1e-6, 'qh': 1e-6, 'q': 1e-6, 'x': 1e-6, 'y': 1e-6, 'th': 1e-4}
+        for k, name in enumerate(['beff', 'qh', 'q', 'x', 'y', 'th']):
+            h = steps[name]
+            up, dn = dict(base), dict(base)
+            up[name] += h
+            dn[name] -= h
+            fd = (_alpha(O, [_pemd_beff(O, **up), shear], pts) -
+                  _alpha(O, [_pemd_beff(O, **dn), shear], pts)) / (2 * h)
+            an = per[:, 2 * k:2 * k + 2]
+            err = np.max(np.abs(an - fd)) / np.max(np.abs(fd))
+            assert err < 1e-6, (case, name, err)
