@@ -65,6 +65,15 @@ typedef struct {
     generic_mass_model *lenses;
 } global_lens_model;
 
+/* lentils/backend/common.h:25-36: a triangulated (Delaunay) source plane */
+typedef struct {
+    int num_points, num_tris;
+    int nchannels, nstokes;
+    double *channels;
+    double *points;    /* num_points x 2 */
+    int *triangles;    /* num_tris x 3 vertex indices */
+} delaunay_space;
+
 /* ------------------------------------------------------------------------ */
 /* 1. Drop-in symbols                                                         */
 /* ------------------------------------------------------------------------ */
@@ -106,6 +115,24 @@ void curv_matrix_findiff_csr(image_space imspace, int *row_inds, int *cols, doub
 /* Replaces grad_matrix_findiff_csr, lentils/backend/nufft.c:163-194
  * (load_backend.py:107-109). */
 void grad_matrix_findiff_csr(image_space imspace, int *row_inds, int *cols, double *vals);
+
+/* Replaces triangle_gradient_csr, lentils/backend/triangles.c:66-103 (bound
+ * at load_backend.py:69-71): two rows (x, y) of three entries per triangle,
+ * the gradient of the linear interpolant times sqrt|area| (negated for a
+ * clockwise triangle); row_inds[2 num_tris + 1], cols / vals[6 num_tris]. */
+void triangle_gradient_csr(delaunay_space trispace, int *row_inds, int *cols, double *vals);
+
+/* Replaces delaunay_lens_matrix_csr, triangles.c:106-145 (load_backend.py:72-74):
+ * per image pixel (x-major) nothing when masked out, 1.0 on the next casted
+ * vertex for i % ncast == j % ncast == 0, else the barycentric weights of the
+ * pixel's deflected position (uncasted_points, in pixel order) in triangle
+ * uncasted_tri_inds[k] (nothing when < 0). */
+void delaunay_lens_matrix_csr(image_space imspace, delaunay_space trispace, int ncast, double *uncasted_points,
+                              int *uncasted_tri_inds, int *row_inds, int *cols, double *vals);
+
+/* Replaces triangle_geometry_from_inds, triangles.c:35-62 (triangles.h:4):
+ * barycentric weights (3 per point) of s in the triangles tri_inds of pos. */
+void triangle_geometry_from_inds(double *s, int *tri_inds, double *weights_out, int num_points, double *pos);
 
 /* ------------------------------------------------------------------------ */
 /* 2. lt_* entry points                                                       */
@@ -169,6 +196,14 @@ lt_status lt_deflect_lattice(image_space imspace, global_lens_model lensmodel, d
 lt_status lt_lens_matrix_from_lattice(image_space imspace, image_space srcspace, const double *lattice,
                                       int *row_inds, int *cols, double *vals, long capacity, long *nnz,
                                       int *raster_status);
+
+/* Checked variants of the Delaunay symbols: status codes, triangle and
+ * vertex indices validated, lens-matrix capacity (entries; < 0 unchecked)
+ * and the entry count returned. */
+lt_status lt_triangle_gradient_csr(delaunay_space trispace, int *row_inds, int *cols, double *vals);
+lt_status lt_delaunay_lens_matrix_csr(image_space imspace, delaunay_space trispace, int ncast,
+                                      const double *uncasted_points, const int *uncasted_tri_inds, int *row_inds,
+                                      int *cols, double *vals, long capacity, long *nnz);
 
 /* Stand-alone pieces of the operator surface, host pointers.
  * rfft2 / irfft2 of `batch` real nx x ny arrays (x-major) with the

@@ -8,7 +8,7 @@ gfx950 + rocFFT), plus batched device-resident solvers.
 from .spaces import Space, VisibilitySpace, FourierSpace, ImageSpace, DelaunaySpace  # noqa: F401
 from .models import GlobalLensModel, PowerLawEllipsoid, ExternalPotential  # noqa: F401
 from .operators import (Operator, CompositeOperatorProduct, CompositeOperatorSum, DiagonalOperator,  # noqa: F401
-                        PriorCovarianceOperator, ManifoldLensOperator, FFTOperator, ZeroPaddingOperator,
+                        PriorCovarianceOperator, ManifoldLensOperator, DelaunayLensOperator, FFTOperator, ZeroPaddingOperator,
                         ApodizationCorrectionOperator, GriddingOperator, NUFFTOperator, DFTOperator)
 from .solvers import LensBatch, NUFFTPlan, BatchedSourceInversion  # noqa: F401
 from .datasets import Dataset, RadioDataset, BlurredCovarianceOperator  # noqa: F401

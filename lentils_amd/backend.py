@@ -27,6 +27,13 @@ class image_space_ctype(C.Structure):
                 ('dx', C.c_double), ('dy', C.c_double), ('_c_channels', c_double_p), ('_c_mask', c_bool_p)]
 
 
+class delaunay_space_ctype(C.Structure):
+    """lentils/backend/common.h:25-36 (load_backend.py:33-35)."""
+    _fields_ = [('num_points', C.c_int), ('num_triangles', C.c_int), ('num_channels', C.c_int),
+                ('num_stokes', C.c_int), ('_c_channels', c_double_p), ('_c_points', c_double_p),
+                ('_c_triangles', c_int_p)]
+
+
 class fourier_space_ctype(C.Structure):
     _fields_ = [('nu', C.c_int), ('nv', C.c_int), ('half_nv', C.c_int), ('num_channels', C.c_int),
                 ('num_stokes', C.c_int), ('du', C.c_double), ('dv', C.c_double), ('gcx', C.c_double),
@@ -84,6 +91,11 @@ def lib():
         'dft_matrix_csr': ([visibility_space_ctype, image_space_ctype, vp, vp, vp], None),
         'curv_matrix_findiff_csr': ([image_space_ctype, vp, vp, vp], None),
         'grad_matrix_findiff_csr': ([image_space_ctype, vp, vp, vp], None),
+        'triangle_gradient_csr': ([delaunay_space_ctype, vp, vp, vp], None),
+        'delaunay_lens_matrix_csr': ([image_space_ctype, delaunay_space_ctype, i, vp, vp, vp, vp, vp], None),
+        'triangle_geometry_from_inds': ([vp, vp, vp, i, vp], None),
+        'lt_triangle_gradient_csr': ([delaunay_space_ctype, vp, vp, vp], i),
+        'lt_delaunay_lens_matrix_csr': ([image_space_ctype, delaunay_space_ctype, i, vp, vp, vp, vp, vp, l, lp], i),
         'lt_deflect_points': ([C.POINTER(global_lens_model_ctype), vp, l, vp], i),
         'lt_deflect_points_deriv': ([C.POINTER(global_lens_model_ctype), vp, l, vp, vp], i),
         'lt_lens_matrix_nnz': ([image_space_ctype, image_space_ctype, global_lens_model_ctype, lp, ip], i),
@@ -156,7 +168,9 @@ def lib():
 def exported_symbols():
     """Names every build of the library must export (include/lentils_hip.h)."""
     return ['lt_init', 'lt_last_error_message', 'lt_version', 'deflect_points', 'manifold_lens_matrix_csr',
-            'grid_cpu', 'dft_matrix_csr', 'curv_matrix_findiff_csr', 'grad_matrix_findiff_csr', 'lt_deflect_points', 'lt_deflect_points_deriv',
+            'grid_cpu', 'dft_matrix_csr', 'curv_matrix_findiff_csr', 'grad_matrix_findiff_csr', 'triangle_gradient_csr', 'delaunay_lens_matrix_csr',
+            'triangle_geometry_from_inds', 'lt_triangle_gradient_csr', 'lt_delaunay_lens_matrix_csr',
+            'lt_deflect_points', 'lt_deflect_points_deriv',
             'lt_lens_matrix_nnz', 'lt_lens_matrix_csr', 'lt_deflect_lattice', 'lt_lens_matrix_from_lattice',
             'lt_lensbatch_create', 'lt_lensbatch_info', 'lt_lensbatch_download', 'lt_lensbatch_destroy',
             'lt_nufft_create', 'lt_nufft_forward_host', 'lt_nufft_adjoint_host', 'lt_nufft_degrid_host',

@@ -22,7 +22,7 @@ COMMON = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}', '-munsafe-fp-a
           f'-I{ROCM}/include']
 EXACT = ['-ffp-contract=off', '-fwrapv']
 # translation units whose results must match the reference bit for bit
-EXACT_UNITS = {'deflect.hip', 'raster.hip'}
+EXACT_UNITS = {'deflect.hip', 'raster.hip', 'triangles.hip'}
 
 
 def hipcc():

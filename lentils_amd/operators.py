@@ -205,11 +205,33 @@ class PriorCovarianceOperator(Operator):
     matrices are built on the GPU (operator_base.py:200-251)."""
 
     def __init__(self, space, type, strength, lambda_boundary=None, **superargs):
+        n = space.size
         if isinstance(space, DelaunaySpace):
-            raise NotImplementedError('Delaunay source spaces are not supported by this backend')
+            # triangle meshes: identity or the weighted triangle gradients
+            # (triangle_gradient_csr, triangles.c:66-103, on the GPU)
+            if type == 'tikhonov':
+                self._op = sparse.spdiags(np.ones(n), 0, n, n)
+            elif type == 'gradient':
+                nt = space.num_triangles
+                rp = np.zeros(2 * nt + 1, dtype=np.int32)
+                ci = np.zeros(max(6 * nt, 1), dtype=np.int32)
+                va = np.zeros(max(6 * nt, 1), dtype=np.float64)
+                B.check(B.lib().lt_triangle_gradient_csr(space, B.ptr(rp), B.ptr(ci), B.ptr(va)))
+                self._op = sparse.csr_matrix((va[:6 * nt], ci[:6 * nt], rp), shape=(2 * nt, n))
+            else:
+                raise NotImplementedError('Curvature regularization on triangles not yet implemented')
+            self.type = type
+            self._lambda = strength
+            self._mat = (self._lambda * self._op.T.dot(self._op)).tocsr()
+            if lambda_boundary is not None:
+                data = np.zeros(n)
+                data[space.edge] = strength * lambda_boundary
+                self._mat = (self._mat + sparse.spdiags(data, 0, n, n)).tocsr()
+            self._transpose = self
+            Operator.__init__(self, space, space)
+            return
         if not isinstance(space, ImageSpace):
             raise NotImplementedError('PriorCovarianceOperator not implement for this space')
-        n = space.size
         if type == 'tikhonov':
             self._op = sparse.spdiags(np.ones(n), 0, n, n)
         elif type == 'gradient':
@@ -261,6 +283,48 @@ class ManifoldLensOperator(LensOperator):
                                      ci.size, C.byref(nnz), C.byref(st)))
         self.raster_status = st.value
         self._mat = sparse.csr_matrix((va[:nnz.value], ci[:nnz.value], rp), shape=(n, source_space.size))
+        super().__init__(image_space, source_space, lensmodel)
+
+
+class DelaunayLensOperator(LensOperator):
+    """Lens matrix onto an adaptive, triangulated source plane (Vegetti &
+    Koopmans; lens_operators.py:45-84): every ncasted-th image pixel (in
+    both directions, inside the mask) is ray-traced to a vertex of the
+    source triangulation, every other unmasked pixel is interpolated
+    linearly in the triangle its deflected position falls in.  Deflection
+    and the CSR assembly run on the GPU; the triangulation and the point
+    location are scipy's, as in the reference."""
+
+    def __init__(self, image_space, lensmodel, z_src, ncasted=1, **superargs):
+        from .models import GlobalLensModel
+        if not isinstance(image_space, ImageSpace):
+            raise TypeError('image_space must be ImageSpace')
+        if not isinstance(lensmodel, GlobalLensModel):
+            raise TypeError('lensmodel must be GlobalLensModel')
+        ncasted = int(ncasted)
+        image_mask = np.asarray(image_space.mask, dtype=bool)
+        image_points = image_space.points
+        casted_mask = np.zeros_like(image_mask)
+        casted_mask[::ncasted, ::ncasted] = True
+        uncasted_mask = ~casted_mask
+        casted_mask &= image_mask
+        uncasted_mask &= image_mask
+        self.casted_points = lensmodel.deflect(np.ascontiguousarray(image_points[casted_mask]), z_s=z_src)
+        self.uncasted_points = np.ascontiguousarray(
+            lensmodel.deflect(np.ascontiguousarray(image_points[uncasted_mask]), z_s=z_src))
+        source_space = DelaunaySpace(self.casted_points)
+        uncasted_tri_inds = np.ascontiguousarray(source_space.delaunay.find_simplex(self.uncasted_points),
+                                                 dtype=np.int32)
+        num_rows = image_space.size
+        num_vals = int(np.sum(casted_mask)) + 3 * int(np.sum(uncasted_mask))
+        rp = np.zeros(num_rows + 1, dtype=np.int32)
+        ci = np.zeros(max(num_vals, 1), dtype=np.int32)
+        va = np.zeros(max(num_vals, 1), dtype=np.float64)
+        nnz = C.c_long(0)
+        B.check(B.lib().lt_delaunay_lens_matrix_csr(image_space, source_space, ncasted, B.ptr(self.uncasted_points),
+                                                    B.ptr(uncasted_tri_inds), B.ptr(rp), B.ptr(ci), B.ptr(va),
+                                                    ci.size, C.byref(nnz)))
+        self._mat = sparse.csr_matrix((va[:nnz.value], ci[:nnz.value], rp), shape=(num_rows, source_space.size))
         super().__init__(image_space, source_space, lensmodel)
 
 

@@ -12,8 +12,8 @@ import ctypes as C
 
 import numpy as np
 
-from .backend import (c_bool_p, c_double_p, fourier_space_ctype, image_space_ctype,
-                      visibility_space_ctype)
+from .backend import (c_bool_p, c_double_p, c_int_p, delaunay_space_ctype, fourier_space_ctype,
+                      image_space_ctype, visibility_space_ctype)
 
 ARCSEC_TO_RADIANS = 4.8481368111e-6  # lentils/backend/common.h:7
 
@@ -121,10 +121,26 @@ class ImageSpace(Space, image_space_ctype):
         self._c_mask = self.mask.ctypes.data_as(c_bool_p)
 
 
-class DelaunaySpace(Space):
-    """Adaptive (Delaunay) source planes are outside this backend's scope;
-    the class exists so that isinstance checks of the reference's operator
-    code behave the same."""
+class DelaunaySpace(Space, delaunay_space_ctype):
+    """A triangulated source plane (space_base.py:139-158): the Delaunay
+    triangulation of the given points (scipy.spatial.Delaunay, as the
+    reference), its convex-hull vertices (`edge`), and the ctypes fields of
+    delaunay_space; vectors have shape (1, 1, num_points)."""
 
     def __init__(self, points, channels=(0.,), stokes=('I',)):
-        raise NotImplementedError('Delaunay source spaces are not supported by lentils_amd')
+        from scipy.spatial import Delaunay
+        self.num_channels = 1
+        self.num_stokes = 1
+        self.delaunay = Delaunay(points)
+        self.edge = np.unique(self.delaunay.convex_hull.flatten())
+        self.vertices = np.ascontiguousarray(self.delaunay.points, dtype=np.float64)
+        self._c_points = self.vertices.ctypes.data_as(c_double_p)
+        self.num_points = self.vertices.shape[0]
+        self.triangles = np.ascontiguousarray(self.delaunay.simplices, dtype=np.int32)
+        self._c_triangles = self.triangles.ctypes.data_as(c_int_p)
+        self.num_triangles = self.triangles.shape[0]
+        Space.__init__(self, shape=(self.num_channels, self.num_stokes, self.num_points), dtype=np.float64)
+
+    @property
+    def points(self):
+        return self.vertices

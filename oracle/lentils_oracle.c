@@ -390,6 +390,102 @@ int oracle_deflect_points_deriv(or_lens_model glm, const double *pin, int n, dou
 }
 
 /* ------------------------------------------------------------------------- */
+/* Delaunay source planes (lentils/backend/triangles.c)                      */
+/* ------------------------------------------------------------------------- */
+
+/* barycentric weights of point s in triangle tri (3 x 2) and its signed
+ * doubled area, triangles.c:7-31 */
+static void or_tri_geometry(const double *s, const double *tri, double *w_out, double *area) {
+    double a0 = tri[0] - s[0], a1 = tri[1] - s[1];
+    double b0 = tri[2] - s[0], b1 = tri[3] - s[1];
+    double c0 = tri[4] - s[0], c1 = tri[5] - s[1];
+    double w0 = b0 * c1 - c0 * b1;
+    double w1 = c0 * a1 - a0 * c1;
+    double w2 = a0 * b1 - b0 * a1;
+    *area = w0 + w1 + w2;
+    double inv = 1.0 / (*area);
+    w_out[0] = w0 * inv;
+    w_out[1] = w1 * inv;
+    w_out[2] = w2 * inv;
+}
+
+/* triangles.c:35-62 */
+void oracle_triangle_geometry_from_inds(const double *s, const int *tri_inds, double *w_out, int n, const double *pos) {
+    for (int i = 0; i < n; ++i) {
+        double tri[6], a;
+        for (int v = 0; v < 3; ++v) {
+            tri[2 * v] = pos[2 * tri_inds[3 * i + v]];
+            tri[2 * v + 1] = pos[2 * tri_inds[3 * i + v] + 1];
+        }
+        or_tri_geometry(&s[2 * i], tri, &w_out[3 * i], &a);
+    }
+}
+
+/* gradient rows of the linear interpolant on each triangle, weighted by
+ * sqrt|area| / (2 area): two rows (x, y) of three entries per triangle,
+ * triangles.c:66-103 */
+void oracle_triangle_gradient_csr(int ntri, const double *pts, const int *tris, int *row_inds, int *cols,
+                                  double *vals) {
+    int i = 0, row = 0;
+    for (int t = 0; t < ntri; ++t) {
+        int v0 = tris[3 * t], v1 = tris[3 * t + 1], v2 = tris[3 * t + 2];
+        double p0x = pts[2 * v0], p0y = pts[2 * v0 + 1];
+        double p1x = pts[2 * v1], p1y = pts[2 * v1 + 1];
+        double p2x = pts[2 * v2], p2y = pts[2 * v2 + 1];
+        double area = 0.5 * (p0y * (p1x - p2x) + p1y * p2x - p1x * p2y + p0x * (-p1y + p2y));
+        double weight = 1.0 / (2 * area);
+        weight *= sqrt(fabs(area));
+        row_inds[row++] = i;
+        vals[i] = (-p1y + p2y) * weight; cols[i++] = v0;
+        vals[i] = (p0y - p2y) * weight; cols[i++] = v1;
+        vals[i] = (-p0y + p1y) * weight; cols[i++] = v2;
+        row_inds[row++] = i;
+        vals[i] = (p1x - p2x) * weight; cols[i++] = v0;
+        vals[i] = (-p0x + p2x) * weight; cols[i++] = v1;
+        vals[i] = (p0x - p1x) * weight; cols[i++] = v2;
+    }
+    row_inds[row] = i;
+}
+
+/* lens matrix of a Delaunay source plane, triangles.c:106-145: per image
+ * pixel (x-major) nothing when masked out, one entry 1.0 on the next casted
+ * vertex for pixels with i % ncast == j % ncast == 0, otherwise the
+ * barycentric weights of the pixel's deflected position in its triangle
+ * (nothing when it fell outside the triangulation, tri index < 0) */
+void oracle_delaunay_lens_matrix_csr(or_image_space im, int ntri, const double *pts, const int *tris, int ncast,
+                                     const double *unc_pts, const int *unc_tri, int *row_inds, int *cols,
+                                     double *vals) {
+    int row = 0, i = 0, cast_idx = 0, unc_idx = 0;
+    for (int iim = 0; iim < im.nx; ++iim)
+        for (int jim = 0; jim < im.ny; ++jim, ++row) {
+            row_inds[row] = i;
+            if (!im.mask[row]) continue;
+            if (iim % ncast || jim % ncast) {
+                const double *s = &unc_pts[2 * unc_idx];
+                int t = unc_tri[unc_idx++];
+                if (t < 0) continue;
+                double tri[6], w[3], a;
+                int p[3];
+                for (int v = 0; v < 3; ++v) {
+                    p[v] = tris[3 * t + v];
+                    tri[2 * v] = pts[2 * p[v]];
+                    tri[2 * v + 1] = pts[2 * p[v] + 1];
+                }
+                or_tri_geometry(s, tri, w, &a);
+                for (int v = 0; v < 3; ++v) {
+                    vals[i] = w[v];
+                    cols[i++] = p[v];
+                }
+            } else {
+                vals[i] = 1.0;
+                cols[i++] = cast_idx++;
+            }
+        }
+    row_inds[row] = i;
+    (void)ntri;
+}
+
+/* ------------------------------------------------------------------------- */
 /* rasterization                                                              */
 /* ------------------------------------------------------------------------- */
 

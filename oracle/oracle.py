@@ -49,6 +49,11 @@ class CVisSpace(C.Structure):
                 ('channels', C.c_void_p)]
 
 
+class CDelaunaySpace(C.Structure):
+    _fields_ = [('num_points', C.c_int), ('num_tris', C.c_int), ('nch', C.c_int), ('nst', C.c_int),
+                ('channels', C.c_void_p), ('points', C.c_void_p), ('triangles', C.c_void_p)]
+
+
 class CLensModel(C.Structure):
     _fields_ = [('num_lenses', C.c_int), ('num_zplanes', C.c_int), ('lenses', C.c_void_p)]
 
@@ -100,6 +105,11 @@ def lib():
         L.oracle_deflect_points_deriv.argtypes = [CLensModel, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                   C.c_void_p]
         L.oracle_deflect_points_deriv.restype = C.c_int
+        L.oracle_triangle_gradient_csr.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p]
+        L.oracle_delaunay_lens_matrix_csr.argtypes = [CImageSpace, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_triangle_geometry_from_inds.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.oracle_lens_matrix_csr.argtypes = [CImageSpace, CImageSpace, CLensModel, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.POINTER(C.c_long)]
         L.oracle_lens_matrix_csr.restype = C.c_int
@@ -137,6 +147,13 @@ def ref_lib():
         _ref.manifold_lens_matrix_csr.argtypes = [CImageSpace, CImageSpace, CLensModel, C.c_void_p, C.c_void_p,
                                                   C.c_void_p]
         _ref.manifold_lens_matrix_csr.restype = None
+        _ref.triangle_gradient_csr.argtypes = [CDelaunaySpace, C.c_void_p, C.c_void_p, C.c_void_p]
+        _ref.triangle_gradient_csr.restype = None
+        _ref.delaunay_lens_matrix_csr.argtypes = [CImageSpace, CDelaunaySpace, C.c_int, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p]
+        _ref.delaunay_lens_matrix_csr.restype = None
+        _ref.triangle_geometry_from_inds.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        _ref.triangle_geometry_from_inds.restype = None
     return _ref
 
 
@@ -269,6 +286,51 @@ def ref_lens_matrix_csr(im, src, lenses, capacity_per_row=50):
 # ---------------------------------------------------------------------------
 # NUFFT (lentils/operators/fourier_operators.py, lentils/common/space_base.py)
 # ---------------------------------------------------------------------------
+
+# ---------------------------------------------------------------------------
+# Delaunay source planes (lentils/backend/triangles.c)
+# ---------------------------------------------------------------------------
+
+def _dspace(pts, tris):
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+    tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+    return CDelaunaySpace(pts.shape[0], tris.shape[0], 1, 1, None, _ptr(pts), _ptr(tris)), (pts, tris)
+
+
+def triangle_gradient_csr(pts, tris, use_ref=False):
+    """(row_inds, cols, vals) of the weighted gradient rows, two per triangle."""
+    ds, (p, t) = _dspace(pts, tris)
+    nt = t.shape[0]
+    rp = np.zeros(2 * nt + 1, dtype=np.int32)
+    ci = np.zeros(6 * nt, dtype=np.int32)
+    va = np.zeros(6 * nt)
+    if use_ref:
+        ref_lib().triangle_gradient_csr(ds, _ptr(rp), _ptr(ci), _ptr(va))
+    else:
+        lib().oracle_triangle_gradient_csr(nt, _ptr(p), _ptr(t), _ptr(rp), _ptr(ci), _ptr(va))
+    return rp, ci, va
+
+
+def delaunay_lens_matrix_csr(im, pts, tris, ncast, unc_pts, unc_tri, use_ref=False):
+    """(row_inds, cols, vals) of the Delaunay lens matrix (triangles.c:106-145);
+    unc_pts / unc_tri: the uncasted pixels' deflected positions and their
+    triangle indices (-1 outside), in x-major pixel order."""
+    ds, (p, t) = _dspace(pts, tris)
+    up = np.ascontiguousarray(unc_pts, dtype=np.float64).reshape(-1, 2)
+    ut = np.ascontiguousarray(unc_tri, dtype=np.int32).reshape(-1)
+    n = im.nx * im.ny
+    cap = p.shape[0] + 3 * up.shape[0]
+    rp = np.zeros(n + 1, dtype=np.int32)
+    ci = np.zeros(max(cap, 1), dtype=np.int32)
+    va = np.zeros(max(cap, 1))
+    if use_ref:
+        ref_lib().delaunay_lens_matrix_csr(im.cstruct(), ds, int(ncast), _ptr(up), _ptr(ut), _ptr(rp), _ptr(ci),
+                                           _ptr(va))
+    else:
+        lib().oracle_delaunay_lens_matrix_csr(im.cstruct(), t.shape[0], _ptr(p), _ptr(t), int(ncast), _ptr(up),
+                                              _ptr(ut), _ptr(rp), _ptr(ci), _ptr(va))
+    return rp, ci[:rp[-1]], va[:rp[-1]]
+
 
 def bessel_i0(x):
     return lib().oracle_bessel_i0(float(x))
