@@ -1,0 +1,192 @@
+"""Parity at the benchmark workloads themselves (the inputs bench.py
+times), against the oracle:
+
+* plain GPU workload: 64 c2-law lenses, 512^2 image at 0.01", 128^2
+  sources, 1e7 visibilities on a 1024 x 513 grid (support 3, beta 14.04),
+  one observation per lens sample;
+* precision workload: 1024^2 image at 0.005", 256^2 sources, 5e7
+  visibilities on 2048 x 1025, FP32 operator inside FP64 CG, q up to 0.999
+  and radii down to 1e-4" for the series;
+* batching workload: 256 lens samples with source grids drawn from
+  {32, 48, 64, 96, 128}^2 and a shared 1e6-visibility uv set.
+
+Bars: lens-matrix row pointers and columns identical to the oracle (values
+to the end-to-end bar), deflections 1e-12, FP64 visibilities 1e-12 and
+FP32 1e-5 (max-relative), every FP64 true relative residual <= 1e-6 and the
+oracle's operator confirming it on sampled lenses."""
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from helpers import rng
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+RESID_BAR = 1e-6 * (1 + 1e-6)
+CORES = len(os.sched_getaffinity(0))
+
+
+def _rare_rows(rp, va, vr, nrows):
+    bad = np.abs(va - vr) > 1e-12
+    rows = np.unique(np.searchsorted(rp, np.nonzero(bad)[0], side='right') - 1)
+    return rows.size <= max(2, 1e-3 * nrows)
+
+
+def _oracle_residual(O, im, src, uvw, w, lam, csr, x, dirty, wsup=3, beta=14.04):
+    """||b - A x|| / ||b|| with the oracle's operator L^T N^T C N L + lam Cc^T Cc
+    (all visibilities) and b = L^T dirty."""
+    og = O.Grid2D(im.nx, im.ny, im.bounds)
+    osr = O.Grid2D(src.nx, src.ny, src.bounds)
+    rp, ci, va = csr
+    L = sp.csr_matrix((va, ci, rp), shape=(im.size, src.size))
+    crp, cci, cva = O.curv_matrix(osr)
+    Cm = sp.csr_matrix((cva, cci, crp), shape=(src.size, src.size))
+    geo = O.NUFFTGeometry(og, pad=2, wsup=wsup, beta=beta)
+    vis = O.nufft_forward((L @ x).reshape(im.nx, im.ny), uvw, [1.0], geo, nthreads=CORES).reshape(-1) * w
+    Ax = L.T @ O.nufft_adjoint(vis.reshape(1, -1), uvw, [1.0], geo, nthreads=CORES).reshape(-1) + \
+        lam * (Cm.T @ (Cm @ x))
+    b = L.T @ np.asarray(dirty).reshape(-1)
+    return np.linalg.norm(b - Ax) / np.linalg.norm(b)
+
+
+def test_plain_gpu_workload(hip, oracle):
+    import bench
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    wl = bench.Workload()
+    og = O.Grid2D(512, 512, wl.im.bounds)
+    osr = O.Grid2D(128, 128, wl.src.bounds)
+    lb = LensBatch(wl.im, wl.srcs, wl.lenses, z_src=2.0)
+    for s in (0, 1):
+        rp, ci, va = lb.csr(s)
+        wl.lenses[s].setup_raytracing(2.0)
+        orp, oci, ova, _ = O.lens_matrix_csr(og, osr, wl.lenses[s].lenses)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+        assert _rare_rows(rp, va, ova, 512 * 512)
+    # NUFFT on the full 1e7-visibility plan against the oracle on a subset
+    k = 200_000
+    geo = O.NUFFTGeometry(og, pad=2, wsup=3, beta=14.04)
+    img = rng(90).normal(size=(512, 512))
+    v64 = wl.plan.forward(img).reshape(-1)
+    vo = O.nufft_forward(img, wl.uvw[:k], [1.0], geo, nthreads=CORES).reshape(-1)
+    assert O.max_relative_error(v64[:k], vo) < 1e-12
+    p32 = NUFFTPlan(wl.vs, wl.im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f32')
+    assert O.max_relative_error(p32.forward(img).reshape(-1)[:k], vo) < 1e-5
+    vis = np.zeros(wl.nvis, dtype=np.complex128)
+    vis[:k] = rng(91).normal(size=(k, 2)).view(np.complex128).reshape(-1)
+    ao = O.nufft_adjoint(vis[:k].reshape(1, -1), wl.uvw[:k], [1.0], geo, nthreads=CORES)
+    assert O.max_relative_error(wl.plan.adjoint(vis).reshape(512, 512), ao) < 1e-12
+    assert O.max_relative_error(p32.adjoint(vis).reshape(512, 512), ao) < 1e-5
+    del p32
+    # the benchmarked solve: every sample to the FP64 true residual, two
+    # confirmed by the oracle's operator over all 1e7 visibilities
+    inv = BatchedSourceInversion(lb, wl.plan, dirty_images=wl.ddirty, lambda_curvature=wl.lambda_curv)
+    xs, its, res = inv.solve(rtol=1e-6, maxiter=20000)
+    print(f'plain workload: iterations median {np.median(its)}, max {its.max()}, max residual {res.max():.3e}')
+    assert np.all(res <= 1e-6)
+    for s in (0, 37):
+        r = _oracle_residual(O, wl.im, wl.src, wl.uvw, wl.weights, wl.lambda_curv, lb.csr(s), xs[s].reshape(-1),
+                             wl.dirty[s])
+        print(f'sample {s}: GPU residual {res[s]:.4e}, oracle residual {r:.4e}')
+        assert r <= RESID_BAR
+
+
+def test_batching_workload(hip, oracle):
+    import bench
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch
+    O = oracle
+    nlens = 256
+    sizes = list(np.random.Generator(np.random.PCG64(15)).choice([32, 48, 64, 96, 128], size=nlens))
+    wl = bench.Workload(nvis=1_000_000, nlens=nlens, src_sizes=sizes)
+    lb = LensBatch(wl.im, wl.srcs, wl.lenses, z_src=2.0)
+    og = O.Grid2D(512, 512, wl.im.bounds)
+    for s in (0, 1):
+        rp, ci, va = lb.csr(s)
+        wl.lenses[s].setup_raytracing(2.0)
+        src = wl.srcs[s]
+        orp, oci, ova, _ = O.lens_matrix_csr(og, O.Grid2D(src.nx, src.ny, src.bounds), wl.lenses[s].lenses)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    inv = BatchedSourceInversion(lb, wl.plan, dirty_images=wl.ddirty, lambda_curvature=wl.lambda_curv)
+    xs, its, res = inv.solve(rtol=1e-6, maxiter=20000)
+    print(f'batching workload: iterations median {np.median(its)}, max {its.max()}, max residual {res.max():.3e}')
+    assert np.all(res <= 1e-6)
+    picks = [sizes.index(n) for n in (32, 64, 128)]
+    for s in picks:
+        r = _oracle_residual(O, wl.im, wl.srcs[s], wl.uvw, wl.weights, wl.lambda_curv, lb.csr(s),
+                             xs[s].reshape(-1), wl.dirty[s])
+        print(f'sample {s} ({sizes[s]}^2): GPU residual {res[s]:.4e}, oracle residual {r:.4e}')
+        assert r <= RESID_BAR
+
+
+def test_precision_workload(hip, oracle):
+    """Series convergence for q up to 0.999 and radii down to 1e-4"
+    (deflections 1e-12); FP32 visibilities over all 5e7 against the FP64
+    plan and against the oracle on a subset (1e-5); the precision form's
+    solutions at the FP64 true residual, one confirmed by the oracle."""
+    import ctypes as C
+    import bench
+    from lentils_amd import ImageSpace, VisibilitySpace
+    from lentils_amd import backend as B
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    g = np.random.Generator(np.random.PCG64(21))
+    for q in (0.5, 0.9, 0.99, 0.999):
+        for gam in (1.8, 2.2):
+            lens = [O.pack_pemd(b=g.uniform(0.5, 1.5), q=q, th=g.uniform(0, 180), qh=(gam - 1) / 2, x=0.013,
+                                y=-0.007), O.pack_shear(ss=0.03, sa=g.uniform(0, 180))]
+            r = np.exp(g.uniform(np.log(1e-4), np.log(3.0), 20000))
+            ph = g.uniform(0, 2 * np.pi, 20000)
+            pts = np.stack([0.013 + r * np.cos(ph), -0.007 + r * np.sin(ph)], 1)
+            arr = np.array(lens, dtype=O.MASS_MODEL_DTYPE)
+            glm = B.global_lens_model_ctype(2, 1, arr.ctypes.data_as(C.c_void_p))
+            got = np.zeros_like(pts)
+            B.check(B.lib().lt_deflect_points(C.byref(glm), B.ptr(pts), pts.shape[0], B.ptr(got)))
+            ref = O.deflect_points(lens, pts)
+            e = np.max(np.linalg.norm((pts - got) - (pts - ref), axis=1) / np.linalg.norm(pts - ref, axis=1))
+            assert e < 1e-12, (q, gam, e)
+    im = ImageSpace(shape=(1024, 1024), bounds=[(-2.56, 2.56), (-2.56, 2.56)])
+    src = ImageSpace(shape=(256, 256), bounds=[(-0.64, 0.64), (-0.64, 0.64)])
+    nvis = 50_000_000
+    uvw = bench.gen_uv(nvis, seed=16)
+    vs = VisibilitySpace(channels=[1.0], uvcoords=uvw)
+    lenses = bench.gen_lenses(4, seed=17)
+    lb = LensBatch(im, [src] * 4, lenses, z_src=2.0)
+    p64 = NUFFTPlan(vs, im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f64')
+    p32 = NUFFTPlan(vs, im, pad_factor=2, kernel_support=3, kb_beta=14.04, precision='f32')
+    imgs = []
+    for s in range(4):
+        rp, ci, va = lb.csr(s)
+        L = sp.csr_matrix((va, ci, rp), shape=(im.size, src.size))
+        imgs.append(L @ bench.blob_source(src, seed=30 + s).reshape(-1))
+    v64 = p64.forward(imgs[0])
+    v32 = p32.forward(imgs[0])
+    assert O.max_relative_error(v32, v64) < 1e-5
+    k = 200_000
+    og = O.Grid2D(1024, 1024, im.bounds)
+    geo = O.NUFFTGeometry(og, pad=2, wsup=3, beta=14.04)
+    vo = O.nufft_forward(imgs[0].reshape(1024, 1024), uvw[:k], [1.0], geo, nthreads=CORES).reshape(-1)
+    assert O.max_relative_error(v32.reshape(-1)[:k], vo) < 1e-5
+    assert O.max_relative_error(v64.reshape(-1)[:k], vo) < 1e-12
+    del p32, v32
+    gn = np.random.Generator(np.random.PCG64(19))
+    w = np.full(nvis, 1e6)
+    dirty = []
+    for s in range(4):
+        vis = p64.forward(imgs[s]).astype(np.complex64)
+        vis.real += gn.standard_normal(nvis, dtype=np.float32) * np.float32(1e-3)
+        vis.imag += gn.standard_normal(nvis, dtype=np.float32) * np.float32(1e-3)
+        dirty.append(p64.adjoint(w * vis.astype(np.complex128)))
+    inv = BatchedSourceInversion(lb, p64, weights=w, dirty_images=dirty, lambda_curvature=1e2)
+    xs, its, res = inv.solve(rtol=1e-6, maxiter=20000, precision='f32')
+    print(f'precision form: iterations {its}, residuals {res}')
+    assert np.all(res <= 1e-6)
+    r = _oracle_residual(O, im, src, uvw, w, 1e2, lb.csr(0), xs[0].reshape(-1), dirty[0])
+    print(f'sample 0: oracle residual {r:.4e}')
+    assert r <= RESID_BAR
