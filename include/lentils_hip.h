@@ -98,9 +98,9 @@ void manifold_lens_matrix_csr(image_space imspace, image_space srcspace, global_
 /* Replaces grid_cpu, lentils/backend/nufft.c:42-122 (bound at
  * load_backend.py:95-97).  direction 0: vis += degrid(grid);
  * direction 1: grid += grid(vis).  vis and grid are complex128 arrays.
- * Requires every kernel tap inside the Nyquist band (the check
- * GriddingOperator performs, fourier_operators.py:77-80); otherwise the
- * outputs are left untouched and an error is recorded. */
+ * Taps beyond the band wrap periodically as in the reference (nufft.c:83-86);
+ * such uv sets take a direct per-visibility pass (FP64 atomics when
+ * gridding) instead of the tiled, atomic-free gridder. */
 void grid_cpu(visibility_space visspace, double *vis, fourier_space gspace, double *grid, int wsup,
               double kb_beta, int direction);
 

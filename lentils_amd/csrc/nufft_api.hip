@@ -43,6 +43,114 @@ struct lt_nufft_s {
 };
 
 namespace lt {
+namespace {
+
+// I0 by its power series and the separable KB kernel, nufft.c:11-21 (the
+// reference evaluates I0 with GSL)
+__device__ double wrap_i0(double x) {
+    const double q = 0.25 * x * x;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 500; ++k) {
+        term *= q / ((double)k * (double)k);
+        sum += term;
+        if (term < 1e-17 * sum) break;
+    }
+    return sum;
+}
+
+__device__ double wrap_kb(double ak, double aq, double beta, double w) {
+    if (!(ak >= -w && ak <= w && aq >= -w && aq <= w)) return 0.0;
+    double sk = 1.0 - (ak / w) * (ak / w);
+    double sq = 1.0 - (aq / w) * (aq / w);
+    if (sk < 0.0) sk = 0.0;
+    if (sq < 0.0) sq = 0.0;
+    return 1.0 / (4.0 * w * w) * wrap_i0(beta * sqrt(sk)) * wrap_i0(beta * sqrt(sq));
+}
+
+struct WrapGeo {
+    int nu, nv, half_nv, nrows, nch, wsup;
+    double du, dv, gcx, gcy, beta;
+};
+
+// grid_cpu's loop for one visibility (nufft.c:54-117), taps wrapped
+// periodically: direction 0 adds into this visibility (its taps in order),
+// direction 1 adds into the grid with FP64 atomics (the order of additions
+// from different visibilities varies, as in the reference's OpenMP loop)
+__global__ void k_grid_wrap(WrapGeo G, const double *__restrict__ uv, const double *__restrict__ chan,
+                            int direction, double *__restrict__ vis, double *__restrict__ grid) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)G.nrows * G.nch) return;
+    const int ch = (int)(e / G.nrows);
+    const long i = e - (long)ch * G.nrows;
+    const double u = uv[3 * i] * chan[ch];
+    const double v = -uv[3 * i + 1] * chan[ch];
+    const double ph = 6.28318530718 * (u * G.gcx + v * G.gcy);
+    const double cosp = cos(ph), sinp = sin(ph);
+    const double tu = u / G.du, tv = v / G.dv;
+    if (!(fabs(tu) < 1e9 && fabs(tv) < 1e9)) return;
+    const int ngu = (int)round(tu), ngv = (int)round(tv);
+    double vr = direction == 0 ? vis[2 * e] : 0.0, vi = direction == 0 ? vis[2 * e + 1] : 0.0;
+    const double xr = direction == 1 ? vis[2 * e] : 0.0, xi = direction == 1 ? vis[2 * e + 1] : 0.0;
+    for (int a = ngu - G.wsup; a <= ngu + G.wsup; ++a)
+        for (int b = ngv - G.wsup; b <= ngv + G.wsup; ++b) {
+            const double kern = wrap_kb(tu - a, tv - b, G.beta, (double)G.wsup);
+            const double pc = (double)(1 - 2 * ((a + b) & 1));
+            double wr = 0.5 * kern * pc, wi = 0.5 * kern * pc;
+            int iu = ((a % G.nu) + G.nu) % G.nu, iv = ((b % G.nv) + G.nv) % G.nv;
+            if (iv >= G.half_nv) {
+                iu = (G.nu - iu) % G.nu;
+                iv = G.nv - iv;
+                wi *= -1;
+            }
+            const long f = (long)G.half_nv * iu + iv;
+            if (direction == 0) {
+                const double tr = wr * grid[2 * f], ti = wi * grid[2 * f + 1];
+                vr += 2 * (cosp * tr + sinp * ti);
+                vi += 2 * (-sinp * tr + cosp * ti);
+            } else {
+                if (iv == 0) {
+                    wr *= 2;
+                    wi *= 2;
+                }
+                atomicAdd(&grid[2 * f], wr * (cosp * xr - sinp * xi));
+                atomicAdd(&grid[2 * f + 1], wi * (sinp * xr + cosp * xi));
+            }
+        }
+    if (direction == 0) {
+        vis[2 * e] = vr;
+        vis[2 * e + 1] = vi;
+    }
+}
+
+}  // namespace
+
+lt_status grid_wrap(const visibility_space &vs, double *vis, const fourier_space &gs, double *grid, int wsup,
+                    double kb_beta, int direction) {
+    LT_TRY(ensure_init());
+    if (vs.nrows < 0 || vs.nchannels <= 0 || gs.nu <= 0 || gs.nv <= 0 || gs.half_nv != gs.nv / 2 + 1 || wsup < 1 ||
+        (direction != 0 && direction != 1)) {
+        set_error("grid_cpu: bad arguments");
+        return LT_ERR_ARG;
+    }
+    const long nvis = (long)vs.nrows * vs.nchannels, ng = 2L * gs.nu * gs.half_nv;
+    DevBuf<double> duv, dch, dv, dg;
+    LT_TRY(duv.upload(vs.uv, std::max(3L * vs.nrows, 1L)));
+    LT_TRY(dch.upload(vs.channels, vs.nchannels));
+    LT_TRY(dv.upload(vis, std::max(2 * nvis, 1L)));
+    LT_TRY(dg.upload(grid, ng));
+    const WrapGeo G{gs.nu, gs.nv, gs.half_nv, vs.nrows, vs.nchannels, wsup, gs.du, gs.dv, gs.gcx, gs.gcy, kb_beta};
+    if (nvis > 0) {
+        hipLaunchKernelGGL(k_grid_wrap, dim3(blocks_for(nvis, 128)), dim3(128), 0, stream(), G, duv.p, dch.p,
+                           direction, dv.p, dg.p);
+        LT_CHECK_LAUNCH();
+    }
+    if (direction == 0)
+        LT_TRY(dv.download(vis, 2 * nvis));
+    else
+        LT_TRY(dg.download(grid, ng));
+    return sync();
+}
+
 NufftPlan *nufft_of(lt_nufft h) { return h ? &h->plan : nullptr; }
 lt_status stencil_time(NufftPlan &P, int nb, int reps, double *ms, double *bytes, double *flops, long *rows,
                        long *win_cells);
@@ -183,6 +291,12 @@ void grid_cpu(visibility_space vs, double *vis, fourier_space gs, double *grid, 
               int direction) {
     NufftPlan P;
     lt_status s = P.create(vs, gs, wsup, kb_beta, LT_F64);
+    if (s == LT_ERR_NYQUIST) {
+        // taps beyond the band: the reference wraps them (nufft.c:83-86)
+        s = grid_wrap(vs, vis, gs, grid, wsup, kb_beta, direction);
+        if (s != LT_OK) fprintf(stderr, "lentils_hip: grid_cpu: %s\n", last_error());
+        return;
+    }
     if (s == LT_OK) {
         const long ng = 2L * P.nu * P.half_nv;
         DevBuf<double> g, v;

@@ -168,3 +168,28 @@ def test_fft_operator(hip, oracle):
         y = op.T * g
         refT = np.fft.irfft2(g, s=(nx, ny), norm='forward')
         assert O.max_relative_error(y.reshape(nx, ny), refT) < 1e-13
+
+
+def test_grid_cpu_wraps_beyond_the_band(hip, oracle):
+    """grid_cpu on visibilities whose taps cross the grid's edge: the
+    reference wraps them periodically (nufft.c:83-86), so does the drop-in
+    (a direct per-visibility pass when the tiled gridder's band check
+    refuses the uv set); both directions against the oracle."""
+    from lentils_amd import FourierSpace, ImageSpace, VisibilitySpace
+    B, O = hip, oracle
+    uv = annulus_uv(6000, r1=4.4e6, r2=5.6e6, seed=68)
+    vs = VisibilitySpace(channels=[1.0], uvcoords=uv)
+    fs = FourierSpace(ImageSpace(shape=(64, 64), bounds=[(-0.64, 0.64), (-0.64, 0.64)]))
+    geo = O.NUFFTGeometry(O.Grid2D(32, 32, [-0.32, 0.32, -0.32, 0.32]), pad=2, wsup=3, beta=14.04)
+    assert np.max(np.abs(uv[:, :2])) / geo.du + 3 + 1 > 32  # the Nyquist guard would refuse these
+    g = rng(69)
+    grid = np.ascontiguousarray(g.normal(size=(64, 33, 2)).view(np.complex128)[..., 0])
+    vis = g.normal(size=(6000, 2)).view(np.complex128).reshape(-1).copy()
+    vis0 = vis.copy()
+    B.lib().grid_cpu(vs, B.ptr(vis), fs, B.ptr(grid), 3, 14.04, 0)
+    ref = vis0 + O.grid(uv, [1.0], None, geo, 0, grid_in=grid).reshape(-1)
+    assert O.max_relative_error(vis, ref) < 1e-12
+    grid0 = grid.copy()
+    B.lib().grid_cpu(vs, B.ptr(vis0), fs, B.ptr(grid), 3, 14.04, 1)
+    refg = grid0 + O.grid(uv, [1.0], vis0.reshape(1, -1), geo, 1)
+    assert O.max_relative_error(grid, refg) < 1e-12
