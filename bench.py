@@ -124,9 +124,10 @@ class Workload:
         self.ddirty = DeviceArray(np.stack(self.dirty))
         log(f'resident weights + stencil + dirty images {time.perf_counter() - t:.2f}s')
 
-    def step(self, rtol=1e-6, maxiter=20000, operator='nufft', precond='none', precision='f64'):
+    def step(self, rtol=1e-6, maxiter=20000, operator='nufft', precond='none', precision='f64', net=None):
         """One pass of the hot path: lens matrices of the batch, solver set-up
-        and batched CG, all from device-resident inputs."""
+        and batched CG, all from device-resident inputs (net: the trained
+        network of precond='learned')."""
         from lentils_amd.solvers import BatchedSourceInversion, LensBatch
         lb = LensBatch(self.im, self.srcs, self.lenses, z_src=2.0)
         if operator == 'bcb':  # the dirty-beam operator derives its beam from host weights
@@ -136,7 +137,11 @@ class Workload:
             inv = BatchedSourceInversion(lb, self.plan, dirty_images=self.ddirty,
                                          lambda_curvature=self.lambda_curv, operator=operator)
         inv.set_graphs(GRAPHS)
+        if precond == 'learned':
+            inv.set_preconditioner(net)
         x, its, res = inv.solve(rtol=rtol, maxiter=maxiter, precond=precond, precision=precision)
+        if precond == 'learned':
+            self.last_fallbacks = inv.fallbacks()
         return its, res
 
 
@@ -290,6 +295,91 @@ def batching_form(args):
            'lens_preconditioner': {'value': nlens / tl, 'ms_per_step': 1e3 * tl, 'iters_median': float(np.median(itsl)),
                                    'iters_max': int(itsl.max()), 'true_rel_resid_max': float(resl.max()),
                                    'all_converged': bool(np.all(resl <= 1e-6))}}
+    print(json.dumps(out), flush=True)
+
+
+def learned_form(args):
+    """Learned-preconditioner workload: a batch of `nlens` (256) lens samples
+    with 128^2 source grids and the 1e7-visibility uv set (one observation per
+    sample, as the plain workload), solved to a 1e-6 FP64 true residual by
+    plain CG, by the lens-structured preconditioner M alone and by M with the
+    network's correction (flexible CG).  The network is first trained on the
+    GPU: over `train_batches` batches of random lenses of the same family
+    (seeds apart from the evaluation batch) with `draws` random blob-source
+    draws each, the solver generates the right-hand sides b = A x and the
+    residuals of M-preconditioned solves of those b stopped after 4, 16 and
+    48 iterations; `epochs` passes of Adam on the energy loss over them.
+    Prints one JSON line."""
+    from lentils_amd import backend as B
+    from lentils_amd.learned import PCNet, blob_sources, solver_residuals
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch
+    B.check(B.lib().lt_init(0))
+    nlens = args.learned_samples
+    t = time.perf_counter()
+    wl = Workload(nlens=nlens)
+    log(f'learned workload setup {time.perf_counter() - t:.1f}s')
+    net = PCNet(128, 128, batch=nlens, seed=3)
+    iters = (0, 4, 16, 48)
+    rng = np.random.default_rng(21)
+    t_gen = t_train = 0.0
+    npairs = 0
+    hist = []
+    for tb in range(args.train_batches):
+        t = time.perf_counter()
+        lb = LensBatch(wl.im, wl.srcs, gen_lenses(nlens, seed=1000 + tb), z_src=2.0)
+        inv = BatchedSourceInversion(lb, wl.plan, dirty_images=wl.ddirty, lambda_curvature=wl.lambda_curv)
+        xs = blob_sources(wl.src, nlens * args.draws, rng)
+        draws = [list(xs[d * nlens:(d + 1) * nlens]) for d in range(args.draws)]
+        vecs = solver_residuals(inv, draws, iterations=iters, precond='lens')
+        t_gen += time.perf_counter() - t
+        t = time.perf_counter()
+        h = inv.train_preconditioner(net, vecs, epochs=args.epochs, lr=args.lr, lr_final=args.lr * 0.1,
+                                     seed=tb)
+        t_train += time.perf_counter() - t
+        npairs += len(vecs) * nlens
+        hist.append([float(v) for v in h])
+        log(f'training batch {tb}: {len(vecs)} vectors x {nlens} samples, loss {h[0]:.4f} -> {h[-1]:.4f} '
+            f'(generation {t_gen:.1f}s, training {t_train:.1f}s so far)')
+        del inv, lb, vecs
+    # held-out loss of M alone (zero network) vs the trained correction
+    results = {}
+    for name, pc in (('plain', 'none'), ('lens', 'lens'), ('learned', 'learned')):
+        for k in range(args.warmup):
+            wl.step(maxiter=args.maxiter, precond=pc, net=net)
+        ts, all_its, all_res = [], [], []
+        for k in range(args.steps):
+            t = time.perf_counter()
+            its, res = wl.step(maxiter=args.maxiter, precond=pc, net=net)
+            ts.append(time.perf_counter() - t)
+            all_its.append(its)
+            all_res.append(res)
+        its, res = np.concatenate(all_its), np.concatenate(all_res)
+        tot = sum(ts)
+        results[name] = {'value': nlens * args.steps / tot, 'unit': 'source solves/s',
+                         'ms_per_step': 1e3 * tot / args.steps, 'iters_median': float(np.median(its)),
+                         'iters_max': int(its.max()), 'true_rel_resid_max': float(res.max()),
+                         'all_converged': bool(np.all(res <= 1e-6))}
+        if name == 'learned':
+            results[name]['guard_fallback_iters'] = int(wl.last_fallbacks[0].sum())
+            results[name]['samples_switched_to_M'] = int(wl.last_fallbacks[1].sum())
+        log(f'{name}: {tot / args.steps:.2f}s/step iters median {np.median(its):.0f} max {its.max()} '
+            f'max res {res.max():.2e}')
+    out = {'metric': 'source solves/s at 1e-6 rel resid (learned-PCG vs plain CG); Gvis/s degrid',
+           'value': results['learned']['value'], 'unit': 'source solves/s', 'n_gpus': 1, 'steps': args.steps,
+           'warmup': args.warmup, 'ms_per_step': results['learned']['ms_per_step'], 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64 (network: bf16 MFMA, f32 accumulate)',
+           'data': 'synthetic: seeded lenses and uv annulus; one observation per lens sample, complex64 vis + '
+                   'noise 1e-3; training pairs from random lenses and blob sources',
+           'config': {'workload': f'learned-preconditioner form: {nlens} lens samples, 512^2 image @0.01", 128^2 '
+                                  'source, 1e7 vis, 1024x513 grid, W=6, FP64 flexible CG to 1e-6',
+                      'parallelism': 'single GPU'},
+           'learned': results['learned'], 'plain_cg': results['plain'], 'lens_preconditioner': results['lens'],
+           'speedup_vs_plain': results['learned']['value'] / results['plain']['value'],
+           'speedup_vs_lens': results['learned']['value'] / results['lens']['value'],
+           'training': {'lens_batches': args.train_batches, 'draws_per_batch': args.draws,
+                        'truncation_iters': list(iters), 'pairs': npairs, 'epochs': args.epochs,
+                        'pair_visits': npairs * args.epochs, 'lr': args.lr, 'generation_s': t_gen,
+                        'training_s': t_train, 'loss_history': hist}}
     print(json.dumps(out), flush=True)
 
 
@@ -528,6 +618,13 @@ def main():
                     help='only run the hot kernels once each batch and the calibration copy (for rocprofv3 --pmc)')
     ap.add_argument('--no-graphs', action='store_true',
                     help='launch the CG kernels one by one instead of as captured HIP graphs (for kernel tracing)')
+    ap.add_argument('--learned', action='store_true',
+                    help='the learned-preconditioner workload (256 samples, network trained first) instead')
+    ap.add_argument('--learned-samples', type=int, default=256)
+    ap.add_argument('--train-batches', type=int, default=4)
+    ap.add_argument('--draws', type=int, default=8)
+    ap.add_argument('--epochs', type=int, default=3)
+    ap.add_argument('--lr', type=float, default=1e-3)
     ap.add_argument('--shared-data', action='store_true',
                     help='one observation for all samples (most samples then fit the data with the wrong lens)')
     args = ap.parse_args()
@@ -541,6 +638,9 @@ def main():
         return
     if args.batching:
         batching_form(args)
+        return
+    if args.learned:
+        learned_form(args)
         return
     if args.precision_report:
         precision_report(args)

@@ -348,7 +348,7 @@ lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, do
 #define LT_OP_NUFFT_EXPLICIT 2 /* exact D^T C D, per-visibility degrid-weight-grid each apply */
 #define LT_PC_NONE 0
 #define LT_PC_JACOBI 1
-#define LT_PC_LEARNED 2 /* learned network (lt_solver_set_pcnet) in flexible CG, Jacobi as its guard */
+#define LT_PC_LEARNED 2 /* lens-structured M (as LT_PC_LENS) plus the network's correction (lt_solver_set_pcnet) */
 #define LT_PC_LENS 3    /* M = lambda_c Cc^T Cc + c L_s^T L_s (the reference's R + w L^T L preconditioner,
                            test_all.py:63-65; L_s: L without its longest rows), built explicitly and
                            applied by a few Jacobi-PCG steps on M inside flexible CG
@@ -436,21 +436,27 @@ lt_status lt_pcnet_destroy(lt_pcnet net);
  * network's H x W and the batch within its capacity) */
 lt_status lt_solver_set_pcnet(lt_solver s, lt_pcnet net);
 /* Train the network (epochs > 0) or evaluate it (epochs = 0) with this
- * solver's operators A_s: residuals holds nres vectors of all samples'
- * sources concatenated (host, FP64), e.g. right-hand sides and residuals of
- * truncated solves; each step takes one vector per sample and descends the
- * energy loss mean_s (z^T A z - 2 z^T r) / (r^T r / mean diag), which is the
- * A-norm error ||z - A^{-1} r||_A^2 up to a constant.  loss_hist[epoch]: mean
- * loss (minimum -r^T A^{-1} r / c; the scaled identity z = r / mean diag
- * gives r^T A r / (mean diag r^T r) - 2). */
+ * solver's operators A_s.  The preconditioner it trains is
+ *   z = m + y(r / rms(r), m / rms(m)) rms(m),  m = M^-1 r
+ * with M the lens-structured preconditioner of LT_PC_LENS, so an untrained
+ * (zero-output) network is exactly LT_PC_LENS.  residuals holds nres vectors
+ * of all samples' sources concatenated (host, FP64), e.g. right-hand sides
+ * A x of known sources x and the residuals of truncated solves (each such
+ * r is the right-hand side of the pair (r, A^-1 r)); each step takes one
+ * vector per sample and descends the energy loss
+ *   mean_s (z^T A z - 2 z^T r) / (r^T m),
+ * the A-norm error ||z - A^-1 r||_A^2 up to a constant.  loss_hist[epoch]:
+ * mean loss (for z = m it is (m^T A m - 2 r^T m) / r^T m; its floor, at
+ * z = A^-1 r, is -r^T A^-1 r / r^T m). */
 lt_status lt_solver_train_pcnet(lt_solver s, const double *residuals, int nres, int epochs, float lr, float lr_final,
                                 unsigned seed, double *loss_hist);
 /* the diagonal of A by 8 x 8 probing (all samples concatenated) */
 lt_status lt_solver_diag(lt_solver s, double *diag);
 /* per sample, for the last LT_PC_LEARNED solve: counts = iterations in which
- * the network's direction was rejected (r.z <= 0) and the Jacobi one taken;
- * plain = 1 if the sample left the network for plain CG (its residual norm
- * fell less than 10x in 100 iterations) (may be NULL) */
+ * the corrected direction was rejected (r.z <= 0 or not finite) and m = M^-1 r
+ * taken; plain = 1 if the sample switched its network off for the rest of
+ * the solve (its residual norm fell less than 10x in 100 iterations) and
+ * continued with M alone (may be NULL) */
 lt_status lt_solver_pc_fallbacks(lt_solver s, int *counts, int *plain);
 
 #ifdef __cplusplus

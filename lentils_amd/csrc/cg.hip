@@ -298,62 +298,66 @@ __global__ void k_fill(long n, double v, double *__restrict__ a) {
     if (i < n) a[i] = v;
 }
 
-// network input of the learned preconditioner: (r / rms(r), diag / mean diag)
-// per sample, FP32 [s][2][HW]
+// Learned preconditioner: a correction by the network on top of the
+// lens-structured preconditioner, z = m + y rms(m) with m ~ M^-1 r.  Input
+// per sample (FP32 [s][2][HW]): r / rms(r) and m / rms(m); rr = r.r and
+// mm = m.m per sample.
 __global__ void k_pc_in(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
-                        const double *__restrict__ r, const double *__restrict__ rr, const float *__restrict__ dhat,
-                        int HW, float *__restrict__ x0) {
+                        const double *__restrict__ r, const double *__restrict__ rr, const double *__restrict__ m,
+                        const double *__restrict__ mm, int HW, float *__restrict__ x0) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = seg[i];
     const long pix = i - off[s];
     const double N = (double)(off[s + 1] - off[s]);
-    const double rms = sqrt(rr[s] / N);
-    x0[((long)s * 2) * HW + pix] = rms > 0.0 ? (float)(r[i] / rms) : 0.f;
-    x0[((long)s * 2 + 1) * HW + pix] = dhat[i];
+    const double rs = sqrt(rr[s] / N), ms = sqrt(mm[s] / N);
+    x0[((long)s * 2) * HW + pix] = rs > 0.0 ? (float)(r[i] / rs) : 0.f;
+    x0[((long)s * 2 + 1) * HW + pix] = ms > 0.0 ? (float)(m[i] / ms) : 0.f;
 }
 
-// z = y rms(r) / mean diag
+// z = m + y rms(m)
 __global__ void k_pc_out(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
-                         const float *__restrict__ y, const double *__restrict__ rr, const double *__restrict__ ainv,
+                         const float *__restrict__ y, const double *__restrict__ m, const double *__restrict__ mm,
                          int HW, double *__restrict__ z) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = seg[i];
     const long pix = i - off[s];
     const double N = (double)(off[s + 1] - off[s]);
-    z[i] = (double)y[(long)s * HW + pix] * sqrt(rr[s] / N) * ainv[s];
+    z[i] = m[i] + (double)y[(long)s * HW + pix] * sqrt(mm[s] / N);
 }
 
-// samples whose network direction is not a descent direction (r.z <= 0 or
-// not finite) take the Jacobi direction this iteration
-__global__ void k_pc_guard(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
-                           const double *__restrict__ dinv, const double *__restrict__ r, double *__restrict__ z,
+// active samples (act == NULL: all) whose corrected direction is not a
+// descent direction (r.z <= 0 or not finite, e.g. a network that produced
+// NaN) take the lens-structured direction m this iteration
+__global__ void k_pc_guard(long n, const int32_t *__restrict__ seg, const int *__restrict__ act,
+                           const double *__restrict__ rz, const double *__restrict__ m, double *__restrict__ z,
                            int *__restrict__ nfall) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = seg[i];
+    if (act && !act[s]) return;
     if (!(rz[s] > 0.0) || !isfinite(rz[s])) {
-        z[i] = dinv[i] * r[i];
+        z[i] = m[i];
         if (i == 0 || seg[i - 1] != s) atomicAdd(&nfall[s], 1);
     }
 }
 
 // Gradient of the energy loss of the learned preconditioner,
-//   L = (1/S) sum_s (z_s^T A z_s - 2 z_s^T r_s) / c_s,  c_s = r_s^T r_s / mean diag_s
-// (the A-norm error ||z - A^{-1} r||_A^2 up to a constant), with respect to
-// the network output y (z = y rms(r) / mean diag).
+//   L = (1/S) sum_s (z_s^T A z_s - 2 z_s^T r_s) / c_s,  c_s = r_s^T m_s
+// (the A-norm error ||z - A^{-1} r||_A^2 up to a constant; m ~ M^-1 r), with
+// respect to the network output y (z = m + y rms(m)).
 __global__ void k_energy_grad(long n, int S, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
                               const double *__restrict__ Az, const double *__restrict__ r,
-                              const double *__restrict__ rr, const double *__restrict__ ainv, int HW,
+                              const double *__restrict__ mm, const double *__restrict__ rm, int HW,
                               float *__restrict__ gy) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = seg[i];
     const long pix = i - off[s];
     const double N = (double)(off[s + 1] - off[s]);
-    const double c = rr[s] * ainv[s];
-    const double scale = c > 0.0 ? 2.0 * sqrt(rr[s] / N) * ainv[s] / (c * S) : 0.0;
+    const double c = rm[s];
+    const double scale = c > 0.0 ? 2.0 * sqrt(mm[s] / N) / (c * S) : 0.0;
     gy[(long)s * HW + pix] = (float)((Az[i] - r[i]) * scale);
 }
 
@@ -550,33 +554,11 @@ __global__ void k_inner_p(long n, const int32_t *__restrict__ seg, const double 
 }
 
 // samples that left the network (plain CG fallback) take z = r
-__global__ void k_pc_plain(long n, const int32_t *__restrict__ seg, const int *__restrict__ usenet,
-                           const double *__restrict__ r, double *__restrict__ z) {
+// samples whose network was switched off take the lens-structured direction
+__global__ void k_pc_lensonly(long n, const int32_t *__restrict__ seg, const int *__restrict__ usenet,
+                              const double *__restrict__ m, double *__restrict__ z) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && !usenet[seg[i]]) z[i] = r[i];
-}
-
-// restart of the samples switched to plain CG: p = r, r.z = r.r
-__global__ void k_restart(long n, const int32_t *__restrict__ seg, const int *__restrict__ flag,
-                          const double *__restrict__ r, double *__restrict__ p) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flag[seg[i]]) p[i] = r[i];
-}
-
-__global__ void k_restart_rz(int ns, const int *__restrict__ flag, const double *__restrict__ rr,
-                             double *__restrict__ rz) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < ns && flag[s]) rz[s] = rr[s];
-}
-
-// normalised diagonal for the network input and 1 / mean diag per sample
-__global__ void k_pc_dhat(long n, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
-                          const double *__restrict__ diag, const double *__restrict__ dsum, float *__restrict__ dhat) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = seg[i];
-    const double mean = dsum[s] / (double)(off[s + 1] - off[s]);
-    dhat[i] = mean > 0.0 ? (float)(diag[i] / mean) : 0.f;
+    if (i < n && !usenet[seg[i]]) z[i] = m[i];
 }
 
 }  // namespace
@@ -624,8 +606,8 @@ struct Solver {
     lt_status lens_build_matrix();
     lt_status lens_apply(const int *af, const int *alist, int nact);  // z = approx M^-1 r
     PcNet *net = nullptr;          // learned preconditioner (LT_PC_LEARNED)
-    DevBuf<float> dhat;            // diag / mean diag per sample, network input
-    DevBuf<double> ainv, zap;      // 1 / mean diag; z . A p
+    DevBuf<double> pm, pmm, prm;   // m ~ M^-1 r, m.m and r.m per sample (network input and scale)
+    DevBuf<double> zap;            // z . A p
     DevBuf<int> nfall;             // Jacobi fallbacks taken per sample
     DevBuf<int> usenet;            // 1: network direction, 0: plain CG fallback
     std::vector<int> nfall_host, usenet_host;
@@ -1059,18 +1041,23 @@ lt_status Solver::precondition(int precond, const double *dinv_p, const int *ali
         LT_CHECK_LAUNCH();
         return LT_OK;
     }
+    // z = m + y rms(m), m ~ M^-1 r (the lens-structured preconditioner);
     // rr = r . r must be current (network input scale)
+    LT_TRY(lens_apply(cur_act, alist, nact));
+    hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, z.p, pm.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(seg_dot(pm.p, pm.p, pmm.p));
     const int HW = net->H * net->W;
-    hipLaunchKernelGGL(k_pc_in, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, r.p, rr.p, dhat.p, HW,
+    hipLaunchKernelGGL(k_pc_in, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, r.p, rr.p, pm.p, pmm.p, HW,
                        net->act[PcNet::X0].p);
     LT_CHECK_LAUNCH();
     LT_TRY(net->forward(S));
-    hipLaunchKernelGGL(k_pc_out, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, net->act[PcNet::Y].p, rr.p,
-                       ainv.p, HW, z.p);
+    hipLaunchKernelGGL(k_pc_out, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, net->act[PcNet::Y].p, pm.p,
+                       pmm.p, HW, z.p);
     LT_TRY(seg_dot(r.p, z.p, rz_new.p));
-    hipLaunchKernelGGL(k_pc_guard, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rz_new.p, dinv_p, r.p, z.p,
+    hipLaunchKernelGGL(k_pc_guard, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, cur_act, rz_new.p, pm.p, z.p,
                        nfall.p);
-    hipLaunchKernelGGL(k_pc_plain, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, usenet.p, r.p, z.p);
+    hipLaunchKernelGGL(k_pc_lensonly, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, usenet.p, pm.p, z.p);
     LT_CHECK_LAUNCH();
     return LT_OK;
 }
@@ -1090,35 +1077,18 @@ lt_status Solver::learned_setup() {
                       lb->src_host[s].ny, net->H, net->W);
             return LT_ERR_ARG;
         }
-    if (!dinv.p) LT_TRY(jacobi_setup());
-    const unsigned gb = blocks_for(ntot, 256);
-    DevBuf<double> dsum, onesv;
-    LT_TRY(dsum.alloc(S));
-    LT_TRY(onesv.alloc(ntot));
-    LT_TRY(dhat.alloc(ntot));
-    LT_TRY(ainv.alloc(S));
-    LT_TRY(zap.alloc(S));
+    LT_TRY(lens_setup());
+    if (pm.n != (size_t)ntot) LT_TRY(pm.alloc(ntot));
+    for (auto *v : {&pmm, &prm, &zap}) LT_TRY(v->alloc(S));
     LT_TRY(nfall.alloc(S));
     LT_TRY(usenet.alloc(S));
-    hipLaunchKernelGGL(k_recip, dim3(gb), dim3(256), 0, stream(), ntot, dinv.p, tmp.p);  // diag
-    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(256), 0, stream(), ntot, 1.0, onesv.p);
-    LT_TRY(seg_dot(tmp.p, onesv.p, dsum.p));
-    hipLaunchKernelGGL(k_pc_dhat, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, tmp.p, dsum.p, dhat.p);
-    LT_CHECK_LAUNCH();
-    std::vector<double> hs(S), hinv(S);
-    LT_TRY(dsum.download(hs.data(), S));
-    LT_TRY(sync());
-    for (int s = 0; s < S; ++s) {
-        const double mean = hs[s] / (double)(soff_host[s + 1] - soff_host[s]);
-        hinv[s] = mean > 0 ? 1.0 / mean : 1.0;
-    }
-    LT_TRY(ainv.upload(hinv.data(), S));
-    return sync();
+    return LT_OK;
 }
 
 lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, float lr_final, unsigned seed,
                               double *loss_hist) {
     LT_TRY(learned_setup());
+    cur_act = nullptr;  // every sample's rows
     const unsigned gb = blocks_for(ntot, 256);
     const int HW = net->H * net->W;
     DevBuf<double> pool, zAz, zr;
@@ -1131,20 +1101,25 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
     const int nep = epochs > 0 ? epochs : 1;
     const long total = (long)nep * nres;
     long it = 0;
-    std::vector<double> hzz(S), hzr(S), hrr(S), hai(S);
-    LT_TRY(ainv.download(hai.data(), S));
+    std::vector<double> hzz(S), hzr(S), hrm(S);
     for (int ep = 0; ep < nep; ++ep) {
         if (epochs > 0) std::shuffle(order.begin(), order.end(), gen);
         double acc = 0.0;
         for (int q = 0; q < nres; ++q, ++it) {
             hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, pool.p + (long)order[q] * ntot, r.p);
+            LT_CHECK_LAUNCH();
             LT_TRY(seg_dot(r.p, r.p, rr.p));
-            hipLaunchKernelGGL(k_pc_in, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, r.p, rr.p, dhat.p, HW,
-                               net->act[PcNet::X0].p);
+            LT_TRY(lens_apply(nullptr, nullptr, 0));
+            hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, z.p, pm.p);
+            LT_CHECK_LAUNCH();
+            LT_TRY(seg_dot(pm.p, pm.p, pmm.p));
+            LT_TRY(seg_dot(r.p, pm.p, prm.p));
+            hipLaunchKernelGGL(k_pc_in, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, r.p, rr.p, pm.p, pmm.p,
+                               HW, net->act[PcNet::X0].p);
             LT_CHECK_LAUNCH();
             LT_TRY(net->forward(S));
             hipLaunchKernelGGL(k_pc_out, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p,
-                               net->act[PcNet::Y].p, rr.p, ainv.p, HW, z.p);
+                               net->act[PcNet::Y].p, pm.p, pmm.p, HW, z.p);
             LT_CHECK_LAUNCH();
             LT_TRY(apply(z.p, Ap.p, all_active.p, S));
             LT_TRY(seg_dot(z.p, Ap.p, zAz.p));
@@ -1152,7 +1127,7 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
             LT_CHECK_LAUNCH();
             if (epochs > 0) {
                 hipLaunchKernelGGL(k_energy_grad, dim3(gb), dim3(256), 0, stream(), ntot, S, seg.p, soff.p, Ap.p,
-                                   r.p, rr.p, ainv.p, HW, net->gact[PcNet::Y].p);
+                                   r.p, pmm.p, prm.p, HW, net->gact[PcNet::Y].p);
                 LT_CHECK_LAUNCH();
                 LT_TRY(net->backward(S));
                 const float fr = total > 1 ? (float)it / (float)(total - 1) : 1.f;
@@ -1161,13 +1136,10 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
             }
             LT_TRY(zAz.download(hzz.data(), S));
             LT_TRY(zr.download(hzr.data(), S));
-            LT_TRY(rr.download(hrr.data(), S));
+            LT_TRY(prm.download(hrm.data(), S));
             LT_TRY(sync());
             double l = 0.0;
-            for (int s = 0; s < S; ++s) {
-                const double c = hrr[s] * hai[s];
-                l += c > 0 ? (hzz[s] - 2.0 * hzr[s]) / c : 0.0;
-            }
+            for (int s = 0; s < S; ++s) l += hrm[s] > 0 ? (hzz[s] - 2.0 * hzr[s]) / hrm[s] : 0.0;
             acc += l / S;
         }
         if (loss_hist) loss_hist[ep] = acc / nres;
@@ -1225,7 +1197,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     }
     // Jacobi: diag(A) ~ beam peak * diag(L^T L) + lambda_c diag(Cc^T Cc) + lambda_t
     const double *dinv_p = nullptr;
-    if (precond == LT_PC_JACOBI || precond == LT_PC_LEARNED) {
+    if (precond == LT_PC_JACOBI) {
         if (!dinv.p) LT_TRY(jacobi_setup());
         dinv_p = dinv.p;
     }
@@ -1242,7 +1214,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     long smax = 0;  // the largest source, for launches over listed samples
     for (int q = 0; q < S; ++q) smax = std::max<long>(smax, soff_host[q + 1] - soff_host[q]);
     const unsigned gsmax = blocks_for(std::max<long>(smax, 1), 256);
-    if (precond == LT_PC_LENS) LT_TRY(lens_setup());
+    if (precond == LT_PC_LENS || precond == LT_PC_LEARNED) LT_TRY(lens_setup());
     if (flexible && zap.n < (size_t)S) LT_TRY(zap.alloc(S));  // z . A p of the flexible beta
     cur_act = nullptr;
     if (learned) {
@@ -1251,16 +1223,16 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         usenet_host.assign(S, 1);
         LT_TRY(usenet.upload(usenet_host.data(), S));
     }
-    // plain-CG fallback of the learned preconditioner: a sample whose
-    // residual norm has not fallen 10x over the last kWindow iterations
-    // (about plain CG's own rate on these problems) restarts as plain CG
+    // fallback of the learned preconditioner: a sample whose residual norm
+    // has not fallen 10x over the last kWindow iterations switches its
+    // network off and goes on with z = M^-1 r
     constexpr int kWindow = 100;
     std::vector<std::vector<double>> rr_hist(S);
     std::vector<double> hrr_chk(S);
     std::vector<int> hflag(S), reflist(S);
     std::vector<double> rr_ref(S);
     DevBuf<int> dflag, dref;
-    if (learned || mixed) LT_TRY(dflag.alloc(S));
+    if (mixed) LT_TRY(dflag.alloc(S));
     if (mixed) LT_TRY(dref.alloc(S));
     std::vector<int> ones(S, 1), zeros(S, 0);
     LT_TRY(act.upload(ones.data(), S));
@@ -1484,22 +1456,16 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
                 const size_t back = kWindow / check_every;
                 int nsw = 0;
                 for (int s = 0; s < S; ++s) {
-                    hflag[s] = 0;
                     rr_hist[s].push_back(hrr_chk[s]);
                     const size_t h = rr_hist[s].size();
                     if (hact[s] && usenet_host[s] && h > back && !(hrr_chk[s] < 1e-2 * rr_hist[s][h - 1 - back])) {
                         usenet_host[s] = 0;
-                        hflag[s] = 1;
                         ++nsw;
                     }
                 }
-                if (nsw) {
-                    LT_TRY(usenet.upload(usenet_host.data(), S));
-                    LT_TRY(dflag.upload(hflag.data(), S));
-                    hipLaunchKernelGGL(k_restart, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, dflag.p, r.p, p.p);
-                    hipLaunchKernelGGL(k_restart_rz, dim3(gs), dim3(64), 0, stream(), S, dflag.p, rr.p, rz.p);
-                    LT_CHECK_LAUNCH();
-                }
+                // flexible CG takes the change of preconditioner in its
+                // beta (z . A p); no restart
+                if (nsw) LT_TRY(usenet.upload(usenet_host.data(), S));
             }
         }
     }

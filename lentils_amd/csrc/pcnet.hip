@@ -399,12 +399,14 @@ lt_status PcNet::create(int H_, int W_, int cap) {
 }
 
 lt_status PcNet::init_random(unsigned seed) {
-    // He initialisation for the ReLU layers, small last layer; zero biases
+    // He initialisation for the ReLU layers, zero biases and a zero last
+    // layer: an untrained network outputs 0 (the solver's correction starts
+    // at none), and the last layer's gradient is still the full one
     std::vector<float> h(param_count(), 0.f);
     std::mt19937 gen(seed);
-    for (int l = 0; l < kLayers; ++l) {
+    for (int l = 0; l < kLayers - 1; ++l) {
         const int fan = kCin[l] * 9;
-        const float sd = (l == kLayers - 1) ? 0.1f * std::sqrt(1.f / fan) : std::sqrt(2.f / fan);
+        const float sd = std::sqrt(2.f / fan);
         std::normal_distribution<float> nd(0.f, sd);
         for (long k = 0; k < (long)kCout[l] * fan; ++k) h[woff[l] + k] = nd(gen);
     }

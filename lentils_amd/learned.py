@@ -1,15 +1,25 @@
 """Learned preconditioner for the batched source inversion.
 
 A small convolutional network (BF16 operands on MFMA, FP32 accumulate; the
-kernels are in csrc/pcnet.hip) maps a CG residual r of one lens sample, with
-that sample's diagonal of A as a second input channel, to a search direction
-z ~ A^{-1} r.  It is trained on the GPU on (right-hand side, solution) pairs
-that the solver's own operator generates: b = A x for random Gaussian-blob
-sources x (the source family of the workload), over many random lens models,
-and is used inside flexible CG (BatchedSourceInversion.solve(precond=
-'learned')), with the Jacobi direction as the guard whenever the network's
-direction is not a descent direction.  The reference has no counterpart (it
-preconditions with a sparse LU, lentils/tests/test_all.py:63-72).
+kernels are in csrc/pcnet.hip) corrects the lens-structured preconditioner
+M = c L_s^T L_s + lambda Cc^T Cc of the source inversion: for a CG residual r
+of one lens sample, with m = M^-1 r (a few inner Jacobi-PCG steps), the
+search direction is
+
+    z = m + y(r / rms(r), m / rms(m)) rms(m),
+
+so the network sees the residual and M's answer and outputs a correction
+towards A^-1 r.  Its last layer starts at zero (an untrained network is M
+alone).  It is trained on the GPU with the solver's own operator on
+(right-hand side, solution) pairs the solver generates: b = A x for random
+Gaussian-blob sources x over random lens models, and the residuals r_k of
+truncated solves of those b (the pair (r_k, x - x_k)), by the energy loss
+z^T A z - 2 z^T r (the A-norm error of z up to a constant, which needs no
+explicit solution).  Inside flexible CG (BatchedSourceInversion.solve(
+precond='learned')) the guard takes m whenever the corrected direction is
+not a descent direction, and a sample that stalls switches the network off.
+The reference has no counterpart (it preconditions with a sparse LU,
+lentils/tests/test_all.py:63-72).
 """
 import ctypes as C
 
@@ -101,26 +111,18 @@ def blob_sources(src_space, n, rng, nblobs=4, spread=0.3, width=(0.05, 0.2), amp
     return out
 
 
-def network_inputs(b, diag):
-    """(b / rms(b), diag / mean(diag)) as the solver forms them; b, diag:
-    arrays of one sample's source grid.  Returns (x0 [2][H][W], rms, mean)."""
-    rms = np.sqrt(np.mean(b * b))
-    mean = np.mean(diag)
-    x0 = np.stack([b / rms if rms > 0 else 0 * b, diag / mean])
-    return x0, rms, mean
-
-
-def training_pairs(inversion, sources):
-    """Pairs for every sample of `inversion` (a BatchedSourceInversion) and
-    every draw in `sources` (list of per-sample source lists): b = A x.
-    Returns (inputs [n][2][H][W], targets [n][H][W]) in float32, targets
-    scaled as the network's output: x * mean(diag) / rms(b)."""
-    diag = inversion.diag()
-    ins, tgs = [], []
-    for xs in sources:
-        bs = inversion.apply(xs)
-        for s, (x, b) in enumerate(zip(xs, bs)):
-            x0, rms, mean = network_inputs(np.asarray(b).reshape(x.shape), diag[s].reshape(x.shape))
-            ins.append(x0.astype(np.float32))
-            tgs.append((x * mean / rms).astype(np.float32))
-    return np.stack(ins), np.stack(tgs)
+def solver_residuals(inversion, sources, iterations=(0, 4, 16, 48), precond='lens'):
+    """Training vectors from `inversion` (a BatchedSourceInversion): for each
+    draw in `sources` (list of per-sample source lists) the right-hand sides
+    b = A x and the true residuals b - A x_k of the solver stopped after each
+    k in `iterations` (k = 0 is b itself).  Returns a list of flat arrays of
+    all samples; the inversion's own right-hand sides are restored."""
+    keep = inversion.rhs()
+    out = []
+    try:
+        for xs in sources:
+            inversion.set_rhs(inversion.apply(xs))
+            out.extend(inversion.residuals(iterations, precond=precond))
+    finally:
+        inversion.set_rhs(keep)
+    return out

@@ -292,24 +292,26 @@ class BatchedSourceInversion:
                                               float(lr_final), int(seed), B.ptr(hist)))
         return hist
 
-    def residuals(self, iterations):
-        """True residuals b - A x_k of plain CG stopped after each k in
-        `iterations` (k = 0 gives b), as flat arrays of all samples."""
+    def residuals(self, iterations, precond='none'):
+        """True residuals b - A x_k of the solver (precond as in solve)
+        stopped after each k in `iterations` (k = 0 gives b), as flat arrays
+        of all samples."""
         b = np.concatenate([v.reshape(-1) for v in self.rhs()])
         out = []
         for k in iterations:
             if k == 0:
                 out.append(b.copy())
                 continue
-            xs, _, _ = self.solve(rtol=1e-12, maxiter=int(k))
+            xs, _, _ = self.solve(rtol=1e-12, maxiter=int(k), precond=precond)
             ax = np.concatenate([v.reshape(-1) for v in self.apply(xs)])
             out.append(b - ax)
         return out
 
     def fallbacks(self):
-        """Per sample, for the last learned solve: (iterations that took the
-        Jacobi direction because the network's was not a descent direction,
-        whether the sample fell back to plain CG)."""
+        """Per sample, for the last learned solve: (iterations that took
+        m = M^-1 r because the corrected direction was not a descent
+        direction, whether the sample switched its network off and went on
+        with M alone)."""
         c = np.zeros(len(self._sizes), dtype=np.int32)
         pl = np.zeros(len(self._sizes), dtype=np.int32)
         B.check(B.lib().lt_solver_pc_fallbacks(self._handle, B.ptr(c), B.ptr(pl)))
@@ -318,7 +320,8 @@ class BatchedSourceInversion:
     def solve(self, rtol=1e-6, maxiter=5000, precond='none', precision='f64', refine_every=20):
         """Returns (sources, iterations, true relative residuals).
         precond: 'none' (plain CG), 'jacobi', 'learned' (flexible CG with
-        the network given to set_preconditioner) or 'lens' (flexible CG with
+        the 'lens' preconditioner corrected by the network given to
+        set_preconditioner) or 'lens' (flexible CG with
         a few inner CG steps on lambda Cc^T Cc + c L_s^T L_s, the reference's
         R + w L^T L preconditioner; see set_lens_preconditioner).  precision 'f32': the
         operator runs in FP32 inside the FP64 CG, with the FP64 true residual
