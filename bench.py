@@ -139,6 +139,7 @@ class Workload:
         inv.set_graphs(GRAPHS)
         if precond == 'learned':
             inv.set_preconditioner(net)
+            inv.set_flexible_depth(4)
         x, its, res = inv.solve(rtol=rtol, maxiter=maxiter, precond=precond, precision=precision)
         if precond == 'learned':
             self.last_fallbacks = inv.fallbacks()
@@ -307,9 +308,10 @@ def learned_form(args):
     GPU: over `train_batches` batches of random lenses of the same family
     (seeds apart from the evaluation batch) with `draws` random blob-source
     draws each, the solver generates the right-hand sides b = A x and the
-    residuals of M-preconditioned solves of those b stopped after 4, 16 and
-    48 iterations; `epochs` passes of Adam on the energy loss over them.
-    Prints one JSON line."""
+    residuals of M-preconditioned solves of those b stopped after 8 and 32
+    iterations; `epochs` passes of Adam on the scale-invariant energy loss
+    over them.  The learned solves use flexible CG keeping 4 directions
+    A-orthogonal.  Prints one JSON line."""
     from lentils_amd import backend as B
     from lentils_amd.learned import PCNet, blob_sources, solver_residuals
     from lentils_amd.solvers import BatchedSourceInversion, LensBatch
@@ -319,7 +321,7 @@ def learned_form(args):
     wl = Workload(nlens=nlens)
     log(f'learned workload setup {time.perf_counter() - t:.1f}s')
     net = PCNet(128, 128, batch=nlens, seed=3)
-    iters = (0, 4, 16, 48)
+    iters = (0, 8, 32)
     rng = np.random.default_rng(21)
     t_gen = t_train = 0.0
     npairs = 0
@@ -333,15 +335,15 @@ def learned_form(args):
         vecs = solver_residuals(inv, draws, iterations=iters, precond='lens')
         t_gen += time.perf_counter() - t
         t = time.perf_counter()
-        h = inv.train_preconditioner(net, vecs, epochs=args.epochs, lr=args.lr, lr_final=args.lr * 0.1,
-                                     seed=tb)
+        last = tb == args.train_batches - 1
+        h = inv.train_preconditioner(net, vecs, epochs=args.epochs, lr=args.lr,
+                                     lr_final=args.lr * (0.03 if last else 0.3), seed=tb)
         t_train += time.perf_counter() - t
         npairs += len(vecs) * nlens
         hist.append([float(v) for v in h])
         log(f'training batch {tb}: {len(vecs)} vectors x {nlens} samples, loss {h[0]:.4f} -> {h[-1]:.4f} '
             f'(generation {t_gen:.1f}s, training {t_train:.1f}s so far)')
         del inv, lb, vecs
-    # held-out loss of M alone (zero network) vs the trained correction
     results = {}
     for name, pc in (('plain', 'none'), ('lens', 'lens'), ('learned', 'learned')):
         for k in range(args.warmup):
@@ -362,6 +364,8 @@ def learned_form(args):
         if name == 'learned':
             results[name]['guard_fallback_iters'] = int(wl.last_fallbacks[0].sum())
             results[name]['samples_switched_to_M'] = int(wl.last_fallbacks[1].sum())
+            results[name]['method'] = ('flexible CG keeping 4 directions A-orthogonal; z = M^-1 r + network '
+                                       'correction, M alone for samples whose residual fell < 10x in 100 iterations')
         log(f'{name}: {tot / args.steps:.2f}s/step iters median {np.median(its):.0f} max {its.max()} '
             f'max res {res.max():.2e}')
     out = {'metric': 'source solves/s at 1e-6 rel resid (learned-PCG vs plain CG); Gvis/s degrid',
@@ -621,10 +625,10 @@ def main():
     ap.add_argument('--learned', action='store_true',
                     help='the learned-preconditioner workload (256 samples, network trained first) instead')
     ap.add_argument('--learned-samples', type=int, default=256)
-    ap.add_argument('--train-batches', type=int, default=4)
+    ap.add_argument('--train-batches', type=int, default=6)
     ap.add_argument('--draws', type=int, default=8)
-    ap.add_argument('--epochs', type=int, default=3)
-    ap.add_argument('--lr', type=float, default=1e-3)
+    ap.add_argument('--epochs', type=int, default=6)
+    ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--shared-data', action='store_true',
                     help='one observation for all samples (most samples then fit the data with the wrong lens)')
     args = ap.parse_args()

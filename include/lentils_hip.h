@@ -399,6 +399,16 @@ lt_status lt_solver_lens_pc_matrix(lt_solver s, long *nnz, double *c_used, int *
  * same kernels launched one by one.  Results are identical. */
 lt_status lt_solver_set_graphs(lt_solver s, int on);
 /* replace the right-hand sides (all samples concatenated, host) */
+/* Flexible CG (LT_PC_LENS, LT_PC_LEARNED) keeps each new direction
+ * A-orthogonal to the last `depth` directions (1..32; 1, the default, is the
+ * Polak-Ribiere form with the last direction only) and takes
+ * alpha = p.r / p.A p; for a preconditioner that changes between
+ * iterations this restores the conjugacy that one direction loses
+ * (Notay's truncated FCG).  depth > 1 runs the iterations uncaptured. */
+lt_status lt_solver_set_flexible_depth(lt_solver s, int depth);
+/* LT_PC_LEARNED: the network corrects M for the first n iterations of a
+ * solve only, M alone afterwards (0, the default: no limit) */
+lt_status lt_solver_set_learned_iterations(lt_solver s, int n);
 lt_status lt_solver_set_rhs(lt_solver s, const double *rhs);
 /* y = A x for all samples, in the precision set by lt_solver_set_precision */
 lt_status lt_solver_apply(lt_solver s, const double *x, double *y);

@@ -288,6 +288,24 @@ __global__ void k_cg_beta_flex(int ns, int *__restrict__ act, double *__restrict
     if (!(rr[s] > rtol2 * bb[s])) act[s] = 0;
 }
 
+// truncated flexible CG: p = z - sum_j (z . A p_j / p_j . A p_j) p_j over
+// the nj stored directions (ring slots, n values each) of every active
+// sample, making p A-orthogonal to all of them and not only to the last
+__global__ void k_fcg_p(long n, const int32_t *__restrict__ seg, const int *__restrict__ act, int nj, int S,
+                        const double *__restrict__ d, const double *__restrict__ pap, const double *__restrict__ P,
+                        const double *__restrict__ z, double *__restrict__ p) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = seg[i];
+    if (!act[s]) return;
+    double v = z[i];
+    for (int j = 0; j < nj; ++j) {
+        const double q = pap[(long)j * S + s];
+        if (q != 0.0) v -= d[(long)j * S + s] / q * P[(long)j * n + i];
+    }
+    p[i] = v;
+}
+
 __global__ void k_recip(long n, const double *__restrict__ a, double *__restrict__ b) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = 1.0 / a[i];
@@ -343,22 +361,27 @@ __global__ void k_pc_guard(long n, const int32_t *__restrict__ seg, const int *_
     }
 }
 
-// Gradient of the energy loss of the learned preconditioner,
-//   L = (1/S) sum_s (z_s^T A z_s - 2 z_s^T r_s) / c_s,  c_s = r_s^T m_s
-// (the A-norm error ||z - A^{-1} r||_A^2 up to a constant; m ~ M^-1 r), with
-// respect to the network output y (z = m + y rms(m)).
+// Gradient of the training loss of the learned preconditioner,
+//   L = (1/S) sum_s -(z_s^T r_s)^2 / (z_s^T A z_s  c_s),  c_s = r_s^T m_s,
+// with respect to the network output y (z = m + y rms(m)).  -(z^T r)^2 /
+// z^T A z is the energy z'^T A z' - 2 z'^T r of the best multiple z' of z
+// (the A-norm error ||z' - A^-1 r||_A^2 up to a constant): the loss grades
+// the direction of z, which is all that CG uses, and not its scale.  Its
+// floor, at z ~ A^-1 r, is -r^T A^-1 r / c.
 __global__ void k_energy_grad(long n, int S, const int32_t *__restrict__ seg, const int64_t *__restrict__ off,
                               const double *__restrict__ Az, const double *__restrict__ r,
-                              const double *__restrict__ mm, const double *__restrict__ rm, int HW,
+                              const double *__restrict__ mm, const double *__restrict__ rm,
+                              const double *__restrict__ zr, const double *__restrict__ zAz, int HW,
                               float *__restrict__ gy) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = seg[i];
     const long pix = i - off[s];
     const double N = (double)(off[s + 1] - off[s]);
-    const double c = rm[s];
-    const double scale = c > 0.0 ? 2.0 * sqrt(mm[s] / N) / (c * S) : 0.0;
-    gy[(long)s * HW + pix] = (float)((Az[i] - r[i]) * scale);
+    const double c = rm[s], q = zAz[s], t = zr[s];
+    double g = 0.0;
+    if (c > 0.0 && q > 0.0) g = -2.0 * t / (q * c) * (r[i] - (t / q) * Az[i]) * sqrt(mm[s] / N) / S;
+    gy[(long)s * HW + pix] = (float)g;
 }
 
 // x32 = x / rms_s per sample (the FP32 operator works on unit-scale input)
@@ -608,9 +631,15 @@ struct Solver {
     PcNet *net = nullptr;          // learned preconditioner (LT_PC_LEARNED)
     DevBuf<double> pm, pmm, prm;   // m ~ M^-1 r, m.m and r.m per sample (network input and scale)
     DevBuf<double> zap;            // z . A p
-    DevBuf<int> nfall;             // Jacobi fallbacks taken per sample
-    DevBuf<int> usenet;            // 1: network direction, 0: plain CG fallback
+    DevBuf<int> nfall;             // guard fallbacks to m taken per sample
+    DevBuf<int> usenet;            // 1: network correction on, 0: switched off (M alone)
+    // truncated flexible CG (fcg_depth > 1): each new direction is made
+    // A-orthogonal to the last fcg_depth directions, and alpha = p.r / p.A p
+    int fcg_depth = 1;
+    DevBuf<double> fP, fAP, fpAp, fd, fpr;  // ring of p, A p, p.A p; z.A p_j; p.r
     std::vector<int> nfall_host, usenet_host;
+    int net_on = 0;                // samples with the network on (0: M alone, no forward pass)
+    int net_iters = 0;             // network on for the first net_iters iterations (0: no limit)
 
     // CG blocks of check_every iterations captured as HIP graphs, one per
     // launch width (active samples rounded up to a multiple of 8)
@@ -1044,6 +1073,7 @@ lt_status Solver::precondition(int precond, const double *dinv_p, const int *ali
     // z = m + y rms(m), m ~ M^-1 r (the lens-structured preconditioner);
     // rr = r . r must be current (network input scale)
     LT_TRY(lens_apply(cur_act, alist, nact));
+    if (net_on == 0) return LT_OK;  // every sample's network is off: z = m
     hipLaunchKernelGGL(k_copy, dim3(gb), dim3(256), 0, stream(), ntot, z.p, pm.p);
     LT_CHECK_LAUNCH();
     LT_TRY(seg_dot(pm.p, pm.p, pmm.p));
@@ -1127,7 +1157,7 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
             LT_CHECK_LAUNCH();
             if (epochs > 0) {
                 hipLaunchKernelGGL(k_energy_grad, dim3(gb), dim3(256), 0, stream(), ntot, S, seg.p, soff.p, Ap.p,
-                                   r.p, pmm.p, prm.p, HW, net->gact[PcNet::Y].p);
+                                   r.p, pmm.p, prm.p, zr.p, zAz.p, HW, net->gact[PcNet::Y].p);
                 LT_CHECK_LAUNCH();
                 LT_TRY(net->backward(S));
                 const float fr = total > 1 ? (float)it / (float)(total - 1) : 1.f;
@@ -1139,7 +1169,8 @@ lt_status Solver::train_pcnet(const double *R, int nres, int epochs, float lr, f
             LT_TRY(prm.download(hrm.data(), S));
             LT_TRY(sync());
             double l = 0.0;
-            for (int s = 0; s < S; ++s) l += hrm[s] > 0 ? (hzz[s] - 2.0 * hzr[s]) / hrm[s] : 0.0;
+            for (int s = 0; s < S; ++s)
+                l += (hrm[s] > 0 && hzz[s] > 0) ? -hzr[s] * hzr[s] / (hzz[s] * hrm[s]) : 0.0;
             acc += l / S;
         }
         if (loss_hist) loss_hist[ep] = acc / nres;
@@ -1216,11 +1247,26 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     const unsigned gsmax = blocks_for(std::max<long>(smax, 1), 256);
     if (precond == LT_PC_LENS || precond == LT_PC_LEARNED) LT_TRY(lens_setup());
     if (flexible && zap.n < (size_t)S) LT_TRY(zap.alloc(S));  // z . A p of the flexible beta
+    const bool deep = flexible && fcg_depth > 1 && !mixed;
+    int nstored = 0;  // directions in the ring (truncated flexible CG)
+    if (deep) {
+        const long dn = (long)fcg_depth * ntot, ds = (long)fcg_depth * S;
+        if (fP.n != (size_t)dn) {
+            LT_TRY(fP.alloc(dn));
+            LT_TRY(fAP.alloc(dn));
+        }
+        if (fpAp.n != (size_t)ds) {
+            LT_TRY(fpAp.alloc(ds));
+            LT_TRY(fd.alloc(ds));
+        }
+        LT_TRY(fpr.alloc(S));
+    }
     cur_act = nullptr;
     if (learned) {
         LT_TRY(learned_setup());
         LT_TRY(nfall.zero());
         usenet_host.assign(S, 1);
+        net_on = S;
         LT_TRY(usenet.upload(usenet_host.data(), S));
     }
     // fallback of the learned preconditioner: a sample whose residual norm
@@ -1323,7 +1369,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     // buffer and FFT plan exists before capture.  Same arithmetic as the
     // per-iteration loop.
     drop_graphs();
-    const bool graph_mode = use_graphs && !mixed && !learned;
+    const bool graph_mode = use_graphs && !mixed && !learned && !deep;
     std::vector<int> padded(S);
     for (int round = 0;; ++round) {
     while (graph_mode && it < maxiter && nact > 0) {
@@ -1385,6 +1431,20 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         if (fused) {
             hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nact : S), dim3(kDotOne), 0, stream(), soff.p, dl, p.p, Ap.p,
                                pAp.p, 1, act.p, rz.p, alpha.p, bb.p, rtol2, iters.p);
+        } else if (deep) {
+            // alpha = p.r / p.A p; the direction joins the ring
+            LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
+            LT_TRY(seg_dot(p.p, r.p, fpr.p));
+            hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, fpr.p, pAp.p, alpha.p);
+            LT_CHECK_LAUNCH();
+            const int slot = nstored % fcg_depth;
+            LT_HIP(hipMemcpyAsync(fP.p + (long)slot * ntot, p.p, ntot * sizeof(double), hipMemcpyDeviceToDevice,
+                                  stream()));
+            LT_HIP(hipMemcpyAsync(fAP.p + (long)slot * ntot, Ap.p, ntot * sizeof(double), hipMemcpyDeviceToDevice,
+                                  stream()));
+            LT_HIP(hipMemcpyAsync(fpAp.p + (long)slot * S, pAp.p, S * sizeof(double), hipMemcpyDeviceToDevice,
+                                  stream()));
+            ++nstored;
         } else {
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
             hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
@@ -1413,9 +1473,21 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             hipLaunchKernelGGL(k_cg_beta, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rzn, rr.p, bb.p, rtol2,
                                beta.p, iters.p);
         }
-        hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, zz, p.p);
+        if (deep) {
+            const int nj = std::min(nstored, fcg_depth);
+            for (int j = 0; j < nj; ++j) LT_TRY(seg_dot(z.p, fAP.p + (long)j * ntot, fd.p + (long)j * S));
+            hipLaunchKernelGGL(k_fcg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, nj, S, fd.p, fpAp.p,
+                               fP.p, z.p, p.p);
+        } else {
+            hipLaunchKernelGGL(k_cg_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, beta.p, zz, p.p);
+        }
         LT_CHECK_LAUNCH();
         ++outer;
+        if (learned && net_iters > 0 && outer == net_iters && net_on > 0) {
+            usenet_host.assign(S, 0);
+            net_on = 0;
+            LT_TRY(usenet.upload(usenet_host.data(), S));
+        }
         if (mixed && refine_every > 0 && outer % refine_every == 0) {
             // reliable update: a sample whose recurrence residual fell 10x
             // since its last update gets the FP64 true residual b - A x in
@@ -1460,6 +1532,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
                     const size_t h = rr_hist[s].size();
                     if (hact[s] && usenet_host[s] && h > back && !(hrr_chk[s] < 1e-2 * rr_hist[s][h - 1 - back])) {
                         usenet_host[s] = 0;
+                        --net_on;
                         ++nsw;
                     }
                 }
@@ -1490,6 +1563,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     LT_TRY(set_list(alist.data(), nact));
     LT_TRY(precondition(precond, dinv_p));
     hipLaunchKernelGGL(k_restart_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, z.p, p.p);
+    nstored = 0;  // a restart forgets the stored directions
     LT_TRY(seg_dot(r.p, z.p, rz.p));
     LT_CHECK_LAUNCH();
     }
@@ -1694,6 +1768,22 @@ lt_status lt_solver_lens_pc_matrix(lt_solver h, long *nnz, double *c_used, int *
 lt_status lt_solver_set_graphs(lt_solver h, int on) {
     if (!h) return LT_ERR_ARG;
     h->s.use_graphs = on != 0;
+    return LT_OK;
+}
+
+lt_status lt_solver_set_learned_iterations(lt_solver h, int n) {
+    if (!h || n < 0) return LT_ERR_ARG;
+    h->s.net_iters = n;
+    return LT_OK;
+}
+
+lt_status lt_solver_set_flexible_depth(lt_solver h, int depth) {
+    if (!h) return LT_ERR_ARG;
+    if (depth < 1 || depth > 32) {
+        set_error("solver: flexible CG depth %d outside 1..32", depth);
+        return LT_ERR_ARG;
+    }
+    h->s.fcg_depth = depth;
     return LT_OK;
 }
 

@@ -239,6 +239,16 @@ class BatchedSourceInversion:
         or as individual launches; the results are identical."""
         B.check(B.lib().lt_solver_set_graphs(self._handle, 1 if on else 0))
 
+    def set_flexible_depth(self, depth=1):
+        """Flexible CG keeps each new direction A-orthogonal to the last
+        `depth` directions (1: the last one only)."""
+        B.check(B.lib().lt_solver_set_flexible_depth(self._handle, int(depth)))
+
+    def set_learned_iterations(self, n=0):
+        """precond='learned': the network corrects M for the first n
+        iterations only (0: no limit)."""
+        B.check(B.lib().lt_solver_set_learned_iterations(self._handle, int(n)))
+
     def set_lens_preconditioner(self, inner_steps=3, max_row=64, c=0.0):
         B.check(B.lib().lt_solver_set_lens_pc(self._handle, int(inner_steps), int(max_row), float(c)))
 
