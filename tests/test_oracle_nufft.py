@@ -142,3 +142,32 @@ def test_dirty_beam_peak_is_weight_sum(oracle):
     assert np.unravel_index(np.argmax(beam), beam.shape) == (im.nx, im.ny)
     inner = beam[1:, 1:]
     assert O.max_relative_error(inner, inner[::-1, ::-1]) < 1e-6
+
+
+def test_grid_wrap_is_periodic_aliasing(oracle):
+    """nufft.c:83-93: taps past the grid's edge wrap periodically (and fold
+    into the stored half plane).  On a centred image (no phase term) a
+    visibility one grid period away, u +- nu du or v +- nv dv, reads and
+    writes the same cells with the same kernel weights and checkerboard signs,
+    so degridding and gridding agree with the unshifted visibility; a clamp,
+    a wrong modulus or a sign taken after the wrap breaks this."""
+    O = oracle
+    geo = O.NUFFTGeometry(O.Grid2D(32, 32, [-0.32, 0.32, -0.32, 0.32]), pad=2, wsup=3, beta=14.04)
+    assert geo.gcx == 0.0 and geo.gcy == 0.0
+    g = rng(70)
+    n = 400
+    uv = np.zeros((n, 3))
+    uv[:, 0] = g.uniform(-0.5, 0.5, n) * geo.nu * geo.du
+    uv[:, 1] = g.uniform(-0.5, 0.5, n) * geo.nv * geo.dv
+    grid = np.ascontiguousarray(g.normal(size=(geo.nu, geo.half_nv, 2)).view(np.complex128)[..., 0])
+    vis = g.normal(size=(1, n, 2)).view(np.complex128)[..., 0]
+    ref_v = O.grid(uv, [1.0], None, geo, 0, grid_in=grid)
+    ref_g = O.grid(uv, [1.0], vis, geo, 1)
+    for du, dv in ((1, 0), (-1, 0), (0, 1), (0, -1), (1, -1)):
+        sh = uv.copy()
+        sh[:, 0] += du * geo.nu * geo.du
+        sh[:, 1] += dv * geo.nv * geo.dv  # v = -V: a whole period either way
+        got_v = O.grid(sh, [1.0], None, geo, 0, grid_in=grid)
+        got_g = O.grid(sh, [1.0], vis, geo, 1)
+        assert O.max_relative_error(got_v, ref_v) < 1e-11, (du, dv)
+        assert O.max_relative_error(got_g, ref_g) < 1e-11, (du, dv)
