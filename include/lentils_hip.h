@@ -289,9 +289,12 @@ lt_status lt_nufft_destroy(lt_nufft h);
  * kernel_support <= 4 builds the normal stencil D^T diag(w) D from them
  * once.  Solvers created with weights == NULL share both. */
 lt_status lt_nufft_set_weights(lt_nufft h, const double *weights);
-/* No reference counterpart: forces the normal stencil's samples per
- * workgroup (1, 2, 4, 8) for this plan instead of the autotuned choice (0).
- * Every choice gives the same bits. */
+/* No reference counterpart: forces the normal stencil's form for this plan
+ * instead of the autotuned choice (0): 1, 2, 4, 8 = the vector form with that
+ * many samples per workgroup (all four give the same bits); -1, -2, -4, -8 =
+ * the FP64 matrix-core form (v_mfma_f64_16x16x4f64, radius-6 stencils) with
+ * that many groups of 8 samples per workgroup (the same bits among
+ * themselves, the vector form's to rounding). */
 lt_status lt_nufft_set_stencil_shape(lt_nufft h, int samples_per_group);
 /* No reference counterpart: times the resident normal stencil's kernel as
  * the solver launches it for nb samples (band form, tuned shape) with HIP

@@ -1679,8 +1679,9 @@ lt_status lt_nufft_set_weights(lt_nufft nfh, const double *weights) {
 
 lt_status lt_nufft_set_stencil_shape(lt_nufft nfh, int samples_per_group) {
     NufftPlan *nf = nufft_of(nfh);
-    if (!nf || samples_per_group < 0 || samples_per_group > 8 || (samples_per_group & (samples_per_group - 1))) {
-        set_error("lt_nufft_set_stencil_shape: samples per group 0, 1, 2, 4 or 8");
+    const int m = samples_per_group < 0 ? -samples_per_group : samples_per_group;
+    if (!nf || m > 8 || (m & (m - 1))) {
+        set_error("lt_nufft_set_stencil_shape: 0, 1, 2, 4, 8 (vector form) or -1, -2, -4, -8 (matrix-core form)");
         return LT_ERR_ARG;
     }
     nf->force_nb = samples_per_group;

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "nufft_plan.h"
 
@@ -1001,6 +1002,148 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, in
     }
     const T sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? (T)-1 : (T)1;
     for (int k = 0; k < ns; ++k) rbuf[(long)smp[k] * nrow + row] = Cplx<T>{sg * ar[k], sg * ai[k]};
+}
+
+// --- FP64 normal stencil on the matrix cores ------------------------------
+// A 4 x 4 tile of stencil rows against its 16 x 16 window of grid cells is a
+// dense 16 x 256 matrix (each row's 169 coefficients placed at their cells,
+// zeros elsewhere) times the window's values of 8 samples (16 real columns:
+// real and imaginary parts, the coefficients being real).
+// v_mfma_f64_16x16x4f64 takes it in 64 steps of 4 window cells.  Operand
+// layout of that instruction: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16], and element r of its result D[l / 16 + 4 r][l % 16].
+// The matrix is packed once per stencil in that layout and read from
+// L2 while a workgroup runs over its sample groups; four waves take the 2 x 2
+// tiles of an 8 x 8 block of rows and share the block's 20 x 20 window,
+// staged in LDS per group of 8 samples.
+constexpr int kMfT = 4;                   // tile edge (rows)
+constexpr int kMfB = 8;                   // block edge (rows)
+constexpr int kMfR = 6;                   // stencil radius
+constexpr int kMfWS = kMfB + 2 * kMfR;    // block window edge (cells)
+typedef double lt_double4 __attribute__((ext_vector_type(4)));
+
+// A[((block * 4 + tile) * 64 + step) * 64 + lane] and the tiles' rows
+// (-1 where a cell is no stencil row)
+__global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks, int ntv_b,
+                          const int32_t *__restrict__ rowmap, long nrow, const double *__restrict__ coef,
+                          double *__restrict__ A, int32_t *__restrict__ rows) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nblock * 4 * 64 * 64) return;
+    const int l = (int)(g & 63), c = (int)((g >> 6) & 63), t = (int)((g >> 12) & 3);
+    const long b = g >> 14;
+    const int blk = blocks[b];
+    const int bu = blk / ntv_b, bv = blk - bu * ntv_b;
+    const int i = l & 15, k = 4 * c + (l >> 4);
+    const int iu = i >> 2, iv = i & 3;   // row within the tile
+    const int wu = k >> 4, wv = k & 15;  // cell within the tile's window
+    const int ru = bu * kMfB + (t >> 1) * kMfT + iu, rv = bv * kMfB + (t & 1) * kMfT + iv;
+    int row = -1;
+    if (ru < G.nu && rv < G.nv) row = rowmap[(long)ru * G.nv + rv];
+    const int ou = wu - iu, ov = wv - iv;  // offset index in [0, 2R]
+    constexpr int NR = 2 * kMfR + 1;
+    double a = 0.0;
+    if (row >= 0 && ou >= 0 && ou < NR && ov >= 0 && ov < NR) a = coef[(long)(ou * NR + ov) * nrow + row];
+    A[g] = a;
+    if (c == 0 && l < 16) rows[(b * 4 + t) * 16 + i] = row;
+}
+
+// R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) on the matrix cores.
+// Workgroup = (block, chunk of GPW sample groups): four waves, one per tile
+// of the block, stage the block's window for a group of 8 samples in LDS
+// together, then each runs its tile's 64 matrix steps with the packed
+// matrix read from L2 as it goes (the chunks of one block run back to back
+// on one XCD) and the window's values from LDS.
+// one group's window: per cell the band position and the fold's signs once,
+// then the 8 samples' values (consecutive threads take consecutive u)
+__device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ spec, long sstride,
+                                const int *__restrict__ active, int nactive, int g, int u0, int v0, int nbv,
+                                int t, int nt, double *__restrict__ win) {
+    long soff[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int q = g * 8 + s;
+        soff[s] = q < nactive ? (long)(active ? active[q] : q) * sstride : -1;
+    }
+    for (int ci = t; ci < kMfWS * kMfWS; ci += nt) {
+        const int wv = ci / kMfWS, wu = ci - wv * kMfWS;
+        const int a = u0 + wu - G.cu, bb = v0 + wv - G.cv;
+        double *cell = win + (wu * kMfWS + wv) * 16;
+        if (abs(a) > G.bu || abs(bb) > G.bv || abs(bb) >= nbv) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) cell[s] = 0.0;
+            continue;
+        }
+        const double sg = ((a + bb) & 1) ? -1.0 : 1.0;
+        long off;
+        double sim;
+        if (bb >= 0) {
+            off = (long)bb * G.nu + (a < 0 ? a + G.nu : a);
+            sim = sg;
+        } else {
+            off = (long)(-bb) * G.nu + (-a < 0 ? -a + G.nu : -a);
+            sim = -sg;
+        }
+        Cplx<double> z[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) z[s] = soff[s] >= 0 ? spec[soff[s] + off] : Cplx<double>{0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            cell[2 * s] = sg * z[s].re;
+            cell[2 * s + 1] = sim * z[s].im;
+        }
+    }
+}
+
+template <int GPW>
+__global__ void __launch_bounds__(256) k_stencil_mfma(Geo G, const int32_t *__restrict__ blocks, int ntv_b, int ngc,
+                                                      const double *__restrict__ A, const int32_t *__restrict__ rows,
+                                                      const Cplx<double> *__restrict__ spec, long sstride,
+                                                      Cplx<double> *__restrict__ rbuf, long nrow,
+                                                      const int *__restrict__ active, int nactive, int nbv) {
+    extern __shared__ unsigned char smem[];
+    double *win = reinterpret_cast<double *>(smem);  // [kMfWS * kMfWS cells][16]
+    const long lb = xcd_block(blockIdx.x, gridDim.x);
+    const long b = lb / ngc;
+    const int gc = (int)(lb - b * ngc);
+    const int blk = blocks[b];
+    const int bu = blk / ntv_b, bv = blk - bu * ntv_b;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int tu = w >> 1, tv = w & 1;
+    const double *Aw = A + (b * 4 + w) * 64 * 64 + l;
+    int orow[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) orow[r] = rows[(b * 4 + w) * 16 + (l >> 4) + 4 * r];
+    const int j = l & 15;  // column: sample j / 2 of the group, real (even) or imaginary (odd) part
+    const int u0 = bu * kMfB - kMfR, v0 = bv * kMfB - kMfR;  // raw cell of window (0, 0)
+    const double *wl = win + ((tu * kMfT) * kMfWS + tv * kMfT + (l >> 4)) * 16 + j;
+    const int ngroups = (nactive + 7) / 8;
+    const int g0 = gc * GPW, g1 = min(ngroups, g0 + GPW);
+    for (int g = g0; g < g1; ++g) {
+        __syncthreads();  // the previous group's reads are done
+        mf_stage(G, spec, sstride, active, nactive, g, u0, v0, nbv, threadIdx.x, blockDim.x, win);
+        __syncthreads();
+        lt_double4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 16
+        for (int c = 0; c < 64; ++c) {
+            const double av = Aw[c * 64];
+            const double bval = wl[((c >> 2) * kMfWS + 4 * (c & 3)) * 16];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bval, acc, 0, 0, 0);
+        }
+        const int q = g * 8 + (j >> 1);
+        if (q < nactive) {
+            const int smp = active ? active[q] : q;
+            double *out = reinterpret_cast<double *>(rbuf) + (long)smp * nrow * 2;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = orow[r];
+                if (row < 0) continue;
+                const int i = (l >> 4) + 4 * r;
+                const int ru = bu * kMfB + tu * kMfT + (i >> 2), rv = bv * kMfB + tv * kMfT + (i & 3);
+                const double sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? -1.0 : 1.0;
+                out[(long)row * 2 + (j & 1)] = sg * acc[r];
+            }
+        }
+    }
 }
 
 // final half-plane cells from their raw preimages (unfolded (a, iv) and the
@@ -2484,20 +2627,25 @@ lt_status NormalStencil::ensure_fold(NufftPlan &P) {
     return LT_OK;
 }
 
-// Autotuner of the stencil's samples per workgroup (1, 2, 4, 8): times the
-// whole band normal operator at the full batch for each and keeps the
-// fastest.  imgs / out: nb images of scratch.
+// Autotuner of the stencil's form: samples per workgroup of the vector form
+// (1, 2, 4, 8) and sample groups per workgroup of the matrix-core form (1, 2,
+// 4, 8); times the whole band normal operator at the full batch for each and
+// keeps the fastest.  imgs / out: nb images of scratch.
 lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, double *out, const int *all, int nb) {
     if (!St.use_band || nb <= 0) return LT_OK;
     hipEvent_t e0, e1;
     LT_HIP(hipEventCreate(&e0));
     LT_HIP(hipEventCreate(&e1));
     // three rounds over the candidates, each candidate's best round kept (so
-    // that clocks settling during the first timings favour none of them)
-    const int cands[4] = {1, 2, 4, 8};
-    float tmin[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    // that clocks settling during the first timings favour none of them);
+    // negative: the matrix-core form with that many groups of 8 samples per
+    // workgroup (radius 6 only)
+    const int cands[8] = {1, 2, 4, 8, -1, -2, -4, -8};
+    const int ncand = St.R == kMfR ? 8 : 4;
+    float tmin[8];
+    for (int c = 0; c < 8; ++c) tmin[c] = 1e30f;
     for (int round = 0; round < 3; ++round)
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < ncand; ++c) {
             St.nb_sel = cands[c];
             LT_TRY((normal_band_impl<double, double>(P, St, imgs, out, all, nb, 1.0, false)));  // warm-up
             LT_HIP(hipEventRecord(e0, stream()));
@@ -2512,7 +2660,7 @@ lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, 
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     int best = 0;
-    for (int c = 1; c < 4; ++c)
+    for (int c = 1; c < ncand; ++c)
         if (tmin[c] < tmin[best]) best = c;
     St.nb_sel = cands[best];
     return LT_OK;
@@ -2686,6 +2834,42 @@ lt_status NormalStencil::build(NufftPlan &P, const double *w_sorted) {
     return sync();
 }
 
+lt_status NormalStencil::mfma_build(NufftPlan &P) {
+    if (mf_nblock >= 0) return LT_OK;
+    if (R != kMfR || !use_band) {
+        set_error("normal stencil: the matrix-core form needs the band form with radius %d", kMfR);
+        return LT_ERR_UNSUPPORTED;
+    }
+    Geo G = make_geo(P);
+    const int ntu_b = (P.nu + kMfB - 1) / kMfB;
+    mf_ntv = (P.nv + kMfB - 1) / kMfB;
+    const long nbs = (long)ntu_b * mf_ntv;
+    DevBuf<int64_t> bmark, boff;
+    LT_TRY(bmark.alloc(nbs + 1));
+    LT_TRY(bmark.zero());
+    LT_TRY(boff.alloc(nbs + 1));
+    hipLaunchKernelGGL(k_stencil_tiles, dim3(blocks_for(nrow, 256)), dim3(256), 0, stream(), nrow, row_cell.p, P.nv,
+                       kMfB, mf_ntv, bmark.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(exclusive_scan_i64(bmark.p, boff.p, nbs));
+    int64_t nb = 0;
+    LT_TRY(boff.download(&nb, 1, nbs));
+    LT_TRY(sync());
+    LT_TRY(mf_blocks.alloc(std::max<long>(nb, 1)));
+    hipLaunchKernelGGL(k_compact, dim3(blocks_for(nbs, 256)), dim3(256), 0, stream(), nbs, boff.p, mf_blocks.p);
+    LT_CHECK_LAUNCH();
+    LT_TRY(mf_A.alloc(std::max<long>(nb, 1) * 4 * 64 * 64));
+    LT_TRY(mf_rows.alloc(std::max<long>(nb, 1) * 4 * 16));
+    const long npk = nb * 4 * 64 * 64;
+    if (npk > 0) {
+        hipLaunchKernelGGL(k_mf_pack, dim3(blocks_for(npk, 256)), dim3(256), 0, stream(), G, (long)nb, mf_blocks.p,
+                           mf_ntv, rowmap.p, nrow, coef.p, mf_A.p, mf_rows.p);
+        LT_CHECK_LAUNCH();
+    }
+    mf_nblock = nb;
+    return sync();
+}
+
 // band: spec_in is a [v][u] band of nbv rows (the stencil's rows only); the
 // fold is then left to the caller (spec_out unused)
 template <class T, class TC>
@@ -2696,7 +2880,34 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
     if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
     const int WS = St.ST + 2 * St.R;
     const int ntv_st = (P.nv + St.ST - 1) / St.ST;
-    const int nb_sel = P.force_nb > 0 ? P.force_nb : St.nb_sel;
+    const int nb_sel = P.force_nb != 0 ? P.force_nb : St.nb_sel;
+    if (nb_sel < 0 && std::is_same<T, double>::value && std::is_same<TC, double>::value && band &&
+        St.R == kMfR && nactive > 0) {
+        // matrix-core form: -nb_sel groups of 8 samples per workgroup
+        LT_TRY(St.mfma_build(P));
+        const int gpw = -nb_sel;
+        const int ngroups = (nactive + 7) / 8;
+        const int ngc = (ngroups + gpw - 1) / gpw;
+        const size_t sh = (size_t)kMfWS * kMfWS * 16 * sizeof(double);
+        const dim3 grid((unsigned)(St.mf_nblock * ngc));
+#define LT_STENCIL_MF(GG)                                                                                          \
+    hipLaunchKernelGGL((k_stencil_mfma<GG>), grid, dim3(256), sh, stream(), G, St.mf_blocks.p, St.mf_ntv, ngc,      \
+                       St.mf_A.p, St.mf_rows.p, (const Cplx<double> *)spec_in, sstride, (Cplx<double> *)St.rbuf.p, \
+                       St.nrow, active, nactive, St.nbv);
+        if (St.mf_nblock > 0) {
+            if (gpw == 1)
+                LT_STENCIL_MF(1)
+            else if (gpw == 2)
+                LT_STENCIL_MF(2)
+            else if (gpw == 4)
+                LT_STENCIL_MF(4)
+            else
+                LT_STENCIL_MF(8)
+            LT_CHECK_LAUNCH();
+        }
+#undef LT_STENCIL_MF
+        return LT_OK;
+    }
     if (St.ntile > 0 && nactive > 0) {
         // NB samples share one staged coefficient read (autotuned: 1, 2, 4 or 8)
 #define LT_STENCIL_NB(NBV)                                                                                        \

@@ -67,7 +67,8 @@ struct NormalStencil {
     double coef32_scale = 1.0;
     int nbv = 0;               // spectral columns 0..nbv-1 hold every value the stencil reads
     bool use_band = true;      // band-pruned separable transform instead of the full 2-D one
-    int nb_sel = 4;            // samples per stencil workgroup (stencil_autotune)
+    int nb_sel = 4;            // samples per stencil workgroup (stencil_autotune); < 0: the matrix-core
+                               // form with -nb_sel groups of 8 samples per workgroup
     DevBuf<uint8_t> rbuf;      // per-sample stencil output, [batch][nrow] complex T
     // band form: the [v][u] band cells the fold can make nonzero (row v = 0
     // whole, the others where a stencil row or its mirror exists) with
@@ -79,7 +80,16 @@ struct NormalStencil {
     // image rows, zero elsewhere from allocation on
     DevBuf<uint8_t> band_in[2], band_out[2];
 
+    // matrix-core form of the FP64 band stencil (radius 6): 8 x 8 blocks of
+    // rows, each as four 4 x 4 tiles whose 16 x 256 coefficient matrices are
+    // packed in the MFMA operand layout (mfma_build, made on first use)
+    long mf_nblock = -1;       // -1: not built
+    int mf_ntv = 0;            // blocks along v
+    DevBuf<int32_t> mf_blocks, mf_rows;
+    DevBuf<double> mf_A;
+
     lt_status build(NufftPlan &P, const double *w_sorted);
+    lt_status mfma_build(NufftPlan &P);
     lt_status ensure_fold(NufftPlan &P);  // fold_* (band form)
     lt_status apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active, int nactive);
     lt_status apply32(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
@@ -155,7 +165,8 @@ struct NufftPlan {
     DevBuf<float> twd32;
     bool fused_rows = true;  // use them where nyp allows
     bool wave_rows = true;   // FP32 with nyp = 1024: the one-wave transform (k_rows_fwd_w / k_rows_inv_w)
-    int force_nb = 0;  // stencil samples per workgroup when > 0 (else the autotuned one)
+    int force_nb = 0;  // stencil samples per workgroup when > 0, the matrix-core form with -force_nb
+                       // sample groups per workgroup when < 0 (else the autotuned choice)
     lt_status band_fft(int n, int precision, int nbv, BandFFT *&out);
     DevBuf<uint8_t> realB, specB, specB2, partialB;
     lt_status batch_fft(int n, BatchFFT *&out, int precision = -1);

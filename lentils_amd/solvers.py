@@ -156,8 +156,10 @@ class NUFFTPlan:
         self._has_weights = True
 
     def set_stencil_shape(self, samples_per_group=0):
-        """Forces the normal stencil's samples per workgroup (1, 2, 4, 8; 0:
-        autotuned); every choice gives the same bits."""
+        """Forces the normal stencil's form (0: autotuned): 1, 2, 4, 8 samples
+        per workgroup of the vector form (the same bits), or -1, -2, -4, -8
+        groups of 8 samples per workgroup of the FP64 matrix-core form (the
+        same bits among themselves, the vector form's to rounding)."""
         B.check(B.lib().lt_nufft_set_stencil_shape(self._handle, int(samples_per_group)))
 
     def time(self, direction, reps=20):

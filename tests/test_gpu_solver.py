@@ -112,6 +112,50 @@ def test_stencil_shapes_same_bits(hip, oracle):
     plan.set_stencil_shape(0)
 
 
+def test_stencil_matrix_core_form(hip, oracle):
+    """The FP64 matrix-core form of the normal stencil (v_mfma_f64_16x16x4f64
+    over 4 x 4 tiles of rows) equals the vector form to rounding and the
+    oracle's operator to 1e-12, gives the same bits for every number of
+    sample groups per workgroup (19 samples: groups of 8, 8 and 3), and a CG
+    solve with it (converged samples leaving the launches) reaches the FP64
+    true residual that the oracle's operator confirms."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    n = 19
+    im, src, uv, vs, models, data = _problem(nlens=n, seed=77)
+    lb = LensBatch(im, [src] * n, models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    xs = [rng(90 + k).normal(size=src.size) for k in range(n)]
+    plan.set_stencil_shape(2)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    yv = np.stack([y.reshape(-1) for y in inv.apply(xs)])
+    base = None
+    for g in (-1, -2, -4, -8):
+        plan.set_stencil_shape(g)
+        inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+        ym = np.stack([y.reshape(-1) for y in inv.apply(xs)])
+        err = O.max_relative_error(ym, yv)
+        print(f'matrix-core form, {-g} groups per workgroup: max relative difference to the vector form {err:.2e}')
+        assert err < 1e-13
+        if base is None:
+            base = ym
+            for k in (0, n - 1):
+                ref, _ = _oracle_apply(O, im, src, uv, lb.csr(k), xs[k], w, 1e2)
+                assert O.max_relative_error(ym[k], ref) < 1e-12
+        assert np.array_equal(ym, base), g
+    plan.set_stencil_shape(-2)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    x, its, res = inv.solve(rtol=1e-6, maxiter=20000)
+    print(f'CG with the matrix-core stencil: iterations {its}')
+    assert np.all(res <= 1e-6)
+    k = int(np.argmax(its))
+    Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x[k].reshape(-1), w, 1e2)
+    b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
+    assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= RESID_BAR
+    plan.set_stencil_shape(0)
+
+
 def test_cg_converges_and_oracle_confirms(hip, oracle):
     from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
     O = oracle
