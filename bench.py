@@ -165,21 +165,29 @@ FP64_PEAK_TFLOPS = 78.6
 
 
 def stencil_roofline(wl, nb=NLENS, reps=20):
-    """The timed step's dominant kernel: the normal stencil k_stencil_apply
-    (band form, the plan's resident stencil, nb samples), event-timed on the
-    library stream.  Algorithmic work per launch (lt_nufft_stencil_time): 8 B
-    per FP64 coefficient (169 per stencil row), 16 B per sample per row
-    written and per distinct band value read; 4 flops per row, offset and
-    sample.  Its intensity (~12 flop/B) is above the FP64 ridge
-    (78.6 TFLOP/s / 8 TB/s), so FP64 throughput is the bound."""
+    """The timed step's dominant kernel: the normal stencil (band form, the
+    plan's resident stencil, nb samples, the autotuned form: the FP64
+    matrix-core kernel k_stencil_mfma or the vector kernel k_stencil_apply),
+    event-timed on the library stream.  Algorithmic work per launch
+    (lt_nufft_stencil_time): 8 B per FP64 coefficient (169 per stencil row),
+    16 B per sample per row written and per distinct band value read; 4 flops
+    per row, offset and sample.  Its intensity (~15 flop/B) is above the FP64
+    ridge (78.6 TFLOP/s / 8 TB/s), so FP64 throughput is the bound."""
     B = wl.B
     ms, by, fl = C.c_double(0), C.c_double(0), C.c_double(0)
     rows, win = C.c_long(0), C.c_long(0)
     B.check(B.lib().lt_nufft_stencil_time(wl.plan._handle, nb, reps, C.byref(ms), C.byref(by), C.byref(fl),
                                           C.byref(rows), C.byref(win)))
+    form = C.c_int(0)
+    B.check(B.lib().lt_nufft_stencil_form(wl.plan._handle, C.byref(form)))
     t = ms.value * 1e-3
+    if form.value < 0:
+        kernel = (f'k_stencil_mfma<{-form.value}> (FP64 normal stencil N^T C N on v_mfma_f64_16x16x4f64, band form, '
+                  f'{nb} samples per launch)')
+    else:
+        kernel = f'k_stencil_apply (FP64 normal stencil N^T C N, vector form, band form, {nb} samples per launch)'
     return {'ms': ms.value, 'bytes': by.value, 'flops': fl.value, 'rows': rows.value, 'win_cells': win.value,
-            'tflops': fl.value / t / 1e12, 'gbs': by.value / t / 1e9}
+            'tflops': fl.value / t / 1e12, 'gbs': by.value / t / 1e9, 'form': form.value, 'kernel': kernel}
 
 
 def stream_copy_gbs(nbytes=1 << 30, reps=20):
@@ -706,7 +714,7 @@ def main():
               'method': 'flexible CG, preconditioner: 3 Jacobi-PCG steps on the explicit lambda Cc^T Cc + c L_s^T L_s (L_s: rows of L with <= 64 nonzero entries)'}
         log(f'lens preconditioner: {tl:.2f}s iters median {np.median(itsl):.0f} max res {resl.max():.2e}')
     st = stencil_roofline(wl)
-    log(f'stencil {st["ms"] * 1e3:.1f} us/launch at {NLENS} samples: {st["tflops"]:.2f} TFLOP/s, {st["gbs"]:.0f} GB/s '
+    log(f'stencil (form {st["form"]}) {st["ms"] * 1e3:.1f} us/launch at {NLENS} samples: {st["tflops"]:.2f} TFLOP/s, {st["gbs"]:.0f} GB/s '
         f'({st["rows"]} rows, {st["win_cells"]} window cells)')
     rl = degrid_roofline(wl)
     log(f'degrid f32 {rl["ms"] * 1e3:.1f} us/launch, {rl["gbs"]:.0f} GB/s, {rl["gvis_s"]:.2f} Gvis/s')
@@ -731,13 +739,15 @@ def main():
         'precision_form': pf,
         'lens_preconditioner': lp,
         'degrid_gvis_s': rl['gvis_s'], 'degrid_gvis_s_f64': rl['gvis_s_f64'],
-        'roofline': {'kernel': f'k_stencil_apply (FP64 normal stencil N^T C N, band form, {NLENS} samples per launch)',
+        'roofline': {'kernel': st['kernel'],
                      'bound': 'mfma', 'achieved': st['tflops'], 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': st['tflops'] / FP64_PEAK_TFLOPS, 'traffic': load_traffic('stencil_c2_pmc.json'),
                      'algorithmic_flops': st['flops'], 'algorithmic_bytes': st['bytes'],
                      'achieved_gbs': st['gbs'], 'us_per_launch': st['ms'] * 1e3,
-                     'compute_unit': 'FP64 (VALU FMA; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); '
-                                     'intensity above the FP64 ridge, so FP64 throughput bounds it'},
+                     'compute_unit': 'FP64 matrix cores (MI355X FP64 matrix and vector peaks are both 78.6 '
+                                     'TFLOP/s); intensity above the FP64 ridge, so FP64 throughput bounds it; the '
+                                     'matrix form also multiplies the zeros of each 4x4 tile of rows against its '
+                                     '16x16 window (executed MFMA work 1.75x the algorithmic flops, counted)'},
         'roofline_degrid': {'kernel': 'k_degrid32<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
                             'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS,
                             'traffic': load_traffic('degrid_f32_c2_pmc.json'), 'algorithmic_bytes': rl['bytes'],

@@ -305,6 +305,10 @@ lt_status lt_nufft_set_stencil_shape(lt_nufft h, int samples_per_group);
 lt_status lt_nufft_stencil_time(lt_nufft h, int nb, int reps, double *ms_per_launch, double *bytes, double *flops,
                                 long *rows, long *win_cells);
 
+/* No reference counterpart: the resident stencil's form in use (the forced
+ * one, else the autotuned one; the values of lt_nufft_set_stencil_shape). */
+lt_status lt_nufft_stencil_form(lt_nufft h, int *form);
+
 /* Device memory for resident inputs (plain hipMalloc'd bytes). */
 lt_status lt_device_alloc(long nbytes, void **dptr);
 lt_status lt_device_upload(void *dptr, const void *host, long nbytes);

@@ -1013,43 +1013,57 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, in
 // layout of that instruction: lane l holds A[l % 16][l / 16] and
 // B[l / 16][l % 16], and element r of its result D[l / 16 + 4 r][l % 16].
 // The matrix is packed once per stencil in that layout and read from
-// L2 while a workgroup runs over its sample groups; four waves take the 2 x 2
-// tiles of an 8 x 8 block of rows and share the block's 20 x 20 window,
-// staged in LDS per group of 8 samples.
-constexpr int kMfT = 4;                   // tile edge (rows)
-constexpr int kMfB = 8;                   // block edge (rows)
-constexpr int kMfR = 6;                   // stencil radius
-constexpr int kMfWS = kMfB + 2 * kMfR;    // block window edge (cells)
+// L2 while a workgroup runs over its sample groups; one wave per tile of a
+// block of kMfBTU x kMfBTV tiles, the block's window staged in LDS per group
+// of 8 samples and shared by its tiles.
+constexpr int kMfT = 4;                        // tile edge (rows)
+constexpr int kMfBTU = 2, kMfBTV = 4;          // tiles per block along u and v (one wave each)
+constexpr int kMfNT = kMfBTU * kMfBTV;         // tiles per block
+constexpr int kMfBU = kMfBTU * kMfT, kMfBV = kMfBTV * kMfT;  // block edges (rows)
+constexpr int kMfR = 6;                        // stencil radius
+constexpr int kMfWU = kMfBU + 2 * kMfR, kMfWV = kMfBV + 2 * kMfR;  // block window (cells)
 typedef double lt_double4 __attribute__((ext_vector_type(4)));
 
-// A[((block * 4 + tile) * 64 + step) * 64 + lane] and the tiles' rows
+__global__ void k_mf_blocks(long nrow, const int32_t *__restrict__ row_cell, int nv, int ntv_b,
+                            int64_t *__restrict__ bmark) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const int rc = row_cell[r];
+    const int ru = rc / nv, rv = rc % nv;
+    bmark[(long)(ru / kMfBU) * ntv_b + rv / kMfBV] = 1;
+}
+
+// A[((block * kMfNT + tile) * 32 + step / 2) * 64 + lane][step % 2] and the
+// tiles' rows
 // (-1 where a cell is no stencil row)
 __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks, int ntv_b,
                           const int32_t *__restrict__ rowmap, long nrow, const double *__restrict__ coef,
                           double *__restrict__ A, int32_t *__restrict__ rows) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nblock * 4 * 64 * 64) return;
-    const int l = (int)(g & 63), c = (int)((g >> 6) & 63), t = (int)((g >> 12) & 3);
-    const long b = g >> 14;
+    if (g >= nblock * kMfNT * 64 * 64) return;
+    const int l = (int)(g & 63), c = (int)((g >> 6) & 63);
+    const long bt = g >> 12;
+    const long b = bt / kMfNT;
+    const int t = (int)(bt - b * kMfNT);
     const int blk = blocks[b];
     const int bu = blk / ntv_b, bv = blk - bu * ntv_b;
     const int i = l & 15, k = 4 * c + (l >> 4);
     const int iu = i >> 2, iv = i & 3;   // row within the tile
     const int wu = k >> 4, wv = k & 15;  // cell within the tile's window
-    const int ru = bu * kMfB + (t >> 1) * kMfT + iu, rv = bv * kMfB + (t & 1) * kMfT + iv;
+    const int ru = bu * kMfBU + (t / kMfBTV) * kMfT + iu, rv = bv * kMfBV + (t % kMfBTV) * kMfT + iv;
     int row = -1;
     if (ru < G.nu && rv < G.nv) row = rowmap[(long)ru * G.nv + rv];
     const int ou = wu - iu, ov = wv - iv;  // offset index in [0, 2R]
     constexpr int NR = 2 * kMfR + 1;
     double a = 0.0;
     if (row >= 0 && ou >= 0 && ou < NR && ov >= 0 && ov < NR) a = coef[(long)(ou * NR + ov) * nrow + row];
-    A[g] = a;
-    if (c == 0 && l < 16) rows[(b * 4 + t) * 16 + i] = row;
+    A[((bt * 32 + (c >> 1)) * 64 + l) * 2 + (c & 1)] = a;  // steps in pairs: one 16-byte load per two
+    if (c == 0 && l < 16) rows[bt * 16 + i] = row;
 }
 
 // R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) on the matrix cores.
-// Workgroup = (block, chunk of GPW sample groups): four waves, one per tile
-// of the block, stage the block's window for a group of 8 samples in LDS
+// Workgroup = (block, chunk of GPW sample groups): one wave per tile of the
+// block; the waves stage the block's window for a group of 8 samples in LDS
 // together, then each runs its tile's 64 matrix steps with the packed
 // matrix read from L2 as it goes (the chunks of one block run back to back
 // on one XCD) and the window's values from LDS.
@@ -1064,10 +1078,10 @@ __device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ s
         const int q = g * 8 + s;
         soff[s] = q < nactive ? (long)(active ? active[q] : q) * sstride : -1;
     }
-    for (int ci = t; ci < kMfWS * kMfWS; ci += nt) {
-        const int wv = ci / kMfWS, wu = ci - wv * kMfWS;
+    for (int ci = t; ci < kMfWU * kMfWV; ci += nt) {
+        const int wv = ci / kMfWU, wu = ci - wv * kMfWU;
         const int a = u0 + wu - G.cu, bb = v0 + wv - G.cv;
-        double *cell = win + (wu * kMfWS + wv) * 16;
+        double *cell = win + (wu * kMfWV + wv) * 16;
         if (abs(a) > G.bu || abs(bb) > G.bv || abs(bb) >= nbv) {
 #pragma unroll
             for (int s = 0; s < 16; ++s) cell[s] = 0.0;
@@ -1095,27 +1109,27 @@ __device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ s
 }
 
 template <int GPW>
-__global__ void __launch_bounds__(256) k_stencil_mfma(Geo G, const int32_t *__restrict__ blocks, int ntv_b, int ngc,
+__global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma(Geo G, const int32_t *__restrict__ blocks, int ntv_b, int ngc,
                                                       const double *__restrict__ A, const int32_t *__restrict__ rows,
                                                       const Cplx<double> *__restrict__ spec, long sstride,
                                                       Cplx<double> *__restrict__ rbuf, long nrow,
                                                       const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
-    double *win = reinterpret_cast<double *>(smem);  // [kMfWS * kMfWS cells][16]
+    double *win = reinterpret_cast<double *>(smem);  // [kMfWU * kMfWV cells][16]
     const long lb = xcd_block(blockIdx.x, gridDim.x);
     const long b = lb / ngc;
     const int gc = (int)(lb - b * ngc);
     const int blk = blocks[b];
     const int bu = blk / ntv_b, bv = blk - bu * ntv_b;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int tu = w >> 1, tv = w & 1;
-    const double *Aw = A + (b * 4 + w) * 64 * 64 + l;
+    const int tu = w / kMfBTV, tv = w - tu * kMfBTV;
+    const double2 *Aw = reinterpret_cast<const double2 *>(A) + (b * kMfNT + w) * 32 * 64 + l;
     int orow[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) orow[r] = rows[(b * 4 + w) * 16 + (l >> 4) + 4 * r];
+    for (int r = 0; r < 4; ++r) orow[r] = rows[(b * kMfNT + w) * 16 + (l >> 4) + 4 * r];
     const int j = l & 15;  // column: sample j / 2 of the group, real (even) or imaginary (odd) part
-    const int u0 = bu * kMfB - kMfR, v0 = bv * kMfB - kMfR;  // raw cell of window (0, 0)
-    const double *wl = win + ((tu * kMfT) * kMfWS + tv * kMfT + (l >> 4)) * 16 + j;
+    const int u0 = bu * kMfBU - kMfR, v0 = bv * kMfBV - kMfR;  // raw cell of window (0, 0)
+    const double *wl = win + ((tu * kMfT) * kMfWV + tv * kMfT + (l >> 4)) * 16 + j;
     const int ngroups = (nactive + 7) / 8;
     const int g0 = gc * GPW, g1 = min(ngroups, g0 + GPW);
     for (int g = g0; g < g1; ++g) {
@@ -1123,11 +1137,14 @@ __global__ void __launch_bounds__(256) k_stencil_mfma(Geo G, const int32_t *__re
         mf_stage(G, spec, sstride, active, nactive, g, u0, v0, nbv, threadIdx.x, blockDim.x, win);
         __syncthreads();
         lt_double4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 16
-        for (int c = 0; c < 64; ++c) {
-            const double av = Aw[c * 64];
-            const double bval = wl[((c >> 2) * kMfWS + 4 * (c & 3)) * 16];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bval, acc, 0, 0, 0);
+#pragma unroll 8
+        for (int c2 = 0; c2 < 32; ++c2) {
+            const double2 av = Aw[c2 * 64];
+            const int c = 2 * c2;
+            const double b0 = wl[((c >> 2) * kMfWV + 4 * (c & 3)) * 16];
+            const double b1 = wl[(((c + 1) >> 2) * kMfWV + 4 * ((c + 1) & 3)) * 16];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, b1, acc, 0, 0, 0);
         }
         const int q = g * 8 + (j >> 1);
         if (q < nactive) {
@@ -1138,7 +1155,7 @@ __global__ void __launch_bounds__(256) k_stencil_mfma(Geo G, const int32_t *__re
                 const int row = orow[r];
                 if (row < 0) continue;
                 const int i = (l >> 4) + 4 * r;
-                const int ru = bu * kMfB + tu * kMfT + (i >> 2), rv = bv * kMfB + tv * kMfT + (i & 3);
+                const int ru = bu * kMfBU + tu * kMfT + (i >> 2), rv = bv * kMfBV + tv * kMfT + (i & 3);
                 const double sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? -1.0 : 1.0;
                 out[(long)row * 2 + (j & 1)] = sg * acc[r];
             }
@@ -2841,15 +2858,15 @@ lt_status NormalStencil::mfma_build(NufftPlan &P) {
         return LT_ERR_UNSUPPORTED;
     }
     Geo G = make_geo(P);
-    const int ntu_b = (P.nu + kMfB - 1) / kMfB;
-    mf_ntv = (P.nv + kMfB - 1) / kMfB;
+    const int ntu_b = (P.nu + kMfBU - 1) / kMfBU;
+    mf_ntv = (P.nv + kMfBV - 1) / kMfBV;
     const long nbs = (long)ntu_b * mf_ntv;
     DevBuf<int64_t> bmark, boff;
     LT_TRY(bmark.alloc(nbs + 1));
     LT_TRY(bmark.zero());
     LT_TRY(boff.alloc(nbs + 1));
-    hipLaunchKernelGGL(k_stencil_tiles, dim3(blocks_for(nrow, 256)), dim3(256), 0, stream(), nrow, row_cell.p, P.nv,
-                       kMfB, mf_ntv, bmark.p);
+    hipLaunchKernelGGL(k_mf_blocks, dim3(blocks_for(nrow, 256)), dim3(256), 0, stream(), nrow, row_cell.p, P.nv,
+                       mf_ntv, bmark.p);
     LT_CHECK_LAUNCH();
     LT_TRY(exclusive_scan_i64(bmark.p, boff.p, nbs));
     int64_t nb = 0;
@@ -2858,9 +2875,9 @@ lt_status NormalStencil::mfma_build(NufftPlan &P) {
     LT_TRY(mf_blocks.alloc(std::max<long>(nb, 1)));
     hipLaunchKernelGGL(k_compact, dim3(blocks_for(nbs, 256)), dim3(256), 0, stream(), nbs, boff.p, mf_blocks.p);
     LT_CHECK_LAUNCH();
-    LT_TRY(mf_A.alloc(std::max<long>(nb, 1) * 4 * 64 * 64));
-    LT_TRY(mf_rows.alloc(std::max<long>(nb, 1) * 4 * 16));
-    const long npk = nb * 4 * 64 * 64;
+    LT_TRY(mf_A.alloc(std::max<long>(nb, 1) * kMfNT * 64 * 64));
+    LT_TRY(mf_rows.alloc(std::max<long>(nb, 1) * kMfNT * 16));
+    const long npk = nb * kMfNT * 64 * 64;
     if (npk > 0) {
         hipLaunchKernelGGL(k_mf_pack, dim3(blocks_for(npk, 256)), dim3(256), 0, stream(), G, (long)nb, mf_blocks.p,
                            mf_ntv, rowmap.p, nrow, coef.p, mf_A.p, mf_rows.p);
@@ -2888,10 +2905,11 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
         const int gpw = -nb_sel;
         const int ngroups = (nactive + 7) / 8;
         const int ngc = (ngroups + gpw - 1) / gpw;
-        const size_t sh = (size_t)kMfWS * kMfWS * 16 * sizeof(double);
+        const size_t sh = (size_t)kMfWU * kMfWV * 16 * sizeof(double);
         const dim3 grid((unsigned)(St.mf_nblock * ngc));
 #define LT_STENCIL_MF(GG)                                                                                          \
-    hipLaunchKernelGGL((k_stencil_mfma<GG>), grid, dim3(256), sh, stream(), G, St.mf_blocks.p, St.mf_ntv, ngc,      \
+    hipLaunchKernelGGL((k_stencil_mfma<GG>), grid, dim3(64 * kMfNT), sh, stream(), G, St.mf_blocks.p, St.mf_ntv,   \
+                       ngc,                                                                                         \
                        St.mf_A.p, St.mf_rows.p, (const Cplx<double> *)spec_in, sstride, (Cplx<double> *)St.rbuf.p, \
                        St.nrow, active, nactive, St.nbv);
         if (St.mf_nblock > 0) {

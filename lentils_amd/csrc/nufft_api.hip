@@ -282,6 +282,13 @@ lt_status lt_nufft_stencil_time(lt_nufft h, int nb, int reps, double *ms, double
     return stencil_time(h->plan, nb, reps, ms, bytes, flops, rows, win_cells);
 }
 
+lt_status lt_nufft_stencil_form(lt_nufft h, int *form) {
+    if (!h || !form) return LT_ERR_ARG;
+    const NormalStencil *St = h->plan.wstencil.get();
+    *form = h->plan.force_nb != 0 ? h->plan.force_nb : (St ? St->nb_sel : 0);
+    return LT_OK;
+}
+
 lt_status lt_nufft_destroy(lt_nufft h) {
     delete h;
     return LT_OK;
