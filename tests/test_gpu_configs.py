@@ -8,7 +8,10 @@ times), against the oracle:
   visibilities on 2048 x 1025, FP32 operator inside FP64 CG, q up to 0.999
   and radii down to 1e-4" for the series;
 * batching workload: 256 lens samples with source grids drawn from
-  {32, 48, 64, 96, 128}^2 and a shared 1e6-visibility uv set.
+  {32, 48, 64, 96, 128}^2 and a shared 1e6-visibility uv set;
+* learned-preconditioner workload: 256 lens samples, 128^2 sources, 1e7
+  visibilities, the network trained on the GPU on the solver's vectors and
+  used inside flexible CG.
 
 Bars: lens-matrix row pointers and columns identical to the oracle (values
 to the end-to-end bar), deflections 1e-12, FP64 visibilities 1e-12 and
@@ -122,6 +125,47 @@ def test_batching_workload(hip, oracle):
         r = _oracle_residual(O, wl.im, wl.srcs[s], wl.uvw, wl.weights, wl.lambda_curv, lb.csr(s),
                              xs[s].reshape(-1), wl.dirty[s])
         print(f'sample {s} ({sizes[s]}^2): GPU residual {res[s]:.4e}, oracle residual {r:.4e}')
+        assert r <= RESID_BAR
+
+
+def test_learned_workload(hip, oracle):
+    """The learned-preconditioner workload at its size (256 samples of 128^2,
+    1e7 visibilities): a network trained briefly on the GPU on the solver's
+    own vectors (random lenses of the same family, blob sources) is used in
+    flexible CG; every sample reaches the FP64 true residual, the oracle's
+    operator confirms two of them, and the guard and switch-off counters are
+    reported (bench.py --learned measures the full training)."""
+    import bench
+    from lentils_amd.learned import PCNet, blob_sources, solver_residuals
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch
+    O = oracle
+    nlens = 256
+    wl = bench.Workload(nlens=nlens)
+    net = PCNet(128, 128, batch=nlens, seed=3)
+    lbt = LensBatch(wl.im, wl.srcs, bench.gen_lenses(nlens, seed=1000), z_src=2.0)
+    invt = BatchedSourceInversion(lbt, wl.plan, dirty_images=wl.ddirty, lambda_curvature=wl.lambda_curv)
+    xs0 = blob_sources(wl.src, nlens * 4, rng(95))
+    draws = [list(xs0[d * nlens:(d + 1) * nlens]) for d in range(4)]
+    vecs = solver_residuals(invt, draws, iterations=(0, 8, 32), precond='lens')
+    first = invt.train_preconditioner(PCNet(128, 128, batch=nlens, seed=3), vecs, epochs=0)[0]
+    hist = invt.train_preconditioner(net, vecs, epochs=4, lr=2e-3, lr_final=2e-4)
+    last = invt.train_preconditioner(net, vecs, epochs=0)[0]
+    print(f'loss of M alone {first:.4f}; training {np.round(hist, 4)}; trained {last:.4f}')
+    assert last < first
+    del invt, lbt
+    lb = LensBatch(wl.im, wl.srcs, wl.lenses, z_src=2.0)
+    inv = BatchedSourceInversion(lb, wl.plan, dirty_images=wl.ddirty, lambda_curvature=wl.lambda_curv)
+    inv.set_preconditioner(net)
+    inv.set_flexible_depth(4)
+    xs, its, res = inv.solve(rtol=1e-6, maxiter=20000, precond='learned')
+    c, sw = inv.fallbacks()
+    print(f'learned workload: iterations median {np.median(its)}, max {its.max()}, max residual {res.max():.3e}; '
+          f'guard fallbacks {c.sum()}, samples switched to M {sw.sum()}')
+    assert np.all(res <= 1e-6)
+    for s in (0, 200):
+        r = _oracle_residual(O, wl.im, wl.src, wl.uvw, wl.weights, wl.lambda_curv, lb.csr(s), xs[s].reshape(-1),
+                             wl.dirty[s])
+        print(f'sample {s}: GPU residual {res[s]:.4e}, oracle residual {r:.4e}')
         assert r <= RESID_BAR
 
 
