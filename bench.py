@@ -190,6 +190,27 @@ def stencil_roofline(wl, nb=NLENS, reps=20):
             'tflops': fl.value / t / 1e12, 'gbs': by.value / t / 1e9, 'form': form.value, 'kernel': kernel}
 
 
+def spmv_roofline(wl, reps=20):
+    """The lens-matrix products of the plain workload's solver at the full
+    batch, event-timed on the library stream (lt_solver_time_products):
+    L x over the nonempty rows and L^T y (with its long-row chunks).
+    Algorithmic bytes per product: (4 + 8) per nonzero (int32 column, FP64
+    value) + 8 per row + 8 per column (y written, x read once)."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch
+    B = wl.B
+    lb = LensBatch(wl.im, wl.srcs, wl.lenses, z_src=2.0)
+    inv = BatchedSourceInversion(lb, wl.plan, dirty_images=wl.ddirty, lambda_curvature=wl.lambda_curv)
+    ml, mt = C.c_double(0), C.c_double(0)
+    shl, sht = (C.c_long * 3)(), (C.c_long * 3)()
+    B.check(B.lib().lt_solver_time_products(inv._handle, reps, C.byref(ml), C.byref(mt), shl, sht))
+    out = {}
+    for name, ms, sh in (('L', ml.value, shl), ('LT', mt.value, sht)):
+        nrow, ncol, nnz = sh[0], sh[1], sh[2]
+        by = 12.0 * nnz + 8.0 * nrow + 8.0 * ncol
+        out[name] = {'ms': ms, 'bytes': by, 'gbs': by / (ms * 1e-3) / 1e9, 'rows': nrow, 'cols': ncol, 'nnz': nnz}
+    return out
+
+
 def stream_copy_gbs(nbytes=1 << 30, reps=20):
     """Achievable HBM bandwidth: a plain device-to-device streaming copy."""
     from lentils_amd import backend as B
@@ -720,6 +741,9 @@ def main():
     log(f'degrid f32 {rl["ms"] * 1e3:.1f} us/launch, {rl["gbs"]:.0f} GB/s, {rl["gvis_s"]:.2f} Gvis/s')
     copy_gbs, _ = stream_copy_gbs()
     log(f'streaming copy {copy_gbs:.0f} GB/s')
+    sp = spmv_roofline(wl)
+    log(f'L^T product {sp["LT"]["ms"] * 1e3:.1f} us ({sp["LT"]["gbs"]:.0f} GB/s), L product {sp["L"]["ms"] * 1e3:.1f} us '
+        f'({sp["L"]["gbs"]:.0f} GB/s)')
     out = {
         'metric': 'source solves/s at 1e-6 rel resid (learned-PCG vs plain CG); Gvis/s degrid',
         'value': value, 'unit': 'source solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
@@ -748,6 +772,14 @@ def main():
                                      'TFLOP/s); intensity above the FP64 ridge, so FP64 throughput bounds it; the '
                                      'matrix form also multiplies the zeros of each 4x4 tile of rows against its '
                                      '16x16 window (executed MFMA work 1.75x the algorithmic flops, counted)'},
+        'roofline_spmv': {'kernel': 'L^T y (k_spmv_stream + long-row chunks, FP64, all 64 samples)', 'bound': 'hbm',
+                          'achieved': sp['LT']['gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                          'frac': sp['LT']['gbs'] / HBM_PEAK_GBS, 'traffic': None,
+                          'algorithmic_bytes': sp['LT']['bytes'], 'us_per_launch': sp['LT']['ms'] * 1e3,
+                          'nnz': sp['LT']['nnz'],
+                          'L': {'us': sp['L']['ms'] * 1e3, 'achieved_gbs': sp['L']['gbs'],
+                                'frac': sp['L']['gbs'] / HBM_PEAK_GBS, 'algorithmic_bytes': sp['L']['bytes'],
+                                'nnz': sp['L']['nnz']}},
         'roofline_degrid': {'kernel': 'k_degrid32<float,3> (FP32 degridder)', 'bound': 'hbm', 'achieved': rl['gbs'],
                             'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': rl['gbs'] / HBM_PEAK_GBS,
                             'traffic': load_traffic('degrid_f32_c2_pmc.json'), 'algorithmic_bytes': rl['bytes'],

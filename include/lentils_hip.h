@@ -419,6 +419,12 @@ lt_status lt_solver_set_learned_iterations(lt_solver s, int n);
 lt_status lt_solver_set_rhs(lt_solver s, const double *rhs);
 /* y = A x for all samples, in the precision set by lt_solver_set_precision */
 lt_status lt_solver_apply(lt_solver s, const double *x, double *y);
+/* No reference counterpart: times the solver's two lens-matrix products
+ * (L x over the nonempty rows, and L^T y, all samples) with HIP events on the
+ * library stream, reps launches each after one untimed; mean ms per
+ * product and each matrix's (rows, cols, nonzeros) (any may be NULL). */
+lt_status lt_solver_time_products(lt_solver s, int reps, double *ms_L, double *ms_LT, long *shape_L,
+                                  long *shape_LT);
 lt_status lt_solver_destroy(lt_solver s);
 
 /* --- learned preconditioner ------------------------------------------------

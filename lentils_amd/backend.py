@@ -149,6 +149,7 @@ def lib():
         'lt_solver_set_lens_pc': ([vp, i, i, d], i),
         'lt_solver_set_graphs': ([vp, i], i),
         'lt_solver_set_flexible_depth': ([vp, i], i),
+        'lt_solver_time_products': ([vp, i, dp, dp, lp, lp], i),
         'lt_solver_set_learned_iterations': ([vp, i], i),
         'lt_solver_lens_pc_matrix': ([vp, vp, vp, vp, vp, vp], i),
         'lt_solver_apply': ([vp, vp, vp], i),
@@ -184,7 +185,7 @@ def exported_symbols():
             'lt_solver_create', 'lt_solver_create_dirty', 'lt_pcnet_param_count', 'lt_pcnet_create',
             'lt_pcnet_get_params', 'lt_pcnet_set_params', 'lt_pcnet_forward_host', 'lt_pcnet_grad_host',
             'lt_pcnet_train', 'lt_pcnet_eval', 'lt_pcnet_destroy', 'lt_solver_set_pcnet', 'lt_solver_diag', 'lt_solver_train_pcnet',
-            'lt_solver_pc_fallbacks', 'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_set_rhs', 'lt_solver_set_precision', 'lt_solver_set_lens_pc', 'lt_solver_set_graphs', 'lt_solver_set_flexible_depth', 'lt_solver_set_learned_iterations', 'lt_solver_lens_pc_matrix', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
+            'lt_solver_pc_fallbacks', 'lt_solver_solve', 'lt_solver_rhs', 'lt_solver_set_rhs', 'lt_solver_set_precision', 'lt_solver_set_lens_pc', 'lt_solver_set_graphs', 'lt_solver_set_flexible_depth', 'lt_solver_set_learned_iterations', 'lt_solver_time_products', 'lt_solver_lens_pc_matrix', 'lt_solver_apply', 'lt_solver_destroy', 'lt_fft_r2c_host',
             'lt_fft_c2r_host', 'lt_csr_spmv_host', 'lt_csr_spmv_host_kernel', 'lt_csr_spmv_device_noalloc', 'lt_stream_copy']
 
 

@@ -1808,6 +1808,50 @@ lt_status lt_solver_apply(lt_solver h, const double *x, double *y) {
     return sync();
 }
 
+lt_status lt_solver_time_products(lt_solver h, int reps, double *ms_L, double *ms_LT, long *shape_L,
+                                  long *shape_LT) {
+    if (!h || reps <= 0) return LT_ERR_ARG;
+    Solver &S = h->s;
+    LensBatch *lb = S.lb;
+    DevBuf<double> x, y;
+    LT_TRY(x.alloc(S.ntot));
+    LT_TRY(y.alloc(S.ntot));
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(S.ntot, 256)), dim3(256), 0, stream(), S.ntot, 1.0, x.p);
+    LT_CHECK_LAUNCH();
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    float t = 0.f;
+    // L x over the nonempty rows (as the solver forms it), then L^T of that image
+    LT_TRY(csr_spmv_nz(lb->Lp, x.p, S.img1.p, nullptr, S.npix, nullptr, 0));
+    LT_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r) LT_TRY(csr_spmv_nz(lb->Lp, x.p, S.img1.p, nullptr, S.npix, nullptr, 0));
+    LT_HIP(hipEventRecord(e1, stream()));
+    LT_HIP(hipEventSynchronize(e1));
+    LT_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms_L) *ms_L = t / reps;
+    LT_TRY(csr_spmv(lb->LT, S.img1.p, y.p, 1.0, 0.0, 0));
+    LT_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r) LT_TRY(csr_spmv(lb->LT, S.img1.p, y.p, 1.0, 0.0, 0));
+    LT_HIP(hipEventRecord(e1, stream()));
+    LT_HIP(hipEventSynchronize(e1));
+    LT_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms_LT) *ms_LT = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (shape_L) {
+        shape_L[0] = lb->Lp.nrows;
+        shape_L[1] = lb->Lp.ncols;
+        shape_L[2] = lb->Lp.nnz;
+    }
+    if (shape_LT) {
+        shape_LT[0] = lb->LT.nrows;
+        shape_LT[1] = lb->LT.ncols;
+        shape_LT[2] = lb->LT.nnz;
+    }
+    return sync();
+}
+
 lt_status lt_solver_set_pcnet(lt_solver h, lt_pcnet net) {
     if (!h) return LT_ERR_ARG;
     h->s.net = pcnet_of(net);
