@@ -306,6 +306,64 @@ __global__ void k_fcg_p(long n, const int32_t *__restrict__ seg, const int *__re
     p[i] = v;
 }
 
+// y += lam C (C x) on the listed samples (list == NULL: all; inactive ones
+// skipped), C the curvature matrix of each sample's source grid: the 5-point
+// Laplacian times 1 / (dx dy) with the terms beyond the grid's edge dropped
+// (the matrix build_curvature makes).  Both products in one pass over a
+// 16 x 16 tile of the output with its halo staged in LDS; each row's terms
+// are summed in the matrix's column order (up, left, centre, right, down).
+constexpr int kCvT = 16;
+__device__ inline double curv_row(int i, int j, int nx, int ny, double h, double up, double lf, double c, double rt,
+                                  double dn) {
+    double s = 0.0;
+    if (i > 0) s += h * up;
+    if (j > 0) s += h * lf;
+    s += (-4.0 * h) * c;
+    if (j < ny - 1) s += h * rt;
+    if (i < nx - 1) s += h * dn;
+    return s;
+}
+
+__global__ void __launch_bounds__(256) k_curv2(const int *__restrict__ list, const int *__restrict__ act,
+                                               const int64_t *__restrict__ off, const int *__restrict__ nxs,
+                                               const int *__restrict__ nys, const double *__restrict__ hs, double lam,
+                                               const double *__restrict__ x, double *__restrict__ y) {
+    __shared__ double xs[kCvT + 4][kCvT + 4];
+    __shared__ double ts[kCvT + 2][kCvT + 2];
+    const int s = list ? list[blockIdx.y] : (int)blockIdx.y;
+    if (act && !act[s]) return;
+    const int nx = nxs[s], ny = nys[s];
+    const int ntj = (ny + kCvT - 1) / kCvT;
+    const int ti = (int)blockIdx.x / ntj, tj = (int)blockIdx.x - ti * ntj;
+    if (ti * kCvT >= nx) return;
+    const int i0 = ti * kCvT - 2, j0 = tj * kCvT - 2;
+    const double *xv = x + off[s];
+    for (int q = threadIdx.x; q < (kCvT + 4) * (kCvT + 4); q += blockDim.x) {
+        const int a = q / (kCvT + 4), b = q - a * (kCvT + 4);
+        const int i = i0 + a, j = j0 + b;
+        xs[a][b] = (i >= 0 && i < nx && j >= 0 && j < ny) ? xv[(long)i * ny + j] : 0.0;
+    }
+    __syncthreads();
+    const double h = hs[s];
+    for (int q = threadIdx.x; q < (kCvT + 2) * (kCvT + 2); q += blockDim.x) {
+        const int a = q / (kCvT + 2), b = q - a * (kCvT + 2);
+        const int i = i0 + 1 + a, j = j0 + 1 + b;
+        double t = 0.0;
+        if (i >= 0 && i < nx && j >= 0 && j < ny)
+            t = curv_row(i, j, nx, ny, h, xs[a][b + 1], xs[a + 1][b], xs[a + 1][b + 1], xs[a + 1][b + 2],
+                         xs[a + 2][b + 1]);
+        ts[a][b] = t;
+    }
+    __syncthreads();
+    const int a = threadIdx.x / kCvT, b = threadIdx.x - a * kCvT;
+    const int i = ti * kCvT + a, j = tj * kCvT + b;
+    if (i >= nx || j >= ny) return;
+    const double v = curv_row(i, j, nx, ny, h, ts[a][b + 1], ts[a + 1][b], ts[a + 1][b + 1], ts[a + 1][b + 2],
+                              ts[a + 2][b + 1]);
+    double *yo = y + off[s] + (long)i * ny + j;
+    *yo = lam * v + *yo;
+}
+
 __global__ void k_recip(long n, const double *__restrict__ a, double *__restrict__ b) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = 1.0 / a[i];
@@ -597,6 +655,9 @@ struct Solver {
     DevBuf<int64_t> soff;
     DevBuf<int32_t> seg;
     DevCSR C;                 // block-diagonal curvature of every source grid
+    DevBuf<int> cnx, cny;     // per-sample source grid shape and 1 / (dx dy) (k_curv2)
+    DevBuf<double> ch;
+    int cv_tiles = 0;         // 16 x 16 tiles of the largest source grid
     DevBuf<double> w;         // visibility weights in plan order (own copy)
     const double *wp = nullptr;  // the weights used: w.p, or the plan's resident ones
     DevBuf<double> dirty;     // D^T C d_s (ndirty x nx*ny)
@@ -772,6 +833,19 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
         LT_HIP(hipMemcpyAsync(C.rp.p + ntot, &last, sizeof(int32_t), hipMemcpyHostToDevice, stream()));
         LT_TRY(sync());
         LT_TRY(csr_prepare(C));
+        std::vector<int> hnx(S), hny(S);
+        std::vector<double> hh(S);
+        cv_tiles = 0;
+        for (int s = 0; s < S; ++s) {
+            const SrcGrid &g = lb->src_host[s];
+            hnx[s] = g.nx;
+            hny[s] = g.ny;
+            hh[s] = 1.0 / (g.dx * g.dy);
+            cv_tiles = std::max(cv_tiles, ((g.nx + kCvT - 1) / kCvT) * ((g.ny + kCvT - 1) / kCvT));
+        }
+        LT_TRY(cnx.upload(hnx.data(), S));
+        LT_TRY(cny.upload(hny.data(), S));
+        LT_TRY(ch.upload(hh.data(), S));
     }
     // weights and data into plan order; dirty image D^T C d
     const long nv = nf->nvis;
@@ -881,9 +955,12 @@ lt_status Solver::apply(const double *xin, double *yout, const int *alist, int n
     else
         LT_TRY(nf->convolve_batched(img1.p, img2.p, beamspec.p, alist, nact, 1.0, false));
     LT_TRY(csr_spmv(lb->LT, img2.p, yout, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
-    if (lam_c != 0.0) {
-        LT_TRY(csr_spmv(C, xin, tmp.p, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
-        LT_TRY(csr_spmv(C, tmp.p, yout, lam_c, 1.0, 0, af, 0, seg.p, alist, nact));
+    if (lam_c != 0.0 && cv_tiles > 0) {
+        // alist with nact == 0 means all samples
+        const bool listed = alist && nact > 0;
+        hipLaunchKernelGGL(k_curv2, dim3(cv_tiles, listed ? nact : S), dim3(kCvT * kCvT), 0, stream(),
+                           listed ? alist : nullptr, af, soff.p, cnx.p, cny.p, ch.p, lam_c, xin, yout);
+        LT_CHECK_LAUNCH();
     }
     if (lam_t != 0.0) {
         hipLaunchKernelGGL(k_axpby, dim3(blocks_for(ntot, 256)), dim3(256), 0, stream(), ntot, lam_t, xin, 1.0, yout);
