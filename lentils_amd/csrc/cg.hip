@@ -624,6 +624,75 @@ __global__ void k_inner_xr(long n, const int32_t *__restrict__ seg, const double
     z[i] = r[i] / dM[i];
 }
 
+// The inner steps with their per-sample dots folded in (one workgroup per
+// sample, the elements and the reduction in k_seg_dot_one's order, so the
+// results are those of the separate kernels bit for bit).
+__device__ inline double block_sum_one(double acc) {
+    __shared__ double sh[kDotOne / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kDotOne / 64; ++k) t += sh[k];
+    return t;  // valid in thread 0
+}
+
+// start: x = 0, r = r0, z = p = r0 / dM, rz = r . z
+__global__ void __launch_bounds__(kDotOne) k_inner_start_dot(const int64_t *__restrict__ off,
+                                                              const double *__restrict__ r0,
+                                                              const double *__restrict__ dM, double *__restrict__ x,
+                                                              double *__restrict__ r, double *__restrict__ z,
+                                                              double *__restrict__ p, double *__restrict__ rz) {
+    const int s = blockIdx.x;
+    const long lo = off[s], hi = off[s + 1];
+    double acc = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) {
+        const double ri = r0[i], zi = ri / dM[i];
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        acc += ri * zi;
+    }
+    const double t = block_sum_one(acc);
+    if (threadIdx.x == 0) rz[s] = t;
+}
+
+// step: x += a p; r -= a Mp; z = r / dM (a = rz / pMp); rz_new = r . z; and,
+// when another step follows, p = z + (rz_new / rz) p
+__global__ void __launch_bounds__(kDotOne) k_inner_step_dot(const int64_t *__restrict__ off,
+                                                             const double *__restrict__ rz,
+                                                             const double *__restrict__ pMp, double *__restrict__ p,
+                                                             const double *__restrict__ Mp,
+                                                             const double *__restrict__ dM, double *__restrict__ x,
+                                                             double *__restrict__ r, double *__restrict__ z,
+                                                             double *__restrict__ rz_new, int update_p) {
+    __shared__ double bsh;
+    const int s = blockIdx.x;
+    const long lo = off[s], hi = off[s + 1];
+    const double a = pMp[s] != 0.0 ? rz[s] / pMp[s] : 0.0;
+    double acc = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) {
+        x[i] += a * p[i];
+        const double ri = r[i] - a * Mp[i];
+        r[i] = ri;
+        const double zi = ri / dM[i];
+        z[i] = zi;
+        acc += ri * zi;
+    }
+    const double t = block_sum_one(acc);
+    if (threadIdx.x == 0) {
+        rz_new[s] = t;
+        bsh = rz[s] != 0.0 ? t / rz[s] : 0.0;
+    }
+    if (!update_p) return;
+    __syncthreads();
+    const double b = bsh;
+    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) p[i] = z[i] + b * p[i];
+}
+
 // p = z + (rz_new / rz) p
 __global__ void k_inner_p(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
                           const double *__restrict__ rz_new, const double *__restrict__ z, double *__restrict__ p) {
@@ -634,7 +703,6 @@ __global__ void k_inner_p(long n, const int32_t *__restrict__ seg, const double 
     p[i] = z[i] + b * p[i];
 }
 
-// samples that left the network (plain CG fallback) take z = r
 // samples whose network was switched off take the lens-structured direction
 __global__ void k_pc_lensonly(long n, const int32_t *__restrict__ seg, const int *__restrict__ usenet,
                               const double *__restrict__ m, double *__restrict__ z) {
@@ -1114,18 +1182,34 @@ lt_status Solver::lens_apply(const int *af, const int *alist, int nact) {
     // the inner iterate is z itself (no final copy); with no inner steps z is
     // the Jacobi step r / dM
     double *xi = lens_inner > 0 ? z.p : ix.p;
-    hipLaunchKernelGGL(k_inner_start, dim3(gb), dim3(256), 0, stream(), ntot, r.p, dM.p, xi, ir.p, iz.p, ip.p);
     double *rzc = irz.p, *rzn = irz2.p;  // r . z of the current and the next inner step (swapped, not copied)
-    LT_TRY(seg_dot(ir.p, iz.p, rzc));
+    // with one workgroup per sample's dot, the updates and the dots share
+    // one launch (same bits as the separate kernels)
+    const bool fused = (dot_parts == 1);
+    if (fused) {
+        hipLaunchKernelGGL(k_inner_start_dot, dim3(S), dim3(kDotOne), 0, stream(), soff.p, r.p, dM.p, xi, ir.p, iz.p,
+                           ip.p, rzc);
+    } else {
+        hipLaunchKernelGGL(k_inner_start, dim3(gb), dim3(256), 0, stream(), ntot, r.p, dM.p, xi, ir.p, iz.p, ip.p);
+        LT_TRY(seg_dot(ir.p, iz.p, rzc));
+    }
     LT_CHECK_LAUNCH();
     for (int k = 0; k < lens_inner; ++k) {
         // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc
         LT_TRY(csr_spmv(Mpc, ip.p, iMp.p, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
         LT_TRY(seg_dot(ip.p, iMp.p, ipMp.p));
+        const bool more = k + 1 < lens_inner;
+        if (fused) {
+            hipLaunchKernelGGL(k_inner_step_dot, dim3(S), dim3(kDotOne), 0, stream(), soff.p, rzc, ipMp.p, ip.p,
+                               iMp.p, dM.p, xi, ir.p, iz.p, rzn, more ? 1 : 0);
+            LT_CHECK_LAUNCH();
+            if (more) std::swap(rzc, rzn);
+            continue;
+        }
         hipLaunchKernelGGL(k_inner_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rzc, ipMp.p, ip.p, iMp.p,
                            dM.p, xi, ir.p, iz.p);
         LT_CHECK_LAUNCH();
-        if (k + 1 < lens_inner) {
+        if (more) {
             LT_TRY(seg_dot(ir.p, iz.p, rzn));
             hipLaunchKernelGGL(k_inner_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rzc, rzn, iz.p, ip.p);
             LT_CHECK_LAUNCH();
