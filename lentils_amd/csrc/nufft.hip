@@ -1022,6 +1022,11 @@ constexpr int kMfNT = kMfBTU * kMfBTV;         // tiles per block
 constexpr int kMfBU = kMfBTU * kMfT, kMfBV = kMfBTV * kMfT;  // block edges (rows)
 constexpr int kMfR = 6;                        // stencil radius
 constexpr int kMfWU = kMfBU + 2 * kMfR, kMfWV = kMfBV + 2 * kMfR;  // block window (cells)
+// staged window: cells in pairs along u (u even, u + 1), each column's two
+// values side by side, so that one 16-byte LDS read gives a lane its values
+// for two matrix steps; rows of pairs padded by 4 doubles
+constexpr int kMfR2 = kMfWV * 32 + 4;
+static_assert(kMfWU % 2 == 0 && kMfT % 2 == 0, "pairs along u need even window and tile edges");
 typedef double lt_double4 __attribute__((ext_vector_type(4)));
 
 __global__ void k_mf_blocks(long nrow, const int32_t *__restrict__ row_cell, int nv, int ntv_b,
@@ -1033,8 +1038,8 @@ __global__ void k_mf_blocks(long nrow, const int32_t *__restrict__ row_cell, int
     bmark[(long)(ru / kMfBU) * ntv_b + rv / kMfBV] = 1;
 }
 
-// A[((block * kMfNT + tile) * 32 + step / 2) * 64 + lane][step % 2] and the
-// tiles' rows
+// A[((block * kMfNT + tile) * 32 + pair) * 64 + lane][half] with step
+// 4 w + q = pair (w / 2) * 4 + q, half w % 2, and the tiles' rows
 // (-1 where a cell is no stencil row)
 __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks, int ntv_b,
                           const int32_t *__restrict__ rowmap, long nrow, const double *__restrict__ coef,
@@ -1057,7 +1062,8 @@ __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks
     constexpr int NR = 2 * kMfR + 1;
     double a = 0.0;
     if (row >= 0 && ou >= 0 && ou < NR && ov >= 0 && ov < NR) a = coef[(long)(ou * NR + ov) * nrow + row];
-    A[((bt * 32 + (c >> 1)) * 64 + l) * 2 + (c & 1)] = a;  // steps in pairs: one 16-byte load per two
+    // steps in pairs (4 w + q, 4 (w + 1) + q), w even: one 16-byte load per two
+    A[((bt * 32 + (c >> 3) * 4 + (c & 3)) * 64 + l) * 2 + ((c >> 2) & 1)] = a;
     if (c == 0 && l < 16) rows[bt * 16 + i] = row;
 }
 
@@ -1081,10 +1087,10 @@ __device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ s
     for (int ci = t; ci < kMfWU * kMfWV; ci += nt) {
         const int wv = ci / kMfWU, wu = ci - wv * kMfWU;
         const int a = u0 + wu - G.cu, bb = v0 + wv - G.cv;
-        double *cell = win + (wu * kMfWV + wv) * 16;
+        double *cell = win + (wu >> 1) * kMfR2 + wv * 32 + (wu & 1);
         if (abs(a) > G.bu || abs(bb) > G.bv || abs(bb) >= nbv) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) cell[s] = 0.0;
+            for (int s = 0; s < 16; ++s) cell[2 * s] = 0.0;
             continue;
         }
         const double sg = ((a + bb) & 1) ? -1.0 : 1.0;
@@ -1102,8 +1108,8 @@ __device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ s
         for (int s = 0; s < 8; ++s) z[s] = soff[s] >= 0 ? spec[soff[s] + off] : Cplx<double>{0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            cell[2 * s] = sg * z[s].re;
-            cell[2 * s + 1] = sim * z[s].im;
+            cell[4 * s] = sg * z[s].re;
+            cell[4 * s + 2] = sim * z[s].im;
         }
     }
 }
@@ -1115,7 +1121,7 @@ __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma(Geo G, const int32_
                                                       Cplx<double> *__restrict__ rbuf, long nrow,
                                                       const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
-    double *win = reinterpret_cast<double *>(smem);  // [kMfWU * kMfWV cells][16]
+    double *win = reinterpret_cast<double *>(smem);  // [kMfWU / 2][kMfR2]: cell pairs along u
     const long lb = xcd_block(blockIdx.x, gridDim.x);
     const long b = lb / ngc;
     const int gc = (int)(lb - b * ngc);
@@ -1129,7 +1135,8 @@ __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma(Geo G, const int32_
     for (int r = 0; r < 4; ++r) orow[r] = rows[(b * kMfNT + w) * 16 + (l >> 4) + 4 * r];
     const int j = l & 15;  // column: sample j / 2 of the group, real (even) or imaginary (odd) part
     const int u0 = bu * kMfBU - kMfR, v0 = bv * kMfBV - kMfR;  // raw cell of window (0, 0)
-    const double *wl = win + ((tu * kMfT) * kMfWV + tv * kMfT + (l >> 4)) * 16 + j;
+    const double2 *wl =
+        reinterpret_cast<const double2 *>(win + (tu * kMfT / 2) * kMfR2 + (tv * kMfT + (l >> 4)) * 32 + 2 * j);
     const int ngroups = (nactive + 7) / 8;
     const int g0 = gc * GPW, g1 = min(ngroups, g0 + GPW);
     for (int g = g0; g < g1; ++g) {
@@ -1137,14 +1144,20 @@ __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma(Geo G, const int32_
         mf_stage(G, spec, sstride, active, nactive, g, u0, v0, nbv, threadIdx.x, blockDim.x, win);
         __syncthreads();
         lt_double4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-        for (int c2 = 0; c2 < 32; ++c2) {
-            const double2 av = Aw[c2 * 64];
-            const int c = 2 * c2;
-            const double b0 = wl[((c >> 2) * kMfWV + 4 * (c & 3)) * 16];
-            const double b1 = wl[(((c + 1) >> 2) * kMfWV + 4 * ((c + 1) & 3)) * 16];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, b0, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, b1, acc, 0, 0, 0);
+        // pairs of window rows (w, w + 1): steps 4 w + q, then 4 (w + 1) + q,
+        // in the order of the vector form
+#pragma unroll 2
+        for (int pr = 0; pr < 8; ++pr) {
+            double2 av[4], bv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = Aw[(pr * 4 + q) * 64];
+                bv[q] = wl[(pr * kMfR2 + q * 4 * 32) / 2];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].x, bv[q].x, acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].y, bv[q].y, acc, 0, 0, 0);
         }
         const int q = g * 8 + (j >> 1);
         if (q < nactive) {
@@ -2905,7 +2918,7 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
         const int gpw = -nb_sel;
         const int ngroups = (nactive + 7) / 8;
         const int ngc = (ngroups + gpw - 1) / gpw;
-        const size_t sh = (size_t)kMfWU * kMfWV * 16 * sizeof(double);
+        const size_t sh = (size_t)(kMfWU / 2) * kMfR2 * sizeof(double);
         const dim3 grid((unsigned)(St.mf_nblock * ngc));
 #define LT_STENCIL_MF(GG)                                                                                          \
     hipLaunchKernelGGL((k_stencil_mfma<GG>), grid, dim3(64 * kMfNT), sh, stream(), G, St.mf_blocks.p, St.mf_ntv,   \
