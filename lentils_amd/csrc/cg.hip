@@ -1358,6 +1358,7 @@ lt_status Solver::prepare32() {
     LT_TRY(i1_32.zero());  // rows of L without entries are never written
     LT_TRY(i2_32.alloc(npix * S));
     LT_TRY(Ax.alloc(ntot));
+    if (stp->nb_sel32 == 0) LT_TRY(stencil_autotune32(*nf, *stp, i1_32.p, i2_32.p, all_active.p, S));
     return LT_OK;
 }
 
@@ -1369,13 +1370,22 @@ lt_status Solver::apply32(const double *xin, double *yout, const int *alist, int
     LT_TRY(csr_spmv32_nz(lb->Lp, x32.p, i1_32.p, af, npix, alist, nact));
     LT_TRY(nf->normal_batched32(i1_32.p, i2_32.p, alist, nact, 1.f, false, *stp, rowmask.p));
     LT_TRY(csr_spmv32(lb->LT, i2_32.p, y32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));
-    if (lam_c != 0.0) {
+    // the curvature prior: the fused FP64 tile kernel on x itself where the
+    // source grids allow it, else two FP32 products with the CSR matrix
+    const bool curv32 = lam_c != 0.0 && cv_tiles == 0;
+    if (curv32) {
         LT_TRY(csr_spmv32(C, x32.p, t32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));
         LT_TRY(csr_spmv32(C, t32.p, c32.p, 1.f, 0.f, af, 0, seg.p, alist, nact));
     }
     hipLaunchKernelGGL(k_from_f32, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, soff.p, xx.p, y32.p,
-                       stp->coef32_scale, lam_c != 0.0 ? c32.p : nullptr, lam_c, lam_t, xin, yout);
+                       stp->coef32_scale, curv32 ? c32.p : nullptr, lam_c, lam_t, xin, yout);
     LT_CHECK_LAUNCH();
+    if (lam_c != 0.0 && !curv32) {
+        const bool listed = alist && nact > 0;
+        hipLaunchKernelGGL(k_curv2, dim3(cv_tiles, listed ? nact : S), dim3(kCvT * kCvT), 0, stream(),
+                           listed ? alist : nullptr, af, soff.p, cnx.p, cny.p, ch.p, lam_c, xin, yout);
+        LT_CHECK_LAUNCH();
+    }
     return LT_OK;
 }
 

@@ -1041,9 +1041,10 @@ __global__ void k_mf_blocks(long nrow, const int32_t *__restrict__ row_cell, int
 // A[((block * kMfNT + tile) * 32 + pair) * 64 + lane][half] with step
 // 4 w + q = pair (w / 2) * 4 + q, half w % 2, and the tiles' rows
 // (-1 where a cell is no stencil row)
+template <class TC>
 __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks, int ntv_b,
-                          const int32_t *__restrict__ rowmap, long nrow, const double *__restrict__ coef,
-                          double *__restrict__ A, int32_t *__restrict__ rows) {
+                          const int32_t *__restrict__ rowmap, long nrow, const TC *__restrict__ coef,
+                          TC *__restrict__ A, int32_t *__restrict__ rows) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nblock * kMfNT * 64 * 64) return;
     const int l = (int)(g & 63), c = (int)((g >> 6) & 63);
@@ -1060,7 +1061,7 @@ __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks
     if (ru < G.nu && rv < G.nv) row = rowmap[(long)ru * G.nv + rv];
     const int ou = wu - iu, ov = wv - iv;  // offset index in [0, 2R]
     constexpr int NR = 2 * kMfR + 1;
-    double a = 0.0;
+    TC a = (TC)0;
     if (row >= 0 && ou >= 0 && ou < NR && ov >= 0 && ov < NR) a = coef[(long)(ou * NR + ov) * nrow + row];
     // steps in pairs (4 w + q, 4 (w + 1) + q), w even: one 16-byte load per two
     A[((bt * 32 + (c >> 3) * 4 + (c & 3)) * 64 + l) * 2 + ((c >> 2) & 1)] = a;
@@ -1075,9 +1076,10 @@ __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks
 // on one XCD) and the window's values from LDS.
 // one group's window: per cell the band position and the fold's signs once,
 // then the 8 samples' values (consecutive threads take consecutive u)
-__device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ spec, long sstride,
+template <class T>
+__device__ inline void mf_stage(const Geo &G, const Cplx<T> *__restrict__ spec, long sstride,
                                 const int *__restrict__ active, int nactive, int g, int u0, int v0, int nbv,
-                                int t, int nt, double *__restrict__ win) {
+                                int t, int nt, T *__restrict__ win) {
     long soff[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -1087,15 +1089,15 @@ __device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ s
     for (int ci = t; ci < kMfWU * kMfWV; ci += nt) {
         const int wv = ci / kMfWU, wu = ci - wv * kMfWU;
         const int a = u0 + wu - G.cu, bb = v0 + wv - G.cv;
-        double *cell = win + (wu >> 1) * kMfR2 + wv * 32 + (wu & 1);
+        T *cell = win + (wu >> 1) * kMfR2 + wv * 32 + (wu & 1);
         if (abs(a) > G.bu || abs(bb) > G.bv || abs(bb) >= nbv) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) cell[2 * s] = 0.0;
+            for (int s = 0; s < 16; ++s) cell[2 * s] = (T)0;
             continue;
         }
-        const double sg = ((a + bb) & 1) ? -1.0 : 1.0;
+        const T sg = ((a + bb) & 1) ? (T)-1 : (T)1;
         long off;
-        double sim;
+        T sim;
         if (bb >= 0) {
             off = (long)bb * G.nu + (a < 0 ? a + G.nu : a);
             sim = sg;
@@ -1103,9 +1105,9 @@ __device__ inline void mf_stage(const Geo &G, const Cplx<double> *__restrict__ s
             off = (long)(-bb) * G.nu + (-a < 0 ? -a + G.nu : -a);
             sim = -sg;
         }
-        Cplx<double> z[8];
+        Cplx<T> z[8];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) z[s] = soff[s] >= 0 ? spec[soff[s] + off] : Cplx<double>{0.0, 0.0};
+        for (int s = 0; s < 8; ++s) z[s] = soff[s] >= 0 ? spec[soff[s] + off] : Cplx<T>{(T)0, (T)0};
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             cell[4 * s] = sg * z[s].re;
@@ -1170,6 +1172,76 @@ __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma(Geo G, const int32_
                 const int i = (l >> 4) + 4 * r;
                 const int ru = bu * kMfBU + tu * kMfT + (i >> 2), rv = bv * kMfBV + tv * kMfT + (i & 3);
                 const double sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? -1.0 : 1.0;
+                out[(long)row * 2 + (j & 1)] = sg * acc[r];
+            }
+        }
+    }
+}
+
+// The FP32 stencil of the precision form on the matrix cores: the same
+// tiles, packed matrices (of the FP32 coefficients) and staged windows as
+// k_stencil_mfma, with v_mfma_f32_16x16x4f32 (twice the FP64 matrix rate on
+// this chip, half the operand bytes).  Its result layout differs from the
+// FP64 instruction's: element r of D sits at row 4 (l / 16) + r, column
+// l % 16 (checked on the GPU with a probe of known operands).  FP32
+// accumulation: the results agree with the vector form to FP32 rounding.
+typedef float lt_float4 __attribute__((ext_vector_type(4)));
+
+template <int GPW>
+__global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma32(Geo G, const int32_t *__restrict__ blocks, int ntv_b,
+                                                        int ngc, const float *__restrict__ A,
+                                                        const int32_t *__restrict__ rows,
+                                                        const Cplx<float> *__restrict__ spec, long sstride,
+                                                        Cplx<float> *__restrict__ rbuf, long nrow,
+                                                        const int *__restrict__ active, int nactive, int nbv) {
+    extern __shared__ unsigned char smem[];
+    float *win = reinterpret_cast<float *>(smem);  // [kMfWU / 2][kMfR2]: cell pairs along u
+    const long lb = xcd_block(blockIdx.x, gridDim.x);
+    const long b = lb / ngc;
+    const int gc = (int)(lb - b * ngc);
+    const int blk = blocks[b];
+    const int bu = blk / ntv_b, bv = blk - bu * ntv_b;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int tu = w / kMfBTV, tv = w - tu * kMfBTV;
+    const float2 *Aw = reinterpret_cast<const float2 *>(A) + (b * kMfNT + w) * 32 * 64 + l;
+    int orow[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) orow[r] = rows[(b * kMfNT + w) * 16 + 4 * (l >> 4) + r];
+    const int j = l & 15;  // column: sample j / 2 of the group, real (even) or imaginary (odd) part
+    const int u0 = bu * kMfBU - kMfR, v0 = bv * kMfBV - kMfR;
+    const float2 *wl =
+        reinterpret_cast<const float2 *>(win + (tu * kMfT / 2) * kMfR2 + (tv * kMfT + (l >> 4)) * 32 + 2 * j);
+    const int ngroups = (nactive + 7) / 8;
+    const int g0 = gc * GPW, g1 = min(ngroups, g0 + GPW);
+    for (int g = g0; g < g1; ++g) {
+        __syncthreads();  // the previous group's reads are done
+        mf_stage(G, spec, sstride, active, nactive, g, u0, v0, nbv, threadIdx.x, blockDim.x, win);
+        __syncthreads();
+        lt_float4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+        for (int pr = 0; pr < 8; ++pr) {
+            float2 av[4], bq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = Aw[(pr * 4 + q) * 64];
+                bq[q] = wl[(pr * kMfR2 + q * 4 * 32) / 2];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q].x, bq[q].x, acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q].y, bq[q].y, acc, 0, 0, 0);
+        }
+        const int q = g * 8 + (j >> 1);
+        if (q < nactive) {
+            const int smp = active ? active[q] : q;
+            float *out = reinterpret_cast<float *>(rbuf) + (long)smp * nrow * 2;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = orow[r];
+                if (row < 0) continue;
+                const int i = 4 * (l >> 4) + r;
+                const int ru = bu * kMfBU + tu * kMfT + (i >> 2), rv = bv * kMfBV + tv * kMfT + (i & 3);
+                const float sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? -1.f : 1.f;
                 out[(long)row * 2 + (j & 1)] = sg * acc[r];
             }
         }
@@ -2696,6 +2768,55 @@ lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, 
     return LT_OK;
 }
 
+// The FP32 stencil's form (precision form): the vector form with 4 samples
+// per workgroup against the matrix-core form with 1, 2 or 4 groups of 8
+// samples per workgroup (radius 6 only), timed like stencil_autotune.
+lt_status stencil_autotune32(NufftPlan &P, NormalStencil &St, const float *imgs, float *out, const int *all, int nb) {
+    if (!St.use_band || nb <= 0) return LT_OK;
+    if (St.R != kMfR) {
+        St.nb_sel32 = 4;
+        return LT_OK;
+    }
+    if (!St.coef32.p) LT_TRY(St.apply32(P, nullptr, 0, nullptr, nullptr, 0));  // makes coef32
+    const int cands[4] = {4, -1, -2, -4};
+    float tmin[4];
+    for (int c = 0; c < 4; ++c) tmin[c] = 1e30f;
+    hipEvent_t e0, e1;
+    LT_HIP(hipEventCreate(&e0));
+    LT_HIP(hipEventCreate(&e1));
+    auto rounds = [&]() -> lt_status {
+        for (int round = 0; round < 3; ++round)
+            for (int c = 0; c < 4; ++c) {
+                St.nb_sel32 = cands[c];
+                LT_TRY((normal_band_impl<float, float>(P, St, imgs, out, all, nb, 1.f, false)));  // warm-up
+                LT_HIP(hipEventRecord(e0, stream()));
+                for (int r = 0; r < 3; ++r)
+                    LT_TRY((normal_band_impl<float, float>(P, St, imgs, out, all, nb, 1.f, false)));
+                LT_HIP(hipEventRecord(e1, stream()));
+                LT_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                LT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                tmin[c] = std::min(tmin[c], ms);
+            }
+        return LT_OK;
+    };
+    const int forced = P.force_nb;  // a forced shape would stand in for every candidate
+    P.force_nb = 0;
+    const lt_status st = rounds();
+    P.force_nb = forced;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (st != LT_OK) {
+        St.nb_sel32 = 0;
+        return st;
+    }
+    int best = 0;
+    for (int c = 1; c < 4; ++c)
+        if (tmin[c] < tmin[best]) best = c;
+    St.nb_sel32 = cands[best];
+    return LT_OK;
+}
+
 lt_status NufftPlan::normal_batched32(const float *imgs, float *out, const int *active, int nactive, float alpha,
                                       bool acc, NormalStencil &st, const uint8_t *rowmask) {
     if (st.use_band) {
@@ -2892,11 +3013,30 @@ lt_status NormalStencil::mfma_build(NufftPlan &P) {
     LT_TRY(mf_rows.alloc(std::max<long>(nb, 1) * kMfNT * 16));
     const long npk = nb * kMfNT * 64 * 64;
     if (npk > 0) {
-        hipLaunchKernelGGL(k_mf_pack, dim3(blocks_for(npk, 256)), dim3(256), 0, stream(), G, (long)nb, mf_blocks.p,
+        hipLaunchKernelGGL(k_mf_pack<double>, dim3(blocks_for(npk, 256)), dim3(256), 0, stream(), G, (long)nb, mf_blocks.p,
                            mf_ntv, rowmap.p, nrow, coef.p, mf_A.p, mf_rows.p);
         LT_CHECK_LAUNCH();
     }
     mf_nblock = nb;
+    return sync();
+}
+
+lt_status NormalStencil::mfma_build32(NufftPlan &P) {
+    if (mf_A32.p) return LT_OK;
+    if (!coef32.p) {
+        set_error("normal stencil: FP32 coefficients missing");
+        return LT_ERR_ARG;
+    }
+    LT_TRY(mfma_build(P));
+    Geo G = make_geo(P);
+    const long nb = mf_nblock;
+    LT_TRY(mf_A32.alloc(std::max<long>(nb, 1) * kMfNT * 64 * 64));
+    const long npk = nb * kMfNT * 64 * 64;
+    if (npk > 0) {
+        hipLaunchKernelGGL(k_mf_pack<float>, dim3(blocks_for(npk, 256)), dim3(256), 0, stream(), G, nb, mf_blocks.p,
+                           mf_ntv, rowmap.p, nrow, coef32.p, mf_A32.p, mf_rows.p);
+        LT_CHECK_LAUNCH();
+    }
     return sync();
 }
 
@@ -2910,7 +3050,34 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
     if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
     const int WS = St.ST + 2 * St.R;
     const int ntv_st = (P.nv + St.ST - 1) / St.ST;
-    const int nb_sel = P.force_nb != 0 ? P.force_nb : St.nb_sel;
+    const bool f32 = std::is_same<T, float>::value && std::is_same<TC, float>::value;
+    const int nb_sel = P.force_nb != 0 ? P.force_nb : (f32 ? (St.nb_sel32 != 0 ? St.nb_sel32 : 4) : St.nb_sel);
+    if (nb_sel < 0 && f32 && band && St.R == kMfR && nactive > 0) {
+        // FP32 matrix-core form: -nb_sel groups of 8 samples per workgroup
+        LT_TRY(St.mfma_build32(P));
+        const int gpw = -nb_sel;
+        const int ngroups = (nactive + 7) / 8;
+        const int ngc = (ngroups + gpw - 1) / gpw;
+        const size_t sh = (size_t)(kMfWU / 2) * kMfR2 * sizeof(float);
+        const dim3 grid((unsigned)(St.mf_nblock * ngc));
+#define LT_STENCIL_MF32(GG)                                                                                       \
+    hipLaunchKernelGGL((k_stencil_mfma32<GG>), grid, dim3(64 * kMfNT), sh, stream(), G, St.mf_blocks.p,          \
+                       St.mf_ntv, ngc, St.mf_A32.p, St.mf_rows.p, (const Cplx<float> *)spec_in, sstride,         \
+                       (Cplx<float> *)St.rbuf.p, St.nrow, active, nactive, St.nbv);
+        if (St.mf_nblock > 0) {
+            if (gpw == 1)
+                LT_STENCIL_MF32(1)
+            else if (gpw == 2)
+                LT_STENCIL_MF32(2)
+            else if (gpw == 4)
+                LT_STENCIL_MF32(4)
+            else
+                LT_STENCIL_MF32(8)
+            LT_CHECK_LAUNCH();
+        }
+#undef LT_STENCIL_MF32
+        return LT_OK;
+    }
     if (nb_sel < 0 && std::is_same<T, double>::value && std::is_same<TC, double>::value && band &&
         St.R == kMfR && nactive > 0) {
         // matrix-core form: -nb_sel groups of 8 samples per workgroup

@@ -87,9 +87,12 @@ struct NormalStencil {
     int mf_ntv = 0;            // blocks along v
     DevBuf<int32_t> mf_blocks, mf_rows;
     DevBuf<double> mf_A;
+    DevBuf<float> mf_A32;      // the same matrices of the FP32 coefficients (mfma_build32)
+    int nb_sel32 = 0;          // the FP32 stencil's form (stencil_autotune32); 0: not tuned (vector form, 4)
 
     lt_status build(NufftPlan &P, const double *w_sorted);
     lt_status mfma_build(NufftPlan &P);
+    lt_status mfma_build32(NufftPlan &P);
     lt_status ensure_fold(NufftPlan &P);  // fold_* (band form)
     lt_status apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active, int nactive);
     lt_status apply32(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
@@ -215,5 +218,6 @@ struct NufftPlan {
 
 // pick the stencil's samples per workgroup by timing the normal operator
 lt_status stencil_autotune(NufftPlan &P, NormalStencil &St, const double *imgs, double *out, const int *all, int nb);
+lt_status stencil_autotune32(NufftPlan &P, NormalStencil &St, const float *imgs, float *out, const int *all, int nb);
 
 }  // namespace lt

@@ -156,6 +156,45 @@ def test_stencil_matrix_core_form(hip, oracle):
     plan.set_stencil_shape(0)
 
 
+def test_stencil_matrix_core_form_f32(hip, oracle):
+    """The precision form's FP32 stencil on the matrix cores
+    (v_mfma_f32_16x16x4f32, its own result layout): the FP32 operator with it
+    agrees with the FP32 vector form and with the FP64 operator to FP32
+    rounding for every number of sample groups per workgroup (19 samples:
+    groups of 8, 8 and 3), and an FP32-operator CG solve with it reaches the
+    FP64 true residual that the oracle's operator confirms."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    n = 19
+    im, src, uv, vs, models, data = _problem(nlens=n, seed=79)
+    lb = LensBatch(im, [src] * n, models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    xs = [rng(110 + k).normal(size=src.size) for k in range(n)]
+    plan.set_stencil_shape(4)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    y64 = np.stack([y.reshape(-1) for y in inv.apply(xs)])
+    yv32 = np.stack([y.reshape(-1) for y in inv.apply(xs, precision='f32')])
+    assert O.max_relative_error(yv32, y64) < 1e-5
+    for g in (-1, -2, -4):
+        plan.set_stencil_shape(g)
+        inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+        ym = np.stack([y.reshape(-1) for y in inv.apply(xs, precision='f32')])
+        e64, ev = O.max_relative_error(ym, y64), O.max_relative_error(ym, yv32)
+        print(f'FP32 matrix-core form, {-g} groups per workgroup: vs FP64 {e64:.2e}, vs FP32 vector form {ev:.2e}')
+        assert e64 < 1e-5 and ev < 1e-5
+    plan.set_stencil_shape(-2)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    x, its, res = inv.solve(rtol=1e-6, maxiter=20000, precision='f32', refine_every=20)
+    print(f'FP32-operator CG with the matrix-core stencil: iterations {its}')
+    assert np.all(res <= 1e-6)
+    k = int(np.argmax(its))
+    Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x[k].reshape(-1), w, 1e2)
+    b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
+    assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= RESID_BAR
+    plan.set_stencil_shape(0)
+
+
 def test_cg_converges_and_oracle_confirms(hip, oracle):
     from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
     O = oracle
