@@ -734,6 +734,17 @@ def main():
               'true_rel_resid_max': float(resl.max()), 'all_converged': bool(np.all(resl <= 1e-6)),
               'method': 'flexible CG, preconditioner: 3 Jacobi-PCG steps on the explicit lambda Cc^T Cc + c L_s^T L_s (L_s: rows of L with <= 64 nonzero entries)'}
         log(f'lens preconditioner: {tl:.2f}s iters median {np.median(itsl):.0f} max res {resl.max():.2e}')
+        # both: the lens-structured preconditioner with the FP32 operator
+        for k in range(args.warmup):
+            wl.step(operator=args.operator, maxiter=args.maxiter, precond='lens', precision='f32')
+        t = time.perf_counter()
+        itsp, resp = wl.step(operator=args.operator, maxiter=args.maxiter, precond='lens', precision='f32')
+        tp = time.perf_counter() - t
+        lp['precision_form'] = {'value': NLENS / tp, 'unit': 'source solves/s', 'ms_per_step': 1e3 * tp,
+                                'iters_median': float(np.median(itsp)), 'iters_max': int(itsp.max()),
+                                'true_rel_resid_max': float(resp.max()), 'all_converged': bool(np.all(resp <= 1e-6)),
+                                'dtype': 'f32 operator inside the flexible FP64 CG'}
+        log(f'lens preconditioner, FP32 operator: {tp:.2f}s iters median {np.median(itsp):.0f} max res {resp.max():.2e}')
     st = stencil_roofline(wl)
     log(f'stencil (form {st["form"]}) {st["ms"] * 1e3:.1f} us/launch at {NLENS} samples: {st["tflops"]:.2f} TFLOP/s, {st["gbs"]:.0f} GB/s '
         f'({st["rows"]} rows, {st["win_cells"]} window cells)')

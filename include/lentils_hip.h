@@ -292,10 +292,18 @@ lt_status lt_nufft_set_weights(lt_nufft h, const double *weights);
 /* No reference counterpart: forces the normal stencil's form for this plan
  * instead of the autotuned choice (0): 1, 2, 4, 8 = the vector form with that
  * many samples per workgroup (all four give the same bits); -1, -2, -4, -8 =
- * the FP64 matrix-core form (v_mfma_f64_16x16x4f64, radius-6 stencils) with
- * that many groups of 8 samples per workgroup (the same bits among
+ * the matrix-core form (radius-6 stencils; v_mfma_f64_16x16x4f64 for the FP64
+ * operator, v_mfma_f32_16x16x4f32 for the precision form's FP32 operator)
+ * with that many groups of 8 samples per workgroup (the same bits among
  * themselves, the vector form's to rounding). */
 lt_status lt_nufft_set_stencil_shape(lt_nufft h, int samples_per_group);
+/* No reference counterpart: the degridder's tap evaluation for this plan:
+ * 1 (default) = the 2W taps of the window below and above the visibility as
+ * polynomials of its offset from the cell centre (nothing to select);
+ * 0 = the 2W+1 per-offset taps around the nearest grid point, the window's
+ * 2W picked by the offset's sign.  Both agree to the precision's rounding;
+ * visibilities exactly on a grid line take the 2W+1 taps either way. */
+lt_status lt_nufft_set_degrid_form(lt_nufft h, int form);
 /* No reference counterpart: times the resident normal stencil's kernel as
  * the solver launches it for nb samples (band form, tuned shape) with HIP
  * events on the library stream; mean per-launch time, algorithmic bytes
@@ -338,6 +346,13 @@ lt_status lt_bcb_apply_host(image_space imspace, const double *beam, int nimg, c
  * 1/(4W^2) per axis pair; edge taps are zero where |f + W - k| > W.
  * max_err: fit error relative to I0(kb_beta).  Host only. */
 lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, double *max_err);
+/* No reference counterpart: the degridder's window taps (the default form of
+ * lt_nufft_set_degrid_form).  Tap j of the 2W grid cells floor(x) - W + 1 ..
+ * floor(x) + W, x = u/du, at s = x - floor(x) - 1/2 in [-1/2, 1/2] (distance
+ * s + W - 1/2 - j, always inside the support) has value
+ * sum_m coeffs[j*(D+1) + m] s^m, D as for lt_kb_taps.  The kernels evaluate
+ * taps 0..W-1 and take tap 2W-1-j at s as tap j at -s.  Host only. */
+lt_status lt_kb_taps_window(double kb_beta, int wsup, int precision, double *coeffs, double *max_err);
 
 /* Source-inversion solver for every sample of a lens batch:
  *   (L^T D^T C D L + lambda_curv Cc^T Cc + lambda_tikh I) x = L^T D^T C d

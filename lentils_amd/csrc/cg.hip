@@ -1859,6 +1859,16 @@ lt_status lt_nufft_set_stencil_shape(lt_nufft nfh, int samples_per_group) {
     return LT_OK;
 }
 
+lt_status lt_nufft_set_degrid_form(lt_nufft nfh, int form) {
+    NufftPlan *nf = nufft_of(nfh);
+    if (!nf || (form != 0 && form != 1)) {
+        set_error("lt_nufft_set_degrid_form: 0 (per-offset taps) or 1 (window taps)");
+        return LT_ERR_ARG;
+    }
+    nf->degrid_half = form == 1;
+    return LT_OK;
+}
+
 lt_status lt_device_alloc(long nbytes, void **dptr) {
     if (!dptr || nbytes < 0) return LT_ERR_ARG;
     *dptr = nullptr;

@@ -266,6 +266,31 @@ __device__ inline void kb_taps_sym(const T *__restrict__ kc, T f, T (&out)[2 * W
     }
 }
 
+// The 2W taps of the window that starts W - 1 cells below floor(x), from
+// the W polynomials of taps 0..W-1 in s = x - floor(x) - 1/2, split into
+// even and odd parts: tap 2W-1-j at s is tap j at -s (the kernel is even),
+// so P_{2W-1-j}(s) = E_j(s^2) - s O_j(s^2).  Every tap of this window is
+// inside the support for any s, so there is nothing to select or zero.
+template <class T, int W>
+__device__ inline void kb_taps_half(const T *__restrict__ kc, T s, T (&out)[2 * W]) {
+    constexpr int D = TapDeg<T, W>::D;
+    constexpr int DE = D / 2 * 2;
+    constexpr int DO = (D % 2) ? D : D - 1;
+    const T q = s * s;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const T *c = kc + j * (D + 1);
+        T e = c[DE];
+#pragma unroll
+        for (int m = DE - 2; m >= 0; m -= 2) e = e * q + c[m];
+        T o = c[DO];
+#pragma unroll
+        for (int m = DO - 2; m >= 1; m -= 2) o = o * q + c[m];
+        out[j] = e + s * o;
+        out[2 * W - 1 - j] = e - s * o;
+    }
+}
+
 template <class T>
 struct Vec2;
 template <>
@@ -287,6 +312,7 @@ struct Vec2<double> {
 // the full (2W+1)^2 product.  Same results as k_degrid up to rounding.
 template <class T, int W>
 __global__ void __launch_bounds__(kNThreads) k_degrid32(Geo G, const T *__restrict__ kc,
+                                                        const T *__restrict__ kch,
                                                         const Cplx<T> *__restrict__ grid,
                                                         const int32_t *__restrict__ ck_tile,
                                                         const int32_t *__restrict__ ck_beg,
@@ -322,18 +348,24 @@ __global__ void __launch_bounds__(kNThreads) k_degrid32(Geo G, const T *__restri
         int na, nb;
         unpack_ngp((uint32_t)ngp[q], na, nb);
         const T fu = frac[2 * (long)q], fv = frac[2 * (long)q + 1];
-        T ku[NT], kv[NT];
-        kb_taps_sym<T, W>(kc, fu, ku);
-        kb_taps_sym<T, W>(kc, fv, kv);
         const int r0 = na - a0 - W, s0 = nb - b0 - W;
         T ar = 0, ai = 0;
         if (__all(fu != (T)0 && fv != (T)0)) {
             const int du = fu > (T)0 ? 1 : 0, dv = fv > (T)0 ? 1 : 0;
             T ku6[NF], kv6[NF];
+            if (kch) {
+                // the window starts W - 1 cells below floor(x): r0 + du
+                kb_taps_half<T, W>(kch, du ? fu - (T)0.5 : fu + (T)0.5, ku6);
+                kb_taps_half<T, W>(kch, dv ? fv - (T)0.5 : fv + (T)0.5, kv6);
+            } else {
+                T ku[NT], kv[NT];
+                kb_taps_sym<T, W>(kc, fu, ku);
+                kb_taps_sym<T, W>(kc, fv, kv);
 #pragma unroll
-            for (int j = 0; j < NF; ++j) {
-                ku6[j] = du ? ku[j + 1] : ku[j];
-                kv6[j] = dv ? kv[j + 1] : kv[j];
+                for (int j = 0; j < NF; ++j) {
+                    ku6[j] = du ? ku[j + 1] : ku[j];
+                    kv6[j] = dv ? kv[j + 1] : kv[j];
+                }
             }
             const V *base = wv + (r0 + du) * WV + s0 + dv;
 #pragma unroll
@@ -349,6 +381,9 @@ __global__ void __launch_bounds__(kNThreads) k_degrid32(Geo G, const T *__restri
                 ai += ku6[k] * ri;
             }
         } else {
+            T ku[NT], kv[NT];
+            kb_taps_sym<T, W>(kc, fu, ku);
+            kb_taps_sym<T, W>(kc, fv, kv);
             if (fu > (T)0) ku[0] = (T)0;
             if (fu < (T)0) ku[NT - 1] = (T)0;
             if (fv > (T)0) kv[0] = (T)0;
@@ -1879,55 +1914,77 @@ double kb_entire(double beta, int W, double x) {
     return acc;
 }
 
-// Chebyshev interpolation of every tap on f in [-1/2, 1/2] at D+1 nodes,
-// converted to monomials in f; returns the max error over a dense sample
-// (edge taps only where they are nonzero) relative to the peak I0(beta)
-double kb_taps_fit(double beta, int W, int D, std::vector<double> &mono) {
-    const int NT = 2 * W + 1, n = D + 1;
+namespace {
+
+// Chebyshev interpolation at n = D+1 nodes of y(t), t in [-1, 1] (the
+// samples y[j] at t_j = cos(pi (j + 1/2) / n)), converted to monomials in
+// f = t / 2: mono[e], e = 0..D
+void cheb_to_mono(const std::vector<double> &y, int n, double *mono) {
+    std::vector<double> cheb(n), tk(n), tkm1(n), tkp1(n), m(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double a = 0.0;
+        for (int j = 0; j < n; ++j) a += y[j] * std::cos(M_PI * i * (j + 0.5) / n);
+        cheb[i] = (i == 0 ? 1.0 : 2.0) * a / n;
+    }
+    // sum_i cheb_i T_i(t) as monomials in t, with
+    // T_0 = 1, T_1 = t, T_{i+1} = 2 t T_i - T_{i-1}
+    std::fill(tkm1.begin(), tkm1.end(), 0.0);
+    std::fill(tk.begin(), tk.end(), 0.0);
+    tkm1[0] = 1.0;
+    for (int e = 0; e < n; ++e) m[e] += cheb[0] * tkm1[e];
+    if (n > 1) {
+        tk[1] = 1.0;
+        for (int e = 0; e < n; ++e) m[e] += cheb[1] * tk[e];
+    }
+    for (int i = 2; i < n; ++i) {
+        for (int e = 0; e < n; ++e) tkp1[e] = (e > 0 ? 2.0 * tk[e - 1] : 0.0) - tkm1[e];
+        for (int e = 0; e < n; ++e) m[e] += cheb[i] * tkp1[e];
+        tkm1.swap(tk);
+        tk.swap(tkp1);
+    }
+    double sc = 1.0;
+    for (int e = 0; e < n; ++e, sc *= 2.0) mono[e] = m[e] * sc;
+}
+
+// fits of the taps at distances f + c_k, f in [-1/2, 1/2] (tap k), and the
+// max error relative to the peak over a dense sample; edge_zero: tap 0 is
+// skipped for f > 0 and the last tap for f < 0 (outside the support there)
+double fit_taps(double beta, int W, int D, int NT, double c0, bool edge_zero, std::vector<double> &mono) {
+    const int n = D + 1;
     mono.assign((size_t)NT * n, 0.0);
-    std::vector<double> y(n), cheb(n), tk(n), tkm1(n), tkp1(n);
+    std::vector<double> y(n);
     for (int k = 0; k < NT; ++k) {
-        for (int j = 0; j < n; ++j) {
-            const double t = std::cos(M_PI * (j + 0.5) / n);
-            y[j] = kb_entire(beta, W, 0.5 * t + W - k);
-        }
-        for (int i = 0; i < n; ++i) {
-            double a = 0.0;
-            for (int j = 0; j < n; ++j) a += y[j] * std::cos(M_PI * i * (j + 0.5) / n);
-            cheb[i] = (i == 0 ? 1.0 : 2.0) * a / n;
-        }
-        // sum_i cheb_i T_i(t) as monomials in t = 2 f, with
-        // T_0 = 1, T_1 = t, T_{i+1} = 2 t T_i - T_{i-1}
-        std::vector<double> m(n, 0.0);
-        std::fill(tkm1.begin(), tkm1.end(), 0.0);
-        std::fill(tk.begin(), tk.end(), 0.0);
-        tkm1[0] = 1.0;
-        for (int e = 0; e < n; ++e) m[e] += cheb[0] * tkm1[e];
-        if (n > 1) {
-            tk[1] = 1.0;
-            for (int e = 0; e < n; ++e) m[e] += cheb[1] * tk[e];
-        }
-        for (int i = 2; i < n; ++i) {
-            for (int e = 0; e < n; ++e) tkp1[e] = (e > 0 ? 2.0 * tk[e - 1] : 0.0) - tkm1[e];
-            for (int e = 0; e < n; ++e) m[e] += cheb[i] * tkp1[e];
-            tkm1.swap(tk);
-            tk.swap(tkp1);
-        }
-        double sc = 1.0;
-        for (int e = 0; e < n; ++e, sc *= 2.0) mono[(size_t)k * n + e] = m[e] * sc;
+        for (int j = 0; j < n; ++j) y[j] = kb_entire(beta, W, 0.5 * std::cos(M_PI * (j + 0.5) / n) + c0 - k);
+        cheb_to_mono(y, n, &mono[(size_t)k * n]);
     }
     const double peak = kb_entire(beta, W, 0.0);
     double err = 0.0;
     for (int k = 0; k < NT; ++k) {
         for (int i = 0; i <= 512; ++i) {
             const double f = -0.5 + i / 512.0;
-            if ((k == 0 && f > 0) || (k == NT - 1 && f < 0)) continue;
+            if (edge_zero && ((k == 0 && f > 0) || (k == NT - 1 && f < 0))) continue;
             double acc = mono[(size_t)k * n + D];
             for (int e = D - 1; e >= 0; --e) acc = acc * f + mono[(size_t)k * n + e];
-            err = std::max(err, std::fabs(acc - kb_entire(beta, W, f + W - k)) / peak);
+            err = std::max(err, std::fabs(acc - kb_entire(beta, W, f + c0 - k)) / peak);
         }
     }
     return err;
+}
+
+}  // namespace
+
+// Chebyshev interpolation of every tap on f in [-1/2, 1/2] at D+1 nodes,
+// converted to monomials in f; returns the max error over a dense sample
+// (edge taps only where they are nonzero) relative to the peak I0(beta)
+double kb_taps_fit(double beta, int W, int D, std::vector<double> &mono) {
+    return fit_taps(beta, W, D, 2 * W + 1, (double)W, true, mono);
+}
+
+// The 2W taps of the window starting W - 1 cells below floor(x), x = u/du:
+// tap j at distance s + W - 1/2 - j with s = x - floor(x) - 1/2 in
+// [-1/2, 1/2], all inside the support (|distance| <= W)
+double kb_taps_fit_half(double beta, int W, int D, std::vector<double> &mono) {
+    return fit_taps(beta, W, D, 2 * W, W - 0.5, false, mono);
 }
 
 namespace {
@@ -1944,10 +2001,22 @@ const double *taps_of<double>(const NufftPlan &P) {
 }
 
 template <class T>
+const T *taps_half_of(const NufftPlan &P);
+template <>
+const float *taps_half_of<float>(const NufftPlan &P) {
+    return P.taps32h.p;
+}
+template <>
+const double *taps_half_of<double>(const NufftPlan &P) {
+    return P.taps64h.p;
+}
+
+template <class T>
 lt_status launch_degrid(NufftPlan &P, const void *grid, void *vis, const double *wts, bool acc) {
     if (P.nchunks == 0) return LT_OK;
     Geo G = make_geo(P);
     const T *kc = taps_of<T>(P);
+    const T *kch = P.degrid_half ? taps_half_of<T>(P) : nullptr;
     const size_t sh = (size_t)P.WU * P.WV * sizeof(Cplx<T>);
     const T *ph = P.phase ? (P.prec == LT_F64 ? (const T *)P.ph64.p : (const T *)P.ph32.p) : nullptr;
     const T *fr = P.prec == LT_F64 ? (const T *)P.frac64.p : (const T *)P.frac32.p;
@@ -1955,7 +2024,7 @@ lt_status launch_degrid(NufftPlan &P, const void *grid, void *vis, const double 
     case WW:                                                                                                      \
         if (P.TV == 32)                                                                                           \
             hipLaunchKernelGGL((k_degrid32<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G, \
-                               kc, (const Cplx<T> *)grid, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph,   \
+                               kc, kch, (const Cplx<T> *)grid, P.ck_tile.p, P.ck_beg.p, P.ck_end.p, P.ngp.p, fr, ph,   \
                                wts, (Cplx<T> *)vis, acc ? 1 : 0);                                                 \
         else                                                                                                      \
             hipLaunchKernelGGL((k_degrid<T, WW>), dim3((unsigned)P.nchunks), dim3(kNThreads), sh, stream(), G,   \
@@ -2078,6 +2147,15 @@ lt_status NufftPlan::create(const visibility_space &vs, const fourier_space &fs,
         } else {
             std::vector<float> m32(mono.begin(), mono.end());
             LT_TRY(taps32.upload(m32.data(), (long)m32.size()));
+        }
+        std::vector<double> half;
+        if (kb_taps_fit_half(beta, W, D, half) <= tol) {
+            if (prec == LT_F64) {
+                LT_TRY(taps64h.upload(half.data(), (long)half.size()));
+            } else {
+                std::vector<float> h32(half.begin(), half.end());
+                LT_TRY(taps32h.upload(h32.data(), (long)h32.size()));
+            }
         }
     }
     cu = nu / 2;

@@ -247,6 +247,18 @@ lt_status lt_kb_taps(double kb_beta, int wsup, int precision, double *coeffs, do
     return LT_OK;
 }
 
+lt_status lt_kb_taps_window(double kb_beta, int wsup, int precision, double *coeffs, double *max_err) {
+    if (wsup < 1 || wsup > 8 || !(kb_beta > 0.0) || (precision != LT_F64 && precision != LT_F32)) {
+        set_error("lt_kb_taps_window: bad arguments");
+        return LT_ERR_ARG;
+    }
+    std::vector<double> mono;
+    const double err = kb_taps_fit_half(kb_beta, wsup, tap_degree(precision == LT_F64, wsup), mono);
+    if (coeffs) std::copy(mono.begin(), mono.end(), coeffs);
+    if (max_err) *max_err = err;
+    return LT_OK;
+}
+
 lt_status lt_nufft_time(lt_nufft h, int direction, int reps, double *ms) {
     if (!h || reps <= 0 || !ms) return LT_ERR_ARG;
     NufftPlan &P = h->plan;

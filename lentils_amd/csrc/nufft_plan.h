@@ -44,6 +44,9 @@ struct NufftPlan;
 // per-tap monomial coefficients [2W+1][D+1] of the kernel (see TapDeg);
 // returns the max fit error relative to the kernel peak
 double kb_taps_fit(double beta, int W, int D, std::vector<double> &mono);
+// the same for the 2W taps at distances s + W - 1/2 - j (j = 0..2W-1) from
+// the visibility, s in [-1/2, 1/2]: [2W][D+1]
+double kb_taps_fit_half(double beta, int W, int D, std::vector<double> &mono);
 double kb_entire(double beta, int W, double x);
 
 // D^T C D restricted to the uv grid, precomputed once per (uv set, weights):
@@ -122,6 +125,11 @@ struct NufftPlan {
     DevBuf<float> taps32;   // (2W+1) x (tap_degree(false, W)+1)
     DevBuf<double> taps64;  // (2W+1) x (kTapDeg64+1)
     double taps_err = 0.0;  // max fit error relative to the kernel peak
+    // the 2W taps of the window that starts W - 1 cells below floor(u/du),
+    // as polynomials of s = u/du - floor(u/du) - 1/2 (kb_taps_fit_half);
+    // empty when their fit misses the precision's tolerance
+    DevBuf<float> taps32h;   // 2W x (tap_degree(false, W)+1)
+    DevBuf<double> taps64h;  // 2W x (kTapDeg64+1)
     DevBuf<int32_t> perm;        // sorted position -> channel-expanded index
     DevBuf<uint32_t> key_sorted;  // tile-major cell key of each sorted visibility
     DevBuf<int32_t> ngp;   // packed (ngp_u + 32768) | (ngp_v + 32768) << 16
@@ -170,6 +178,7 @@ struct NufftPlan {
     bool wave_rows = true;   // FP32 with nyp = 1024: the one-wave transform (k_rows_fwd_w / k_rows_inv_w)
     int force_nb = 0;  // stencil samples per workgroup when > 0, the matrix-core form with -force_nb
                        // sample groups per workgroup when < 0 (else the autotuned choice)
+    bool degrid_half = true;  // degridder fast path with the window taps (taps32h / taps64h)
     lt_status band_fft(int n, int precision, int nbv, BandFFT *&out);
     DevBuf<uint8_t> realB, specB, specB2, partialB;
     lt_status batch_fft(int n, BatchFFT *&out, int precision = -1);

@@ -162,6 +162,12 @@ class NUFFTPlan:
         same bits among themselves, the vector form's to rounding)."""
         B.check(B.lib().lt_nufft_set_stencil_shape(self._handle, int(samples_per_group)))
 
+    def set_degrid_form(self, form=1):
+        """The degridder's tap evaluation: 1 (default) the window's 2W taps as
+        polynomials of the offset from the cell centre, 0 the 2W+1 taps around
+        the nearest grid point with the window picked by the offset's sign."""
+        B.check(B.lib().lt_nufft_set_degrid_form(self._handle, int(form)))
+
     def time(self, direction, reps=20):
         """Mean device time (ms) of one degrid (0) or grid (1) launch."""
         ms = C.c_double(0)

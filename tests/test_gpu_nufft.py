@@ -90,19 +90,25 @@ def test_degrid_on_grid_points(hip, oracle, prec, tol):
     """Visibilities exactly on grid lines (fractional offset 0 on an axis: all
     2W+1 taps inside the support) mixed with general ones, in the same tiles
     and waves: the degridder's f == 0 path and its 2W-tap path both match the
-    oracle.  The geometry is exact in binary (dx = 1/64, du = 1/4)."""
+    oracle.  On-line coordinates are integer multiples k du whose quotient by
+    du (what the plan and the oracle compute) is exactly k."""
     B, O = hip, oracle
     g = rng(67)
     n = 60000
+    geo = O.NUFFTGeometry(O.Grid2D(128, 128, [-1.0, 1.0, -1.0, 1.0]), pad=2, wsup=3, beta=14.04)
     uv = np.zeros((n, 3))
-    uv[:, :2] = g.uniform(-30.0, 30.0, size=(n, 2))
+    uv[:, 0] = g.uniform(-100.0, 100.0, size=n) * geo.du
+    uv[:, 1] = g.uniform(-100.0, 100.0, size=n) * geo.dv
     on_u = g.random(n) < 0.3
     on_v = g.random(n) < 0.3
-    uv[on_u, 0] = np.round(uv[on_u, 0] * 4.0) / 4.0
-    uv[on_v, 1] = np.round(uv[on_v, 1] * 4.0) / 4.0
+    ku = np.round(uv[:, 0] / geo.du)
+    kv = np.round(uv[:, 1] / geo.dv)
+    uv[on_u, 0] = ku[on_u] * geo.du
+    uv[on_v, 1] = kv[on_v] * geo.dv
+    on_u &= uv[:, 0] / geo.du == ku
+    on_v &= uv[:, 1] / geo.dv == kv
     assert np.sum(on_u & on_v) > 1000
     h, keep = _plan(B, uv, [1.0], 128, 128, [(-1.0, 1.0), (-1.0, 1.0)], 3, 14.04, prec)
-    geo = O.NUFFTGeometry(O.Grid2D(128, 128, [-1.0, 1.0, -1.0, 1.0]), pad=2, wsup=3, beta=14.04)
     grid = np.ascontiguousarray(g.normal(size=(256, 129, 2)).view(np.complex128)[..., 0])
     vis = np.zeros(n, dtype=np.complex128)
     B.check(B.lib().lt_nufft_degrid_host(h, B.ptr(grid), B.ptr(vis)))
@@ -110,6 +116,42 @@ def test_degrid_on_grid_points(hip, oracle, prec, tol):
     assert O.max_relative_error(vis, ref) < tol
     for sel in (on_u | on_v, ~(on_u | on_v)):
         assert np.max(np.abs(vis[sel] - ref[sel])) < tol * np.max(np.abs(ref))
+    B.lib().lt_nufft_destroy(h)
+
+
+@pytest.mark.parametrize('prec,tol', [(0, 1e-12), (1, 1e-5)])
+def test_degrid_forms_agree(hip, oracle, prec, tol):
+    """Both tap evaluations of the degridder (lt_nufft_set_degrid_form): the
+    window taps in the offset from the cell centre (default) and the
+    per-offset taps around the nearest grid point picked by the offset's
+    sign, on random visibilities plus a block of visibilities exactly half a
+    cell off the grid on one or both axes (the rounding boundary of the
+    nearest grid point, offset +-1/2): each matches the oracle and the other."""
+    B, O = hip, oracle
+    g = rng(69)
+    uv = annulus_uv(100000, seed=69)
+    du = 1.0 / (4.8481368111e-6 * 5.12)  # the padded c1 image: 256 cells of 0.02"
+    half = np.zeros((4096, 3))
+    half[:, :2] = (np.floor(g.uniform(-45.0, 45.0, size=(4096, 2))) + 0.5) * du
+    half[::2, 1] = g.uniform(-45.0, 45.0, size=2048) * du
+    uv = np.concatenate([uv, half])
+    n = uv.shape[0]
+    h, keep = _plan(B, uv, [1.0], 128, 128, [(-1.28, 1.28), (-1.28, 1.28)], 3, 14.04, prec)
+    geo = O.NUFFTGeometry(O.Grid2D(128, 128, [-1.28, 1.28, -1.28, 1.28]), pad=2, wsup=3, beta=14.04)
+    grid = np.ascontiguousarray(g.normal(size=(256, 129, 2)).view(np.complex128)[..., 0])
+    ref = O.grid(uv, [1.0], None, geo, 0, grid_in=grid, nthreads=8).reshape(-1)
+    out = []
+    for form in (1, 0):
+        B.check(B.lib().lt_nufft_set_degrid_form(h, form))
+        vis = np.zeros(n, dtype=np.complex128)
+        B.check(B.lib().lt_nufft_degrid_host(h, B.ptr(grid), B.ptr(vis)))
+        err = O.max_relative_error(vis, ref)
+        print(f'precision {prec} degrid form {form}: {err:.2e} against the oracle')
+        assert err < tol
+        assert np.max(np.abs(vis[-4096:] - ref[-4096:])) < tol * np.max(np.abs(ref))
+        out.append(vis)
+    assert O.max_relative_error(out[0], out[1]) < tol
+    assert B.lib().lt_nufft_set_degrid_form(h, 2) == B.LT_ERR_ARG
     B.lib().lt_nufft_destroy(h)
 
 

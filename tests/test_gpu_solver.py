@@ -176,6 +176,7 @@ def test_stencil_matrix_core_form_f32(hip, oracle):
     y64 = np.stack([y.reshape(-1) for y in inv.apply(xs)])
     yv32 = np.stack([y.reshape(-1) for y in inv.apply(xs, precision='f32')])
     assert O.max_relative_error(yv32, y64) < 1e-5
+    base = None
     for g in (-1, -2, -4):
         plan.set_stencil_shape(g)
         inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
@@ -183,6 +184,9 @@ def test_stencil_matrix_core_form_f32(hip, oracle):
         e64, ev = O.max_relative_error(ym, y64), O.max_relative_error(ym, yv32)
         print(f'FP32 matrix-core form, {-g} groups per workgroup: vs FP64 {e64:.2e}, vs FP32 vector form {ev:.2e}')
         assert e64 < 1e-5 and ev < 1e-5
+        if base is None:
+            base = ym
+        assert np.array_equal(ym, base), g
     plan.set_stencil_shape(-2)
     inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
     x, its, res = inv.solve(rtol=1e-6, maxiter=20000, precision='f32', refine_every=20)
@@ -414,3 +418,27 @@ def test_lens_preconditioner(hip, oracle):
             assert np.sum(np.diff(lb.csr(0)[0]) > 4) > 0  # the limit does remove rows
     x2, it2, r2 = inv.solve(rtol=1e-6, maxiter=20000, precond='lens')
     assert np.all(r2 <= 1e-6)
+
+
+def test_lens_preconditioner_precision_form(hip, oracle):
+    """The lens-structured preconditioner with the FP32 operator (precision
+    form) inside the flexible FP64 CG: every sample reaches the FP64 true
+    residual 1e-6 that the oracle's operator confirms, in about the FP64
+    lens-preconditioned iteration count."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    im, src, uv, vs, models, data = _problem(nlens=3, seed=83)
+    lb = LensBatch(im, [src] * 3, models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+    x64, it64, r64 = inv.solve(rtol=1e-6, maxiter=20000, precond='lens')
+    x32, it32, r32 = inv.solve(rtol=1e-6, maxiter=20000, precond='lens', precision='f32', refine_every=20)
+    print(f'lens-preconditioned iterations: FP64 operator {it64}, FP32 operator {it32}; residuals {r32}')
+    assert np.all(r64 <= 1e-6) and np.all(r32 <= 1e-6)
+    assert np.max(it32) <= 1.5 * np.max(it64)
+    for k in range(3):
+        x = x32[k].reshape(-1)
+        Ax, (L, Cm, geo) = _oracle_apply(O, im, src, uv, lb.csr(k), x, w, 1e2)
+        b = L.T @ O.nufft_adjoint((data * w).reshape(1, -1), uv, [1.0], geo, nthreads=8).reshape(-1)
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= RESID_BAR
