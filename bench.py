@@ -301,13 +301,19 @@ def batching_form(args):
         all_res.append(res)
     total = time.perf_counter() - t0
     its, res = np.concatenate(all_its), np.concatenate(all_res)
-    t = time.perf_counter()
-    its32, res32 = wl.step(maxiter=args.maxiter, precision='f32')
-    t32 = time.perf_counter() - t
-    t = time.perf_counter()
-    itsl, resl = wl.step(maxiter=args.maxiter, precond='lens')
-    tl = time.perf_counter() - t
-    log(f'batching: FP64 {total / args.steps:.2f}s, precision form {t32:.2f}s, lens preconditioner {tl:.2f}s')
+
+    def leg(**kw):  # warm-up steps (FP32 plans, tuning, M's build buffers), then one timed step
+        for k in range(args.warmup):
+            wl.step(maxiter=args.maxiter, **kw)
+        t = time.perf_counter()
+        out = wl.step(maxiter=args.maxiter, **kw)
+        return out, time.perf_counter() - t
+
+    (its32, res32), t32 = leg(precision='f32')
+    (itsl, resl), tl = leg(precond='lens')
+    (itsp, resp), tp = leg(precond='lens', precision='f32')
+    log(f'batching: FP64 {total / args.steps:.2f}s, precision form {t32:.2f}s, lens preconditioner {tl:.2f}s, '
+        f'lens preconditioner with the FP32 operator {tp:.2f}s')
     out = {'metric': 'source solves/s at 1e-6 rel resid (batching form)', 'value': nlens * args.steps / total,
            'unit': 'source solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
            'ms_per_step': 1e3 * total / args.steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
@@ -324,7 +330,12 @@ def batching_form(args):
                               'all_converged': bool(np.all(res32 <= 1e-6))},
            'lens_preconditioner': {'value': nlens / tl, 'ms_per_step': 1e3 * tl, 'iters_median': float(np.median(itsl)),
                                    'iters_max': int(itsl.max()), 'true_rel_resid_max': float(resl.max()),
-                                   'all_converged': bool(np.all(resl <= 1e-6))}}
+                                   'all_converged': bool(np.all(resl <= 1e-6)),
+                                   'precision_form': {'value': nlens / tp, 'ms_per_step': 1e3 * tp,
+                                                      'iters_median': float(np.median(itsp)),
+                                                      'iters_max': int(itsp.max()),
+                                                      'true_rel_resid_max': float(resp.max()),
+                                                      'all_converged': bool(np.all(resp <= 1e-6))}}}
     print(json.dumps(out), flush=True)
 
 

@@ -597,58 +597,50 @@ __global__ void k_csr_diag(long n, const int32_t *__restrict__ rp, const int32_t
     d[r] = v > 0.0 ? v : 1.0;
 }
 
-// inner CG start: x = 0, r = the outer residual, z = p = r / dM
-__global__ void k_inner_start(long n, const double *__restrict__ r0, const double *__restrict__ dM,
-                              double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
-                              double *__restrict__ p) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double ri = r0[i], zi = ri / dM[i];
-    x[i] = 0.0;
-    r[i] = ri;
-    z[i] = zi;
-    p[i] = zi;
+// The inner Jacobi-PCG steps on M spread every sample over P workgroups
+// (gridDim.x = P, blockIdx.y = sample), each taking the contiguous slice
+// k_seg_dot_part takes.  A per-sample dot is left as its P block partials;
+// the kernel that needs the scalar sums the P partials of its sample in
+// order 0..P-1 in every workgroup (the same value in all of them), so the
+// results do not depend on scheduling and no workgroup waits for another.
+__device__ inline void part_slice(const int64_t *__restrict__ off, int s, int q, int P, long &lo, long &hi) {
+    const long b0 = off[s], len = off[s + 1] - b0;
+    const long per = (len + P - 1) / P;
+    lo = b0 + min(len, per * q);
+    hi = b0 + min(len, per * (q + 1));
 }
 
-// inner CG update: x += a_s p; r -= a_s Mp; z = r / dM
-__global__ void k_inner_xr(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
-                           const double *__restrict__ pMp, const double *__restrict__ p, const double *__restrict__ Mp,
-                           const double *__restrict__ dM, double *__restrict__ x, double *__restrict__ r,
-                           double *__restrict__ z) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = seg[i];
-    const double a = pMp[s] != 0.0 ? rz[s] / pMp[s] : 0.0;
-    x[i] += a * p[i];
-    r[i] -= a * Mp[i];
-    z[i] = r[i] / dM[i];
-}
-
-// The inner steps with their per-sample dots folded in (one workgroup per
-// sample, the elements and the reduction in k_seg_dot_one's order, so the
-// results are those of the separate kernels bit for bit).
-__device__ inline double block_sum_one(double acc) {
-    __shared__ double sh[kDotOne / 64];
+// fixed-order sum over the block (kDotThreads threads); valid in thread 0
+__device__ inline double block_sum_part(double acc) {
+    __shared__ double sh[kDotThreads / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
     double t = 0.0;
     if (threadIdx.x == 0)
-        for (int k = 0; k < kDotOne / 64; ++k) t += sh[k];
-    return t;  // valid in thread 0
+        for (int k = 0; k < kDotThreads / 64; ++k) t += sh[k];
+    return t;
 }
 
-// start: x = 0, r = r0, z = p = r0 / dM, rz = r . z
-__global__ void __launch_bounds__(kDotOne) k_inner_start_dot(const int64_t *__restrict__ off,
-                                                              const double *__restrict__ r0,
-                                                              const double *__restrict__ dM, double *__restrict__ x,
-                                                              double *__restrict__ r, double *__restrict__ z,
-                                                              double *__restrict__ p, double *__restrict__ rz) {
-    const int s = blockIdx.x;
-    const long lo = off[s], hi = off[s + 1];
+__device__ inline double parts_sum(const double *__restrict__ part, int s, int P) {
+    double t = 0.0;
+    for (int q = 0; q < P; ++q) t += part[(long)s * P + q];
+    return t;
+}
+
+// start: x = 0, r = r0, z = p = r0 / dM; partials of r . z
+__global__ void __launch_bounds__(kDotThreads) k_inner_start_part(const int64_t *__restrict__ off,
+                                                                   const double *__restrict__ r0,
+                                                                   const double *__restrict__ dM,
+                                                                   double *__restrict__ x, double *__restrict__ r,
+                                                                   double *__restrict__ z, double *__restrict__ p,
+                                                                   double *__restrict__ part_rz) {
+    const int s = blockIdx.y, q = blockIdx.x, P = gridDim.x;
+    long lo, hi;
+    part_slice(off, s, q, P, lo, hi);
     double acc = 0.0;
-    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) {
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) {
         const double ri = r0[i], zi = ri / dM[i];
         x[i] = 0.0;
         r[i] = ri;
@@ -656,25 +648,32 @@ __global__ void __launch_bounds__(kDotOne) k_inner_start_dot(const int64_t *__re
         p[i] = zi;
         acc += ri * zi;
     }
-    const double t = block_sum_one(acc);
-    if (threadIdx.x == 0) rz[s] = t;
+    const double t = block_sum_part(acc);
+    if (threadIdx.x == 0) part_rz[(long)s * P + q] = t;
 }
 
-// step: x += a p; r -= a Mp; z = r / dM (a = rz / pMp); rz_new = r . z; and,
-// when another step follows, p = z + (rz_new / rz) p
-__global__ void __launch_bounds__(kDotOne) k_inner_step_dot(const int64_t *__restrict__ off,
-                                                             const double *__restrict__ rz,
-                                                             const double *__restrict__ pMp, double *__restrict__ p,
-                                                             const double *__restrict__ Mp,
-                                                             const double *__restrict__ dM, double *__restrict__ x,
-                                                             double *__restrict__ r, double *__restrict__ z,
-                                                             double *__restrict__ rz_new, int update_p) {
-    __shared__ double bsh;
-    const int s = blockIdx.x;
-    const long lo = off[s], hi = off[s + 1];
-    const double a = pMp[s] != 0.0 ? rz[s] / pMp[s] : 0.0;
+// step: a = rz / pMp; x += a p; r -= a Mp; z = r / dM; partials of r . z
+__global__ void __launch_bounds__(kDotThreads) k_inner_step_part(const int64_t *__restrict__ off,
+                                                                  const double *__restrict__ part_rz,
+                                                                  const double *__restrict__ part_pMp,
+                                                                  const double *__restrict__ p,
+                                                                  const double *__restrict__ Mp,
+                                                                  const double *__restrict__ dM,
+                                                                  double *__restrict__ x, double *__restrict__ r,
+                                                                  double *__restrict__ z,
+                                                                  double *__restrict__ part_rz_new) {
+    __shared__ double a_sh;
+    const int s = blockIdx.y, q = blockIdx.x, P = gridDim.x;
+    if (threadIdx.x == 0) {
+        const double rz = parts_sum(part_rz, s, P), pMp = parts_sum(part_pMp, s, P);
+        a_sh = pMp != 0.0 ? rz / pMp : 0.0;
+    }
+    __syncthreads();
+    const double a = a_sh;
+    long lo, hi;
+    part_slice(off, s, q, P, lo, hi);
     double acc = 0.0;
-    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) {
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) {
         x[i] += a * p[i];
         const double ri = r[i] - a * Mp[i];
         r[i] = ri;
@@ -682,25 +681,26 @@ __global__ void __launch_bounds__(kDotOne) k_inner_step_dot(const int64_t *__res
         z[i] = zi;
         acc += ri * zi;
     }
-    const double t = block_sum_one(acc);
-    if (threadIdx.x == 0) {
-        rz_new[s] = t;
-        bsh = rz[s] != 0.0 ? t / rz[s] : 0.0;
-    }
-    if (!update_p) return;
-    __syncthreads();
-    const double b = bsh;
-    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) p[i] = z[i] + b * p[i];
+    const double t = block_sum_part(acc);
+    if (threadIdx.x == 0) part_rz_new[(long)s * P + q] = t;
 }
 
 // p = z + (rz_new / rz) p
-__global__ void k_inner_p(long n, const int32_t *__restrict__ seg, const double *__restrict__ rz,
-                          const double *__restrict__ rz_new, const double *__restrict__ z, double *__restrict__ p) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = seg[i];
-    const double b = rz[s] != 0.0 ? rz_new[s] / rz[s] : 0.0;
-    p[i] = z[i] + b * p[i];
+__global__ void __launch_bounds__(kDotThreads) k_inner_p_part(const int64_t *__restrict__ off,
+                                                               const double *__restrict__ part_rz,
+                                                               const double *__restrict__ part_rz_new,
+                                                               const double *__restrict__ z, double *__restrict__ p) {
+    __shared__ double b_sh;
+    const int s = blockIdx.y, q = blockIdx.x, P = gridDim.x;
+    if (threadIdx.x == 0) {
+        const double rz = parts_sum(part_rz, s, P), rzn = parts_sum(part_rz_new, s, P);
+        b_sh = rz != 0.0 ? rzn / rz : 0.0;
+    }
+    __syncthreads();
+    const double b = b_sh;
+    long lo, hi;
+    part_slice(off, s, q, P, lo, hi);
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) p[i] = z[i] + b * p[i];
 }
 
 // samples whose network was switched off take the lens-structured direction
@@ -751,7 +751,10 @@ struct Solver {
     double lens_c = 0.0;           // weight of L_s^T L_s (0: the data operator's impulse-response peak)
     double lens_c_used = 0.0;
     const int *cur_act = nullptr;  // active flags of the running solve (sparse products skip the rest)
-    DevBuf<double> dM, ix, ir, iz, ip, iMp, irz, irz2, ipMp;
+    DevBuf<double> dM, ix, ir, iz, ip, iMp;
+    // inner dots as partials: inner_parts workgroups per sample (lens_setup)
+    int inner_parts = 1;
+    DevBuf<double> ipart_rz, ipart_rz2, ipart_pMp;
     bool lens_ready = false;
     DevCSR Mpc;                    // explicit M (block-diagonal over samples)
     lt_status lens_setup();
@@ -1074,7 +1077,10 @@ lt_status Solver::lens_setup() {
     if (lens_ready) return LT_OK;
     const unsigned gb = blocks_for(ntot, 256);
     for (auto *v : {&dM, &ix, &ir, &iz, &ip, &iMp}) LT_TRY(v->alloc(ntot));
-    for (auto *v : {&irz, &irz2, &ipMp}) LT_TRY(v->alloc(S));
+    // enough workgroups to spread the inner vectors over the chip: about
+    // 1024 in all, at most 64 per sample
+    inner_parts = std::max(1, std::min(64, (1024 + S - 1) / S));
+    for (auto *v : {&ipart_rz, &ipart_rz2, &ipart_pMp}) LT_TRY(v->alloc((long)S * inner_parts));
     double c = lens_c;
     if (!(c > 0.0)) {
         // impulse response of the data operator at the image centre (sample
@@ -1170,6 +1176,7 @@ lt_status Solver::lens_build_matrix() {
                        Mpc.rp.p);
     LT_CHECK_LAUNCH();
     LT_TRY(csr_prepare(Mpc));
+    LT_TRY(csr_make_f32(Mpc));  // the inner solve's products (csr_spmv_fv)
     LT_TRY(csr_autotune(Mpc, false));
     LT_TRY(csr_set_samples(Mpc, soff_host.data(), S));
     return sync();
@@ -1182,36 +1189,23 @@ lt_status Solver::lens_apply(const int *af, const int *alist, int nact) {
     // the inner iterate is z itself (no final copy); with no inner steps z is
     // the Jacobi step r / dM
     double *xi = lens_inner > 0 ? z.p : ix.p;
-    double *rzc = irz.p, *rzn = irz2.p;  // r . z of the current and the next inner step (swapped, not copied)
-    // with one workgroup per sample's dot, the updates and the dots share
-    // one launch (same bits as the separate kernels)
-    const bool fused = (dot_parts == 1);
-    if (fused) {
-        hipLaunchKernelGGL(k_inner_start_dot, dim3(S), dim3(kDotOne), 0, stream(), soff.p, r.p, dM.p, xi, ir.p, iz.p,
-                           ip.p, rzc);
-    } else {
-        hipLaunchKernelGGL(k_inner_start, dim3(gb), dim3(256), 0, stream(), ntot, r.p, dM.p, xi, ir.p, iz.p, ip.p);
-        LT_TRY(seg_dot(ir.p, iz.p, rzc));
-    }
+    // per-sample dots as partials of inner_parts workgroups per sample: r . z
+    // of the current and the next inner step (swapped, not copied), p . Mp
+    double *rzc = ipart_rz.p, *rzn = ipart_rz2.p;
+    const int P = inner_parts;
+    const dim3 grid((unsigned)P, (unsigned)S);
+    hipLaunchKernelGGL(k_inner_start_part, grid, dim3(kDotThreads), 0, stream(), soff.p, r.p, dM.p, xi, ir.p, iz.p,
+                       ip.p, rzc);
     LT_CHECK_LAUNCH();
     for (int k = 0; k < lens_inner; ++k) {
-        // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc
-        LT_TRY(csr_spmv(Mpc, ip.p, iMp.p, 1.0, 0.0, 0, af, 0, seg.p, alist, nact));
-        LT_TRY(seg_dot(ip.p, iMp.p, ipMp.p));
-        const bool more = k + 1 < lens_inner;
-        if (fused) {
-            hipLaunchKernelGGL(k_inner_step_dot, dim3(S), dim3(kDotOne), 0, stream(), soff.p, rzc, ipMp.p, ip.p,
-                               iMp.p, dM.p, xi, ir.p, iz.p, rzn, more ? 1 : 0);
-            LT_CHECK_LAUNCH();
-            if (more) std::swap(rzc, rzn);
-            continue;
-        }
-        hipLaunchKernelGGL(k_inner_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rzc, ipMp.p, ip.p, iMp.p,
-                           dM.p, xi, ir.p, iz.p);
+        // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc (FP32 values)
+        LT_TRY(csr_spmv_fv(Mpc, ip.p, iMp.p, af, seg.p));
+        hipLaunchKernelGGL(k_seg_dot_part, grid, dim3(kDotThreads), 0, stream(), soff.p, ip.p, iMp.p, ipart_pMp.p);
+        hipLaunchKernelGGL(k_inner_step_part, grid, dim3(kDotThreads), 0, stream(), soff.p, rzc, ipart_pMp.p, ip.p,
+                           iMp.p, dM.p, xi, ir.p, iz.p, rzn);
         LT_CHECK_LAUNCH();
-        if (more) {
-            LT_TRY(seg_dot(ir.p, iz.p, rzn));
-            hipLaunchKernelGGL(k_inner_p, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, rzc, rzn, iz.p, ip.p);
+        if (k + 1 < lens_inner) {
+            hipLaunchKernelGGL(k_inner_p_part, grid, dim3(kDotThreads), 0, stream(), soff.p, rzc, rzn, iz.p, ip.p);
             LT_CHECK_LAUNCH();
             std::swap(rzc, rzn);
         }

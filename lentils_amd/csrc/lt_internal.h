@@ -186,6 +186,12 @@ lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, do
 int csr_pick_lanes(const DevCSR &A);
 // FP32 copy of the values (after csr_prepare) and y = alpha A x + beta y in FP32
 lt_status csr_make_f32(DevCSR &A);
+// y = A x with FP32 copies of the values (made on first use) and FP64 x, y,
+// products and sums, by the streaming kernel (every row summed whole by the
+// workgroup its first entry falls in): for a preconditioner's matrix, where
+// the values' rounding only changes the preconditioner.  Rows of samples with
+// act == 0 (rowseg: row -> sample) are skipped.
+lt_status csr_spmv_fv(DevCSR &A, const double *x, double *y, const int *act, const int32_t *rowseg);
 // pick the fastest lane-group width for this matrix by timing (FP64 or FP32)
 lt_status csr_autotune(DevCSR &A, bool fp32);
 lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, float beta_y, const int *act = nullptr,

@@ -280,10 +280,12 @@ __device__ inline long row_sample(const RowFilter &F, long row) {
     return F.rowseg ? (long)F.rowseg[row] : row / F.rps;
 }
 
-template <class T>
+// TV: the stored value type (T, or FP32 values against FP64 vectors for a
+// preconditioner's matrix, csr_spmv_fv); products and sums in T
+template <class T, class TV = T>
 __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__restrict__ rblk,
                                                       const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                      const T *__restrict__ va, const T *__restrict__ x,
+                                                      const TV *__restrict__ va, const T *__restrict__ x,
                                                       T *__restrict__ y, T alpha, T beta, int skip_long, RowFilter F) {
     __shared__ T prod[kStreamNnz];
     __shared__ int32_t rps[kStreamNnz + 1];
@@ -304,18 +306,18 @@ __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__
         for (long r = threadIdx.x; r <= nr; r += 256) rps[r] = rp[r0 + r];
     if (k1 - k0 == kStreamNnz) {  // full range: all loads of a thread in flight together
         int c[PER];
-        T v[PER], xv[PER];
+        T v[PER], xv[PER];  // values widened to T as loaded
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             c[u] = ci[k0 + threadIdx.x + 256 * u];
-            v[u] = va[k0 + threadIdx.x + 256 * u];
+            v[u] = (T)va[k0 + threadIdx.x + 256 * u];
         }
 #pragma unroll
         for (int u = 0; u < PER; ++u) xv[u] = x[c[u]];
 #pragma unroll
         for (int u = 0; u < PER; ++u) prod[threadIdx.x + 256 * u] = v[u] * xv[u];
     } else {
-        for (long k = k0 + threadIdx.x; k < k1; k += 256) prod[k - k0] = va[k] * x[ci[k]];
+        for (long k = k0 + threadIdx.x; k < k1; k += 256) prod[k - k0] = (T)va[k] * x[ci[k]];
     }
     __syncthreads();
     for (long r = r0 + threadIdx.x; r < r1; r += 256) {
@@ -325,7 +327,7 @@ __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__
         const int ein = (int)min((long)re, k1);
         T s = (T)0;
         for (int k = rb; k < ein; ++k) s += prod[k - k0];
-        for (int k = ein; k < re; ++k) s += va[k] * x[ci[k]];
+        for (int k = ein; k < re; ++k) s += (T)va[k] * x[ci[k]];
         y[r] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[r];
     }
 }
@@ -732,6 +734,17 @@ lt_status csr_spmv(const DevCSR &A, const double *x, double *y, double alpha, do
                    const int *act, long rows_per_sample, const int32_t *rowseg, const int *alist, int nact) {
     return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, alpha, beta_y, lanes, act, rows_per_sample, rowseg, false,
                           alist, nact);
+}
+
+lt_status csr_spmv_fv(DevCSR &A, const double *x, double *y, const int *act, const int32_t *rowseg) {
+    if (A.nrows == 0) return LT_OK;
+    if (!A.prepared || A.nblk <= 0) return csr_spmv(A, x, y, 1.0, 0.0, 0, act, 0, rowseg);
+    if (!A.va32.p) LT_TRY(csr_make_f32(A));
+    const RowFilter F{act, 1, rowseg};
+    hipLaunchKernelGGL((k_spmv_stream<double, float>), dim3((unsigned)A.nblk), dim3(256), 0, stream(), A.nnz,
+                       A.rblk.p, A.rp.p, A.ci.p, A.va32.p, x, y, 1.0, 0.0, 0, F);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
 }
 
 lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act, long rows_per_sample,
