@@ -153,7 +153,7 @@ struct GlobalFrags {
 // Long rows with at most kMedium fragments are walked by one thread each
 // (sequentially, like short rows); longer ("huge") rows take the frontier
 // decomposition below.  This pass counts every long row up to kMedium + 1.
-constexpr long kMedium = 128;
+constexpr long kMedium = 512;
 
 struct CappedCount {
     long n = 0;
