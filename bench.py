@@ -339,6 +339,29 @@ def batching_form(args):
     print(json.dumps(out), flush=True)
 
 
+def validation_set(wl, nval, seed):
+    """Held-out lens samples for model selection: their own lens parameters
+    (seed), blob sources and noise on the workload's uv set, entering as
+    device-resident dirty images like the workload's own."""
+    from lentils_amd.solvers import DeviceArray, LensBatch
+    B = wl.B
+    lenses = gen_lenses(nval, seed=seed)
+    truth = LensBatch(wl.im, [wl.src] * nval, lenses, z_src=2.0)
+    g = np.random.Generator(np.random.PCG64(seed + 1))
+    dirty = []
+    for s in range(nval):
+        rp, ci, va = truth.csr(s)
+        x = np.ascontiguousarray(blob_source(wl.src, seed=seed + 2 + s).reshape(-1))
+        img = np.zeros(wl.im.size)
+        B.check(B.lib().lt_csr_spmv_host(wl.im.size, wl.src.size, B.ptr(rp), B.ptr(ci), B.ptr(va), B.ptr(x),
+                                         B.ptr(img)))
+        vis = wl.plan.forward(img).astype(np.complex64)
+        vis.real += g.standard_normal(wl.nvis, dtype=np.float32) * np.float32(1e-3)
+        vis.imag += g.standard_normal(wl.nvis, dtype=np.float32) * np.float32(1e-3)
+        dirty.append(wl.plan.adjoint(wl.weights * vis.astype(np.complex128)))
+    return lenses, DeviceArray(np.stack(dirty))
+
+
 def learned_form(args):
     """Learned-preconditioner workload: a batch of `nlens` (256) lens samples
     with 128^2 source grids and the 1e7-visibility uv set (one observation per
@@ -350,10 +373,14 @@ def learned_form(args):
     draws each, the solver generates the right-hand sides b = A x and the
     residuals of M-preconditioned solves of those b stopped after 8 and 32
     iterations; `epochs` passes of Adam on the scale-invariant energy loss
-    over them.  The learned solves use flexible CG keeping 4 directions
+    over them.  The network is then selected on held-out lenses: the
+    parameters after each training batch, with the correction scaled by 1,
+    1/2 or 1/4, and the zero correction (M alone), each timed on a
+    validation batch (`val_samples` lens samples of another seed); the
+    fastest is kept.  The learned solves use flexible CG keeping 4 directions
     A-orthogonal.  Prints one JSON line."""
     from lentils_amd import backend as B
-    from lentils_amd.learned import PCNet, blob_sources, solver_residuals
+    from lentils_amd.learned import PCNet, blob_sources, scale_correction, solver_residuals
     from lentils_amd.solvers import BatchedSourceInversion, LensBatch
     B.check(B.lib().lt_init(0))
     nlens = args.learned_samples
@@ -366,6 +393,7 @@ def learned_form(args):
     t_gen = t_train = 0.0
     npairs = 0
     hist = []
+    checkpoints = []
     for tb in range(args.train_batches):
         t = time.perf_counter()
         lb = LensBatch(wl.im, wl.srcs, gen_lenses(nlens, seed=1000 + tb), z_src=2.0)
@@ -383,7 +411,33 @@ def learned_form(args):
         hist.append([float(v) for v in h])
         log(f'training batch {tb}: {len(vecs)} vectors x {nlens} samples, loss {h[0]:.4f} -> {h[-1]:.4f} '
             f'(generation {t_gen:.1f}s, training {t_train:.1f}s so far)')
+        checkpoints.append(net.params)
         del inv, lb, vecs
+    # model selection on held-out lenses: solve time of the validation batch
+    t = time.perf_counter()
+    vlenses, vdirty = validation_set(wl, args.val_samples, seed=3000)
+    vlb = LensBatch(wl.im, [wl.src] * args.val_samples, vlenses, z_src=2.0)
+    vinv = BatchedSourceInversion(vlb, wl.plan, dirty_images=vdirty, lambda_curvature=wl.lambda_curv)
+    vinv.set_flexible_depth(4)
+    cands = [(-1, 0.0)] + [(k, a) for k in range(len(checkpoints)) for a in (1.0, 0.5, 0.25)]
+    trials = []
+    for k, a in cands:
+        net.params = scale_correction(checkpoints[max(k, 0)], a)
+        vinv.set_preconditioner(net)
+        vinv.solve(rtol=1e-6, maxiter=args.maxiter, precond='learned')  # warm-up
+        t1 = time.perf_counter()
+        _, vits, vres = vinv.solve(rtol=1e-6, maxiter=args.maxiter, precond='learned')
+        trials.append({'checkpoint': k, 'scale': a, 'solve_s': time.perf_counter() - t1,
+                       'iters_median': float(np.median(vits)), 'iters_max': int(vits.max()),
+                       'all_converged': bool(np.all(vres <= 1e-6))})
+    ok = [tr for tr in trials if tr['all_converged']] or trials
+    best = min(ok, key=lambda tr: tr['solve_s'])
+    net.params = scale_correction(checkpoints[max(best['checkpoint'], 0)], best['scale'])
+    t_sel = time.perf_counter() - t
+    log(f'selection on {args.val_samples} held-out lenses ({t_sel:.1f}s): checkpoint {best["checkpoint"]} '
+        f'scale {best["scale"]}: {best["solve_s"]:.2f}s, iters median {best["iters_median"]:.0f}; '
+        f'M alone {trials[0]["solve_s"]:.2f}s, iters median {trials[0]["iters_median"]:.0f}')
+    del vinv, vlb, vdirty
     results = {}
     for name, pc in (('plain', 'none'), ('lens', 'lens'), ('learned', 'learned')):
         for k in range(args.warmup):
@@ -405,7 +459,8 @@ def learned_form(args):
             results[name]['guard_fallback_iters'] = int(wl.last_fallbacks[0].sum())
             results[name]['samples_switched_to_M'] = int(wl.last_fallbacks[1].sum())
             results[name]['method'] = ('flexible CG keeping 4 directions A-orthogonal; z = M^-1 r + network '
-                                       'correction, M alone for samples whose residual fell < 10x in 100 iterations')
+                                       'correction (checkpoint and scale selected on held-out lenses), M alone for '
+                                       'samples whose residual fell < 10x in 100 iterations')
         log(f'{name}: {tot / args.steps:.2f}s/step iters median {np.median(its):.0f} max {its.max()} '
             f'max res {res.max():.2e}')
     out = {'metric': 'source solves/s at 1e-6 rel resid (learned-PCG vs plain CG); Gvis/s degrid',
@@ -423,7 +478,11 @@ def learned_form(args):
            'training': {'lens_batches': args.train_batches, 'draws_per_batch': args.draws,
                         'truncation_iters': list(iters), 'pairs': npairs, 'epochs': args.epochs,
                         'pair_visits': npairs * args.epochs, 'lr': args.lr, 'generation_s': t_gen,
-                        'training_s': t_train, 'loss_history': hist}}
+                        'training_s': t_train, 'loss_history': hist},
+           'selection': {'criterion': f'solve time of a validation batch of {args.val_samples} held-out lens '
+                                      'samples (lens seed 3000, own sources and noise); checkpoint -1 with scale 0 '
+                                      'is the zero correction (M alone)',
+                         'chosen': best, 'trials': trials, 'selection_s': t_sel}}
     print(json.dumps(out), flush=True)
 
 
@@ -669,6 +728,7 @@ def main():
     ap.add_argument('--draws', type=int, default=8)
     ap.add_argument('--epochs', type=int, default=6)
     ap.add_argument('--lr', type=float, default=2e-3)
+    ap.add_argument('--val-samples', type=int, default=32)
     ap.add_argument('--shared-data', action='store_true',
                     help='one observation for all samples (most samples then fit the data with the wrong lens)')
     args = ap.parse_args()

@@ -97,6 +97,21 @@ class PCNet:
             self._handle = None
 
 
+# the last layer (16 -> 1 channels, 3 x 3 kernels: 144 weights and one bias)
+# closes the parameter vector (csrc/pcnet.hip: layers in order, each its
+# [co][ci][3][3] weights then its biases); it is linear, so scaling it scales
+# the network's correction
+LAST_LAYER = 16 * 9 + 1
+
+
+def scale_correction(params, alpha):
+    """The parameters with the network's correction y scaled by alpha (the last
+    layer's weights and bias times alpha; alpha = 0 gives M alone)."""
+    p = np.array(params, dtype=np.float32, copy=True)
+    p[-LAST_LAYER:] *= np.float32(alpha)
+    return p
+
+
 def blob_sources(src_space, n, rng, nblobs=4, spread=0.3, width=(0.05, 0.2), amp=(0.5, 1.0)):
     """n random sources, each a sum of Gaussian blobs evaluated at the source
     grid's pixel positions (the workload's source family)."""

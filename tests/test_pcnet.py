@@ -68,6 +68,23 @@ def test_bf16_rounding_is_nearest_even():
     assert abs(r[4] - 3.0e-30) / 3.0e-30 < 2 ** -8
 
 
+def test_scale_correction_scales_the_output():
+    """scale_correction scales exactly the last layer (the network's output is
+    linear in it): in the oracle's network, alpha times the output."""
+    from lentils_amd.learned import LAST_LAYER, scale_correction
+    ci, co = R.CIN[-1], R.COUT[-1]
+    assert LAST_LAYER == co * ci * 9 + co
+    p = R.init_params(7)
+    p[-LAST_LAYER:] = _rng(8).normal(size=LAST_LAYER) * 0.1  # a trained-looking last layer
+    x0 = _rng(9).normal(size=(2, 2, 8, 8))
+    y1, _ = R.forward(p, x0)
+    for alpha in (0.0, 0.25, 1.0):
+        q = scale_correction(p, alpha).astype(np.float64)
+        assert np.array_equal(q[:-LAST_LAYER], p[:-LAST_LAYER].astype(np.float32).astype(np.float64))
+        ya, _ = R.forward(q, x0)
+        assert np.max(np.abs(ya - alpha * y1)) <= 1e-6 * np.max(np.abs(y1))
+
+
 def _net(B, H, W, nb, params):
     import ctypes as C
     h = C.c_void_p()
