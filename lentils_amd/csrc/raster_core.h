@@ -352,8 +352,13 @@ __host__ __device__ int raster_dfs(const RTri &root, int base, const SrcGrid &g,
         // the reference pushes every child and pops the last one next; here the
         // last child is kept as the current node and only the others go to the
         // stack (same order, same stack depths)
+        // The stack's top entry is kept in registers (top, valid when
+        // has_top): a leaf's pop then takes the entry pushed last without a
+        // round trip through memory; ns counts every entry, nm those in stk.
         RTri stk[kRasterStack + 3];
-        int ns = 0, pops = 0;
+        RTri top;
+        bool has_top = false;
+        int ns = 0, nm = 0, pops = 0;
         RTri cur = root;
         for (;;) {
             if (++pops > kRasterPopLimit) return 4;  // LT_RASTER_POP_LIMIT
@@ -376,15 +381,25 @@ __host__ __device__ int raster_dfs(const RTri &root, int base, const SrcGrid &g,
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     if (!ok[c]) continue;
-                    if (--left == 0)
+                    if (--left == 0) {
                         cur = kids[c];
-                    else
-                        stk[ns++] = kids[c];
+                    } else {
+                        if (has_top) stk[nm++] = top;
+                        top = kids[c];
+                        has_top = true;
+                        ++ns;
+                    }
                 }
                 continue;
             }
             if (ns == 0) break;
-            cur = stk[--ns];
+            --ns;
+            if (has_top) {
+                cur = top;
+                has_top = false;
+            } else {
+                cur = stk[--nm];
+            }
         }
         return 0;
     }
@@ -524,7 +539,8 @@ LT_HD void fan_vertices(const double *lat, int ly, int i, int j, int t, double t
 // per entry and returns the entry count; *first_col is the first entry's
 // column.  Rows of up to 64 fragments take one pass over the list per
 // entry (the next larger index found while summing the current one), so
-// the lists are only read; longer ones are sorted in place first.
+// the lists are only read -- in blocks of 8, so area and idx must then hold
+// 64 entries whatever n is; longer ones are sorted in place first.
 template <class Emit>
 __host__ __device__ int reduce_fragments(double *area, int *idx, int n, int stale, Emit &emit, int *first_col) {
     if (n == 0) {
@@ -533,24 +549,45 @@ __host__ __device__ int reduce_fragments(double *area, int *idx, int n, int stal
         return 1;
     }
     if (n <= 64) {
+        // the lists hold 64 entries (the short rows' buffers): each pass reads
+        // them in blocks of 8, all loads of a block issued together (entries
+        // beyond n are read but not used)
         int col = idx[0];
-        for (int f = 1; f < n; ++f) col = idx[f] < col ? idx[f] : col;
+        for (int f0 = 0; f0 < n; f0 += 8) {
+            int iv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) iv[u] = idx[f0 + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (f0 + u < n) col = iv[u] < col ? iv[u] : col;
+        }
         *first_col = col;
         int k = 0;
         for (bool first = true;; first = false) {
             double val = 0.0;
             bool skip = !first, more = false;
             int next = 0;
-            for (int f = 0; f < n; ++f) {
-                const int id = idx[f];
-                if (id == col) {
-                    if (skip)
-                        skip = false;
-                    else
-                        val += area[f];
-                } else if (id > col && (!more || id < next)) {
-                    next = id;
-                    more = true;
+            for (int f0 = 0; f0 < n; f0 += 8) {
+                int iv[8];
+                double av[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    iv[u] = idx[f0 + u];
+                    av[u] = area[f0 + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (f0 + u >= n) break;
+                    const int id = iv[u];
+                    if (id == col) {
+                        if (skip)
+                            skip = false;
+                        else
+                            val += av[u];
+                    } else if (id > col && (!more || id < next)) {
+                        next = id;
+                        more = true;
+                    }
                 }
             }
             emit(k++, col, val);

@@ -89,6 +89,10 @@ long hc_lens_matrix_from_lattice(int nx, int ny, const char *mask, int snx, int 
             st |= raster_pixel(lat, ly, i, j, g, s);
             StoreEmit e{cols, vals, k};
             const int n = (int)s.area.size();
+            if (n < 64) {  // reduce_fragments reads short lists in blocks up to 64 entries
+                s.area.resize(64, 0.0);
+                s.idx.resize(64, 0);
+            }
             int first = 0;
             k += reduce_fragments(s.area.data(), s.idx.data(), n, stale, e, &first);
             if (n > 0) stale = first;
