@@ -947,6 +947,69 @@ __global__ void k_coef(Geo G, long nrow, const int32_t *__restrict__ row_cell, c
     coef[g] = scale * acc;
 }
 
+// Hermitian fold of the stencil's rows.  The stencil's input S (the
+// checkerboard-signed spectrum on raw cells) satisfies S(-p) = conj(S(p)),
+// and the final half-plane spectrum at (a, b >= 0) is the raw row at (a, b)
+// plus the conjugate of the raw row at (-a, -b) (for b = 0 the Hermitian
+// pair that the c2r transform takes).  With real coefficients
+//   conj(R(-p)) = sign(p) sum_o c_(-p)[o] S(p - o),
+// so the two rows are one stencil row at p with coefficients
+//   c_p[o] + c_(-p)[-o]  (offset -o is index NO - 1 - o).
+// Folded rows: cells (a, b >= 0) inside the band where either raw row exists.
+__device__ inline int mirror_cell(const Geo &G, long g) {
+    const int a = (int)(g / G.nv) - G.cu, b = (int)(g % G.nv) - G.cv;
+    const int ma = -a + G.cu, mb = -b + G.cv;
+    return (ma >= 0 && ma < G.nu && mb >= 0 && mb < G.nv) ? ma * G.nv + mb : -1;
+}
+
+__global__ void k_fold_mark(Geo G, const int32_t *__restrict__ rowmap, int64_t *__restrict__ mark) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ng = (long)G.nu * G.nv;
+    if (g >= ng) return;
+    const int a = (int)(g / G.nv) - G.cu, b = (int)(g % G.nv) - G.cv;
+    int m = 0;
+    if (b >= 0 && b <= G.bv && abs(a) <= G.bu) {
+        const int mc = mirror_cell(G, g);
+        m = (rowmap[g] >= 0 || (mc >= 0 && rowmap[mc] >= 0)) ? 1 : 0;
+    }
+    mark[g] = m;
+}
+
+// new rowmap / row_cell, the raw rows of each folded row (r1: the cell's,
+// r2: its mirror's; -1: none) and its band cell b * nu + (a mod nu)
+__global__ void k_fold_rows(Geo G, const int64_t *__restrict__ off, const int32_t *__restrict__ rowmap_raw,
+                            int32_t *__restrict__ rowmap, int32_t *__restrict__ row_cell, int32_t *__restrict__ r1,
+                            int32_t *__restrict__ r2, int32_t *__restrict__ dst) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ng = (long)G.nu * G.nv;
+    if (g >= ng) return;
+    if (off[g + 1] == off[g]) {
+        rowmap[g] = -1;
+        return;
+    }
+    const int32_t fr = (int32_t)off[g];
+    rowmap[g] = fr;
+    row_cell[fr] = (int32_t)g;
+    const int mc = mirror_cell(G, g);
+    r1[fr] = rowmap_raw[g];
+    r2[fr] = mc >= 0 ? rowmap_raw[mc] : -1;
+    const int a = (int)(g / G.nv) - G.cu, b = (int)(g % G.nv) - G.cv;
+    dst[fr] = b * G.nu + (a < 0 ? a + G.nu : a);
+}
+
+__global__ void k_fold_coef(long nrow, int NO, const int32_t *__restrict__ r1, const int32_t *__restrict__ r2,
+                            long nrow_raw, const double *__restrict__ coef_raw, double *__restrict__ coef) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nrow * NO) return;
+    const int o = (int)(g / nrow);
+    const long row = g - (long)o * nrow;
+    const int a = r1[row], b = r2[row];
+    double c = 0.0;
+    if (a >= 0) c = coef_raw[(long)o * nrow_raw + a];
+    if (b >= 0) c += coef_raw[(long)(NO - 1 - o) * nrow_raw + b];
+    coef[g] = c;
+}
+
 __global__ void k_stencil_tiles(long nrow, const int32_t *__restrict__ row_cell, int nv, int st, int ntv,
                                 int64_t *__restrict__ tmark) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -962,7 +1025,9 @@ __global__ void k_compact(long n, const int64_t *__restrict__ off, int32_t *__re
 }
 
 // R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) for NB samples per
-// block; S windows of the tile staged in LDS with fold / wrap / sign.  The
+// block, written to out[s * ostride + (odst ? odst[row] : row)] (the rows'
+// band cells, or a compact per-row buffer); S windows of the tile staged
+// in LDS with fold / wrap / sign.  The
 // workgroups of one tile (one per group of NB samples) run back to back on
 // one XCD, so the tile's coefficients come from that XCD's L2 after the
 // first group.  RT > 0: the radius R compiled in (the offset loop unrolled,
@@ -971,7 +1036,8 @@ template <class T, int NB, class TC, bool BAND = false, int RT = 0>
 __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, int ntv_st, const int32_t *__restrict__ tiles,
                                                        int ngroup, const int32_t *__restrict__ rowmap, long nrow,
                                                        const TC *__restrict__ coef, const Cplx<T> *__restrict__ spec,
-                                                       long sstride, Cplx<T> *__restrict__ rbuf,
+                                                       long sstride, Cplx<T> *__restrict__ out,
+                                                       const int32_t *__restrict__ odst, long ostride,
                                                        const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
     const int R = RT > 0 ? RT : R_;
@@ -1036,7 +1102,8 @@ __global__ void __launch_bounds__(256) k_stencil_apply(Geo G, int R_, int ST, in
         }
     }
     const T sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? (T)-1 : (T)1;
-    for (int k = 0; k < ns; ++k) rbuf[(long)smp[k] * nrow + row] = Cplx<T>{sg * ar[k], sg * ai[k]};
+    const long d = odst ? (long)odst[row] : (long)row;
+    for (int k = 0; k < ns; ++k) out[(long)smp[k] * ostride + d] = Cplx<T>{sg * ar[k], sg * ai[k]};
 }
 
 // --- FP64 normal stencil on the matrix cores ------------------------------
@@ -1074,12 +1141,12 @@ __global__ void k_mf_blocks(long nrow, const int32_t *__restrict__ row_cell, int
 }
 
 // A[((block * kMfNT + tile) * 32 + pair) * 64 + lane][half] with step
-// 4 w + q = pair (w / 2) * 4 + q, half w % 2, and the tiles' rows
-// (-1 where a cell is no stencil row)
+// 4 w + q = pair (w / 2) * 4 + q, half w % 2, and the band cells of the
+// tiles' rows (odst; -1 where a cell is no stencil row)
 template <class TC>
 __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks, int ntv_b,
                           const int32_t *__restrict__ rowmap, long nrow, const TC *__restrict__ coef,
-                          TC *__restrict__ A, int32_t *__restrict__ rows) {
+                          const int32_t *__restrict__ odst, TC *__restrict__ A, int32_t *__restrict__ rows) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nblock * kMfNT * 64 * 64) return;
     const int l = (int)(g & 63), c = (int)((g >> 6) & 63);
@@ -1100,11 +1167,11 @@ __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks
     if (row >= 0 && ou >= 0 && ou < NR && ov >= 0 && ov < NR) a = coef[(long)(ou * NR + ov) * nrow + row];
     // steps in pairs (4 w + q, 4 (w + 1) + q), w even: one 16-byte load per two
     A[((bt * 32 + (c >> 3) * 4 + (c & 3)) * 64 + l) * 2 + ((c >> 2) & 1)] = a;
-    if (c == 0 && l < 16) rows[bt * 16 + i] = row;
+    if (c == 0 && l < 16) rows[bt * 16 + i] = row >= 0 ? odst[row] : -1;
 }
 
-// R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) on the matrix cores.
-// Workgroup = (block, chunk of GPW sample groups): one wave per tile of the
+// R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) on the matrix cores,
+// written to the rows' band cells out[s * ostride + cell].  Workgroup = (block, chunk of GPW sample groups): one wave per tile of the
 // block; the waves stage the block's window for a group of 8 samples in LDS
 // together, then each runs its tile's 64 matrix steps with the packed
 // matrix read from L2 as it goes (the chunks of one block run back to back
@@ -1155,7 +1222,7 @@ template <int GPW>
 __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma(Geo G, const int32_t *__restrict__ blocks, int ntv_b, int ngc,
                                                       const double *__restrict__ A, const int32_t *__restrict__ rows,
                                                       const Cplx<double> *__restrict__ spec, long sstride,
-                                                      Cplx<double> *__restrict__ rbuf, long nrow,
+                                                      Cplx<double> *__restrict__ outp, long ostride,
                                                       const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
     double *win = reinterpret_cast<double *>(smem);  // [kMfWU / 2][kMfR2]: cell pairs along u
@@ -1199,15 +1266,15 @@ __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma(Geo G, const int32_
         const int q = g * 8 + (j >> 1);
         if (q < nactive) {
             const int smp = active ? active[q] : q;
-            double *out = reinterpret_cast<double *>(rbuf) + (long)smp * nrow * 2;
+            double *out = reinterpret_cast<double *>(outp) + (long)smp * ostride * 2;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = orow[r];
-                if (row < 0) continue;
+                const int cell = orow[r];
+                if (cell < 0) continue;
                 const int i = (l >> 4) + 4 * r;
                 const int ru = bu * kMfBU + tu * kMfT + (i >> 2), rv = bv * kMfBV + tv * kMfT + (i & 3);
                 const double sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? -1.0 : 1.0;
-                out[(long)row * 2 + (j & 1)] = sg * acc[r];
+                out[(long)cell * 2 + (j & 1)] = sg * acc[r];
             }
         }
     }
@@ -1227,7 +1294,7 @@ __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma32(Geo G, const int3
                                                         int ngc, const float *__restrict__ A,
                                                         const int32_t *__restrict__ rows,
                                                         const Cplx<float> *__restrict__ spec, long sstride,
-                                                        Cplx<float> *__restrict__ rbuf, long nrow,
+                                                        Cplx<float> *__restrict__ outp, long ostride,
                                                         const int *__restrict__ active, int nactive, int nbv) {
     extern __shared__ unsigned char smem[];
     float *win = reinterpret_cast<float *>(smem);  // [kMfWU / 2][kMfR2]: cell pairs along u
@@ -1269,22 +1336,22 @@ __global__ void __launch_bounds__(64 * kMfNT) k_stencil_mfma32(Geo G, const int3
         const int q = g * 8 + (j >> 1);
         if (q < nactive) {
             const int smp = active ? active[q] : q;
-            float *out = reinterpret_cast<float *>(rbuf) + (long)smp * nrow * 2;
+            float *out = reinterpret_cast<float *>(outp) + (long)smp * ostride * 2;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = orow[r];
-                if (row < 0) continue;
+                const int cell = orow[r];
+                if (cell < 0) continue;
                 const int i = 4 * (l >> 4) + r;
                 const int ru = bu * kMfBU + tu * kMfT + (i >> 2), rv = bv * kMfBV + tv * kMfT + (i & 3);
                 const float sg = (((ru - G.cu) + (rv - G.cv)) & 1) ? -1.f : 1.f;
-                out[(long)row * 2 + (j & 1)] = sg * acc[r];
+                out[(long)cell * 2 + (j & 1)] = sg * acc[r];
             }
         }
     }
 }
 
-// final half-plane cells from their raw preimages (unfolded (a, iv) and the
-// folded mirror (-a, -iv)); column 0 doubled
+// the full-spectrum form: the half-plane spectrum from the folded rows'
+// compact outputs (the cells without a row are zero)
 template <class T>
 __global__ void k_stencil_fold(Geo G, const int32_t *__restrict__ rowmap, long nrow, const Cplx<T> *__restrict__ rbuf,
                                Cplx<T> *__restrict__ spec, long sstride, const int *__restrict__ active) {
@@ -1295,76 +1362,10 @@ __global__ void k_stencil_fold(Geo G, const int32_t *__restrict__ rowmap, long n
     const int a = iu <= G.bu ? iu : iu - G.nu;
     Cplx<T> tot{(T)0, (T)0};
     if (abs(a) <= G.bu && iv <= G.bv) {
-        const int r1 = rowmap[(long)(a + G.cu) * G.nv + (iv + G.cv)];
-        if (r1 >= 0) tot = rbuf[(long)smp * nrow + r1];
-        if (iv >= 1) {
-            const int r2 = rowmap[(long)(-a + G.cu) * G.nv + (-iv + G.cv)];
-            if (r2 >= 0) {
-                const Cplx<T> f = rbuf[(long)smp * nrow + r2];
-                tot.re += f.re;
-                tot.im -= f.im;
-            }
-        }
-        if (iv == 0) {
-            tot.re *= (T)2;
-            tot.im *= (T)2;
-        }
+        const int r = rowmap[(long)(a + G.cu) * G.nv + (iv + G.cv)];
+        if (r >= 0) tot = rbuf[(long)smp * nrow + r];
     }
     spec[(long)smp * sstride + g] = tot;
-}
-
-// the fold's cells: row v = 0 whole (the Hermitian step below writes any
-// of it), other rows where a stencil row or its mirror exists; r1 / r2 are
-// the stencil rows of (a, v) and of (-a, -v) (-1: none)
-__global__ void k_fold_cells(Geo G, const int32_t *__restrict__ rowmap, int nbv, int64_t *__restrict__ flag,
-                             int32_t *__restrict__ r1o, int32_t *__restrict__ r2o) {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long)nbv * G.nu) return;
-    const int iv = (int)(g / G.nu), iu = (int)(g - (long)iv * G.nu);
-    const int a = iu <= G.bu ? iu : iu - G.nu;
-    int r1 = -1, r2 = -1;
-    if (abs(a) <= G.bu && iv <= G.bv) {
-        r1 = rowmap[(long)(a + G.cu) * G.nv + (iv + G.cv)];
-        if (iv >= 1) r2 = rowmap[(long)(-a + G.cu) * G.nv + (-iv + G.cv)];
-    }
-    r1o[g] = r1;
-    r2o[g] = r2;
-    flag[g] = (iv == 0 || r1 >= 0 || r2 >= 0) ? 1 : 0;
-}
-
-__global__ void k_fold_compact(long n, const int64_t *__restrict__ pos, const int32_t *__restrict__ r1,
-                               const int32_t *__restrict__ r2, int32_t *__restrict__ cell, int32_t *__restrict__ o1,
-                               int32_t *__restrict__ o2) {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n || pos[g + 1] == pos[g]) return;
-    cell[pos[g]] = (int32_t)g;
-    o1[pos[g]] = r1[g];
-    o2[pos[g]] = r2[g];
-}
-
-// The fold into a band stored [v][u] (v < nbv) over the fold's cells only
-// (the rest of the band is zero and stays so): unfolded (a, v) plus the
-// conjugate of (-a, -v), row v = 0 doubled
-template <class T>
-__global__ void k_stencil_fold_list(long nfold, int nu, const int32_t *__restrict__ cell,
-                                    const int32_t *__restrict__ r1, const int32_t *__restrict__ r2, long nrow,
-                                    const Cplx<T> *__restrict__ rbuf, Cplx<T> *__restrict__ band, long bsz) {
-    const int slot = blockIdx.y;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nfold) return;
-    const int g = cell[i], a1 = r1[i], a2 = r2[i];
-    Cplx<T> tot{(T)0, (T)0};
-    if (a1 >= 0) tot = rbuf[(long)slot * nrow + a1];
-    if (a2 >= 0) {
-        const Cplx<T> f = rbuf[(long)slot * nrow + a2];
-        tot.re += f.re;
-        tot.im -= f.im;
-    }
-    if (g < nu) {  // row v = 0
-        tot.re *= (T)2;
-        tot.im *= (T)2;
-    }
-    band[(long)slot * bsz + g] = tot;
 }
 
 // Hermitian symmetrisation of band row v = 0 (what c2r assumes of column 0)
@@ -2705,7 +2706,7 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     const bool fused = fused_rows_ok(P, nbv);
     if (!fused && P.bandR.n < need_r) LT_TRY(P.bandR.alloc(need_r));
     if (!fused && P.bandC.n < need_c) LT_TRY(P.bandC.alloc(need_c));
-    // the band buffers the row transforms and the fold write hold zeros
+    // the band buffers the row transforms and the stencil write hold zeros
     // outside the cells they write (from allocation on); the column
     // transforms write their outputs to the plan's scratch bandB / bandF
     const int pi = prec == LT_F64 ? 0 : 1;
@@ -2720,7 +2721,6 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     }
     if (P.bandB.n < need_b) LT_TRY(P.bandB.alloc(need_b));
     if (P.bandF.n < need_b) LT_TRY(P.bandF.alloc(need_b));
-    LT_TRY(St.ensure_fold(P));
     T *R = (T *)P.bandR.p;
     Cplx<T> *Cb = (Cplx<T> *)P.bandC.p, *Bin = (Cplx<T> *)bin.p, *Bf = (Cplx<T> *)P.bandB.p,
             *Fin = (Cplx<T> *)bout.p, *Fo = (Cplx<T> *)P.bandF.p;
@@ -2743,16 +2743,13 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
         void *o[1] = {Bf};
         LT_TRY(fft_check(rocfft_execute(bf->plan[1], in, o, bf->info[1]), "execute band columns forward"));
     }
+    // the stencil writes its (folded) rows' band cells of Fin directly
     if (prec == LT_F64)
-        LT_TRY((stencil_apply_t<double, double>(St, P, St.coef.p, Bf, bsz, nullptr, nullptr, nactive, true)));
+        LT_TRY((stencil_apply_t<double, double>(St, P, St.coef.p, Bf, bsz, Fin, nullptr, nactive, true)));
     else
-        LT_TRY((stencil_apply_t<float, float>(St, P, St.coef32.p, Bf, bsz, nullptr, nullptr, nactive, true)));
-    if (St.nfold > 0) {
-        hipLaunchKernelGGL((k_stencil_fold_list<T>), dim3(blocks_for(St.nfold, 256), nactive), dim3(256), 0, stream(),
-                           St.nfold, P.nu, St.fold_cell.p, St.fold_r1.p, St.fold_r2.p, St.nrow,
-                           (const Cplx<T> *)St.rbuf.p, Fin, bsz);
-        LT_CHECK_LAUNCH();
-    }
+        LT_TRY((stencil_apply_t<float, float>(St, P, St.coef32.p, Bf, bsz, Fin, nullptr, nactive, true)));
+    // row v = 0 exactly Hermitian along u (its two halves are computed as
+    // separate rows)
     hipLaunchKernelGGL((k_herm_band0<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0, stream(), P.nu,
                        bsz, Fin);
     LT_CHECK_LAUNCH();
@@ -2773,37 +2770,6 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     hipLaunchKernelGGL((k_band_crop<T, TI>), dim3(blocks_for((long)P.nx * P.ny, 256), nactive), dim3(256), 0, stream(),
                        P.nx, P.ny, P.pady, P.nyp, R, P.apod.p, alpha, acc ? 1 : 0, active, out);
     LT_CHECK_LAUNCH();
-    return LT_OK;
-}
-
-lt_status NormalStencil::ensure_fold(NufftPlan &P) {
-    if (nfold >= 0) return LT_OK;
-    const Geo G = make_geo(P);
-    const long n = (long)nbv * P.nu;
-    DevBuf<int64_t> flag, pos;
-    DevBuf<int32_t> r1, r2;
-    LT_TRY(flag.alloc(n + 1));
-    LT_TRY(pos.alloc(n + 1));
-    LT_TRY(r1.alloc(std::max<long>(n, 1)));
-    LT_TRY(r2.alloc(std::max<long>(n, 1)));
-    if (n > 0) {
-        hipLaunchKernelGGL(k_fold_cells, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), G, rowmap.p, nbv, flag.p,
-                           r1.p, r2.p);
-        LT_CHECK_LAUNCH();
-    }
-    LT_TRY(exclusive_scan_i64(flag.p, pos.p, n));
-    int64_t cnt = 0;
-    LT_TRY(pos.download(&cnt, 1, n));
-    LT_TRY(fold_cell.alloc(std::max<long>(cnt, 1)));
-    LT_TRY(fold_r1.alloc(std::max<long>(cnt, 1)));
-    LT_TRY(fold_r2.alloc(std::max<long>(cnt, 1)));
-    if (n > 0) {
-        hipLaunchKernelGGL(k_fold_compact, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, pos.p, r1.p, r2.p,
-                           fold_cell.p, fold_r1.p, fold_r2.p);
-        LT_CHECK_LAUNCH();
-    }
-    LT_TRY(sync());
-    nfold = cnt;
     return LT_OK;
 }
 
@@ -3029,6 +2995,36 @@ lt_status NormalStencil::build(NufftPlan &P, const double *w_sorted) {
     }
 #undef LT_COEF_CASE
     LT_CHECK_LAUNCH();
+    // Hermitian fold of the rows (k_fold_mark): half the rows, and the
+    // stencil's outputs are the final band cells
+    {
+        DevBuf<int32_t> rowmap_raw = std::move(rowmap);
+        DevBuf<double> coef_raw = std::move(coef);
+        nrow_raw = nrow;
+        hipLaunchKernelGGL(k_fold_mark, dim3(blocks_for(ng, 256)), dim3(256), 0, stream(), G, rowmap_raw.p, mark.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(exclusive_scan_i64(mark.p, off.p, ng));
+        int64_t nf = 0;
+        LT_TRY(off.download(&nf, 1, ng));
+        LT_TRY(sync());
+        nrow = nf;
+        DevBuf<int32_t> r1, r2;
+        LT_TRY(rowmap.alloc(ng));
+        LT_TRY(row_cell.alloc(std::max<long>(nrow, 1)));
+        LT_TRY(row_dst.alloc(std::max<long>(nrow, 1)));
+        LT_TRY(r1.alloc(std::max<long>(nrow, 1)));
+        LT_TRY(r2.alloc(std::max<long>(nrow, 1)));
+        hipLaunchKernelGGL(k_fold_rows, dim3(blocks_for(ng, 256)), dim3(256), 0, stream(), G, off.p, rowmap_raw.p,
+                           rowmap.p, row_cell.p, r1.p, r2.p, row_dst.p);
+        LT_CHECK_LAUNCH();
+        LT_TRY(coef.alloc(std::max<long>((long)NO * nrow, 1)));
+        if (nrow > 0) {
+            hipLaunchKernelGGL(k_fold_coef, dim3(blocks_for((long)NO * nrow, 256)), dim3(256), 0, stream(), nrow, NO,
+                               r1.p, r2.p, nrow_raw, coef_raw.p, coef.p);
+            LT_CHECK_LAUNCH();
+        }
+        LT_TRY(sync());
+    }
     // stencil tiles
     const int ntu_st = (P.nu + ST - 1) / ST, ntv_st = (P.nv + ST - 1) / ST;
     const long nts = (long)ntu_st * ntv_st;
@@ -3092,7 +3088,7 @@ lt_status NormalStencil::mfma_build(NufftPlan &P) {
     const long npk = nb * kMfNT * 64 * 64;
     if (npk > 0) {
         hipLaunchKernelGGL(k_mf_pack<double>, dim3(blocks_for(npk, 256)), dim3(256), 0, stream(), G, (long)nb, mf_blocks.p,
-                           mf_ntv, rowmap.p, nrow, coef.p, mf_A.p, mf_rows.p);
+                           mf_ntv, rowmap.p, nrow, coef.p, row_dst.p, mf_A.p, mf_rows.p);
         LT_CHECK_LAUNCH();
     }
     mf_nblock = nb;
@@ -3112,20 +3108,26 @@ lt_status NormalStencil::mfma_build32(NufftPlan &P) {
     const long npk = nb * kMfNT * 64 * 64;
     if (npk > 0) {
         hipLaunchKernelGGL(k_mf_pack<float>, dim3(blocks_for(npk, 256)), dim3(256), 0, stream(), G, nb, mf_blocks.p,
-                           mf_ntv, rowmap.p, nrow, coef32.p, mf_A32.p, mf_rows.p);
+                           mf_ntv, rowmap.p, nrow, coef32.p, row_dst.p, mf_A32.p, mf_rows.p);
         LT_CHECK_LAUNCH();
     }
     return sync();
 }
 
-// band: spec_in is a [v][u] band of nbv rows (the stencil's rows only); the
-// fold is then left to the caller (spec_out unused)
+// band: spec_in is a [v][u] band of nbv rows and the stencil writes its
+// rows' cells of the band spec_out (same stride); otherwise the rows go to
+// the compact St.rbuf and k_stencil_fold makes the half-plane spectrum
 template <class T, class TC>
 static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef, const void *spec_in, long sstride,
                                  void *spec_out, const int *active, int nactive, bool band = false) {
     Geo G = make_geo(P);
-    const size_t need = (size_t)std::max(P.batch, 1) * St.nrow * sizeof(Cplx<double>);
-    if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
+    if (!band) {
+        const size_t need = (size_t)std::max(P.batch, 1) * St.nrow * sizeof(Cplx<double>);
+        if (St.rbuf.n < need) LT_TRY(St.rbuf.alloc(need));
+    }
+    Cplx<T> *ob = band ? (Cplx<T> *)spec_out : (Cplx<T> *)St.rbuf.p;
+    const int32_t *odst = band ? St.row_dst.p : nullptr;
+    const long ostride = band ? sstride : St.nrow;
     const int WS = St.ST + 2 * St.R;
     const int ntv_st = (P.nv + St.ST - 1) / St.ST;
     const bool f32 = std::is_same<T, float>::value && std::is_same<TC, float>::value;
@@ -3141,7 +3143,7 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
 #define LT_STENCIL_MF32(GG)                                                                                       \
     hipLaunchKernelGGL((k_stencil_mfma32<GG>), grid, dim3(64 * kMfNT), sh, stream(), G, St.mf_blocks.p,          \
                        St.mf_ntv, ngc, St.mf_A32.p, St.mf_rows.p, (const Cplx<float> *)spec_in, sstride,         \
-                       (Cplx<float> *)St.rbuf.p, St.nrow, active, nactive, St.nbv);
+                       (Cplx<float> *)ob, ostride, active, nactive, St.nbv);
         if (St.mf_nblock > 0) {
             if (gpw == 1)
                 LT_STENCIL_MF32(1)
@@ -3168,8 +3170,8 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
 #define LT_STENCIL_MF(GG)                                                                                          \
     hipLaunchKernelGGL((k_stencil_mfma<GG>), grid, dim3(64 * kMfNT), sh, stream(), G, St.mf_blocks.p, St.mf_ntv,   \
                        ngc,                                                                                         \
-                       St.mf_A.p, St.mf_rows.p, (const Cplx<double> *)spec_in, sstride, (Cplx<double> *)St.rbuf.p, \
-                       St.nrow, active, nactive, St.nbv);
+                       St.mf_A.p, St.mf_rows.p, (const Cplx<double> *)spec_in, sstride, (Cplx<double> *)ob,       \
+                       ostride, active, nactive, St.nbv);
         if (St.mf_nblock > 0) {
             if (gpw == 1)
                 LT_STENCIL_MF(1)
@@ -3194,15 +3196,15 @@ static lt_status stencil_apply_t(NormalStencil &St, NufftPlan &P, const TC *coef
         if (band && St.R == 6)                                                                                    \
             hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, true, 6>), grid, dim3(St.ST * St.ST), sh, stream(), G, \
                                St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
-                               (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
+                               (const Cplx<T> *)spec_in, sstride, ob, odst, ostride, active, nactive, St.nbv); \
         else if (band)                                                                                            \
             hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, true>), grid, dim3(St.ST * St.ST), sh, stream(), G,   \
                                St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
-                               (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
+                               (const Cplx<T> *)spec_in, sstride, ob, odst, ostride, active, nactive, St.nbv); \
         else                                                                                                      \
             hipLaunchKernelGGL((k_stencil_apply<T, NBV, TC, false>), grid, dim3(St.ST * St.ST), sh, stream(), G,  \
                                St.R, St.ST, ntv_st, St.tiles.p, ngroup, St.rowmap.p, St.nrow, coef,               \
-                               (const Cplx<T> *)spec_in, sstride, (Cplx<T> *)St.rbuf.p, active, nactive, St.nbv); \
+                               (const Cplx<T> *)spec_in, sstride, ob, odst, ostride, active, nactive, St.nbv); \
     }
         if (nb_sel == 1)
             LT_STENCIL_NB(1)
@@ -3289,8 +3291,9 @@ __global__ void k_gather_weights(long n, const int32_t *__restrict__ perm, const
 // Times the normal stencil's kernel as the solver launches it (band form,
 // the plan's resident stencil, nb samples, the tuned samples per workgroup)
 // with HIP events on the library stream: mean duration per launch over reps
-// launches.  Algorithmic work per launch: every stencil row's NO FP64
-// coefficients once (8 B each), and per sample its output (16 B per row) and
+// launches.  Algorithmic work per launch: every stencil row's (the folded
+// rows: half-plane cells) NO FP64 coefficients once (8 B each), and per
+// sample its output (16 B per row) and
 // the distinct complex band values its windows read (16 B each: raw cells
 // within R of a row, a cell and its Hermitian mirror counted once); 4 flops
 // (a complex value times a real coefficient, added) per row, offset and
@@ -3326,10 +3329,10 @@ lt_status stencil_time(NufftPlan &P, int nb, int reps, double *ms, double *bytes
     hipEvent_t e0, e1;
     LT_HIP(hipEventCreate(&e0));
     LT_HIP(hipEventCreate(&e1));
-    LT_TRY((stencil_apply_t<double, double>(*St, P, St->coef.p, P.bandB.p, bsz, nullptr, nullptr, nb, true)));
+    LT_TRY((stencil_apply_t<double, double>(*St, P, St->coef.p, P.bandB.p, bsz, St->band_out[0].p, nullptr, nb, true)));
     LT_HIP(hipEventRecord(e0, stream()));
     for (int r = 0; r < reps; ++r)
-        LT_TRY((stencil_apply_t<double, double>(*St, P, St->coef.p, P.bandB.p, bsz, nullptr, nullptr, nb, true)));
+        LT_TRY((stencil_apply_t<double, double>(*St, P, St->coef.p, P.bandB.p, bsz, St->band_out[0].p, nullptr, nb, true)));
     LT_HIP(hipEventRecord(e1, stream()));
     LT_HIP(hipEventSynchronize(e1));
     float t = 0.f;

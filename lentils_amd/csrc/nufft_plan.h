@@ -55,15 +55,23 @@ double kb_entire(double beta, int W, double x);
 //   R[r'] = sum_o coef[o][r'] S[r' + o],   o in [-2W, 2W]^2,
 //   coef[o][r'] = 0.5 / (4W^2)^2 sum_v w_v k_v(r') k_v(r' + o),
 // which replaces the per-visibility work inside every solver iteration by
-// a stencil over the ~(active cells) x (4W+1)^2 coefficients.
+// a stencil over the ~(active cells) x (4W+1)^2 coefficients.  The rows are
+// folded onto the half-plane of the r2c spectrum at build time (nrow below).
 struct NormalStencil {
     int W = 0, R = 0, NO = 0;  // half-width, stencil radius 2W, (4W+1)^2 offsets
     long ncell = 0;            // grid cells holding at least one visibility
-    long nrow = 0;             // raw cells within W of those (stencil rows)
+    // stencil rows: the half-plane cells (a, b >= 0) of the final spectrum
+    // where a raw cell within W of a visibility cell lies at (a, b) or at
+    // its mirror (-a, -b); each row's coefficients are those of the raw
+    // cell plus the mirror's reflected (the Hermitian fold done once, at
+    // build time)
+    long nrow = 0;
+    long nrow_raw = 0;         // raw cells within W of the visibility cells (both half-planes)
     int ST = 16;               // stencil tile edge
     long ntile = 0;
     DevBuf<int32_t> rowmap;    // centred raw cell -> row or -1 (nu * nv)
     DevBuf<int32_t> row_cell;  // row -> centred raw cell index
+    DevBuf<int32_t> row_dst;   // row -> its cell in a [v][u] band (v * nu + u)
     DevBuf<int32_t> tiles;     // stencil tiles holding rows
     DevBuf<double> coef;       // [NO][nrow]
     DevBuf<float> coef32;      // FP32 copy / coef32_scale for the precision form (made on first use)
@@ -72,15 +80,10 @@ struct NormalStencil {
     bool use_band = true;      // band-pruned separable transform instead of the full 2-D one
     int nb_sel = 4;            // samples per stencil workgroup (stencil_autotune); < 0: the matrix-core
                                // form with -nb_sel groups of 8 samples per workgroup
-    DevBuf<uint8_t> rbuf;      // per-sample stencil output, [batch][nrow] complex T
-    // band form: the [v][u] band cells the fold can make nonzero (row v = 0
-    // whole, the others where a stencil row or its mirror exists) with
-    // their two stencil rows (-1: none); every other cell stays zero
-    long nfold = -1;
-    DevBuf<int32_t> fold_cell, fold_r1, fold_r2;
-    // band spectra of the row transforms (band_in) and of the fold
-    // (band_out), per precision: written only on the cells above and the
-    // image rows, zero elsewhere from allocation on
+    DevBuf<uint8_t> rbuf;      // per-sample stencil output of the full-spectrum form, [batch][nrow] complex T
+    // band spectra of the row transforms (band_in) and of the stencil
+    // (band_out, written at the rows' cells), per precision: written only
+    // there and on the image rows, zero elsewhere from allocation on
     DevBuf<uint8_t> band_in[2], band_out[2];
 
     // matrix-core form of the FP64 band stencil (radius 6): 8 x 8 blocks of
@@ -96,7 +99,6 @@ struct NormalStencil {
     lt_status build(NufftPlan &P, const double *w_sorted);
     lt_status mfma_build(NufftPlan &P);
     lt_status mfma_build32(NufftPlan &P);
-    lt_status ensure_fold(NufftPlan &P);  // fold_* (band form)
     lt_status apply(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active, int nactive);
     lt_status apply32(NufftPlan &P, const void *spec_in, long sstride, void *spec_out, const int *active,
                       int nactive);  // FP32 spectra
