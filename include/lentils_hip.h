@@ -218,13 +218,14 @@ lt_status lt_fft_c2r_host(int nx, int ny, int batch, const double *in, double *o
  * explicit matrix, operator_base.py:96-98). */
 lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                            const double *x, double *y);
-/* Same with the device kernel chosen: 0 the default, 1..64 (powers of two)
- * the lane-group kernel with that many lanes per row, 128 the streaming
- * (LDS-staged) kernel, 256 one thread per row over a sliced copy (64-row
- * slices, entries interleaved), -1..-64 the solver's L x: rows of 1..4 entries from
- * a padded 4-wide table, rows of 5..64 by that many lanes per row, longer
- * rows in 2048-entry chunks, rows without entries 0.  Every choice sums
- * each row in a fixed order. */
+/* Same with the device kernel chosen, for rows of up to 64 entries: 0 the
+ * default, 1..64 (powers of two) the lane-group kernel with that many lanes
+ * per row, 128 the streaming (LDS-staged) kernel, 256 one thread per row
+ * over a sliced copy (64-row slices, entries interleaved), -1..-64 the
+ * solver's L x: rows of 1..4 entries from a padded 4-wide table, rows of
+ * 5..64 by that many lanes per row, rows without entries 0.  Longer rows
+ * as in the solver's products: 65..512 entries one wave per row, longer
+ * ones in 2048-entry chunks.  Every choice sums each row in a fixed order. */
 lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                                   const double *x, double *y, int kernel);
 
@@ -304,6 +305,12 @@ lt_status lt_nufft_set_stencil_shape(lt_nufft h, int samples_per_group);
  * 2W picked by the offset's sign.  Both agree to the precision's rounding;
  * visibilities exactly on a grid line take the 2W+1 taps either way. */
 lt_status lt_nufft_set_degrid_form(lt_nufft h, int form);
+/* No reference counterpart: the band row transforms of padded rows of 1024
+ * for this plan: 1 (default) = FP64 rows by a radix-4 transform whose first
+ * and last passes run in registers (the same bits as form 0), FP32 rows by
+ * a one-wave radix-16 transform (FP32 rounding apart); 0 = the 256-thread
+ * radix-4 transform staged in LDS for both. */
+lt_status lt_nufft_set_row_form(lt_nufft h, int form);
 /* No reference counterpart: times the resident normal stencil's kernel as
  * the solver launches it for nb samples (band form, tuned shape) with HIP
  * events on the library stream; mean per-launch time, algorithmic bytes

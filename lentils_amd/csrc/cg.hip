@@ -1863,6 +1863,16 @@ lt_status lt_nufft_set_degrid_form(lt_nufft nfh, int form) {
     return LT_OK;
 }
 
+lt_status lt_nufft_set_row_form(lt_nufft nfh, int form) {
+    NufftPlan *nf = nufft_of(nfh);
+    if (!nf || (form != 0 && form != 1)) {
+        set_error("lt_nufft_set_row_form: 0 (256-thread LDS radix-4 form) or 1 (register-fed / one-wave forms)");
+        return LT_ERR_ARG;
+    }
+    nf->wave_rows = form == 1;
+    return LT_OK;
+}
+
 lt_status lt_device_alloc(long nbytes, void **dptr) {
     if (!dptr || nbytes < 0) return LT_ERR_ARG;
     *dptr = nullptr;

@@ -133,8 +133,11 @@ struct DevCSR {
     DevBuf<int32_t> ci;  // global column indices
     DevBuf<double> va;
     // long-row plan (csr_prepare): rows above long_row nonzeros are summed
-    // in fixed-size chunks by whole workgroups, then combined in order
+    // by one wave each up to 512 entries (med_rows), longer ones in
+    // fixed-size chunks by whole workgroups, then combined in order
     int long_row = 2048;
+    long nmed = 0;
+    DevBuf<int32_t> med_rows;
     bool prepared = false;
     int lanes = 1;  // lane-group width of k_spmv, kLanesStream (streaming kernel) or kLanesSlice (sliced copy)
     long nblk = 0;        // streaming-kernel workgroups (ceil(nnz / its nonzero range))

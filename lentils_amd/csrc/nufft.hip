@@ -1497,6 +1497,15 @@ __device__ inline void dft4(Cplx<T> (&v)[4]) {
     }
 }
 
+// a times the twiddle w (its conjugate for the inverse), each part one
+// explicit fma over one rounded product, so that every transform using it
+// rounds the same way whatever the compiler's contraction choices
+template <class T, bool INV>
+__device__ inline Cplx<T> tw_mul(Cplx<T> a, Cplx<T> w) {
+    const T wi = INV ? -w.im : w.im;
+    return Cplx<T>{fma(a.re, w.re, -(a.im * wi)), fma(a.re, wi, a.im * w.re)};
+}
+
 // one Stockham stage of radix R over PAIRS sequences of length N in buf
 template <class T, int N, int R, int PAIRS, bool INV>
 __device__ inline void stockham_stage(Cplx<T> (*buf)[N], int Ns, const Cplx<T> *__restrict__ twd) {
@@ -1511,10 +1520,7 @@ __device__ inline void stockham_stage(Cplx<T> (*buf)[N], int Ns, const Cplx<T> *
             const int step = N / (Ns * R);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const Cplx<T> a = buf[pr][j + r * (N / R)];
-                const Cplx<T> w = twd[k * r * step];
-                const T wi = INV ? -w.im : w.im;
-                v[q][r] = Cplx<T>{a.re * w.re - a.im * wi, a.re * wi + a.im * w.re};
+                v[q][r] = tw_mul<T, INV>(buf[pr][j + r * (N / R)], twd[k * r * step]);
             }
         }
     }
@@ -1650,6 +1656,186 @@ __global__ void __launch_bounds__(256) k_rows_inv(int nx, int ny, int pady, int 
         const long g = (long)row * ny + j;
         const TO val = alpha * ((TO)((rr & 1) ? z.im : z.re) * (TO)apod[g]);
         o[g] = accumulate ? o[g] + val : val;
+    }
+}
+
+// FP64 row transforms of length 1024 with the first and the last radix-4
+// pass in registers: one row pair per 256-thread workgroup, thread j doing
+// butterfly j of each of the five Stockham passes (the same butterflies,
+// twiddles and operation order as k_rows_fwd / k_rows_inv, so the same
+// bits).  The forward transform feeds its first pass from the image in
+// global memory and writes back only the outputs the band needs; the
+// inverse builds its first pass's inputs from the band's Hermitian
+// extension in global memory and writes the image from its last pass.  The
+// LDS exchanges between passes use 16-byte units swizzled within blocks of
+// 16 (low 4 bits XORed with a function of the next two), which makes the
+// writes after the passes of span 1 and 4 free of bank conflicts; reads of
+// 16 consecutive units stay conflict-free under any such swizzle.
+constexpr int kR4N = 1024;
+template <int SW>
+__device__ inline int r4swz(int i) {
+    return SW == 1 ? (i ^ ((i >> 4) & 3)) : SW == 2 ? (i ^ (((i >> 4) & 3) << 2)) : i;
+}
+
+// a middle pass (span NS = 4, 16, 64): read with swizzle SI, write with SO
+template <bool INV, int NS, int SI, int SO>
+__device__ inline void r4_pass(Cplx<double> *buf, const Cplx<double> *__restrict__ twd) {
+    const int j = threadIdx.x, k = j & (NS - 1);
+    constexpr int step = kR4N / (NS * 4);
+    Cplx<double> w[3];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) w[r - 1] = twd[k * r * step];
+    __syncthreads();  // the previous pass's writes
+    Cplx<double> v[4];
+    v[0] = buf[r4swz<SI>(j)];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) v[r] = tw_mul<double, INV>(buf[r4swz<SI>(j + 256 * r)], w[r - 1]);
+    __syncthreads();  // every read before the writes
+    dft4<double, INV>(v);
+    const int d = (j / NS) * NS * 4 + k;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[r4swz<SO>(d + r * NS)] = v[r];
+}
+
+// the last pass (span 256): its four outputs j + 256 r in v
+template <bool INV>
+__device__ inline void r4_last(Cplx<double> (&v)[4], const Cplx<double> *buf, const Cplx<double> *__restrict__ twd) {
+    const int j = threadIdx.x;
+    Cplx<double> w[3];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) w[r - 1] = twd[j * r];
+    __syncthreads();
+    v[0] = buf[j];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) v[r] = tw_mul<double, INV>(buf[j + 256 * r], w[r - 1]);
+    dft4<double, INV>(v);
+}
+
+template <class TI>
+__global__ void __launch_bounds__(256) k_rows_fwd_r(int nx, int ny, int pady, int padx, int nu, int nbv,
+                                                    const TI *__restrict__ img, const double *__restrict__ apod,
+                                                    const int *__restrict__ active, const uint8_t *__restrict__ rowmask,
+                                                    const Cplx<double> *__restrict__ twd,
+                                                    Cplx<double> *__restrict__ band) {
+    constexpr int N = kR4N;
+    __shared__ Cplx<double> buf[N];
+    // row pairs in XCD order: the four pairs whose band values share a
+    // 128-byte line run on one XCD, one after another
+    const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
+    const int slot = (int)(lb / gridDim.x);
+    const int smp = active ? active[slot] : slot;
+    const int row0 = (int)(lb - (long)slot * gridDim.x) * 2, ia = row0, ib = row0 + 1;
+    const TI *im = img + (long)smp * nx * ny;
+    const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
+    const bool ua = ia < nx && (!rm || rm[ia]), ub = ib < nx && (!rm || rm[ib]);
+    Cplx<double> *B = band + (long)slot * nbv * nu;
+    if (!ua && !ub) {
+        for (int q = threadIdx.x; q < nbv * 2; q += 256) {
+            const int v = q >> 1, rr = q & 1;
+            if (row0 + rr < nx) B[(long)v * nu + padx + row0 + rr] = Cplx<double>{0.0, 0.0};
+        }
+        return;
+    }
+    const int j = threadIdx.x;
+    // pass 1 (span 1, unit twiddles) from the apodized rows x[j + 256 r]
+    Cplx<double> v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int c = j + 256 * r - pady;
+        double re = 0.0, imv = 0.0;
+        if (c >= 0 && c < ny) {
+            if (ua) re = (double)(im[(long)ia * ny + c] * (TI)apod[(long)ia * ny + c]);
+            if (ub) imv = (double)(im[(long)ib * ny + c] * (TI)apod[(long)ib * ny + c]);
+        }
+        v[r] = Cplx<double>{re, imv};
+    }
+    dft4<double, false>(v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
+    r4_pass<false, 4, 1, 2>(buf, twd);
+    r4_pass<false, 16, 2, 0>(buf, twd);
+    r4_pass<false, 64, 0, 0>(buf, twd);
+    r4_last<false>(v, buf, twd);
+    __syncthreads();  // every read of the last pass before its writes
+    // the outputs the band needs: z[o] for o < nbv and o > N - nbv
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int o = j + 256 * r;
+        if (o < nbv || N - o < nbv) buf[o] = v[r];
+    }
+    __syncthreads();
+    const double h = 0.5;
+    for (int q = threadIdx.x; q < nbv * 2; q += 256) {
+        const int vv = q >> 1, rr = q & 1;
+        if (row0 + rr >= nx) continue;
+        const Cplx<double> z = buf[vv], zm = buf[(N - vv) & (N - 1)];
+        const Cplx<double> x = rr ? Cplx<double>{h * (z.im + zm.im), h * (zm.re - z.re)}
+                                  : Cplx<double>{h * (z.re + zm.re), h * (z.im - zm.im)};
+        B[(long)vv * nu + padx + row0 + rr] = x;
+    }
+}
+
+template <class TO>
+__global__ void __launch_bounds__(256) k_rows_inv_r(int nx, int ny, int pady, int padx, int nu, int nbv,
+                                                    const Cplx<double> *__restrict__ band,
+                                                    const double *__restrict__ apod, TO alpha, int accumulate,
+                                                    const int *__restrict__ active,
+                                                    const uint8_t *__restrict__ rowmask,
+                                                    const Cplx<double> *__restrict__ twd, TO *__restrict__ out) {
+    constexpr int N = kR4N, H = N / 2;
+    __shared__ Cplx<double> buf[N];
+    // row pairs in XCD order: the four pairs whose band values share a
+    // 128-byte line run on one XCD, one after another
+    const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
+    const int slot = (int)(lb / gridDim.x);
+    const int smp = active ? active[slot] : slot;
+    const int row0 = (int)(lb - (long)slot * gridDim.x) * 2, ia = row0, ib = row0 + 1;
+    const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
+    const bool ua = ia < nx && (!rm || rm[ia]), ub = ib < nx && (!rm || rm[ib]);
+    if (!ua && !ub) return;  // no output row needed
+    const Cplx<double> *B = band + (long)slot * nbv * nu;
+    const int j = threadIdx.x;
+    // pass 1 inputs: z[k] = A[k] + i Bb[k] of the band rows' Hermitian
+    // extension (conjugates above H; imaginary parts of columns 0 and H
+    // dropped; rows not needed taken as zero)
+    Cplx<double> v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = j + 256 * r;
+        const int vv = k <= H ? k : N - k;
+        Cplx<double> ca{0.0, 0.0}, cb{0.0, 0.0};
+        if (vv < nbv) {
+            if (ua) ca = B[(long)vv * nu + padx + ia];
+            if (ub) cb = B[(long)vv * nu + padx + ib];
+        }
+        if (vv == 0 || vv == H) {
+            ca.im = 0.0;
+            cb.im = 0.0;
+        }
+        v[r] = k <= H ? Cplx<double>{ca.re - cb.im, ca.im + cb.re} : Cplx<double>{ca.re + cb.im, -ca.im + cb.re};
+    }
+    dft4<double, true>(v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
+    r4_pass<true, 4, 1, 2>(buf, twd);
+    r4_pass<true, 16, 2, 0>(buf, twd);
+    r4_pass<true, 64, 0, 0>(buf, twd);
+    r4_last<true>(v, buf, twd);
+    TO *o = out + (long)smp * nx * ny;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int c = j + 256 * r - pady;
+        if (c < 0 || c >= ny) continue;
+        if (ua) {
+            const long g = (long)ia * ny + c;
+            const TO val = alpha * ((TO)v[r].re * (TO)apod[g]);
+            o[g] = accumulate ? o[g] + val : val;
+        }
+        if (ub) {
+            const long g = (long)ib * ny + c;
+            const TO val = alpha * ((TO)v[r].im * (TO)apod[g]);
+            o[g] = accumulate ? o[g] + val : val;
+        }
     }
 }
 
@@ -2624,6 +2810,14 @@ static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *
         LT_CHECK_LAUNCH();
         return LT_OK;
     }
+    if constexpr (sizeof(T) == 8) {
+        if (P.nyp == kR4N && P.wave_rows) {
+            hipLaunchKernelGGL((k_rows_fwd_r<TI>), dim3((P.nx + 1) / 2, nactive), dim3(256), 0, stream(), P.nx, P.ny,
+                               P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, rowmask, tw, B);
+            LT_CHECK_LAUNCH();
+            return LT_OK;
+        }
+    }
 #define LT_ROWS_FWD(NN)                                                                                            \
     case NN: {                                                                                                     \
         constexpr int PR = rows_pairs<T, NN>();                                                                    \
@@ -2657,6 +2851,14 @@ static lt_status rows_inverse(NufftPlan &P, int nbv, const Cplx<T> *F, const int
                            P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0, active, rowmask, tw, out);
         LT_CHECK_LAUNCH();
         return LT_OK;
+    }
+    if constexpr (sizeof(T) == 8) {
+        if (P.nyp == kR4N && P.wave_rows) {
+            hipLaunchKernelGGL((k_rows_inv_r<TO>), dim3((P.nx + 1) / 2, nactive), dim3(256), 0, stream(), P.nx, P.ny,
+                               P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0, active, rowmask, tw, out);
+            LT_CHECK_LAUNCH();
+            return LT_OK;
+        }
     }
 #define LT_ROWS_INV(NN)                                                                                            \
     case NN: {                                                                                                     \

@@ -25,6 +25,9 @@ namespace lt {
 namespace {
 
 constexpr int kChunk = 2048;
+// rows of long_row + 1 .. kMedRow entries (when long_row < kMedRow): one
+// wave each (k_spmv_med); longer ones in kChunk chunks
+constexpr int kMedRow = 512;
 
 // Rows of a block-diagonal batch belong to samples: sample = row / rps, or
 // rowseg[row] when given.  With act set, rows of inactive samples are
@@ -380,6 +383,39 @@ __global__ void __launch_bounds__(256) k_spmv_chunks(long nchunk, const int32_t 
     chunk_sum<T>(chunk_beg[c], chunk_end_row[c], rp, ci, va, x, out + c);
 }
 
+// medium rows (long_row < length <= kMedRow): one wave per row, four rows
+// per workgroup; each lane's entries b + lane + 64 i loaded together, summed
+// in increasing i, then a butterfly sum across the wave read at lane 0 (a
+// fixed order)
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_med(long nmed, const int32_t *__restrict__ rows,
+                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                   const T *__restrict__ va, const T *__restrict__ x,
+                                                   T *__restrict__ y, T alpha, T beta, RowFilter F) {
+    const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (w >= nmed) return;  // whole waves
+    const int row = rows[w];
+    if (F.skip(row)) return;
+    const int b = rp[row], e = rp[row + 1];
+    constexpr int PER = kMedRow / 64;
+    int c[PER];
+    T v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int k = b + l + 64 * i;
+        c[i] = k < e ? ci[k] : -1;
+        v[i] = k < e ? va[k] : (T)0;
+    }
+    T s = (T)0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (c[i] >= 0) s += v[i] * x[c[i]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (l == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
+}
+
 // long rows: sum their chunks in order
 template <class T>
 __global__ void k_spmv_long(long nlong, const int32_t *__restrict__ rows, const int32_t *__restrict__ chunk0,
@@ -398,13 +434,20 @@ __global__ void k_d2f_sp(long n, const double *__restrict__ a, float *__restrict
 }
 
 __global__ void k_long_rows_flag(long n, const int32_t *__restrict__ rp, int long_row, int64_t *__restrict__ flag,
-                                 int64_t *__restrict__ nch) {
+                                 int64_t *__restrict__ nch, int64_t *__restrict__ mflag) {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int len = rp[r + 1] - rp[r];
-    const bool lg = len > long_row;
+    const bool md = len > long_row && len <= kMedRow;
+    const bool lg = len > long_row && !md;
     flag[r] = lg ? 1 : 0;
     nch[r] = lg ? (len + kChunk - 1) / kChunk : 0;
+    mflag[r] = md ? 1 : 0;
+}
+
+__global__ void k_flag_list(long n, const int64_t *__restrict__ pos, int32_t *__restrict__ out) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n && pos[r + 1] != pos[r]) out[pos[r]] = (int32_t)r;
 }
 
 __global__ void k_long_rows_fill(long n, const int32_t *__restrict__ rp, const int64_t *__restrict__ pos,
@@ -501,26 +544,36 @@ int csr_pick_lanes(const DevCSR &A) {
 lt_status csr_prepare(DevCSR &A) {
     int32_t last = 0;  // host source of an async copy: lives until the final sync
     A.lanes = csr_pick_lanes(A);
-    A.nlong = A.nchunk = 0;
+    A.nlong = A.nchunk = A.nmed = 0;
     if (A.nrows > 0) {
         const long n = A.nrows;
-        DevBuf<int64_t> flag, pos, nch, choff;
+        DevBuf<int64_t> flag, pos, nch, choff, mflag, mpos;
         LT_TRY(flag.alloc(n + 1));
         LT_TRY(pos.alloc(n + 1));
         LT_TRY(nch.alloc(n + 1));
         LT_TRY(choff.alloc(n + 1));
+        LT_TRY(mflag.alloc(n + 1));
+        LT_TRY(mpos.alloc(n + 1));
         hipLaunchKernelGGL(k_long_rows_flag, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, A.rp.p, A.long_row,
-                           flag.p,
-                           nch.p);
+                           flag.p, nch.p, mflag.p);
         LT_CHECK_LAUNCH();
         LT_TRY(exclusive_scan_i64(flag.p, pos.p, n));
         LT_TRY(exclusive_scan_i64(nch.p, choff.p, n));
-        int64_t nl = 0, nc = 0;
+        LT_TRY(exclusive_scan_i64(mflag.p, mpos.p, n));
+        int64_t nl = 0, nc = 0, nm = 0;
         LT_TRY(pos.download(&nl, 1, n));
         LT_TRY(choff.download(&nc, 1, n));
+        LT_TRY(mpos.download(&nm, 1, n));
         LT_TRY(sync());
         A.nlong = nl;
         A.nchunk = nc;
+        A.nmed = nm;
+        if (nm > 0) {
+            LT_TRY(A.med_rows.alloc(nm));
+            hipLaunchKernelGGL(k_flag_list, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), n, mpos.p,
+                               A.med_rows.p);
+            LT_CHECK_LAUNCH();
+        }
         if (nl > 0) {
             DevBuf<int32_t> chunk_row;
             LT_TRY(A.long_rows.alloc(nl));
@@ -581,7 +634,8 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
     const bool sg = alist && A.ns > 0 && nact < A.ns;
     if (sg && nact <= 0) return LT_OK;
     if (lanes <= 0) lanes = A.prepared ? A.lanes : csr_pick_lanes(A);
-    const int skip = (A.prepared && A.nlong > 0) ? A.long_row : 0;  // row length above which the chunked path sums
+    // row length above which the medium-row and chunked paths sum
+    const int skip = (A.prepared && (A.nlong > 0 || A.nmed > 0)) ? A.long_row : 0;
     if (nz_only) {
         if (A.nnzr < 0 || !A.prepared || alpha != (T)1 || beta_y != (T)0 || lanes > 64) {
             set_error("spmv: nonempty-row product needs csr_nonempty_rows, alpha 1, beta 0, lanes <= 64");
@@ -657,7 +711,12 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
             }
         }
     }
-    if (skip) {
+    if (skip && A.nmed > 0) {
+        hipLaunchKernelGGL((k_spmv_med<T>), dim3((unsigned)((A.nmed + 3) / 4)), dim3(256), 0, stream(), A.nmed,
+                           A.med_rows.p, A.rp.p, A.ci.p, va, x, y, alpha, beta_y, F);
+        LT_CHECK_LAUNCH();
+    }
+    if (skip && A.nlong > 0) {
         hipLaunchKernelGGL((k_spmv_chunks<T>), dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk,
                            A.chunk_beg.p, A.long_rows_of_chunk.p, A.rp.p, A.ci.p, va, x, chunk_sum, F);
         LT_CHECK_LAUNCH();
@@ -672,7 +731,7 @@ lt_status csr_make_slices(DevCSR &A) {
     if (A.nslice > 0 || A.nrows == 0 || A.nnz == 0) return LT_OK;
     if (!A.prepared) LT_TRY(csr_prepare(A));
     const long nsl = (A.nrows + 63) / 64;
-    const int skip = A.nlong > 0 ? A.long_row : 0;
+    const int skip = (A.nlong > 0 || A.nmed > 0) ? A.long_row : 0;
     DevBuf<int64_t> width;
     LT_TRY(width.alloc(nsl + 1));
     LT_TRY(A.sl_ptr.alloc(nsl + 1));
@@ -899,7 +958,7 @@ lt_status csr_autotune(DevCSR &A, bool fp32) {
     // product over the listed rows, each with its own memo entry
     for (int nz = 0; nz <= (A.nnzr >= 0 ? 1 : 0); ++nz) {
         const double mean = (double)A.nnz / (double)(nz ? std::max(1L, A.nnzr) : A.nrows);
-        const TuneKey key{dev, fp32 ? 1 : 0, (int)std::lround(4.0 * std::log2(mean)), A.nlong > 0 ? 1 : 0, nz,
+        const TuneKey key{dev, fp32 ? 1 : 0, (int)std::lround(4.0 * std::log2(mean)), (A.nlong > 0 || A.nmed > 0) ? 1 : 0, nz,
                           nz ? A.nnzr : A.nrows};
         int &slot = nz ? (fp32 ? A.lanes_nz32 : A.lanes_nz) : (fp32 ? A.lanes32 : A.lanes);
         {

@@ -168,6 +168,14 @@ class NUFFTPlan:
         the nearest grid point with the window picked by the offset's sign."""
         B.check(B.lib().lt_nufft_set_degrid_form(self._handle, int(form)))
 
+    def set_row_form(self, form=1):
+        """The band row transforms of rows padded to 1024: 1 (default) FP64 by
+        the radix-4 transform with its first and last passes in registers,
+        FP32 by the one-wave radix-16 transform; 0 the 256-thread radix-4
+        transform staged in LDS for both (the FP64 results are the same
+        bits)."""
+        B.check(B.lib().lt_nufft_set_row_form(self._handle, int(form)))
+
     def time(self, direction, reps=20):
         """Mean device time (ms) of one degrid (0) or grid (1) launch."""
         ms = C.c_double(0)

@@ -89,6 +89,35 @@ def test_operator_rows_of_1024(hip, oracle):
         assert err32 < 1e-5
 
 
+def test_row_forms_of_1024(hip, oracle):
+    """Rows padded to 1024: the register-fed FP64 row transforms give the
+    256-thread LDS form's bits, FP64 and FP32 (one-wave form) against the
+    oracle's operator."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    im, src, uv, vs, models, data = _problem(n_img=512, nlens=3)
+    lb = LensBatch(im, [src] * len(models), models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    xs = [rng(81 + k).normal(size=src.size) for k in range(len(models))]
+    out = {}
+    for form in (0, 1):
+        plan.set_row_form(form)
+        inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+        out[form] = (np.stack([y.reshape(-1) for y in inv.apply(xs)]),
+                     np.stack([y.reshape(-1) for y in inv.apply(xs, precision='f32')]))
+    plan.set_row_form(1)
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in range(len(models)):
+        ref, _ = _oracle_apply(O, im, src, uv, lb.csr(k), xs[k], w, 1e2)
+        err = O.max_relative_error(out[1][0][k], ref)
+        e32 = [O.max_relative_error(out[f][1][k], ref) for f in (0, 1)]
+        print(f'sample {k}: FP64 vs oracle {err:.2e}, FP32 (LDS form, one-wave form) vs oracle {e32[0]:.2e}, '
+              f'{e32[1]:.2e}')
+        assert err < 1e-12
+        assert max(e32) < 1e-5
+
+
 def test_stencil_shapes_same_bits(hip, oracle):
     """Every samples-per-workgroup choice of the normal stencil gives the
     same operator bits; one of them against the oracle's operator."""

@@ -2,8 +2,8 @@
 (LDS-staged) kernel, the sliced copy and the product over the list of
 nonempty rows, on matrices with the row patterns of the lens matrices and
 their transposes (empty rows, trailing empty rows, rows that run across the
-streaming kernel's 2048-nonzero ranges, rows long enough for the chunked
-path).  Every kernel sums a row in a fixed order, so repeated products are
+streaming kernel's 2048-nonzero ranges, rows of 65..512 entries for the
+one-wave path, longer ones for the chunked path).  Every kernel sums a row in a fixed order, so repeated products are
 bit-identical."""
 import numpy as np
 import pytest
@@ -44,7 +44,13 @@ def _cases():
     lens = g.integers(10, 60, size=n)
     lens[[5, 777, 4000]] = [3000, 9000, 2049]
     lens[4001] = 2048
+    lens[[10, 11, 3000, 5999]] = [65, 512, 513, 200]
     out.append(('medium+long', n, 20000, lens))
+    # L^T's tail: many rows of 65..512 entries (one wave each) among 5..64
+    n = 20000
+    lens = g.integers(5, 64, size=n)
+    lens[g.integers(0, n, size=1500)] = g.integers(65, 513, size=1500)
+    out.append(('medium rows', n, 30000, lens))
     # rows of ~1000 entries: most cross a streaming range boundary
     n = 300
     lens = g.integers(900, 1100, size=n)
