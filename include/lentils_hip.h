@@ -218,14 +218,15 @@ lt_status lt_fft_c2r_host(int nx, int ny, int batch, const double *in, double *o
  * explicit matrix, operator_base.py:96-98). */
 lt_status lt_csr_spmv_host(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                            const double *x, double *y);
-/* Same with the device kernel chosen, for rows of up to 64 entries: 0 the
- * default, 1..64 (powers of two) the lane-group kernel with that many lanes
- * per row, 128 the streaming (LDS-staged) kernel, 256 one thread per row
- * over a sliced copy (64-row slices, entries interleaved), -1..-64 the
- * solver's L x: rows of 1..4 entries from a padded 4-wide table, rows of
- * 5..64 by that many lanes per row, rows without entries 0.  Longer rows
- * as in the solver's products: 65..512 entries one wave per row, longer
- * ones in 2048-entry chunks.  Every choice sums each row in a fixed order. */
+/* Same with the device kernel chosen: 0 the default (as lt_csr_spmv_host:
+ * rows above 2048 entries in 2048-entry chunks); for the others, rows of up
+ * to 64 entries by: 1..64 (powers of two) the lane-group kernel with that
+ * many lanes per row, 128 the streaming (LDS-staged) kernel, 256 one thread
+ * per row over a sliced copy (64-row slices, entries interleaved), -1..-64
+ * the solver's L x: rows of 1..4 entries from a padded 4-wide table, rows of
+ * 5..64 by that many lanes per row, rows without entries 0; longer rows as
+ * in the solver's products: 65..512 entries one wave per row, longer ones in
+ * 2048-entry chunks.  Every choice sums each row in a fixed order. */
 lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *row_ptr, const int *cols, const double *vals,
                                   const double *x, double *y, int kernel);
 

@@ -150,7 +150,10 @@ lt_status lt_csr_spmv_host_kernel(long nrows, long ncols, const int *rp, const i
     LT_TRY(A.rp.upload(rp, nrows + 1));
     LT_TRY(A.ci.upload(ci, A.nnz));
     LT_TRY(A.va.upload(va, A.nnz));
-    A.long_row = 64;  // as the solver's L and L^T: rows of 65..512 entries one wave each, longer ones chunked
+    // a chosen kernel as in the solver's L and L^T: rows of 65..512 entries
+    // one wave each, longer ones chunked; the default (lt_csr_spmv_host):
+    // rows above 2048 entries chunked
+    if (kernel != 0) A.long_row = 64;
     LT_TRY(csr_prepare(A));
     DevBuf<double> dx, dy;
     LT_TRY(dx.upload(x, ncols));

@@ -273,11 +273,14 @@ __global__ void k_row_split(long n, const int32_t *__restrict__ rp, const int32_
 // Streaming SpMV (CSR-stream): workgroup b owns the nonzeros
 // [b*kStreamNnz, (b+1)*kStreamNnz) and the rows that start among them
 // (rblk[b] .. rblk[b+1]).  It first forms every product va[k] * x[ci[k]] of
-// its range with coalesced loads into LDS, then one thread per row adds the
-// row's products in order (a row running past the range reads its tail
-// from memory).  Long rows are left to the chunked path.  With a row filter,
-// workgroups whose rows all belong to inactive samples return at once.
+// its range and of the kStreamTail nonzeros after it with coalesced loads
+// into LDS, then one thread per row adds the row's products in order (the
+// row running past the range finds its tail staged when it has at most
+// kStreamTail + 1 entries, else reads the rest from memory).  Long rows are
+// left to the medium-row and chunked paths.  With a row filter, workgroups
+// whose rows all belong to inactive samples return at once.
 constexpr int kStreamNnz = 2048;
+constexpr int kStreamTail = 64;
 
 __device__ inline long row_sample(const RowFilter &F, long row) {
     return F.rowseg ? (long)F.rowseg[row] : row / F.rps;
@@ -290,7 +293,7 @@ __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__
                                                       const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                       const TV *__restrict__ va, const T *__restrict__ x,
                                                       T *__restrict__ y, T alpha, T beta, int skip_long, RowFilter F) {
-    __shared__ T prod[kStreamNnz];
+    __shared__ T prod[kStreamNnz + kStreamTail];
     __shared__ int32_t rps[kStreamNnz + 1];
     const long b = xcd_block(blockIdx.x, gridDim.x);
     const long r0 = rblk[b], r1 = rblk[b + 1];
@@ -300,25 +303,33 @@ __global__ void __launch_bounds__(256) k_spmv_stream(long nnz, const int32_t *__
         for (long sm = row_sample(F, r0), s1 = row_sample(F, r1 - 1); sm <= s1 && !any; ++sm) any = F.act[sm] != 0;
         if (!any) return;
     }
-    const long k0 = b * kStreamNnz, k1 = min(k0 + kStreamNnz, nnz);
+    const long k0 = b * kStreamNnz, k1 = min(k0 + kStreamNnz + kStreamTail, nnz);  // staged range
     constexpr int PER = kStreamNnz / 256;
+    static_assert(kStreamTail <= 256, "the tail is one partial load per thread");
     // the row pointers of the block's rows go to LDS alongside the products
     const long nr = r1 - r0;
     const bool rp_lds = nr < kStreamNnz;
     if (rp_lds)
         for (long r = threadIdx.x; r <= nr; r += 256) rps[r] = rp[r0 + r];
-    if (k1 - k0 == kStreamNnz) {  // full range: all loads of a thread in flight together
-        int c[PER];
-        T v[PER], xv[PER];  // values widened to T as loaded
+    if (k1 - k0 == kStreamNnz + kStreamTail) {  // full range: all loads of a thread in flight together
+        int c[PER + 1];
+        T v[PER + 1], xv[PER + 1];  // values widened to T as loaded
+        const bool tl = threadIdx.x < kStreamTail;
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             c[u] = ci[k0 + threadIdx.x + 256 * u];
             v[u] = (T)va[k0 + threadIdx.x + 256 * u];
         }
+        if (tl) {
+            c[PER] = ci[k0 + kStreamNnz + threadIdx.x];
+            v[PER] = (T)va[k0 + kStreamNnz + threadIdx.x];
+        }
 #pragma unroll
         for (int u = 0; u < PER; ++u) xv[u] = x[c[u]];
+        if (tl) xv[PER] = x[c[PER]];
 #pragma unroll
         for (int u = 0; u < PER; ++u) prod[threadIdx.x + 256 * u] = v[u] * xv[u];
+        if (tl) prod[kStreamNnz + threadIdx.x] = v[PER] * xv[PER];
     } else {
         for (long k = k0 + threadIdx.x; k < k1; k += 256) prod[k - k0] = (T)va[k] * x[ci[k]];
     }
@@ -362,8 +373,21 @@ __device__ inline void chunk_sum(int b, int row, const int32_t *__restrict__ rp,
                                  const T *__restrict__ va, const T *__restrict__ x, T *__restrict__ out) {
     __shared__ T sh[4];
     const int e = min(b + kChunk, rp[row + 1]);
+    // every load of a thread issued together, then the products added in
+    // increasing position
+    constexpr int PER = kChunk / 256;
+    int c[PER];
+    T v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int k = b + threadIdx.x + 256 * i;
+        c[i] = k < e ? ci[k] : -1;
+        v[i] = k < e ? va[k] : (T)0;
+    }
     T s = (T)0;
-    for (int k = b + threadIdx.x; k < e; k += 256) s += va[k] * x[ci[k]];
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (c[i] >= 0) s += v[i] * x[c[i]];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
