@@ -1672,20 +1672,53 @@ __global__ void __launch_bounds__(256) k_rows_inv(int nx, int ny, int pady, int 
 // writes after the passes of span 1 and 4 free of bank conflicts; reads of
 // 16 consecutive units stay conflict-free under any such swizzle.
 constexpr int kR4N = 1024;
+
+// Workgroup order of the row-pair transforms over the 8 XCDs (workgroup b
+// runs on XCD b % 8): quads of 4 consecutive row pairs, whose band values
+// share 128-byte lines, run back to back on one XCD, and the quads (sample
+// major) are dealt to the XCDs in turn, so that each XCD reads the same
+// eighth of the apodization rows for every sample.  Grid:
+// 32 * ceil(nslot * ceil(P / 4) / 8) workgroups for P row pairs per sample;
+// false for the padding workgroups.
+__device__ inline bool rowpair_of(long b, int P, int nslot, int &slot, int &rpair) {
+    const int P4 = (P + 3) / 4;
+    const long x = b & 7, i = b >> 3;
+    const long Q = 8 * (i >> 2) + x;
+    if (Q >= (long)nslot * P4) return false;
+    slot = (int)(Q / P4);
+    rpair = (int)(Q - (long)slot * P4) * 4 + (int)(i & 3);
+    return rpair < P;
+}
+static unsigned rowpair_grid(int P, int nslot) {
+    const long quads = (long)nslot * ((P + 3) / 4);
+    return (unsigned)(32 * ((quads + 7) / 8));
+}
+
 template <int SW>
 __device__ inline int r4swz(int i) {
     return SW == 1 ? (i ^ ((i >> 4) & 3)) : SW == 2 ? (i ^ (((i >> 4) & 3) << 2)) : i;
 }
 
+// The twiddles in LDS: the first quarter period tq[e] = w^e (e < 256, its
+// position swizzled like the exchanges so that strided reads spread over the
+// banks) and w^(e + 256 q) = (-i)^q w^e by exact quarter turns (the global
+// table is built the same way, so the values are its bits).
+__device__ inline int r4tsw(int e) { return e ^ ((e >> 4) & 15); }
+__device__ inline Cplx<double> r4tw(const Cplx<double> *tq, int m) {
+    const Cplx<double> t = tq[r4tsw(m & 255)];
+    const int q = m >> 8;
+    return q == 0 ? t : q == 1 ? Cplx<double>{t.im, -t.re} : Cplx<double>{-t.re, -t.im};
+}
+
 // a middle pass (span NS = 4, 16, 64): read with swizzle SI, write with SO
 template <bool INV, int NS, int SI, int SO>
-__device__ inline void r4_pass(Cplx<double> *buf, const Cplx<double> *__restrict__ twd) {
+__device__ inline void r4_pass(Cplx<double> *buf, const Cplx<double> *tq) {
     const int j = threadIdx.x, k = j & (NS - 1);
     constexpr int step = kR4N / (NS * 4);
+    __syncthreads();  // the previous pass's writes (and the twiddle table's)
     Cplx<double> w[3];
 #pragma unroll
-    for (int r = 1; r < 4; ++r) w[r - 1] = twd[k * r * step];
-    __syncthreads();  // the previous pass's writes
+    for (int r = 1; r < 4; ++r) w[r - 1] = r4tw(tq, k * r * step);
     Cplx<double> v[4];
     v[0] = buf[r4swz<SI>(j)];
 #pragma unroll
@@ -1699,12 +1732,12 @@ __device__ inline void r4_pass(Cplx<double> *buf, const Cplx<double> *__restrict
 
 // the last pass (span 256): its four outputs j + 256 r in v
 template <bool INV>
-__device__ inline void r4_last(Cplx<double> (&v)[4], const Cplx<double> *buf, const Cplx<double> *__restrict__ twd) {
+__device__ inline void r4_last(Cplx<double> (&v)[4], const Cplx<double> *buf, const Cplx<double> *tq) {
     const int j = threadIdx.x;
+    __syncthreads();
     Cplx<double> w[3];
 #pragma unroll
-    for (int r = 1; r < 4; ++r) w[r - 1] = twd[j * r];
-    __syncthreads();
+    for (int r = 1; r < 4; ++r) w[r - 1] = r4tw(tq, j * r);
     v[0] = buf[j];
 #pragma unroll
     for (int r = 1; r < 4; ++r) v[r] = tw_mul<double, INV>(buf[j + 256 * r], w[r - 1]);
@@ -1712,19 +1745,18 @@ __device__ inline void r4_last(Cplx<double> (&v)[4], const Cplx<double> *buf, co
 }
 
 template <class TI>
-__global__ void __launch_bounds__(256) k_rows_fwd_r(int nx, int ny, int pady, int padx, int nu, int nbv,
+__global__ void __launch_bounds__(256) k_rows_fwd_r(int nslot, int nx, int ny, int pady, int padx, int nu, int nbv,
                                                     const TI *__restrict__ img, const double *__restrict__ apod,
                                                     const int *__restrict__ active, const uint8_t *__restrict__ rowmask,
                                                     const Cplx<double> *__restrict__ twd,
                                                     Cplx<double> *__restrict__ band) {
     constexpr int N = kR4N;
     __shared__ Cplx<double> buf[N];
-    // row pairs in XCD order: the four pairs whose band values share a
-    // 128-byte line run on one XCD, one after another
-    const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
-    const int slot = (int)(lb / gridDim.x);
+    __shared__ Cplx<double> tq[256];
+    int slot, rpair;
+    if (!rowpair_of(blockIdx.x, (nx + 1) / 2, nslot, slot, rpair)) return;
     const int smp = active ? active[slot] : slot;
-    const int row0 = (int)(lb - (long)slot * gridDim.x) * 2, ia = row0, ib = row0 + 1;
+    const int row0 = rpair * 2, ia = row0, ib = row0 + 1;
     const TI *im = img + (long)smp * nx * ny;
     const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
     const bool ua = ia < nx && (!rm || rm[ia]), ub = ib < nx && (!rm || rm[ib]);
@@ -1737,6 +1769,7 @@ __global__ void __launch_bounds__(256) k_rows_fwd_r(int nx, int ny, int pady, in
         return;
     }
     const int j = threadIdx.x;
+    tq[r4tsw(j)] = twd[j];
     // pass 1 (span 1, unit twiddles) from the apodized rows x[j + 256 r]
     Cplx<double> v[4];
 #pragma unroll
@@ -1752,10 +1785,10 @@ __global__ void __launch_bounds__(256) k_rows_fwd_r(int nx, int ny, int pady, in
     dft4<double, false>(v);
 #pragma unroll
     for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
-    r4_pass<false, 4, 1, 2>(buf, twd);
-    r4_pass<false, 16, 2, 0>(buf, twd);
-    r4_pass<false, 64, 0, 0>(buf, twd);
-    r4_last<false>(v, buf, twd);
+    r4_pass<false, 4, 1, 2>(buf, tq);
+    r4_pass<false, 16, 2, 0>(buf, tq);
+    r4_pass<false, 64, 0, 0>(buf, tq);
+    r4_last<false>(v, buf, tq);
     __syncthreads();  // every read of the last pass before its writes
     // the outputs the band needs: z[o] for o < nbv and o > N - nbv
 #pragma unroll
@@ -1776,7 +1809,7 @@ __global__ void __launch_bounds__(256) k_rows_fwd_r(int nx, int ny, int pady, in
 }
 
 template <class TO>
-__global__ void __launch_bounds__(256) k_rows_inv_r(int nx, int ny, int pady, int padx, int nu, int nbv,
+__global__ void __launch_bounds__(256) k_rows_inv_r(int nslot, int nx, int ny, int pady, int padx, int nu, int nbv,
                                                     const Cplx<double> *__restrict__ band,
                                                     const double *__restrict__ apod, TO alpha, int accumulate,
                                                     const int *__restrict__ active,
@@ -1784,17 +1817,17 @@ __global__ void __launch_bounds__(256) k_rows_inv_r(int nx, int ny, int pady, in
                                                     const Cplx<double> *__restrict__ twd, TO *__restrict__ out) {
     constexpr int N = kR4N, H = N / 2;
     __shared__ Cplx<double> buf[N];
-    // row pairs in XCD order: the four pairs whose band values share a
-    // 128-byte line run on one XCD, one after another
-    const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
-    const int slot = (int)(lb / gridDim.x);
+    __shared__ Cplx<double> tq[256];
+    int slot, rpair;
+    if (!rowpair_of(blockIdx.x, (nx + 1) / 2, nslot, slot, rpair)) return;
     const int smp = active ? active[slot] : slot;
-    const int row0 = (int)(lb - (long)slot * gridDim.x) * 2, ia = row0, ib = row0 + 1;
+    const int row0 = rpair * 2, ia = row0, ib = row0 + 1;
     const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
     const bool ua = ia < nx && (!rm || rm[ia]), ub = ib < nx && (!rm || rm[ib]);
     if (!ua && !ub) return;  // no output row needed
     const Cplx<double> *B = band + (long)slot * nbv * nu;
     const int j = threadIdx.x;
+    tq[r4tsw(j)] = twd[j];
     // pass 1 inputs: z[k] = A[k] + i Bb[k] of the band rows' Hermitian
     // extension (conjugates above H; imaginary parts of columns 0 and H
     // dropped; rows not needed taken as zero)
@@ -1817,10 +1850,10 @@ __global__ void __launch_bounds__(256) k_rows_inv_r(int nx, int ny, int pady, in
     dft4<double, true>(v);
 #pragma unroll
     for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
-    r4_pass<true, 4, 1, 2>(buf, twd);
-    r4_pass<true, 16, 2, 0>(buf, twd);
-    r4_pass<true, 64, 0, 0>(buf, twd);
-    r4_last<true>(v, buf, twd);
+    r4_pass<true, 4, 1, 2>(buf, tq);
+    r4_pass<true, 16, 2, 0>(buf, tq);
+    r4_pass<true, 64, 0, 0>(buf, tq);
+    r4_last<true>(v, buf, tq);
     TO *o = out + (long)smp * nx * ny;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -2762,10 +2795,18 @@ static lt_status ensure_twiddles(NufftPlan &P) {
     if (P.twd64.n == 2 * (size_t)N && P.twd32.n == 2 * (size_t)N) return LT_OK;
     std::vector<double> t64(2 * (size_t)N);
     std::vector<float> t32(2 * (size_t)N);
+    // the first quarter period from cos / sin, the others by exact quarter
+    // turns (times -i), so that a kernel holding only the first quarter
+    // rebuilds every entry bit for bit
     for (int e = 0; e < N; ++e) {
-        const double a = 2.0 * M_PI * (double)e / (double)N;
-        t64[2 * e] = std::cos(a);
-        t64[2 * e + 1] = -std::sin(a);
+        if (e < N / 4) {
+            const double a = 2.0 * M_PI * (double)e / (double)N;
+            t64[2 * e] = std::cos(a);
+            t64[2 * e + 1] = -std::sin(a);
+        } else {
+            t64[2 * e] = t64[2 * (e - N / 4) + 1];
+            t64[2 * e + 1] = -t64[2 * (e - N / 4)];
+        }
         t32[2 * e] = (float)t64[2 * e];
         t32[2 * e + 1] = (float)t64[2 * e + 1];
     }
@@ -2812,8 +2853,8 @@ static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *
     }
     if constexpr (sizeof(T) == 8) {
         if (P.nyp == kR4N && P.wave_rows) {
-            hipLaunchKernelGGL((k_rows_fwd_r<TI>), dim3((P.nx + 1) / 2, nactive), dim3(256), 0, stream(), P.nx, P.ny,
-                               P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, rowmask, tw, B);
+            hipLaunchKernelGGL((k_rows_fwd_r<TI>), dim3(rowpair_grid((P.nx + 1) / 2, nactive)), dim3(256), 0, stream(),
+                               nactive, P.nx, P.ny, P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, rowmask, tw, B);
             LT_CHECK_LAUNCH();
             return LT_OK;
         }
@@ -2854,8 +2895,9 @@ static lt_status rows_inverse(NufftPlan &P, int nbv, const Cplx<T> *F, const int
     }
     if constexpr (sizeof(T) == 8) {
         if (P.nyp == kR4N && P.wave_rows) {
-            hipLaunchKernelGGL((k_rows_inv_r<TO>), dim3((P.nx + 1) / 2, nactive), dim3(256), 0, stream(), P.nx, P.ny,
-                               P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0, active, rowmask, tw, out);
+            hipLaunchKernelGGL((k_rows_inv_r<TO>), dim3(rowpair_grid((P.nx + 1) / 2, nactive)), dim3(256), 0, stream(),
+                               nactive, P.nx, P.ny, P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0, active,
+                               rowmask, tw, out);
             LT_CHECK_LAUNCH();
             return LT_OK;
         }
