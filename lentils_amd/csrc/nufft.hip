@@ -1704,10 +1704,17 @@ __device__ inline int r4swz(int i) {
 // banks) and w^(e + 256 q) = (-i)^q w^e by exact quarter turns (the global
 // table is built the same way, so the values are its bits).
 __device__ inline int r4tsw(int e) { return e ^ ((e >> 4) & 15); }
-__device__ inline Cplx<double> r4tw(const Cplx<double> *tq, int m) {
-    const Cplx<double> t = tq[r4tsw(m & 255)];
+// (-i)^q t is written as a swap (odd q) and two sign selects; a four-way
+// select of the rotated pairs compiled to a wrong real part for q = 3 in
+// FP32 (a probe of every m against the full table showed it), this form
+// matches the table in both precisions.
+template <class T>
+__device__ inline Cplx<T> r4tw(const Cplx<T> *tq, int m) {  // m < 1024
+    const Cplx<T> t = tq[r4tsw(m & 255)];
     const int q = m >> 8;
-    return q == 0 ? t : q == 1 ? Cplx<double>{t.im, -t.re} : Cplx<double>{-t.re, -t.im};
+    const bool sw = (q & 1) != 0;
+    const T x = sw ? t.im : t.re, y = sw ? t.re : t.im;
+    return Cplx<T>{(q & 2) ? -x : x, ((q + 1) & 2) ? -y : y};
 }
 
 // a middle pass (span NS = 4, 16, 64): read with swizzle SI, write with SO
@@ -1924,7 +1931,7 @@ __device__ inline int wpad(int i) { return i + (i >> 4); }
 // passes on the lane's pass-1 inputs v (x[l + 64 r]); the transform ends in
 // buf (padded index)
 template <class T, bool INV>
-__device__ inline void wave_fft(Cplx<T> (&v)[16], Cplx<T> *buf, const Cplx<T> *__restrict__ twd) {
+__device__ inline void wave_fft(Cplx<T> (&v)[16], Cplx<T> *buf, const Cplx<T> *tq) {
     const int l = threadIdx.x;
     // pass 1: radix 16, Ns = 1 (no twiddles), outputs at 16 l + r
     dft16<T, INV>(v);
@@ -1936,7 +1943,7 @@ __device__ inline void wave_fft(Cplx<T> (&v)[16], Cplx<T> *buf, const Cplx<T> *_
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const Cplx<T> a = buf[wpad(l + 64 * r)];
-        const Cplx<T> w = twd[k * r * 4];
+        const Cplx<T> w = r4tw(tq, k * r * 4);
         const T wi = INV ? -w.im : w.im;
         v[r] = Cplx<T>{a.re * w.re - a.im * wi, a.re * wi + a.im * w.re};
     }
@@ -1954,7 +1961,7 @@ __device__ inline void wave_fft(Cplx<T> (&v)[16], Cplx<T> *buf, const Cplx<T> *_
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const Cplx<T> a = buf[wpad(j + 256 * r)];
-            const Cplx<T> w = twd[j * r];
+            const Cplx<T> w = r4tw(tq, j * r);
             const T wi = INV ? -w.im : w.im;
             t[r] = Cplx<T>{a.re * w.re - a.im * wi, a.re * wi + a.im * w.re};
         }
@@ -1973,6 +1980,7 @@ __global__ void __launch_bounds__(64) k_rows_fwd_w(int nx, int ny, int pady, int
                                                    const Cplx<T> *__restrict__ twd, Cplx<T> *__restrict__ band) {
     constexpr int N = kWaveN;
     __shared__ Cplx<T> buf[N + N / 16];
+    __shared__ Cplx<T> tq[256];  // quarter-period twiddles (r4tw)
     const int slot = blockIdx.y;
     const int smp = active ? active[slot] : slot;
     const int row0 = blockIdx.x * 2;
@@ -1999,7 +2007,8 @@ __global__ void __launch_bounds__(64) k_rows_fwd_w(int nx, int ny, int pady, int
         }
         v[r] = Cplx<T>{re, imv};
     }
-    wave_fft<T, false>(v, buf, twd);
+    for (int e = l; e < 256; e += 64) tq[r4tsw(e)] = twd[e];  // visible after pass 1's barrier
+    wave_fft<T, false>(v, buf, tq);
     const T h = (T)0.5;
     for (int q = l; q < nbv * 2; q += 64) {
         const int vv = q >> 1, rr = q & 1;
@@ -2022,6 +2031,7 @@ __global__ void __launch_bounds__(64) k_rows_inv_w(int nx, int ny, int pady, int
                                                    const Cplx<T> *__restrict__ twd, TO *__restrict__ out) {
     constexpr int N = kWaveN, H = N / 2;
     __shared__ Cplx<T> buf[N + N / 16];
+    __shared__ Cplx<T> tq[256];  // quarter-period twiddles (r4tw)
     const int slot = blockIdx.y;
     const int smp = active ? active[slot] : slot;
     const int row0 = blockIdx.x * 2;
@@ -2043,7 +2053,8 @@ __global__ void __launch_bounds__(64) k_rows_inv_w(int nx, int ny, int pady, int
         if (vv == 0 || vv == H) ca.im = cb.im = (T)0;
         v[r] = m <= H ? Cplx<T>{ca.re - cb.im, ca.im + cb.re} : Cplx<T>{ca.re + cb.im, cb.re - ca.im};
     }
-    wave_fft<T, true>(v, buf, twd);
+    for (int e = l; e < 256; e += 64) tq[r4tsw(e)] = twd[e];  // visible after pass 1's barrier
+    wave_fft<T, true>(v, buf, tq);
     TO *o = out + (long)smp * nx * ny;
     for (int q = l; q < 2 * ny; q += 64) {
         const int rr = q / ny, j = q - rr * ny;
