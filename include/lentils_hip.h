@@ -307,10 +307,10 @@ lt_status lt_nufft_set_stencil_shape(lt_nufft h, int samples_per_group);
  * visibilities exactly on a grid line take the 2W+1 taps either way. */
 lt_status lt_nufft_set_degrid_form(lt_nufft h, int form);
 /* No reference counterpart: the band row transforms of padded rows of 1024
- * for this plan: 1 (default) = FP64 rows by a radix-4 transform whose first
- * and last passes run in registers (the same bits as form 0), FP32 rows by
- * a one-wave radix-16 transform (FP32 rounding apart); 0 = the 256-thread
- * radix-4 transform staged in LDS for both. */
+ * for this plan: 1 (default) = a radix-4 transform whose first and last
+ * passes run in registers, twiddles from a quarter-period table in LDS;
+ * 0 = the 256-thread radix-4 transform staged in LDS.  Both give the same
+ * bits, in FP64 and in FP32. */
 lt_status lt_nufft_set_row_form(lt_nufft h, int form);
 /* No reference counterpart: times the resident normal stencil's kernel as
  * the solver launches it for nb samples (band form, tuned shape) with HIP

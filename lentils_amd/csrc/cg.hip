@@ -1866,7 +1866,7 @@ lt_status lt_nufft_set_degrid_form(lt_nufft nfh, int form) {
 lt_status lt_nufft_set_row_form(lt_nufft nfh, int form) {
     NufftPlan *nf = nufft_of(nfh);
     if (!nf || (form != 0 && form != 1)) {
-        set_error("lt_nufft_set_row_form: 0 (256-thread LDS radix-4 form) or 1 (register-fed / one-wave forms)");
+        set_error("lt_nufft_set_row_form: 0 (256-thread LDS radix-4 form) or 1 (register-fed radix-4 form)");
         return LT_ERR_ARG;
     }
     nf->wave_rows = form == 1;

@@ -1171,8 +1171,9 @@ __global__ void k_mf_pack(Geo G, long nblock, const int32_t *__restrict__ blocks
 }
 
 // R[s][row] = sign(r') sum_o coef[o][row] S_s(r' + o) on the matrix cores,
-// written to the rows' band cells out[s * ostride + cell].  Workgroup = (block, chunk of GPW sample groups): one wave per tile of the
-// block; the waves stage the block's window for a group of 8 samples in LDS
+// written to the rows' band cells out[s * ostride + cell].  Workgroup =
+// (block, chunk of GPW sample groups): one wave per tile of the block; the
+// waves stage the block's window for a group of 8 samples in LDS
 // together, then each runs its tile's 64 matrix steps with the packed
 // matrix read from L2 as it goes (the chunks of one block run back to back
 // on one XCD) and the window's values from LDS.
@@ -1718,48 +1719,48 @@ __device__ inline Cplx<T> r4tw(const Cplx<T> *tq, int m) {  // m < 1024
 }
 
 // a middle pass (span NS = 4, 16, 64): read with swizzle SI, write with SO
-template <bool INV, int NS, int SI, int SO>
-__device__ inline void r4_pass(Cplx<double> *buf, const Cplx<double> *tq) {
+template <class T, bool INV, int NS, int SI, int SO>
+__device__ inline void r4_pass(Cplx<T> *buf, const Cplx<T> *tq) {
     const int j = threadIdx.x, k = j & (NS - 1);
     constexpr int step = kR4N / (NS * 4);
     __syncthreads();  // the previous pass's writes (and the twiddle table's)
-    Cplx<double> w[3];
+    Cplx<T> w[3];
 #pragma unroll
     for (int r = 1; r < 4; ++r) w[r - 1] = r4tw(tq, k * r * step);
-    Cplx<double> v[4];
+    Cplx<T> v[4];
     v[0] = buf[r4swz<SI>(j)];
 #pragma unroll
-    for (int r = 1; r < 4; ++r) v[r] = tw_mul<double, INV>(buf[r4swz<SI>(j + 256 * r)], w[r - 1]);
+    for (int r = 1; r < 4; ++r) v[r] = tw_mul<T, INV>(buf[r4swz<SI>(j + 256 * r)], w[r - 1]);
     __syncthreads();  // every read before the writes
-    dft4<double, INV>(v);
+    dft4<T, INV>(v);
     const int d = (j / NS) * NS * 4 + k;
 #pragma unroll
     for (int r = 0; r < 4; ++r) buf[r4swz<SO>(d + r * NS)] = v[r];
 }
 
 // the last pass (span 256): its four outputs j + 256 r in v
-template <bool INV>
-__device__ inline void r4_last(Cplx<double> (&v)[4], const Cplx<double> *buf, const Cplx<double> *tq) {
+template <class T, bool INV>
+__device__ inline void r4_last(Cplx<T> (&v)[4], const Cplx<T> *buf, const Cplx<T> *tq) {
     const int j = threadIdx.x;
     __syncthreads();
-    Cplx<double> w[3];
+    Cplx<T> w[3];
 #pragma unroll
     for (int r = 1; r < 4; ++r) w[r - 1] = r4tw(tq, j * r);
     v[0] = buf[j];
 #pragma unroll
-    for (int r = 1; r < 4; ++r) v[r] = tw_mul<double, INV>(buf[j + 256 * r], w[r - 1]);
-    dft4<double, INV>(v);
+    for (int r = 1; r < 4; ++r) v[r] = tw_mul<T, INV>(buf[j + 256 * r], w[r - 1]);
+    dft4<T, INV>(v);
 }
 
-template <class TI>
+template <class T, class TI>
 __global__ void __launch_bounds__(256) k_rows_fwd_r(int nslot, int nx, int ny, int pady, int padx, int nu, int nbv,
                                                     const TI *__restrict__ img, const double *__restrict__ apod,
                                                     const int *__restrict__ active, const uint8_t *__restrict__ rowmask,
-                                                    const Cplx<double> *__restrict__ twd,
-                                                    Cplx<double> *__restrict__ band) {
+                                                    const Cplx<T> *__restrict__ twd,
+                                                    Cplx<T> *__restrict__ band) {
     constexpr int N = kR4N;
-    __shared__ Cplx<double> buf[N];
-    __shared__ Cplx<double> tq[256];
+    __shared__ Cplx<T> buf[N];
+    __shared__ Cplx<T> tq[256];
     int slot, rpair;
     if (!rowpair_of(blockIdx.x, (nx + 1) / 2, nslot, slot, rpair)) return;
     const int smp = active ? active[slot] : slot;
@@ -1767,35 +1768,35 @@ __global__ void __launch_bounds__(256) k_rows_fwd_r(int nslot, int nx, int ny, i
     const TI *im = img + (long)smp * nx * ny;
     const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
     const bool ua = ia < nx && (!rm || rm[ia]), ub = ib < nx && (!rm || rm[ib]);
-    Cplx<double> *B = band + (long)slot * nbv * nu;
+    Cplx<T> *B = band + (long)slot * nbv * nu;
     if (!ua && !ub) {
         for (int q = threadIdx.x; q < nbv * 2; q += 256) {
             const int v = q >> 1, rr = q & 1;
-            if (row0 + rr < nx) B[(long)v * nu + padx + row0 + rr] = Cplx<double>{0.0, 0.0};
+            if (row0 + rr < nx) B[(long)v * nu + padx + row0 + rr] = Cplx<T>{(T)0, (T)0};
         }
         return;
     }
     const int j = threadIdx.x;
     tq[r4tsw(j)] = twd[j];
     // pass 1 (span 1, unit twiddles) from the apodized rows x[j + 256 r]
-    Cplx<double> v[4];
+    Cplx<T> v[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int c = j + 256 * r - pady;
-        double re = 0.0, imv = 0.0;
+        T re = (T)0, imv = (T)0;
         if (c >= 0 && c < ny) {
-            if (ua) re = (double)(im[(long)ia * ny + c] * (TI)apod[(long)ia * ny + c]);
-            if (ub) imv = (double)(im[(long)ib * ny + c] * (TI)apod[(long)ib * ny + c]);
+            if (ua) re = (T)(im[(long)ia * ny + c] * (TI)apod[(long)ia * ny + c]);
+            if (ub) imv = (T)(im[(long)ib * ny + c] * (TI)apod[(long)ib * ny + c]);
         }
-        v[r] = Cplx<double>{re, imv};
+        v[r] = Cplx<T>{re, imv};
     }
-    dft4<double, false>(v);
+    dft4<T, false>(v);
 #pragma unroll
     for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
-    r4_pass<false, 4, 1, 2>(buf, tq);
-    r4_pass<false, 16, 2, 0>(buf, tq);
-    r4_pass<false, 64, 0, 0>(buf, tq);
-    r4_last<false>(v, buf, tq);
+    r4_pass<T, false, 4, 1, 2>(buf, tq);
+    r4_pass<T, false, 16, 2, 0>(buf, tq);
+    r4_pass<T, false, 64, 0, 0>(buf, tq);
+    r4_last<T, false>(v, buf, tq);
     __syncthreads();  // every read of the last pass before its writes
     // the outputs the band needs: z[o] for o < nbv and o > N - nbv
 #pragma unroll
@@ -1804,27 +1805,27 @@ __global__ void __launch_bounds__(256) k_rows_fwd_r(int nslot, int nx, int ny, i
         if (o < nbv || N - o < nbv) buf[o] = v[r];
     }
     __syncthreads();
-    const double h = 0.5;
+    const T h = (T)0.5;
     for (int q = threadIdx.x; q < nbv * 2; q += 256) {
         const int vv = q >> 1, rr = q & 1;
         if (row0 + rr >= nx) continue;
-        const Cplx<double> z = buf[vv], zm = buf[(N - vv) & (N - 1)];
-        const Cplx<double> x = rr ? Cplx<double>{h * (z.im + zm.im), h * (zm.re - z.re)}
-                                  : Cplx<double>{h * (z.re + zm.re), h * (z.im - zm.im)};
+        const Cplx<T> z = buf[vv], zm = buf[(N - vv) & (N - 1)];
+        const Cplx<T> x = rr ? Cplx<T>{h * (z.im + zm.im), h * (zm.re - z.re)}
+                                  : Cplx<T>{h * (z.re + zm.re), h * (z.im - zm.im)};
         B[(long)vv * nu + padx + row0 + rr] = x;
     }
 }
 
-template <class TO>
+template <class T, class TO>
 __global__ void __launch_bounds__(256) k_rows_inv_r(int nslot, int nx, int ny, int pady, int padx, int nu, int nbv,
-                                                    const Cplx<double> *__restrict__ band,
+                                                    const Cplx<T> *__restrict__ band,
                                                     const double *__restrict__ apod, TO alpha, int accumulate,
                                                     const int *__restrict__ active,
                                                     const uint8_t *__restrict__ rowmask,
-                                                    const Cplx<double> *__restrict__ twd, TO *__restrict__ out) {
+                                                    const Cplx<T> *__restrict__ twd, TO *__restrict__ out) {
     constexpr int N = kR4N, H = N / 2;
-    __shared__ Cplx<double> buf[N];
-    __shared__ Cplx<double> tq[256];
+    __shared__ Cplx<T> buf[N];
+    __shared__ Cplx<T> tq[256];
     int slot, rpair;
     if (!rowpair_of(blockIdx.x, (nx + 1) / 2, nslot, slot, rpair)) return;
     const int smp = active ? active[slot] : slot;
@@ -1832,35 +1833,35 @@ __global__ void __launch_bounds__(256) k_rows_inv_r(int nslot, int nx, int ny, i
     const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
     const bool ua = ia < nx && (!rm || rm[ia]), ub = ib < nx && (!rm || rm[ib]);
     if (!ua && !ub) return;  // no output row needed
-    const Cplx<double> *B = band + (long)slot * nbv * nu;
+    const Cplx<T> *B = band + (long)slot * nbv * nu;
     const int j = threadIdx.x;
     tq[r4tsw(j)] = twd[j];
     // pass 1 inputs: z[k] = A[k] + i Bb[k] of the band rows' Hermitian
     // extension (conjugates above H; imaginary parts of columns 0 and H
     // dropped; rows not needed taken as zero)
-    Cplx<double> v[4];
+    Cplx<T> v[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int k = j + 256 * r;
         const int vv = k <= H ? k : N - k;
-        Cplx<double> ca{0.0, 0.0}, cb{0.0, 0.0};
+        Cplx<T> ca{(T)0, (T)0}, cb{(T)0, (T)0};
         if (vv < nbv) {
             if (ua) ca = B[(long)vv * nu + padx + ia];
             if (ub) cb = B[(long)vv * nu + padx + ib];
         }
         if (vv == 0 || vv == H) {
-            ca.im = 0.0;
-            cb.im = 0.0;
+            ca.im = (T)0;
+            cb.im = (T)0;
         }
-        v[r] = k <= H ? Cplx<double>{ca.re - cb.im, ca.im + cb.re} : Cplx<double>{ca.re + cb.im, -ca.im + cb.re};
+        v[r] = k <= H ? Cplx<T>{ca.re - cb.im, ca.im + cb.re} : Cplx<T>{ca.re + cb.im, -ca.im + cb.re};
     }
-    dft4<double, true>(v);
+    dft4<T, true>(v);
 #pragma unroll
     for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
-    r4_pass<true, 4, 1, 2>(buf, tq);
-    r4_pass<true, 16, 2, 0>(buf, tq);
-    r4_pass<true, 64, 0, 0>(buf, tq);
-    r4_last<true>(v, buf, tq);
+    r4_pass<T, true, 4, 1, 2>(buf, tq);
+    r4_pass<T, true, 16, 2, 0>(buf, tq);
+    r4_pass<T, true, 64, 0, 0>(buf, tq);
+    r4_last<T, true>(v, buf, tq);
     TO *o = out + (long)smp * nx * ny;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -1876,194 +1877,6 @@ __global__ void __launch_bounds__(256) k_rows_inv_r(int nslot, int nx, int ny, i
             const TO val = alpha * ((TO)v[r].im * (TO)apod[g]);
             o[g] = accumulate ? o[g] + val : val;
         }
-    }
-}
-
-// 16-point DFT in registers as 4 x 4 (natural order in and out):
-// V[k1 + 4 k2] = sum_n2 w4^(n2 k2) w16^(n2 k1) sum_n1 v[4 n1 + n2] w4^(n1 k1)
-template <class T, bool INV>
-__device__ inline void dft16(Cplx<T> (&v)[16]) {
-    constexpr T C1 = (T)0.92387953251128675613, S1 = (T)0.38268343236508977173, HS = (T)0.70710678118654752440;
-    Cplx<T> y[4][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2) {
-        Cplx<T> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
-        dft4<T, INV>(t);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
-    }
-    // w16^m = (c, -s) forward, (c, s) inverse, m = n2 k1
-#pragma unroll
-    for (int n2 = 1; n2 < 4; ++n2)
-#pragma unroll
-        for (int k1 = 1; k1 < 4; ++k1) {
-            const int m = n2 * k1;
-            const Cplx<T> a = y[n2][k1];
-            if (m == 4) {  // -i forward, +i inverse
-                y[n2][k1] = INV ? Cplx<T>{-a.im, a.re} : Cplx<T>{a.im, -a.re};
-                continue;
-            }
-            const T c = m == 1 ? C1 : m == 2 ? HS : m == 3 ? S1 : m == 6 ? -HS : -C1;
-            const T sn = m == 1 ? S1 : m == 2 ? HS : m == 3 ? C1 : m == 6 ? HS : -S1;
-            const T wi = INV ? sn : -sn;
-            y[n2][k1] = Cplx<T>{a.re * c - a.im * wi, a.re * wi + a.im * c};
-        }
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        Cplx<T> t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
-        dft4<T, INV>(t);
-#pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
-    }
-}
-
-// One-wave row transforms of length 1024: one row pair per 64-lane
-// workgroup, Stockham passes of radix 16, 16, 4 (three LDS exchanges instead
-// of five radix-4 stages), the first pass fed from the lane's own 16 inputs
-// x[l + 64 r] in registers.  LDS index i is stored at i + i / 16, which
-// keeps every pass's 16-byte accesses free of bank conflicts.  Used for FP32
-// (43 + 43 us against 55 + 50 us per application at 64 samples); in FP64 the
-// 17 KB of LDS per wave leaves too few waves per CU to cover memory latency
-// (65 + 59 against 63 + 54 us), so FP64 keeps the 256-thread radix-4 form.
-constexpr int kWaveN = 1024;
-__device__ inline int wpad(int i) { return i + (i >> 4); }
-
-// passes on the lane's pass-1 inputs v (x[l + 64 r]); the transform ends in
-// buf (padded index)
-template <class T, bool INV>
-__device__ inline void wave_fft(Cplx<T> (&v)[16], Cplx<T> *buf, const Cplx<T> *tq) {
-    const int l = threadIdx.x;
-    // pass 1: radix 16, Ns = 1 (no twiddles), outputs at 16 l + r
-    dft16<T, INV>(v);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) buf[wpad(16 * l + r)] = v[r];
-    __syncthreads();
-    // pass 2: radix 16, Ns = 16, butterfly l, twiddle exponent k r N / 256
-    const int k = l & 15;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const Cplx<T> a = buf[wpad(l + 64 * r)];
-        const Cplx<T> w = r4tw(tq, k * r * 4);
-        const T wi = INV ? -w.im : w.im;
-        v[r] = Cplx<T>{a.re * w.re - a.im * wi, a.re * wi + a.im * w.re};
-    }
-    __syncthreads();
-    dft16<T, INV>(v);
-    const int d = (l >> 4) * 256 + k;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) buf[wpad(d + 16 * r)] = v[r];
-    __syncthreads();
-    // pass 3: radix 4, Ns = 256; butterfly j writes where it reads
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int j = l + 64 * q;
-        Cplx<T> t[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const Cplx<T> a = buf[wpad(j + 256 * r)];
-            const Cplx<T> w = r4tw(tq, j * r);
-            const T wi = INV ? -w.im : w.im;
-            t[r] = Cplx<T>{a.re * w.re - a.im * wi, a.re * wi + a.im * w.re};
-        }
-        dft4<T, INV>(t);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) buf[wpad(j + 256 * r)] = t[r];
-    }
-    __syncthreads();
-}
-
-// k_rows_fwd for N = 1024 with the one-wave transform (64 threads)
-template <class T, class TI>
-__global__ void __launch_bounds__(64) k_rows_fwd_w(int nx, int ny, int pady, int padx, int nu, int nbv,
-                                                   const TI *__restrict__ img, const double *__restrict__ apod,
-                                                   const int *__restrict__ active, const uint8_t *__restrict__ rowmask,
-                                                   const Cplx<T> *__restrict__ twd, Cplx<T> *__restrict__ band) {
-    constexpr int N = kWaveN;
-    __shared__ Cplx<T> buf[N + N / 16];
-    __shared__ Cplx<T> tq[256];  // quarter-period twiddles (r4tw)
-    const int slot = blockIdx.y;
-    const int smp = active ? active[slot] : slot;
-    const int row0 = blockIdx.x * 2;
-    const TI *im = img + (long)smp * nx * ny;
-    const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
-    const bool ha = row0 < nx && (!rm || rm[row0]), hb = row0 + 1 < nx && (!rm || rm[row0 + 1]);
-    Cplx<T> *B = band + (long)slot * nbv * nu;
-    const int l = threadIdx.x;
-    if (!ha && !hb) {
-        for (int q = l; q < nbv * 2; q += 64) {
-            const int v = q >> 1, rr = q & 1;
-            if (row0 + rr < nx) B[(long)v * nu + padx + row0 + rr] = Cplx<T>{(T)0, (T)0};
-        }
-        return;
-    }
-    Cplx<T> v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int j = l + 64 * r - pady;
-        T re = (T)0, imv = (T)0;
-        if (j >= 0 && j < ny) {
-            if (ha) re = (T)(im[(long)row0 * ny + j] * (TI)apod[(long)row0 * ny + j]);
-            if (hb) imv = (T)(im[(long)(row0 + 1) * ny + j] * (TI)apod[(long)(row0 + 1) * ny + j]);
-        }
-        v[r] = Cplx<T>{re, imv};
-    }
-    for (int e = l; e < 256; e += 64) tq[r4tsw(e)] = twd[e];  // visible after pass 1's barrier
-    wave_fft<T, false>(v, buf, tq);
-    const T h = (T)0.5;
-    for (int q = l; q < nbv * 2; q += 64) {
-        const int vv = q >> 1, rr = q & 1;
-        if (row0 + rr >= nx) continue;
-        const Cplx<T> z = buf[wpad(vv)], zm = buf[wpad((N - vv) & (N - 1))];
-        const Cplx<T> x = rr ? Cplx<T>{h * (z.im + zm.im), h * (zm.re - z.re)}
-                             : Cplx<T>{h * (z.re + zm.re), h * (z.im - zm.im)};
-        B[(long)vv * nu + padx + row0 + rr] = x;
-    }
-}
-
-// k_rows_inv for N = 1024 with the one-wave transform (64 threads): lane l
-// forms its inputs x[l + 64 r] = A + i B (A, B the Hermitian extensions of
-// the two rows' band spectra) straight from the band
-template <class T, class TO>
-__global__ void __launch_bounds__(64) k_rows_inv_w(int nx, int ny, int pady, int padx, int nu, int nbv,
-                                                   const Cplx<T> *__restrict__ band, const double *__restrict__ apod,
-                                                   TO alpha, int accumulate, const int *__restrict__ active,
-                                                   const uint8_t *__restrict__ rowmask,
-                                                   const Cplx<T> *__restrict__ twd, TO *__restrict__ out) {
-    constexpr int N = kWaveN, H = N / 2;
-    __shared__ Cplx<T> buf[N + N / 16];
-    __shared__ Cplx<T> tq[256];  // quarter-period twiddles (r4tw)
-    const int slot = blockIdx.y;
-    const int smp = active ? active[slot] : slot;
-    const int row0 = blockIdx.x * 2;
-    const uint8_t *rm = rowmask ? rowmask + (long)smp * nx : nullptr;
-    const bool ha = row0 < nx && (!rm || rm[row0]), hb = row0 + 1 < nx && (!rm || rm[row0 + 1]);
-    if (!ha && !hb) return;
-    const Cplx<T> *B = band + (long)slot * nbv * nu;
-    const int l = threadIdx.x;
-    Cplx<T> v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int m = l + 64 * r;
-        const int vv = m <= H ? m : N - m;
-        Cplx<T> ca{(T)0, (T)0}, cb{(T)0, (T)0};
-        if (vv < nbv) {  // rows not needed are taken as zero (not read)
-            if (ha) ca = B[(long)vv * nu + padx + row0];
-            if (hb) cb = B[(long)vv * nu + padx + row0 + 1];
-        }
-        if (vv == 0 || vv == H) ca.im = cb.im = (T)0;
-        v[r] = m <= H ? Cplx<T>{ca.re - cb.im, ca.im + cb.re} : Cplx<T>{ca.re + cb.im, cb.re - ca.im};
-    }
-    for (int e = l; e < 256; e += 64) tq[r4tsw(e)] = twd[e];  // visible after pass 1's barrier
-    wave_fft<T, true>(v, buf, tq);
-    TO *o = out + (long)smp * nx * ny;
-    for (int q = l; q < 2 * ny; q += 64) {
-        const int rr = q / ny, j = q - rr * ny;
-        const int row = row0 + rr;
-        if (row >= nx || (rm && !rm[row])) continue;
-        const Cplx<T> z = buf[wpad(pady + j)];
-        const long g = (long)row * ny + j;
-        const TO val = alpha * ((TO)(rr ? z.im : z.re) * (TO)apod[g]);
-        o[g] = accumulate ? o[g] + val : val;
     }
 }
 
@@ -2856,15 +2669,9 @@ template <class T, class TI>
 static lt_status rows_forward(NufftPlan &P, int nbv, const TI *imgs, const int *active, int nactive, Cplx<T> *B,
                               const uint8_t *rowmask) {
     const Cplx<T> *tw = twiddles_of<T>(P);
-    if (P.nyp == kWaveN && P.wave_rows && sizeof(T) == 4) {
-        hipLaunchKernelGGL((k_rows_fwd_w<T, TI>), dim3((P.nx + 1) / 2, nactive), dim3(64), 0, stream(), P.nx, P.ny,
-                           P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, rowmask, tw, B);
-        LT_CHECK_LAUNCH();
-        return LT_OK;
-    }
-    if constexpr (sizeof(T) == 8) {
+    {
         if (P.nyp == kR4N && P.wave_rows) {
-            hipLaunchKernelGGL((k_rows_fwd_r<TI>), dim3(rowpair_grid((P.nx + 1) / 2, nactive)), dim3(256), 0, stream(),
+            hipLaunchKernelGGL((k_rows_fwd_r<T, TI>), dim3(rowpair_grid((P.nx + 1) / 2, nactive)), dim3(256), 0, stream(),
                                nactive, P.nx, P.ny, P.pady, P.padx, P.nu, nbv, imgs, P.apod.p, active, rowmask, tw, B);
             LT_CHECK_LAUNCH();
             return LT_OK;
@@ -2898,15 +2705,9 @@ template <class T, class TO>
 static lt_status rows_inverse(NufftPlan &P, int nbv, const Cplx<T> *F, const int *active, int nactive, TO alpha,
                               bool acc, TO *out, const uint8_t *rowmask) {
     const Cplx<T> *tw = twiddles_of<T>(P);
-    if (P.nyp == kWaveN && P.wave_rows && sizeof(T) == 4) {
-        hipLaunchKernelGGL((k_rows_inv_w<T, TO>), dim3((P.nx + 1) / 2, nactive), dim3(64), 0, stream(), P.nx, P.ny,
-                           P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0, active, rowmask, tw, out);
-        LT_CHECK_LAUNCH();
-        return LT_OK;
-    }
-    if constexpr (sizeof(T) == 8) {
+    {
         if (P.nyp == kR4N && P.wave_rows) {
-            hipLaunchKernelGGL((k_rows_inv_r<TO>), dim3(rowpair_grid((P.nx + 1) / 2, nactive)), dim3(256), 0, stream(),
+            hipLaunchKernelGGL((k_rows_inv_r<T, TO>), dim3(rowpair_grid((P.nx + 1) / 2, nactive)), dim3(256), 0, stream(),
                                nactive, P.nx, P.ny, P.pady, P.padx, P.nu, nbv, F, P.apod.p, alpha, acc ? 1 : 0, active,
                                rowmask, tw, out);
             LT_CHECK_LAUNCH();

@@ -177,9 +177,9 @@ struct NufftPlan {
     DevBuf<double> twd64;
     DevBuf<float> twd32;
     bool fused_rows = true;  // use them where nyp allows
-    // nyp = 1024: FP32 rows by the one-wave transform (k_rows_fwd_w /
-    // k_rows_inv_w), FP64 rows by the register-fed radix-4 transform
-    // (k_rows_fwd_r / k_rows_inv_r); false: the 256-thread LDS radix-4 form
+    // nyp = 1024: rows by the register-fed radix-4 transform (k_rows_fwd_r /
+    // k_rows_inv_r, the same bits as the LDS form); false: the 256-thread
+    // LDS radix-4 form
     bool wave_rows = true;
     int force_nb = 0;  // stencil samples per workgroup when > 0, the matrix-core form with -force_nb
                        // sample groups per workgroup when < 0 (else the autotuned choice)

@@ -169,11 +169,10 @@ class NUFFTPlan:
         B.check(B.lib().lt_nufft_set_degrid_form(self._handle, int(form)))
 
     def set_row_form(self, form=1):
-        """The band row transforms of rows padded to 1024: 1 (default) FP64 by
-        the radix-4 transform with its first and last passes in registers,
-        FP32 by the one-wave radix-16 transform; 0 the 256-thread radix-4
-        transform staged in LDS for both (the FP64 results are the same
-        bits)."""
+        """The band row transforms of rows padded to 1024: 1 (default) the
+        radix-4 transform with its first and last passes in registers and
+        its twiddles in LDS, 0 the 256-thread radix-4 transform staged in LDS
+        (the same bits, FP64 and FP32)."""
         B.check(B.lib().lt_nufft_set_row_form(self._handle, int(form)))
 
     def time(self, direction, reps=20):
