@@ -312,6 +312,12 @@ lt_status lt_nufft_set_degrid_form(lt_nufft h, int form);
  * 0 = the 256-thread radix-4 transform staged in LDS.  Both give the same
  * bits, in FP64 and in FP32. */
 lt_status lt_nufft_set_row_form(lt_nufft h, int form);
+/* No reference counterpart: the band's column transforms along u for this
+ * plan when the padded grid is 1024 x 1024: 1 (default) = the row
+ * transforms' register-fed radix-4 passes, reading only the image rows of
+ * the forward input and writing only the image rows of the inverse output;
+ * 0 = rocFFT over whole columns.  The two agree to rounding. */
+lt_status lt_nufft_set_col_form(lt_nufft h, int form);
 /* No reference counterpart: times the resident normal stencil's kernel as
  * the solver launches it for nb samples (band form, tuned shape) with HIP
  * events on the library stream; mean per-launch time, algorithmic bytes

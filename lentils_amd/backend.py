@@ -123,6 +123,7 @@ def lib():
         'lt_nufft_set_stencil_shape': ([vp, i], i),
         'lt_nufft_set_degrid_form': ([vp, i], i),
         'lt_nufft_set_row_form': ([vp, i], i),
+        'lt_nufft_set_col_form': ([vp, i], i),
         'lt_nufft_stencil_time': ([vp, i, i, dp, dp, dp, lp, lp], i),
         'lt_nufft_stencil_form': ([vp, ip], i),
         'lt_device_alloc': ([l, C.POINTER(vp)], i),
@@ -182,7 +183,7 @@ def exported_symbols():
             'lt_lensbatch_create', 'lt_lensbatch_info', 'lt_lensbatch_download', 'lt_lensbatch_destroy',
             'lt_nufft_create', 'lt_nufft_forward_host', 'lt_nufft_adjoint_host', 'lt_nufft_degrid_host',
             'lt_nufft_grid_host', 'lt_nufft_info', 'lt_nufft_time', 'lt_nufft_destroy', 'lt_nufft_set_weights',
-            'lt_nufft_set_stencil_shape', 'lt_nufft_set_degrid_form', 'lt_nufft_set_row_form',
+            'lt_nufft_set_stencil_shape', 'lt_nufft_set_degrid_form', 'lt_nufft_set_row_form', 'lt_nufft_set_col_form',
             'lt_nufft_stencil_time', 'lt_nufft_stencil_form',
             'lt_device_alloc', 'lt_device_upload', 'lt_device_download', 'lt_device_free', 'lt_solver_create_resident',
             'lt_kb_taps', 'lt_kb_taps_window', 'lt_dirty_beam_host', 'lt_bcb_apply_host',

@@ -1863,6 +1863,16 @@ lt_status lt_nufft_set_degrid_form(lt_nufft nfh, int form) {
     return LT_OK;
 }
 
+lt_status lt_nufft_set_col_form(lt_nufft nfh, int form) {
+    NufftPlan *nf = nufft_of(nfh);
+    if (!nf || (form != 0 && form != 1)) {
+        set_error("lt_nufft_set_col_form: 0 (rocFFT) or 1 (own radix-4 column transforms)");
+        return LT_ERR_ARG;
+    }
+    nf->own_cols = form == 1;
+    return LT_OK;
+}
+
 lt_status lt_nufft_set_row_form(lt_nufft nfh, int form) {
     NufftPlan *nf = nufft_of(nfh);
     if (!nf || (form != 0 && form != 1)) {

@@ -1880,6 +1880,69 @@ __global__ void __launch_bounds__(256) k_rows_inv_r(int nslot, int nx, int ny, i
     }
 }
 
+// The band's column transforms along u (length 1024, complex, one
+// contiguous column per workgroup: slot s, band row v), with the same
+// passes as the row transforms.  Forward: only the image rows u in
+// [padx, padx + nx) hold values (the band's other cells are zero), so they
+// alone are read; the last pass writes all 1024 outputs from registers.
+// Inverse: all 1024 inputs; the last pass writes only the image rows, the
+// only cells the row transforms read back.
+template <class T>
+__global__ void __launch_bounds__(256) k_cols_fwd_r(int nbv, int nu, int padx, int nx, long bsz,
+                                                    const Cplx<T> *__restrict__ in, Cplx<T> *__restrict__ out,
+                                                    const Cplx<T> *__restrict__ twd) {
+    __shared__ Cplx<T> buf[kR4N];
+    __shared__ Cplx<T> tq[256];
+    const int slot = (int)(blockIdx.x / nbv), vv = (int)(blockIdx.x - (long)slot * nbv);
+    const Cplx<T> *X = in + (long)slot * bsz + (long)vv * nu;
+    Cplx<T> *Y = out + (long)slot * bsz + (long)vv * nu;
+    const int j = threadIdx.x;
+    tq[r4tsw(j)] = twd[j];
+    Cplx<T> v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int u = j + 256 * r;
+        v[r] = (u >= padx && u < padx + nx) ? X[u] : Cplx<T>{(T)0, (T)0};
+    }
+    dft4<T, false>(v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
+    r4_pass<T, false, 4, 1, 2>(buf, tq);
+    r4_pass<T, false, 16, 2, 0>(buf, tq);
+    r4_pass<T, false, 64, 0, 0>(buf, tq);
+    r4_last<T, false>(v, buf, tq);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Y[j + 256 * r] = v[r];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_cols_inv_r(int nbv, int nu, int padx, int nx, long bsz,
+                                                    const Cplx<T> *__restrict__ in, Cplx<T> *__restrict__ out,
+                                                    const Cplx<T> *__restrict__ twd) {
+    __shared__ Cplx<T> buf[kR4N];
+    __shared__ Cplx<T> tq[256];
+    const int slot = (int)(blockIdx.x / nbv), vv = (int)(blockIdx.x - (long)slot * nbv);
+    const Cplx<T> *X = in + (long)slot * bsz + (long)vv * nu;
+    Cplx<T> *Y = out + (long)slot * bsz + (long)vv * nu;
+    const int j = threadIdx.x;
+    tq[r4tsw(j)] = twd[j];
+    Cplx<T> v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = X[j + 256 * r];
+    dft4<T, true>(v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) buf[r4swz<1>(4 * j + r)] = v[r];
+    r4_pass<T, true, 4, 1, 2>(buf, tq);
+    r4_pass<T, true, 16, 2, 0>(buf, tq);
+    r4_pass<T, true, 64, 0, 0>(buf, tq);
+    r4_last<T, true>(v, buf, tq);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int u = j + 256 * r;
+        if (u >= padx && u < padx + nx) Y[u] = v[r];
+    }
+}
+
 Geo make_geo(const NufftPlan &P) {
     Geo G;
     G.nu = P.nu;
@@ -2794,7 +2857,14 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
                            stream(), P.nx, P.padx, P.nu, P.half_nv, nbv, Cb, Bin);
         LT_CHECK_LAUNCH();
     }
-    {
+    // own column transforms for columns of 1024 (the row transforms' twiddle
+    // table), rocFFT otherwise
+    const bool own_cols = fused && P.own_cols && P.nu == kR4N && P.nyp == kR4N;
+    if (own_cols) {
+        hipLaunchKernelGGL((k_cols_fwd_r<T>), dim3((unsigned)((long)nactive * nbv)), dim3(256), 0, stream(), nbv, P.nu,
+                           P.padx, P.nx, bsz, Bin, Bf, twiddles_of<T>(P));
+        LT_CHECK_LAUNCH();
+    } else {
         void *in[1] = {Bin};
         void *o[1] = {Bf};
         LT_TRY(fft_check(rocfft_execute(bf->plan[1], in, o, bf->info[1]), "execute band columns forward"));
@@ -2809,7 +2879,11 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     hipLaunchKernelGGL((k_herm_band0<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0, stream(), P.nu,
                        bsz, Fin);
     LT_CHECK_LAUNCH();
-    {
+    if (own_cols) {
+        hipLaunchKernelGGL((k_cols_inv_r<T>), dim3((unsigned)((long)nactive * nbv)), dim3(256), 0, stream(), nbv, P.nu,
+                           P.padx, P.nx, bsz, Fin, Fo, twiddles_of<T>(P));
+        LT_CHECK_LAUNCH();
+    } else {
         void *in[1] = {Fin};
         void *o[1] = {Fo};
         LT_TRY(fft_check(rocfft_execute(bf->plan[2], in, o, bf->info[2]), "execute band columns inverse"));

@@ -181,6 +181,9 @@ struct NufftPlan {
     // k_rows_inv_r, the same bits as the LDS form); false: the 256-thread
     // LDS radix-4 form
     bool wave_rows = true;
+    // nu = nyp = 1024: the band's column transforms by k_cols_fwd_r /
+    // k_cols_inv_r (the same passes); false: rocFFT
+    bool own_cols = true;
     int force_nb = 0;  // stencil samples per workgroup when > 0, the matrix-core form with -force_nb
                        // sample groups per workgroup when < 0 (else the autotuned choice)
     bool degrid_half = true;  // degridder fast path with the window taps (taps32h / taps64h)

@@ -440,16 +440,29 @@ __global__ void __launch_bounds__(256) k_spmv_med(long nmed, const int32_t *__re
     if (l == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
 }
 
-// long rows: sum their chunks in order
+// long rows: one wave per row (four per workgroup): lane i adds the row's chunk
+// sums i, i + 64, ... in order, then a butterfly sum read at lane 0 (a fixed
+// order; the allocation-free path sums the same way)
 template <class T>
-__global__ void k_spmv_long(long nlong, const int32_t *__restrict__ rows, const int32_t *__restrict__ chunk0,
-                            const T *__restrict__ part, T *__restrict__ y, T alpha, T beta, RowFilter F) {
-    long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlong || F.skip(rows[l])) return;
+__device__ inline T chunks_total(const T *__restrict__ part, int c0, int c1, int lane) {
     T s = (T)0;
-    for (int c = chunk0[l]; c < chunk0[l + 1]; ++c) s += part[c];
+    for (int c = c0 + lane; c < c1; c += 64) s += part[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_long(long nlong, const int32_t *__restrict__ rows,
+                                                    const int32_t *__restrict__ chunk0, const T *__restrict__ part,
+                                                    T *__restrict__ y, T alpha, T beta, RowFilter F) {
+    const long l = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (l >= nlong) return;  // whole waves
     const int row = rows[l];
-    y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
+    if (F.skip(row)) return;
+    const T s = chunks_total(part, chunk0[l], chunk0[l + 1], lane);
+    if (lane == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
 }
 
 __global__ void k_d2f_sp(long n, const double *__restrict__ a, float *__restrict__ b) {
@@ -744,7 +757,7 @@ static lt_status spmv_t(const DevCSR &A, const T *va, T *chunk_sum, const T *x, 
         hipLaunchKernelGGL((k_spmv_chunks<T>), dim3((unsigned)A.nchunk), dim3(256), 0, stream(), A.nchunk,
                            A.chunk_beg.p, A.long_rows_of_chunk.p, A.rp.p, A.ci.p, va, x, chunk_sum, F);
         LT_CHECK_LAUNCH();
-        hipLaunchKernelGGL((k_spmv_long<T>), dim3(blocks_for(A.nlong, 256)), dim3(256), 0, stream(), A.nlong,
+        hipLaunchKernelGGL((k_spmv_long<T>), dim3((unsigned)((A.nlong + 3) / 4)), dim3(256), 0, stream(), A.nlong,
                            A.long_rows.p, A.long_chunk0.p, chunk_sum, y, alpha, beta_y, F);
         LT_CHECK_LAUNCH();
     }
@@ -1131,7 +1144,7 @@ __global__ void k_na_reset(int *cnt) {
 
 // every row above long_row entries takes a slot and a contiguous range of
 // chunk slots; the slots' order varies from run to run, a row's sum does not
-// (its chunks are summed in row order by k_na_long)
+// (its chunks are summed in a fixed order by k_na_long)
 __global__ void k_na_plan(long nrows, const int32_t *__restrict__ rp, NaPlan P) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
@@ -1158,12 +1171,12 @@ __global__ void __launch_bounds__(256) k_na_chunks(const int32_t *__restrict__ r
     chunk_sum<double>(P.beg[c], P.row[c], rp, ci, va, x, P.sum + c);
 }
 
-__global__ void k_na_long(NaPlan P, double *__restrict__ y) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= P.cnt[0]) return;
-    double s = 0.0;
-    for (int c = P.chunk0[l]; c < P.chunk0[l] + P.nch[l]; ++c) s += P.sum[c];
-    y[P.rows[l]] = 1.0 * s;
+__global__ void __launch_bounds__(256) k_na_long(NaPlan P, double *__restrict__ y) {
+    const int l = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (l >= P.cnt[0]) return;  // whole waves
+    const double s = chunks_total(P.sum, P.chunk0[l], P.chunk0[l] + P.nch[l], lane);
+    if (lane == 0) y[P.rows[l]] = 1.0 * s;
 }
 
 std::mutex g_na_mu;
@@ -1253,7 +1266,8 @@ lt_status lt_csr_spmv_device_noalloc(long nrows, long ncols, long nnz, const int
     if (max_chunks > 0) {
         hipLaunchKernelGGL(k_na_chunks, dim3((unsigned)max_chunks), dim3(256), 0, stream(), d_row_ptr, d_cols, d_vals,
                            d_x, P);
-        hipLaunchKernelGGL(k_na_long, dim3(blocks_for(std::max(max_long, 1L), 256)), dim3(256), 0, stream(), P, d_y);
+        hipLaunchKernelGGL(k_na_long, dim3((unsigned)((std::max(max_long, 1L) + 3) / 4)), dim3(256), 0, stream(), P,
+                           d_y);
         LT_CHECK_LAUNCH();
     }
     return sync();

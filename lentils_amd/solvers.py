@@ -175,6 +175,13 @@ class NUFFTPlan:
         (the same bits, FP64 and FP32)."""
         B.check(B.lib().lt_nufft_set_row_form(self._handle, int(form)))
 
+    def set_col_form(self, form=1):
+        """The band's column transforms when the padded grid is 1024 x 1024:
+        1 (default) the row transforms' radix-4 passes over the image rows'
+        part of each column, 0 rocFFT over whole columns (the same values to
+        rounding)."""
+        B.check(B.lib().lt_nufft_set_col_form(self._handle, int(form)))
+
     def time(self, direction, reps=20):
         """Mean device time (ms) of one degrid (0) or grid (1) launch."""
         ms = C.c_double(0)

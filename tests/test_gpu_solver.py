@@ -118,6 +118,35 @@ def test_row_forms_of_1024(hip, oracle):
         assert e32 < 1e-5
 
 
+def test_column_forms_of_1024(hip, oracle):
+    """Padded grid of 1024 x 1024: the own column transforms against rocFFT's
+    (rounding apart), both precisions against the oracle's operator."""
+    from lentils_amd.solvers import BatchedSourceInversion, LensBatch, NUFFTPlan
+    O = oracle
+    im, src, uv, vs, models, data = _problem(n_img=512, nlens=3, seed=83)
+    lb = LensBatch(im, [src] * len(models), models, z_src=2.0)
+    plan = NUFFTPlan(vs, im, 2, 3, 14.04)
+    w = np.full(uv.shape[0], 1e6)
+    xs = [rng(84 + k).normal(size=src.size) for k in range(len(models))]
+    out = {}
+    for form in (0, 1):
+        plan.set_col_form(form)
+        inv = BatchedSourceInversion(lb, plan, data, weights=w, lambda_curvature=1e2)
+        out[form] = (np.stack([y.reshape(-1) for y in inv.apply(xs)]),
+                     np.stack([y.reshape(-1) for y in inv.apply(xs, precision='f32')]))
+    plan.set_col_form(1)
+    for k in range(len(models)):
+        ref, _ = _oracle_apply(O, im, src, uv, lb.csr(k), xs[k], w, 1e2)
+        d64 = O.max_relative_error(out[1][0][k], out[0][0][k])
+        err = O.max_relative_error(out[1][0][k], ref)
+        e32 = O.max_relative_error(out[1][1][k], ref)
+        print(f'sample {k}: FP64 own vs rocFFT columns {d64:.2e}, own vs oracle {err:.2e}, FP32 own vs oracle '
+              f'{e32:.2e}')
+        assert d64 < 1e-13
+        assert err < 1e-12
+        assert e32 < 1e-5
+
+
 def test_stencil_shapes_same_bits(hip, oracle):
     """Every samples-per-workgroup choice of the normal stencil gives the
     same operator bits; one of them against the oracle's operator."""
