@@ -93,45 +93,26 @@ __global__ void __launch_bounds__(kDotOne) k_seg_dot_one(const int64_t *__restri
     }
 }
 
-// The same dot followed by the CG scalar step of its sample (plain CG, one
-// workgroup per sample, or per listed sample): step 1, after p . Ap: alpha = rz / pAp (as
-// k_cg_alpha); step 2, after r . r: beta = rr / rz, rz = rr, iteration count
-// and convergence flag (as k_cg_beta with rz_new = rr).
-__global__ void __launch_bounds__(kDotOne) k_seg_dot_cg(const int64_t *__restrict__ off, const int *__restrict__ list,
-                                                         const double *__restrict__ a,
-                                                         const double *__restrict__ b, double *__restrict__ out,
-                                                         int step, int *__restrict__ act, double *__restrict__ rz,
-                                                         double *__restrict__ ab, const double *__restrict__ bb,
-                                                         double rtol2, int *__restrict__ iters) {
-    __shared__ double sh[kDotOne / 64];
-    const int s = list ? list[blockIdx.x] : (int)blockIdx.x;  // with a list: its samples only
-    const long lo = off[s], hi = off[s + 1];
-    double acc = 0.0;
-    for (long i = lo + threadIdx.x; i < hi; i += kDotOne) acc += a[i] * b[i];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+// sum of sample s's P partials in increasing q; the loads of each block of
+// 16 issued together (one latency per block instead of one per partial)
+__device__ inline double parts_sum(const double *__restrict__ part, int s, int P) {
+    const double *ps = part + (long)s * P;
     double t = 0.0;
-    for (int k = 0; k < kDotOne / 64; ++k) t += sh[k];
-    out[s] = t;
-    if (step == 1) {
-        ab[s] = (act[s] && t != 0.0) ? rz[s] / t : 0.0;
-    } else if (act[s]) {
-        iters[s] += 1;
-        ab[s] = rz[s] != 0.0 ? t / rz[s] : 0.0;
-        rz[s] = t;
-        if (!(t > rtol2 * bb[s])) act[s] = 0;  // also stops on NaN
+    for (int q0 = 0; q0 < P; q0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = q0 + k < P ? ps[q0 + k] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (q0 + k < P) t += v[k];
     }
+    return t;
 }
 
 __global__ void k_seg_dot_fin(int S, int P, const double *__restrict__ part, double *__restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
-    double t = 0.0;
-    for (int q = 0; q < P; ++q) t += part[(long)s * P + q];
-    out[s] = t;
+    out[s] = parts_sum(part, s, P);
 }
 
 // copy a CSR block into a block-diagonal matrix at (row0, col0), its
@@ -623,10 +604,72 @@ __device__ inline double block_sum_part(double acc) {
     return t;
 }
 
-__device__ inline double parts_sum(const double *__restrict__ part, int s, int P) {
-    double t = 0.0;
-    for (int q = 0; q < P; ++q) t += part[(long)s * P + q];
-    return t;
+
+// Plain CG's scalar steps over all CUs: the two dots of an iteration as P
+// workgroup partials per sample, summed in a fixed order by their consumers.
+// Workgroup (q, listed sample): p . Ap over its slice.
+__global__ void __launch_bounds__(kDotThreads) k_cg_dot_part(const int64_t *__restrict__ off,
+                                                              const int *__restrict__ list,
+                                                              const double *__restrict__ a,
+                                                              const double *__restrict__ b, double *__restrict__ part,
+                                                              int P) {
+    const int s = list ? list[blockIdx.y] : (int)blockIdx.y, q = blockIdx.x;
+    long lo, hi;
+    part_slice(off, s, q, P, lo, hi);
+    double acc = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) acc += a[i] * b[i];
+    const double t = block_sum_part(acc);
+    if (threadIdx.x == 0) part[(long)s * P + q] = t;
+}
+
+// alpha = rz / p.Ap from the partials (each workgroup of the sample forms
+// it the same way), x += alpha p and r -= alpha Ap over the slice (active
+// samples), and the partials of the new r . r (of every listed sample)
+__global__ void __launch_bounds__(kDotThreads) k_cg_xr_part(const int64_t *__restrict__ off,
+                                                             const int *__restrict__ list,
+                                                             const int *__restrict__ act,
+                                                             const double *__restrict__ rz,
+                                                             const double *__restrict__ part_pap, int P,
+                                                             const double *__restrict__ p,
+                                                             const double *__restrict__ Ap, double *__restrict__ x,
+                                                             double *__restrict__ r, double *__restrict__ part_rr) {
+    const int s = list ? list[blockIdx.y] : (int)blockIdx.y, q = blockIdx.x;
+    long lo, hi;
+    part_slice(off, s, q, P, lo, hi);
+    double acc = 0.0;
+    if (act[s]) {
+        const double pap = parts_sum(part_pap, s, P);
+        const double a = pap != 0.0 ? rz[s] / pap : 0.0;
+        for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) {
+            x[i] += a * p[i];
+            const double ri = r[i] - a * Ap[i];
+            r[i] = ri;
+            acc += ri * ri;
+        }
+    } else {
+        for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) acc += r[i] * r[i];
+    }
+    const double t = block_sum_part(acc);
+    if (threadIdx.x == 0) part_rr[(long)s * P + q] = t;
+}
+
+// one thread per listed sample: rr from its partials; for active samples
+// the iteration count, beta = rr / rz, rz = rr and the convergence test on
+// the recurrence residual (as k_cg_beta with rz_new = rr)
+__global__ void k_cg_beta_part(const int *__restrict__ list, int n, const double *__restrict__ part_rr, int P,
+                               int *__restrict__ act, double *__restrict__ rz, double *__restrict__ beta,
+                               double *__restrict__ rr, const double *__restrict__ bb, double rtol2,
+                               int *__restrict__ iters) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = list ? list[k] : k;
+    const double t = parts_sum(part_rr, s, P);
+    rr[s] = t;
+    if (!act[s]) return;
+    iters[s] += 1;
+    beta[s] = rz[s] != 0.0 ? t / rz[s] : 0.0;
+    rz[s] = t;
+    if (!(t > rtol2 * bb[s])) act[s] = 0;  // also stops on NaN
 }
 
 // start: x = 0, r = r0, z = p = r0 / dM; partials of r . z
@@ -787,6 +830,11 @@ struct Solver {
     ~Solver() { drop_graphs(); }
     DevBuf<double> dotpart;  // S x dot_parts partial sums of seg_dot
     int dot_parts = 1;
+    // plain CG's dot partials: p . Ap, then r . r (2 x S x cg_parts; a
+    // member, so that its address stays fixed while captured graphs refer
+    // to it)
+    DevBuf<double> cgpart;
+    int cg_parts = 16;
     // out[s] = a_s . b_s for every sample s (deterministic order): one
     // workgroup per sample when there are many samples, else partial sums
     // over dot_parts workgroups per sample and a second pass
@@ -862,6 +910,7 @@ lt_status Solver::create(LensBatch *lb_, NufftPlan *nf_, const double *weights, 
     ntot = soff_host[S];
     LT_TRY(soff.upload(soff_host.data(), S + 1));
     dot_parts = S >= 32 ? 1 : std::max(1, std::min(64, (1024 + S - 1) / S));
+    cg_parts = std::max(1, std::min(64, (1024 + S - 1) / S));
     LT_TRY(dotpart.alloc((long)S * dot_parts));
     LT_TRY(seg.alloc(ntot));
     hipLaunchKernelGGL(k_seg_ids, dim3(64, S), dim3(256), 0, stream(), soff.p, S, seg.p);
@@ -1404,9 +1453,11 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     const bool plain = (precond == LT_PC_NONE);
     const double *zz = plain ? r.p : z.p;
     const double *rzn = plain ? rr.p : rz_new.p;
-    // plain CG with one workgroup per sample's dot: alpha and the beta step
-    // are formed by the dot kernels themselves (two launches fewer)
-    const bool fused = plain && dot_parts == 1;
+    // plain CG: p . Ap as partials over all CUs, then x, r and the r . r
+    // partials in one pass (alpha formed there), then the beta step (one
+    // thread per sample)
+    const bool fused = plain;
+    if (fused && cgpart.n < (size_t)2 * S * cg_parts) LT_TRY(cgpart.alloc((size_t)2 * S * cg_parts));
     long smax = 0;  // the largest source, for launches over listed samples
     for (int q = 0; q < S; ++q) smax = std::max<long>(smax, soff_host[q + 1] - soff_host[q]);
     const unsigned gsmax = blocks_for(std::max<long>(smax, 1), 256);
@@ -1464,6 +1515,17 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         LT_TRY(sync());
     }
     const double rtol2 = rtol * rtol;
+    auto plain_steps = [&](const int *list, int ny) -> lt_status {
+        double *p1 = cgpart.p, *p2 = cgpart.p + (long)S * cg_parts;
+        hipLaunchKernelGGL(k_cg_dot_part, dim3(cg_parts, ny), dim3(kDotThreads), 0, stream(), soff.p, list, p.p, Ap.p,
+                           p1, cg_parts);
+        hipLaunchKernelGGL(k_cg_xr_part, dim3(cg_parts, ny), dim3(kDotThreads), 0, stream(), soff.p, list, act.p,
+                           rz.p, p1, cg_parts, p.p, Ap.p, x.p, r.p, p2);
+        hipLaunchKernelGGL(k_cg_beta_part, dim3(blocks_for(ny, 64)), dim3(64), 0, stream(), list, ny, p2, cg_parts,
+                           act.p, rz.p, beta.p, rr.p, bb.p, rtol2, iters.p);
+        LT_CHECK_LAUNCH();
+        return LT_OK;
+    };
     std::vector<int> hact(S, 1), alist(S);
     int nact = S;
     for (int s = 0; s < S; ++s) alist[s] = s;
@@ -1487,25 +1549,19 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         // with fewer listed samples than samples, the updates and dots run
         // over the list only (same values)
         const bool part = nl < S;
-        const int *dl = (fused && part) ? list : nullptr;
         if (fused) {
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nl : S), dim3(kDotOne), 0, stream(), soff.p, dl, p.p, Ap.p,
-                               pAp.p, 1, act.p, rz.p, alpha.p, bb.p, rtol2, iters.p);
+            LT_TRY(plain_steps(part ? list : nullptr, part ? nl : S));
         } else {
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
             hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
-        }
-        if (part)
-            hipLaunchKernelGGL(k_cg_xr_sg, dim3(gsmax, nl), dim3(256), 0, stream(), list, soff.p, act.p, alpha.p,
-                               p.p, Ap.p, x.p, r.p);
-        else
-            hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p,
-                               r.p);
-        if (fused)
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nl : S), dim3(kDotOne), 0, stream(), soff.p, dl, r.p, r.p,
-                               rr.p, 2, act.p, rz.p, beta.p, bb.p, rtol2, iters.p);
-        else
+            if (part)
+                hipLaunchKernelGGL(k_cg_xr_sg, dim3(gsmax, nl), dim3(256), 0, stream(), list, soff.p, act.p, alpha.p,
+                                   p.p, Ap.p, x.p, r.p);
+            else
+                hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p,
+                                   x.p, r.p);
             LT_TRY(seg_dot(r.p, r.p, rr.p));
+        }
         LT_CHECK_LAUNCH();
         if (!plain) {
             LT_TRY(precondition(precond, dinv_p, list, nl));
@@ -1592,10 +1648,8 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         // with fewer listed samples than samples, the updates and dots run
         // over the list only (same values)
         const bool part = nact < S;
-        const int *dl = (fused && part) ? dlist.p : nullptr;
         if (fused) {
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nact : S), dim3(kDotOne), 0, stream(), soff.p, dl, p.p, Ap.p,
-                               pAp.p, 1, act.p, rz.p, alpha.p, bb.p, rtol2, iters.p);
+            LT_TRY(plain_steps(part ? dlist.p : nullptr, part ? nact : S));
         } else if (deep) {
             // alpha = p.r / p.A p; the direction joins the ring
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
@@ -1614,17 +1668,15 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
             hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         }
-        if (part)
-            hipLaunchKernelGGL(k_cg_xr_sg, dim3(gsmax, nact), dim3(256), 0, stream(), dlist.p, soff.p, act.p, alpha.p,
-                               p.p, Ap.p, x.p, r.p);
-        else
-            hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p, x.p,
-                               r.p);
-        if (fused)
-            hipLaunchKernelGGL(k_seg_dot_cg, dim3(dl ? nact : S), dim3(kDotOne), 0, stream(), soff.p, dl, r.p, r.p,
-                               rr.p, 2, act.p, rz.p, beta.p, bb.p, rtol2, iters.p);
-        else
+        if (!fused) {
+            if (part)
+                hipLaunchKernelGGL(k_cg_xr_sg, dim3(gsmax, nact), dim3(256), 0, stream(), dlist.p, soff.p, act.p,
+                                   alpha.p, p.p, Ap.p, x.p, r.p);
+            else
+                hipLaunchKernelGGL(k_cg_xr, dim3(gb), dim3(256), 0, stream(), ntot, seg.p, act.p, alpha.p, p.p, Ap.p,
+                                   x.p, r.p);
             LT_TRY(seg_dot(r.p, r.p, rr.p));
+        }
         LT_CHECK_LAUNCH();
         if (!plain) {
             LT_TRY(precondition(precond, dinv_p, dlist.p, nact));

@@ -2875,10 +2875,15 @@ static lt_status normal_band_impl(NufftPlan &P, NormalStencil &St, const TI *img
     else
         LT_TRY((stencil_apply_t<float, float>(St, P, St.coef32.p, Bf, bsz, Fin, nullptr, nactive, true)));
     // row v = 0 exactly Hermitian along u (its two halves are computed as
-    // separate rows)
-    hipLaunchKernelGGL((k_herm_band0<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0, stream(), P.nu,
-                       bsz, Fin);
-    LT_CHECK_LAUNCH();
+    // separate rows) for rocFFT's c2r; the fused inverse row transform takes
+    // the real part of column 0 after the column transform, which equals the
+    // transform of the Hermitian part (Re IFFT(X) = IFFT((X + conj X(-u)) / 2)),
+    // so it needs no such step
+    if (!fused) {
+        hipLaunchKernelGGL((k_herm_band0<T>), dim3(blocks_for(P.nu / 2 + 1, 256), nactive), dim3(256), 0, stream(),
+                           P.nu, bsz, Fin);
+        LT_CHECK_LAUNCH();
+    }
     if (own_cols) {
         hipLaunchKernelGGL((k_cols_inv_r<T>), dim3((unsigned)((long)nactive * nbv)), dim3(256), 0, stream(), nbv, P.nu,
                            P.padx, P.nx, bsz, Fin, Fo, twiddles_of<T>(P));
