@@ -265,13 +265,14 @@ def profile_kernels(reps=10):
                       'copy_bytes': nbytes, 'copy_ms': cms, 'copy_gbs': gbs, 'copy_launches': reps + 1}), flush=True)
 
 
-def load_traffic(name):
-    """HBM bytes per launch from a committed counter summary under profiles/,
-    or None."""
+def load_traffic(name, key='traffic_bytes_per_launch'):
+    """HBM bytes per launch from a committed counter summary under profiles/
+    (the entry `key`, else the default one), or None."""
     path = os.path.join(ROOT, 'profiles', name)
     try:
         with open(path) as f:
-            return json.load(f).get('traffic_bytes_per_launch')
+            d = json.load(f)
+        return d.get(key, d.get('traffic_bytes_per_launch'))
     except (OSError, ValueError):
         return None
 
@@ -847,13 +848,16 @@ def main():
         'degrid_gvis_s': rl['gvis_s'], 'degrid_gvis_s_f64': rl['gvis_s_f64'],
         'roofline': {'kernel': st['kernel'],
                      'bound': 'mfma', 'achieved': st['tflops'], 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': st['tflops'] / FP64_PEAK_TFLOPS, 'traffic': load_traffic('stencil_c2_pmc.json'),
+                     'frac': st['tflops'] / FP64_PEAK_TFLOPS,
+                     'traffic': load_traffic('stencil_c2_pmc.json',
+                                             'traffic_bytes_per_launch_2_groups_per_workgroup' if st['form'] == -2
+                                             else 'traffic_bytes_per_launch'),
                      'algorithmic_flops': st['flops'], 'algorithmic_bytes': st['bytes'],
                      'achieved_gbs': st['gbs'], 'us_per_launch': st['ms'] * 1e3,
                      'compute_unit': 'FP64 matrix cores (MI355X FP64 matrix and vector peaks are both 78.6 '
                                      'TFLOP/s); intensity above the FP64 ridge, so FP64 throughput bounds it; the '
                                      'matrix form also multiplies the zeros of each 4x4 tile of rows against its '
-                                     '16x16 window (executed MFMA work 1.75x the algorithmic flops, counted)'},
+                                     '16x16 window (executed MFMA work 1.73x the algorithmic flops, counted)'},
         'roofline_spmv': {'kernel': 'L^T y (k_spmv_stream + long-row chunks, FP64, all 64 samples)', 'bound': 'hbm',
                           'achieved': sp['LT']['gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                           'frac': sp['LT']['gbs'] / HBM_PEAK_GBS, 'traffic': None,
