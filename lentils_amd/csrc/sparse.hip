@@ -38,7 +38,8 @@ struct RowFilter {
     const int32_t *rowseg;
     __device__ bool skip(long row) const {
         if (!act) return false;
-        const long s = rowseg ? (long)rowseg[row] : row / rps;
+        // rows and rows per sample fit 32 bits (int32 row pointers)
+        const long s = rowseg ? (long)rowseg[row] : (long)((uint32_t)row / (uint32_t)rps);
         return act[s] == 0;
     }
 };
@@ -57,6 +58,40 @@ __device__ inline void row_spmv(long row, int lane, const int32_t *__restrict__ 
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
     if (lane == 0) y[row] = (beta == (T)0) ? alpha * s : alpha * s + beta * y[row];
+}
+
+// The same for the nonempty-row list (alpha 1, beta 0): a row of at most 64
+// entries has every lane's loads (at most 64 / LANES) issued together, then
+// the gathers, then the sum in the loop's order (the same bits); longer rows
+// not left to the other paths take the loop.  `store` false computes
+// without writing (a filtered row: its loads go out before the filter is
+// known).
+template <int LANES, class T>
+__device__ inline void row_spmv_short(long row, int lane, const int32_t *__restrict__ rp,
+                                      const int32_t *__restrict__ ci, const T *__restrict__ va,
+                                      const T *__restrict__ x, T *__restrict__ y, int skip_long, bool store) {
+    constexpr int PER = 64 / LANES;
+    const int b = rp[row], e = rp[row + 1];
+    if (skip_long > 0 && e - b > skip_long) return;  // summed by the medium-row / chunked paths
+    if (e - b > 64) {  // whole lane groups
+        if (store) row_spmv<LANES, T>(row, lane, rp, ci, va, x, y, (T)1, (T)0, skip_long);
+        return;
+    }
+    int c[PER];
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int k = b + lane + LANES * u;
+        c[u] = k < e ? ci[k] : -1;
+        v[u] = k < e ? va[k] : (T)0;
+    }
+    T s = (T)0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (c[u] >= 0) s += v[u] * x[c[u]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
+    if (lane == 0 && store) y[row] = s;
 }
 
 template <int LANES, class T>
@@ -98,8 +133,7 @@ __global__ void __launch_bounds__(256) k_spmv_rows(long i0, long n, const int32_
     const long i = i0 + gid / LANES;
     if (i >= n) return;
     const long row = nzr[i];
-    if (F.skip(row)) return;
-    row_spmv<LANES, T>(row, (int)(gid % LANES), rp, ci, va, x, y, (T)1, (T)0, skip_long);
+    row_spmv_short<LANES, T>(row, (int)(gid % LANES), rp, ci, va, x, y, skip_long, !F.skip(row));
 }
 
 // list entries of sample alist[y]: [snzr[s], snzr[s+1])
@@ -114,7 +148,7 @@ __global__ void __launch_bounds__(256) k_spmv_rows_sg(const int *__restrict__ al
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long i = snzr[s] + gid / LANES;
     if (i >= snzr[s + 1]) return;
-    row_spmv<LANES, T>(nzr[i], (int)(gid % LANES), rp, ci, va, x, y, (T)1, (T)0, skip_long);
+    row_spmv_short<LANES, T>(nzr[i], (int)(gid % LANES), rp, ci, va, x, y, skip_long, true);
 }
 
 // Rows of 1..4 entries, one thread per row of the padded table: the row
@@ -138,9 +172,18 @@ __global__ void __launch_bounds__(256) k_spmv_ell4(long n, const int32_t *__rest
                                                     const T *__restrict__ x, T *__restrict__ y, RowFilter F) {
     const long i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // the entry's loads go out with the row's (all in bounds whatever the
+    // filter says) and the gathers with the filter's: two memory round trips
+    // before the store instead of four
     const long row = erow[i];
-    if (F.skip(row)) return;
-    ell4_row<T>(i, row, eci, eva, x, y);
+    const int4 c = eci[i];
+    const T v0 = eva[4 * i], v1 = eva[4 * i + 1], v2 = eva[4 * i + 2], v3 = eva[4 * i + 3];
+    const bool sk = F.skip(row);
+    T s = v0 * x[c.x];
+    s += v1 * x[c.y];
+    s += v2 * x[c.z];
+    s += v3 * x[c.w];
+    if (!sk) y[row] = s;
 }
 
 // table entries of sample alist[y]: [sell[s], sell[s+1])
