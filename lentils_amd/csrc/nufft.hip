@@ -1507,6 +1507,14 @@ __device__ inline Cplx<T> tw_mul(Cplx<T> a, Cplx<T> w) {
     return Cplx<T>{fma(a.re, w.re, -(a.im * wi)), fma(a.re, wi, a.im * w.re)};
 }
 
+// a b with the same explicit fma form: the radix-4 passes read w = w^k
+// from their table and form w^2 = w w and w^3 = w w^2 (one table read per
+// butterfly instead of three; every transform derives them alike)
+template <class T>
+__device__ inline Cplx<T> tw_sq(Cplx<T> a, Cplx<T> b) {
+    return Cplx<T>{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
+}
+
 // one Stockham stage of radix R over PAIRS sequences of length N in buf
 template <class T, int N, int R, int PAIRS, bool INV>
 __device__ inline void stockham_stage(Cplx<T> (*buf)[N], int Ns, const Cplx<T> *__restrict__ twd) {
@@ -1519,9 +1527,14 @@ __device__ inline void stockham_stage(Cplx<T> (*buf)[N], int Ns, const Cplx<T> *
         if (g < NB) {
             const int pr = g / (N / R), j = g - pr * (N / R), k = j % Ns;
             const int step = N / (Ns * R);
+            Cplx<T> w[R];
+            w[0] = twd[0];
+            if (R > 1) w[1] = twd[k * step];
+            if (R > 2) w[2] = tw_sq(w[1], w[1]);
+            if (R > 3) w[3] = tw_sq(w[1], w[2]);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                v[q][r] = tw_mul<T, INV>(buf[pr][j + r * (N / R)], twd[k * r * step]);
+                v[q][r] = tw_mul<T, INV>(buf[pr][j + r * (N / R)], w[r]);
             }
         }
     }
@@ -1724,9 +1737,10 @@ __device__ inline void r4_pass(Cplx<T> *buf, const Cplx<T> *tq) {
     const int j = threadIdx.x, k = j & (NS - 1);
     constexpr int step = kR4N / (NS * 4);
     __syncthreads();  // the previous pass's writes (and the twiddle table's)
-    Cplx<T> w[3];
-#pragma unroll
-    for (int r = 1; r < 4; ++r) w[r - 1] = r4tw(tq, k * r * step);
+    Cplx<T> w[3];  // w^k, w^2k, w^3k
+    w[0] = r4tw(tq, k * step);
+    w[1] = tw_sq(w[0], w[0]);
+    w[2] = tw_sq(w[0], w[1]);
     Cplx<T> v[4];
     v[0] = buf[r4swz<SI>(j)];
 #pragma unroll
@@ -1743,9 +1757,10 @@ template <class T, bool INV>
 __device__ inline void r4_last(Cplx<T> (&v)[4], const Cplx<T> *buf, const Cplx<T> *tq) {
     const int j = threadIdx.x;
     __syncthreads();
-    Cplx<T> w[3];
-#pragma unroll
-    for (int r = 1; r < 4; ++r) w[r - 1] = r4tw(tq, j * r);
+    Cplx<T> w[3];  // w^k, w^2k, w^3k
+    w[0] = r4tw(tq, j);
+    w[1] = tw_sq(w[0], w[0]);
+    w[2] = tw_sq(w[0], w[1]);
     v[0] = buf[j];
 #pragma unroll
     for (int r = 1; r < 4; ++r) v[r] = tw_mul<T, INV>(buf[j + 256 * r], w[r - 1]);
