@@ -632,14 +632,17 @@ __global__ void __launch_bounds__(kDotThreads) k_cg_xr_part(const int64_t *__res
                                                              const double *__restrict__ part_pap, int P,
                                                              const double *__restrict__ p,
                                                              const double *__restrict__ Ap, double *__restrict__ x,
-                                                             double *__restrict__ r, double *__restrict__ part_rr) {
+                                                             double *__restrict__ r, double *__restrict__ part_rr,
+                                                             double *__restrict__ alpha_out) {
     const int s = list ? list[blockIdx.y] : (int)blockIdx.y, q = blockIdx.x;
     long lo, hi;
     part_slice(off, s, q, P, lo, hi);
     double acc = 0.0;
+    if (alpha_out && q == 0 && threadIdx.x == 0 && !act[s]) alpha_out[s] = 0.0;
     if (act[s]) {
         const double pap = parts_sum(part_pap, s, P);
         const double a = pap != 0.0 ? rz[s] / pap : 0.0;
+        if (alpha_out && q == 0 && threadIdx.x == 0) alpha_out[s] = a;
         for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) {
             x[i] += a * p[i];
             const double ri = r[i] - a * Ap[i];
@@ -651,6 +654,56 @@ __global__ void __launch_bounds__(kDotThreads) k_cg_xr_part(const int64_t *__res
     }
     const double t = block_sum_part(acc);
     if (threadIdx.x == 0) part_rr[(long)s * P + q] = t;
+}
+
+// r . z and z . Ap of flexible CG in one pass: partials of a . b and c . d
+__global__ void __launch_bounds__(kDotThreads) k_cg_dot2_part(const int64_t *__restrict__ off,
+                                                               const int *__restrict__ list,
+                                                               const double *__restrict__ a,
+                                                               const double *__restrict__ b,
+                                                               const double *__restrict__ c,
+                                                               const double *__restrict__ d,
+                                                               double *__restrict__ part_ab,
+                                                               double *__restrict__ part_cd, int P) {
+    const int s = list ? list[blockIdx.y] : (int)blockIdx.y, q = blockIdx.x;
+    long lo, hi;
+    part_slice(off, s, q, P, lo, hi);
+    double acc1 = 0.0, acc2 = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) {
+        acc1 += a[i] * b[i];
+        acc2 += c[i] * d[i];
+    }
+    const double t1 = block_sum_part(acc1);
+    __syncthreads();  // block_sum_part's shared slots are reused
+    const double t2 = block_sum_part(acc2);
+    if (threadIdx.x == 0) {
+        part_ab[(long)s * P + q] = t1;
+        part_cd[(long)s * P + q] = t2;
+    }
+}
+
+// flexible CG's beta step from the partials (one thread per listed
+// sample): rr; for active samples the iteration count, r . z and z . Ap,
+// beta = -alpha z.Ap / rz, rz = r . z and the convergence test on rr (as
+// k_cg_beta_flex)
+__global__ void k_cg_beta_flex_part(const int *__restrict__ list, int n, const double *__restrict__ part_rr,
+                                    const double *__restrict__ part_rz, const double *__restrict__ part_zap, int P,
+                                    int *__restrict__ act, double *__restrict__ rz, double *__restrict__ rz_new,
+                                    const double *__restrict__ alpha, double *__restrict__ beta,
+                                    double *__restrict__ rr, const double *__restrict__ bb, double rtol2,
+                                    int *__restrict__ iters) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = list ? list[k] : k;
+    const double t = parts_sum(part_rr, s, P);
+    rr[s] = t;
+    if (!act[s]) return;
+    iters[s] += 1;
+    const double rzn = parts_sum(part_rz, s, P), zap = parts_sum(part_zap, s, P);
+    rz_new[s] = rzn;
+    beta[s] = rz[s] != 0.0 ? -alpha[s] * zap / rz[s] : 0.0;
+    rz[s] = rzn;
+    if (!(t > rtol2 * bb[s])) act[s] = 0;
 }
 
 // one thread per listed sample: rr from its partials; for active samples
@@ -1457,13 +1510,15 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
     // partials in one pass (alpha formed there), then the beta step (one
     // thread per sample)
     const bool fused = plain;
-    if (fused && cgpart.n < (size_t)2 * S * cg_parts) LT_TRY(cgpart.alloc((size_t)2 * S * cg_parts));
+
     long smax = 0;  // the largest source, for launches over listed samples
     for (int q = 0; q < S; ++q) smax = std::max<long>(smax, soff_host[q + 1] - soff_host[q]);
     const unsigned gsmax = blocks_for(std::max<long>(smax, 1), 256);
     if (precond == LT_PC_LENS || precond == LT_PC_LEARNED) LT_TRY(lens_setup());
     if (flexible && zap.n < (size_t)S) LT_TRY(zap.alloc(S));  // z . A p of the flexible beta
     const bool deep = flexible && fcg_depth > 1 && !mixed;
+    const bool flexp = flexible && !deep;  // flexible CG with the partial-sum steps
+    if ((fused || flexp) && cgpart.n < (size_t)4 * S * cg_parts) LT_TRY(cgpart.alloc((size_t)4 * S * cg_parts));
     int nstored = 0;  // directions in the ring (truncated flexible CG)
     if (deep) {
         const long dn = (long)fcg_depth * ntot, ds = (long)fcg_depth * S;
@@ -1520,9 +1575,31 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         hipLaunchKernelGGL(k_cg_dot_part, dim3(cg_parts, ny), dim3(kDotThreads), 0, stream(), soff.p, list, p.p, Ap.p,
                            p1, cg_parts);
         hipLaunchKernelGGL(k_cg_xr_part, dim3(cg_parts, ny), dim3(kDotThreads), 0, stream(), soff.p, list, act.p,
-                           rz.p, p1, cg_parts, p.p, Ap.p, x.p, r.p, p2);
+                           rz.p, p1, cg_parts, p.p, Ap.p, x.p, r.p, p2, (double *)nullptr);
         hipLaunchKernelGGL(k_cg_beta_part, dim3(blocks_for(ny, 64)), dim3(64), 0, stream(), list, ny, p2, cg_parts,
                            act.p, rz.p, beta.p, rr.p, bb.p, rtol2, iters.p);
+        LT_CHECK_LAUNCH();
+        return LT_OK;
+    };
+    // flexible CG (one direction): the same first half, alpha kept for the
+    // beta step; after the preconditioner r . z and z . Ap in one pass, then
+    // the beta step from the partials
+    auto flex_steps_a = [&](const int *list, int ny) -> lt_status {
+        double *p1 = cgpart.p, *p2 = cgpart.p + (long)S * cg_parts;
+        hipLaunchKernelGGL(k_cg_dot_part, dim3(cg_parts, ny), dim3(kDotThreads), 0, stream(), soff.p, list, p.p, Ap.p,
+                           p1, cg_parts);
+        hipLaunchKernelGGL(k_cg_xr_part, dim3(cg_parts, ny), dim3(kDotThreads), 0, stream(), soff.p, list, act.p,
+                           rz.p, p1, cg_parts, p.p, Ap.p, x.p, r.p, p2, alpha.p);
+        LT_CHECK_LAUNCH();
+        return LT_OK;
+    };
+    auto flex_steps_b = [&](const int *list, int ny) -> lt_status {
+        double *p2 = cgpart.p + (long)S * cg_parts, *p3 = cgpart.p + 2L * S * cg_parts,
+               *p4 = cgpart.p + 3L * S * cg_parts;
+        hipLaunchKernelGGL(k_cg_dot2_part, dim3(cg_parts, ny), dim3(kDotThreads), 0, stream(), soff.p, list, r.p, z.p,
+                           z.p, Ap.p, p3, p4, cg_parts);
+        hipLaunchKernelGGL(k_cg_beta_flex_part, dim3(blocks_for(ny, 64)), dim3(64), 0, stream(), list, ny, p2, p3, p4,
+                           cg_parts, act.p, rz.p, rz_new.p, alpha.p, beta.p, rr.p, bb.p, rtol2, iters.p);
         LT_CHECK_LAUNCH();
         return LT_OK;
     };
@@ -1551,6 +1628,8 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         const bool part = nl < S;
         if (fused) {
             LT_TRY(plain_steps(part ? list : nullptr, part ? nl : S));
+        } else if (flexp) {
+            LT_TRY(flex_steps_a(part ? list : nullptr, part ? nl : S));
         } else {
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
             hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
@@ -1565,9 +1644,11 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         LT_CHECK_LAUNCH();
         if (!plain) {
             LT_TRY(precondition(precond, dinv_p, list, nl));
-            LT_TRY(seg_dot(r.p, z.p, rz_new.p));
+            if (!flexp) LT_TRY(seg_dot(r.p, z.p, rz_new.p));
         }
-        if (flexible) {
+        if (flexp) {
+            LT_TRY(flex_steps_b(part ? list : nullptr, part ? nl : S));
+        } else if (flexible) {
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));
             hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
                                alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
@@ -1650,6 +1731,8 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         const bool part = nact < S;
         if (fused) {
             LT_TRY(plain_steps(part ? dlist.p : nullptr, part ? nact : S));
+        } else if (flexp) {
+            LT_TRY(flex_steps_a(part ? dlist.p : nullptr, part ? nact : S));
         } else if (deep) {
             // alpha = p.r / p.A p; the direction joins the ring
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
@@ -1668,7 +1751,7 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
             LT_TRY(seg_dot(p.p, Ap.p, pAp.p));
             hipLaunchKernelGGL(k_cg_alpha, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, pAp.p, alpha.p);
         }
-        if (!fused) {
+        if (!fused && !flexp) {
             if (part)
                 hipLaunchKernelGGL(k_cg_xr_sg, dim3(gsmax, nact), dim3(256), 0, stream(), dlist.p, soff.p, act.p,
                                    alpha.p, p.p, Ap.p, x.p, r.p);
@@ -1680,9 +1763,11 @@ lt_status Solver::solve(double rtol, int maxiter, int precond, double *xout, int
         LT_CHECK_LAUNCH();
         if (!plain) {
             LT_TRY(precondition(precond, dinv_p, dlist.p, nact));
-            LT_TRY(seg_dot(r.p, z.p, rz_new.p));
+            if (!flexp) LT_TRY(seg_dot(r.p, z.p, rz_new.p));
         }
-        if (flexible) {
+        if (flexp) {
+            LT_TRY(flex_steps_b(part ? dlist.p : nullptr, part ? nact : S));
+        } else if (flexible) {
             LT_TRY(seg_dot(z.p, Ap.p, zap.p));
             hipLaunchKernelGGL(k_cg_beta_flex, dim3(gs), dim3(64), 0, stream(), S, act.p, rz.p, rz_new.p, zap.p,
                                alpha.p, rr.p, bb.p, rtol2, beta.p, iters.p);
