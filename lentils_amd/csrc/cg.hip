@@ -43,9 +43,10 @@ constexpr int kDotThreads = 256;
 // (fixed-order block reduction), then one thread per segment adds its parts
 // in order.  The order depends only on the segment bounds and the part
 // count, so results are run-to-run identical.
+template <class TV = double>
 __global__ void __launch_bounds__(kDotThreads) k_seg_dot_part(const int64_t *__restrict__ off,
-                                                               const double *__restrict__ a,
-                                                               const double *__restrict__ b,
+                                                               const TV *__restrict__ a,
+                                                               const TV *__restrict__ b,
                                                                double *__restrict__ part) {
     __shared__ double sh[kDotThreads / 64];
     const int s = blockIdx.y, P = gridDim.x, q = blockIdx.x;
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(kDotThreads) k_seg_dot_part(const int64_t *__r
     const long per = (len + P - 1) / P;
     const long lo = b0 + min(len, per * q), hi = b0 + min(len, per * (q + 1));
     double acc = 0.0;
-    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) acc += a[i] * b[i];
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) acc += (double)a[i] * (double)b[i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
@@ -725,38 +726,42 @@ __global__ void k_cg_beta_part(const int *__restrict__ list, int n, const double
     if (!(t > rtol2 * bb[s])) act[s] = 0;  // also stops on NaN
 }
 
-// start: x = 0, r = r0, z = p = r0 / dM; partials of r . z
+// The inner solve's vectors r, z, p, Mp in TV (FP32: the preconditioner
+// only approximates M^-1 r), its iterate x (the outer z) and its dots in
+// FP64.  start: x = 0, r = r0, z = p = r0 / dM; partials of r . z
+template <class TV>
 __global__ void __launch_bounds__(kDotThreads) k_inner_start_part(const int64_t *__restrict__ off,
                                                                    const double *__restrict__ r0,
                                                                    const double *__restrict__ dM,
-                                                                   double *__restrict__ x, double *__restrict__ r,
-                                                                   double *__restrict__ z, double *__restrict__ p,
+                                                                   double *__restrict__ x, TV *__restrict__ r,
+                                                                   TV *__restrict__ z, TV *__restrict__ p,
                                                                    double *__restrict__ part_rz) {
     const int s = blockIdx.y, q = blockIdx.x, P = gridDim.x;
     long lo, hi;
     part_slice(off, s, q, P, lo, hi);
     double acc = 0.0;
     for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) {
-        const double ri = r0[i], zi = ri / dM[i];
+        const TV ri = (TV)r0[i], zi = (TV)((double)ri / dM[i]);
         x[i] = 0.0;
         r[i] = ri;
         z[i] = zi;
         p[i] = zi;
-        acc += ri * zi;
+        acc += (double)ri * (double)zi;
     }
     const double t = block_sum_part(acc);
     if (threadIdx.x == 0) part_rz[(long)s * P + q] = t;
 }
 
 // step: a = rz / pMp; x += a p; r -= a Mp; z = r / dM; partials of r . z
+template <class TV>
 __global__ void __launch_bounds__(kDotThreads) k_inner_step_part(const int64_t *__restrict__ off,
                                                                   const double *__restrict__ part_rz,
                                                                   const double *__restrict__ part_pMp,
-                                                                  const double *__restrict__ p,
-                                                                  const double *__restrict__ Mp,
+                                                                  const TV *__restrict__ p,
+                                                                  const TV *__restrict__ Mp,
                                                                   const double *__restrict__ dM,
-                                                                  double *__restrict__ x, double *__restrict__ r,
-                                                                  double *__restrict__ z,
+                                                                  double *__restrict__ x, TV *__restrict__ r,
+                                                                  TV *__restrict__ z,
                                                                   double *__restrict__ part_rz_new) {
     __shared__ double a_sh;
     const int s = blockIdx.y, q = blockIdx.x, P = gridDim.x;
@@ -770,22 +775,23 @@ __global__ void __launch_bounds__(kDotThreads) k_inner_step_part(const int64_t *
     part_slice(off, s, q, P, lo, hi);
     double acc = 0.0;
     for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) {
-        x[i] += a * p[i];
-        const double ri = r[i] - a * Mp[i];
+        x[i] += a * (double)p[i];
+        const TV ri = (TV)((double)r[i] - a * (double)Mp[i]);
         r[i] = ri;
-        const double zi = ri / dM[i];
+        const TV zi = (TV)((double)ri / dM[i]);
         z[i] = zi;
-        acc += ri * zi;
+        acc += (double)ri * (double)zi;
     }
     const double t = block_sum_part(acc);
     if (threadIdx.x == 0) part_rz_new[(long)s * P + q] = t;
 }
 
 // p = z + (rz_new / rz) p
+template <class TV>
 __global__ void __launch_bounds__(kDotThreads) k_inner_p_part(const int64_t *__restrict__ off,
                                                                const double *__restrict__ part_rz,
                                                                const double *__restrict__ part_rz_new,
-                                                               const double *__restrict__ z, double *__restrict__ p) {
+                                                               const TV *__restrict__ z, TV *__restrict__ p) {
     __shared__ double b_sh;
     const int s = blockIdx.y, q = blockIdx.x, P = gridDim.x;
     if (threadIdx.x == 0) {
@@ -796,7 +802,7 @@ __global__ void __launch_bounds__(kDotThreads) k_inner_p_part(const int64_t *__r
     const double b = b_sh;
     long lo, hi;
     part_slice(off, s, q, P, lo, hi);
-    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) p[i] = z[i] + b * p[i];
+    for (long i = lo + threadIdx.x; i < hi; i += kDotThreads) p[i] = (TV)((double)z[i] + b * (double)p[i]);
 }
 
 // samples whose network was switched off take the lens-structured direction
@@ -897,7 +903,7 @@ struct Solver {
             LT_CHECK_LAUNCH();
             return LT_OK;
         }
-        hipLaunchKernelGGL(k_seg_dot_part, dim3(dot_parts, S), dim3(kDotThreads), 0, stream(), soff.p, a, b,
+        hipLaunchKernelGGL(k_seg_dot_part<double>, dim3(dot_parts, S), dim3(kDotThreads), 0, stream(), soff.p, a, b,
                            dotpart.p);
         LT_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_seg_dot_fin, dim3(blocks_for(S, 64)), dim3(64), 0, stream(), S, dot_parts, dotpart.p,
@@ -1296,20 +1302,30 @@ lt_status Solver::lens_apply(const int *af, const int *alist, int nact) {
     double *rzc = ipart_rz.p, *rzn = ipart_rz2.p;
     const int P = inner_parts;
     const dim3 grid((unsigned)P, (unsigned)S);
-    hipLaunchKernelGGL(k_inner_start_part, grid, dim3(kDotThreads), 0, stream(), soff.p, r.p, dM.p, xi, ir.p, iz.p,
-                       ip.p, rzc);
-    LT_CHECK_LAUNCH();
-    for (int k = 0; k < lens_inner; ++k) {
-        // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc (FP32 values)
-        LT_TRY(csr_spmv_fv(Mpc, ip.p, iMp.p, af, seg.p));
-        hipLaunchKernelGGL(k_seg_dot_part, grid, dim3(kDotThreads), 0, stream(), soff.p, ip.p, iMp.p, ipart_pMp.p);
-        hipLaunchKernelGGL(k_inner_step_part, grid, dim3(kDotThreads), 0, stream(), soff.p, rzc, ipart_pMp.p, ip.p,
-                           iMp.p, dM.p, xi, ir.p, iz.p, rzn);
+    if (lens_inner == 0) {
+        hipLaunchKernelGGL(k_inner_start_part<double>, grid, dim3(kDotThreads), 0, stream(), soff.p, r.p, dM.p, xi,
+                           ir.p, iz.p, ip.p, rzc);
         LT_CHECK_LAUNCH();
-        if (k + 1 < lens_inner) {
-            hipLaunchKernelGGL(k_inner_p_part, grid, dim3(kDotThreads), 0, stream(), soff.p, rzc, rzn, iz.p, ip.p);
+    } else {
+        // FP32 inner vectors; M's products with its FP32 values
+        float *r32 = (float *)ir.p, *z32 = (float *)iz.p, *p32 = (float *)ip.p, *Mp32 = (float *)iMp.p;
+        hipLaunchKernelGGL(k_inner_start_part<float>, grid, dim3(kDotThreads), 0, stream(), soff.p, r.p, dM.p, xi,
+                           r32, z32, p32, rzc);
+        LT_CHECK_LAUNCH();
+        for (int k = 0; k < lens_inner; ++k) {
+            // Mp with the explicit M = c L_s^T L_s + lam_c Cc^T Cc
+            LT_TRY(csr_spmv_ff(Mpc, p32, Mp32, af, seg.p));
+            hipLaunchKernelGGL(k_seg_dot_part<float>, grid, dim3(kDotThreads), 0, stream(), soff.p, p32, Mp32,
+                               ipart_pMp.p);
+            hipLaunchKernelGGL(k_inner_step_part<float>, grid, dim3(kDotThreads), 0, stream(), soff.p, rzc,
+                               ipart_pMp.p, p32, Mp32, dM.p, xi, r32, z32, rzn);
             LT_CHECK_LAUNCH();
-            std::swap(rzc, rzn);
+            if (k + 1 < lens_inner) {
+                hipLaunchKernelGGL(k_inner_p_part<float>, grid, dim3(kDotThreads), 0, stream(), soff.p, rzc, rzn, z32,
+                                   p32);
+                LT_CHECK_LAUNCH();
+                std::swap(rzc, rzn);
+            }
         }
     }
     if (lens_inner == 0) {

@@ -195,6 +195,8 @@ lt_status csr_make_f32(DevCSR &A);
 // the values' rounding only changes the preconditioner.  Rows of samples with
 // act == 0 (rowseg: row -> sample) are skipped.
 lt_status csr_spmv_fv(DevCSR &A, const double *x, double *y, const int *act, const int32_t *rowseg);
+// y = A x in FP32 (FP32 values and vectors), the streaming kernel, no long-row path
+lt_status csr_spmv_ff(DevCSR &A, const float *x, float *y, const int *act, const int32_t *rowseg);
 // pick the fastest lane-group width for this matrix by timing (FP64 or FP32)
 lt_status csr_autotune(DevCSR &A, bool fp32);
 lt_status csr_spmv32(const DevCSR &A, const float *x, float *y, float alpha, float beta_y, const int *act = nullptr,

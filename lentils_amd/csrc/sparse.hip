@@ -886,6 +886,20 @@ lt_status csr_spmv_fv(DevCSR &A, const double *x, double *y, const int *act, con
     return LT_OK;
 }
 
+lt_status csr_spmv_ff(DevCSR &A, const float *x, float *y, const int *act, const int32_t *rowseg) {
+    if (A.nrows == 0) return LT_OK;
+    if (!A.prepared || A.nblk <= 0) {
+        set_error("spmv: the FP32 streaming product needs a prepared matrix");
+        return LT_ERR_ARG;
+    }
+    if (!A.va32.p) LT_TRY(csr_make_f32(A));
+    const RowFilter F{act, 1, rowseg};
+    hipLaunchKernelGGL((k_spmv_stream<float, float>), dim3((unsigned)A.nblk), dim3(256), 0, stream(), A.nnz, A.rblk.p,
+                       A.rp.p, A.ci.p, A.va32.p, x, y, 1.f, 0.f, 0, F);
+    LT_CHECK_LAUNCH();
+    return LT_OK;
+}
+
 lt_status csr_spmv_nz(const DevCSR &A, const double *x, double *y, const int *act, long rows_per_sample,
                       const int *alist, int nact) {
     return spmv_t<double>(A, A.va.p, A.chunk_sum.p, x, y, 1.0, 0.0, A.lanes_nz > 0 ? A.lanes_nz : 4, act,
