@@ -38,7 +38,10 @@ struct LensBatch {
             LT_TRY(csr_prepare(Lp));
             LT_TRY(csr_nonempty_rows(Lp));
             LT_TRY(csr_transpose(Lp, LT));
-            LT.long_row = 64;  // one thread or lane group per row up to 64 entries, chunks beyond
+            // rows up to 256 entries summed where they are (the streaming
+            // kernel stages 256 nonzeros past its range), 257..512 one wave
+            // each, chunks beyond
+            LT.long_row = 256;
             LT_TRY(csr_prepare(LT));
         }
         return LT_OK;

@@ -323,7 +323,7 @@ __global__ void k_row_split(long n, const int32_t *__restrict__ rp, const int32_
 // left to the medium-row and chunked paths.  With a row filter, workgroups
 // whose rows all belong to inactive samples return at once.
 constexpr int kStreamNnz = 2048;
-constexpr int kStreamTail = 64;
+constexpr int kStreamTail = 256;
 
 __device__ inline long row_sample(const RowFilter &F, long row) {
     return F.rowseg ? (long)F.rowseg[row] : row / F.rps;
@@ -1027,6 +1027,9 @@ static lt_status tune_lanes(DevCSR &A, const T *va, T *chunk_sum, bool nz, int &
 // presence of long rows) reuse the first measurement, so rebuilding lens
 // matrices of a new batch of the same geometry does not re-run the sweep.
 namespace {
+// shape class of a product: device, precision, mean row length (quarter
+// octaves), long rows present, product over the nonempty rows, row count
+// (half octaves: successive lens batches of one workload share a class)
 struct TuneKey {
     int dev, fp32, lenq, has_long, nz;
     long nrows;
@@ -1043,8 +1046,10 @@ lt_status csr_autotune(DevCSR &A, bool fp32) {
     if (A.nrows == 0 || A.nnz == 0) return LT_OK;
     if (!A.prepared) LT_TRY(csr_prepare(A));
     // the sliced copy is a candidate for matrices of several entries per row
-    // (not for those multiplied over their nonempty rows)
-    if (A.nnzr < 0 && (double)A.nnz >= 4.0 * (double)A.nrows) LT_TRY(csr_make_slices(A));
+    // (not for those multiplied over their nonempty rows); built only when
+    // tuning or when a memoized choice needs it
+    // (slices as wide as their longest row: only for rows of up to 64 entries)
+    const bool slice_cand = A.nnzr < 0 && (double)A.nnz >= 4.0 * (double)A.nrows && A.long_row <= 64;
     if (fp32) LT_TRY(csr_make_f32(A));
     int dev = 0;
     LT_HIP(hipGetDevice(&dev));
@@ -1052,16 +1057,29 @@ lt_status csr_autotune(DevCSR &A, bool fp32) {
     // product over the listed rows, each with its own memo entry
     for (int nz = 0; nz <= (A.nnzr >= 0 ? 1 : 0); ++nz) {
         const double mean = (double)A.nnz / (double)(nz ? std::max(1L, A.nnzr) : A.nrows);
-        const TuneKey key{dev, fp32 ? 1 : 0, (int)std::lround(4.0 * std::log2(mean)), (A.nlong > 0 || A.nmed > 0) ? 1 : 0, nz,
-                          nz ? A.nnzr : A.nrows};
+        const long rows = std::max(1L, nz ? A.nnzr : A.nrows);
+        const TuneKey key{dev, fp32 ? 1 : 0, (int)std::lround(4.0 * std::log2(mean)),
+                          (A.nlong > 0 || A.nmed > 0) ? 1 : 0, nz, std::lround(2.0 * std::log2((double)rows))};
         int &slot = nz ? (fp32 ? A.lanes_nz32 : A.lanes_nz) : (fp32 ? A.lanes32 : A.lanes);
+        bool hit = false;
         {
             std::lock_guard<std::mutex> g(g_tune_mu);
             auto it = g_tune.find(key);
             if (it != g_tune.end()) {
                 slot = it->second;
-                continue;
+                hit = true;
             }
+        }
+        if (hit) {
+            if (slot == kLanesSlice) {
+                LT_TRY(csr_make_slices(A));
+                if (fp32) LT_TRY(csr_make_f32(A));  // the slices' FP32 values
+            }
+            continue;
+        }
+        if (slice_cand) {
+            LT_TRY(csr_make_slices(A));
+            if (fp32) LT_TRY(csr_make_f32(A));
         }
         int best = 0;
         if (fp32)
