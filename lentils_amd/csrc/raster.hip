@@ -407,6 +407,32 @@ __device__ inline int row_of(const int64_t *__restrict__ foff, int nlong, long f
     return lo;
 }
 
+// The long rows' fragments sorted in one radix sort instead of per row
+// (a segmented sort gives each row one workgroup, and the lens-centre rows
+// of 1e5 fragments then run alone): the key (source index << 32 | emission
+// position) of row l becomes l << (bi + bn) | index << bn | position, which
+// orders by row first, then as the per-row key; keys are unique, so the
+// order is the segmented sort's.  *bad = 1 if a field does not fit.
+__global__ void k_frag_pack(const int64_t *__restrict__ foff, int nlong, uint64_t *__restrict__ key, int bi, int bn,
+                            int *__restrict__ bad) {
+    const long F = foff[nlong];
+    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const int l = row_of(foff, nlong, f);
+    const uint64_t k = key[f];
+    const uint64_t i = k >> 32, n = k & 0xffffffffull;
+    if ((i >> bi) != 0 || (n >> bn) != 0) *bad = 1;
+    key[f] = ((uint64_t)l << (bi + bn)) | (i << bn) | n;
+}
+
+__global__ void k_frag_unpack(long F, uint64_t *__restrict__ key, int bi, int bn) {
+    const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint64_t k = key[f];
+    const uint64_t n = k & ((1ull << bn) - 1), i = (k >> bn) & ((1ull << bi) - 1);
+    key[f] = (i << 32) | n;
+}
+
 __global__ void k_group_flags(const int64_t *__restrict__ foff, int nlong, const uint64_t *__restrict__ key,
                               int64_t *__restrict__ flag) {
     const long F = foff[nlong];
@@ -721,16 +747,52 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
                            drows.p, dfoff.p, nlong, dseq.p, farea.p, fkey.p);
         LT_CHECK_LAUNCH();
         {
+            auto bits = [](uint64_t v) {  // bits holding 0 .. v - 1
+                int b = 0;
+                while (b < 64 && (v - 1) >> b) ++b;
+                return std::max(b, 1);
+            };
+            int64_t nsrc_max = 1, cnt_max = 1;
+            for (int q = 0; q < ns; ++q) nsrc_max = std::max(nsrc_max, srcoff_host[q + 1] - srcoff_host[q]);
+            for (int l = 0; l < nlong; ++l) cnt_max = std::max(cnt_max, hcnt[l]);
+            const int bl = bits((uint64_t)nlong), bi = bits((uint64_t)nsrc_max), bn = bits((uint64_t)cnt_max);
+            bool packed = bl + bi + bn <= 64 && F > 0;
+            if (packed) {
+                DevBuf<int> bad;
+                LT_TRY(bad.alloc(1));
+                LT_TRY(bad.zero());
+                hipLaunchKernelGGL(k_frag_pack, dim3(blocks_for(F, 256)), dim3(256), 0, stream(), dfoff.p, nlong,
+                                   fkey.p, bi, bn, bad.p);
+                LT_CHECK_LAUNCH();
+                int hb = 1;
+                LT_TRY(bad.download(&hb, 1));
+                LT_TRY(sync());
+                if (hb) {  // an index outside its sample's grid: unpack and sort per row
+                    hipLaunchKernelGGL(k_frag_unpack, dim3(blocks_for(F, 256)), dim3(256), 0, stream(), F, fkey.p,
+                                       bi, bn);
+                    LT_CHECK_LAUNCH();
+                    packed = false;
+                }
+            }
             size_t tb = 0;
-            if (rocprim::segmented_radix_sort_keys(nullptr, tb, fkey.p, fkey2.p, (unsigned)F, (unsigned)nlong,
-                                                   dfoff.p, dfoff.p + 1, 0, 64, stream()) != hipSuccess) {
+            if (packed) {
+                LT_HIP(rocprim::radix_sort_keys(nullptr, tb, fkey.p, fkey2.p, (size_t)F, 0, bl + bi + bn, stream()));
+            } else if (rocprim::segmented_radix_sort_keys(nullptr, tb, fkey.p, fkey2.p, (unsigned)F, (unsigned)nlong,
+                                                          dfoff.p, dfoff.p + 1, 0, 64, stream()) != hipSuccess) {
                 set_error("segmented sort sizing failed");
                 return LT_ERR_HIP;
             }
             DevBuf<uint8_t> tmp;
             LT_TRY(tmp.alloc(std::max<size_t>(tb, 1)));
-            LT_HIP(rocprim::segmented_radix_sort_keys(tmp.p, tb, fkey.p, fkey2.p, (unsigned)F, (unsigned)nlong,
-                                                      dfoff.p, dfoff.p + 1, 0, 64, stream()));
+            if (packed) {
+                LT_HIP(rocprim::radix_sort_keys(tmp.p, tb, fkey.p, fkey2.p, (size_t)F, 0, bl + bi + bn, stream()));
+                hipLaunchKernelGGL(k_frag_unpack, dim3(blocks_for(F, 256)), dim3(256), 0, stream(), F, fkey2.p, bi,
+                                   bn);
+                LT_CHECK_LAUNCH();
+            } else {
+                LT_HIP(rocprim::segmented_radix_sort_keys(tmp.p, tb, fkey.p, fkey2.p, (unsigned)F, (unsigned)nlong,
+                                                          dfoff.p, dfoff.p + 1, 0, 64, stream()));
+            }
         }
         DevBuf<int64_t> gflag, gpos;
         LT_TRY(gflag.alloc(F));
