@@ -150,9 +150,9 @@ struct GlobalFrags {
     __device__ bool stop() const { return false; }
 };
 
-// Long rows with at most kMedium fragments are walked by one thread each
-// (sequentially, like short rows); longer ("huge") rows take the frontier
-// decomposition below.  This pass counts every long row up to kMedium + 1.
+// Long rows with at most kMedium fragments are walked by one thread per fan
+// triangle; longer ("huge") rows take the frontier decomposition below.
+// The first pass counts every long row up to kMedium + 1.
 constexpr long kMedium = 512;
 
 struct CappedCount {
@@ -161,20 +161,40 @@ struct CappedCount {
     __device__ bool stop() const { return n > kMedium; }
 };
 
-__global__ void __launch_bounds__(64) k_long_count_capped(BatchGeom G, const SrcGrid *__restrict__ src,
-                                                          const double *__restrict__ lat,
-                                                          const int64_t *__restrict__ rows, int nlong,
-                                                          int64_t *__restrict__ cnt, int *__restrict__ status) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlong) return;
-    const long r = rows[l];
-    const int s = (int)(r / G.npix);
-    const long p = r - (long)s * G.npix;
-    const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
-    CappedCount cc;
-    const int st = raster_pixel<true>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], cc);
-    if (!cc.stop() && st) atomicOr(&status[s], st);
-    cnt[l] = cc.n;
+// The long rows' counts, each row's 8 fan triangles walked in parallel by 8
+// consecutive lanes (each triangle's walk starts from an empty stack, as in
+// raster_pixel, and stops past kMedium fragments): a row of at most kMedium
+// fragments gets its exact count, and tcnt its triangles' counts for the
+// emission; a longer row gets kMedium + 1, as the one-walk count gives.
+__global__ void __launch_bounds__(64) k_long_count8(BatchGeom G, const SrcGrid *__restrict__ src,
+                                                    const double *__restrict__ lat, const int64_t *__restrict__ rows,
+                                                    int nlong, int32_t *__restrict__ tcnt, int64_t *__restrict__ cnt,
+                                                    int *__restrict__ status) {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long l = gt >> 3;
+    const int t = (int)(gt & 7);
+    int n = 0, st = 0, s = 0;
+    if (l < nlong) {
+        const long r = rows[l];
+        s = (int)(r / G.npix);
+        const long p = r - (long)s * G.npix;
+        const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
+        double tv[6];
+        fan_vertices(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, t, tv);
+        CappedCount cc;
+        st = raster_triangle<true>(tv, src[s], cc);
+        n = (int)cc.n;
+        tcnt[gt] = n;
+    }
+    int sum = n, sto = st;  // the row's 8 lanes share one wave
+    for (int m = 1; m < 8; m <<= 1) {
+        sum += __shfl_xor(sum, m);
+        sto |= __shfl_xor(sto, m);
+    }
+    if (l < nlong && t == 0) {
+        if (sum <= kMedium && sto) atomicOr(&status[s], sto);
+        cnt[l] = sum > kMedium ? kMedium + 1 : sum;
+    }
 }
 
 // Huge rows, step 1: one workgroup per (huge row, fan triangle) expands the
@@ -376,21 +396,42 @@ __global__ void __launch_bounds__(kRasterThreads) k_long_sub_emit(const int64_t 
     raster_dfs<true>(it.tri, it.depth, g, gf);
 }
 
-// sequential emission of the long rows flagged for it (medium rows and huge
-// rows the frontier path gave back)
-__global__ void __launch_bounds__(64) k_long_emit(BatchGeom G, const SrcGrid *__restrict__ src,
-                                                  const double *__restrict__ lat, const int64_t *__restrict__ rows,
-                                                  const int64_t *__restrict__ foff, int nlong,
-                                                  const int *__restrict__ seq, double *__restrict__ area,
-                                                  uint64_t *__restrict__ key) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+// Emission of the long rows flagged for it (medium rows, and huge rows the
+// frontier path gave back): the 8 fan triangles of a row of at most kMedium
+// fragments walked in parallel, each from its position in the row's
+// sequence (the counts of the triangles before it); a huge row given back by
+// the frontier path takes the one sequential walk (lane t = 0).
+__global__ void __launch_bounds__(64) k_long_emit8(BatchGeom G, const SrcGrid *__restrict__ src,
+                                                   const double *__restrict__ lat, const int64_t *__restrict__ rows,
+                                                   const int64_t *__restrict__ foff, int nlong,
+                                                   const int *__restrict__ seq, const int32_t *__restrict__ tcnt,
+                                                   double *__restrict__ area, uint64_t *__restrict__ key) {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long l = gt >> 3;
+    const int t = (int)(gt & 7);
     if (l >= nlong || !seq[l]) return;
+    int n0 = 0, sum = 0;
+    for (int q = 0; q < 8; ++q) {
+        const int c = tcnt[8 * l + q];
+        n0 += q < t ? c : 0;
+        sum += c;
+    }
     const long r = rows[l];
     const int s = (int)(r / G.npix);
     const long p = r - (long)s * G.npix;
     const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
-    GlobalFrags gf{area, key, foff[l]};
-    raster_pixel<true>(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, src[s], gf);
+    const double *ls = lat + (long)s * G.lx * G.ly * 2;
+    if (sum > kMedium) {
+        if (t == 0) {
+            GlobalFrags gf{area, key, foff[l]};
+            raster_pixel<true>(ls, G.ly, i, j, src[s], gf);
+        }
+        return;
+    }
+    double tv[6];
+    fan_vertices(ls, G.ly, i, j, t, tv);
+    GlobalFrags gf{area, key, foff[l], n0};
+    raster_triangle<true>(tv, src[s], gf);
 }
 
 // Reduction of the sorted long rows (raster.c:375-391) in parallel over
@@ -647,9 +688,11 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
         }
         // exact counts of medium rows; rows beyond kMedium are huge
         DevBuf<int64_t> lcnt;
+        DevBuf<int32_t> tcnt;  // per fan triangle
         LT_TRY(lcnt.alloc(nlong));
-        hipLaunchKernelGGL(k_long_count_capped, dim3(blocks_for(nlong, 64)), dim3(64), 0, stream(), G, src.p,
-                           lattice.p, drows.p, nlong, lcnt.p, dstatus.p);
+        LT_TRY(tcnt.alloc(8L * nlong));
+        hipLaunchKernelGGL(k_long_count8, dim3(blocks_for(8L * nlong, 64)), dim3(64), 0, stream(), G, src.p,
+                           lattice.p, drows.p, nlong, tcnt.p, lcnt.p, dstatus.p);
         LT_CHECK_LAUNCH();
         std::vector<int64_t> hcnt(nlong), hrows_all(nlong);
         LT_TRY(lcnt.download(hcnt.data(), nlong));
@@ -743,8 +786,8 @@ lt_status LensBatch::build(const image_space &im_, const image_space *srcs, int 
                                fkey.p);
             LT_CHECK_LAUNCH();
         }
-        hipLaunchKernelGGL(k_long_emit, dim3(blocks_for(nlong, 64)), dim3(64), 0, stream(), G, src.p, lattice.p,
-                           drows.p, dfoff.p, nlong, dseq.p, farea.p, fkey.p);
+        hipLaunchKernelGGL(k_long_emit8, dim3(blocks_for(8L * nlong, 64)), dim3(64), 0, stream(), G, src.p,
+                           lattice.p, drows.p, dfoff.p, nlong, dseq.p, tcnt.p, farea.p, fkey.p);
         LT_CHECK_LAUNCH();
         {
             auto bits = [](uint64_t v) {  // bits holding 0 .. v - 1
