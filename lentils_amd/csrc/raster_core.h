@@ -425,7 +425,7 @@ __host__ __device__ int raster_triangle(const double *tv, const SrcGrid &g, Sink
 // reference would have at that node.  Concatenating the frontier's outputs
 // in order reproduces the reference's fragment sequence exactly (any cut of
 // the split tree listed in depth-first order does).
-constexpr int kLongFrontier = 2048;
+constexpr int kLongFrontier = 4096;
 constexpr int kLongLevels = 64;
 constexpr long kLongSplitCells = 32;
 
