@@ -155,21 +155,31 @@ struct GlobalFrags {
 // The first pass counts every long row up to kMedium + 1.
 constexpr long kMedium = 512;
 
-struct CappedCount {
+// count of one fan triangle's walk that also adds to its row's count in
+// LDS (shared by the row's 8 lanes) and stops once the row is past kMedium
+struct RowCappedCount {
+    int *row;
     long n = 0;
-    __device__ void operator()(double, int) { ++n; }
-    __device__ bool stop() const { return n > kMedium; }
+    __device__ void operator()(double, int) {
+        ++n;
+        atomicAdd(row, 1);
+    }
+    __device__ bool stop() const { return *(volatile int *)row > kMedium; }
 };
 
 // The long rows' counts, each row's 8 fan triangles walked in parallel by 8
 // consecutive lanes (each triangle's walk starts from an empty stack, as in
-// raster_pixel, and stops past kMedium fragments): a row of at most kMedium
-// fragments gets its exact count, and tcnt its triangles' counts for the
-// emission; a longer row gets kMedium + 1, as the one-walk count gives.
+// raster_pixel; all 8 stop once the row is past kMedium fragments): a row of
+// at most kMedium fragments gets its exact count, and tcnt its triangles'
+// counts for the emission; a longer row gets kMedium + 1, as the one-walk
+// count gives, and triangle counts that add up to more than kMedium.
 __global__ void __launch_bounds__(64) k_long_count8(BatchGeom G, const SrcGrid *__restrict__ src,
                                                     const double *__restrict__ lat, const int64_t *__restrict__ rows,
                                                     int nlong, int32_t *__restrict__ tcnt, int64_t *__restrict__ cnt,
                                                     int *__restrict__ status) {
+    __shared__ int rowc[8];  // the workgroup's 8 rows
+    if (threadIdx.x < 8) rowc[threadIdx.x] = 0;
+    __syncthreads();
     const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long l = gt >> 3;
     const int t = (int)(gt & 7);
@@ -181,7 +191,7 @@ __global__ void __launch_bounds__(64) k_long_count8(BatchGeom G, const SrcGrid *
         const int i = (int)(p / G.ny), j = (int)(p - (long)i * G.ny);
         double tv[6];
         fan_vertices(lat + (long)s * G.lx * G.ly * 2, G.ly, i, j, t, tv);
-        CappedCount cc;
+        RowCappedCount cc{&rowc[threadIdx.x >> 3]};
         st = raster_triangle<true>(tv, src[s], cc);
         n = (int)cc.n;
         tcnt[gt] = n;
